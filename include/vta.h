@@ -1,0 +1,180 @@
+/*
+ * vta.h -- C ABI of libvta_hip, the MI355X (gfx950) batch-indicator library.
+ *
+ * Drop-in boundary for VectorTA's GPU batch-indicator path.  Each entry point
+ * replaces one method of the reference's Rust CUDA wrappers
+ * (the src/cuda/ wrapper files) and keeps its argument meaning:
+ *   - *_batch_dev            one series x many params, host input, device-resident
+ *                            output, synchronised before return;
+ *   - *_batch_device         device input, caller-owned device output, enqueue only
+ *                            (no allocation, no synchronisation);
+ *   - *_many_series_one_param_time_major_dev
+ *                            many time-major series x one param, host input,
+ *                            device-resident [rows=T][cols=N] output.
+ * Kernels are HIP code objects (one .hsaco per indicator family) loaded with
+ * hipModuleLoad from the directory holding libvta_hip.so (or $VTA_KERNEL_DIR).
+ *
+ * Conventions: every call returns vta_status; on failure vta_last_error()
+ * returns a thread-local message.  Integers are size_t / int32_t, device
+ * pointers are plain float* / int32_t*.  One-series outputs are row-major
+ * [combo][t]; many-series data and outputs are time-major [t][series].
+ * A vta_ctx owns one device and one non-blocking stream; like the reference
+ * wrappers (alma_wrapper.rs:972-974) it is not thread-safe.
+ */
+#ifndef VTA_H
+#define VTA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error enum, one-to-one with CudaAlmaError (alma_wrapper.rs:72-101) plus
+ * CudaEmaError::ArithmeticOverflow (ema_wrapper.rs:21-45). */
+typedef enum vta_status {
+    VTA_OK = 0,
+    VTA_ERR_HIP = 1,                     /* Cuda(CudaError) */
+    VTA_ERR_OUT_OF_MEMORY = 2,           /* OutOfMemory{required, free, headroom} */
+    VTA_ERR_MISSING_KERNEL_SYMBOL = 3,   /* MissingKernelSymbol{name} */
+    VTA_ERR_INVALID_INPUT = 4,           /* InvalidInput(String) */
+    VTA_ERR_INVALID_POLICY = 5,          /* InvalidPolicy */
+    VTA_ERR_LAUNCH_CONFIG_TOO_LARGE = 6, /* LaunchConfigTooLarge{gx..bz} */
+    VTA_ERR_DEVICE_MISMATCH = 7,         /* DeviceMismatch{buf, current} */
+    VTA_ERR_NOT_IMPLEMENTED = 8,         /* NotImplemented */
+    VTA_ERR_ARITHMETIC_OVERFLOW = 9,     /* ArithmeticOverflow{context} */
+    VTA_ERR_NO_DEVICE = 10               /* cuda_available() == false */
+} vta_status;
+
+const char *vta_last_error(void);
+const char *vta_version(void);
+
+/* Device probe: src/cuda/mod.rs:412-602 (cuda_available, cuda_device_count).
+ * Honours VTA_FORCE_SKIP=1 like CUDA_FORCE_SKIP (mod.rs:419-423). */
+int vta_device_available(void);
+int vta_device_count(void);
+
+/* ------------------------------------------------------------ context */
+/* CudaAlma::new (alma_wrapper.rs:133-169): device, modules, NON_BLOCKING stream. */
+typedef struct vta_ctx vta_ctx;
+vta_status vta_ctx_create(int device_id, vta_ctx **out);
+vta_status vta_ctx_destroy(vta_ctx *ctx);
+vta_status vta_ctx_synchronize(vta_ctx *ctx);   /* CudaAlma::synchronize (:171) */
+void *vta_ctx_stream(vta_ctx *ctx);             /* hipStream_t of the context */
+int vta_ctx_device_id(const vta_ctx *ctx);      /* CudaAlma::device_id */
+/* Loads every code object eagerly and checks that each expected kernel
+ * symbol exists (the reference JITs its PTX in new()). */
+vta_status vta_ctx_preload(vta_ctx *ctx);
+
+/* --------------------------------------------------- device arrays */
+/* DeviceArrayF32 {buf, rows, cols} (alma_wrapper.rs:103-117): owned device
+ * buffer, freed with vta_device_array_free (Rust frees on drop). */
+typedef struct vta_device_array_f32 {
+    float *ptr;
+    size_t rows;
+    size_t cols;
+} vta_device_array_f32;
+vta_status vta_device_array_free(vta_ctx *ctx, vta_device_array_f32 *arr);
+/* DeviceBuffer::copy_to -- synchronous device-to-host copy of rows*cols floats. */
+vta_status vta_device_array_copy_to_host(vta_ctx *ctx, const vta_device_array_f32 *arr,
+                                         float *host, size_t host_len);
+
+/* Raw device memory for the enqueue-only *_device entry points. */
+vta_status vta_device_alloc(vta_ctx *ctx, size_t bytes, void **out);
+vta_status vta_device_free(vta_ctx *ctx, void *ptr);
+vta_status vta_memcpy_htod(vta_ctx *ctx, void *dst, const void *src, size_t bytes);
+vta_status vta_memcpy_dtoh(vta_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+/* -------------------------------------------------------- ranges */
+typedef struct vta_range_usize { size_t start, end, step; } vta_range_usize;
+typedef struct vta_range_f64 { double start, end, step; } vta_range_f64;
+
+/* =============================================================== ALMA */
+/* AlmaParams / AlmaBatchRange (alma.rs:60-80, 1067-1082). */
+typedef struct vta_alma_params {
+    size_t period;
+    double offset;
+    double sigma;
+} vta_alma_params;
+typedef struct vta_alma_batch_range {
+    vta_range_usize period;
+    vta_range_f64 offset;
+    vta_range_f64 sigma;
+} vta_alma_batch_range;
+
+/* expand_grid (alma_wrapper.rs:1550-1631): period -> offset -> sigma order;
+ * f64 axes accumulate x += step while x <= end + 1e-12.  Writes up to cap
+ * combos (out may be NULL to query *n_out). */
+vta_status vta_alma_expand_grid(const vta_alma_batch_range *range, vta_alma_params *out,
+                                size_t cap, size_t *n_out);
+
+/* Batch kernel policy (BatchKernelPolicy / BatchKernelSelected,
+ * alma_wrapper.rs:18-70).  FP32 = LDS-staged sliding dot (the reference's
+ * Plain/Tiled family); MFMA_BF16X3 = split-BF16 Hankel x weights contraction
+ * on the matrix cores with FP32 accumulation. */
+typedef enum vta_alma_batch_kernel {
+    VTA_ALMA_BATCH_AUTO = 0,
+    VTA_ALMA_BATCH_FP32 = 1,
+    VTA_ALMA_BATCH_MFMA_BF16X3 = 2
+} vta_alma_batch_kernel;
+vta_status vta_alma_set_batch_policy(vta_ctx *ctx, vta_alma_batch_kernel policy);
+/* selected_batch_kernel (:239-245): kernel used by the last batch call, 0 if none. */
+vta_alma_batch_kernel vta_alma_selected_batch_kernel(const vta_ctx *ctx);
+
+/* alma_batch_dev (alma_wrapper.rs:625-669): host f32 series -> device
+ * [combos][len] output. */
+vta_status vta_alma_batch_dev(vta_ctx *ctx, const float *prices, size_t len,
+                              const vta_alma_batch_range *range, vta_device_array_f32 *out);
+/* alma_batch_from_device_prices (:671-681). */
+vta_status vta_alma_batch_from_device_prices(vta_ctx *ctx, const float *d_prices, size_t len,
+                                             size_t first_valid,
+                                             const vta_alma_batch_range *range,
+                                             vta_device_array_f32 *out);
+/* alma_batch_into_host_f32 (:783-817): out_len must be combos*len. */
+vta_status vta_alma_batch_into_host_f32(vta_ctx *ctx, const float *prices, size_t len,
+                                        const vta_alma_batch_range *range, float *out,
+                                        size_t out_len);
+/* alma_batch_device (:565-593): enqueue-only FP32 sweep over caller-owned
+ * buffers.  d_weights is [n_combos][max_period] f32, multiplied by
+ * d_inv_norms[c] (the reference passes pre-normalised weights and 1.0). */
+vta_status vta_alma_batch_device(vta_ctx *ctx, const float *d_prices, const float *d_weights,
+                                 const int32_t *d_periods, const float *d_inv_norms,
+                                 int32_t max_period, int32_t series_len, int32_t n_combos,
+                                 int32_t first_valid, float *d_out);
+/* alma_batch_device_tm (:595-623): same, output time-major [t][combo]. */
+vta_status vta_alma_batch_device_tm(vta_ctx *ctx, const float *d_prices, const float *d_weights,
+                                    const int32_t *d_periods, const float *d_inv_norms,
+                                    int32_t max_period, int32_t series_len, int32_t n_combos,
+                                    int32_t first_valid, float *d_out_tm);
+
+/* Device-resident sweep plan for the MFMA form: the per-combo constants the
+ * reference rebuilds on every call (compute_weights_cpu_f32, :721-747) are
+ * built and uploaded once -- split-BF16 weight fragments in matrix-core
+ * operand order, combos grouped by period.  vta_alma_sweep_launch is the
+ * MFMA analogue of alma_batch_device: enqueue only, no allocation, no sync. */
+typedef struct vta_alma_sweep_plan vta_alma_sweep_plan;
+vta_status vta_alma_sweep_plan_create(vta_ctx *ctx, const vta_alma_batch_range *range,
+                                      size_t series_len, vta_alma_batch_kernel kernel,
+                                      vta_alma_sweep_plan **out);
+vta_status vta_alma_sweep_plan_destroy(vta_ctx *ctx, vta_alma_sweep_plan *plan);
+size_t vta_alma_sweep_plan_combos(const vta_alma_sweep_plan *plan);
+vta_status vta_alma_sweep_launch(vta_ctx *ctx, const vta_alma_sweep_plan *plan,
+                                 const float *d_prices, size_t first_valid, float *d_out);
+
+/* alma_multi_series_one_param_time_major_dev (:819-899): data_tm is
+ * [rows=T][cols=N]; output DeviceArrayF32{rows=T, cols=N}. */
+vta_status vta_alma_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *data_tm,
+                                                         size_t cols, size_t rows,
+                                                         const vta_alma_params *params,
+                                                         vta_device_array_f32 *out);
+/* alma_many_series_one_param_time_major_device_precomputed (:1521-1547). */
+vta_status vta_alma_many_series_one_param_time_major_device(
+    vta_ctx *ctx, const float *d_prices_tm, const float *d_weights, size_t period,
+    float inv_norm, size_t cols, size_t rows, const int32_t *d_first_valids, float *d_out_tm);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VTA_H */

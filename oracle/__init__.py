@@ -1,0 +1,197 @@
+"""CPU f64 oracle for the HIP batch-indicator path -- TEST INFRASTRUCTURE ONLY.
+
+Thin ctypes binding over ``liboracle.so`` (built from ``vta_oracle.c`` by the
+Makefile beside it).  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import this package, and only as the
+checker (or the timed CPU baseline).  The ``vectorta_amd`` product path never
+imports it.
+
+Each function follows the reference's scalar f64 kernel named in the C source
+(file:line citations there).  Inputs are float64 arrays (the f32 series the GPU
+sees, widened), outputs are float64 arrays with the reference's NaN prefixes.
+
+Parity pins (see tests/test_oracle.py): the reference's own known-answer values
+on the Bitfinex 4h fixture (alma.rs:1835-1856, ema.rs:971-983, rsi.rs:1142-1156,
+atr.rs:1422-1436, adx.rs:2163-2175, bollinger_bands.rs:1705-1745,
+percentile_nearest_rank.rs:1395-1425, medium_ad.rs:1478-1497), the tiny PNR
+vectors (tests/python/test_utils.py:165-179), the ALMA re-input values
+(test_utils.py:120-126), and closed-form / brute-force invariants.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+STATUS = {0: "ok", 1: "empty input", 2: "all values NaN", 3: "invalid period",
+          4: "not enough valid data", 5: "invalid parameter"}
+
+
+class OracleError(ValueError):
+    def __init__(self, code: int, what: str):
+        super().__init__(f"{what}: {STATUS.get(code, code)}")
+        self.code = code
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        sz = ctypes.c_size_t
+        L.or_first_valid.argtypes = [dp, sz]
+        L.or_first_valid.restype = sz
+        L.or_alma_weights_single.argtypes = [sz, ctypes.c_double, ctypes.c_double, dp]
+        L.or_alma_weights_single.restype = ctypes.c_double
+        L.or_alma_weights_batch.argtypes = [sz, ctypes.c_double, ctypes.c_double, dp]
+        L.or_alma_weights_batch.restype = ctypes.c_double
+        L.or_alma_single.argtypes = [dp, sz, sz, ctypes.c_double, ctypes.c_double, dp]
+        L.or_alma_batch.argtypes = [dp, sz, ctypes.POINTER(sz), dp, dp, sz, dp, ctypes.c_int]
+        L.or_ema.argtypes = [dp, sz, sz, dp]
+        L.or_rsi.argtypes = [dp, sz, sz, dp]
+        L.or_atr.argtypes = [dp, dp, dp, sz, sz, dp]
+        L.or_atr_batch_row.argtypes = [dp, dp, dp, sz, sz, dp]
+        L.or_bb.argtypes = [dp, sz, sz, ctypes.c_double, ctypes.c_double, dp, dp, dp]
+        L.or_pnr.argtypes = [dp, sz, sz, ctypes.c_double, dp]
+        L.or_mad.argtypes = [dp, sz, sz, dp]
+        L.or_adx.argtypes = [dp, dp, dp, sz, sz, dp]
+        L.or_first_valid_hlc.argtypes = [dp, dp, dp, sz]
+        L.or_first_valid_hlc.restype = sz
+        L.or_many_series_1in.argtypes = [ctypes.c_int, dp, sz, sz, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_double, dp,
+                                         ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+        L.or_many_series_1in.restype = None
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _check(code, what):
+    if code != 0:
+        raise OracleError(code, what)
+
+
+def alma(x, period=9, offset=0.85, sigma=6.0):
+    x, xp = _d(x)
+    out = np.full(x.size, np.nan)
+    _check(lib().or_alma_single(xp, x.size, period, offset, sigma, _d(out)[1]), "alma")
+    return out
+
+
+def alma_weights_batch(period, offset, sigma):
+    w = np.zeros(period)
+    inv = lib().or_alma_weights_batch(period, offset, sigma, _d(w)[1])
+    return w, inv
+
+
+def alma_batch(x, combos, nthreads=1):
+    """combos: sequence of (period, offset, sigma).  Returns [len(combos), len(x)]."""
+    x, xp = _d(x)
+    n = len(combos)
+    per = (ctypes.c_size_t * n)(*[int(c[0]) for c in combos])
+    off = np.array([c[1] for c in combos], dtype=np.float64)
+    sig = np.array([c[2] for c in combos], dtype=np.float64)
+    out = np.empty((n, x.size))
+    _check(lib().or_alma_batch(xp, x.size, per, _d(off)[1], _d(sig)[1], n, _d(out)[1],
+                               int(nthreads)), "alma_batch")
+    return out
+
+
+def ema(x, period=9):
+    x, xp = _d(x)
+    out = np.full(x.size, np.nan)
+    _check(lib().or_ema(xp, x.size, period, _d(out)[1]), "ema")
+    return out
+
+
+def rsi(x, period=14):
+    x, xp = _d(x)
+    out = np.full(x.size, np.nan)
+    _check(lib().or_rsi(xp, x.size, period, _d(out)[1]), "rsi")
+    return out
+
+
+def atr(h, l, c, length=14):
+    h, hp = _d(h)
+    l, lp = _d(l)
+    c, cp = _d(c)
+    out = np.full(c.size, np.nan)
+    _check(lib().or_atr(hp, lp, cp, c.size, length, _d(out)[1]), "atr")
+    return out
+
+
+def atr_batch_row(h, l, c, length=14):
+    h, hp = _d(h)
+    l, lp = _d(l)
+    c, cp = _d(c)
+    out = np.full(c.size, np.nan)
+    _check(lib().or_atr_batch_row(hp, lp, cp, c.size, length, _d(out)[1]), "atr_batch_row")
+    return out
+
+
+def bollinger_bands(x, period=20, devup=2.0, devdn=2.0):
+    """Returns (upper, middle, lower), the order of BollingerBandsOutput."""
+    x, xp = _d(x)
+    u = np.full(x.size, np.nan)
+    m = np.full(x.size, np.nan)
+    lo = np.full(x.size, np.nan)
+    _check(lib().or_bb(xp, x.size, period, devup, devdn, _d(u)[1], _d(m)[1], _d(lo)[1]), "bb")
+    return u, m, lo
+
+
+def pnr(x, length=15, percentage=50.0):
+    x, xp = _d(x)
+    out = np.full(x.size, np.nan)
+    _check(lib().or_pnr(xp, x.size, length, percentage, _d(out)[1]), "pnr")
+    return out
+
+
+def medium_ad(x, period=5):
+    x, xp = _d(x)
+    out = np.full(x.size, np.nan)
+    _check(lib().or_mad(xp, x.size, period, _d(out)[1]), "medium_ad")
+    return out
+
+
+def adx(h, l, c, period=14):
+    h, hp = _d(h)
+    l, lp = _d(l)
+    c, cp = _d(c)
+    out = np.full(c.size, np.nan)
+    _check(lib().or_adx(hp, lp, cp, c.size, period, _d(out)[1]), "adx")
+    return out
+
+
+def first_valid(x):
+    x, xp = _d(x)
+    return int(lib().or_first_valid(xp, x.size))
+
+
+_KIND = {"alma": 0, "ema": 1, "rsi": 2, "pnr": 3, "medium_ad": 4}
+
+
+def many_series(kind, tm, p0, p1=0.0, p2=0.0, nthreads=0):
+    """Applies a single-series oracle to every column of a [rows, cols] matrix."""
+    tm, tp = _d(tm)
+    rows, cols = tm.shape
+    out = np.full((rows, cols), np.nan)
+    st = (ctypes.c_int * cols)()
+    lib().or_many_series_1in(_KIND[kind], tp, cols, rows, float(p0), float(p1), float(p2),
+                             _d(out)[1], st, int(nthreads))
+    return out, np.frombuffer(st, dtype=np.int32).copy()

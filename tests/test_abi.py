@@ -1,0 +1,105 @@
+"""CPU-side checks of the C-ABI library: it loads, exports every symbol that
+include/vta.h declares, ships a gfx950 code object holding every kernel the
+runtime looks up, and its host logic (grid expansion, validation, device probe)
+behaves like the reference wrappers.  No kernel is launched here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import vectorta_amd as vta
+from vectorta_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "vta.h")
+PKG = os.path.join(ROOT, "vectorta_amd")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(vta_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_symbols_exported():
+    L = _lib.lib()
+    names = declared_symbols()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # and the Python binding covers every declared symbol
+    assert set(names) <= set(_lib.SIGNATURES), set(names) - set(_lib.SIGNATURES)
+
+
+def _kernel_names(path):
+    data = open(path, "rb").read()
+    return set(re.findall(rb"[a-z_0-9]+", data))
+
+
+def test_code_objects_hold_kernels():
+    from vectorta_amd._lib import LIB_PATH
+    src = open(os.path.join(PKG, "csrc", "runtime.cpp")).read()
+    # every (module, symbol) pair vta_ctx_preload checks
+    pairs = re.findall(r'\{"([a-z]+)", "([a-z0-9_]+)"\}', src)
+    assert pairs
+    for module, sym in pairs:
+        path = os.path.join(PKG, f"vta_{module}.hsaco")
+        assert os.path.exists(path), path
+        assert sym.encode() in _kernel_names(path), (module, sym)
+        assert b"gfx950" in open(path, "rb").read()
+    assert os.path.exists(LIB_PATH)
+
+
+def test_version_and_probe_forced_off(monkeypatch):
+    assert b"gfx950" in _lib.lib().vta_version()
+    monkeypatch.setenv("VTA_FORCE_SKIP", "1")
+    assert _lib.lib().vta_device_available() == 0
+    p = ctypes.c_void_p()
+    st = _lib.lib().vta_ctx_create(0, ctypes.byref(p))
+    assert st == 10 and not p.value          # VTA_ERR_NO_DEVICE
+    assert b"no HIP device" in _lib.lib().vta_last_error()
+
+
+def _rust_axis_f64(start, end, step):
+    # alma.rs:1220-1250 semantics: x accumulates by repeated addition
+    if abs(step) < 1e-12 or abs(start - end) < 1e-12:
+        return [start]
+    out = []
+    if start < end:
+        x = start
+        while x <= end + 1e-12:
+            out.append(x)
+            x += abs(step)
+        return out
+    x = start
+    while x + 1e-12 >= end:
+        out.append(x)
+        x -= abs(step)
+    return out
+
+
+def test_alma_expand_grid_matches_reference_semantics():
+    r = vta.AlmaBatchRange(period=(10, 250, 10), offset=(0.05, 0.95, 0.1), sigma=(2.0, 10.0, 2.0))
+    combos = vta.expand_grid(r)
+    assert len(combos) == 1250
+    offs = _rust_axis_f64(0.05, 0.95, 0.1)
+    assert len(offs) == 10
+    # exact bits of the accumulated offsets (0.15000000000000002, ...)
+    assert [c.offset for c in combos[:50:5]] == offs
+    assert combos[1].offset == 0.05 and combos[5].offset == offs[1]
+    assert offs[1] == 0.15000000000000002
+    # order: period -> offset -> sigma
+    assert [c.period for c in combos[::50]] == list(range(10, 251, 10))
+    assert [c.sigma for c in combos[:5]] == [2.0, 4.0, 6.0, 8.0, 10.0]
+    # descending period axis walks down from start (alma_wrapper.rs:1557-1567)
+    d = vta.expand_grid(vta.AlmaBatchRange(period=(30, 10, 7)))
+    assert [c.period for c in d] == [30, 23, 16]
+    s = vta.expand_grid(vta.AlmaBatchRange(period=(9, 9, 0)))
+    assert [c.period for c in s] == [9]
+
+
+def test_alma_invalid_policy_rejected_without_device():
+    with pytest.raises(KeyError):
+        vta.alma._POLICY["bogus"]

@@ -1,0 +1,16 @@
+"""vectorta_amd -- MI355X-native (gfx950) implementation of VectorTA's GPU
+batch-indicator path.
+
+The product is ``libvta_hip.so`` (C ABI in ``include/vta.h``) plus one gfx950
+code object per indicator family; this package is the Python host layer over
+it, mirroring the reference's ``src/cuda`` wrapper classes (``CudaAlma`` ...).
+"""
+from ._lib import VtaError, lib
+from .alma import AlmaBatchRange, AlmaParams, AlmaSweepPlan, CudaAlma, HipAlma, expand_grid
+from .device import DeviceArrayF32, DeviceBuffer, HipContext, cuda_available, cuda_device_count
+
+__all__ = [
+    "VtaError", "lib", "AlmaBatchRange", "AlmaParams", "AlmaSweepPlan", "CudaAlma", "HipAlma",
+    "expand_grid", "DeviceArrayF32", "DeviceBuffer", "HipContext", "cuda_available",
+    "cuda_device_count",
+]

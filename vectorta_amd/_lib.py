@@ -1,0 +1,122 @@
+"""ctypes binding of libvta_hip.so (C ABI declared in include/vta.h).
+
+The library and its gfx950 code objects (vta_<family>.hsaco) live next to this
+file.  Loading fails loudly when they are missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvta_hip.so")
+
+VTA_OK = 0
+STATUS_NAMES = {
+    0: "Ok", 1: "Hip", 2: "OutOfMemory", 3: "MissingKernelSymbol", 4: "InvalidInput",
+    5: "InvalidPolicy", 6: "LaunchConfigTooLarge", 7: "DeviceMismatch", 8: "NotImplemented",
+    9: "ArithmeticOverflow", 10: "NoDevice",
+}
+
+
+class VtaError(RuntimeError):
+    """Mirrors the reference wrapper error enums (alma_wrapper.rs:72-101)."""
+
+    def __init__(self, status: int, message: str):
+        self.status = status
+        self.kind = STATUS_NAMES.get(status, str(status))
+        super().__init__(f"{self.kind}: {message}")
+
+
+class RangeUsize(ctypes.Structure):
+    _fields_ = [("start", ctypes.c_size_t), ("end", ctypes.c_size_t), ("step", ctypes.c_size_t)]
+
+
+class RangeF64(ctypes.Structure):
+    _fields_ = [("start", ctypes.c_double), ("end", ctypes.c_double), ("step", ctypes.c_double)]
+
+
+class DeviceArrayF32C(ctypes.Structure):
+    _fields_ = [("ptr", ctypes.c_void_p), ("rows", ctypes.c_size_t), ("cols", ctypes.c_size_t)]
+
+
+class AlmaParamsC(ctypes.Structure):
+    _fields_ = [("period", ctypes.c_size_t), ("offset", ctypes.c_double),
+                ("sigma", ctypes.c_double)]
+
+
+class AlmaBatchRangeC(ctypes.Structure):
+    _fields_ = [("period", RangeUsize), ("offset", RangeF64), ("sigma", RangeF64)]
+
+
+# Every symbol include/vta.h declares, with (restype, argtypes).
+_P = ctypes.c_void_p
+_SZ = ctypes.c_size_t
+_I32 = ctypes.c_int32
+_F32 = ctypes.c_float
+_ST = ctypes.c_int
+SIGNATURES = {
+    "vta_last_error": (ctypes.c_char_p, []),
+    "vta_version": (ctypes.c_char_p, []),
+    "vta_device_available": (ctypes.c_int, []),
+    "vta_device_count": (ctypes.c_int, []),
+    "vta_ctx_create": (_ST, [ctypes.c_int, ctypes.POINTER(_P)]),
+    "vta_ctx_destroy": (_ST, [_P]),
+    "vta_ctx_synchronize": (_ST, [_P]),
+    "vta_ctx_stream": (_P, [_P]),
+    "vta_ctx_device_id": (ctypes.c_int, [_P]),
+    "vta_ctx_preload": (_ST, [_P]),
+    "vta_device_array_free": (_ST, [_P, ctypes.POINTER(DeviceArrayF32C)]),
+    "vta_device_array_copy_to_host": (_ST, [_P, ctypes.POINTER(DeviceArrayF32C), _P, _SZ]),
+    "vta_device_alloc": (_ST, [_P, _SZ, ctypes.POINTER(_P)]),
+    "vta_device_free": (_ST, [_P, _P]),
+    "vta_memcpy_htod": (_ST, [_P, _P, _P, _SZ]),
+    "vta_memcpy_dtoh": (_ST, [_P, _P, _P, _SZ]),
+    # ALMA
+    "vta_alma_expand_grid": (_ST, [ctypes.POINTER(AlmaBatchRangeC), ctypes.POINTER(AlmaParamsC),
+                                   _SZ, ctypes.POINTER(_SZ)]),
+    "vta_alma_set_batch_policy": (_ST, [_P, ctypes.c_int]),
+    "vta_alma_selected_batch_kernel": (ctypes.c_int, [_P]),
+    "vta_alma_batch_dev": (_ST, [_P, _P, _SZ, ctypes.POINTER(AlmaBatchRangeC),
+                                 ctypes.POINTER(DeviceArrayF32C)]),
+    "vta_alma_batch_from_device_prices": (_ST, [_P, _P, _SZ, _SZ, ctypes.POINTER(AlmaBatchRangeC),
+                                                ctypes.POINTER(DeviceArrayF32C)]),
+    "vta_alma_batch_into_host_f32": (_ST, [_P, _P, _SZ, ctypes.POINTER(AlmaBatchRangeC), _P,
+                                           _SZ]),
+    "vta_alma_batch_device": (_ST, [_P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _P]),
+    "vta_alma_batch_device_tm": (_ST, [_P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _P]),
+    "vta_alma_sweep_plan_create": (_ST, [_P, ctypes.POINTER(AlmaBatchRangeC), _SZ, ctypes.c_int,
+                                         ctypes.POINTER(_P)]),
+    "vta_alma_sweep_plan_destroy": (_ST, [_P, _P]),
+    "vta_alma_sweep_plan_combos": (_SZ, [_P]),
+    "vta_alma_sweep_launch": (_ST, [_P, _P, _P, _SZ, _P]),
+    "vta_alma_many_series_one_param_time_major_dev": (
+        _ST, [_P, _P, _SZ, _SZ, ctypes.POINTER(AlmaParamsC), ctypes.POINTER(DeviceArrayF32C)]),
+    "vta_alma_many_series_one_param_time_major_device": (
+        _ST, [_P, _P, _P, _SZ, _F32, _SZ, _SZ, _P, _P]),
+}
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Loads libvta_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                "(make -C vectorta_amd/csrc); there is no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != VTA_OK:
+        msg = lib().vta_last_error()
+        raise VtaError(status, msg.decode() if msg else "")

@@ -1,0 +1,619 @@
+// ALMA host entry points: replaces CudaAlma (src/cuda/moving_averages/
+// alma_wrapper.rs).  Grid expansion, input validation, per-combo weights and
+// kernel selection run here; the arithmetic runs in kernels/alma.hip.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "runtime_internal.h"
+
+using namespace vta;
+
+namespace {
+
+// expand_grid (alma_wrapper.rs:1550-1631; same as alma.rs:1198-1289).
+bool axis_usize(vta_range_usize r, std::vector<size_t> &v) {
+    v.clear();
+    if (r.step == 0 || r.start == r.end) {
+        v.push_back(r.start);
+        return true;
+    }
+    if (r.start < r.end) {
+        for (size_t x = r.start; x <= r.end; x += r.step) {
+            v.push_back(x);
+            if (x > r.end - r.step) break;  // next step would pass end (or overflow)
+        }
+        return true;
+    }
+    long long x = (long long)r.start, e = (long long)r.end, st = (long long)r.step;
+    while (x >= e) {
+        v.push_back((size_t)x);
+        x -= st;
+    }
+    return !v.empty();
+}
+
+bool axis_f64(vta_range_f64 r, std::vector<double> &v) {
+    v.clear();
+    if (std::fabs(r.step) < 1e-12 || std::fabs(r.start - r.end) < 1e-12) {
+        v.push_back(r.start);
+        return true;
+    }
+    double st = std::fabs(r.step);
+    if (r.start < r.end) {
+        for (double x = r.start; x <= r.end + 1e-12; x += st) v.push_back(x);
+        return !v.empty();
+    }
+    for (double x = r.start; x + 1e-12 >= r.end; x -= st) v.push_back(x);
+    return !v.empty();
+}
+
+vta_status expand(const vta_alma_batch_range *range, std::vector<vta_alma_params> &out) {
+    if (!range) return fail(VTA_ERR_INVALID_INPUT, "range is null");
+    std::vector<size_t> ps;
+    std::vector<double> os, ss;
+    if (!axis_usize(range->period, ps) || !axis_f64(range->offset, os) ||
+        !axis_f64(range->sigma, ss))
+        return fail(VTA_ERR_INVALID_INPUT, "Invalid range: start=%zu, end=%zu, step=%zu",
+                    range->period.start, range->period.end, range->period.step);
+    size_t cap;
+    if (!mul_ok(ps.size(), os.size(), &cap) || !mul_ok(cap, ss.size(), &cap))
+        return fail(VTA_ERR_INVALID_INPUT, "range size overflow");
+    out.clear();
+    out.reserve(cap);
+    for (size_t p : ps)
+        for (double o : os)
+            for (double s : ss) out.push_back({p, o, s});
+    if (out.empty()) return fail(VTA_ERR_INVALID_INPUT, "no parameter combinations");
+    return VTA_OK;
+}
+
+// prepare_batch_inputs / prepare_batch_inputs_device (alma_wrapper.rs:419-505).
+vta_status validate_combos(const std::vector<vta_alma_params> &combos, size_t len,
+                           size_t first_valid, size_t *max_period) {
+    size_t mp = 0;
+    for (auto &c : combos) mp = std::max(mp, c.period);
+    if (mp == 0 || first_valid >= len || len - first_valid < mp)
+        return fail(VTA_ERR_INVALID_INPUT, "not enough valid data (needed >= %zu, valid = %zu)",
+                    mp, first_valid < len ? len - first_valid : (size_t)0);
+    for (auto &c : combos) {
+        if (c.period == 0 || !(c.sigma > 0.0) || !(c.offset >= 0.0 && c.offset <= 1.0))
+            return fail(VTA_ERR_INVALID_INPUT, "invalid params: period=%zu, offset=%g, sigma=%g",
+                        c.period, c.offset, c.sigma);
+    }
+    if (mp > (size_t)INT32_MAX || len > (size_t)INT32_MAX)
+        return fail(VTA_ERR_INVALID_INPUT, "series or period exceeds int32 range");
+    *max_period = mp;
+    return VTA_OK;
+}
+
+// Normalised Gaussian weights in f64 (the batch form of alma.rs:1426-1436:
+// w = exp(-(d*d)/s2), divided by their sum).
+void weights_f64(size_t period, double offset, double sigma, std::vector<double> &w) {
+    w.resize(period);
+    double m = offset * (double)(period - 1);
+    double s = (double)period / sigma;
+    double s2 = 2.0 * s * s;
+    double norm = 0.0;
+    for (size_t i = 0; i < period; ++i) {
+        double d = (double)i - m;
+        w[i] = std::exp(-(d * d) / s2);
+        norm += w[i];
+    }
+    double inv = 1.0 / norm;
+    for (auto &x : w) x *= inv;
+}
+
+// Flat [C][max_period] f32 table of pre-normalised weights, inv_norms = 1
+// (the layout alma_batch_device consumes, alma_wrapper.rs:721-747).
+void weights_table_f32(const std::vector<vta_alma_params> &combos, size_t max_period,
+                       std::vector<float> &w, std::vector<int32_t> &periods,
+                       std::vector<float> &inv) {
+    size_t C = combos.size();
+    w.assign(C * max_period, 0.0f);
+    periods.resize(C);
+    inv.assign(C, 1.0f);
+    std::vector<double> wd;
+    for (size_t c = 0; c < C; ++c) {
+        weights_f64(combos[c].period, combos[c].offset, combos[c].sigma, wd);
+        for (size_t k = 0; k < combos[c].period; ++k) w[c * max_period + k] = (float)wd[k];
+        periods[c] = (int32_t)combos[c].period;
+    }
+}
+
+constexpr int kFp32TT = 1024;
+constexpr int kFp32CB = 8;
+constexpr int kTmTT = 256;
+
+vta_status launch_fp32(vta_ctx *ctx, const float *d_prices, const float *d_weights,
+                       const int32_t *d_periods, const float *d_inv, int max_period, int T, int C,
+                       int fv, float *d_out, bool time_major) {
+    size_t lds_limit = ctx->props.sharedMemPerBlock;
+    hipFunction_t f;
+    if (!time_major) {
+        int p4max = (max_period + 3) & ~3;
+        size_t smem = (size_t)(p4max + kFp32TT + 8 + kFp32CB * p4max) * sizeof(float);
+        if (smem <= lds_limit) {
+            VTA_TRY(get_function(ctx, "alma", "alma_batch_f32", &f));
+            dim3 grid((T + kFp32TT - 1) / kFp32TT, (C + kFp32CB - 1) / kFp32CB, 1);
+            // grid.y chunking (alma_wrapper.rs:357-363), with the per-combo
+            // pointers offset by the chunk start (the reference does not, :1105-1128)
+            const unsigned max_gy = (unsigned)ctx->props.maxGridSize[1];
+            for (unsigned y0 = 0; y0 < grid.y; y0 += max_gy) {
+                unsigned gy = std::min(max_gy, grid.y - y0);
+                size_t c0 = (size_t)y0 * kFp32CB;
+                int cn = (int)std::min((size_t)C - c0, (size_t)gy * kFp32CB);
+                VTA_TRY(launch(ctx, f, dim3(grid.x, gy, 1), dim3(256), (unsigned)smem, d_prices,
+                               d_weights + c0 * max_period, d_periods + c0, d_inv + c0,
+                               max_period, T, cn, fv, d_out + c0 * (size_t)T));
+            }
+            return VTA_OK;
+        }
+    } else {
+        size_t smem = (size_t)(max_period + kTmTT + 4 + max_period * 64) * sizeof(float);
+        if (smem <= lds_limit && (size_t)(C + 63) / 64 <= (size_t)ctx->props.maxGridSize[1]) {
+            VTA_TRY(get_function(ctx, "alma", "alma_batch_f32_tm", &f));
+            dim3 grid((T + kTmTT - 1) / kTmTT, (C + 63) / 64, 1);
+            return launch(ctx, f, grid, dim3(256), (unsigned)smem, d_prices, d_weights,
+                          d_periods, d_inv, max_period, T, C, fv, d_out);
+        }
+    }
+    VTA_TRY(get_function(ctx, "alma", "alma_batch_f32_direct", &f));
+    size_t total = (size_t)T * (size_t)C;
+    size_t blocks = (total + 255) / 256;
+    if (blocks > (size_t)ctx->props.maxGridSize[0])
+        return fail(VTA_ERR_LAUNCH_CONFIG_TOO_LARGE, "launch config too large: grid=(%zu,1,1)",
+                    blocks);
+    int tm = time_major ? 1 : 0;
+    return launch(ctx, f, dim3((unsigned)blocks), dim3(256), 0, d_prices, d_weights, d_periods,
+                  d_inv, max_period, T, C, fv, d_out, tm);
+}
+
+}  // namespace
+
+// Host-side mirror of struct alma_mf_args in kernels/alma.hip (same field
+// order and types, so the same layout).
+struct alma_mf_args_host {
+    const float *prices;
+    const void *frags;
+    const void *nblocks;
+    const int32_t *sorted_period;
+    const int32_t *sorted_row;
+    const float *w_f32;
+    float *out;
+    int series_len;
+    int first_valid;
+    int n_nblocks;
+    int tiles_per_wave;
+    int n_tiles;
+    int kmax;
+};
+
+struct vta_alma_sweep_plan {
+    vta_alma_batch_kernel kernel = VTA_ALMA_BATCH_FP32;
+    size_t series_len = 0;
+    size_t n_combos = 0;
+    size_t max_period = 0;
+    // FP32 form
+    DevBuf w, periods, inv;
+    // MFMA form
+    DevBuf frags, nblocks, sorted_period, sorted_row, w_f32;
+    int n_nblocks = 0;
+    int kmax = 0;
+};
+
+namespace {
+
+constexpr int kMfTilesPerWave = 16;
+constexpr size_t kMfMaxPeriod = 256;
+
+// Builds the MFMA operand tables.  Combos are sorted by period (stable) and
+// cut into N-blocks of 32; block b contracts over K_b = round_up(max period of
+// the block, 16) taps with shorter periods zero padded at the front.  Each
+// weight is split hi = bf16(w), lo = bf16(w - hi) and stored in
+// v_mfma_f32_32x32x16_bf16 B-operand order: for K-step s and half h, lane l
+// holds taps 16s + 8(l/32) + j (j < 8) of combo l%32.
+vta_status build_mfma_tables(vta_alma_sweep_plan *plan,
+                             const std::vector<vta_alma_params> &combos, vta_ctx *ctx) {
+    const size_t C = combos.size();
+    std::vector<size_t> order(C);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t b) { return combos[a].period < combos[b].period; });
+    const int nb = (int)((C + 31) / 32);
+    const size_t cpad = (size_t)nb * 32;
+    std::vector<int32_t> sp(cpad, 1), sr(cpad, -1);
+    std::vector<int32_t> nbd((size_t)nb * 4, 0);
+    std::vector<uint16_t> frags;
+    int kmax = 0;
+    for (int b = 0; b < nb; ++b) {
+        size_t maxp = 0;
+        for (int n = 0; n < 32; ++n) {
+            size_t i = (size_t)b * 32 + n;
+            if (i < C) maxp = std::max(maxp, combos[order[i]].period);
+        }
+        int K = (int)((maxp + 15) / 16) * 16;
+        kmax = std::max(kmax, K);
+        nbd[b * 4 + 0] = K / 16;
+        nbd[b * 4 + 1] = (int32_t)(frags.size() / 8);  // offset in 16-byte chunks
+        nbd[b * 4 + 2] = b * 32;
+        // W[k][n], zero padded at the front
+        std::vector<double> W((size_t)K * 32, 0.0), wd;
+        for (int n = 0; n < 32; ++n) {
+            size_t i = (size_t)b * 32 + n;
+            if (i >= C) continue;
+            const auto &cp = combos[order[i]];
+            weights_f64(cp.period, cp.offset, cp.sigma, wd);
+            size_t lead = (size_t)K - cp.period;
+            for (size_t k = 0; k < cp.period; ++k) W[(lead + k) * 32 + n] = wd[k];
+            sp[i] = (int32_t)cp.period;
+            sr[i] = (int32_t)order[i];
+        }
+        for (int s = 0; s < K / 16; ++s) {
+            for (int h = 0; h < 2; ++h) {
+                for (int l = 0; l < 64; ++l) {
+                    for (int j = 0; j < 8; ++j) {
+                        int k = 16 * s + 8 * (l / 32) + j;
+                        double w = W[(size_t)k * 32 + (l % 32)];
+                        uint16_t hi = bf16_rne((float)w);
+                        uint16_t lo = bf16_rne((float)(w - (double)bf16_to_f32(hi)));
+                        frags.push_back(h == 0 ? hi : lo);
+                    }
+                }
+            }
+        }
+    }
+    // f32 weight table for the non-finite fallback, [cpad][kmax], taps from
+    // the window start.
+    std::vector<float> wf(cpad * (size_t)kmax, 0.0f);
+    std::vector<double> wd;
+    for (size_t i = 0; i < C; ++i) {
+        const auto &cp = combos[order[i]];
+        weights_f64(cp.period, cp.offset, cp.sigma, wd);
+        for (size_t k = 0; k < cp.period; ++k) wf[i * kmax + k] = (float)wd[k];
+    }
+    VTA_TRY(upload(ctx, plan->frags, frags.data(), frags.size() * sizeof(uint16_t)));
+    VTA_TRY(upload(ctx, plan->nblocks, nbd.data(), nbd.size() * sizeof(int32_t)));
+    VTA_TRY(upload(ctx, plan->sorted_period, sp.data(), sp.size() * sizeof(int32_t)));
+    VTA_TRY(upload(ctx, plan->sorted_row, sr.data(), sr.size() * sizeof(int32_t)));
+    VTA_TRY(upload(ctx, plan->w_f32, wf.data(), wf.size() * sizeof(float)));
+    plan->n_nblocks = nb;
+    plan->kmax = kmax;
+    return VTA_OK;
+}
+
+vta_alma_batch_kernel resolve_kernel(vta_alma_batch_kernel policy, size_t n_combos,
+                                     size_t max_period) {
+    if (policy != VTA_ALMA_BATCH_AUTO) return policy;
+    // The matrix-core form needs wide sweeps to fill 32-combo N-blocks.
+    if (n_combos >= 16 && max_period <= kMfMaxPeriod) return VTA_ALMA_BATCH_MFMA_BF16X3;
+    return VTA_ALMA_BATCH_FP32;
+}
+
+vta_status plan_create(vta_ctx *ctx, const std::vector<vta_alma_params> &combos, size_t T,
+                       size_t max_period, vta_alma_batch_kernel kernel,
+                       vta_alma_sweep_plan **out) {
+    auto *plan = new vta_alma_sweep_plan();
+    plan->series_len = T;
+    plan->n_combos = combos.size();
+    plan->max_period = max_period;
+    plan->kernel = kernel;
+    vta_status st = VTA_OK;
+    if (kernel == VTA_ALMA_BATCH_MFMA_BF16X3) {
+        if (max_period > kMfMaxPeriod) {
+            delete plan;
+            return fail(VTA_ERR_INVALID_POLICY,
+                        "invalid policy: MFMA sweep supports periods <= %zu (got %zu)",
+                        kMfMaxPeriod, max_period);
+        }
+        st = build_mfma_tables(plan, combos, ctx);
+    } else if (kernel == VTA_ALMA_BATCH_FP32) {
+        std::vector<float> w, inv;
+        std::vector<int32_t> per;
+        weights_table_f32(combos, max_period, w, per, inv);
+        st = upload(ctx, plan->w, w.data(), w.size() * sizeof(float));
+        if (st == VTA_OK) st = upload(ctx, plan->periods, per.data(), per.size() * 4);
+        if (st == VTA_OK) st = upload(ctx, plan->inv, inv.data(), inv.size() * 4);
+    } else {
+        st = fail(VTA_ERR_INVALID_POLICY, "invalid policy");
+    }
+    if (st != VTA_OK) {
+        delete plan;
+        return st;
+    }
+    *out = plan;
+    return VTA_OK;
+}
+
+vta_status plan_launch(vta_ctx *ctx, const vta_alma_sweep_plan *plan, const float *d_prices,
+                       size_t fv, float *d_out) {
+    const int T = (int)plan->series_len;
+    if (plan->kernel == VTA_ALMA_BATCH_MFMA_BF16X3) {
+        hipFunction_t f;
+        VTA_TRY(get_function(ctx, "alma", "alma_batch_mfma_bf16x3", &f));
+        alma_mf_args_host a;
+        a.prices = d_prices;
+        a.frags = plan->frags.p;
+        a.nblocks = plan->nblocks.p;
+        a.sorted_period = plan->sorted_period.as<int32_t>();
+        a.sorted_row = plan->sorted_row.as<int32_t>();
+        a.w_f32 = plan->w_f32.as<float>();
+        a.out = d_out;
+        a.series_len = T;
+        a.first_valid = (int)fv;
+        a.n_nblocks = plan->n_nblocks;
+        a.tiles_per_wave = kMfTilesPerWave;
+        a.n_tiles = (T + 31) / 32;
+        a.kmax = plan->kmax;
+        dim3 grid((a.n_tiles + kMfTilesPerWave - 1) / kMfTilesPerWave, plan->n_nblocks, 1);
+        ctx->alma_selected = VTA_ALMA_BATCH_MFMA_BF16X3;
+        return launch(ctx, f, grid, dim3(64), 0, a);
+    }
+    ctx->alma_selected = VTA_ALMA_BATCH_FP32;
+    return launch_fp32(ctx, d_prices, plan->w.as<float>(), plan->periods.as<int32_t>(),
+                       plan->inv.as<float>(), (int)plan->max_period, T, (int)plan->n_combos,
+                       (int)fv, d_out, false);
+}
+
+// run_batch_with_prices_device (alma_wrapper.rs:683-781).
+vta_status run_batch(vta_ctx *ctx, const float *d_prices, size_t T, size_t fv,
+                     const std::vector<vta_alma_params> &combos, size_t max_period,
+                     vta_device_array_f32 *out) {
+    size_t n_out;
+    if (!mul_ok(combos.size(), T, &n_out) || !mul_ok(n_out, sizeof(float), &n_out))
+        return fail(VTA_ERR_INVALID_INPUT, "n_combos*series_len overflow");
+    vta_alma_batch_kernel k = resolve_kernel(ctx->alma_policy, combos.size(), max_period);
+    vta_alma_sweep_plan *plan = nullptr;
+    VTA_TRY(plan_create(ctx, combos, T, max_period, k, &plan));
+    DevBuf dout;
+    vta_status st = dout.alloc(n_out);
+    if (st == VTA_OK) st = plan_launch(ctx, plan, d_prices, fv, dout.as<float>());
+    if (st == VTA_OK) {
+        hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) st = fail(VTA_ERR_HIP, "hipStreamSynchronize: %s", hipGetErrorString(e));
+    }
+    delete plan;
+    if (st != VTA_OK) return st;
+    out->ptr = static_cast<float *>(dout.release());
+    out->rows = combos.size();
+    out->cols = T;
+    return VTA_OK;
+}
+
+vta_status prepare_host(const float *prices, size_t len, const vta_alma_batch_range *range,
+                        std::vector<vta_alma_params> &combos, size_t *fv, size_t *max_period) {
+    if (!prices || len == 0) return fail(VTA_ERR_INVALID_INPUT, "empty data");
+    *fv = first_valid_f32(prices, len);
+    if (*fv >= len) return fail(VTA_ERR_INVALID_INPUT, "all values are NaN");
+    VTA_TRY(expand(range, combos));
+    return validate_combos(combos, len, *fv, max_period);
+}
+
+}  // namespace
+
+extern "C" {
+
+vta_status vta_alma_expand_grid(const vta_alma_batch_range *range, vta_alma_params *out,
+                                size_t cap, size_t *n_out) {
+    clear_error();
+    std::vector<vta_alma_params> v;
+    VTA_TRY(expand(range, v));
+    if (n_out) *n_out = v.size();
+    if (out) memcpy(out, v.data(), std::min(cap, v.size()) * sizeof(vta_alma_params));
+    return VTA_OK;
+}
+
+vta_status vta_alma_set_batch_policy(vta_ctx *ctx, vta_alma_batch_kernel policy) {
+    if (!ctx) return fail(VTA_ERR_INVALID_INPUT, "ctx is null");
+    if (policy < VTA_ALMA_BATCH_AUTO || policy > VTA_ALMA_BATCH_MFMA_BF16X3)
+        return fail(VTA_ERR_INVALID_POLICY, "invalid policy: %d", (int)policy);
+    ctx->alma_policy = policy;
+    return VTA_OK;
+}
+
+vta_alma_batch_kernel vta_alma_selected_batch_kernel(const vta_ctx *ctx) {
+    return ctx ? ctx->alma_selected : VTA_ALMA_BATCH_AUTO;
+}
+
+vta_status vta_alma_batch_dev(vta_ctx *ctx, const float *prices, size_t len,
+                              const vta_alma_batch_range *range, vta_device_array_f32 *out) {
+    clear_error();
+    if (!ctx || !out) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    std::vector<vta_alma_params> combos;
+    size_t fv, mp;
+    VTA_TRY(prepare_host(prices, len, range, combos, &fv, &mp));
+    size_t req = len * sizeof(float), o;
+    if (!mul_ok(combos.size(), len, &o) || !mul_ok(o, sizeof(float), &o) || !add_ok(req, o, &req))
+        return fail(VTA_ERR_INVALID_INPUT, "output bytes overflow");
+    VTA_TRY(check_fits(req));
+    DevBuf dp;
+    VTA_TRY(upload(ctx, dp, prices, len * sizeof(float)));
+    return run_batch(ctx, dp.as<float>(), len, fv, combos, mp, out);
+}
+
+vta_status vta_alma_batch_from_device_prices(vta_ctx *ctx, const float *d_prices, size_t len,
+                                             size_t first_valid,
+                                             const vta_alma_batch_range *range,
+                                             vta_device_array_f32 *out) {
+    clear_error();
+    if (!ctx || !out || !d_prices) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    if (len == 0) return fail(VTA_ERR_INVALID_INPUT, "series_len is zero");
+    std::vector<vta_alma_params> combos;
+    size_t mp;
+    VTA_TRY(expand(range, combos));
+    VTA_TRY(validate_combos(combos, len, first_valid, &mp));
+    return run_batch(ctx, d_prices, len, first_valid, combos, mp, out);
+}
+
+vta_status vta_alma_batch_into_host_f32(vta_ctx *ctx, const float *prices, size_t len,
+                                        const vta_alma_batch_range *range, float *out,
+                                        size_t out_len) {
+    clear_error();
+    if (!ctx || !out) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    std::vector<vta_alma_params> combos;
+    size_t fv, mp;
+    VTA_TRY(prepare_host(prices, len, range, combos, &fv, &mp));
+    size_t expected;
+    if (!mul_ok(combos.size(), len, &expected))
+        return fail(VTA_ERR_INVALID_INPUT, "combos*series_len overflow");
+    if (out_len != expected)
+        return fail(VTA_ERR_INVALID_INPUT, "out slice wrong length: got %zu, expected %zu",
+                    out_len, expected);
+    DevBuf dp;
+    VTA_TRY(upload(ctx, dp, prices, len * sizeof(float)));
+    vta_device_array_f32 arr{};
+    VTA_TRY(run_batch(ctx, dp.as<float>(), len, fv, combos, mp, &arr));
+    vta_status st = vta_device_array_copy_to_host(ctx, &arr, out, out_len);
+    vta_device_array_free(ctx, &arr);
+    return st;
+}
+
+vta_status vta_alma_batch_device(vta_ctx *ctx, const float *d_prices, const float *d_weights,
+                                 const int32_t *d_periods, const float *d_inv_norms,
+                                 int32_t max_period, int32_t series_len, int32_t n_combos,
+                                 int32_t first_valid, float *d_out) {
+    clear_error();
+    if (!ctx) return fail(VTA_ERR_INVALID_INPUT, "ctx is null");
+    if (series_len <= 0 || n_combos <= 0 || max_period <= 0)
+        return fail(VTA_ERR_INVALID_INPUT,
+                    "series_len, n_combos, and max_period must be positive");
+    ctx->alma_selected = VTA_ALMA_BATCH_FP32;
+    return launch_fp32(ctx, d_prices, d_weights, d_periods, d_inv_norms, max_period, series_len,
+                       n_combos, std::max(first_valid, 0), d_out, false);
+}
+
+vta_status vta_alma_batch_device_tm(vta_ctx *ctx, const float *d_prices, const float *d_weights,
+                                    const int32_t *d_periods, const float *d_inv_norms,
+                                    int32_t max_period, int32_t series_len, int32_t n_combos,
+                                    int32_t first_valid, float *d_out_tm) {
+    clear_error();
+    if (!ctx) return fail(VTA_ERR_INVALID_INPUT, "ctx is null");
+    if (series_len <= 0 || n_combos <= 0 || max_period <= 0)
+        return fail(VTA_ERR_INVALID_INPUT,
+                    "series_len, n_combos, and max_period must be positive");
+    ctx->alma_selected = VTA_ALMA_BATCH_FP32;
+    return launch_fp32(ctx, d_prices, d_weights, d_periods, d_inv_norms, max_period, series_len,
+                       n_combos, std::max(first_valid, 0), d_out_tm, true);
+}
+
+vta_status vta_alma_sweep_plan_create(vta_ctx *ctx, const vta_alma_batch_range *range,
+                                      size_t series_len, vta_alma_batch_kernel kernel,
+                                      vta_alma_sweep_plan **out) {
+    clear_error();
+    if (!ctx || !out) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    if (series_len == 0 || series_len > (size_t)INT32_MAX)
+        return fail(VTA_ERR_INVALID_INPUT, "invalid series_len %zu", series_len);
+    std::vector<vta_alma_params> combos;
+    VTA_TRY(expand(range, combos));
+    size_t mp = 0;
+    for (auto &c : combos) {
+        if (c.period == 0 || !(c.sigma > 0.0) || !(c.offset >= 0.0 && c.offset <= 1.0))
+            return fail(VTA_ERR_INVALID_INPUT, "invalid params: period=%zu, offset=%g, sigma=%g",
+                        c.period, c.offset, c.sigma);
+        mp = std::max(mp, c.period);
+    }
+    if (mp > series_len)
+        return fail(VTA_ERR_INVALID_INPUT, "not enough valid data (needed >= %zu, valid = %zu)",
+                    mp, series_len);
+    vta_alma_batch_kernel k = resolve_kernel(kernel, combos.size(), mp);
+    return plan_create(ctx, combos, series_len, mp, k, out);
+}
+
+vta_status vta_alma_sweep_plan_destroy(vta_ctx *ctx, vta_alma_sweep_plan *plan) {
+    if (ctx) (void)hipStreamSynchronize(ctx->stream);
+    delete plan;
+    return VTA_OK;
+}
+
+size_t vta_alma_sweep_plan_combos(const vta_alma_sweep_plan *plan) {
+    return plan ? plan->n_combos : 0;
+}
+
+vta_status vta_alma_sweep_launch(vta_ctx *ctx, const vta_alma_sweep_plan *plan,
+                                 const float *d_prices, size_t first_valid, float *d_out) {
+    clear_error();
+    if (!ctx || !plan || !d_prices || !d_out) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    if (first_valid >= plan->series_len || plan->series_len - first_valid < plan->max_period)
+        return fail(VTA_ERR_INVALID_INPUT, "not enough valid data (needed >= %zu, valid = %zu)",
+                    plan->max_period,
+                    first_valid < plan->series_len ? plan->series_len - first_valid : (size_t)0);
+    return plan_launch(ctx, plan, d_prices, first_valid, d_out);
+}
+
+vta_status vta_alma_many_series_one_param_time_major_device(
+    vta_ctx *ctx, const float *d_prices_tm, const float *d_weights, size_t period,
+    float inv_norm, size_t cols, size_t rows, const int32_t *d_first_valids, float *d_out_tm) {
+    clear_error();
+    if (!ctx) return fail(VTA_ERR_INVALID_INPUT, "ctx is null");
+    if (cols == 0 || rows == 0 || period == 0)
+        return fail(VTA_ERR_INVALID_INPUT, "cols, rows, and period must be positive");
+    if (cols > (size_t)INT32_MAX || rows > (size_t)INT32_MAX || period > rows)
+        return fail(VTA_ERR_INVALID_INPUT, "invalid dimensions");
+    hipFunction_t f;
+    // largest tile height (multiple of 16) whose [tt + period - 1][64] tile fits the LDS
+    size_t limit = ctx->props.sharedMemPerBlock / sizeof(float);
+    long tt = (long)((limit - period) / 64) - (long)(period - 1);
+    tt = (tt / 16) * 16;
+    if (tt >= 16 && (size_t)((cols + 63) / 64) <= (size_t)ctx->props.maxGridSize[1]) {
+        tt = std::min(tt, 256L);
+        VTA_TRY(get_function(ctx, "alma", "alma_ms1p_f32", &f));
+        size_t smem = ((size_t)(tt + period - 1) * 64 + period) * sizeof(float);
+        dim3 grid((unsigned)((rows + tt - 1) / tt), (unsigned)((cols + 63) / 64), 1);
+        return launch(ctx, f, grid, dim3(256), (unsigned)smem, d_prices_tm, d_weights,
+                      (int)period, inv_norm, (int)cols, (int)rows, d_first_valids, d_out_tm,
+                      (int)tt);
+    }
+    VTA_TRY(get_function(ctx, "alma", "alma_ms1p_f32_direct", &f));
+    size_t blocks = (rows * cols + 255) / 256;
+    return launch(ctx, f, dim3((unsigned)blocks), dim3(256), 0, d_prices_tm, d_weights,
+                  (int)period, inv_norm, (int)cols, (int)rows, d_first_valids, d_out_tm);
+}
+
+vta_status vta_alma_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *data_tm,
+                                                         size_t cols, size_t rows,
+                                                         const vta_alma_params *params,
+                                                         vta_device_array_f32 *out) {
+    clear_error();
+    if (!ctx || !out || !params || !data_tm) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    if (cols == 0 || rows == 0) return fail(VTA_ERR_INVALID_INPUT, "cols or rows is zero");
+    size_t elems;
+    if (!mul_ok(cols, rows, &elems)) return fail(VTA_ERR_INVALID_INPUT, "cols*rows overflow");
+    const size_t period = params->period;
+    if (period == 0 || !(params->sigma > 0.0) || !(params->offset >= 0.0 && params->offset <= 1.0))
+        return fail(VTA_ERR_INVALID_INPUT, "invalid params: period=%zu, offset=%g, sigma=%g",
+                    period, params->offset, params->sigma);
+    // prepare_many_series_inputs (alma_wrapper.rs:507-563): per-series first_valid
+    std::vector<int32_t> fvs(cols);
+    for (size_t s = 0; s < cols; ++s) {
+        size_t t = 0;
+        while (t < rows && std::isnan(data_tm[t * cols + s])) ++t;
+        if (t >= rows) return fail(VTA_ERR_INVALID_INPUT, "series %zu all NaN", s);
+        if (rows - t < period)
+            return fail(VTA_ERR_INVALID_INPUT,
+                        "series %zu not enough valid data (needed >= %zu, valid = %zu)", s,
+                        period, rows - t);
+        fvs[s] = (int32_t)t;
+    }
+    VTA_TRY(check_fits(2 * elems * sizeof(float) + period * sizeof(float)));
+    std::vector<double> wd;
+    weights_f64(period, params->offset, params->sigma, wd);
+    std::vector<float> wf(wd.begin(), wd.end());
+    DevBuf dp, dw, dfv, dout;
+    VTA_TRY(upload(ctx, dp, data_tm, elems * sizeof(float)));
+    VTA_TRY(upload(ctx, dw, wf.data(), wf.size() * sizeof(float)));
+    VTA_TRY(upload(ctx, dfv, fvs.data(), fvs.size() * sizeof(int32_t)));
+    VTA_TRY(dout.alloc(elems * sizeof(float)));
+    VTA_TRY(vta_alma_many_series_one_param_time_major_device(ctx, dp.as<float>(), dw.as<float>(),
+                                                             period, 1.0f, cols, rows,
+                                                             dfv.as<int32_t>(),
+                                                             dout.as<float>()));
+    VTA_HIP(hipStreamSynchronize(ctx->stream));
+    out->ptr = static_cast<float *>(dout.release());
+    out->rows = rows;
+    out->cols = cols;
+    return VTA_OK;
+}
+
+}  // extern "C"

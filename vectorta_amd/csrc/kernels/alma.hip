@@ -1,0 +1,491 @@
+// ALMA kernels for gfx950 (MI355X).
+//
+// Semantics follow the reference GPU path (kernels/cuda/moving_averages/
+// alma_kernel.cu, launched from src/cuda/moving_averages/alma_wrapper.rs):
+// out[c][t] = sum_k w_c[k] * x[t - p_c + 1 + k] * inv_norm_c for
+// t >= first_valid + p_c - 1, NaN before.  The CPU oracle is alma.rs:426-463 /
+// 1505-1527 (f64).
+//
+//   alma_batch_f32         one series x many combos, row-major [combo][t];
+//                          LDS-staged price tile, 4 outputs per lane, FP32 FMA.
+//   alma_batch_f32_tm      same, time-major output [t][combo].
+//   alma_ms1p_f32          many series (time-major) x one param.
+//   alma_batch_mfma_bf16x3 one series x many combos on the matrix cores: a
+//                          Hankel tile of anchored prices (M = 32 bars) times a
+//                          per-combo weight matrix (N = 32 combos), split-BF16
+//                          hi/lo with three products and FP32 accumulation.
+#include "vta_common.h"
+
+// ---------------------------------------------------------------- FP32 sliding dot
+//
+// Block: 256 lanes, time tile of 1024 bars (4 consecutive outputs per lane),
+// ALMA_CB combos per block sharing the staged price tile.  Weights are front
+// padded with zeros to a multiple of 4 taps so every price read is an aligned
+// 16-byte LDS load; each 4-tap step is 16 FMAs against two float4 of prices.
+#define ALMA_TT 1024
+#define ALMA_CB 8
+
+extern "C" __global__ __launch_bounds__(256) void alma_batch_f32(
+    const float *__restrict__ prices, const float *__restrict__ weights,
+    const int32_t *__restrict__ periods, const float *__restrict__ inv_norms, int max_period,
+    int series_len, int n_combos, int first_valid, float *__restrict__ out) {
+    extern __shared__ float4 smem4[];
+    float *smem = reinterpret_cast<float *>(smem4);
+    const int p4max = (max_period + 3) & ~3;
+    const int xs_len = p4max + ALMA_TT + 8;
+    float *xs = smem;                 // xs[e] = x[t0 - p4max + e]
+    float *ws = smem + xs_len;        // [ALMA_CB][p4max]
+    const int t0 = blockIdx.x * ALMA_TT;
+    const int c0 = blockIdx.y * ALMA_CB;
+    const int tid = threadIdx.x;
+
+    // Staged prices are sanitised (0 outside [first_valid, T) and for
+    // non-finite values) because zero-padded taps read bars outside each
+    // combo's window; a block whose tile holds a non-finite input at or after
+    // first_valid takes the exact per-window path below instead.
+    int bad = 0;
+    for (int e = tid; e < xs_len; e += 256) {
+        long idx = (long)t0 - p4max + e;
+        float v = 0.0f;
+        if (idx >= first_valid && idx < series_len) {
+            float x = prices[idx];
+            if (vta_isfinite(x)) v = x; else bad = 1;
+        }
+        xs[e] = v;
+    }
+    for (int cc = 0; cc < ALMA_CB; ++cc) {
+        int c = c0 + cc;
+        if (c >= n_combos) break;
+        int p = periods[c];
+        int p4 = (p + 3) & ~3;
+        float inv = inv_norms[c];
+        for (int k = tid; k < p4max; k += 256) {
+            int src = k - (p4 - p);
+            ws[cc * p4max + k] =
+                (k < p4 && src >= 0) ? weights[(size_t)c * max_period + src] * inv : 0.0f;
+        }
+    }
+    const int block_bad = __syncthreads_or(bad);
+
+    const int tl = tid * 4;
+    const bool vec_ok = ((series_len & 3) == 0) && (t0 + tl + 3 < series_len);
+    for (int cc = 0; cc < ALMA_CB; ++cc) {
+        const int c = c0 + cc;
+        if (c >= n_combos) break;
+        const int p = periods[c];
+        const int p4 = (p + 3) & ~3;
+        const float *w = ws + cc * p4max;
+        // output t = t0+tl+j uses x[t - p4 + 1 + k], k < p4 -> xs[tl + j + p4max - p4 + 1 + k]
+        const float *xb = xs + tl + (p4max - p4);
+        float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+        float4 A = *reinterpret_cast<const float4 *>(xb);
+        // xb is 16-byte aligned; the window starts one element in (offset +1).
+        for (int k = 0; k < p4 && !block_bad; k += 4) {
+            const float4 B = *reinterpret_cast<const float4 *>(xb + k + 4);
+            const float4 W = *reinterpret_cast<const float4 *>(w + k);
+            // window element q of output j is x at offset (k + 1 + j + q)
+            a0 = fmaf(W.x, A.y, a0); a0 = fmaf(W.y, A.z, a0); a0 = fmaf(W.z, A.w, a0); a0 = fmaf(W.w, B.x, a0);
+            a1 = fmaf(W.x, A.z, a1); a1 = fmaf(W.y, A.w, a1); a1 = fmaf(W.z, B.x, a1); a1 = fmaf(W.w, B.y, a1);
+            a2 = fmaf(W.x, A.w, a2); a2 = fmaf(W.y, B.x, a2); a2 = fmaf(W.z, B.y, a2); a2 = fmaf(W.w, B.z, a2);
+            a3 = fmaf(W.x, B.x, a3); a3 = fmaf(W.y, B.y, a3); a3 = fmaf(W.z, B.z, a3); a3 = fmaf(W.w, B.w, a3);
+            A = B;
+        }
+        const int warm = first_valid + p - 1;
+        const int t = t0 + tl;
+        if (block_bad) {
+            // exact window, taps in order, NaN/Inf propagate as in the reference
+            const float inv = inv_norms[c];
+            const float *wr = weights + (size_t)c * max_period;
+            float acc[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                float s = 0.0f;
+                const int tj = t + j;
+                if (tj >= warm && tj < series_len)
+                    for (int k = 0; k < p; ++k) s = fmaf(wr[k] * inv, prices[tj - p + 1 + k], s);
+                acc[j] = s;
+            }
+            a0 = acc[0]; a1 = acc[1]; a2 = acc[2]; a3 = acc[3];
+        }
+        float r0 = (t + 0 >= warm) ? a0 : VTA_QNAN_F32;
+        float r1 = (t + 1 >= warm) ? a1 : VTA_QNAN_F32;
+        float r2 = (t + 2 >= warm) ? a2 : VTA_QNAN_F32;
+        float r3 = (t + 3 >= warm) ? a3 : VTA_QNAN_F32;
+        float *row = out + (size_t)c * series_len;
+        if (vec_ok) {
+            *reinterpret_cast<float4 *>(row + t) = make_float4(r0, r1, r2, r3);
+        } else {
+            if (t + 0 < series_len) row[t + 0] = r0;
+            if (t + 1 < series_len) row[t + 1] = r1;
+            if (t + 2 < series_len) row[t + 2] = r2;
+            if (t + 3 < series_len) row[t + 3] = r3;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- FP32, time-major out
+//
+// Lanes map to combos so each output row [t][c0..c0+63] is one coalesced
+// store.  Block: 64 combos x 4 waves; each wave walks 64 bars of the tile with
+// four outputs in flight.  Weights are stored tap-major in LDS ([k][64]) so a
+// wave reads 64 consecutive floats per tap; prices are wave-uniform broadcasts.
+#define ALMA_TM_TT 256
+extern "C" __global__ __launch_bounds__(256) void alma_batch_f32_tm(
+    const float *__restrict__ prices, const float *__restrict__ weights,
+    const int32_t *__restrict__ periods, const float *__restrict__ inv_norms, int max_period,
+    int series_len, int n_combos, int first_valid, float *__restrict__ out_tm) {
+    extern __shared__ float4 smem4[];
+    float *smem = reinterpret_cast<float *>(smem4);
+    const int xs_len = max_period + ALMA_TM_TT + 4;
+    float *xs = smem;               // xs[e] = x[t0 - max_period + 1 + e]
+    float *ws = smem + xs_len;      // ws[k*64 + lane] = w_c[k - (max_period - p_c)]
+    const int t0 = blockIdx.x * ALMA_TM_TT;
+    const int c0 = blockIdx.y * 64;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wv = tid >> 6;
+    int bad = 0;
+    for (int e = tid; e < xs_len; e += 256) {
+        long idx = (long)t0 - max_period + 1 + e;
+        float v = 0.0f;
+        if (idx >= first_valid && idx < series_len) {
+            float x = prices[idx];
+            if (vta_isfinite(x)) v = x; else bad = 1;
+        }
+        xs[e] = v;
+    }
+    for (int i = tid; i < max_period * 64; i += 256) {
+        int k = i >> 6, cl = i & 63, c = c0 + cl;
+        float v = 0.0f;
+        if (c < n_combos) {
+            int p = periods[c];
+            int src = k - (max_period - p);
+            if (src >= 0) v = weights[(size_t)c * max_period + src] * inv_norms[c];
+        }
+        ws[i] = v;
+    }
+    const int block_bad = __syncthreads_or(bad);
+    const int c = c0 + lane;
+    const int p = (c < n_combos) ? periods[c] : 1;
+    const int warm = first_valid + p - 1;
+    for (int tb = wv * 4; tb < ALMA_TM_TT; tb += 16) {
+        // outputs t0+tb .. t0+tb+3, window xs[tb + j + k], k < max_period
+        float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+        float x0 = xs[tb], x1 = xs[tb + 1], x2 = xs[tb + 2];
+        for (int k = 0; k < max_period; ++k) {
+            float x3 = xs[tb + k + 3];
+            float w = ws[k * 64 + lane];
+            a0 = fmaf(w, x0, a0);
+            a1 = fmaf(w, x1, a1);
+            a2 = fmaf(w, x2, a2);
+            a3 = fmaf(w, x3, a3);
+            x0 = x1; x1 = x2; x2 = x3;
+        }
+        if (c < n_combos) {
+            float acc[4] = {a0, a1, a2, a3};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                int t = t0 + tb + j;
+                if (block_bad && t >= warm && t < series_len) {
+                    const float *wr = weights + (size_t)c * max_period;
+                    const float inv = inv_norms[c];
+                    float s = 0.0f;
+                    for (int k = 0; k < p; ++k) s = fmaf(wr[k] * inv, prices[t - p + 1 + k], s);
+                    acc[j] = s;
+                }
+                if (t < series_len)
+                    out_tm[(size_t)t * n_combos + c] = (t >= warm) ? acc[j] : VTA_QNAN_F32;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- many series, one param
+//
+// Time-major [T][N] input: lanes map to series (coalesced rows).  A block
+// stages a [MS_TT + period - 1][64] tile in LDS (MS_TT, a multiple of 16, is
+// chosen by the host to fit the LDS budget); each lane produces 4 consecutive
+// bars of its series per step.  Weights are wave-uniform.
+extern "C" __global__ __launch_bounds__(256) void alma_ms1p_f32(
+    const float *__restrict__ prices_tm, const float *__restrict__ weights, int period,
+    float inv_norm, int num_series, int series_len, const int32_t *__restrict__ first_valids,
+    float *__restrict__ out_tm, int MS_TT) {
+    extern __shared__ float4 smem4[];
+    float *tile = reinterpret_cast<float *>(smem4);   // [(MS_TT + period - 1)][64]
+    float *ws = tile + (size_t)(MS_TT + period - 1) * 64;
+    const int t0 = blockIdx.x * MS_TT;
+    const int s0 = blockIdx.y * 64;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wv = tid >> 6;
+    const int rows = MS_TT + period - 1;
+    for (int i = tid; i < rows * 64; i += 256) {
+        int r = i >> 6, sl = i & 63;
+        long t = (long)t0 - period + 1 + r;
+        int s = s0 + sl;
+        tile[i] = (t >= 0 && t < series_len && s < num_series)
+                      ? prices_tm[(size_t)t * num_series + s] : 0.0f;
+    }
+    for (int k = tid; k < period; k += 256) ws[k] = weights[k] * inv_norm;
+    __syncthreads();
+    const int s = s0 + lane;
+    if (s >= num_series) return;
+    const int warm = first_valids[s] + period - 1;
+    for (int tb = wv * 4; tb < MS_TT; tb += 16) {
+        float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+        float x0 = tile[(tb + 0) * 64 + lane];
+        float x1 = tile[(tb + 1) * 64 + lane];
+        float x2 = tile[(tb + 2) * 64 + lane];
+        for (int k = 0; k < period; ++k) {
+            float x3 = (tb + k + 3 < rows) ? tile[(tb + k + 3) * 64 + lane] : 0.0f;
+            float w = ws[k];
+            a0 = fmaf(w, x0, a0);
+            a1 = fmaf(w, x1, a1);
+            a2 = fmaf(w, x2, a2);
+            a3 = fmaf(w, x3, a3);
+            x0 = x1; x1 = x2; x2 = x3;
+        }
+        float acc[4] = {a0, a1, a2, a3};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            int t = t0 + tb + j;
+            if (t < series_len)
+                out_tm[(size_t)t * num_series + s] = (t >= warm) ? acc[j] : VTA_QNAN_F32;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- MFMA split-BF16
+//
+// Problem: out[row(n)][t] = sum_{k<K} W[k][n] * x[t - K + 1 + k] for the 32
+// combos n of an N-block, where K (a multiple of 16) covers the block's longest
+// period and shorter periods are zero padded at the front.  Each wave owns one
+// N-block and a run of 32-bar M-tiles.  Per tile it
+//   1. stages x' = x - a (a = one anchor price per tile, x' = 0 outside
+//      [first_valid, T)) as BF16 hi and lo = bf16(x' - hi) into eight copies
+//      shifted by 0..7 elements, so every A-fragment read is an aligned
+//      16-byte LDS load whatever the lane's row;
+//   2. runs K/16 steps of v_mfma_f32_32x32x16_bf16: hi*hi into one
+//      accumulator, hi*lo + lo*hi into a second;
+//   3. writes out = (acc_hi + acc_corr) + a (weights sum to 1), NaN before the
+//      warm-up of each combo.
+// Split error: |x' - hi - lo| <= 2^-18 |x'|, same for the weights, dropped
+// lo*lo <= 2^-18 |w x'|; anchoring makes |x'| a few percent of the price, so
+// the contraction carries FP32-level relative accuracy.  Tiles that contain a
+// non-finite input at or after first_valid are recomputed with a plain FP32
+// dot from the f32 weight table so NaN/Inf behave like the FP32 kernel.
+//
+// Operand layouts (v_mfma_f32_32x32x16_bf16, wave64):
+//   A (32x16): lane l holds A[l%32][8*(l/32) + j], j = 0..7
+//   B (16x32): lane l holds B[8*(l/32) + j][l%32]
+//   D (32x32): lane l, element i holds D[8*(i/4) + 4*(l/32) + i%4][l%32]
+typedef __attribute__((ext_vector_type(8))) __bf16 vta_bf16x8;
+typedef __attribute__((ext_vector_type(16))) float vta_f32x16;
+
+#define MF_COPY_STRIDE_MAX 656   // bytes per shifted copy for K <= 256 (mf_copy_stride(16))
+
+struct alma_mf_args {
+    const float *prices;
+    const uint4 *frags;          // [sum_b ks_b*2*64] 16-byte lane chunks
+    const int4 *nblocks;         // {ks, frag_off, combo_base, 0}
+    const int32_t *sorted_period;
+    const int32_t *sorted_row;   // output row, -1 for padding
+    const float *w_f32;          // [ncombos_sorted][kmax] f32 weights (fallback path)
+    float *out;
+    int series_len;
+    int first_valid;
+    int n_nblocks;
+    int tiles_per_wave;
+    int n_tiles;
+    int kmax;
+};
+
+__device__ __forceinline__ uint32_t mf_copy_stride(int ks) {
+    // copy length in elements: K + 32 (M rows) rounded to 8; bytes = 2*len,
+    // rounded to 128 and offset by 16 so the eight copies fall on distinct banks.
+    uint32_t len = (uint32_t)(ks * 16 + 32);
+    uint32_t bytes = ((2u * len + 127u) & ~127u) + 16u;
+    return bytes;
+}
+
+template <int KS>
+__device__ __forceinline__ void alma_mf_wave(const alma_mf_args &a, int nb, int tile_begin,
+                                             unsigned char *lds) {
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 5;    // K-half for A/B, row-quad selector for D
+    const int m = lane & 31;    // A row (bar in tile) / B,D column (combo)
+    const int4 nbd = a.nblocks[nb];
+    const int frag_off = nbd.y;
+    const int combo_base = nbd.z;
+    constexpr int K = KS * 16;
+    const uint32_t cstride = mf_copy_stride(KS);
+    unsigned char *lds_hi = lds;
+    unsigned char *lds_lo = lds + 8 * cstride;
+
+    // B fragments stay in registers for all tiles of this wave.
+    uint4 bhi[KS], blo[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        bhi[s] = a.frags[frag_off + (2 * s + 0) * 64 + lane];
+        blo[s] = a.frags[frag_off + (2 * s + 1) * 64 + lane];
+    }
+    const int my_period = a.sorted_period[combo_base + m];
+    const int my_row = a.sorted_row[combo_base + m];
+    const int my_warm = a.first_valid + my_period - 1;
+    const int T = a.series_len;
+    const int fv = a.first_valid;
+
+    const int tile_end = min(tile_begin + a.tiles_per_wave, a.n_tiles);
+    for (int tile = tile_begin; tile < tile_end; ++tile) {
+        const int t_tile = tile * 32;
+        const int base_idx = t_tile - K + 1;   // x index of X'[0]
+        // anchor: the price at the tile's last in-range bar at or after first_valid
+        int ai = min(t_tile + 31, T - 1);
+        if (ai < fv) ai = fv;
+        float anchor = a.prices[ai];
+        if (!vta_isfinite(anchor)) anchor = 0.0f;
+
+        // ---- stage X'[e], e in [0, K + 32), into eight shifted copies
+        bool bad = false;
+        for (int e = lane; e < K + 32; e += 64) {
+            int idx = base_idx + e;
+            float v = 0.0f;
+            if (idx >= fv && idx < T) {
+                float x = a.prices[idx];
+                if (!vta_isfinite(x)) bad = true;
+                v = x - anchor;
+            }
+            uint16_t hi = vta_bf16_rne(v);
+            uint16_t lo = vta_bf16_rne(v - vta_bf16_to_f32(hi));
+            if (!vta_isfinite(v)) { hi = 0; lo = 0; }
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                int pos = e - r;
+                if (pos >= 0) {
+                    *reinterpret_cast<uint16_t *>(lds_hi + r * cstride + 2 * pos) = hi;
+                    *reinterpret_cast<uint16_t *>(lds_lo + r * cstride + 2 * pos) = lo;
+                }
+            }
+        }
+        const bool tile_bad = __any(bad);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+        // ---- contraction
+        vta_f32x16 acc_h = {};
+        vta_f32x16 acc_c = {};
+        const int r = m & 7;
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const int e0 = m + 16 * s + 8 * g;          // first element of this lane's chunk
+            const uint32_t off = (uint32_t)r * cstride + 2u * (uint32_t)(e0 - r);
+            const uint4 ahi_u = *reinterpret_cast<const uint4 *>(lds_hi + off);
+            const uint4 alo_u = *reinterpret_cast<const uint4 *>(lds_lo + off);
+            const vta_bf16x8 ahi = __builtin_bit_cast(vta_bf16x8, ahi_u);
+            const vta_bf16x8 alo = __builtin_bit_cast(vta_bf16x8, alo_u);
+            const vta_bf16x8 bh = __builtin_bit_cast(vta_bf16x8, bhi[s]);
+            const vta_bf16x8 bl = __builtin_bit_cast(vta_bf16x8, blo[s]);
+            acc_h = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bh, acc_h, 0, 0, 0);
+            acc_c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bl, acc_c, 0, 0, 0);
+            acc_c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bh, acc_c, 0, 0, 0);
+        }
+        // LDS is rewritten by the next tile: make sure every lane's reads are done.
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+        // ---- epilogue: lane owns combo m, bars 8q + 4g + (0..3)
+        if (my_row >= 0) {
+            float *row = a.out + (size_t)my_row * (size_t)T;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int tb = t_tile + 8 * q + 4 * g;
+                float v[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int t = tb + i;
+                    float val = (acc_h[4 * q + i] + acc_c[4 * q + i]) + anchor;
+                    if (tile_bad && t >= my_warm && t < T) {
+                        // plain FP32 dot, same taps as the FP32 kernel
+                        const float *wrow = a.w_f32 + (size_t)(combo_base + m) * a.kmax;
+                        float acc = 0.0f;
+                        for (int k = 0; k < my_period; ++k)
+                            acc = fmaf(wrow[k], a.prices[t - my_period + 1 + k], acc);
+                        val = acc;
+                    }
+                    v[i] = (t >= my_warm) ? val : VTA_QNAN_F32;
+                }
+                if (((T & 3) == 0) && tb + 3 < T) {
+                    *reinterpret_cast<float4 *>(row + tb) = make_float4(v[0], v[1], v[2], v[3]);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        if (tb + i < T) row[tb + i] = v[i];
+                }
+            }
+        }
+    }
+}
+
+// grid.x = number of tile runs, grid.y = number of N-blocks; block = 64 lanes.
+extern "C" __global__ __launch_bounds__(64) void alma_batch_mfma_bf16x3(alma_mf_args a) {
+    __shared__ __attribute__((aligned(16))) unsigned char lds[16 * MF_COPY_STRIDE_MAX];
+    const int nb = blockIdx.y;
+    const int tile_begin = blockIdx.x * a.tiles_per_wave;
+    const int ks = a.nblocks[nb].x;
+    switch (ks) {
+#define MF_CASE(N) case N: alma_mf_wave<N>(a, nb, tile_begin, lds); break;
+        MF_CASE(1) MF_CASE(2) MF_CASE(3) MF_CASE(4) MF_CASE(5) MF_CASE(6) MF_CASE(7) MF_CASE(8)
+        MF_CASE(9) MF_CASE(10) MF_CASE(11) MF_CASE(12) MF_CASE(13) MF_CASE(14) MF_CASE(15)
+        MF_CASE(16)
+#undef MF_CASE
+        default: break;
+    }
+}
+
+// ---------------------------------------------------------------- direct fallbacks
+//
+// One output per lane, window read straight from global memory.  Used when a
+// period is too long for the LDS tiles above (launch-time choice on the host).
+extern "C" __global__ __launch_bounds__(256) void alma_batch_f32_direct(
+    const float *__restrict__ prices, const float *__restrict__ weights,
+    const int32_t *__restrict__ periods, const float *__restrict__ inv_norms, int max_period,
+    int series_len, int n_combos, int first_valid, float *__restrict__ out, int time_major) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t total = (size_t)series_len * (size_t)n_combos;
+    if (i >= total) return;
+    int c, t;
+    if (time_major) { t = (int)(i / n_combos); c = (int)(i % n_combos); }
+    else { c = (int)(i / series_len); t = (int)(i % series_len); }
+    const int p = periods[c];
+    const int warm = first_valid + p - 1;
+    float r = VTA_QNAN_F32;
+    if (t >= warm) {
+        const float *wr = weights + (size_t)c * max_period;
+        const float inv = inv_norms[c];
+        float s = 0.0f;
+        for (int k = 0; k < p; ++k) s = fmaf(wr[k] * inv, prices[t - p + 1 + k], s);
+        r = s;
+    }
+    out[i] = r;
+}
+
+extern "C" __global__ __launch_bounds__(256) void alma_ms1p_f32_direct(
+    const float *__restrict__ prices_tm, const float *__restrict__ weights, int period,
+    float inv_norm, int num_series, int series_len, const int32_t *__restrict__ first_valids,
+    float *__restrict__ out_tm) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t total = (size_t)series_len * (size_t)num_series;
+    if (i >= total) return;
+    const int t = (int)(i / num_series), s = (int)(i % num_series);
+    const int warm = first_valids[s] + period - 1;
+    float r = VTA_QNAN_F32;
+    if (t >= warm) {
+        float acc = 0.0f;
+        for (int k = 0; k < period; ++k)
+            acc = fmaf(weights[k] * inv_norm, prices_tm[(size_t)(t - period + 1 + k) * num_series + s], acc);
+        r = acc;
+    }
+    out_tm[i] = r;
+}

@@ -1,0 +1,306 @@
+// libvta_hip host runtime: context, code-object loading, memory helpers.
+//
+// Replaces the reference's per-wrapper CUDA plumbing (cust Context/Module/
+// Stream in src/cuda/moving_averages/alma_wrapper.rs:133-169, the device probe
+// in src/cuda/mod.rs:412-602, and build.rs's PTX staging) with HIP: one device,
+// one non-blocking stream, gfx950 code objects loaded by hipModuleLoad.
+#include <dlfcn.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "runtime_internal.h"
+
+namespace vta {
+
+static thread_local std::string g_err;
+
+vta_status fail(vta_status st, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return st;
+}
+
+void clear_error() { g_err.clear(); }
+
+static std::string library_dir() {
+    Dl_info info;
+    if (dladdr(reinterpret_cast<void *>(&vta_version), &info) && info.dli_fname) {
+        std::string p(info.dli_fname);
+        size_t slash = p.rfind('/');
+        if (slash != std::string::npos) return p.substr(0, slash);
+    }
+    return ".";
+}
+
+static vta_status get_module(vta_ctx *ctx, const char *module, hipModule_t *out) {
+    auto it = ctx->modules.find(module);
+    if (it != ctx->modules.end()) {
+        *out = it->second;
+        return VTA_OK;
+    }
+    std::string path = ctx->kernel_dir + "/vta_" + module + ".hsaco";
+    hipModule_t m = nullptr;
+    hipError_t e = hipModuleLoad(&m, path.c_str());
+    if (e != hipSuccess)
+        return fail(VTA_ERR_MISSING_KERNEL_SYMBOL, "hipModuleLoad(%s) failed: %s", path.c_str(),
+                    hipGetErrorString(e));
+    ctx->modules[module] = m;
+    *out = m;
+    return VTA_OK;
+}
+
+vta_status get_function(vta_ctx *ctx, const char *module, const char *name, hipFunction_t *out) {
+    std::string key = std::string(module) + ":" + name;
+    auto it = ctx->funcs.find(key);
+    if (it != ctx->funcs.end()) {
+        *out = it->second;
+        return VTA_OK;
+    }
+    hipModule_t m = nullptr;
+    VTA_TRY(get_module(ctx, module, &m));
+    hipFunction_t f = nullptr;
+    hipError_t e = hipModuleGetFunction(&f, m, name);
+    if (e != hipSuccess || !f)
+        return fail(VTA_ERR_MISSING_KERNEL_SYMBOL, "missing kernel symbol: %s (module %s)", name,
+                    module);
+    ctx->funcs[key] = f;
+    *out = f;
+    return VTA_OK;
+}
+
+bool has_function(vta_ctx *ctx, const char *module, const char *name) {
+    hipFunction_t f;
+    std::string saved = g_err;
+    bool ok = get_function(ctx, module, name, &f) == VTA_OK;
+    g_err = saved;
+    return ok;
+}
+
+vta_status validate_launch(vta_ctx *ctx, dim3 grid, dim3 block) {
+    const hipDeviceProp_t &p = ctx->props;
+    unsigned threads = block.x * block.y * block.z;
+    if (threads == 0 || grid.x == 0 || grid.y == 0 || grid.z == 0 ||
+        threads > (unsigned)p.maxThreadsPerBlock || block.x > (unsigned)p.maxThreadsDim[0] ||
+        block.y > (unsigned)p.maxThreadsDim[1] || block.z > (unsigned)p.maxThreadsDim[2] ||
+        grid.x > (unsigned)p.maxGridSize[0] || grid.y > (unsigned)p.maxGridSize[1] ||
+        grid.z > (unsigned)p.maxGridSize[2])
+        return fail(VTA_ERR_LAUNCH_CONFIG_TOO_LARGE,
+                    "launch config too large: grid=(%u,%u,%u) block=(%u,%u,%u)", grid.x, grid.y,
+                    grid.z, block.x, block.y, block.z);
+    return VTA_OK;
+}
+
+vta_status launch_raw(vta_ctx *ctx, hipFunction_t f, dim3 grid, dim3 block, unsigned shmem,
+                      void **args) {
+    VTA_TRY(validate_launch(ctx, grid, block));
+    VTA_HIP(hipModuleLaunchKernel(f, grid.x, grid.y, grid.z, block.x, block.y, block.z, shmem,
+                                  ctx->stream, args, nullptr));
+    return VTA_OK;
+}
+
+vta_status check_fits(size_t required) {
+    const char *env = getenv("VTA_MEM_CHECK");
+    if (env && (strcmp(env, "0") == 0 || strcasecmp(env, "false") == 0)) return VTA_OK;
+    size_t free_b = 0, total = 0;
+    if (hipMemGetInfo(&free_b, &total) != hipSuccess) return VTA_OK;
+    const size_t headroom = 64ull << 20;
+    if (required > free_b || free_b - required < headroom)
+        return fail(VTA_ERR_OUT_OF_MEMORY, "out of memory: required=%zu free=%zu headroom=%zu",
+                    required, free_b, headroom);
+    return VTA_OK;
+}
+
+bool mul_ok(size_t a, size_t b, size_t *out) { return !__builtin_mul_overflow(a, b, out); }
+bool add_ok(size_t a, size_t b, size_t *out) { return !__builtin_add_overflow(a, b, out); }
+
+vta_status DevBuf::alloc(size_t bytes) {
+    if (p) {
+        (void)hipFree(p);
+        p = nullptr;
+    }
+    if (bytes == 0) bytes = 4;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        p = nullptr;
+        return fail(VTA_ERR_OUT_OF_MEMORY, "hipMalloc(%zu) failed: %s", bytes,
+                    hipGetErrorString(e));
+    }
+    return VTA_OK;
+}
+
+vta_status upload(vta_ctx *ctx, DevBuf &buf, const void *src, size_t bytes) {
+    VTA_TRY(buf.alloc(bytes));
+    if (bytes) {
+        // The source is pageable host memory that the caller may free right
+        // after return, so the copy completes before we hand control back.
+        VTA_HIP(hipMemcpyAsync(buf.p, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        VTA_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    return VTA_OK;
+}
+
+size_t first_valid_f32(const float *x, size_t n) {
+    size_t i = 0;
+    while (i < n && std::isnan(x[i])) ++i;
+    return i;
+}
+
+uint16_t bf16_rne(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+float bf16_to_f32(uint16_t h) {
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+}  // namespace vta
+
+using namespace vta;
+
+extern "C" {
+
+const char *vta_last_error(void) { return g_err.c_str(); }
+
+const char *vta_version(void) { return "vta-hip 0.1.0 (gfx950)"; }
+
+int vta_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int vta_device_available(void) {
+    const char *skip = getenv("VTA_FORCE_SKIP");
+    if (skip && (strcmp(skip, "1") == 0 || strcasecmp(skip, "true") == 0)) return 0;
+    return vta_device_count() > 0 ? 1 : 0;
+}
+
+vta_status vta_ctx_create(int device_id, vta_ctx **out) {
+    clear_error();
+    if (!out) return fail(VTA_ERR_INVALID_INPUT, "out is null");
+    *out = nullptr;
+    if (!vta_device_available()) return fail(VTA_ERR_NO_DEVICE, "no HIP device available");
+    int n = vta_device_count();
+    if (device_id < 0 || device_id >= n)
+        return fail(VTA_ERR_DEVICE_MISMATCH, "device mismatch: requested=%d count=%d", device_id,
+                    n);
+    VTA_HIP(hipSetDevice(device_id));
+    vta_ctx *ctx = new vta_ctx();
+    ctx->device = device_id;
+    if (hipGetDeviceProperties(&ctx->props, device_id) != hipSuccess) {
+        delete ctx;
+        return fail(VTA_ERR_HIP, "hipGetDeviceProperties failed");
+    }
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return fail(VTA_ERR_HIP, "hipStreamCreateWithFlags failed");
+    }
+    const char *kd = getenv("VTA_KERNEL_DIR");
+    ctx->kernel_dir = (kd && *kd) ? std::string(kd) : library_dir();
+    *out = ctx;
+    return VTA_OK;
+}
+
+vta_status vta_ctx_destroy(vta_ctx *ctx) {
+    if (!ctx) return VTA_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &kv : ctx->modules) (void)hipModuleUnload(kv.second);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return VTA_OK;
+}
+
+vta_status vta_ctx_synchronize(vta_ctx *ctx) {
+    if (!ctx) return fail(VTA_ERR_INVALID_INPUT, "ctx is null");
+    VTA_HIP(hipStreamSynchronize(ctx->stream));
+    return VTA_OK;
+}
+
+void *vta_ctx_stream(vta_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int vta_ctx_device_id(const vta_ctx *ctx) { return ctx ? ctx->device : -1; }
+
+vta_status vta_ctx_preload(vta_ctx *ctx) {
+    static const char *const kSymbols[][2] = {
+        {"alma", "alma_batch_f32"},
+        {"alma", "alma_batch_f32_tm"},
+        {"alma", "alma_ms1p_f32"},
+        {"alma", "alma_batch_mfma_bf16x3"},
+    };
+    for (auto &s : kSymbols) {
+        hipFunction_t f;
+        VTA_TRY(get_function(ctx, s[0], s[1], &f));
+    }
+    return VTA_OK;
+}
+
+vta_status vta_device_array_free(vta_ctx *ctx, vta_device_array_f32 *arr) {
+    if (!arr) return VTA_OK;
+    if (arr->ptr) {
+        if (ctx) (void)hipStreamSynchronize(ctx->stream);
+        VTA_HIP(hipFree(arr->ptr));
+    }
+    arr->ptr = nullptr;
+    arr->rows = arr->cols = 0;
+    return VTA_OK;
+}
+
+vta_status vta_device_array_copy_to_host(vta_ctx *ctx, const vta_device_array_f32 *arr,
+                                         float *host, size_t host_len) {
+    if (!ctx || !arr || !host) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    size_t n;
+    if (!mul_ok(arr->rows, arr->cols, &n)) return fail(VTA_ERR_ARITHMETIC_OVERFLOW, "rows*cols");
+    if (host_len != n)
+        return fail(VTA_ERR_INVALID_INPUT, "host buffer wrong length: got %zu, expected %zu",
+                    host_len, n);
+    VTA_HIP(hipMemcpyAsync(host, arr->ptr, n * sizeof(float), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    VTA_HIP(hipStreamSynchronize(ctx->stream));
+    return VTA_OK;
+}
+
+vta_status vta_device_alloc(vta_ctx *ctx, size_t bytes, void **out) {
+    if (!ctx || !out) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    VTA_TRY(check_fits(bytes));
+    VTA_HIP(hipMalloc(out, bytes ? bytes : 4));
+    return VTA_OK;
+}
+
+vta_status vta_device_free(vta_ctx *ctx, void *ptr) {
+    if (!ptr) return VTA_OK;
+    if (ctx) (void)hipStreamSynchronize(ctx->stream);
+    VTA_HIP(hipFree(ptr));
+    return VTA_OK;
+}
+
+vta_status vta_memcpy_htod(vta_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx) return fail(VTA_ERR_INVALID_INPUT, "ctx is null");
+    VTA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    VTA_HIP(hipStreamSynchronize(ctx->stream));
+    return VTA_OK;
+}
+
+vta_status vta_memcpy_dtoh(vta_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx) return fail(VTA_ERR_INVALID_INPUT, "ctx is null");
+    VTA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    VTA_HIP(hipStreamSynchronize(ctx->stream));
+    return VTA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,100 @@
+// Internal helpers shared by the libvta_hip host runtime translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "vta.h"
+
+struct vta_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipDeviceProp_t props{};
+    std::string kernel_dir;
+    std::unordered_map<std::string, hipModule_t> modules;
+    std::unordered_map<std::string, hipFunction_t> funcs;
+    // per-indicator policies and last selections (CudaAlmaPolicy etc.)
+    vta_alma_batch_kernel alma_policy = VTA_ALMA_BATCH_AUTO;
+    vta_alma_batch_kernel alma_selected = VTA_ALMA_BATCH_AUTO;
+};
+
+namespace vta {
+
+vta_status fail(vta_status st, const char *fmt, ...);
+void clear_error();
+
+#define VTA_HIP(expr)                                                                    \
+    do {                                                                                 \
+        hipError_t vta_e_ = (expr);                                                      \
+        if (vta_e_ != hipSuccess)                                                        \
+            return ::vta::fail(VTA_ERR_HIP, "%s failed: %s", #expr,                      \
+                               hipGetErrorString(vta_e_));                               \
+    } while (0)
+
+#define VTA_TRY(expr)                                                                    \
+    do {                                                                                 \
+        vta_status vta_s_ = (expr);                                                      \
+        if (vta_s_ != VTA_OK) return vta_s_;                                             \
+    } while (0)
+
+// Module / symbol lookup: <kernel_dir>/vta_<module>.hsaco, hipModuleGetFunction.
+vta_status get_function(vta_ctx *ctx, const char *module, const char *name, hipFunction_t *out);
+bool has_function(vta_ctx *ctx, const char *module, const char *name);
+
+// Launch-limit check against the device (validate_launch, alma_wrapper.rs:365-417).
+vta_status validate_launch(vta_ctx *ctx, dim3 grid, dim3 block);
+
+// Raw launch: args is the kernelParams array.
+vta_status launch_raw(vta_ctx *ctx, hipFunction_t f, dim3 grid, dim3 block, unsigned shmem,
+                      void **args);
+
+template <typename... A>
+vta_status launch(vta_ctx *ctx, hipFunction_t f, dim3 grid, dim3 block, unsigned shmem,
+                  A... a) {
+    void *args[] = {static_cast<void *>(&a)...};
+    return launch_raw(ctx, f, grid, block, shmem, args);
+}
+
+// will_fit with 64 MiB headroom (alma_wrapper.rs:287-310, 645-664);
+// VTA_MEM_CHECK=0 disables it like CUDA_MEM_CHECK=0.
+vta_status check_fits(size_t required_bytes);
+
+// checked size arithmetic
+bool mul_ok(size_t a, size_t b, size_t *out);
+bool add_ok(size_t a, size_t b, size_t *out);
+
+// RAII device buffer used for temporaries.
+struct DevBuf {
+    void *p = nullptr;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    vta_status alloc(size_t bytes);
+    void *release() {
+        void *q = p;
+        p = nullptr;
+        return q;
+    }
+    template <typename T>
+    T *as() const {
+        return static_cast<T *>(p);
+    }
+};
+
+vta_status upload(vta_ctx *ctx, DevBuf &buf, const void *src, size_t bytes);
+
+size_t first_valid_f32(const float *x, size_t n);  // n if all NaN
+
+// bf16 round-to-nearest-even of an f32 (bit pattern).
+uint16_t bf16_rne(float f);
+float bf16_to_f32(uint16_t h);
+
+}  // namespace vta

@@ -1,0 +1,90 @@
+"""Seeded synthetic market data for the benchmark workloads and parity tests.
+
+All series are f32 (what the GPU consumes); the oracle sees the same values
+widened to f64.  Generators use numpy's PCG64 with explicit seeds so the CPU
+and GPU legs of a run see identical bytes.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def gbm_close(n: int, sd: float, seed: int, start: float = 100.0) -> np.ndarray:
+    """Geometric random walk: close[0] = start, log-returns i.i.d. N(0, sd^2)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    r = rng.normal(0.0, sd, size=n)
+    r[0] = 0.0
+    return (start * np.exp(np.cumsum(r))).astype(np.float32)
+
+
+def alma_single_series(seed: int = 1) -> np.ndarray:
+    """Plain ALMA workload: 100,000 closes, log-return sd 0.01."""
+    return gbm_close(100_000, 0.01, seed)
+
+
+def alma_sweep_series(n: int = 200_000, seed: int = 2) -> np.ndarray:
+    """ALMA parameter-sweep workload: 200,000 bars, log-return sd 0.015."""
+    return gbm_close(n, 0.015, seed)
+
+
+# Grid of the 1,250-combo sweep: periods 10..=250 step 10, offsets 0.05..=0.95
+# step 0.1, sigmas 2..=10 step 2.
+ALMA_SWEEP_RANGE = dict(period=(10, 250, 10), offset=(0.05, 0.95, 0.1), sigma=(2.0, 10.0, 2.0))
+
+
+def ohlc_from_close(close: np.ndarray, rng: np.random.Generator, sd: float = 0.005):
+    """high/low = close * (1 +/- |N(0, sd)|), open = previous close."""
+    c = close.astype(np.float64)
+    up = np.abs(rng.normal(0.0, sd, size=c.shape))
+    dn = np.abs(rng.normal(0.0, sd, size=c.shape))
+    high = c * (1.0 + up)
+    low = c * (1.0 - dn)
+    open_ = np.empty_like(c)
+    open_[0] = c[0]
+    open_[1:] = c[:-1]
+    return (open_.astype(np.float32), high.astype(np.float32), low.astype(np.float32),
+            c.astype(np.float32))
+
+
+def many_series_ohlc(n_series: int = 4096, rows: int = 50_000, min_len: int = 2_000,
+                     seed: int = 3, sd: float = 0.01):
+    """Time-major [rows, n_series] OHLC with uneven lengths.
+
+    Lengths are log-uniform in [min_len, rows]; each ticker is right-aligned
+    (its last bar is row rows-1) and padded with leading NaN.
+    Returns (open, high, low, close, lengths).
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lengths = np.exp(rng.uniform(np.log(min_len), np.log(rows), size=n_series)).astype(np.int64)
+    lengths = np.clip(lengths, min_len, rows)
+    o = np.full((rows, n_series), np.nan, dtype=np.float32)
+    h = np.full_like(o, np.nan)
+    lo = np.full_like(o, np.nan)
+    c = np.full_like(o, np.nan)
+    r = rng.normal(0.0, sd, size=(rows, n_series))
+    logp = np.cumsum(r, axis=0)
+    for s in range(n_series):
+        L = int(lengths[s])
+        start = rows - L
+        lp = logp[start:, s] - logp[start, s]
+        close = (100.0 * np.exp(lp)).astype(np.float32)
+        oo, hh, ll, cc = ohlc_from_close(close, rng)
+        o[start:, s] = oo
+        h[start:, s] = hh
+        lo[start:, s] = ll
+        c[start:, s] = cc
+    return o, h, lo, c, lengths
+
+
+def many_series_with_jumps(n_series: int = 1024, rows: int = 50_000, seed: int = 4,
+                           sd: float = 0.01, jump_frac: float = 0.01) -> np.ndarray:
+    """Time-major [rows, n_series] random walks with 1% of bars replaced by
+    +/-5-sigma jumps (in price terms) to stress rank statistics."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    r = rng.normal(0.0, sd, size=(rows, n_series))
+    r[0] = 0.0
+    p = 100.0 * np.exp(np.cumsum(r, axis=0))
+    mask = rng.random(size=(rows, n_series)) < jump_frac
+    sign = np.where(rng.random(size=(rows, n_series)) < 0.5, -1.0, 1.0)
+    p = np.where(mask, p * (1.0 + sign * 5.0 * sd), p)
+    return p.astype(np.float32)

@@ -1,0 +1,145 @@
+"""Context and device-array handles over the C ABI.
+
+``HipContext`` plays the role of the reference wrappers' cust Context + Stream
+(src/cuda/moving_averages/alma_wrapper.rs:119-169); ``DeviceArrayF32`` is the
+device-resident result ``{buf, rows, cols}`` (alma_wrapper.rs:103-117).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from ._lib import DeviceArrayF32C, VtaError, check, lib
+
+
+def cuda_available() -> bool:
+    """Device probe (src/cuda/mod.rs:412-582); VTA_FORCE_SKIP=1 forces False."""
+    try:
+        return bool(lib().vta_device_available())
+    except (ImportError, OSError):
+        return False
+
+
+def cuda_device_count() -> int:
+    return int(lib().vta_device_count())
+
+
+class HipContext:
+    """One device, one non-blocking stream, lazily loaded gfx950 code objects."""
+
+    def __init__(self, device_id: int = 0):
+        self._ptr = ctypes.c_void_p()
+        check(lib().vta_ctx_create(int(device_id), ctypes.byref(self._ptr)))
+        self.device_id = int(device_id)
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        if not self._ptr:
+            raise VtaError(4, "context is closed")
+        return self._ptr
+
+    @property
+    def stream(self) -> int:
+        return int(lib().vta_ctx_stream(self.handle) or 0)
+
+    def synchronize(self) -> None:
+        check(lib().vta_ctx_synchronize(self.handle))
+
+    def preload(self) -> None:
+        check(lib().vta_ctx_preload(self.handle))
+
+    def close(self) -> None:
+        if self._ptr:
+            lib().vta_ctx_destroy(self._ptr)
+            self._ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # raw device memory for the enqueue-only entry points
+    def alloc(self, nbytes: int) -> "DeviceBuffer":
+        p = ctypes.c_void_p()
+        check(lib().vta_device_alloc(self.handle, int(nbytes), ctypes.byref(p)))
+        return DeviceBuffer(self, p.value, int(nbytes))
+
+    def to_device(self, host: np.ndarray) -> "DeviceBuffer":
+        host = np.ascontiguousarray(host)
+        buf = self.alloc(host.nbytes)
+        check(lib().vta_memcpy_htod(self.handle, ctypes.c_void_p(buf.ptr),
+                                    host.ctypes.data_as(ctypes.c_void_p), host.nbytes))
+        return buf
+
+
+class DeviceBuffer:
+    """Caller-owned device allocation (cust DeviceBuffer)."""
+
+    def __init__(self, ctx: HipContext, ptr: int, nbytes: int):
+        self.ctx = ctx
+        self.ptr = ptr
+        self.nbytes = nbytes
+
+    def copy_to_host(self, dtype=np.float32) -> np.ndarray:
+        out = np.empty(self.nbytes // np.dtype(dtype).itemsize, dtype=dtype)
+        check(lib().vta_memcpy_dtoh(self.ctx.handle, out.ctypes.data_as(ctypes.c_void_p),
+                                    ctypes.c_void_p(self.ptr), self.nbytes))
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().vta_device_free(self.ctx.handle, ctypes.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            if self.ctx._ptr:
+                self.free()
+        except Exception:
+            pass
+
+
+class DeviceArrayF32:
+    """Device-resident f32 matrix owned by the library; freed on close/GC."""
+
+    def __init__(self, ctx: HipContext, c: DeviceArrayF32C):
+        self.ctx = ctx
+        self._c = c
+
+    @property
+    def rows(self) -> int:
+        return int(self._c.rows)
+
+    @property
+    def cols(self) -> int:
+        return int(self._c.cols)
+
+    def __len__(self) -> int:
+        return self.rows * self.cols
+
+    def device_ptr(self) -> int:
+        return int(self._c.ptr or 0)
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty((self.rows, self.cols), dtype=np.float32)
+        check(lib().vta_device_array_copy_to_host(
+            self.ctx.handle, ctypes.byref(self._c), out.ctypes.data_as(ctypes.c_void_p),
+            out.size))
+        return out
+
+    def free(self) -> None:
+        if self._c.ptr:
+            lib().vta_device_array_free(self.ctx.handle, ctypes.byref(self._c))
+
+    def __del__(self):
+        try:
+            if self.ctx._ptr:
+                self.free()
+        except Exception:
+            pass
+
+
+def f32_ptr(a: np.ndarray) -> ctypes.c_void_p:
+    return a.ctypes.data_as(ctypes.c_void_p)
