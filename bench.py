@@ -115,6 +115,28 @@ def max_rel_err(g, ref):
     return err, nan_ok
 
 
+def calibrate_copy(nbytes: int = 256 << 20, reps: int = 3) -> None:
+    """Device-to-device copies of a known size (read nbytes + write nbytes each)."""
+    ctx = vta.HipContext(0)
+    src, dst = ctx.alloc(nbytes), ctx.alloc(nbytes)
+    h = ctypes.CDLL("libamdhip64.so")
+    h.hipMemcpyDtoDAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                     ctypes.c_void_p]
+    ev = HipEvents()
+    for _ in range(reps):
+        a, b = ev.create(), ev.create()
+        ev.record(a, ctx.stream)
+        assert h.hipMemcpyDtoDAsync(ctypes.c_void_p(dst.ptr), ctypes.c_void_p(src.ptr), nbytes,
+                                    ctypes.c_void_p(ctx.stream)) == 0
+        ev.record(b, ctx.stream)
+        ms = ev.elapsed_ms(a, b)
+        print(f"[calib] copy {nbytes >> 20} MiB: {ms:.4f} ms, {2 * nbytes / ms / 1e6:.1f} GB/s "
+              "(read + write)", flush=True)
+    ctx.synchronize()
+    src.free()
+    dst.free()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,11 +145,18 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and error check")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
     ap.add_argument("--forms", default="mfma,fp32")
+    ap.add_argument("--mode", default="sweep", choices=["sweep", "calib"],
+                    help="calib: 256 MiB device-to-device copies, the known-size reference "
+                         "for scaling the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU sharding is design-only", file=sys.stderr)
         sys.exit(2)
+
+    if args.mode == "calib":
+        calibrate_copy()
+        return
 
     x = datagen.alma_sweep_series()
     sweep = vta.AlmaBatchRange(**datagen.ALMA_SWEEP_RANGE)

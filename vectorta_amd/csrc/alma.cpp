@@ -2,6 +2,7 @@
 // alma_wrapper.rs).  Grid expansion, input validation, per-combo weights and
 // kernel selection run here; the arithmetic runs in kernels/alma.hip.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -331,8 +332,18 @@ vta_status plan_launch(vta_ctx *ctx, const vta_alma_sweep_plan *plan, const floa
                        size_t fv, float *d_out) {
     const int T = (int)plan->series_len;
     if (plan->kernel == VTA_ALMA_BATCH_MFMA_BF16X3) {
+        // tiles per wave: 16 by default; ALMA_MF_TILES=8|32 overrides (tuning knob,
+        // like the reference's ALMA_TILE, alma_wrapper.rs:314)
+        int tiles = kMfTilesPerWave;
+        if (const char *e = getenv("ALMA_MF_TILES")) {
+            int v = atoi(e);
+            if (v == 8 || v == 16 || v == 32) tiles = v;
+        }
+        const char *name = tiles == 8    ? "alma_batch_mfma_bf16x3_r8"
+                           : tiles == 32 ? "alma_batch_mfma_bf16x3_r32"
+                                         : "alma_batch_mfma_bf16x3";
         hipFunction_t f;
-        VTA_TRY(get_function(ctx, "alma", "alma_batch_mfma_bf16x3", &f));
+        VTA_TRY(get_function(ctx, "alma", name, &f));
         alma_mf_args_host a;
         a.prices = d_prices;
         a.frags = plan->frags.p;
@@ -344,10 +355,10 @@ vta_status plan_launch(vta_ctx *ctx, const vta_alma_sweep_plan *plan, const floa
         a.series_len = T;
         a.first_valid = (int)fv;
         a.n_nblocks = plan->n_nblocks;
-        a.tiles_per_wave = kMfTilesPerWave;
+        a.tiles_per_wave = tiles;
         a.n_tiles = (T + 31) / 32;
         a.kmax = plan->kmax;
-        dim3 grid((a.n_tiles + kMfTilesPerWave - 1) / kMfTilesPerWave, plan->n_nblocks, 1);
+        dim3 grid((a.n_tiles + tiles - 1) / tiles, plan->n_nblocks, 1);
         ctx->alma_selected = VTA_ALMA_BATCH_MFMA_BF16X3;
         return launch(ctx, f, grid, dim3(64), 0, a);
     }

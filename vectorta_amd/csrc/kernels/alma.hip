@@ -260,15 +260,19 @@ extern "C" __global__ __launch_bounds__(256) void alma_ms1p_f32(
 // Problem: out[row(n)][t] = sum_{k<K} W[k][n] * x[t - K + 1 + k] for the 32
 // combos n of an N-block, where K (a multiple of 16) covers the block's longest
 // period and shorter periods are zero padded at the front.  Each wave owns one
-// N-block and a run of 32-bar M-tiles.  Per tile it
-//   1. stages x' = x - a (a = one anchor price per tile, x' = 0 outside
-//      [first_valid, T)) as BF16 hi and lo = bf16(x' - hi) into eight copies
-//      shifted by 0..7 elements, so every A-fragment read is an aligned
-//      16-byte LDS load whatever the lane's row;
-//   2. runs K/16 steps of v_mfma_f32_32x32x16_bf16: hi*hi into one
-//      accumulator, hi*lo + lo*hi into a second;
+// N-block and a run of MF_TILES_PER_WAVE 32-bar M-tiles.  It loads the prices
+// of its whole run into LDS once, then per tile
+//   1. stages x' = x - a (a = one anchor price per tile) as BF16 hi and
+//      lo = bf16(x' - hi) in two copies: copy 0 holds x'[0..], copy 1 holds
+//      x'[1..], so every lane's 8-element A-fragment starts on a dword
+//      boundary in one of them whatever the lane's row parity;
+//   2. runs K/16 steps of v_mfma_f32_32x32x16_bf16 with the Hankel tile as
+//      the A operand and the weights as B, so D is [bar][combo]: hi*hi into
+//      one accumulator, hi*lo + lo*hi into a second;
 //   3. writes out = (acc_hi + acc_corr) + a (weights sum to 1), NaN before the
-//      warm-up of each combo.
+//      warm-up of each combo, as 16-byte stores of 4 consecutive bars.
+// (The transposed [combo][bar] D layout makes every store a full 128-byte row
+// segment but needs four times the store instructions; it measured ~8% slower.)
 // Split error: |x' - hi - lo| <= 2^-18 |x'|, same for the weights, dropped
 // lo*lo <= 2^-18 |w x'|; anchoring makes |x'| a few percent of the price, so
 // the contraction carries FP32-level relative accuracy.  Tiles that contain a
@@ -276,13 +280,19 @@ extern "C" __global__ __launch_bounds__(256) void alma_ms1p_f32(
 // dot from the f32 weight table so NaN/Inf behave like the FP32 kernel.
 //
 // Operand layouts (v_mfma_f32_32x32x16_bf16, wave64):
-//   A (32x16): lane l holds A[l%32][8*(l/32) + j], j = 0..7
-//   B (16x32): lane l holds B[8*(l/32) + j][l%32]
+//   A (32x16): lane l holds A[l%32][8*(l/32) + j], j = 0..7   (Hankel x')
+//   B (16x32): lane l holds B[8*(l/32) + j][l%32]             (weights W)
 //   D (32x32): lane l, element i holds D[8*(i/4) + 4*(l/32) + i%4][l%32]
 typedef __attribute__((ext_vector_type(8))) __bf16 vta_bf16x8;
 typedef __attribute__((ext_vector_type(16))) float vta_f32x16;
 
-#define MF_COPY_STRIDE_MAX 656   // bytes per shifted copy for K <= 256 (mf_copy_stride(16))
+// R = 32-bar M-tiles per wave; one kernel per R (8, 16, 32), picked at launch.
+#define MF_XS_MAX(R) ((((R) * 32 + 256 + 63) / 64) * 64)
+// One staged array (a copy of hi or lo) holds K/2 + 15 dwords; 160 dwords
+// covers K <= 256.  The four arrays sit 168 dwords apart so the two copies a
+// wave reads in one instruction fall on different LDS banks.
+#define MF_ARR_STRIDE 168
+#define MF_ARR_WORDS (4 * MF_ARR_STRIDE)
 
 struct alma_mf_args {
     const float *prices;
@@ -300,17 +310,13 @@ struct alma_mf_args {
     int kmax;
 };
 
-__device__ __forceinline__ uint32_t mf_copy_stride(int ks) {
-    // copy length in elements: K + 32 (M rows) rounded to 8; bytes = 2*len,
-    // rounded to 128 and offset by 16 so the eight copies fall on distinct banks.
-    uint32_t len = (uint32_t)(ks * 16 + 32);
-    uint32_t bytes = ((2u * len + 127u) & ~127u) + 16u;
-    return bytes;
+__device__ __forceinline__ uint32_t mf_pack(uint16_t lo_elem, uint16_t hi_elem) {
+    return (uint32_t)lo_elem | ((uint32_t)hi_elem << 16);
 }
 
-template <int KS>
+template <int KS, int R>
 __device__ __forceinline__ void alma_mf_wave(const alma_mf_args &a, int nb, int tile_begin,
-                                             unsigned char *lds) {
+                                             uint32_t *arr, float *xs) {
     const int lane = threadIdx.x & 63;
     const int g = lane >> 5;    // K-half for A/B, row-quad selector for D
     const int m = lane & 31;    // A row (bar in tile) / B,D column (combo)
@@ -318,9 +324,11 @@ __device__ __forceinline__ void alma_mf_wave(const alma_mf_args &a, int nb, int 
     const int frag_off = nbd.y;
     const int combo_base = nbd.z;
     constexpr int K = KS * 16;
-    const uint32_t cstride = mf_copy_stride(KS);
-    unsigned char *lds_hi = lds;
-    unsigned char *lds_lo = lds + 8 * cstride;
+    constexpr int NPAIR = K / 2 + 15;          // dwords staged per array
+    uint32_t *hi0 = arr;                       // copy 0 (x'[2i], x'[2i+1])
+    uint32_t *hi1 = arr + MF_ARR_STRIDE;       // copy 1 (x'[2i+1], x'[2i+2])
+    uint32_t *lo0 = arr + 2 * MF_ARR_STRIDE;
+    uint32_t *lo1 = arr + 3 * MF_ARR_STRIDE;
 
     // B fragments stay in registers for all tiles of this wave.
     uint4 bhi[KS], blo[KS];
@@ -329,59 +337,83 @@ __device__ __forceinline__ void alma_mf_wave(const alma_mf_args &a, int nb, int 
         bhi[s] = a.frags[frag_off + (2 * s + 0) * 64 + lane];
         blo[s] = a.frags[frag_off + (2 * s + 1) * 64 + lane];
     }
-    const int my_period = a.sorted_period[combo_base + m];
-    const int my_row = a.sorted_row[combo_base + m];
-    const int my_warm = a.first_valid + my_period - 1;
     const int T = a.series_len;
     const int fv = a.first_valid;
+    const int my_period = a.sorted_period[combo_base + m];
+    const int my_row = a.sorted_row[combo_base + m];
+    const int my_warm = fv + my_period - 1;
 
-    const int tile_end = min(tile_begin + a.tiles_per_wave, a.n_tiles);
+    const int tile_end = min(tile_begin + R, a.n_tiles);
+    // ---- prologue: the wave's whole price range x[t_lo, t_lo + NX) into LDS
+    // (0 outside [first_valid, T), non-finite values kept), one batch of loads.
+    const int t_lo = tile_begin * 32 - K + 1;
+    constexpr int NXI = (R * 32 + K + 63) / 64;
+    {
+        float v[NXI];
+#pragma unroll
+        for (int i = 0; i < NXI; ++i) {
+            const int idx = t_lo + lane + 64 * i;
+            v[i] = (idx >= fv && idx < T) ? a.prices[idx] : 0.0f;
+        }
+#pragma unroll
+        for (int i = 0; i < NXI; ++i) xs[lane + 64 * i] = v[i];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+    // A-fragment dword offset of this lane inside a copy, K-step 0
+    const int r = m & 1;
+    const int d0 = (m >> 1) + 4 * g;
+    const uint32_t *ahi_base = (r ? hi1 : hi0) + d0;
+    const uint32_t *alo_base = (r ? lo1 : lo0) + d0;
+
     for (int tile = tile_begin; tile < tile_end; ++tile) {
         const int t_tile = tile * 32;
-        const int base_idx = t_tile - K + 1;   // x index of X'[0]
-        // anchor: the price at the tile's last in-range bar at or after first_valid
-        int ai = min(t_tile + 31, T - 1);
-        if (ai < fv) ai = fv;
-        float anchor = a.prices[ai];
+        const int xoff = t_tile - K + 1 - t_lo;   // xs index of x'[0]
+        const int ai = min(t_tile + 31, T - 1);   // anchor: the tile's last in-range bar
+        float anchor = xs[ai - t_lo];
         if (!vta_isfinite(anchor)) anchor = 0.0f;
 
-        // ---- stage X'[e], e in [0, K + 32), into eight shifted copies
+        // ---- stage pairs i: copy0[i] = (x'[2i], x'[2i+1]), copy1[i] = (x'[2i+1], x'[2i+2])
         bool bad = false;
-        for (int e = lane; e < K + 32; e += 64) {
-            int idx = base_idx + e;
-            float v = 0.0f;
-            if (idx >= fv && idx < T) {
-                float x = a.prices[idx];
-                if (!vta_isfinite(x)) bad = true;
-                v = x - anchor;
-            }
-            uint16_t hi = vta_bf16_rne(v);
-            uint16_t lo = vta_bf16_rne(v - vta_bf16_to_f32(hi));
-            if (!vta_isfinite(v)) { hi = 0; lo = 0; }
 #pragma unroll
-            for (int r = 0; r < 8; ++r) {
-                int pos = e - r;
-                if (pos >= 0) {
-                    *reinterpret_cast<uint16_t *>(lds_hi + r * cstride + 2 * pos) = hi;
-                    *reinterpret_cast<uint16_t *>(lds_lo + r * cstride + 2 * pos) = lo;
-                }
-            }
+        for (int it = 0; it < (NPAIR + 63) / 64; ++it) {
+            const int i = lane + 64 * it;
+            if (i >= NPAIR) break;
+            const float x0 = xs[xoff + 2 * i];
+            const float x1 = xs[xoff + 2 * i + 1];
+            const float x2 = xs[xoff + 2 * i + 2];
+            float v0 = x0 - anchor, v1 = x1 - anchor, v2 = x2 - anchor;
+            if (!vta_isfinite(x0)) { bad = true; v0 = 0.0f; }
+            if (!vta_isfinite(x1)) { bad = true; v1 = 0.0f; }
+            if (!vta_isfinite(x2)) { bad = true; v2 = 0.0f; }
+            const uint16_t h0 = vta_bf16_rne(v0), h1 = vta_bf16_rne(v1), h2 = vta_bf16_rne(v2);
+            const uint16_t l0 = vta_bf16_rne(v0 - vta_bf16_to_f32(h0));
+            const uint16_t l1 = vta_bf16_rne(v1 - vta_bf16_to_f32(h1));
+            const uint16_t l2 = vta_bf16_rne(v2 - vta_bf16_to_f32(h2));
+            hi0[i] = mf_pack(h0, h1);
+            hi1[i] = mf_pack(h1, h2);
+            lo0[i] = mf_pack(l0, l1);
+            lo1[i] = mf_pack(l1, l2);
         }
         const bool tile_bad = __any(bad);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
-        // ---- contraction
+        // ---- contraction: lane chunk at K-step s starts at x'[m + 16s + 8g],
+        // i.e. dword d0 + 8s of copy (m & 1)
         vta_f32x16 acc_h = {};
         vta_f32x16 acc_c = {};
-        const int r = m & 7;
+        __builtin_amdgcn_s_setprio(1);   // favour the wave that is feeding the matrix cores
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
-            const int e0 = m + 16 * s + 8 * g;          // first element of this lane's chunk
-            const uint32_t off = (uint32_t)r * cstride + 2u * (uint32_t)(e0 - r);
-            const uint4 ahi_u = *reinterpret_cast<const uint4 *>(lds_hi + off);
-            const uint4 alo_u = *reinterpret_cast<const uint4 *>(lds_lo + off);
+            uint4 ahi_u, alo_u;
+            ahi_u.x = ahi_base[8 * s + 0]; ahi_u.y = ahi_base[8 * s + 1];
+            ahi_u.z = ahi_base[8 * s + 2]; ahi_u.w = ahi_base[8 * s + 3];
+            alo_u.x = alo_base[8 * s + 0]; alo_u.y = alo_base[8 * s + 1];
+            alo_u.z = alo_base[8 * s + 2]; alo_u.w = alo_base[8 * s + 3];
             const vta_bf16x8 ahi = __builtin_bit_cast(vta_bf16x8, ahi_u);
             const vta_bf16x8 alo = __builtin_bit_cast(vta_bf16x8, alo_u);
             const vta_bf16x8 bh = __builtin_bit_cast(vta_bf16x8, bhi[s]);
@@ -390,6 +422,7 @@ __device__ __forceinline__ void alma_mf_wave(const alma_mf_args &a, int nb, int 
             acc_c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bl, acc_c, 0, 0, 0);
             acc_c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bh, acc_c, 0, 0, 0);
         }
+        __builtin_amdgcn_s_setprio(0);
         // LDS is rewritten by the next tile: make sure every lane's reads are done.
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -429,19 +462,31 @@ __device__ __forceinline__ void alma_mf_wave(const alma_mf_args &a, int nb, int 
 }
 
 // grid.x = number of tile runs, grid.y = number of N-blocks; block = 64 lanes.
-extern "C" __global__ __launch_bounds__(64) void alma_batch_mfma_bf16x3(alma_mf_args a) {
-    __shared__ __attribute__((aligned(16))) unsigned char lds[16 * MF_COPY_STRIDE_MAX];
+template <int R>
+__device__ __forceinline__ void alma_mf_entry(const alma_mf_args &a) {
+    __shared__ __attribute__((aligned(16))) uint32_t arr[MF_ARR_WORDS];
+    __shared__ __attribute__((aligned(16))) float xs[MF_XS_MAX(R)];
     const int nb = blockIdx.y;
-    const int tile_begin = blockIdx.x * a.tiles_per_wave;
+    const int tile_begin = blockIdx.x * R;
     const int ks = a.nblocks[nb].x;
     switch (ks) {
-#define MF_CASE(N) case N: alma_mf_wave<N>(a, nb, tile_begin, lds); break;
+#define MF_CASE(N) case N: alma_mf_wave<N, R>(a, nb, tile_begin, arr, xs); break;
         MF_CASE(1) MF_CASE(2) MF_CASE(3) MF_CASE(4) MF_CASE(5) MF_CASE(6) MF_CASE(7) MF_CASE(8)
         MF_CASE(9) MF_CASE(10) MF_CASE(11) MF_CASE(12) MF_CASE(13) MF_CASE(14) MF_CASE(15)
         MF_CASE(16)
 #undef MF_CASE
         default: break;
     }
+}
+
+extern "C" __global__ __launch_bounds__(64) void alma_batch_mfma_bf16x3(alma_mf_args a) {
+    alma_mf_entry<16>(a);
+}
+extern "C" __global__ __launch_bounds__(64) void alma_batch_mfma_bf16x3_r8(alma_mf_args a) {
+    alma_mf_entry<8>(a);
+}
+extern "C" __global__ __launch_bounds__(64) void alma_batch_mfma_bf16x3_r32(alma_mf_args a) {
+    alma_mf_entry<32>(a);
 }
 
 // ---------------------------------------------------------------- direct fallbacks
