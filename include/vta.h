@@ -173,6 +173,116 @@ vta_status vta_alma_many_series_one_param_time_major_device(
     vta_ctx *ctx, const float *d_prices_tm, const float *d_weights, size_t period,
     float inv_norm, size_t cols, size_t rows, const int32_t *d_first_valids, float *d_out_tm);
 
+
+/* ====================================================== recurrences */
+/* One-parameter batch ranges are vta_range_usize over the period, expanded
+ * like the reference wrappers: EMA/RSI ascending (ema_wrapper.rs:862-873,
+ * rsi_wrapper.rs:389-413), ATR (atr_wrapper.rs:286-297), ADX
+ * (adx_wrapper.rs:158-178).  Engines run in f64 with the reference's
+ * operation order and round to f32 on output. */
+
+/* EMA: ema_wrapper.rs:193 (batch_dev), :256 (batch_device), :325
+ * (many_series_one_param_time_major_dev), :381 (many_series_one_param_device).
+ * d_alphas is accepted for signature parity and ignored (alpha = 2/(p+1) in f64).
+ * d_clean (per series, 1 = every bar from first_valid finite) comes from
+ * vta_many_series_prepare; NULL runs every series exactly in one pass. */
+vta_status vta_ema_expand_grid(const vta_range_usize *range, size_t *periods, size_t cap,
+                               size_t *n_out);
+vta_status vta_ema_batch_dev(vta_ctx *ctx, const float *prices, size_t len,
+                             const vta_range_usize *range, vta_device_array_f32 *out);
+vta_status vta_ema_batch_device(vta_ctx *ctx, const float *d_prices, const int32_t *d_periods,
+                                const float *d_alphas, int32_t max_period, int32_t series_len,
+                                int32_t first_valid, int32_t n_combos, float *d_out);
+vta_status vta_ema_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *data_tm,
+                                                        size_t cols, size_t rows, size_t period,
+                                                        vta_device_array_f32 *out);
+vta_status vta_ema_many_series_one_param_device(vta_ctx *ctx, const float *d_prices_tm,
+                                                const int32_t *d_first_valids, size_t period,
+                                                size_t num_series, size_t series_len,
+                                                const int32_t *d_clean, float *d_out_tm);
+
+/* RSI (Wilder): rsi_wrapper.rs:152 (rsi_batch_dev), :271
+ * (rsi_many_series_one_param_time_major_dev). */
+vta_status vta_rsi_expand_grid(const vta_range_usize *range, size_t *periods, size_t cap,
+                               size_t *n_out);
+vta_status vta_rsi_batch_dev(vta_ctx *ctx, const float *prices, size_t len,
+                             const vta_range_usize *range, vta_device_array_f32 *out);
+vta_status vta_rsi_batch_device(vta_ctx *ctx, const float *d_prices, const int32_t *d_periods,
+                                int32_t max_period, int32_t series_len, int32_t first_valid,
+                                int32_t n_combos, float *d_out);
+vta_status vta_rsi_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *prices_tm,
+                                                        size_t cols, size_t rows, size_t period,
+                                                        vta_device_array_f32 *out);
+vta_status vta_rsi_many_series_one_param_device(vta_ctx *ctx, const float *d_prices_tm,
+                                                const int32_t *d_first_valids, size_t period,
+                                                size_t num_series, size_t series_len,
+                                                const int32_t *d_clean, float *d_out_tm);
+
+/* ATR: atr_wrapper.rs:270 (atr_batch_dev), :626 (atr_batch_device_with_tr),
+ * :834 (atr_many_series_one_param_time_major_dev). */
+vta_status vta_atr_expand_grid(const vta_range_usize *range, size_t *periods, size_t cap,
+                               size_t *n_out);
+vta_status vta_atr_batch_dev(vta_ctx *ctx, const float *high, const float *low,
+                             const float *close, size_t len, const vta_range_usize *range,
+                             vta_device_array_f32 *out);
+vta_status vta_atr_batch_device(vta_ctx *ctx, const float *d_high, const float *d_low,
+                                const float *d_close, const int32_t *d_periods,
+                                int32_t max_period, int32_t series_len, int32_t first_valid,
+                                int32_t n_combos, float *d_out);
+vta_status vta_atr_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *high_tm,
+                                                        const float *low_tm,
+                                                        const float *close_tm, size_t cols,
+                                                        size_t rows, size_t period,
+                                                        vta_device_array_f32 *out);
+vta_status vta_atr_many_series_one_param_device(vta_ctx *ctx, const float *d_high_tm,
+                                                const float *d_low_tm, const float *d_close_tm,
+                                                const int32_t *d_first_valids, size_t period,
+                                                size_t num_series, size_t series_len,
+                                                const int32_t *d_clean, float *d_out_tm);
+
+/* ADX: adx_wrapper.rs:199 (adx_batch_dev), :340
+ * (adx_many_series_one_param_time_major_dev). */
+vta_status vta_adx_expand_grid(const vta_range_usize *range, size_t *periods, size_t cap,
+                               size_t *n_out);
+vta_status vta_adx_batch_dev(vta_ctx *ctx, const float *high, const float *low,
+                             const float *close, size_t len, const vta_range_usize *range,
+                             vta_device_array_f32 *out);
+vta_status vta_adx_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *high_tm,
+                                                        const float *low_tm,
+                                                        const float *close_tm, size_t cols,
+                                                        size_t rows, size_t period,
+                                                        vta_device_array_f32 *out);
+
+/* Per-series metadata of device-resident time-major inputs, enqueue only:
+ * first_valid (mode 0: x; 1: first bar with high, low, x all non-NaN; 2: ADX
+ * triple rule; 3: x, with the mode-1 bar in d_first_valids_hlc) and the clean
+ * flag.  The reference computes first_valids on the host
+ * (ema_wrapper.rs:637-700); this is its device-side counterpart. */
+vta_status vta_many_series_prepare(vta_ctx *ctx, const float *d_x_tm, const float *d_high_tm,
+                                   const float *d_low_tm, size_t cols, size_t rows, int mode,
+                                   int32_t *d_first_valids, int32_t *d_first_valids_hlc,
+                                   int32_t *d_clean);
+
+/* Fused EMA(p1) + RSI(p2) of close and ATR(p3) of (high, low, close) over
+ * time-major [rows][cols] series in one launch (no reference counterpart:
+ * the reference runs three wrappers, three uploads and three launches,
+ * ema_wrapper.rs:325, rsi_wrapper.rs:271, atr_wrapper.rs:834).  _device is
+ * enqueue only with metadata from vta_many_series_prepare(mode 3); _dev takes
+ * host data and returns three device arrays {ema, rsi, atr}. */
+vta_status vta_ema_rsi_atr_many_series_device(vta_ctx *ctx, const float *d_close_tm,
+                                              const float *d_high_tm, const float *d_low_tm,
+                                              size_t cols, size_t rows, size_t ema_period,
+                                              size_t rsi_period, size_t atr_period,
+                                              const int32_t *d_fv_close,
+                                              const int32_t *d_fv_hlc, const int32_t *d_clean,
+                                              float *d_ema_tm, float *d_rsi_tm,
+                                              float *d_atr_tm);
+vta_status vta_ema_rsi_atr_many_series_dev(vta_ctx *ctx, const float *close_tm,
+                                           const float *high_tm, const float *low_tm,
+                                           size_t cols, size_t rows, size_t ema_period,
+                                           size_t rsi_period, size_t atr_period,
+                                           vta_device_array_f32 *out3);
+
 #ifdef __cplusplus
 }
 #endif

@@ -73,6 +73,9 @@ def lib():
                                          ctypes.c_double, ctypes.c_double, dp,
                                          ctypes.POINTER(ctypes.c_int), ctypes.c_int]
         L.or_many_series_1in.restype = None
+        L.or_many_series_3in.argtypes = [ctypes.c_int, dp, dp, dp, sz, sz, sz, dp,
+                                         ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+        L.or_many_series_3in.restype = None
         _lib = L
     return _lib
 
@@ -195,3 +198,36 @@ def many_series(kind, tm, p0, p1=0.0, p2=0.0, nthreads=0):
     lib().or_many_series_1in(_KIND[kind], tp, cols, rows, float(p0), float(p1), float(p2),
                              _d(out)[1], st, int(nthreads))
     return out, np.frombuffer(st, dtype=np.int32).copy()
+
+
+_KIND3 = {"atr": 0, "adx": 1, "atr_batch_row": 2}
+
+
+def many_series_hlc(kind, h_tm, l_tm, c_tm, period, nthreads=0):
+    """ATR / ADX of every column of time-major [rows, cols] high/low/close."""
+    h, hp = _d(h_tm)
+    l, lp = _d(l_tm)
+    c, cp = _d(c_tm)
+    rows, cols = c.shape
+    out = np.full((rows, cols), np.nan)
+    st = (ctypes.c_int * cols)()
+    lib().or_many_series_3in(_KIND3[kind], hp, lp, cp, cols, rows, int(period), _d(out)[1], st,
+                             int(nthreads))
+    return out, np.frombuffer(st, dtype=np.int32).copy()
+
+
+def ema_batch(x, periods):
+    return np.stack([ema(x, p) for p in periods])
+
+
+def rsi_batch(x, periods):
+    return np.stack([rsi(x, p) for p in periods])
+
+
+def atr_batch(h, l, c, periods):
+    """The batch path's prefix-sum seed (atr.rs:1101-1213)."""
+    return np.stack([atr_batch_row(h, l, c, p) for p in periods])
+
+
+def adx_batch(h, l, c, periods):
+    return np.stack([adx(h, l, c, p) for p in periods])

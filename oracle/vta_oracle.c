@@ -642,3 +642,36 @@ void or_many_series_1in(int kind, const double *tm, size_t cols, size_t rows, do
         free(y);
     }
 }
+
+/* Three-input (high, low, close) single-series oracle over each column of
+ * time-major matrices.  kind: 0 atr, 1 adx, 2 atr batch-prefix row. */
+void or_many_series_3in(int kind, const double *h_tm, const double *l_tm, const double *c_tm,
+                        size_t cols, size_t rows, size_t period, double *out_tm, int *status,
+                        int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(static) if (nthreads != 1)
+#endif
+    for (long s = 0; s < (long)cols; ++s) {
+        double *h = (double *)malloc(sizeof(double) * rows);
+        double *l = (double *)malloc(sizeof(double) * rows);
+        double *c = (double *)malloc(sizeof(double) * rows);
+        double *y = (double *)malloc(sizeof(double) * rows);
+        for (size_t t = 0; t < rows; ++t) {
+            h[t] = h_tm[t * cols + s];
+            l[t] = l_tm[t * cols + s];
+            c[t] = c_tm[t * cols + s];
+        }
+        fill_nan(y, rows);
+        int st = OR_PARAM;
+        if (kind == 0) st = or_atr(h, l, c, rows, period, y);
+        else if (kind == 1) st = or_adx(h, l, c, rows, period, y);
+        else if (kind == 2) st = or_atr_batch_row(h, l, c, rows, period, y);
+        status[s] = st;
+        for (size_t t = 0; t < rows; ++t) out_tm[t * cols + s] = y[t];
+        free(h);
+        free(l);
+        free(c);
+        free(y);
+    }
+}

@@ -8,9 +8,12 @@ it, mirroring the reference's ``src/cuda`` wrapper classes (``CudaAlma`` ...).
 from ._lib import VtaError, lib
 from .alma import AlmaBatchRange, AlmaParams, AlmaSweepPlan, CudaAlma, HipAlma, expand_grid
 from .device import DeviceArrayF32, DeviceBuffer, HipContext, cuda_available, cuda_device_count
+from .rec import (CudaAdx, CudaAtr, CudaEma, CudaRsi, ManySeriesOhlcPlan,
+                  ema_rsi_atr_many_series_dev)
 
 __all__ = [
     "VtaError", "lib", "AlmaBatchRange", "AlmaParams", "AlmaSweepPlan", "CudaAlma", "HipAlma",
     "expand_grid", "DeviceArrayF32", "DeviceBuffer", "HipContext", "cuda_available",
-    "cuda_device_count",
+    "cuda_device_count", "CudaEma", "CudaRsi", "CudaAtr", "CudaAdx", "ManySeriesOhlcPlan",
+    "ema_rsi_atr_many_series_dev",
 ]

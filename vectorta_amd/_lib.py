@@ -94,6 +94,38 @@ SIGNATURES = {
         _ST, [_P, _P, _SZ, _SZ, ctypes.POINTER(AlmaParamsC), ctypes.POINTER(DeviceArrayF32C)]),
     "vta_alma_many_series_one_param_time_major_device": (
         _ST, [_P, _P, _P, _SZ, _F32, _SZ, _SZ, _P, _P]),
+    # recurrences
+    "vta_ema_expand_grid": (_ST, [ctypes.POINTER(RangeUsize), _P, _SZ, ctypes.POINTER(_SZ)]),
+    "vta_ema_batch_dev": (_ST, [_P, _P, _SZ, ctypes.POINTER(RangeUsize),
+                                ctypes.POINTER(DeviceArrayF32C)]),
+    "vta_ema_batch_device": (_ST, [_P, _P, _P, _P, _I32, _I32, _I32, _I32, _P]),
+    "vta_ema_many_series_one_param_time_major_dev": (_ST, [_P, _P, _SZ, _SZ, _SZ,
+                                                           ctypes.POINTER(DeviceArrayF32C)]),
+    "vta_ema_many_series_one_param_device": (_ST, [_P, _P, _P, _SZ, _SZ, _SZ, _P, _P]),
+    "vta_rsi_expand_grid": (_ST, [ctypes.POINTER(RangeUsize), _P, _SZ, ctypes.POINTER(_SZ)]),
+    "vta_rsi_batch_dev": (_ST, [_P, _P, _SZ, ctypes.POINTER(RangeUsize),
+                                ctypes.POINTER(DeviceArrayF32C)]),
+    "vta_rsi_batch_device": (_ST, [_P, _P, _P, _I32, _I32, _I32, _I32, _P]),
+    "vta_rsi_many_series_one_param_time_major_dev": (_ST, [_P, _P, _SZ, _SZ, _SZ,
+                                                           ctypes.POINTER(DeviceArrayF32C)]),
+    "vta_rsi_many_series_one_param_device": (_ST, [_P, _P, _P, _SZ, _SZ, _SZ, _P, _P]),
+    "vta_atr_expand_grid": (_ST, [ctypes.POINTER(RangeUsize), _P, _SZ, ctypes.POINTER(_SZ)]),
+    "vta_atr_batch_dev": (_ST, [_P, _P, _P, _P, _SZ, ctypes.POINTER(RangeUsize),
+                                ctypes.POINTER(DeviceArrayF32C)]),
+    "vta_atr_batch_device": (_ST, [_P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _P]),
+    "vta_atr_many_series_one_param_time_major_dev": (_ST, [_P, _P, _P, _P, _SZ, _SZ, _SZ,
+                                                           ctypes.POINTER(DeviceArrayF32C)]),
+    "vta_atr_many_series_one_param_device": (_ST, [_P, _P, _P, _P, _P, _SZ, _SZ, _SZ, _P, _P]),
+    "vta_adx_expand_grid": (_ST, [ctypes.POINTER(RangeUsize), _P, _SZ, ctypes.POINTER(_SZ)]),
+    "vta_adx_batch_dev": (_ST, [_P, _P, _P, _P, _SZ, ctypes.POINTER(RangeUsize),
+                                ctypes.POINTER(DeviceArrayF32C)]),
+    "vta_adx_many_series_one_param_time_major_dev": (_ST, [_P, _P, _P, _P, _SZ, _SZ, _SZ,
+                                                           ctypes.POINTER(DeviceArrayF32C)]),
+    "vta_many_series_prepare": (_ST, [_P, _P, _P, _P, _SZ, _SZ, ctypes.c_int, _P, _P, _P]),
+    "vta_ema_rsi_atr_many_series_device": (_ST, [_P, _P, _P, _P, _SZ, _SZ, _SZ, _SZ, _SZ, _P,
+                                                 _P, _P, _P, _P, _P]),
+    "vta_ema_rsi_atr_many_series_dev": (_ST, [_P, _P, _P, _P, _SZ, _SZ, _SZ, _SZ, _SZ,
+                                              ctypes.POINTER(DeviceArrayF32C)]),
 }
 
 _lib = None

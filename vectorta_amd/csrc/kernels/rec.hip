@@ -1,0 +1,561 @@
+// Wilder / exponential recurrences for gfx950: EMA, RSI, ATR, ADX and the fused
+// EMA+RSI+ATR many-series launch.
+//
+// Each lane runs one stream (a combo of a one-series batch, or one series of
+// a time-major matrix) through an f64 engine that restates the reference's
+// scalar arithmetic in the same order:
+//   EMA  src/indicators/moving_averages/ema.rs:430-468
+//   RSI  src/indicators/rsi.rs:327-414
+//   ATR  src/indicators/atr.rs:329-460
+//   ADX  src/indicators/adx.rs:299-422
+// so a lane that starts exactly at first_valid reproduces the oracle's f64
+// values and only rounds them to f32 on output.
+//
+// Time parallelism: the series is cut into chunks of L bars, one wave per
+// (64 streams, chunk).  A wave whose chunk starts far enough after the
+// warm-up begins K bars early with an approximate state and discards those
+// outputs ("burn-in").  All four recurrences contract the state error by
+// beta = 1 - 1/p (EMA: 1 - 2/(p+1)) per bar, so after K >= 40 / -ln(beta)
+// bars (ADX, two coupled stages: 48 / -ln(beta)) the error is below
+// e^-40 ~ 4e-18 of the state -- under FP64 rounding.  This holds only when
+// every bar after first_valid is finite (non-finite bars stop EMA decay and
+// poison ATR forever), so each stream carries a "clean" flag; an unclean
+// stream always starts exactly at first_valid.
+#include "vta_common.h"
+
+struct rec_args {
+    const float *x;          // close / prices
+    const float *h;          // high (ATR, ADX)
+    const float *l;          // low  (ATR, ADX)
+    const int32_t *periods;  // batch: per combo; many-series: [0] only
+    const int32_t *fvs;      // many-series: per series; batch: [0] only
+    const int32_t *clean;    // many-series: per series; batch: [0] only
+    const int32_t *fvs2;     // fused kernel: per-series first_valid of (high, low, close)
+    float *out0;
+    float *out1;
+    float *out2;
+    int T;                   // bars
+    int nlanes;              // combos (batch) or series (many-series)
+    int chunk;               // bars per chunk
+    int p1, p2, p3;          // fused kernel periods (EMA, RSI, ATR)
+};
+
+#define REC_NAN __builtin_bit_cast(double, 0x7ff8000000000000ull)
+
+__device__ __forceinline__ bool fin(double v) { return vta_isfinite_d(v); }
+
+__device__ __forceinline__ int burn_bars(double beta, double nats) {
+    if (!(beta > 0.0)) return 1;
+    double k = nats / -log(beta);
+    if (k > 1.0e8) return 100000000;
+    return max(1, (int)ceil(k));
+}
+
+// ---------------------------------------------------------------- engines
+
+struct EmaEng {
+    static constexpr bool kSeedScan = false;
+    double alpha, beta, prev;
+    int p, fv, vc;
+    __device__ void setup(int p_, int fv_) {
+        p = p_; fv = fv_;
+        alpha = 2.0 / ((double)p + 1.0);
+        beta = 1.0 - alpha;
+    }
+    __device__ int kb() const { return burn_bars(beta, 40.0); }
+    __device__ int exact_start() const { return fv; }
+    __device__ int burn_ok_from() const { return fv + p; }   // b >= this may burn in
+    // exact start at bar fv
+    __device__ double begin_exact(double x) { prev = x; vc = 1; return prev; }
+    __device__ void begin_burn(double x) { prev = fin(x) ? x : 0.0; vc = p; }
+    __device__ double step(int t, double x) {
+        if (t < fv + p) {                       // running-mean warm-up (ema.rs:441-452)
+            if (fin(x)) {
+                vc += 1;
+                double v = (double)vc;
+                prev = ((v - 1.0) * prev + x) / v;
+            }
+        } else if (fin(x)) {
+            prev = fma(beta, prev, alpha * x);  // ema.rs:458-466
+        }
+        return prev;
+    }
+    __device__ double out_before_start(int t) const { return REC_NAN; }
+};
+
+struct RsiEng {
+    static constexpr bool kSeedScan = true;
+    double inv_p, beta, ag, al, px;
+    int p, fv;
+    bool poisoned;
+    __device__ void setup(int p_, int fv_) {
+        p = p_; fv = fv_;
+        inv_p = 1.0 / (double)p;
+        beta = 1.0 - inv_p;
+        poisoned = false;
+    }
+    __device__ int kb() const { return burn_bars(beta, 40.0); }
+    __device__ int exact_start() const { return fv; }
+    __device__ int burn_ok_from() const { return fv + p + 1; }
+    __device__ double begin_exact(double x) { px = x; ag = 0.0; al = 0.0; return REC_NAN; }
+    __device__ void begin_burn(double x) { px = x; ag = 0.0; al = 0.0; }
+    // the seed window (fv, fv+p] has a non-finite delta -> NaN forever (rsi.rs:340-361)
+    __device__ void seed_scan(int t, double x) {
+        if (t > fv && t <= fv + p && !fin(x - px)) poisoned = true;
+        px = x;
+    }
+    __device__ double step(int t, double x) {
+        double d = x - px;
+        px = x;
+        if (t <= fv + p) {
+            if (!poisoned) {
+                if (!fin(d)) poisoned = true;
+                else if (d > 0.0) ag += d;
+                else if (d < 0.0) al -= d;
+            }
+            if (t < fv + p) return REC_NAN;
+            if (poisoned) { ag = REC_NAN; al = REC_NAN; return REC_NAN; }
+            ag *= inv_p;
+            al *= inv_p;
+        } else {
+            double g = d > 0.0 ? d : 0.0;
+            double lo = d < 0.0 ? -d : 0.0;
+            ag = fma(ag, beta, inv_p * g);
+            al = fma(al, beta, inv_p * lo);
+        }
+        if (poisoned) return REC_NAN;
+        double den = ag + al;
+        return den == 0.0 ? 50.0 : 100.0 * ag / den;
+    }
+};
+
+__device__ __forceinline__ double true_range(double hi, double lo, double pc) {
+    double tr = hi - lo;
+    double hc = fabs(hi - pc);
+    if (hc > tr) tr = hc;
+    double lc = fabs(lo - pc);
+    if (lc > tr) tr = lc;
+    return tr;
+}
+
+struct AtrEng {
+    static constexpr bool kSeedScan = false;
+    double alpha, sum, rma, pc;
+    int p, fv;
+    __device__ void setup(int p_, int fv_) {
+        p = p_; fv = fv_;
+        alpha = 1.0 / (double)p;
+    }
+    __device__ int kb() const { return burn_bars(1.0 - alpha, 40.0); }
+    __device__ int exact_start() const { return fv; }
+    __device__ int burn_ok_from() const { return fv + p; }
+    __device__ double begin_exact(double h, double l, double c) {
+        sum = h - l;
+        pc = c;
+        if (p == 1) { rma = sum / (double)p; return rma; }
+        return REC_NAN;
+    }
+    __device__ void begin_burn(double h, double l, double c) { rma = h - l; pc = c; }
+    __device__ double step(int t, double h, double l, double c) {
+        double tr = true_range(h, l, pc);
+        pc = c;
+        if (t < fv + p - 1) { sum += tr; return REC_NAN; }
+        if (t == fv + p - 1) { sum += tr; rma = sum / (double)p; return rma; }
+        rma = fma(-alpha, rma, rma) + alpha * tr;  // atr.rs:393
+        return rma;
+    }
+};
+
+struct AdxEng {
+    static constexpr bool kSeedScan = false;
+    double rp, omr, pm1, atr, pds, mds, dx_sum, adx, ph, pl, pc;
+    int p, fv, dx_count;
+    __device__ void setup(int p_, int fv_) {
+        p = p_; fv = fv_;
+        double pf = (double)p;
+        rp = 1.0 / pf;
+        omr = 1.0 - rp;
+        pm1 = pf - 1.0;
+    }
+    __device__ int kb() const { return burn_bars(omr, 48.0); }
+    __device__ int exact_start() const { return fv; }
+    __device__ int burn_ok_from() const { return fv + 2 * p; }
+    __device__ double begin_exact(double h, double l, double c) {
+        atr = 0.0; pds = 0.0; mds = 0.0; dx_sum = 0.0; adx = 0.0; dx_count = 0;
+        ph = h; pl = l; pc = c;
+        return REC_NAN;
+    }
+    __device__ void begin_burn(double h, double l, double c) {
+        atr = 0.0; pds = 0.0; mds = 0.0; adx = 0.0; dx_sum = 0.0; dx_count = p;
+        ph = h; pl = l; pc = c;
+    }
+    __device__ double step(int t, double ch, double cl, double cc) {
+        double hl = ch - cl;
+        double hpc = fabs(ch - pc);
+        double lpc = fabs(cl - pc);
+        double tr = fmax(fmax(hl, hpc), lpc);
+        double up = ch - ph;
+        double down = pl - cl;
+        ph = ch; pl = cl; pc = cc;
+        int i = t - fv;   // index within the slice starting at first_valid
+        if (i <= p) {     // adx.rs:327-348 seed sums
+            if (up > down && up > 0.0) pds += up;
+            if (down > up && down > 0.0) mds += down;
+            atr += tr;
+            if (i < p) return REC_NAN;
+            double pdi = 0.0, mdi = 0.0;
+            if (atr != 0.0) { pdi = (pds / atr) * 100.0; mdi = (mds / atr) * 100.0; }
+            double sdi = pdi + mdi;
+            dx_sum = sdi != 0.0 ? (fabs(pdi - mdi) / sdi) * 100.0 : 0.0;
+            dx_count = 1;
+            return REC_NAN;
+        }
+        double pdm = (up > down && up > 0.0) ? up : 0.0;
+        double mdm = (down > up && down > 0.0) ? down : 0.0;
+        atr = atr * omr + tr;
+        pds = pds * omr + pdm;
+        mds = mds * omr + mdm;
+        double pdi = 0.0, mdi = 0.0;
+        if (atr != 0.0) { pdi = (pds / atr) * 100.0; mdi = (mds / atr) * 100.0; }
+        double sdi = pdi + mdi;
+        double dx = sdi != 0.0 ? (fabs(pdi - mdi) / sdi) * 100.0 : 0.0;
+        if (dx_count < p) {
+            dx_sum += dx;
+            dx_count += 1;
+            if (dx_count == p) { adx = dx_sum * rp; return adx; }
+            return REC_NAN;
+        }
+        adx = (adx * pm1 + dx) * rp;
+        return adx;
+    }
+};
+
+// ---------------------------------------------------------------- chunk planning
+
+// Bar at which a lane starts (exact or burn-in) for chunk [c0, c1).
+template <class E>
+__device__ __forceinline__ int lane_start(const E &e, int c0, bool clean, bool *burn) {
+    int es = e.exact_start();
+    if (clean) {
+        int b = c0 - e.kb();
+        if (b >= e.burn_ok_from() && b > es) { *burn = true; return b; }
+    }
+    *burn = false;
+    return es;
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return v;
+}
+
+// ---------------------------------------------------------------- many series (time-major)
+//
+// grid.x: groups of 64 series, grid.y: chunks.  Lane = series; every load and
+// store is one coalesced row segment of the [T][N] matrix.
+template <class E, int NIN>
+__device__ __forceinline__ void drive_ms(const rec_args &a, int p, float *out) {
+    const int lane = threadIdx.x & 63;
+    const int s = blockIdx.x * 64 + lane;
+    const bool active = s < a.nlanes;
+    const int N = a.nlanes;
+    const int c0 = blockIdx.y * a.chunk;
+    const int c1 = min(c0 + a.chunk, a.T);
+    const int fv = active ? a.fvs[s] : a.T;
+    const bool clean = active ? (a.clean != nullptr && a.clean[s] != 0) : true;
+    E e;
+    e.setup(p, fv);
+    bool burn = false;
+    int start = active ? lane_start(e, c0, clean, &burn) : a.T;
+    if (start > a.T) start = a.T;
+    const int t_begin = wave_min(min(start, c0));
+    for (int t = t_begin; t < c1; ++t) {
+        const size_t off = (size_t)t * N + s;
+        double x = 0.0, hh = 0.0, ll = 0.0;
+        if (active && t >= min(start, c0)) {
+            x = (double)a.x[off];
+            if (NIN == 3) { hh = (double)a.h[off]; ll = (double)a.l[off]; }
+        }
+        double y = REC_NAN;
+        if (active) {
+            if (t == start) {
+                if constexpr (NIN == 1) {
+                    if (burn) e.begin_burn(x); else y = e.begin_exact(x);
+                } else {
+                    if (burn) e.begin_burn(hh, ll, x); else y = e.begin_exact(hh, ll, x);
+                }
+            } else if (t > start) {
+                if constexpr (NIN == 1) y = e.step(t, x); else y = e.step(t, hh, ll, x);
+            }
+            if (t >= c0) out[off] = (float)y;
+        }
+    }
+}
+
+// RSI burn-in needs the seed-window NaN check, done by a short scan first.
+__device__ __forceinline__ void drive_ms_rsi(const rec_args &a, int p, float *out) {
+    const int lane = threadIdx.x & 63;
+    const int s = blockIdx.x * 64 + lane;
+    const bool active = s < a.nlanes;
+    const int N = a.nlanes;
+    const int c0 = blockIdx.y * a.chunk;
+    const int c1 = min(c0 + a.chunk, a.T);
+    const int fv = active ? a.fvs[s] : a.T;
+    const bool clean = active ? (a.clean != nullptr && a.clean[s] != 0) : true;
+    RsiEng e;
+    e.setup(p, fv);
+    bool burn = false;
+    int start = active ? lane_start(e, c0, clean, &burn) : a.T;
+    if (burn) {
+        e.px = (double)a.x[(size_t)fv * N + s];
+        for (int t = fv + 1; t <= min(fv + p, a.T - 1); ++t) e.seed_scan(t, (double)a.x[(size_t)t * N + s]);
+    }
+    const int t_begin = wave_min(min(start, c0));
+    for (int t = t_begin; t < c1; ++t) {
+        const size_t off = (size_t)t * N + s;
+        double x = 0.0;
+        if (active && t >= min(start, c0)) x = (double)a.x[off];
+        double y = REC_NAN;
+        if (active) {
+            if (t == start) {
+                if (burn) e.begin_burn(x); else y = e.begin_exact(x);
+            } else if (t > start) {
+                y = e.step(t, x);
+            }
+            if (t >= c0) out[off] = (float)y;
+        }
+    }
+}
+
+extern "C" __global__ __launch_bounds__(64) void ema_ms1p_f32(rec_args a) {
+    drive_ms<EmaEng, 1>(a, a.periods[0], a.out0);
+}
+extern "C" __global__ __launch_bounds__(64) void rsi_ms1p_f32(rec_args a) {
+    drive_ms_rsi(a, a.periods[0], a.out0);
+}
+extern "C" __global__ __launch_bounds__(64) void atr_ms1p_f32(rec_args a) {
+    drive_ms<AtrEng, 3>(a, a.periods[0], a.out0);
+}
+extern "C" __global__ __launch_bounds__(64) void adx_ms1p_f32(rec_args a) {
+    drive_ms<AdxEng, 3>(a, a.periods[0], a.out0);
+}
+
+// ---------------------------------------------------------------- fused EMA + RSI + ATR
+//
+// One pass over close/high/low [T][N]: EMA(p1) and RSI(p2) of close and
+// ATR(p3) of (high, low, close), three time-major outputs.  Loads of the
+// next U rows are issued before the U rows in registers are consumed.
+#define FUSED_U 8
+extern "C" __global__ __launch_bounds__(64) void ms_ema_rsi_atr_f32(rec_args a) {
+    const int lane = threadIdx.x & 63;
+    const int s = blockIdx.x * 64 + lane;
+    const bool active = s < a.nlanes;
+    const int N = a.nlanes;
+    const int c0 = blockIdx.y * a.chunk;
+    const int c1 = min(c0 + a.chunk, a.T);
+    const int fv = active ? a.fvs[s] : a.T;          // first non-NaN close
+    const int fv3 = active ? a.fvs2[s] : a.T;        // first bar with high, low, close
+    const bool clean = active ? (a.clean != nullptr && a.clean[s] != 0) : true;
+    EmaEng ee; RsiEng re; AtrEng ae;
+    ee.setup(a.p1, fv); re.setup(a.p2, fv); ae.setup(a.p3, fv3);
+    bool be = false, br = false, ba = false;
+    int se = active ? lane_start(ee, c0, clean, &be) : a.T;
+    int sr = active ? lane_start(re, c0, clean, &br) : a.T;
+    int sa = active ? lane_start(ae, c0, clean, &ba) : a.T;
+    if (br) {
+        re.px = (double)a.x[(size_t)fv * N + s];
+        for (int t = fv + 1; t <= min(fv + a.p2, a.T - 1); ++t)
+            re.seed_scan(t, (double)a.x[(size_t)t * N + s]);
+    }
+    const int smin = min(min(se, sr), min(sa, c0));
+    const int t_begin = wave_min(smin);
+    for (int t0 = t_begin; t0 < c1; t0 += FUSED_U) {
+        float xc[FUSED_U], xh[FUSED_U], xl[FUSED_U];
+#pragma unroll
+        for (int u = 0; u < FUSED_U; ++u) {
+            const int t = t0 + u;
+            const size_t off = (size_t)t * N + s;
+            const bool ld = active && t < c1 && t >= smin;
+            xc[u] = ld ? a.x[off] : 0.0f;
+            xh[u] = ld ? a.h[off] : 0.0f;
+            xl[u] = ld ? a.l[off] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < FUSED_U; ++u) {
+            const int t = t0 + u;
+            if (!active || t >= c1) break;
+            const double x = xc[u], hh = xh[u], ll = xl[u];
+            double y0 = REC_NAN, y1 = REC_NAN, y2 = REC_NAN;
+            if (t == se) { if (be) ee.begin_burn(x); else y0 = ee.begin_exact(x); }
+            else if (t > se) y0 = ee.step(t, x);
+            if (t == sr) { if (br) re.begin_burn(x); else y1 = re.begin_exact(x); }
+            else if (t > sr) y1 = re.step(t, x);
+            if (t == sa) { if (ba) ae.begin_burn(hh, ll, x); else y2 = ae.begin_exact(hh, ll, x); }
+            else if (t > sa) y2 = ae.step(t, hh, ll, x);
+            if (t >= c0) {
+                const size_t off = (size_t)t * N + s;
+                a.out0[off] = (float)y0;
+                a.out1[off] = (float)y1;
+                a.out2[off] = (float)y2;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- one series, many params
+//
+// grid.x: groups of 64 combos, grid.y: chunks.  Lane = combo.  The wave
+// stages 64 bars of the series in LDS (one coalesced load, then broadcast
+// reads), and buffers 64 bars x 64 combos of output in LDS so each output row
+// segment is written as one coalesced 256-byte store.
+template <class E, int NIN>
+__device__ __forceinline__ void drive_batch(const rec_args &a, float *out) {
+    __shared__ float xs[3][64];
+    __shared__ float ob[64][65];
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.x * 64 + lane;
+    const bool active = c < a.nlanes;
+    const int T = a.T;
+    const int c0 = blockIdx.y * a.chunk;
+    const int c1 = min(c0 + a.chunk, T);
+    const int fv = a.fvs[0];
+    const bool clean = a.clean != nullptr && a.clean[0] != 0;
+    const int p = active ? a.periods[c] : 1;
+    E e;
+    e.setup(p, fv);
+    bool burn = false;
+    int start = active ? lane_start(e, c0, clean, &burn) : T;
+    if constexpr (E::kSeedScan) {
+        // RSI burn-in: the seed window's NaN rule still has to be seen
+        if (burn) {
+            e.px = (double)a.x[fv];
+            for (int t = fv + 1; t <= min(fv + p, T - 1); ++t) e.seed_scan(t, (double)a.x[t]);
+        }
+    }
+    const int t_begin = wave_min(min(start, c0)) & ~63;
+    for (int tb = t_begin; tb < c1; tb += 64) {
+        {
+            const int t = tb + lane;
+            const bool ok = t < T;
+            xs[0][lane] = ok ? a.x[t] : 0.0f;
+            if (NIN == 3) {
+                xs[1][lane] = ok ? a.h[t] : 0.0f;
+                xs[2][lane] = ok ? a.l[t] : 0.0f;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int j = 0; j < 64; ++j) {
+            const int t = tb + j;
+            double y = REC_NAN;
+            if (active && t < c1) {
+                const double x = xs[0][j];
+                if (t == start) {
+                    if constexpr (NIN == 1) {
+                        if (burn) e.begin_burn(x); else y = e.begin_exact(x);
+                    } else {
+                        if (burn) e.begin_burn(xs[1][j], xs[2][j], x);
+                        else y = e.begin_exact(xs[1][j], xs[2][j], x);
+                    }
+                } else if (t > start) {
+                    if constexpr (NIN == 1) y = e.step(t, x);
+                    else y = e.step(t, xs[1][j], xs[2][j], x);
+                }
+            }
+            ob[lane][j] = (float)y;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // write rows: lane = bar offset
+        if (tb + 64 > c0) {
+            const int t = tb + lane;
+            const bool ok = t >= c0 && t < c1;
+            const int nrow = min(64, a.nlanes - blockIdx.x * 64);
+            for (int r = 0; r < nrow; ++r)
+                if (ok) out[(size_t)(blockIdx.x * 64 + r) * T + t] = ob[r][lane];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+extern "C" __global__ __launch_bounds__(64) void ema_batch_f32(rec_args a) {
+    drive_batch<EmaEng, 1>(a, a.out0);
+}
+extern "C" __global__ __launch_bounds__(64) void rsi_batch_f32(rec_args a) {
+    drive_batch<RsiEng, 1>(a, a.out0);
+}
+extern "C" __global__ __launch_bounds__(64) void atr_batch_f32(rec_args a) {
+    drive_batch<AtrEng, 3>(a, a.out0);
+}
+extern "C" __global__ __launch_bounds__(64) void adx_batch_f32(rec_args a) {
+    drive_batch<AdxEng, 3>(a, a.out0);
+}
+
+// ---------------------------------------------------------------- preparation
+//
+// Per time-major series: first_valid and the clean flag (every input finite
+// at every bar from first_valid on).  One lane per series walks its column;
+// each row is one coalesced load across lanes.
+//   mode 0: x only (EMA, RSI)
+//   mode 1: first bar with high, low and x all non-NaN (ATR, atr.rs:197-207)
+//   mode 2: max of the per-input first non-NaN bars (ADX, adx.rs:190-198)
+//   mode 3: fv = first non-NaN x, fv2 = mode-1 bar, clean over all inputs
+//           from fv on (fused EMA/RSI/ATR)
+extern "C" __global__ __launch_bounds__(64) void ms_prepare_meta(
+    const float *__restrict__ x, const float *__restrict__ h, const float *__restrict__ l,
+    int T, int N, int mode, int32_t *__restrict__ fvs, int32_t *__restrict__ fvs2,
+    int32_t *__restrict__ clean) {
+    const int s = blockIdx.x * 64 + (threadIdx.x & 63);
+    if (s >= N) return;
+    const bool hl = mode != 0;
+    int fx = T, fh = T, fl = T, f3 = T;
+    for (int t = 0; t < T; ++t) {
+        const size_t off = (size_t)t * N + s;
+        const bool nx = !isnan(x[off]);
+        const bool nh = hl ? !isnan(h[off]) : true;
+        const bool nl = hl ? !isnan(l[off]) : true;
+        if (fx == T && nx) fx = t;
+        if (fh == T && nh) fh = t;
+        if (fl == T && nl) fl = t;
+        if (f3 == T && nx && nh && nl) f3 = t;
+        if (fx < T && f3 < T && fh < T && fl < T) break;
+    }
+    int f = mode == 0 ? fx : mode == 1 ? f3 : mode == 2 ? max(fx, max(fh, fl)) : fx;
+    bool ok = true;
+    for (int t = f; t < T; ++t) {
+        const size_t off = (size_t)t * N + s;
+        ok = ok && vta_isfinite(x[off]) && (!hl || (vta_isfinite(h[off]) && vta_isfinite(l[off])));
+    }
+    fvs[s] = f;
+    if (fvs2) fvs2[s] = f3;
+    clean[s] = ok ? 1 : 0;
+}
+
+// Single series: clean flag = no non-finite input in [first_valid, T).
+// flag[0] must be 0 on entry (see vta_fill_i32); every failing lane stores 1.
+extern "C" __global__ __launch_bounds__(256) void series_nonfinite_flag(
+    const float *__restrict__ x, const float *__restrict__ h, const float *__restrict__ l,
+    int T, int first_valid, int32_t *__restrict__ flag) {
+    const int t = blockIdx.x * 256 + threadIdx.x + first_valid;
+    if (t >= T) return;
+    bool bad = !vta_isfinite(x[t]);
+    if (h) bad = bad || !vta_isfinite(h[t]);
+    if (l) bad = bad || !vta_isfinite(l[t]);
+    if (bad) flag[0] = 1;
+}
+
+// clean[0] = (flag[0] == 0): turns the non-finite flag into the engines' clean word.
+extern "C" __global__ __launch_bounds__(64) void flag_to_clean(const int32_t *__restrict__ flag,
+                                                               int32_t *__restrict__ clean) {
+    if (threadIdx.x == 0) clean[0] = flag[0] == 0 ? 1 : 0;
+}
+
+extern "C" __global__ __launch_bounds__(256) void vta_fill_i32(int32_t *__restrict__ p, int n,
+                                                                int v) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}

@@ -1,0 +1,535 @@
+// Host entry points for the recurrence indicators: CudaEma
+// (src/cuda/moving_averages/ema_wrapper.rs), CudaRsi (src/cuda/rsi_wrapper.rs),
+// CudaAtr (src/cuda/atr_wrapper.rs), CudaAdx (src/cuda/adx_wrapper.rs), plus the
+// fused EMA + RSI + ATR many-series launch.  Kernels: kernels/rec.hip.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "runtime_internal.h"
+
+using namespace vta;
+
+namespace {
+
+// Host mirror of struct rec_args in kernels/rec.hip (same field order/types).
+struct rec_args_host {
+    const float *x;
+    const float *h;
+    const float *l;
+    const int32_t *periods;
+    const int32_t *fvs;
+    const int32_t *clean;
+    const int32_t *fvs2;
+    float *out0;
+    float *out1;
+    float *out2;
+    int T;
+    int nlanes;
+    int chunk;
+    int p1, p2, p3;
+};
+
+enum Ind { IND_EMA = 0, IND_RSI = 1, IND_ATR = 2, IND_ADX = 3 };
+
+const char *batch_kernel(Ind k) {
+    static const char *n[] = {"ema_batch_f32", "rsi_batch_f32", "atr_batch_f32", "adx_batch_f32"};
+    return n[k];
+}
+const char *ms_kernel(Ind k) {
+    static const char *n[] = {"ema_ms1p_f32", "rsi_ms1p_f32", "atr_ms1p_f32", "adx_ms1p_f32"};
+    return n[k];
+}
+
+// Bars of burn-in each engine needs (same formula as burn_bars in rec.hip).
+double burn_bars(Ind k, size_t p) {
+    double beta = k == IND_EMA ? 1.0 - 2.0 / ((double)p + 1.0) : 1.0 - 1.0 / (double)p;
+    double nats = k == IND_ADX ? 48.0 : 40.0;
+    if (!(beta > 0.0)) return 1.0;
+    return std::ceil(nats / -std::log(beta));
+}
+
+// Chunk length: about four times the longest burn-in (so burn-in costs at
+// most ~25% extra bars), at least 1024 bars, small enough that the launch
+// has roughly 2048+ waves; VTA_REC_CHUNK overrides.
+int pick_chunk(Ind k, size_t max_p, size_t T, size_t lane_groups, bool clean) {
+    if (!clean) return (int)T;  // exact from first_valid in one pass
+    if (const char *e = getenv("VTA_REC_CHUNK")) {
+        long v = atol(e);
+        if (v >= 64) return (int)std::min<size_t>((size_t)v, T);
+    }
+    double kb = burn_bars(k, max_p);
+    size_t want_chunks = std::max<size_t>(1, 2048 / std::max<size_t>(lane_groups, 1));
+    size_t by_par = (T + want_chunks - 1) / want_chunks;
+    size_t c = std::max<size_t>({(size_t)(4.0 * kb), 1024, by_par});
+    c = (c + 63) & ~(size_t)63;
+    return (int)std::min(c, T);
+}
+
+// Grid expansion for the one-parameter indicators.
+vta_status expand_periods(Ind k, const vta_range_usize *r, std::vector<size_t> &ps) {
+    if (!r) return fail(VTA_ERR_INVALID_INPUT, "range is null");
+    AxisKind ak = (k == IND_ATR) ? AXIS_ATR : (k == IND_ADX) ? AXIS_DOWN : AXIS_SORTED;
+    if (!expand_usize(*r, ak, ps) || ps.empty())
+        return fail(VTA_ERR_INVALID_INPUT, "invalid period range: start=%zu end=%zu step=%zu",
+                    r->start, r->end, r->step);
+    for (size_t p : ps)
+        if (p == 0 || p > (size_t)INT32_MAX)
+            return fail(VTA_ERR_INVALID_INPUT, "invalid period %zu", p);
+    return VTA_OK;
+}
+
+// Minimum valid bars after first_valid each indicator needs (the CPU
+// validation: ema.rs:307-312, rsi.rs:211-216, atr.rs:218-225, adx.rs:252-257).
+size_t need_bars(Ind k, size_t p) { return k == IND_ADX ? p + 1 : p; }
+
+size_t first_valid_of(Ind k, const float *x, const float *h, const float *l, size_t n) {
+    if (k == IND_EMA || k == IND_RSI) return first_valid_f32(x, n);
+    if (k == IND_ATR) {
+        size_t i = 0;
+        while (i < n && (std::isnan(h[i]) || std::isnan(l[i]) || std::isnan(x[i]))) ++i;
+        return i;
+    }
+    size_t a = first_valid_f32(x, n), b = first_valid_f32(h, n), c = first_valid_f32(l, n);
+    return std::max(a, std::max(b, c));
+}
+
+bool host_clean(const float *x, const float *h, const float *l, size_t n, size_t fv) {
+    for (size_t i = fv; i < n; ++i) {
+        if (!std::isfinite(x[i])) return false;
+        if (h && (!std::isfinite(h[i]) || !std::isfinite(l[i]))) return false;
+    }
+    return true;
+}
+
+vta_status launch_rec(vta_ctx *ctx, const char *kname, rec_args_host &a, unsigned groups,
+                      unsigned chunks, unsigned block = 64) {
+    hipFunction_t f;
+    VTA_TRY(get_function(ctx, "rec", kname, &f));
+    return launch(ctx, f, dim3(groups, chunks, 1), dim3(block), 0, a);
+}
+
+// One series x many periods, device input and output, fv and clean known.
+vta_status batch_launch(vta_ctx *ctx, Ind k, const float *d_x, const float *d_h,
+                        const float *d_l, const int32_t *d_periods, size_t max_p, size_t T,
+                        size_t C, const int32_t *d_fv1, const int32_t *d_clean1, bool clean,
+                        float *d_out) {
+    rec_args_host a{};
+    a.x = d_x;
+    a.h = d_h;
+    a.l = d_l;
+    a.periods = d_periods;
+    a.fvs = d_fv1;
+    a.clean = d_clean1;
+    a.out0 = d_out;
+    a.T = (int)T;
+    a.nlanes = (int)C;
+    unsigned groups = (unsigned)((C + 63) / 64);
+    a.chunk = pick_chunk(k, max_p, T, groups, clean);
+    unsigned chunks = (unsigned)((T + a.chunk - 1) / a.chunk);
+    return launch_rec(ctx, batch_kernel(k), a, groups, chunks);
+}
+
+vta_status batch_dev(vta_ctx *ctx, Ind k, const float *x, const float *h, const float *l,
+                     size_t len, const vta_range_usize *range, vta_device_array_f32 *out) {
+    clear_error();
+    if (!ctx || !out || !x || (k >= IND_ATR && (!h || !l)))
+        return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    if (len == 0) return fail(VTA_ERR_INVALID_INPUT, "empty data");
+    std::vector<size_t> ps;
+    VTA_TRY(expand_periods(k, range, ps));
+    size_t fv = first_valid_of(k, x, h, l, len);
+    if (fv >= len) return fail(VTA_ERR_INVALID_INPUT, "all values are NaN");
+    size_t max_p = *std::max_element(ps.begin(), ps.end());
+    if (len - fv < need_bars(k, max_p))
+        return fail(VTA_ERR_INVALID_INPUT, "not enough valid data (needed >= %zu, valid = %zu)",
+                    need_bars(k, max_p), len - fv);
+    if (len > (size_t)INT32_MAX) return fail(VTA_ERR_INVALID_INPUT, "series too long");
+    bool clean = host_clean(x, k >= IND_ATR ? h : nullptr, l, len, fv);
+    size_t C = ps.size(), n_out;
+    if (!mul_ok(C, len, &n_out) || !mul_ok(n_out, sizeof(float), &n_out))
+        return fail(VTA_ERR_INVALID_INPUT, "output bytes overflow");
+    size_t nin = k >= IND_ATR ? 3 : 1;
+    VTA_TRY(check_fits(n_out + nin * len * sizeof(float)));
+    std::vector<int32_t> per(ps.begin(), ps.end());
+    int32_t meta[2] = {(int32_t)fv, clean ? 1 : 0};
+    DevBuf dx, dh, dl, dp, dm, dout;
+    VTA_TRY(upload(ctx, dx, x, len * sizeof(float)));
+    if (nin == 3) {
+        VTA_TRY(upload(ctx, dh, h, len * sizeof(float)));
+        VTA_TRY(upload(ctx, dl, l, len * sizeof(float)));
+    }
+    VTA_TRY(upload(ctx, dp, per.data(), per.size() * sizeof(int32_t)));
+    VTA_TRY(upload(ctx, dm, meta, sizeof(meta)));
+    VTA_TRY(dout.alloc(n_out));
+    VTA_TRY(batch_launch(ctx, k, dx.as<float>(), dh.as<float>(), dl.as<float>(),
+                         dp.as<int32_t>(), max_p, len, C, dm.as<int32_t>(),
+                         dm.as<int32_t>() + 1, clean, dout.as<float>()));
+    VTA_HIP(hipStreamSynchronize(ctx->stream));
+    out->ptr = static_cast<float *>(dout.release());
+    out->rows = C;
+    out->cols = len;
+    return VTA_OK;
+}
+
+// Device-resident one-series batch: the clean flag is computed on the device
+// into the context scratch.
+vta_status batch_device(vta_ctx *ctx, Ind k, const float *d_x, const float *d_h,
+                        const float *d_l, const int32_t *d_periods, int32_t max_period,
+                        int32_t series_len, int32_t first_valid, int32_t n_combos,
+                        float *d_out) {
+    clear_error();
+    if (!ctx || !d_x || !d_periods || !d_out || (k >= IND_ATR && (!d_h || !d_l)))
+        return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    if (series_len <= 0 || n_combos <= 0 || max_period <= 0 || first_valid < 0 ||
+        first_valid >= series_len)
+        return fail(VTA_ERR_INVALID_INPUT, "series_len, n_combos and max_period must be positive "
+                                           "and first_valid < series_len");
+    // context scratch: [0] first_valid, [2] non-finite flag, [3] clean word
+    int32_t *meta = static_cast<int32_t *>(ctx->scratch);
+    hipFunction_t fill, flag, inv;
+    VTA_TRY(get_function(ctx, "rec", "vta_fill_i32", &fill));
+    VTA_TRY(get_function(ctx, "rec", "series_nonfinite_flag", &flag));
+    VTA_TRY(get_function(ctx, "rec", "flag_to_clean", &inv));
+    VTA_TRY(launch(ctx, fill, dim3(1), dim3(256), 0, meta, 1, (int)first_valid));
+    VTA_TRY(launch(ctx, fill, dim3(1), dim3(256), 0, meta + 2, 1, 0));
+    unsigned nb = (unsigned)((series_len - first_valid + 255) / 256);
+    VTA_TRY(launch(ctx, flag, dim3(nb), dim3(256), 0, d_x, k >= IND_ATR ? d_h : nullptr,
+                   k >= IND_ATR ? d_l : nullptr, (int)series_len, (int)first_valid, meta + 2));
+    VTA_TRY(launch(ctx, inv, dim3(1), dim3(64), 0, meta + 2, meta + 3));
+    // Chunked with burn-in: a series found unclean still runs correctly, since
+    // every lane of an unclean stream starts exactly at first_valid.
+    return batch_launch(ctx, k, d_x, d_h, d_l, d_periods, (size_t)max_period,
+                        (size_t)series_len, (size_t)n_combos, meta, meta + 3, true, d_out);
+}
+
+vta_status ms_launch(vta_ctx *ctx, Ind k, const float *d_x, const float *d_h, const float *d_l,
+                     size_t cols, size_t rows, size_t period, const int32_t *d_fvs,
+                     const int32_t *d_clean, float *d_out) {
+    int32_t *pp = static_cast<int32_t *>(ctx->scratch) + 64;
+    hipFunction_t fill;
+    VTA_TRY(get_function(ctx, "rec", "vta_fill_i32", &fill));
+    VTA_TRY(launch(ctx, fill, dim3(1), dim3(256), 0, pp, 1, (int)period));
+    rec_args_host a{};
+    a.x = d_x;
+    a.h = d_h;
+    a.l = d_l;
+    a.periods = pp;
+    a.fvs = d_fvs;
+    a.clean = d_clean;
+    a.out0 = d_out;
+    a.T = (int)rows;
+    a.nlanes = (int)cols;
+    unsigned groups = (unsigned)((cols + 63) / 64);
+    a.chunk = pick_chunk(k, period, rows, groups, d_clean != nullptr);
+    // without clean flags every series runs exactly from first_valid in one chunk
+    if (!d_clean) a.chunk = (int)rows;
+    unsigned chunks = (unsigned)((rows + a.chunk - 1) / a.chunk);
+    return launch_rec(ctx, ms_kernel(k), a, groups, chunks);
+}
+
+vta_status ms_dev(vta_ctx *ctx, Ind k, const float *x_tm, const float *h_tm, const float *l_tm,
+                  size_t cols, size_t rows, size_t period, vta_device_array_f32 *out) {
+    clear_error();
+    if (!ctx || !out || !x_tm || (k >= IND_ATR && (!h_tm || !l_tm)))
+        return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    if (cols == 0 || rows == 0) return fail(VTA_ERR_INVALID_INPUT, "cols or rows is zero");
+    if (period == 0) return fail(VTA_ERR_INVALID_INPUT, "period must be > 0");
+    size_t elems;
+    if (!mul_ok(cols, rows, &elems) || elems > (size_t)INT32_MAX * 16)
+        return fail(VTA_ERR_INVALID_INPUT, "cols*rows overflow");
+    if (rows > (size_t)INT32_MAX || cols > (size_t)INT32_MAX)
+        return fail(VTA_ERR_INVALID_INPUT, "dimensions exceed int32");
+    // per-series first_valid and clean flag on the host (the reference scans
+    // each column on the host too, e.g. ema_wrapper.rs:637-700)
+    std::vector<int32_t> fvs(cols), cl(cols);
+    std::vector<float> col_x(rows), col_h, col_l;
+    if (k >= IND_ATR) {
+        col_h.resize(rows);
+        col_l.resize(rows);
+    }
+    for (size_t s = 0; s < cols; ++s) {
+        for (size_t t = 0; t < rows; ++t) col_x[t] = x_tm[t * cols + s];
+        if (k >= IND_ATR)
+            for (size_t t = 0; t < rows; ++t) {
+                col_h[t] = h_tm[t * cols + s];
+                col_l[t] = l_tm[t * cols + s];
+            }
+        size_t fv = first_valid_of(k, col_x.data(), col_h.data(), col_l.data(), rows);
+        if (fv >= rows) return fail(VTA_ERR_INVALID_INPUT, "series %zu all NaN", s);
+        if (rows - fv < need_bars(k, period))
+            return fail(VTA_ERR_INVALID_INPUT,
+                        "series %zu not enough valid data (needed >= %zu, valid = %zu)", s,
+                        need_bars(k, period), rows - fv);
+        fvs[s] = (int32_t)fv;
+        cl[s] = host_clean(col_x.data(), k >= IND_ATR ? col_h.data() : nullptr, col_l.data(),
+                           rows, fv) ? 1 : 0;
+    }
+    size_t nin = k >= IND_ATR ? 3 : 1;
+    VTA_TRY(check_fits((nin + 1) * elems * sizeof(float)));
+    DevBuf dx, dh, dl, dfv, dcl, dout;
+    VTA_TRY(upload(ctx, dx, x_tm, elems * sizeof(float)));
+    if (nin == 3) {
+        VTA_TRY(upload(ctx, dh, h_tm, elems * sizeof(float)));
+        VTA_TRY(upload(ctx, dl, l_tm, elems * sizeof(float)));
+    }
+    VTA_TRY(upload(ctx, dfv, fvs.data(), cols * sizeof(int32_t)));
+    VTA_TRY(upload(ctx, dcl, cl.data(), cols * sizeof(int32_t)));
+    VTA_TRY(dout.alloc(elems * sizeof(float)));
+    VTA_TRY(ms_launch(ctx, k, dx.as<float>(), dh.as<float>(), dl.as<float>(), cols, rows, period,
+                      dfv.as<int32_t>(), dcl.as<int32_t>(), dout.as<float>()));
+    VTA_HIP(hipStreamSynchronize(ctx->stream));
+    out->ptr = static_cast<float *>(dout.release());
+    out->rows = rows;
+    out->cols = cols;
+    return VTA_OK;
+}
+
+vta_status expand_out(Ind k, const vta_range_usize *range, size_t *out, size_t cap,
+                      size_t *n_out) {
+    clear_error();
+    std::vector<size_t> ps;
+    VTA_TRY(expand_periods(k, range, ps));
+    if (n_out) *n_out = ps.size();
+    if (out) memcpy(out, ps.data(), std::min(cap, ps.size()) * sizeof(size_t));
+    return VTA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ------------------------------------------------------------------ EMA
+vta_status vta_ema_expand_grid(const vta_range_usize *range, size_t *periods, size_t cap,
+                               size_t *n_out) {
+    return expand_out(IND_EMA, range, periods, cap, n_out);
+}
+vta_status vta_ema_batch_dev(vta_ctx *ctx, const float *prices, size_t len,
+                             const vta_range_usize *range, vta_device_array_f32 *out) {
+    return batch_dev(ctx, IND_EMA, prices, nullptr, nullptr, len, range, out);
+}
+vta_status vta_ema_batch_device(vta_ctx *ctx, const float *d_prices, const int32_t *d_periods,
+                                const float *d_alphas, int32_t max_period, int32_t series_len,
+                                int32_t first_valid, int32_t n_combos, float *d_out) {
+    (void)d_alphas;  // alpha = 2/(p+1) is recomputed in f64 from d_periods
+    return batch_device(ctx, IND_EMA, d_prices, nullptr, nullptr, d_periods, max_period,
+                        series_len, first_valid, n_combos, d_out);
+}
+vta_status vta_ema_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *data_tm,
+                                                        size_t cols, size_t rows, size_t period,
+                                                        vta_device_array_f32 *out) {
+    return ms_dev(ctx, IND_EMA, data_tm, nullptr, nullptr, cols, rows, period, out);
+}
+vta_status vta_ema_many_series_one_param_device(vta_ctx *ctx, const float *d_prices_tm,
+                                                const int32_t *d_first_valids, size_t period,
+                                                size_t num_series, size_t series_len,
+                                                const int32_t *d_clean, float *d_out_tm) {
+    clear_error();
+    if (!ctx || !d_prices_tm || !d_first_valids || !d_out_tm || period == 0 || num_series == 0 ||
+        series_len == 0)
+        return fail(VTA_ERR_INVALID_INPUT, "invalid arguments");
+    return ms_launch(ctx, IND_EMA, d_prices_tm, nullptr, nullptr, num_series, series_len, period,
+                     d_first_valids, d_clean, d_out_tm);
+}
+
+// ------------------------------------------------------------------ RSI
+vta_status vta_rsi_expand_grid(const vta_range_usize *range, size_t *periods, size_t cap,
+                               size_t *n_out) {
+    return expand_out(IND_RSI, range, periods, cap, n_out);
+}
+vta_status vta_rsi_batch_dev(vta_ctx *ctx, const float *prices, size_t len,
+                             const vta_range_usize *range, vta_device_array_f32 *out) {
+    return batch_dev(ctx, IND_RSI, prices, nullptr, nullptr, len, range, out);
+}
+vta_status vta_rsi_batch_device(vta_ctx *ctx, const float *d_prices, const int32_t *d_periods,
+                                int32_t max_period, int32_t series_len, int32_t first_valid,
+                                int32_t n_combos, float *d_out) {
+    return batch_device(ctx, IND_RSI, d_prices, nullptr, nullptr, d_periods, max_period,
+                        series_len, first_valid, n_combos, d_out);
+}
+vta_status vta_rsi_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *prices_tm,
+                                                        size_t cols, size_t rows, size_t period,
+                                                        vta_device_array_f32 *out) {
+    return ms_dev(ctx, IND_RSI, prices_tm, nullptr, nullptr, cols, rows, period, out);
+}
+vta_status vta_rsi_many_series_one_param_device(vta_ctx *ctx, const float *d_prices_tm,
+                                                const int32_t *d_first_valids, size_t period,
+                                                size_t num_series, size_t series_len,
+                                                const int32_t *d_clean, float *d_out_tm) {
+    clear_error();
+    if (!ctx || !d_prices_tm || !d_first_valids || !d_out_tm || period == 0 || num_series == 0 ||
+        series_len == 0)
+        return fail(VTA_ERR_INVALID_INPUT, "invalid arguments");
+    return ms_launch(ctx, IND_RSI, d_prices_tm, nullptr, nullptr, num_series, series_len, period,
+                     d_first_valids, d_clean, d_out_tm);
+}
+
+// ------------------------------------------------------------------ ATR
+vta_status vta_atr_expand_grid(const vta_range_usize *range, size_t *periods, size_t cap,
+                               size_t *n_out) {
+    return expand_out(IND_ATR, range, periods, cap, n_out);
+}
+vta_status vta_atr_batch_dev(vta_ctx *ctx, const float *high, const float *low,
+                             const float *close, size_t len, const vta_range_usize *range,
+                             vta_device_array_f32 *out) {
+    return batch_dev(ctx, IND_ATR, close, high, low, len, range, out);
+}
+vta_status vta_atr_batch_device(vta_ctx *ctx, const float *d_high, const float *d_low,
+                                const float *d_close, const int32_t *d_periods,
+                                int32_t max_period, int32_t series_len, int32_t first_valid,
+                                int32_t n_combos, float *d_out) {
+    return batch_device(ctx, IND_ATR, d_close, d_high, d_low, d_periods, max_period, series_len,
+                        first_valid, n_combos, d_out);
+}
+vta_status vta_atr_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *high_tm,
+                                                        const float *low_tm,
+                                                        const float *close_tm, size_t cols,
+                                                        size_t rows, size_t period,
+                                                        vta_device_array_f32 *out) {
+    return ms_dev(ctx, IND_ATR, close_tm, high_tm, low_tm, cols, rows, period, out);
+}
+vta_status vta_atr_many_series_one_param_device(vta_ctx *ctx, const float *d_high_tm,
+                                                const float *d_low_tm, const float *d_close_tm,
+                                                const int32_t *d_first_valids, size_t period,
+                                                size_t num_series, size_t series_len,
+                                                const int32_t *d_clean, float *d_out_tm) {
+    clear_error();
+    if (!ctx || !d_high_tm || !d_low_tm || !d_close_tm || !d_first_valids || !d_out_tm ||
+        period == 0 || num_series == 0 || series_len == 0)
+        return fail(VTA_ERR_INVALID_INPUT, "invalid arguments");
+    return ms_launch(ctx, IND_ATR, d_close_tm, d_high_tm, d_low_tm, num_series, series_len,
+                     period, d_first_valids, d_clean, d_out_tm);
+}
+
+// ------------------------------------------------------------------ ADX
+vta_status vta_adx_expand_grid(const vta_range_usize *range, size_t *periods, size_t cap,
+                               size_t *n_out) {
+    return expand_out(IND_ADX, range, periods, cap, n_out);
+}
+vta_status vta_adx_batch_dev(vta_ctx *ctx, const float *high, const float *low,
+                             const float *close, size_t len, const vta_range_usize *range,
+                             vta_device_array_f32 *out) {
+    return batch_dev(ctx, IND_ADX, close, high, low, len, range, out);
+}
+vta_status vta_adx_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *high_tm,
+                                                        const float *low_tm,
+                                                        const float *close_tm, size_t cols,
+                                                        size_t rows, size_t period,
+                                                        vta_device_array_f32 *out) {
+    return ms_dev(ctx, IND_ADX, close_tm, high_tm, low_tm, cols, rows, period, out);
+}
+
+// ------------------------------------------------------------------ shared prep
+vta_status vta_many_series_prepare(vta_ctx *ctx, const float *d_x_tm, const float *d_high_tm,
+                                   const float *d_low_tm, size_t cols, size_t rows, int mode,
+                                   int32_t *d_first_valids, int32_t *d_first_valids_hlc,
+                                   int32_t *d_clean) {
+    clear_error();
+    if (!ctx || !d_x_tm || !d_first_valids || !d_clean || cols == 0 || rows == 0 || mode < 0 ||
+        mode > 3 || (mode != 0 && (!d_high_tm || !d_low_tm)))
+        return fail(VTA_ERR_INVALID_INPUT, "invalid arguments");
+    hipFunction_t f;
+    VTA_TRY(get_function(ctx, "rec", "ms_prepare_meta", &f));
+    return launch(ctx, f, dim3((unsigned)((cols + 63) / 64)), dim3(64), 0, d_x_tm, d_high_tm,
+                  d_low_tm, (int)rows, (int)cols, mode, d_first_valids, d_first_valids_hlc,
+                  d_clean);
+}
+
+// ------------------------------------------------------------------ fused EMA + RSI + ATR
+vta_status vta_ema_rsi_atr_many_series_device(vta_ctx *ctx, const float *d_close_tm,
+                                              const float *d_high_tm, const float *d_low_tm,
+                                              size_t cols, size_t rows, size_t ema_period,
+                                              size_t rsi_period, size_t atr_period,
+                                              const int32_t *d_fv_close,
+                                              const int32_t *d_fv_hlc, const int32_t *d_clean,
+                                              float *d_ema_tm, float *d_rsi_tm,
+                                              float *d_atr_tm) {
+    clear_error();
+    if (!ctx || !d_close_tm || !d_high_tm || !d_low_tm || !d_fv_close || !d_fv_hlc ||
+        !d_clean || !d_ema_tm || !d_rsi_tm || !d_atr_tm || cols == 0 || rows == 0 ||
+        ema_period == 0 || rsi_period == 0 || atr_period == 0)
+        return fail(VTA_ERR_INVALID_INPUT, "invalid arguments");
+    if (rows > (size_t)INT32_MAX || cols > (size_t)INT32_MAX)
+        return fail(VTA_ERR_INVALID_INPUT, "dimensions exceed int32");
+    rec_args_host a{};
+    a.x = d_close_tm;
+    a.h = d_high_tm;
+    a.l = d_low_tm;
+    a.fvs = d_fv_close;
+    a.fvs2 = d_fv_hlc;
+    a.clean = d_clean;
+    a.out0 = d_ema_tm;
+    a.out1 = d_rsi_tm;
+    a.out2 = d_atr_tm;
+    a.T = (int)rows;
+    a.nlanes = (int)cols;
+    a.p1 = (int)ema_period;
+    a.p2 = (int)rsi_period;
+    a.p3 = (int)atr_period;
+    unsigned groups = (unsigned)((cols + 63) / 64);
+    size_t kb = std::max({(size_t)burn_bars(IND_EMA, ema_period),
+                          (size_t)burn_bars(IND_RSI, rsi_period),
+                          (size_t)burn_bars(IND_ATR, atr_period)});
+    a.chunk = pick_chunk(IND_ATR, 1, rows, groups, true);
+    if (!getenv("VTA_REC_CHUNK")) {
+        size_t c = std::max<size_t>({4 * kb, 1024, (size_t)a.chunk});
+        a.chunk = (int)std::min(((c + 63) & ~(size_t)63), rows);
+    }
+    unsigned chunks = (unsigned)((rows + a.chunk - 1) / a.chunk);
+    return launch_rec(ctx, "ms_ema_rsi_atr_f32", a, groups, chunks);
+}
+
+vta_status vta_ema_rsi_atr_many_series_dev(vta_ctx *ctx, const float *close_tm,
+                                           const float *high_tm, const float *low_tm,
+                                           size_t cols, size_t rows, size_t ema_period,
+                                           size_t rsi_period, size_t atr_period,
+                                           vta_device_array_f32 *out3) {
+    clear_error();
+    if (!ctx || !out3 || !close_tm || !high_tm || !low_tm)
+        return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    if (cols == 0 || rows == 0) return fail(VTA_ERR_INVALID_INPUT, "cols or rows is zero");
+    size_t elems;
+    if (!mul_ok(cols, rows, &elems)) return fail(VTA_ERR_INVALID_INPUT, "cols*rows overflow");
+    VTA_TRY(check_fits(6 * elems * sizeof(float)));
+    DevBuf dc, dh, dl, dfv, dfv3, dcl, o0, o1, o2;
+    VTA_TRY(upload(ctx, dc, close_tm, elems * sizeof(float)));
+    VTA_TRY(upload(ctx, dh, high_tm, elems * sizeof(float)));
+    VTA_TRY(upload(ctx, dl, low_tm, elems * sizeof(float)));
+    VTA_TRY(dfv.alloc(cols * sizeof(int32_t)));
+    VTA_TRY(dfv3.alloc(cols * sizeof(int32_t)));
+    VTA_TRY(dcl.alloc(cols * sizeof(int32_t)));
+    VTA_TRY(vta_many_series_prepare(ctx, dc.as<float>(), dh.as<float>(), dl.as<float>(), cols,
+                                    rows, 3, dfv.as<int32_t>(), dfv3.as<int32_t>(),
+                                    dcl.as<int32_t>()));
+    // validation against the CPU rules needs the first_valids on the host
+    std::vector<int32_t> fvc(cols), fv3(cols);
+    VTA_HIP(hipMemcpyAsync(fvc.data(), dfv.p, cols * 4, hipMemcpyDeviceToHost, ctx->stream));
+    VTA_HIP(hipMemcpyAsync(fv3.data(), dfv3.p, cols * 4, hipMemcpyDeviceToHost, ctx->stream));
+    VTA_HIP(hipStreamSynchronize(ctx->stream));
+    size_t need = std::max({ema_period, rsi_period, atr_period});
+    for (size_t s = 0; s < cols; ++s) {
+        if ((size_t)fv3[s] >= rows || (size_t)fvc[s] >= rows)
+            return fail(VTA_ERR_INVALID_INPUT, "series %zu all NaN", s);
+        if (rows - (size_t)std::max(fvc[s], fv3[s]) < need)
+            return fail(VTA_ERR_INVALID_INPUT, "series %zu not enough valid data", s);
+    }
+    VTA_TRY(o0.alloc(elems * sizeof(float)));
+    VTA_TRY(o1.alloc(elems * sizeof(float)));
+    VTA_TRY(o2.alloc(elems * sizeof(float)));
+    VTA_TRY(vta_ema_rsi_atr_many_series_device(
+        ctx, dc.as<float>(), dh.as<float>(), dl.as<float>(), cols, rows, ema_period, rsi_period,
+        atr_period, dfv.as<int32_t>(), dfv3.as<int32_t>(), dcl.as<int32_t>(), o0.as<float>(),
+        o1.as<float>(), o2.as<float>()));
+    VTA_HIP(hipStreamSynchronize(ctx->stream));
+    DevBuf *outs[3] = {&o0, &o1, &o2};
+    for (int i = 0; i < 3; ++i) {
+        out3[i].ptr = static_cast<float *>(outs[i]->release());
+        out3[i].rows = rows;
+        out3[i].cols = cols;
+    }
+    return VTA_OK;
+}
+
+}  // extern "C"

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -147,6 +148,46 @@ vta_status upload(vta_ctx *ctx, DevBuf &buf, const void *src, size_t bytes) {
     return VTA_OK;
 }
 
+bool expand_usize(vta_range_usize r, AxisKind kind, std::vector<size_t> &v) {
+    v.clear();
+    if (r.step == 0 || r.start == r.end) {
+        v.push_back(r.start);
+        return true;
+    }
+    size_t lo = r.start < r.end ? r.start : r.end;
+    size_t hi = r.start < r.end ? r.end : r.start;
+    if (r.start < r.end || kind == AXIS_SORTED || kind == AXIS_ATR) {
+        for (size_t x = lo;; x += r.step) {
+            v.push_back(x);
+            if (x > hi - r.step || x + r.step > hi) break;
+        }
+        if (r.start > r.end && kind == AXIS_ATR) std::reverse(v.begin(), v.end());
+        return !v.empty();
+    }
+    // AXIS_DOWN with start > end
+    for (size_t x = r.start;; x -= r.step) {
+        v.push_back(x);
+        if (x < r.step || x - r.step < r.end) break;
+    }
+    return !v.empty();
+}
+
+vta_status host_first_valids(const float *tm, size_t cols, size_t rows, size_t need,
+                             std::vector<int32_t> &fvs) {
+    fvs.assign(cols, 0);
+    for (size_t s = 0; s < cols; ++s) {
+        size_t t = 0;
+        while (t < rows && std::isnan(tm[t * cols + s])) ++t;
+        if (t >= rows) return fail(VTA_ERR_INVALID_INPUT, "series %zu all NaN", s);
+        if (rows - t < need)
+            return fail(VTA_ERR_INVALID_INPUT,
+                        "series %zu not enough valid data (needed >= %zu, valid = %zu)", s, need,
+                        rows - t);
+        fvs[s] = (int32_t)t;
+    }
+    return VTA_OK;
+}
+
 size_t first_valid_f32(const float *x, size_t n) {
     size_t i = 0;
     while (i < n && std::isnan(x[i])) ++i;
@@ -210,6 +251,11 @@ vta_status vta_ctx_create(int device_id, vta_ctx **out) {
         delete ctx;
         return fail(VTA_ERR_HIP, "hipStreamCreateWithFlags failed");
     }
+    if (hipMalloc(&ctx->scratch, vta_ctx::kScratchBytes) != hipSuccess) {
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return fail(VTA_ERR_OUT_OF_MEMORY, "hipMalloc of the context scratch failed");
+    }
     const char *kd = getenv("VTA_KERNEL_DIR");
     ctx->kernel_dir = (kd && *kd) ? std::string(kd) : library_dir();
     *out = ctx;
@@ -221,6 +267,7 @@ vta_status vta_ctx_destroy(vta_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto &kv : ctx->modules) (void)hipModuleUnload(kv.second);
+    if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return VTA_OK;
@@ -242,6 +289,23 @@ vta_status vta_ctx_preload(vta_ctx *ctx) {
         {"alma", "alma_batch_f32_tm"},
         {"alma", "alma_ms1p_f32"},
         {"alma", "alma_batch_mfma_bf16x3"},
+        {"alma", "alma_batch_mfma_bf16x3_r8"},
+        {"alma", "alma_batch_mfma_bf16x3_r32"},
+        {"alma", "alma_batch_f32_direct"},
+        {"alma", "alma_ms1p_f32_direct"},
+        {"rec", "ema_batch_f32"},
+        {"rec", "rsi_batch_f32"},
+        {"rec", "atr_batch_f32"},
+        {"rec", "adx_batch_f32"},
+        {"rec", "ema_ms1p_f32"},
+        {"rec", "rsi_ms1p_f32"},
+        {"rec", "atr_ms1p_f32"},
+        {"rec", "adx_ms1p_f32"},
+        {"rec", "ms_ema_rsi_atr_f32"},
+        {"rec", "ms_prepare_meta"},
+        {"rec", "series_nonfinite_flag"},
+        {"rec", "flag_to_clean"},
+        {"rec", "vta_fill_i32"},
     };
     for (auto &s : kSymbols) {
         hipFunction_t f;

@@ -21,6 +21,10 @@ struct vta_ctx {
     // per-indicator policies and last selections (CudaAlmaPolicy etc.)
     vta_alma_batch_kernel alma_policy = VTA_ALMA_BATCH_AUTO;
     vta_alma_batch_kernel alma_selected = VTA_ALMA_BATCH_AUTO;
+    // small device buffer owned by the context (flags written by prep kernels
+    // of the enqueue-only entry points; reused in stream order)
+    void *scratch = nullptr;
+    static constexpr size_t kScratchBytes = 64 * 1024;
 };
 
 namespace vta {
@@ -92,6 +96,20 @@ struct DevBuf {
 vta_status upload(vta_ctx *ctx, DevBuf &buf, const void *src, size_t bytes);
 
 size_t first_valid_f32(const float *x, size_t n);  // n if all NaN
+
+// Parameter axis expansions used by the reference wrappers:
+//   AXIS_SORTED  start..=end ascending, either order of start/end (ema_wrapper.rs:862-873,
+//                rsi_wrapper.rs:389-413)
+//   AXIS_DOWN    ascending if start < end, else start, start-step, ... >= end
+//                (alma_wrapper.rs:1551-1575, adx_wrapper.rs:158-178)
+//   AXIS_ATR     ascending if start < end, else (end..=start step) reversed (atr_wrapper.rs:286-297)
+enum AxisKind { AXIS_SORTED, AXIS_DOWN, AXIS_ATR };
+bool expand_usize(vta_range_usize r, AxisKind kind, std::vector<size_t> &v);
+
+// Per-series first_valid of a time-major [rows][cols] host matrix; fails with
+// InvalidInput if a series is all NaN or has fewer than `need` valid bars.
+vta_status host_first_valids(const float *tm, size_t cols, size_t rows, size_t need,
+                             std::vector<int32_t> &fvs);
 
 // bf16 round-to-nearest-even of an f32 (bit pattern).
 uint16_t bf16_rne(float f);

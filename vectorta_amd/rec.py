@@ -1,0 +1,163 @@
+"""EMA / RSI / ATR / ADX operator surfaces: mirrors of CudaEma
+(src/cuda/moving_averages/ema_wrapper.rs), CudaRsi (src/cuda/rsi_wrapper.rs),
+CudaAtr (src/cuda/atr_wrapper.rs) and CudaAdx (src/cuda/adx_wrapper.rs), plus
+the fused EMA + RSI + ATR many-series launch."""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Sequence, Tuple
+
+import numpy as np
+
+from ._lib import DeviceArrayF32C, RangeUsize, check, lib
+from .device import DeviceArrayF32, DeviceBuffer, HipContext, f32_ptr
+
+
+def _rng(r: Sequence[int]) -> RangeUsize:
+    return RangeUsize(*[int(v) for v in r])
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _expand(fn, period: Sequence[int]) -> List[int]:
+    rc = _rng(period)
+    n = ctypes.c_size_t()
+    check(fn(ctypes.byref(rc), None, 0, ctypes.byref(n)))
+    arr = (ctypes.c_size_t * max(n.value, 1))()
+    check(fn(ctypes.byref(rc), arr, n.value, ctypes.byref(n)))
+    return [int(v) for v in arr[:n.value]]
+
+
+class _OneParam:
+    """Shared body of the one-parameter wrappers."""
+    _name = ""
+    _hlc = False
+
+    def __init__(self, device_id: int = 0):
+        self.ctx = HipContext(device_id)
+
+    def synchronize(self) -> None:
+        self.ctx.synchronize()
+
+    def expand_grid(self, period: Tuple[int, int, int]) -> List[int]:
+        return _expand(getattr(lib(), f"vta_{self._name}_expand_grid"), period)
+
+    def _batch_dev(self, arrays, period) -> DeviceArrayF32:
+        arrs = [_f32(a) for a in arrays]
+        n = arrs[0].size
+        rc = _rng(period)
+        out = DeviceArrayF32C()
+        fn = getattr(lib(), f"vta_{self._name}_batch_dev")
+        check(fn(self.ctx.handle, *[f32_ptr(a) for a in arrs], n, ctypes.byref(rc),
+                 ctypes.byref(out)))
+        return DeviceArrayF32(self.ctx, out)
+
+    def _ms_dev(self, arrays, cols: int, rows: int, period: int) -> DeviceArrayF32:
+        arrs = [_f32(a).reshape(-1) for a in arrays]
+        out = DeviceArrayF32C()
+        fn = getattr(lib(), f"vta_{self._name}_many_series_one_param_time_major_dev")
+        check(fn(self.ctx.handle, *[f32_ptr(a) for a in arrs], int(cols), int(rows), int(period),
+                 ctypes.byref(out)))
+        return DeviceArrayF32(self.ctx, out)
+
+
+class CudaEma(_OneParam):
+    _name = "ema"
+
+    def ema_batch_dev(self, prices_f32, period: Tuple[int, int, int]) -> DeviceArrayF32:
+        return self._batch_dev([prices_f32], period)
+
+    def ema_batch_device(self, d_prices: DeviceBuffer, d_periods: DeviceBuffer, max_period: int,
+                         series_len: int, first_valid: int, n_combos: int,
+                         d_out: DeviceBuffer) -> None:
+        check(lib().vta_ema_batch_device(self.ctx.handle, ctypes.c_void_p(d_prices.ptr),
+                                         ctypes.c_void_p(d_periods.ptr), None, int(max_period),
+                                         int(series_len), int(first_valid), int(n_combos),
+                                         ctypes.c_void_p(d_out.ptr)))
+
+    def ema_many_series_one_param_time_major_dev(self, data_tm, cols, rows, period):
+        return self._ms_dev([data_tm], cols, rows, period)
+
+    def ema_many_series_one_param_device(self, d_prices_tm, d_first_valids, period, num_series,
+                                         series_len, d_out_tm, d_clean=None) -> None:
+        check(lib().vta_ema_many_series_one_param_device(
+            self.ctx.handle, ctypes.c_void_p(d_prices_tm.ptr), ctypes.c_void_p(d_first_valids.ptr),
+            int(period), int(num_series), int(series_len),
+            ctypes.c_void_p(d_clean.ptr) if d_clean is not None else None,
+            ctypes.c_void_p(d_out_tm.ptr)))
+
+
+class CudaRsi(_OneParam):
+    _name = "rsi"
+
+    def rsi_batch_dev(self, prices_f32, period: Tuple[int, int, int]) -> DeviceArrayF32:
+        return self._batch_dev([prices_f32], period)
+
+    def rsi_many_series_one_param_time_major_dev(self, prices_tm, cols, rows, period):
+        return self._ms_dev([prices_tm], cols, rows, period)
+
+
+class CudaAtr(_OneParam):
+    _name = "atr"
+
+    def atr_batch_dev(self, high, low, close, length: Tuple[int, int, int]) -> DeviceArrayF32:
+        return self._batch_dev([high, low, close], length)
+
+    def atr_many_series_one_param_time_major_dev(self, high_tm, low_tm, close_tm, cols, rows,
+                                                 period):
+        return self._ms_dev([high_tm, low_tm, close_tm], cols, rows, period)
+
+
+class CudaAdx(_OneParam):
+    _name = "adx"
+
+    def adx_batch_dev(self, high, low, close, period: Tuple[int, int, int]) -> DeviceArrayF32:
+        return self._batch_dev([high, low, close], period)
+
+    def adx_many_series_one_param_time_major_dev(self, high_tm, low_tm, close_tm, cols, rows,
+                                                 period):
+        return self._ms_dev([high_tm, low_tm, close_tm], cols, rows, period)
+
+
+def ema_rsi_atr_many_series_dev(ctx: HipContext, close_tm, high_tm, low_tm, cols: int,
+                                rows: int, ema_period: int = 20, rsi_period: int = 14,
+                                atr_period: int = 14):
+    """Fused EMA + RSI + ATR over time-major series; returns (ema, rsi, atr) device arrays."""
+    c, h, lo = (_f32(a).reshape(-1) for a in (close_tm, high_tm, low_tm))
+    outs = (DeviceArrayF32C * 3)()
+    check(lib().vta_ema_rsi_atr_many_series_dev(ctx.handle, f32_ptr(c), f32_ptr(h), f32_ptr(lo),
+                                                int(cols), int(rows), int(ema_period),
+                                                int(rsi_period), int(atr_period), outs))
+    return tuple(DeviceArrayF32(ctx, outs[i]) for i in range(3))
+
+
+class ManySeriesOhlcPlan:
+    """Device-resident time-major close/high/low with prepared per-series
+    metadata, for repeated enqueue-only fused launches."""
+
+    def __init__(self, ctx: HipContext, close_tm, high_tm, low_tm):
+        c, h, lo = (_f32(a) for a in (close_tm, high_tm, low_tm))
+        self.rows, self.cols = c.shape
+        self.ctx = ctx
+        self.d_c, self.d_h, self.d_l = ctx.to_device(c), ctx.to_device(h), ctx.to_device(lo)
+        n = self.cols
+        self.d_fv, self.d_fv3, self.d_clean = ctx.alloc(4 * n), ctx.alloc(4 * n), ctx.alloc(4 * n)
+        check(lib().vta_many_series_prepare(
+            ctx.handle, ctypes.c_void_p(self.d_c.ptr), ctypes.c_void_p(self.d_h.ptr),
+            ctypes.c_void_p(self.d_l.ptr), n, self.rows, 3, ctypes.c_void_p(self.d_fv.ptr),
+            ctypes.c_void_p(self.d_fv3.ptr), ctypes.c_void_p(self.d_clean.ptr)))
+        ctx.synchronize()
+        nbytes = 4 * self.rows * self.cols
+        self.d_out = [ctx.alloc(nbytes) for _ in range(3)]
+
+    def launch(self, ema_period=20, rsi_period=14, atr_period=14) -> None:
+        check(lib().vta_ema_rsi_atr_many_series_device(
+            self.ctx.handle, ctypes.c_void_p(self.d_c.ptr), ctypes.c_void_p(self.d_h.ptr),
+            ctypes.c_void_p(self.d_l.ptr), self.cols, self.rows, int(ema_period), int(rsi_period),
+            int(atr_period), ctypes.c_void_p(self.d_fv.ptr), ctypes.c_void_p(self.d_fv3.ptr),
+            ctypes.c_void_p(self.d_clean.ptr), *[ctypes.c_void_p(b.ptr) for b in self.d_out]))
+
+    def outputs(self):
+        return [b.copy_to_host().reshape(self.rows, self.cols) for b in self.d_out]
