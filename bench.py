@@ -137,6 +137,61 @@ def calibrate_copy(nbytes: int = 256 << 20, reps: int = 3) -> None:
     dst.free()
 
 
+def bench_many_series_ohlc(steps, warmup, ev):
+    """Fused EMA(20) + RSI(14) + ATR(14) over 4,096 uneven tickers right-aligned
+    in a [50,000 x 4,096] time-major matrix (one enqueue-only launch per step).
+    Algorithmic bytes = the valid high/low/close bars read + all three output
+    matrices written."""
+    o, h, lo, c, lens = datagen.many_series_ohlc(4096, 50_000, seed=3)
+    rows, cols = c.shape
+    ctx = vta.HipContext(0)
+    plan = vta.ManySeriesOhlcPlan(ctx, c, h, lo)
+    del o, h, lo, c
+    for _ in range(warmup):
+        plan.launch(20, 14, 14)
+    ctx.synchronize()
+    ks = []
+    for _ in range(steps):
+        a, b = ev.create(), ev.create()
+        ev.record(a, ctx.stream)
+        plan.launch(20, 14, 14)
+        ev.record(b, ctx.stream)
+        ks.append(ev.elapsed_ms(a, b))
+        ev.destroy(a)
+        ev.destroy(b)
+    ks = np.array(ks)
+    alg = 3 * 4 * int(lens.sum()) + 3 * 4 * rows * cols
+    t = float(ks.mean()) * 1e-3
+    return {"workload": "ema20_rsi14_atr14_many_series_4096x50000", "kernel": "ms_ema_rsi_atr_f32",
+            "kernel_ms_mean": float(ks.mean()), "kernel_ms_p50": float(np.percentile(ks, 50)),
+            "kernel_ms_p99": float(np.percentile(ks, 99)), "outputs": 3 * rows * cols,
+            "outputs_per_s": 3 * rows * cols / t, "algorithmic_bytes": alg,
+            "achieved_GBs": alg / t / 1e9, "frac_of_hbm_peak": alg / t / 1e9 / HBM_PEAK_GBS}
+
+
+def bench_alma_single(steps, ev):
+    """Plain ALMA on one 100,000-bar series (period 9): launch-latency bound."""
+    x = datagen.alma_single_series()
+    alma = vta.CudaAlma(0, policy="fp32")
+    sweep = vta.AlmaBatchRange(period=(9, 9, 0), offset=(0.85, 0.85, 0.0), sigma=(6.0, 6.0, 0.0))
+    plan = alma.sweep_plan(sweep, x.size, kernel="fp32")
+    d_x = alma.ctx.to_device(x)
+    d_out = alma.ctx.alloc(x.size * 4)
+    for _ in range(3):
+        plan.launch(d_x.ptr, 0, d_out.ptr)
+    alma.synchronize()
+    ks = []
+    for _ in range(steps):
+        a, b = ev.create(), ev.create()
+        ev.record(a, alma.ctx.stream)
+        plan.launch(d_x.ptr, 0, d_out.ptr)
+        ev.record(b, alma.ctx.stream)
+        ks.append(ev.elapsed_ms(a, b) * 1e3)
+    ks = np.array(ks)
+    return {"workload": "alma_single_100k_p9", "kernel_us_p50": float(np.percentile(ks, 50)),
+            "kernel_us_p99": float(np.percentile(ks, 99))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,7 +200,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and error check")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
     ap.add_argument("--forms", default="mfma,fp32")
-    ap.add_argument("--mode", default="sweep", choices=["sweep", "calib"],
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the secondary workloads (many-series OHLC, single ALMA)")
+    ap.add_argument("--mode", default="sweep", choices=["sweep", "calib", "ohlc"],
                     help="calib: 256 MiB device-to-device copies, the known-size reference "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
@@ -156,6 +213,10 @@ def main():
 
     if args.mode == "calib":
         calibrate_copy()
+        return
+    if args.mode == "ohlc":
+        r = bench_many_series_ohlc(args.steps, args.warmup, HipEvents())
+        print(json.dumps(r))
         return
 
     x = datagen.alma_sweep_series()
@@ -241,6 +302,13 @@ def main():
         "cpu_baseline": cpu,
     }
     d_out.free()
+    if not args.no_extra:
+        extra = {}
+        for name, fn in (("many_series_ohlc", lambda: bench_many_series_ohlc(10, 2, ev)),
+                         ("alma_single", lambda: bench_alma_single(20, ev))):
+            extra[name] = fn()
+            print(f"[bench] {name}: {extra[name]}", file=sys.stderr, flush=True)
+        out["extra_workloads"] = extra
     print(json.dumps(out))
 
 

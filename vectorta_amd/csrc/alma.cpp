@@ -490,6 +490,9 @@ vta_status vta_alma_batch_device(vta_ctx *ctx, const float *d_prices, const floa
     if (series_len <= 0 || n_combos <= 0 || max_period <= 0)
         return fail(VTA_ERR_INVALID_INPUT,
                     "series_len, n_combos, and max_period must be positive");
+    if (first_valid >= series_len)
+        return fail(VTA_ERR_INVALID_INPUT, "first_valid %d >= series_len %d", first_valid,
+                    series_len);
     ctx->alma_selected = VTA_ALMA_BATCH_FP32;
     return launch_fp32(ctx, d_prices, d_weights, d_periods, d_inv_norms, max_period, series_len,
                        n_combos, std::max(first_valid, 0), d_out, false);
@@ -504,6 +507,9 @@ vta_status vta_alma_batch_device_tm(vta_ctx *ctx, const float *d_prices, const f
     if (series_len <= 0 || n_combos <= 0 || max_period <= 0)
         return fail(VTA_ERR_INVALID_INPUT,
                     "series_len, n_combos, and max_period must be positive");
+    if (first_valid >= series_len)
+        return fail(VTA_ERR_INVALID_INPUT, "first_valid %d >= series_len %d", first_valid,
+                    series_len);
     ctx->alma_selected = VTA_ALMA_BATCH_FP32;
     return launch_fp32(ctx, d_prices, d_weights, d_periods, d_inv_norms, max_period, series_len,
                        n_combos, std::max(first_valid, 0), d_out_tm, true);

@@ -46,9 +46,10 @@ extern "C" __global__ __launch_bounds__(256) void alma_batch_f32(
     int bad = 0;
     for (int e = tid; e < xs_len; e += 256) {
         long idx = (long)t0 - p4max + e;
+        const float x = vta_ld_clamped(prices, idx, first_valid, series_len);
+        const bool in = idx >= first_valid && idx < series_len;
         float v = 0.0f;
-        if (idx >= first_valid && idx < series_len) {
-            float x = prices[idx];
+        if (in) {
             if (vta_isfinite(x)) v = x; else bad = 1;
         }
         xs[e] = v;
@@ -353,7 +354,8 @@ __device__ __forceinline__ void alma_mf_wave(const alma_mf_args &a, int nb, int 
 #pragma unroll
         for (int i = 0; i < NXI; ++i) {
             const int idx = t_lo + lane + 64 * i;
-            v[i] = (idx >= fv && idx < T) ? a.prices[idx] : 0.0f;
+            const float x = vta_ld_clamped(a.prices, idx, fv, T);
+            v[i] = (idx >= fv && idx < T) ? x : 0.0f;
         }
 #pragma unroll
         for (int i = 0; i < NXI; ++i) xs[lane + 64 * i] = v[i];

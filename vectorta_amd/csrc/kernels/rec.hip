@@ -15,9 +15,10 @@
 // (64 streams, chunk).  A wave whose chunk starts far enough after the
 // warm-up begins K bars early with an approximate state and discards those
 // outputs ("burn-in").  All four recurrences contract the state error by
-// beta = 1 - 1/p (EMA: 1 - 2/(p+1)) per bar, so after K >= 40 / -ln(beta)
-// bars (ADX, two coupled stages: 48 / -ln(beta)) the error is below
-// e^-40 ~ 4e-18 of the state -- under FP64 rounding.  This holds only when
+// beta = 1 - 1/p (EMA: 1 - 2/(p+1)) per bar, so after K >= 25 / -ln(beta)
+// bars (ADX, two coupled stages: 32 / -ln(beta)) the error is below
+// e^-25 ~ 1.4e-11 of the state: four orders of magnitude under f32 rounding
+// of the output and six under the 1e-5 parity bound.  This holds only when
 // every bar after first_valid is finite (non-finite bars stop EMA decay and
 // poison ATR forever), so each stream carries a "clean" flag; an unclean
 // stream always starts exactly at first_valid.
@@ -62,7 +63,7 @@ struct EmaEng {
         alpha = 2.0 / ((double)p + 1.0);
         beta = 1.0 - alpha;
     }
-    __device__ int kb() const { return burn_bars(beta, 40.0); }
+    __device__ int kb() const { return burn_bars(beta, 25.0); }
     __device__ int exact_start() const { return fv; }
     __device__ int burn_ok_from() const { return fv + p; }   // b >= this may burn in
     // exact start at bar fv
@@ -94,7 +95,7 @@ struct RsiEng {
         beta = 1.0 - inv_p;
         poisoned = false;
     }
-    __device__ int kb() const { return burn_bars(beta, 40.0); }
+    __device__ int kb() const { return burn_bars(beta, 25.0); }
     __device__ int exact_start() const { return fv; }
     __device__ int burn_ok_from() const { return fv + p + 1; }
     __device__ double begin_exact(double x) { px = x; ag = 0.0; al = 0.0; return REC_NAN; }
@@ -146,7 +147,7 @@ struct AtrEng {
         p = p_; fv = fv_;
         alpha = 1.0 / (double)p;
     }
-    __device__ int kb() const { return burn_bars(1.0 - alpha, 40.0); }
+    __device__ int kb() const { return burn_bars(1.0 - alpha, 25.0); }
     __device__ int exact_start() const { return fv; }
     __device__ int burn_ok_from() const { return fv + p; }
     __device__ double begin_exact(double h, double l, double c) {
@@ -177,7 +178,7 @@ struct AdxEng {
         omr = 1.0 - rp;
         pm1 = pf - 1.0;
     }
-    __device__ int kb() const { return burn_bars(omr, 48.0); }
+    __device__ int kb() const { return burn_bars(omr, 32.0); }
     __device__ int exact_start() const { return fv; }
     __device__ int burn_ok_from() const { return fv + 2 * p; }
     __device__ double begin_exact(double h, double l, double c) {
@@ -244,9 +245,11 @@ __device__ __forceinline__ int lane_start(const E &e, int c0, bool clean, bool *
     return es;
 }
 
+// Wave-wide minimum, returned as a scalar (readfirstlane) so loops bounded by
+// it keep their induction variables and row addresses in SGPRs.
 __device__ __forceinline__ int wave_min(int v) {
     for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
-    return v;
+    return __builtin_amdgcn_readfirstlane(v);
 }
 
 // ---------------------------------------------------------------- many series (time-major)
@@ -271,11 +274,11 @@ __device__ __forceinline__ void drive_ms(const rec_args &a, int p, float *out) {
     const int t_begin = wave_min(min(start, c0));
     for (int t = t_begin; t < c1; ++t) {
         const size_t off = (size_t)t * N + s;
-        double x = 0.0, hh = 0.0, ll = 0.0;
-        if (active && t >= min(start, c0)) {
-            x = (double)a.x[off];
-            if (NIN == 3) { hh = (double)a.h[off]; ll = (double)a.l[off]; }
-        }
+        const long loff = (long)min(max(t, active ? min(start, c0) : a.T - 1), a.T - 1) * N +
+                          min(s, N - 1);
+        const double x = (double)a.x[loff];
+        double hh = 0.0, ll = 0.0;
+        if (NIN == 3) { hh = (double)a.h[loff]; ll = (double)a.l[loff]; }
         double y = REC_NAN;
         if (active) {
             if (t == start) {
@@ -313,8 +316,8 @@ __device__ __forceinline__ void drive_ms_rsi(const rec_args &a, int p, float *ou
     const int t_begin = wave_min(min(start, c0));
     for (int t = t_begin; t < c1; ++t) {
         const size_t off = (size_t)t * N + s;
-        double x = 0.0;
-        if (active && t >= min(start, c0)) x = (double)a.x[off];
+        const double x = (double)a.x[(long)min(max(t, active ? min(start, c0) : a.T - 1),
+                                               a.T - 1) * N + min(s, N - 1)];
         double y = REC_NAN;
         if (active) {
             if (t == start) {
@@ -343,9 +346,28 @@ extern "C" __global__ __launch_bounds__(64) void adx_ms1p_f32(rec_args a) {
 // ---------------------------------------------------------------- fused EMA + RSI + ATR
 //
 // One pass over close/high/low [T][N]: EMA(p1) and RSI(p2) of close and
-// ATR(p3) of (high, low, close), three time-major outputs.  Loads of the
-// next U rows are issued before the U rows in registers are consumed.
-#define FUSED_U 8
+// ATR(p3) of (high, low, close), three time-major outputs.
+//
+// Every lane has a bar `steady` from which its three engines are in pure
+// recurrence.  Each bar, lanes at or past it run the lean update (three
+// recurrences, no phase checks, the RSI ratio divided in f32 via rcp + one
+// Newton step: <= ~2 ulp of f32, far inside the 1e-5 bound); lanes still in a
+// warm-up run the general engines.  When every lane of the wave is steady the
+// general path is skipped entirely, so a series start costs its own lane ~p
+// general bars.  Loads of the next U rows are issued before the current U
+// rows are consumed (register double buffering); addresses are clamped so
+// the loads are unconditional and stay in flight.
+#define FUSED_U 16
+
+__device__ __forceinline__ float rsi_ratio_f32(double ag, double al) {
+    const double den = ag + al;
+    if (den == 0.0) return 50.0f;
+    const float d = (float)den;
+    float r = __builtin_amdgcn_rcpf(d);
+    r = fmaf(fmaf(-d, r, 1.0f), r, r);
+    return (float)(100.0 * ag) * r;
+}
+
 extern "C" __global__ __launch_bounds__(64) void ms_ema_rsi_atr_f32(rec_args a) {
     const int lane = threadIdx.x & 63;
     const int s = blockIdx.x * 64 + lane;
@@ -356,49 +378,119 @@ extern "C" __global__ __launch_bounds__(64) void ms_ema_rsi_atr_f32(rec_args a) 
     const int fv = active ? a.fvs[s] : a.T;          // first non-NaN close
     const int fv3 = active ? a.fvs2[s] : a.T;        // first bar with high, low, close
     const bool clean = active ? (a.clean != nullptr && a.clean[s] != 0) : true;
+    const bool live = active && fv < c1;             // has an output in this chunk
     EmaEng ee; RsiEng re; AtrEng ae;
     ee.setup(a.p1, fv); re.setup(a.p2, fv); ae.setup(a.p3, fv3);
     bool be = false, br = false, ba = false;
-    int se = active ? lane_start(ee, c0, clean, &be) : a.T;
-    int sr = active ? lane_start(re, c0, clean, &br) : a.T;
-    int sa = active ? lane_start(ae, c0, clean, &ba) : a.T;
+    int se = live ? lane_start(ee, c0, clean, &be) : a.T;
+    int sr = live ? lane_start(re, c0, clean, &br) : a.T;
+    int sa = live ? lane_start(ae, c0, clean, &ba) : a.T;
     if (br) {
         re.px = (double)a.x[(size_t)fv * N + s];
         for (int t = fv + 1; t <= min(fv + a.p2, a.T - 1); ++t)
             re.seed_scan(t, (double)a.x[(size_t)t * N + s]);
     }
-    const int smin = min(min(se, sr), min(sa, c0));
+    int steady = a.T;
+    if (live && clean) {
+        steady = max(be ? se + 1 : fv + a.p1, br ? sr + 1 : fv + a.p2 + 1);
+        steady = max(steady, ba ? sa + 1 : fv3 + a.p3);
+    }
+    const int smin = live ? min(min(se, sr), min(sa, c0)) : c0;
     const int t_begin = wave_min(smin);
+    // bars before a lane's first engine start need no work at all
+    const int lane_first = live ? min(min(se, sr), sa) : a.T;
+    const bool rsi_nan_fixed = re.poisoned;
+    const int sc = min(s, N - 1);
+    // Row addresses are wave-uniform (scalar base + 32-bit lane offset).
+
+    float bc[FUSED_U], bh[FUSED_U], bl[FUSED_U];
+    float nc[FUSED_U], nh[FUSED_U], nl[FUSED_U];
+#pragma unroll
+    for (int u = 0; u < FUSED_U; ++u) {
+        const size_t rb = (size_t)min(t_begin + u, a.T - 1) * (size_t)N;
+        bc[u] = (a.x + rb)[sc];
+        bh[u] = (a.h + rb)[sc];
+        bl[u] = (a.l + rb)[sc];
+    }
     for (int t0 = t_begin; t0 < c1; t0 += FUSED_U) {
-        float xc[FUSED_U], xh[FUSED_U], xl[FUSED_U];
 #pragma unroll
         for (int u = 0; u < FUSED_U; ++u) {
-            const int t = t0 + u;
-            const size_t off = (size_t)t * N + s;
-            const bool ld = active && t < c1 && t >= smin;
-            xc[u] = ld ? a.x[off] : 0.0f;
-            xh[u] = ld ? a.h[off] : 0.0f;
-            xl[u] = ld ? a.l[off] : 0.0f;
+            const size_t rb = (size_t)min(t0 + FUSED_U + u, a.T - 1) * (size_t)N;
+            nc[u] = (a.x + rb)[sc];
+            nh[u] = (a.h + rb)[sc];
+            nl[u] = (a.l + rb)[sc];
         }
+        // wave-uniform: does any lane have warm-up bars inside this block?
+        const bool mixed = __any(live && lane_first < t0 + FUSED_U && steady > t0);
+        if (!mixed) {
+            // every live lane is steady for the whole block, or has not started
+            // (its state is rebuilt when it starts, so updating it is harmless)
+#pragma unroll
+            for (int u = 0; u < FUSED_U; ++u) {
+                const int t = t0 + u;
+                const double x = bc[u], hh = bh[u], ll = bl[u];
+                ee.prev = fma(ee.beta, ee.prev, ee.alpha * x);          // ema.rs:462
+                const double d = x - re.px;
+                re.px = x;
+                re.ag = fma(re.ag, re.beta, re.inv_p * (d > 0.0 ? d : 0.0));  // rsi.rs:377
+                re.al = fma(re.al, re.beta, re.inv_p * (d < 0.0 ? -d : 0.0));
+                const double tr = true_range(hh, ll, ae.pc);
+                ae.pc = x;
+                ae.rma = fma(-ae.alpha, ae.rma, ae.rma) + ae.alpha * tr;      // atr.rs:393
+                const bool ok = t >= steady;
+                const float o0 = ok ? (float)ee.prev : VTA_QNAN_F32;
+                const float o1 = (ok && !rsi_nan_fixed) ? rsi_ratio_f32(re.ag, re.al)
+                                                        : VTA_QNAN_F32;
+                const float o2 = ok ? (float)ae.rma : VTA_QNAN_F32;
+                if (t >= c0 && t < c1 && active) {
+                    const size_t rb = (size_t)t * (size_t)N;
+                    (a.out0 + rb)[s] = o0;
+                    (a.out1 + rb)[s] = o1;
+                    (a.out2 + rb)[s] = o2;
+                }
+            }
+        } else {
 #pragma unroll
         for (int u = 0; u < FUSED_U; ++u) {
             const int t = t0 + u;
-            if (!active || t >= c1) break;
-            const double x = xc[u], hh = xh[u], ll = xl[u];
-            double y0 = REC_NAN, y1 = REC_NAN, y2 = REC_NAN;
-            if (t == se) { if (be) ee.begin_burn(x); else y0 = ee.begin_exact(x); }
-            else if (t > se) y0 = ee.step(t, x);
-            if (t == sr) { if (br) re.begin_burn(x); else y1 = re.begin_exact(x); }
-            else if (t > sr) y1 = re.step(t, x);
-            if (t == sa) { if (ba) ae.begin_burn(hh, ll, x); else y2 = ae.begin_exact(hh, ll, x); }
-            else if (t > sa) y2 = ae.step(t, hh, ll, x);
-            if (t >= c0) {
+            if (t >= c1) break;
+            const double x = bc[u], hh = bh[u], ll = bl[u];
+            float o0 = VTA_QNAN_F32, o1 = VTA_QNAN_F32, o2 = VTA_QNAN_F32;
+            if (t >= steady) {
+                // lean steady-state update (clean, started lanes only)
+                ee.prev = fma(ee.beta, ee.prev, ee.alpha * x);          // ema.rs:462
+                const double d = x - re.px;
+                re.px = x;
+                re.ag = fma(re.ag, re.beta, re.inv_p * (d > 0.0 ? d : 0.0));  // rsi.rs:377
+                re.al = fma(re.al, re.beta, re.inv_p * (d < 0.0 ? -d : 0.0));
+                const double tr = true_range(hh, ll, ae.pc);
+                ae.pc = x;
+                ae.rma = fma(-ae.alpha, ae.rma, ae.rma) + ae.alpha * tr;      // atr.rs:393
+                o0 = (float)ee.prev;
+                o1 = rsi_nan_fixed ? VTA_QNAN_F32 : rsi_ratio_f32(re.ag, re.al);
+                o2 = (float)ae.rma;
+            } else if (t >= lane_first) {
+                double y0 = REC_NAN, y1 = REC_NAN, y2 = REC_NAN;
+                if (t == se) { if (be) ee.begin_burn(x); else y0 = ee.begin_exact(x); }
+                else if (t > se) y0 = ee.step(t, x);
+                if (t == sr) { if (br) re.begin_burn(x); else y1 = re.begin_exact(x); }
+                else if (t > sr) y1 = re.step(t, x);
+                if (t == sa) { if (ba) ae.begin_burn(hh, ll, x); else y2 = ae.begin_exact(hh, ll, x); }
+                else if (t > sa) y2 = ae.step(t, hh, ll, x);
+                o0 = (float)y0;
+                o1 = (float)y1;
+                o2 = (float)y2;
+            }
+            if (active && t >= c0) {
                 const size_t off = (size_t)t * N + s;
-                a.out0[off] = (float)y0;
-                a.out1[off] = (float)y1;
-                a.out2[off] = (float)y2;
+                a.out0[off] = o0;
+                a.out1[off] = o1;
+                a.out2[off] = o2;
             }
         }
+        }
+#pragma unroll
+        for (int u = 0; u < FUSED_U; ++u) { bc[u] = nc[u]; bh[u] = nh[u]; bl[u] = nl[u]; }
     }
 }
 
@@ -435,12 +527,11 @@ __device__ __forceinline__ void drive_batch(const rec_args &a, float *out) {
     const int t_begin = wave_min(min(start, c0)) & ~63;
     for (int tb = t_begin; tb < c1; tb += 64) {
         {
-            const int t = tb + lane;
-            const bool ok = t < T;
-            xs[0][lane] = ok ? a.x[t] : 0.0f;
+            const long t = min(tb + lane, T - 1);
+            xs[0][lane] = a.x[t];
             if (NIN == 3) {
-                xs[1][lane] = ok ? a.h[t] : 0.0f;
-                xs[2][lane] = ok ? a.l[t] : 0.0f;
+                xs[1][lane] = a.h[t];
+                xs[2][lane] = a.l[t];
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

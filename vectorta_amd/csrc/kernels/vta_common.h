@@ -20,3 +20,13 @@ __device__ __forceinline__ uint16_t vta_bf16_rne(float x) {
 __device__ __forceinline__ float vta_bf16_to_f32(uint16_t h) {
     return __builtin_bit_cast(float, (uint32_t)h << 16);
 }
+
+// Unconditional load from a clamped (always in-bounds) index.  Guarding a
+// load with a branch makes the compiler wait for it inside the branch, which
+// serialises prefetch batches; callers load unconditionally and apply their
+// predicate to the value.  Requires hi > lo.
+__device__ __forceinline__ float vta_ld_clamped(const float *__restrict__ p, long idx, long lo,
+                                                long hi) {
+    const long j = idx < lo ? lo : (idx >= hi ? hi - 1 : idx);
+    return p[j];
+}
