@@ -169,6 +169,44 @@ def bench_many_series_ohlc(steps, warmup, ev):
             "achieved_GBs": alg / t / 1e9, "frac_of_hbm_peak": alg / t / 1e9 / HBM_PEAK_GBS}
 
 
+def bench_rolling(steps, warmup, ev):
+    """Bollinger Bands (20, 2.0), percentile_nearest_rank (100, 70) and rolling
+    MAD (64) over 1,024 random-walk series x 50,000 bars with 1% +/-5-sigma
+    jumps, time-major, device resident; one enqueue-only launch per step.
+    Algorithmic bytes = every bar read once + every output written."""
+    x = datagen.many_series_with_jumps(1024, 50_000, seed=4)
+    rows, cols = x.shape
+    ctx = vta.HipContext(0)
+    plan = vta.RollingManySeriesPlan(ctx, x)
+    del x
+    res = {"workload": "bb20_pnr100p70_mad64_many_series_1024x50000"}
+    for name, fn, nout, kern in (("bb", lambda: plan.launch_bb(20, 2.0, 2.0), 3, "bb_ms1p_f32"),
+                                 ("pnr", lambda: plan.launch_pnr(100, 70.0), 1, "pnr_ms1p_f32"),
+                                 ("mad", lambda: plan.launch_mad(64), 1, "mad_ms1p_f32")):
+        for _ in range(warmup):
+            fn()
+        ctx.synchronize()
+        ks = []
+        for _ in range(steps):
+            a, b = ev.create(), ev.create()
+            ev.record(a, ctx.stream)
+            fn()
+            ev.record(b, ctx.stream)
+            ks.append(ev.elapsed_ms(a, b))
+            ev.destroy(a)
+            ev.destroy(b)
+        ks = np.array(ks)
+        t = float(ks.mean()) * 1e-3
+        alg = 4 * rows * cols * (1 + nout)
+        res[name] = {"kernel": kern, "kernel_ms_mean": float(ks.mean()),
+                     "kernel_ms_p50": float(np.percentile(ks, 50)),
+                     "kernel_ms_p99": float(np.percentile(ks, 99)),
+                     "series_bars_per_s": rows * cols / t, "algorithmic_bytes": alg,
+                     "achieved_GBs": alg / t / 1e9,
+                     "frac_of_hbm_peak": alg / t / 1e9 / HBM_PEAK_GBS}
+    return res
+
+
 def bench_alma_single(steps, ev):
     """Plain ALMA on one 100,000-bar series (period 9): launch-latency bound."""
     x = datagen.alma_single_series()
@@ -201,8 +239,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
     ap.add_argument("--forms", default="mfma,fp32")
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the secondary workloads (many-series OHLC, single ALMA)")
-    ap.add_argument("--mode", default="sweep", choices=["sweep", "calib", "ohlc"],
+                    help="skip the secondary workloads (many-series OHLC, rolling windows, "
+                         "single ALMA)")
+    ap.add_argument("--mode", default="sweep", choices=["sweep", "calib", "ohlc", "rolling"],
                     help="calib: 256 MiB device-to-device copies, the known-size reference "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
@@ -216,6 +255,10 @@ def main():
         return
     if args.mode == "ohlc":
         r = bench_many_series_ohlc(args.steps, args.warmup, HipEvents())
+        print(json.dumps(r))
+        return
+    if args.mode == "rolling":
+        r = bench_rolling(args.steps, args.warmup, HipEvents())
         print(json.dumps(r))
         return
 
@@ -305,6 +348,7 @@ def main():
     if not args.no_extra:
         extra = {}
         for name, fn in (("many_series_ohlc", lambda: bench_many_series_ohlc(10, 2, ev)),
+                         ("rolling_windows", lambda: bench_rolling(5, 1, ev)),
                          ("alma_single", lambda: bench_alma_single(20, ev))):
             extra[name] = fn()
             print(f"[bench] {name}: {extra[name]}", file=sys.stderr, flush=True)

@@ -283,6 +283,91 @@ vta_status vta_ema_rsi_atr_many_series_dev(vta_ctx *ctx, const float *close_tm,
                                            size_t rsi_period, size_t atr_period,
                                            vta_device_array_f32 *out3);
 
+/* ------------------------------------------------------------------ rolling windows */
+
+/* Bollinger Bands, SMA middle band and population standard deviation: the
+ * combinations the reference's GPU path accepts (ma_type "sma", devtype 0,
+ * bollinger_bands_wrapper.rs:284-350); the grid is period x devup x devdn in
+ * that nesting order (bollinger_bands_wrapper.rs:179-281).  devup/devdn stay
+ * f64 here (the reference narrows them to f32, :330-334). */
+typedef struct vta_bb_params {
+    size_t period;
+    double devup;
+    double devdn;
+} vta_bb_params;
+typedef struct vta_bb_batch_range {
+    vta_range_usize period;
+    vta_range_f64 devup;
+    vta_range_f64 devdn;
+} vta_bb_batch_range;
+
+vta_status vta_bb_expand_grid(const vta_bb_batch_range *range, vta_bb_params *out, size_t cap,
+                              size_t *n_out);
+/* bollinger_bands_wrapper.rs:567 (bollinger_bands_batch_dev): out3 receives
+ * {upper, middle, lower}, each [combo][len]. */
+vta_status vta_bollinger_bands_batch_dev(vta_ctx *ctx, const float *prices, size_t len,
+                                         const vta_bb_batch_range *range,
+                                         vta_device_array_f32 *out3);
+/* bollinger_bands_wrapper.rs:605
+ * (bollinger_bands_many_series_one_param_time_major_dev): out3 receives
+ * {upper, middle, lower}, each time-major [rows][cols]. */
+vta_status vta_bollinger_bands_many_series_one_param_time_major_dev(
+    vta_ctx *ctx, const float *data_tm, size_t cols, size_t rows, size_t period, double devup,
+    double devdn, vta_device_array_f32 *out3);
+/* Enqueue-only device form (no reference counterpart; the companion of the
+ * *_device entry points of the other wrappers).  d_first_nonfinite[s] is the
+ * first bar at or after first_valid whose value is NaN or +-Inf (rows if
+ * none); NULL runs each series as one exact sequential pass. */
+vta_status vta_bollinger_bands_many_series_one_param_device(
+    vta_ctx *ctx, const float *d_data_tm, const int32_t *d_first_valids,
+    const int32_t *d_first_nonfinite, size_t cols, size_t rows, size_t period, double devup,
+    double devdn, float *d_upper_tm, float *d_middle_tm, float *d_lower_tm);
+
+/* percentile_nearest_rank: grid length x percentage
+ * (percentile_nearest_rank_wrapper.rs:200-290). */
+typedef struct vta_pnr_params {
+    size_t length;
+    double percentage;
+} vta_pnr_params;
+typedef struct vta_pnr_batch_range {
+    vta_range_usize length;
+    vta_range_f64 percentage;
+} vta_pnr_batch_range;
+
+vta_status vta_pnr_expand_grid(const vta_pnr_batch_range *range, vta_pnr_params *out,
+                               size_t cap, size_t *n_out);
+/* percentile_nearest_rank_wrapper.rs:365 (pnr_batch_dev), :623
+ * (pnr_many_series_one_param_time_major_dev). */
+vta_status vta_pnr_batch_dev(vta_ctx *ctx, const float *prices, size_t len,
+                             const vta_pnr_batch_range *range, vta_device_array_f32 *out);
+vta_status vta_pnr_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *prices_tm,
+                                                        size_t cols, size_t rows, size_t length,
+                                                        double percentage,
+                                                        vta_device_array_f32 *out);
+/* Enqueue-only device form (no reference counterpart): per-series
+ * d_first_valids must leave each series at least `length` bars. */
+vta_status vta_pnr_many_series_one_param_device(vta_ctx *ctx, const float *d_data_tm,
+                                                const int32_t *d_first_valids, size_t cols,
+                                                size_t rows, size_t length, double percentage,
+                                                float *d_out_tm);
+
+/* Rolling median absolute deviation: medium_ad_wrapper.rs:120 (expand_grid),
+ * :366 (medium_ad_batch_dev), :470
+ * (medium_ad_many_series_one_param_time_major_dev). */
+vta_status vta_medium_ad_expand_grid(const vta_range_usize *range, size_t *periods, size_t cap,
+                                     size_t *n_out);
+vta_status vta_medium_ad_batch_dev(vta_ctx *ctx, const float *prices, size_t len,
+                                   const vta_range_usize *range, vta_device_array_f32 *out);
+vta_status vta_medium_ad_many_series_one_param_time_major_dev(vta_ctx *ctx,
+                                                              const float *data_tm, size_t cols,
+                                                              size_t rows, size_t period,
+                                                              vta_device_array_f32 *out);
+/* Enqueue-only device form (no reference counterpart). */
+vta_status vta_medium_ad_many_series_one_param_device(vta_ctx *ctx, const float *d_data_tm,
+                                                      const int32_t *d_first_valids, size_t cols,
+                                                      size_t rows, size_t period,
+                                                      float *d_out_tm);
+
 #ifdef __cplusplus
 }
 #endif

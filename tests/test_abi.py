@@ -103,3 +103,20 @@ def test_alma_expand_grid_matches_reference_semantics():
 def test_alma_invalid_policy_rejected_without_device():
     with pytest.raises(KeyError):
         vta.alma._POLICY["bogus"]
+
+
+def test_rolling_expand_grids_match_reference_semantics():
+    # bollinger_bands_wrapper.rs:179-281: period x devup x devdn, descending usize
+    bb = vta.CudaBollingerBands.expand_grid(
+        vta.BollingerBandsBatchRange(period=(30, 10, 10), devup=(1.0, 2.0, 0.5),
+                                     devdn=(2.0, 2.0, 0.0)))
+    assert [c[0] for c in bb] == [30] * 3 + [20] * 3 + [10] * 3
+    assert [c[1] for c in bb[:3]] == [1.0, 1.5, 2.0]
+    assert all(c[2] == 2.0 for c in bb)
+    # percentile_nearest_rank_wrapper.rs:200-260: descending f64 axis
+    pnr = vta.CudaPercentileNearestRank.expand_grid(
+        vta.PercentileNearestRankBatchRange(length=(10, 20, 5), percentage=(90.0, 70.0, 10.0)))
+    assert pnr == [(l, p) for l in (10, 15, 20) for p in (90.0, 80.0, 70.0)]
+    # medium_ad_wrapper.rs:120-150
+    assert vta.CudaMediumAd.expand_grid((64, 60, 2)) == [64, 62, 60]
+    assert vta.CudaMediumAd.expand_grid((5, 9, 0)) == [5]

@@ -10,10 +10,15 @@ from .alma import AlmaBatchRange, AlmaParams, AlmaSweepPlan, CudaAlma, HipAlma, 
 from .device import DeviceArrayF32, DeviceBuffer, HipContext, cuda_available, cuda_device_count
 from .rec import (CudaAdx, CudaAtr, CudaEma, CudaRsi, ManySeriesOhlcPlan,
                   ema_rsi_atr_many_series_dev)
+from .rolling import (BollingerBandsBatchRange, CudaBollingerBands, CudaMediumAd,
+                      CudaPercentileNearestRank, PercentileNearestRankBatchRange,
+                      RollingManySeriesPlan)
 
 __all__ = [
     "VtaError", "lib", "AlmaBatchRange", "AlmaParams", "AlmaSweepPlan", "CudaAlma", "HipAlma",
     "expand_grid", "DeviceArrayF32", "DeviceBuffer", "HipContext", "cuda_available",
     "cuda_device_count", "CudaEma", "CudaRsi", "CudaAtr", "CudaAdx", "ManySeriesOhlcPlan",
-    "ema_rsi_atr_many_series_dev",
+    "ema_rsi_atr_many_series_dev", "BollingerBandsBatchRange", "CudaBollingerBands",
+    "CudaMediumAd", "CudaPercentileNearestRank", "PercentileNearestRankBatchRange",
+    "RollingManySeriesPlan",
 ]

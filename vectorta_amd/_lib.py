@@ -49,6 +49,23 @@ class AlmaBatchRangeC(ctypes.Structure):
     _fields_ = [("period", RangeUsize), ("offset", RangeF64), ("sigma", RangeF64)]
 
 
+class BbParamsC(ctypes.Structure):
+    _fields_ = [("period", ctypes.c_size_t), ("devup", ctypes.c_double),
+                ("devdn", ctypes.c_double)]
+
+
+class BbBatchRangeC(ctypes.Structure):
+    _fields_ = [("period", RangeUsize), ("devup", RangeF64), ("devdn", RangeF64)]
+
+
+class PnrParamsC(ctypes.Structure):
+    _fields_ = [("length", ctypes.c_size_t), ("percentage", ctypes.c_double)]
+
+
+class PnrBatchRangeC(ctypes.Structure):
+    _fields_ = [("length", RangeUsize), ("percentage", RangeF64)]
+
+
 # Every symbol include/vta.h declares, with (restype, argtypes).
 _P = ctypes.c_void_p
 _SZ = ctypes.c_size_t
@@ -126,6 +143,32 @@ SIGNATURES = {
                                                  _P, _P, _P, _P, _P]),
     "vta_ema_rsi_atr_many_series_dev": (_ST, [_P, _P, _P, _P, _SZ, _SZ, _SZ, _SZ, _SZ,
                                               ctypes.POINTER(DeviceArrayF32C)]),
+    # rolling windows
+    "vta_bb_expand_grid": (_ST, [ctypes.POINTER(BbBatchRangeC), ctypes.POINTER(BbParamsC), _SZ,
+                                 ctypes.POINTER(_SZ)]),
+    "vta_bollinger_bands_batch_dev": (_ST, [_P, _P, _SZ, ctypes.POINTER(BbBatchRangeC),
+                                            ctypes.POINTER(DeviceArrayF32C)]),
+    "vta_bollinger_bands_many_series_one_param_time_major_dev": (
+        _ST, [_P, _P, _SZ, _SZ, _SZ, ctypes.c_double, ctypes.c_double,
+              ctypes.POINTER(DeviceArrayF32C)]),
+    "vta_bollinger_bands_many_series_one_param_device": (
+        _ST, [_P, _P, _P, _P, _SZ, _SZ, _SZ, ctypes.c_double, ctypes.c_double, _P, _P, _P]),
+    "vta_pnr_expand_grid": (_ST, [ctypes.POINTER(PnrBatchRangeC), ctypes.POINTER(PnrParamsC),
+                                  _SZ, ctypes.POINTER(_SZ)]),
+    "vta_pnr_batch_dev": (_ST, [_P, _P, _SZ, ctypes.POINTER(PnrBatchRangeC),
+                                ctypes.POINTER(DeviceArrayF32C)]),
+    "vta_pnr_many_series_one_param_time_major_dev": (_ST, [_P, _P, _SZ, _SZ, _SZ,
+                                                           ctypes.c_double,
+                                                           ctypes.POINTER(DeviceArrayF32C)]),
+    "vta_pnr_many_series_one_param_device": (_ST, [_P, _P, _P, _SZ, _SZ, _SZ, ctypes.c_double,
+                                                   _P]),
+    "vta_medium_ad_many_series_one_param_device": (_ST, [_P, _P, _P, _SZ, _SZ, _SZ, _P]),
+    "vta_medium_ad_expand_grid": (_ST, [ctypes.POINTER(RangeUsize), _P, _SZ,
+                                        ctypes.POINTER(_SZ)]),
+    "vta_medium_ad_batch_dev": (_ST, [_P, _P, _SZ, ctypes.POINTER(RangeUsize),
+                                      ctypes.POINTER(DeviceArrayF32C)]),
+    "vta_medium_ad_many_series_one_param_time_major_dev": (_ST, [_P, _P, _SZ, _SZ, _SZ,
+                                                                 ctypes.POINTER(DeviceArrayF32C)]),
 }
 
 _lib = None

@@ -1,0 +1,392 @@
+// Rolling-window indicators for gfx950: Bollinger Bands (SMA, population sd),
+// percentile_nearest_rank and the rolling median absolute deviation.
+//
+// Semantics (CPU oracle):
+//   BB   src/indicators/bollinger_bands.rs:566-672 (running f64 sums; a NaN
+//        after first_valid poisons every later output)
+//   PNR  src/indicators/percentile_nearest_rank.rs:190-266 (sorted window of the
+//        non-NaN values, k = round(p/100 * wl) - 1 clamped)
+//   MAD  src/indicators/medium_ad.rs:293-402 (median, then median of |x - med|;
+//        even windows average the two middle order statistics; any NaN -> NaN)
+//
+// Time is cut into chunks; every chunk rebuilds its window state exactly from
+// the W bars before it (windowed indicators need no burn-in approximation).
+// BB re-anchors its f64 sums at every chunk start; its "poison forever" rule
+// is reproduced from the series' first non-finite bar (see bb_run).
+//
+// PNR and MAD keep each lane's window sorted in LDS (column layout, stride 64
+// lanes, so every lane hits its own bank): one binary search for the outgoing
+// value, one for the incoming value, and a single shift of the span between
+// them.  NaN bars are kept as +Inf keys and counted, so every step is one
+// replace.  PNR reads its rank among the first (window - NaNs) keys; MAD takes
+// the median and selects from the two monotone deviation sequences either side
+// of it by a two-array order-statistic search.  All comparisons are on the
+// f32 inputs and all arithmetic is f64 in the oracle's order, so selected values
+// and MAD are f32(oracle) exactly.
+#include "vta_common.h"
+
+struct roll_args {
+    const float *x;
+    const int32_t *p_len;    // batch: per combo
+    const double *p_a;       // batch: BB devup / PNR percentage per combo
+    const double *p_b;       // batch: BB devdn per combo
+    const int32_t *fvs;      // many-series: per series; batch: [0]
+    const int32_t *poison;   // BB: first non-finite bar >= first_valid, per series (ms)
+                             // or [0] (batch); null means none known (one chunk)
+    float *out0;
+    float *out1;
+    float *out2;
+    double a1;               // many-series: devup / percentage
+    double b1;               // many-series: devdn
+    int T;
+    int nlanes;
+    int chunk;
+    int wcap;                // LDS window capacity per lane (PNR, MAD)
+    int p1;                  // many-series: period / length
+    int pad_;
+};
+
+#define RL_NAN_D __builtin_bit_cast(double, 0x7ff8000000000000ull)
+
+__device__ __forceinline__ int rl_wave_min(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+// ---------------------------------------------------------------- BB engine
+
+struct BbEng {
+    double inv_n, sum, sum_sq, up, dn;
+    int p, fv;
+    __device__ void setup(int p_, int fv_, double up_, double dn_) {
+        p = p_; fv = fv_; up = up_; dn = dn_;
+        inv_n = 1.0 / (double)p;
+    }
+    __device__ void out(double m, double var, float *o) const {
+        const double sd = sqrt(fmax(var, 0.0));
+        o[0] = (float)(m + up * sd);
+        o[1] = (float)m;
+        o[2] = (float)(m - dn * sd);
+    }
+};
+
+// Window sums at bar t from scratch: sum over [t-p+1, t] in order (the
+// oracle's first-window loop, bollinger_bands.rs:613-620).
+template <class LD>
+__device__ __forceinline__ void bb_anchor(BbEng &e, int t, LD ld) {
+    double s = 0.0, q = 0.0;
+    for (int k = 0; k < e.p; ++k) {
+        const double v = ld(t - e.p + 1 + k);
+        s += v;
+        q += v * v;
+    }
+    e.sum = s;
+    e.sum_sq = q;
+}
+
+// Outputs of the chunk [c0, c1).  z is the first non-finite bar at or after
+// first_valid (T if none).  Windows ending before z are finite, so any start
+// bar before z may re-anchor.  The running sums turn non-finite at z and NaN
+// for good once bar z leaves the window (x - Inf = -Inf, Inf + -Inf = NaN; a
+// NaN bar poisons at once), so every output from z + p on is NaN.  Between z
+// and z + p - 1 the sums' class (+-Inf or NaN) depends only on the non-finite
+// bars in the window, which an anchor at max(warm, z) sees too: a chunk
+// starting after z replays at most p bars from there.
+template <class LD, class ST>
+__device__ __forceinline__ void bb_run(BbEng &e, int c0, int c1, int z, bool active, LD ld,
+                                       ST st) {
+    if (!active) return;
+    const int warm = e.fv + e.p - 1;
+    const long zp = (long)z + e.p;
+    const float q = VTA_QNAN_F32;
+    const float nan3[3] = {q, q, q};
+    for (int u = c0; u < min(c1, warm); ++u) st(u, nan3);
+    const int cs = max(c0, warm);
+    const int ce = (int)min((long)c1, zp);
+    if (cs < ce) {
+        int t = cs < z ? cs : max(warm, z);
+        bb_anchor(e, t, ld);
+        for (;;) {
+            const double m = e.sum * e.inv_n;
+            const double var = (e.sum_sq * e.inv_n) - (m * m);
+            float o[3];
+            e.out(m, var, o);
+            if (t >= cs) st(t, o);
+            if (++t >= ce) break;
+            const double nw = ld(t), old = ld(t - e.p);
+            e.sum += nw - old;                              // bollinger_bands.rs:640-641
+            e.sum_sq = (e.sum_sq - old * old) + nw * nw;
+        }
+    }
+    for (int u = (int)max((long)c0, min(zp, (long)c1)); u < c1; ++u) st(u, nan3);
+}
+
+// many series: grid.x = 64-series groups, grid.y = chunks
+extern "C" __global__ __launch_bounds__(64) void bb_ms1p_f32(roll_args a) {
+    const int lane = threadIdx.x & 63;
+    const int s = blockIdx.x * 64 + lane;
+    const bool active = s < a.nlanes;
+    const int N = a.nlanes;
+    const int c0 = blockIdx.y * a.chunk;
+    const int c1 = min(c0 + a.chunk, a.T);
+    const int sc = min(s, N - 1);
+    BbEng e;
+    e.setup(a.p1, a.fvs[sc], a.a1, a.b1);
+    const int z = a.poison ? a.poison[sc] : a.T;
+    auto ld = [&](int t) -> double { return (double)a.x[(size_t)t * N + sc]; };
+    auto st = [&](int t, const float *o) {
+        const size_t off = (size_t)t * N + s;
+        a.out0[off] = o[0];
+        a.out1[off] = o[1];
+        a.out2[off] = o[2];
+    };
+    bb_run(e, c0, c1, z, active, ld, st);
+}
+
+// one series x many (period, devup, devdn): grid.x = combos / 64, grid.y = chunks.
+// Lane = combo; series reads are wave-uniform broadcasts from L1/L2.  Output
+// rows are [combo][t]; consecutive bars of one lane are consecutive addresses.
+extern "C" __global__ __launch_bounds__(64) void bb_batch_f32(roll_args a) {
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.x * 64 + lane;
+    const bool active = c < a.nlanes;
+    const int T = a.T;
+    const int c0 = blockIdx.y * a.chunk;
+    const int c1 = min(c0 + a.chunk, T);
+    const int cc = min(c, a.nlanes - 1);
+    BbEng e;
+    e.setup(a.p_len[cc], a.fvs[0], a.p_a[cc], a.p_b[cc]);
+    const int z = a.poison ? a.poison[0] : T;
+    auto ld = [&](int t) -> double { return (double)a.x[t]; };
+    auto st = [&](int t, const float *o) {
+        const size_t off = (size_t)c * T + t;
+        a.out0[off] = o[0];
+        a.out1[off] = o[1];
+        a.out2[off] = o[2];
+    };
+    bb_run(e, c0, c1, z, active, ld, st);
+}
+
+// ---------------------------------------------------------------- sorted window
+//
+// A lane's sorted multiset lives in LDS column `col` (element i at col[64*i]).
+
+struct SortedWin {
+    float *col;
+    int n;
+    __device__ float at(int i) const { return col[64 * i]; }
+    // number of elements < v (lower bound); `steps` binary-search rounds cover n
+    __device__ int lower(float v, int steps) const {
+        int lo = 0, hi = n;
+        for (int r = 0; r < steps; ++r) {
+            if (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (at(mid) < v) lo = mid + 1; else hi = mid;
+            }
+        }
+        return lo;
+    }
+    __device__ void insert(float v, int steps) {
+        const int pos = lower(v, steps);
+        for (int i = n; i > pos; --i) col[64 * i] = col[64 * (i - 1)];
+        col[64 * pos] = v;
+        ++n;
+    }
+    // Replaces one element equal to `vo` (present) by `vi`: the elements
+    // strictly between the two positions move one slot towards vo's slot.
+    __device__ void replace(float vo, float vi, int steps) {
+        const int po = lower(vo, steps);
+        const int pi = lower(vi, steps);
+        const int dir = pi > po ? 1 : -1;
+        const int span = pi > po ? pi - 1 - po : po - pi;
+        int i = po;
+        for (int s = 0; s < span; ++s, i += dir) col[64 * i] = col[64 * (i + dir)];
+        col[64 * i] = vi;
+    }
+};
+
+__device__ __forceinline__ int bsearch_steps(int cap) {
+    int s = 1;
+    while ((1 << s) <= cap) ++s;
+    return s;
+}
+
+// NaN bars enter the window as +Inf keys (counted separately).
+__device__ __forceinline__ float rl_key(float v) { return isnan(v) ? INFINITY : v; }
+
+// percentile_nearest_rank.rs:218-230: k = round(fma(p_frac, wl, 0)) - 1 (Rust
+// round = half away from zero, as C round), clamped to [0, wl - 1].
+__device__ __forceinline__ int pnr_rank(double p_frac, int wl) {
+    const long raw = (long)round(fma(p_frac, (double)wl, 0.0)) - 1;
+    int k = raw <= 0 ? 0 : (int)raw;
+    if (k >= wl) k = wl - 1;
+    return k;
+}
+
+// Fills the window with the `len` bars ending at t (NaN as +Inf keys).
+template <class LD>
+__device__ __forceinline__ int fill_window(SortedWin &w, int t, int len, int steps, LD ld) {
+    int nan_cnt = 0;
+    w.n = 0;
+    for (int i = t - len + 1; i <= t; ++i) {
+        const float v = (float)ld(i);
+        nan_cnt += isnan(v) ? 1 : 0;
+        w.insert(rl_key(v), steps);
+    }
+    return nan_cnt;
+}
+
+// ---------------------------------------------------------------- PNR
+
+// The order statistics of the non-NaN bars of a window equal the first
+// (len - nan_cnt) order statistics of the keyed window, since every NaN key
+// (+Inf) is >= every real value.
+template <class LD, class ST>
+__device__ __forceinline__ void pnr_run(SortedWin &w, int len, double pct, int fv, int c0, int c1,
+                                        bool active, int steps, LD ld, ST st) {
+    const int start_i = fv + len - 1;                 // first output bar
+    int t = max(c0, start_i);
+    if (!active) return;
+    for (int u = c0; u < min(t, c1); ++u) st(u, VTA_QNAN_F32);
+    if (t >= c1) return;
+    const double p_frac = pct * 0.01;
+    int nan_cnt = fill_window(w, t, len, steps, ld);
+    for (;;) {
+        const int wl = len - nan_cnt;
+        st(t, wl == 0 ? VTA_QNAN_F32 : w.at(pnr_rank(p_frac, wl)));
+        if (t + 1 >= c1) break;
+        const float vo = (float)ld(t + 1 - len);
+        const float vi = (float)ld(t + 1);
+        nan_cnt += (isnan(vi) ? 1 : 0) - (isnan(vo) ? 1 : 0);
+        w.replace(rl_key(vo), rl_key(vi), steps);
+        ++t;
+    }
+}
+
+extern "C" __global__ __launch_bounds__(64) void pnr_ms1p_f32(roll_args a) {
+    extern __shared__ float lds[];
+    const int lane = threadIdx.x & 63;
+    const int s = blockIdx.x * 64 + lane;
+    const bool active = s < a.nlanes;
+    const int N = a.nlanes;
+    const int c0 = blockIdx.y * a.chunk;
+    const int c1 = min(c0 + a.chunk, a.T);
+    SortedWin w{lds + lane, 0};
+    const int sc = min(s, N - 1);
+    auto ld = [&](int t) -> double { return (double)a.x[(size_t)t * N + sc]; };
+    auto st = [&](int t, float v) { a.out0[(size_t)t * N + s] = v; };
+    pnr_run(w, a.p1, a.a1, active ? a.fvs[s] : a.T, c0, c1, active, bsearch_steps(a.wcap), ld,
+            st);
+}
+
+extern "C" __global__ __launch_bounds__(64) void pnr_batch_f32(roll_args a) {
+    extern __shared__ float lds[];
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.x * 64 + lane;
+    const bool active = c < a.nlanes;
+    const int T = a.T;
+    const int c0 = blockIdx.y * a.chunk;
+    const int c1 = min(c0 + a.chunk, T);
+    const int cc = min(c, a.nlanes - 1);
+    SortedWin w{lds + lane, 0};
+    auto ld = [&](int t) -> double { return (double)a.x[t]; };
+    auto st = [&](int t, float v) { a.out0[(size_t)c * T + t] = v; };
+    pnr_run(w, a.p_len[cc], a.p_a[cc], a.fvs[0], c0, c1, active, bsearch_steps(a.wcap), ld, st);
+}
+
+// ---------------------------------------------------------------- MAD
+
+// k-th smallest (0-based) deviation |a_i - med| over the sorted window and the
+// (k+1)-th in *next.  Left of `split` the deviations ascend right-to-left
+// (L_i = |a[split-1-i] - med|), from `split` on they ascend (R_j = |a[split+j]
+// - med|).  Binary search for i = how many of the k+1 smallest come from L:
+// the least i with L_i >= R_{k-i}.
+__device__ __forceinline__ double mad_kth(const SortedWin &w, double med, int split, int k,
+                                          int steps, double *next) {
+    const int nl = split, nr = w.n - split;
+    auto L = [&](int i) { return fabs((double)w.at(split - 1 - i) - med); };
+    auto R = [&](int j) { return fabs((double)w.at(split + j) - med); };
+    int lo = max(0, k + 1 - nr), hi = min(k + 1, nl);
+    for (int r = 0; r < steps; ++r) {
+        if (lo < hi) {
+            const int i = (lo + hi) >> 1;
+            if (L(i) < R(k - i)) lo = i + 1; else hi = i;
+        }
+    }
+    const int i = lo, j = k + 1 - lo;
+    const double a = i > 0 ? L(i - 1) : -INFINITY;
+    const double b = j > 0 ? R(j - 1) : -INFINITY;
+    const double c = i < nl ? L(i) : INFINITY;
+    const double d = j < nr ? R(j) : INFINITY;
+    *next = fmin(c, d);
+    return fmax(a, b);
+}
+
+template <class LD, class ST>
+__device__ __forceinline__ void mad_run(SortedWin &w, int p, int fv, int c0, int c1, bool active,
+                                        int steps, LD ld, ST st) {
+    const int warm = fv + p - 1;
+    int t = max(c0, warm);
+    if (!active) return;
+    for (int u = c0; u < min(t, c1); ++u) st(u, VTA_QNAN_F32);
+    if (t >= c1) return;
+    if (p == 1) {                                     // medium_ad.rs:326-333
+        for (; t < c1; ++t) st(t, isnan((float)ld(t)) ? VTA_QNAN_F32 : 0.0f);
+        return;
+    }
+    // any NaN in the window makes the output NaN (medium_ad.rs:343-348)
+    int nan_cnt = fill_window(w, t, p, steps, ld);
+    const int mid = p >> 1;
+    for (;;) {
+        float outv = VTA_QNAN_F32;
+        if (nan_cnt == 0) {
+            double med;
+            if (p & 1) med = (double)w.at(mid);
+            else med = 0.5 * ((double)w.at(mid - 1) + (double)w.at(mid));
+            // elements < med go left; ties with med (deviation 0) are on the right
+            int sp = w.lower((float)med, steps);
+            while (sp > 0 && (double)w.at(sp - 1) >= med) --sp;
+            while (sp < w.n && (double)w.at(sp) < med) ++sp;
+            double nxt;
+            const double kth = mad_kth(w, med, sp, (p & 1) ? mid : mid - 1, steps, &nxt);
+            outv = (float)((p & 1) ? kth : 0.5 * (kth + nxt));
+        }
+        st(t, outv);
+        if (t + 1 >= c1) break;
+        const float vo = (float)ld(t + 1 - p);
+        const float vi = (float)ld(t + 1);
+        nan_cnt += (isnan(vi) ? 1 : 0) - (isnan(vo) ? 1 : 0);
+        w.replace(rl_key(vo), rl_key(vi), steps);
+        ++t;
+    }
+}
+
+extern "C" __global__ __launch_bounds__(64) void mad_ms1p_f32(roll_args a) {
+    extern __shared__ float lds[];
+    const int lane = threadIdx.x & 63;
+    const int s = blockIdx.x * 64 + lane;
+    const bool active = s < a.nlanes;
+    const int N = a.nlanes;
+    const int c0 = blockIdx.y * a.chunk;
+    const int c1 = min(c0 + a.chunk, a.T);
+    SortedWin w{lds + lane, 0};
+    const int sc = min(s, N - 1);
+    auto ld = [&](int t) -> double { return (double)a.x[(size_t)t * N + sc]; };
+    auto st = [&](int t, float v) { a.out0[(size_t)t * N + s] = v; };
+    mad_run(w, a.p1, active ? a.fvs[s] : a.T, c0, c1, active, bsearch_steps(a.wcap), ld, st);
+}
+
+extern "C" __global__ __launch_bounds__(64) void mad_batch_f32(roll_args a) {
+    extern __shared__ float lds[];
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.x * 64 + lane;
+    const bool active = c < a.nlanes;
+    const int T = a.T;
+    const int c0 = blockIdx.y * a.chunk;
+    const int c1 = min(c0 + a.chunk, T);
+    const int cc = min(c, a.nlanes - 1);
+    SortedWin w{lds + lane, 0};
+    auto ld = [&](int t) -> double { return (double)a.x[t]; };
+    auto st = [&](int t, float v) { a.out0[(size_t)c * T + t] = v; };
+    mad_run(w, a.p_len[cc], a.fvs[0], c0, c1, active, bsearch_steps(a.wcap), ld, st);
+}

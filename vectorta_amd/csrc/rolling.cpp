@@ -1,0 +1,391 @@
+// Host entry points for the rolling-window indicators: CudaBollingerBands
+// (src/cuda/bollinger_bands_wrapper.rs), CudaPercentileNearestRank
+// (src/cuda/percentile_nearest_rank_wrapper.rs) and CudaMediumAd
+// (src/cuda/medium_ad_wrapper.rs).  Kernels: kernels/rolling.hip.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "runtime_internal.h"
+
+using namespace vta;
+
+namespace {
+
+struct roll_args_host {
+    const float *x;
+    const int32_t *p_len;
+    const double *p_a;
+    const double *p_b;
+    const int32_t *fvs;
+    const int32_t *poison;
+    float *out0;
+    float *out1;
+    float *out2;
+    double a1;
+    double b1;
+    int T;
+    int nlanes;
+    int chunk;
+    int wcap;
+    int p1;
+    int pad_;
+};
+static_assert(sizeof(roll_args_host) == 112, "roll_args layout");
+
+enum RInd { R_BB = 0, R_PNR = 1, R_MAD = 2 };
+
+// Chunk length: enough chunks for ~2048 waves, at least 4x the window so the
+// per-chunk rebuild stays a small fraction, at most 4096 bars (BB re-anchors
+// its f64 running sums at every chunk start).  `chunked` false: one chunk.
+int roll_chunk(size_t T, size_t lanes, size_t max_w, bool chunked) {
+    if (!chunked) return (int)T;
+    if (const char *e = getenv("VTA_ROLL_CHUNK")) {
+        long v = atol(e);
+        if (v >= 64) return (int)std::min<size_t>((size_t)v, T);
+    }
+    size_t groups = (lanes + 63) / 64;
+    size_t want = std::max<size_t>(1, 2048 / std::max<size_t>(groups, 1));
+    size_t c = std::max<size_t>({(T + want - 1) / want, 4 * max_w, 256});
+    c = std::min<size_t>(c, 4096);
+    return (int)std::min(c, T);
+}
+
+const char *kname(RInd k, bool batch) {
+    static const char *b[] = {"bb_batch_f32", "pnr_batch_f32", "mad_batch_f32"};
+    static const char *m[] = {"bb_ms1p_f32", "pnr_ms1p_f32", "mad_ms1p_f32"};
+    return batch ? b[k] : m[k];
+}
+
+vta_status roll_launch(vta_ctx *ctx, RInd k, bool batch, roll_args_host &a, size_t max_w,
+                       bool chunked) {
+    hipFunction_t f;
+    VTA_TRY(get_function(ctx, "rolling", kname(k, batch), &f));
+    a.chunk = roll_chunk((size_t)a.T, (size_t)a.nlanes, max_w, chunked);
+    a.wcap = (int)max_w;
+    unsigned groups = (unsigned)((a.nlanes + 63) / 64);
+    unsigned chunks = (unsigned)((a.T + a.chunk - 1) / a.chunk);
+    size_t smem = (k == R_BB) ? 0 : (size_t)64 * (max_w + 1) * sizeof(float);
+    if (smem > ctx->props.sharedMemPerBlock)
+        return fail(VTA_ERR_INVALID_INPUT, "window length %zu exceeds the LDS capacity", max_w);
+    return launch(ctx, f, dim3(groups, chunks, 1), dim3(64), (unsigned)smem, a);
+}
+
+// First non-finite element at or after fv (n if none), stride `ld` apart.
+int32_t first_nonfinite(const float *x, size_t n, size_t fv, size_t ld) {
+    for (size_t i = fv; i < n; ++i)
+        if (!std::isfinite(x[i * ld])) return (int32_t)i;
+    return (int32_t)n;
+}
+
+struct Combo {
+    size_t len;
+    double a, b;
+};
+
+// Grids: BB period x devup x devdn (bollinger_bands_wrapper.rs:179-265), PNR
+// length x percentage (percentile_nearest_rank_wrapper.rs:200-290), MAD period
+// (medium_ad_wrapper.rs:120-150).
+vta_status expand_bb(const vta_bb_batch_range *r, std::vector<Combo> &out) {
+    if (!r) return fail(VTA_ERR_INVALID_INPUT, "range is null");
+    std::vector<size_t> ps;
+    std::vector<double> us, ds;
+    if (!expand_usize(r->period, AXIS_DOWN, ps) || !expand_f64(r->devup, us) ||
+        !expand_f64(r->devdn, ds))
+        return fail(VTA_ERR_INVALID_INPUT, "invalid range");
+    out.clear();
+    for (size_t p : ps)
+        for (double u : us)
+            for (double d : ds) out.push_back({p, u, d});
+    return VTA_OK;
+}
+
+vta_status expand_pnr(const vta_pnr_batch_range *r, std::vector<Combo> &out) {
+    if (!r) return fail(VTA_ERR_INVALID_INPUT, "range is null");
+    std::vector<size_t> ls;
+    std::vector<double> ps;
+    if (!expand_usize(r->length, AXIS_DOWN, ls) || !expand_f64(r->percentage, ps))
+        return fail(VTA_ERR_INVALID_INPUT, "invalid range");
+    out.clear();
+    for (size_t l : ls)
+        for (double p : ps) out.push_back({l, p, 0.0});
+    return VTA_OK;
+}
+
+vta_status expand_mad(const vta_range_usize *r, std::vector<Combo> &out) {
+    if (!r) return fail(VTA_ERR_INVALID_INPUT, "range is null");
+    std::vector<size_t> ps;
+    if (!expand_usize(*r, AXIS_DOWN, ps)) return fail(VTA_ERR_INVALID_INPUT, "invalid range");
+    out.clear();
+    for (size_t p : ps) out.push_back({p, 0.0, 0.0});
+    return VTA_OK;
+}
+
+vta_status validate(RInd k, const std::vector<Combo> &cs, size_t len, size_t fv,
+                    size_t *max_len) {
+    if (cs.empty()) return fail(VTA_ERR_INVALID_INPUT, "no parameter combinations");
+    size_t mx = 0;
+    for (const auto &c : cs) {
+        if (c.len == 0 || c.len > len)
+            return fail(VTA_ERR_INVALID_INPUT, "invalid period %zu (data length %zu)", c.len, len);
+        if (k == R_PNR && !(c.a >= 0.0 && c.a <= 100.0))
+            return fail(VTA_ERR_INVALID_INPUT, "invalid percentage %g", c.a);
+        if (c.len > (size_t)INT32_MAX) return fail(VTA_ERR_INVALID_INPUT, "period too large");
+        mx = std::max(mx, c.len);
+    }
+    if (fv >= len) return fail(VTA_ERR_INVALID_INPUT, "all values are NaN");
+    if (len - fv < mx)
+        return fail(VTA_ERR_INVALID_INPUT, "not enough valid data (needed >= %zu, valid = %zu)",
+                    mx, len - fv);
+    *max_len = mx;
+    return VTA_OK;
+}
+
+vta_status batch_dev(vta_ctx *ctx, RInd k, const float *x, size_t len,
+                     const std::vector<Combo> &cs, int nout, vta_device_array_f32 *out) {
+    if (!ctx || !out || !x) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    if (len == 0) return fail(VTA_ERR_INVALID_INPUT, "empty data");
+    if (len > (size_t)INT32_MAX) return fail(VTA_ERR_INVALID_INPUT, "series too long");
+    size_t fv = first_valid_f32(x, len), mx;
+    VTA_TRY(validate(k, cs, len, fv, &mx));
+    const size_t C = cs.size();
+    size_t n_out;
+    if (!mul_ok(C, len, &n_out) || !mul_ok(n_out, sizeof(float), &n_out))
+        return fail(VTA_ERR_INVALID_INPUT, "output bytes overflow");
+    VTA_TRY(check_fits((size_t)nout * n_out + len * sizeof(float)));
+    std::vector<int32_t> pl(C);
+    std::vector<double> pa(C), pb(C);
+    for (size_t i = 0; i < C; ++i) {
+        pl[i] = (int32_t)cs[i].len;
+        pa[i] = cs[i].a;
+        pb[i] = cs[i].b;
+    }
+    int32_t meta[2] = {(int32_t)fv, first_nonfinite(x, len, fv, 1)};
+    DevBuf dx, dl, da, db, dm, o[3];
+    VTA_TRY(upload(ctx, dx, x, len * sizeof(float)));
+    VTA_TRY(upload(ctx, dl, pl.data(), C * sizeof(int32_t)));
+    VTA_TRY(upload(ctx, da, pa.data(), C * sizeof(double)));
+    VTA_TRY(upload(ctx, db, pb.data(), C * sizeof(double)));
+    VTA_TRY(upload(ctx, dm, meta, sizeof(meta)));
+    for (int i = 0; i < nout; ++i) VTA_TRY(o[i].alloc(n_out));
+    roll_args_host a{};
+    a.x = dx.as<float>();
+    a.p_len = dl.as<int32_t>();
+    a.p_a = da.as<double>();
+    a.p_b = db.as<double>();
+    a.fvs = dm.as<int32_t>();
+    a.poison = dm.as<int32_t>() + 1;
+    a.out0 = o[0].as<float>();
+    a.out1 = o[1].as<float>();
+    a.out2 = o[2].as<float>();
+    a.T = (int)len;
+    a.nlanes = (int)C;
+    VTA_TRY(roll_launch(ctx, k, true, a, mx, true));
+    VTA_HIP(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < nout; ++i) {
+        out[i].ptr = static_cast<float *>(o[i].release());
+        out[i].rows = C;
+        out[i].cols = len;
+    }
+    return VTA_OK;
+}
+
+vta_status ms_dev(vta_ctx *ctx, RInd k, const float *x_tm, size_t cols, size_t rows, size_t len,
+                  double pa, double pb, int nout, vta_device_array_f32 *out) {
+    if (!ctx || !out || !x_tm) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    if (cols == 0 || rows == 0) return fail(VTA_ERR_INVALID_INPUT, "cols or rows is zero");
+    if (len == 0 || len > rows) return fail(VTA_ERR_INVALID_INPUT, "invalid period %zu", len);
+    if (k == R_PNR && !(pa >= 0.0 && pa <= 100.0))
+        return fail(VTA_ERR_INVALID_INPUT, "invalid percentage %g", pa);
+    if (rows > (size_t)INT32_MAX || cols > (size_t)INT32_MAX)
+        return fail(VTA_ERR_INVALID_INPUT, "dimensions exceed int32");
+    size_t elems;
+    if (!mul_ok(cols, rows, &elems)) return fail(VTA_ERR_INVALID_INPUT, "cols*rows overflow");
+    std::vector<int32_t> fvs, zs(cols, (int32_t)rows);
+    VTA_TRY(host_first_valids(x_tm, cols, rows, len, fvs));
+    if (k == R_BB)
+        for (size_t s = 0; s < cols; ++s) zs[s] = first_nonfinite(x_tm + s, rows, fvs[s], cols);
+    VTA_TRY(check_fits((size_t)(nout + 1) * elems * sizeof(float)));
+    DevBuf dx, dfv, dz, o[3];
+    VTA_TRY(upload(ctx, dx, x_tm, elems * sizeof(float)));
+    VTA_TRY(upload(ctx, dfv, fvs.data(), cols * sizeof(int32_t)));
+    VTA_TRY(upload(ctx, dz, zs.data(), cols * sizeof(int32_t)));
+    for (int i = 0; i < nout; ++i) VTA_TRY(o[i].alloc(elems * sizeof(float)));
+    roll_args_host a{};
+    a.x = dx.as<float>();
+    a.p1 = (int)len;
+    a.a1 = pa;
+    a.b1 = pb;
+    a.fvs = dfv.as<int32_t>();
+    a.poison = dz.as<int32_t>();
+    a.out0 = o[0].as<float>();
+    a.out1 = o[1].as<float>();
+    a.out2 = o[2].as<float>();
+    a.T = (int)rows;
+    a.nlanes = (int)cols;
+    VTA_TRY(roll_launch(ctx, k, false, a, len, true));
+    VTA_HIP(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < nout; ++i) {
+        out[i].ptr = static_cast<float *>(o[i].release());
+        out[i].rows = rows;
+        out[i].cols = cols;
+    }
+    return VTA_OK;
+}
+
+// Enqueue-only many-series launch of PNR / MAD on device-resident data.
+vta_status ms_device(vta_ctx *ctx, RInd k, const float *d_x, const int32_t *d_fv, size_t cols,
+                     size_t rows, size_t len, double pa, float *d_out) {
+    if (!ctx || !d_x || !d_fv || !d_out) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    if (cols == 0 || rows == 0) return fail(VTA_ERR_INVALID_INPUT, "cols or rows is zero");
+    if (len == 0 || len > rows) return fail(VTA_ERR_INVALID_INPUT, "invalid period %zu", len);
+    if (k == R_PNR && !(pa >= 0.0 && pa <= 100.0))
+        return fail(VTA_ERR_INVALID_INPUT, "invalid percentage %g", pa);
+    if (rows > (size_t)INT32_MAX || cols > (size_t)INT32_MAX)
+        return fail(VTA_ERR_INVALID_INPUT, "dimensions exceed int32");
+    roll_args_host a{};
+    a.x = d_x;
+    a.p1 = (int)len;
+    a.a1 = pa;
+    a.fvs = d_fv;
+    a.out0 = d_out;
+    a.T = (int)rows;
+    a.nlanes = (int)cols;
+    return roll_launch(ctx, k, false, a, len, true);
+}
+
+}  // namespace
+
+extern "C" {
+
+// ------------------------------------------------------------------ Bollinger Bands
+vta_status vta_bb_expand_grid(const vta_bb_batch_range *range, vta_bb_params *out, size_t cap,
+                              size_t *n_out) {
+    clear_error();
+    std::vector<Combo> cs;
+    VTA_TRY(expand_bb(range, cs));
+    if (n_out) *n_out = cs.size();
+    if (out)
+        for (size_t i = 0; i < std::min(cap, cs.size()); ++i)
+            out[i] = {cs[i].len, cs[i].a, cs[i].b};
+    return VTA_OK;
+}
+
+vta_status vta_bollinger_bands_batch_dev(vta_ctx *ctx, const float *prices, size_t len,
+                                         const vta_bb_batch_range *range,
+                                         vta_device_array_f32 *out3) {
+    clear_error();
+    std::vector<Combo> cs;
+    VTA_TRY(expand_bb(range, cs));
+    return batch_dev(ctx, R_BB, prices, len, cs, 3, out3);
+}
+
+vta_status vta_bollinger_bands_many_series_one_param_time_major_dev(
+    vta_ctx *ctx, const float *data_tm, size_t cols, size_t rows, size_t period, double devup,
+    double devdn, vta_device_array_f32 *out3) {
+    clear_error();
+    return ms_dev(ctx, R_BB, data_tm, cols, rows, period, devup, devdn, 3, out3);
+}
+
+vta_status vta_bollinger_bands_many_series_one_param_device(
+    vta_ctx *ctx, const float *d_data_tm, const int32_t *d_first_valids,
+    const int32_t *d_first_nonfinite,
+    size_t cols, size_t rows, size_t period, double devup, double devdn, float *d_upper_tm,
+    float *d_middle_tm, float *d_lower_tm) {
+    clear_error();
+    if (!ctx || !d_data_tm || !d_first_valids || !d_upper_tm || !d_middle_tm || !d_lower_tm ||
+        cols == 0 || rows == 0 || period == 0 || period > rows)
+        return fail(VTA_ERR_INVALID_INPUT, "invalid arguments");
+    if (rows > (size_t)INT32_MAX || cols > (size_t)INT32_MAX)
+        return fail(VTA_ERR_INVALID_INPUT, "dimensions exceed int32");
+    roll_args_host a{};
+    a.x = d_data_tm;
+    a.p1 = (int)period;
+    a.a1 = devup;
+    a.b1 = devdn;
+    a.fvs = d_first_valids;
+    a.poison = d_first_nonfinite;
+    a.out0 = d_upper_tm;
+    a.out1 = d_middle_tm;
+    a.out2 = d_lower_tm;
+    a.T = (int)rows;
+    a.nlanes = (int)cols;
+    return roll_launch(ctx, R_BB, false, a, period, d_first_nonfinite != nullptr);
+}
+
+// ------------------------------------------------------------------ percentile nearest rank
+vta_status vta_pnr_expand_grid(const vta_pnr_batch_range *range, vta_pnr_params *out,
+                               size_t cap, size_t *n_out) {
+    clear_error();
+    std::vector<Combo> cs;
+    VTA_TRY(expand_pnr(range, cs));
+    if (n_out) *n_out = cs.size();
+    if (out)
+        for (size_t i = 0; i < std::min(cap, cs.size()); ++i) out[i] = {cs[i].len, cs[i].a};
+    return VTA_OK;
+}
+
+vta_status vta_pnr_batch_dev(vta_ctx *ctx, const float *prices, size_t len,
+                             const vta_pnr_batch_range *range, vta_device_array_f32 *out) {
+    clear_error();
+    std::vector<Combo> cs;
+    VTA_TRY(expand_pnr(range, cs));
+    return batch_dev(ctx, R_PNR, prices, len, cs, 1, out);
+}
+
+vta_status vta_pnr_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *prices_tm,
+                                                        size_t cols, size_t rows, size_t length,
+                                                        double percentage,
+                                                        vta_device_array_f32 *out) {
+    clear_error();
+    return ms_dev(ctx, R_PNR, prices_tm, cols, rows, length, percentage, 0.0, 1, out);
+}
+
+vta_status vta_pnr_many_series_one_param_device(vta_ctx *ctx, const float *d_data_tm,
+                                                const int32_t *d_first_valids, size_t cols,
+                                                size_t rows, size_t length, double percentage,
+                                                float *d_out_tm) {
+    clear_error();
+    return ms_device(ctx, R_PNR, d_data_tm, d_first_valids, cols, rows, length, percentage,
+                     d_out_tm);
+}
+
+// ------------------------------------------------------------------ rolling MAD
+vta_status vta_medium_ad_expand_grid(const vta_range_usize *range, size_t *periods, size_t cap,
+                                     size_t *n_out) {
+    clear_error();
+    std::vector<Combo> cs;
+    VTA_TRY(expand_mad(range, cs));
+    if (n_out) *n_out = cs.size();
+    if (periods)
+        for (size_t i = 0; i < std::min(cap, cs.size()); ++i) periods[i] = cs[i].len;
+    return VTA_OK;
+}
+
+vta_status vta_medium_ad_batch_dev(vta_ctx *ctx, const float *prices, size_t len,
+                                   const vta_range_usize *range, vta_device_array_f32 *out) {
+    clear_error();
+    std::vector<Combo> cs;
+    VTA_TRY(expand_mad(range, cs));
+    return batch_dev(ctx, R_MAD, prices, len, cs, 1, out);
+}
+
+vta_status vta_medium_ad_many_series_one_param_time_major_dev(vta_ctx *ctx,
+                                                              const float *data_tm, size_t cols,
+                                                              size_t rows, size_t period,
+                                                              vta_device_array_f32 *out) {
+    clear_error();
+    return ms_dev(ctx, R_MAD, data_tm, cols, rows, period, 0.0, 0.0, 1, out);
+}
+
+vta_status vta_medium_ad_many_series_one_param_device(vta_ctx *ctx, const float *d_data_tm,
+                                                      const int32_t *d_first_valids, size_t cols,
+                                                      size_t rows, size_t period,
+                                                      float *d_out_tm) {
+    clear_error();
+    return ms_device(ctx, R_MAD, d_data_tm, d_first_valids, cols, rows, period, 0.0, d_out_tm);
+}
+
+}  // extern "C"

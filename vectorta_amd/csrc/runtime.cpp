@@ -172,6 +172,21 @@ bool expand_usize(vta_range_usize r, AxisKind kind, std::vector<size_t> &v) {
     return !v.empty();
 }
 
+bool expand_f64(vta_range_f64 r, std::vector<double> &v) {
+    v.clear();
+    if (std::fabs(r.step) < 1e-12 || std::fabs(r.start - r.end) < 1e-12) {
+        v.push_back(r.start);
+        return true;
+    }
+    double st = std::fabs(r.step);
+    if (r.start < r.end) {
+        for (double x = r.start; x <= r.end + 1e-12; x += st) v.push_back(x);
+    } else {
+        for (double x = r.start; x + 1e-12 >= r.end; x -= st) v.push_back(x);
+    }
+    return !v.empty();
+}
+
 vta_status host_first_valids(const float *tm, size_t cols, size_t rows, size_t need,
                              std::vector<int32_t> &fvs) {
     fvs.assign(cols, 0);
@@ -306,6 +321,12 @@ vta_status vta_ctx_preload(vta_ctx *ctx) {
         {"rec", "series_nonfinite_flag"},
         {"rec", "flag_to_clean"},
         {"rec", "vta_fill_i32"},
+        {"rolling", "bb_batch_f32"},
+        {"rolling", "bb_ms1p_f32"},
+        {"rolling", "pnr_batch_f32"},
+        {"rolling", "pnr_ms1p_f32"},
+        {"rolling", "mad_batch_f32"},
+        {"rolling", "mad_ms1p_f32"},
     };
     for (auto &s : kSymbols) {
         hipFunction_t f;

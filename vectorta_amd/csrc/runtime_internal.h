@@ -105,6 +105,9 @@ size_t first_valid_f32(const float *x, size_t n);  // n if all NaN
 //   AXIS_ATR     ascending if start < end, else (end..=start step) reversed (atr_wrapper.rs:286-297)
 enum AxisKind { AXIS_SORTED, AXIS_DOWN, AXIS_ATR };
 bool expand_usize(vta_range_usize r, AxisKind kind, std::vector<size_t> &v);
+// f64 axis: values accumulate x += step while x <= end + 1e-12 (or descend
+// while x + 1e-12 >= end), alma_wrapper.rs:1576-1612.
+bool expand_f64(vta_range_f64 r, std::vector<double> &v);
 
 // Per-series first_valid of a time-major [rows][cols] host matrix; fails with
 // InvalidInput if a series is all NaN or has fewer than `need` valid bars.
