@@ -65,6 +65,23 @@ class HipEvents:
         self.h.hipEventDestroy(e)
 
 
+def profiled_traffic(summary: str, kernel: str):
+    """HBM bytes per launch of `kernel` recorded in profiles/<summary> (two
+    rocprofv3 --pmc runs, FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel
+    calibration); None if absent."""
+    path = os.path.join(ROOT, "profiles", summary)
+    key = f"traffic_bytes_per_launch[{kernel}] = "
+    try:
+        with open(path) as f:
+            for line in f:
+                line = line.strip()
+                if line.startswith(key):
+                    return int(line[len(key):])
+    except OSError:
+        pass
+    return None
+
+
 def algorithmic_work(combos, T):
     """Bytes: outputs written + prices read + f32 weights read; FLOPs: 2*p per valid output."""
     out_b = len(combos) * T * 4
@@ -116,25 +133,29 @@ def max_rel_err(g, ref):
 
 
 def calibrate_copy(nbytes: int = 256 << 20, reps: int = 3) -> None:
-    """Device-to-device copies of a known size (read nbytes + write nbytes each)."""
-    ctx = vta.HipContext(0)
-    src, dst = ctx.alloc(nbytes), ctx.alloc(nbytes)
-    h = ctypes.CDLL("libamdhip64.so")
-    h.hipMemcpyDtoDAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
-                                     ctypes.c_void_p]
-    ev = HipEvents()
-    for _ in range(reps):
-        a, b = ev.create(), ev.create()
-        ev.record(a, ctx.stream)
-        assert h.hipMemcpyDtoDAsync(ctypes.c_void_p(dst.ptr), ctypes.c_void_p(src.ptr), nbytes,
-                                    ctypes.c_void_p(ctx.stream)) == 0
-        ev.record(b, ctx.stream)
-        ms = ev.elapsed_ms(a, b)
-        print(f"[calib] copy {nbytes >> 20} MiB: {ms:.4f} ms, {2 * nbytes / ms / 1e6:.1f} GB/s "
-              "(read + write)", flush=True)
-    ctx.synchronize()
-    src.free()
-    dst.free()
+    """Plain streaming kernels of known traffic -- the references that scale
+    the FETCH_SIZE / WRITE_SIZE counters.  torch.neg(out=) on 16-byte-aligned
+    buffers issues 16-byte-per-lane loads; on views shifted by one float it
+    cannot vectorise and issues 4-byte-per-lane loads (the width the indicator
+    kernels use).  Each reads and writes nbytes (the shifted one 4 bytes
+    less).  A device-to-device hipMemcpy runs on the copy engines, which those
+    counters do not see."""
+    import torch
+    n = nbytes // 4
+    src = torch.ones(n + 1, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for label, s_, d_ in (("vec16", src[:n], dst[:n]), ("vec4", src[1:], dst[1:])):
+        for _ in range(reps):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            torch.neg(s_, out=d_)
+            b.record()
+            torch.cuda.synchronize()
+            ms = a.elapsed_time(b)
+            print(f"[calib] neg {label} {s_.numel() * 4} bytes: {ms:.4f} ms, "
+                  f"{2 * s_.numel() * 4 / ms / 1e6:.1f} GB/s (read + write)", flush=True)
+        assert float(d_[0]) == -1.0
 
 
 def bench_many_series_ohlc(steps, warmup, ev):
@@ -166,6 +187,8 @@ def bench_many_series_ohlc(steps, warmup, ev):
             "kernel_ms_mean": float(ks.mean()), "kernel_ms_p50": float(np.percentile(ks, 50)),
             "kernel_ms_p99": float(np.percentile(ks, 99)), "outputs": 3 * rows * cols,
             "outputs_per_s": 3 * rows * cols / t, "algorithmic_bytes": alg,
+            "traffic": profiled_traffic("ema_rsi_atr_many_series_4096x50000_traffic.txt",
+                                        "ms_ema_rsi_atr_f32"),
             "achieved_GBs": alg / t / 1e9, "frac_of_hbm_peak": alg / t / 1e9 / HBM_PEAK_GBS}
 
 
@@ -202,9 +225,38 @@ def bench_rolling(steps, warmup, ev):
                      "kernel_ms_p50": float(np.percentile(ks, 50)),
                      "kernel_ms_p99": float(np.percentile(ks, 99)),
                      "series_bars_per_s": rows * cols / t, "algorithmic_bytes": alg,
+                     "traffic": profiled_traffic("rolling_windows_1024x50000_traffic.txt", kern),
                      "achieved_GBs": alg / t / 1e9,
                      "frac_of_hbm_peak": alg / t / 1e9 / HBM_PEAK_GBS}
     return res
+
+
+def bench_alma_host_boundary(reps=3):
+    """The sweep through the host-buffer entry points, wall clock: alma_batch_dev
+    (host prices in, device-resident output allocated by the call) and
+    alma_batch_into_host_f32 (output copied back over PCIe into pageable host
+    memory)."""
+    x = datagen.alma_sweep_series()
+    sweep = vta.AlmaBatchRange(**datagen.ALMA_SWEEP_RANGE)
+    alma = vta.CudaAlma(0)
+    n = len(vta.expand_grid(sweep)) * x.size
+    host = np.empty(n, dtype=np.float32)
+    host[:] = 0.0                                   # touch the pages once
+    dev_ms, host_ms = [], []
+    for _ in range(reps + 1):
+        t0 = time.perf_counter()
+        arr = alma.alma_batch_dev(x, sweep)
+        dev_ms.append((time.perf_counter() - t0) * 1e3)
+        arr.free()
+        t0 = time.perf_counter()
+        alma.alma_batch_into_host_f32(x, sweep, host)
+        host_ms.append((time.perf_counter() - t0) * 1e3)
+    dev_ms, host_ms = dev_ms[1:], host_ms[1:]
+    return {"workload": "alma_sweep_200k_x_1250_host_boundary",
+            "batch_dev_ms_median": float(np.median(dev_ms)),
+            "batch_dev_outputs_per_s": n / (np.median(dev_ms) * 1e-3),
+            "into_host_ms_median": float(np.median(host_ms)),
+            "into_host_outputs_per_s": n / (np.median(host_ms) * 1e-3)}
 
 
 def bench_alma_single(steps, ev):
@@ -315,6 +367,7 @@ def main():
 
     head = "mfma" if ok("mfma") else "fp32"
     hf = forms[head]
+    out_kernel = "alma_batch_mfma_bf16x3" if head == "mfma" else "alma_batch_f32"
     mf = forms.get("mfma")
     ff = forms.get("fp32")
     out = {
@@ -333,12 +386,13 @@ def main():
         "config": {"workload": "alma_sweep_200k_x_1250", "series_len": T, "combos": C,
                    "outputs": C * T, "periods": "10..=250 step 10",
                    "offsets": "0.05..=0.95 step 0.1", "sigmas": "2..=10 step 2",
-                   "kernel": "alma_batch_mfma_bf16x3" if head == "mfma" else "alma_batch_f32",
-                   "parallelism": "single-gpu"},
+                   "kernel": out_kernel, "parallelism": "single-gpu"},
         "latency_ms": {"p50": hf["kernel_ms_p50"], "p99": hf["kernel_ms_p99"]},
         "roofline": {"bound": "hbm", "achieved": hf["achieved_GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": hf["achieved_GBs"] / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes": alg_bytes},
+                     "traffic": profiled_traffic("alma_sweep_200k_x1250_traffic.txt",
+                                                 out_kernel),
+                     "algorithmic_bytes": alg_bytes, "kernel": out_kernel},
         "forms": forms,
         "mfma_speedup_vs_fp32": (ff["kernel_ms_mean"] / mf["kernel_ms_mean"])
         if (mf and ff) else None,
@@ -349,6 +403,7 @@ def main():
         extra = {}
         for name, fn in (("many_series_ohlc", lambda: bench_many_series_ohlc(10, 2, ev)),
                          ("rolling_windows", lambda: bench_rolling(5, 1, ev)),
+                         ("alma_host_boundary", lambda: bench_alma_host_boundary()),
                          ("alma_single", lambda: bench_alma_single(20, ev))):
             extra[name] = fn()
             print(f"[bench] {name}: {extra[name]}", file=sys.stderr, flush=True)
