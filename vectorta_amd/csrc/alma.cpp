@@ -192,6 +192,24 @@ struct alma_mf_args_host {
     int kmax;
 };
 
+struct alma_mfg_args_host {
+    const float *prices;
+    const void *frags;
+    const void *nblocks;
+    const void *groups;
+    const int32_t *sorted_period;
+    const int32_t *sorted_row;
+    const float *w_f32;
+    float *out;
+    int series_len;
+    int first_valid;
+    int n_groups;
+    int n_tiles;
+    int kmax;
+    int pad_;
+};
+static_assert(sizeof(alma_mfg_args_host) == 88, "alma_mfg_args layout");
+
 struct vta_alma_sweep_plan {
     vta_alma_batch_kernel kernel = VTA_ALMA_BATCH_FP32;
     size_t series_len = 0;
@@ -200,14 +218,22 @@ struct vta_alma_sweep_plan {
     // FP32 form
     DevBuf w, periods, inv;
     // MFMA form
-    DevBuf frags, nblocks, sorted_period, sorted_row, w_f32;
+    DevBuf frags, nblocks, groups, sorted_period, sorted_row, w_f32;
     int n_nblocks = 0;
+    int n_groups = 0;
     int kmax = 0;
 };
 
 namespace {
 
 constexpr int kMfTilesPerWave = 16;
+
+// ALMA_MF_FORM=wave selects the one-wave-per-N-block kernels; the default is
+// the shared-staging workgroup form.
+bool mf_shared_staging() {
+    const char *e = getenv("ALMA_MF_FORM");
+    return !(e && strcmp(e, "wave") == 0);
+}
 constexpr size_t kMfMaxPeriod = 256;
 
 // Builds the MFMA operand tables.  Combos are sorted by period (stable) and
@@ -275,6 +301,18 @@ vta_status build_mfma_tables(vta_alma_sweep_plan *plan,
         weights_f64(cp.period, cp.offset, cp.sigma, wd);
         for (size_t k = 0; k < cp.period; ++k) wf[i * kmax + k] = (float)wd[k];
     }
+    // groups of up to 4 consecutive N-blocks (ascending K, so the last one has
+    // the group's largest K) for the shared-staging kernel
+    std::vector<int32_t> grp;
+    for (int b = 0; b < nb; b += 4) {
+        const int cnt = std::min(4, nb - b);
+        grp.push_back(b);
+        grp.push_back(cnt);
+        grp.push_back(nbd[(b + cnt - 1) * 4 + 0]);
+        grp.push_back(0);
+    }
+    VTA_TRY(upload(ctx, plan->groups, grp.data(), grp.size() * sizeof(int32_t)));
+    plan->n_groups = (int)(grp.size() / 4);
     VTA_TRY(upload(ctx, plan->frags, frags.data(), frags.size() * sizeof(uint16_t)));
     VTA_TRY(upload(ctx, plan->nblocks, nbd.data(), nbd.size() * sizeof(int32_t)));
     VTA_TRY(upload(ctx, plan->sorted_period, sp.data(), sp.size() * sizeof(int32_t)));
@@ -331,6 +369,44 @@ vta_status plan_create(vta_ctx *ctx, const std::vector<vta_alma_params> &combos,
 vta_status plan_launch(vta_ctx *ctx, const vta_alma_sweep_plan *plan, const float *d_prices,
                        size_t fv, float *d_out) {
     const int T = (int)plan->series_len;
+    if (plan->kernel == VTA_ALMA_BATCH_MFMA_BF16X3 && mf_shared_staging()) {
+        // shared-staging form: 4 N-blocks per workgroup, R tiles per run
+        // (ALMA_MF_TILES=8|16, default 16)
+        int tiles = 16;
+        if (const char *e = getenv("ALMA_MF_TILES")) {
+            int v = atoi(e);
+            if (v == 8 || v == 16) tiles = v;
+        }
+        hipFunction_t f;
+        VTA_TRY(get_function(ctx, "alma",
+                             tiles == 8 ? "alma_batch_mfma_bf16x3_wg8" : "alma_batch_mfma_bf16x3_wg16",
+                             &f));
+        alma_mfg_args_host a{};
+        a.prices = d_prices;
+        a.frags = plan->frags.p;
+        a.nblocks = plan->nblocks.p;
+        a.groups = plan->groups.p;
+        a.sorted_period = plan->sorted_period.as<int32_t>();
+        a.sorted_row = plan->sorted_row.as<int32_t>();
+        a.w_f32 = plan->w_f32.as<float>();
+        a.out = d_out;
+        a.series_len = T;
+        a.first_valid = (int)fv;
+        a.n_groups = plan->n_groups;
+        a.n_tiles = (T + 31) / 32;
+        a.kmax = plan->kmax;
+        const int npair = plan->kmax / 2 + 15;
+        const int stride = ((npair + 15) >> 5) * 32 + 16;
+        const size_t smem =
+            4 * ((size_t)(32 * tiles + plan->kmax + 32) + 2 * (size_t)tiles +
+                 (size_t)tiles * 4 * (size_t)stride);
+        if (smem > ctx->props.sharedMemPerBlock)
+            return fail(VTA_ERR_LAUNCH_CONFIG_TOO_LARGE, "MFMA staging needs %zu bytes of LDS",
+                        smem);
+        dim3 grid((a.n_tiles + tiles - 1) / tiles, plan->n_groups, 1);
+        ctx->alma_selected = VTA_ALMA_BATCH_MFMA_BF16X3;
+        return launch(ctx, f, grid, dim3(256), (unsigned)smem, a);
+    }
     if (plan->kernel == VTA_ALMA_BATCH_MFMA_BF16X3) {
         // tiles per wave: 16 by default; ALMA_MF_TILES=8|32 overrides (tuning knob,
         // like the reference's ALMA_TILE, alma_wrapper.rs:314)

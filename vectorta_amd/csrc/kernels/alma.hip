@@ -491,6 +491,230 @@ extern "C" __global__ __launch_bounds__(64) void alma_batch_mfma_bf16x3_r32(alma
     alma_mf_entry<32>(a);
 }
 
+// ---------------------------------------------------------------- MFMA, shared staging
+//
+// Same contraction, organised so the staging is shared: a workgroup of four
+// waves takes up to four consecutive N-blocks (sorted by period, so their K
+// differ little) over one run of R tiles.  The workgroup loads the run's prices
+// once and stages every tile's hi/lo copies, sized for the group's largest K
+// (K_g), into LDS with all 256 lanes; after one barrier each wave contracts its
+// own N-block over all R tiles with no further synchronisation.  A wave whose
+// K is below K_g starts its window K_g - K elements into the staged tile (a
+// multiple of 16 elements, so the dword alignment of the two copies holds).
+// The four arrays of a tile are `stride` dwords apart with stride = 16 mod 32,
+// which puts the two copies a wave reads in one instruction on disjoint banks.
+
+struct alma_mfg_args {
+    const float *prices;
+    const uint4 *frags;
+    const int4 *nblocks;         // {ks, frag_off, combo_base, 0}
+    const int4 *groups;          // {first N-block, count (<= 4), ks of the group, 0}
+    const int32_t *sorted_period;
+    const int32_t *sorted_row;
+    const float *w_f32;
+    float *out;
+    int series_len;
+    int first_valid;
+    int n_groups;
+    int n_tiles;
+    int kmax;
+    int pad_;
+};
+
+__device__ __forceinline__ int mfg_stride(int npair) { return ((npair + 15) >> 5) * 32 + 16; }
+
+template <int KS, int R>
+__device__ __forceinline__ void alma_mfg_wave(const alma_mfg_args &a, int nb, int tile_begin,
+                                              int nt, int kg, int stride,
+                                              const uint32_t *stage, const float *anch,
+                                              const int *badf) {
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 5;
+    const int m = lane & 31;
+    const int4 nbd = a.nblocks[nb];
+    const int frag_off = nbd.y;
+    const int combo_base = nbd.z;
+    constexpr int K = KS * 16;
+    uint4 bhi[KS], blo[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        bhi[s] = a.frags[frag_off + (2 * s + 0) * 64 + lane];
+        blo[s] = a.frags[frag_off + (2 * s + 1) * 64 + lane];
+    }
+    const int T = a.series_len;
+    const int my_period = a.sorted_period[combo_base + m];
+    const int my_row = a.sorted_row[combo_base + m];
+    const int my_warm = a.first_valid + my_period - 1;
+    int warm_max = my_warm;                       // wave max: tiles past it need no NaN select
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) warm_max = max(warm_max, __shfl_xor(warm_max, o));
+    warm_max = __builtin_amdgcn_readfirstlane(warm_max);
+    const int r = m & 1;
+    const int d0 = (m >> 1) + 4 * g + ((kg - K) >> 1);
+    const int hi_off = (r ? stride : 0) + d0;
+    const int lo_off = 2 * stride + hi_off;
+    float *row = a.out + (size_t)(my_row < 0 ? 0 : my_row) * (size_t)T;
+
+    // A fragments of K-step s of tile j; the next step's pair is loaded while
+    // the current step's three MFMAs run (and tile j+1's first step during
+    // tile j's epilogue).
+    auto load_a = [&](int j, int s, uint4 &h, uint4 &l) {
+        const uint32_t *hb = stage + j * 4 * stride + hi_off + 8 * s;
+        const uint32_t *lb = stage + j * 4 * stride + lo_off + 8 * s;
+        h.x = hb[0]; h.y = hb[1]; h.z = hb[2]; h.w = hb[3];
+        l.x = lb[0]; l.y = lb[1]; l.z = lb[2]; l.w = lb[3];
+    };
+    uint4 ah_n, al_n;
+    load_a(0, 0, ah_n, al_n);
+    for (int j = 0; j < nt; ++j) {
+        // all three products into one FP32 accumulator: the corrections are
+        // rounded at the same 2^-24 |acc| step as the main terms either way
+        vta_f32x16 acc = {};
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const uint4 ah = ah_n, al = al_n;
+            if (s + 1 < KS) load_a(j, s + 1, ah_n, al_n);
+            const vta_bf16x8 ahi = __builtin_bit_cast(vta_bf16x8, ah);
+            const vta_bf16x8 alo = __builtin_bit_cast(vta_bf16x8, al);
+            const vta_bf16x8 bh = __builtin_bit_cast(vta_bf16x8, bhi[s]);
+            const vta_bf16x8 bl = __builtin_bit_cast(vta_bf16x8, blo[s]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bh, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bl, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bh, acc, 0, 0, 0);
+        }
+        if (j + 1 < nt) load_a(j + 1, 0, ah_n, al_n);
+        const float anchor = anch[j];
+        const int t_tile = (tile_begin + j) * 32;
+        if (my_row < 0) continue;
+        if (badf[j] == 0 && t_tile >= warm_max && t_tile + 32 <= T && (T & 3) == 0) {
+            // steady state: every bar of the tile is a finite output
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int tb = t_tile + 8 * q + 4 * g;
+                *reinterpret_cast<float4 *>(row + tb) =
+                    make_float4(acc[4 * q + 0] + anchor, acc[4 * q + 1] + anchor,
+                                acc[4 * q + 2] + anchor, acc[4 * q + 3] + anchor);
+            }
+            continue;
+        }
+        const bool tile_bad = badf[j] != 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int tb = t_tile + 8 * q + 4 * g;
+            float v[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int t = tb + i;
+                float val = acc[4 * q + i] + anchor;
+                if (tile_bad && t >= my_warm && t < T) {
+                    // plain FP32 dot, same taps as the FP32 kernel
+                    const float *wrow = a.w_f32 + (size_t)(combo_base + m) * a.kmax;
+                    float dot = 0.0f;
+                    for (int k = 0; k < my_period; ++k)
+                        dot = fmaf(wrow[k], a.prices[t - my_period + 1 + k], dot);
+                    val = dot;
+                }
+                v[i] = (t >= my_warm) ? val : VTA_QNAN_F32;
+            }
+            if (((T & 3) == 0) && tb + 3 < T) {
+                *reinterpret_cast<float4 *>(row + tb) = make_float4(v[0], v[1], v[2], v[3]);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (tb + i < T) row[tb + i] = v[i];
+            }
+        }
+    }
+}
+
+// grid.x = tile runs of R tiles, grid.y = N-block groups; block = 256 (4 waves).
+// Dynamic LDS (dwords): XSN = 32R + kmax + 32 prices, R anchors, R flags, then
+// R * 4 * mfg_stride(kmax/2 + 15) staged dwords.
+template <int R>
+__device__ __forceinline__ void alma_mfg_entry(const alma_mfg_args &a) {
+    extern __shared__ __attribute__((aligned(16))) uint32_t mfg_lds[];
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
+    // groups are sorted by K ascending; dispatch the heaviest first so the
+    // last workgroups to start are the short ones
+    const int4 gd = a.groups[a.n_groups - 1 - (int)blockIdx.y];
+    const int kg = gd.z * 16;
+    const int npair = kg / 2 + 15;
+    const int stride = mfg_stride(npair);
+    const int xsn = 32 * R + a.kmax + 32;
+    float *xs = reinterpret_cast<float *>(mfg_lds);
+    float *anch = xs + xsn;
+    int *badf = reinterpret_cast<int *>(anch + R);
+    uint32_t *stage = reinterpret_cast<uint32_t *>(badf + R);
+    const int T = a.series_len;
+    const int fv = a.first_valid;
+    const int tile_begin = blockIdx.x * R;
+    const int nt = min(R, a.n_tiles - tile_begin);
+    const int t_lo = tile_begin * 32 - kg + 1;
+    const int nx = nt * 32 + kg;            // staged reads reach index 32 nt + kg - 2
+    for (int i = tid; i < nx; i += 256) {
+        const int idx = t_lo + i;
+        const float x = vta_ld_clamped(a.prices, idx, fv, T);
+        xs[i] = (idx >= fv && idx < T) ? x : 0.0f;
+    }
+    if (tid < R) badf[tid] = 0;
+    __syncthreads();
+    if (tid < nt) {
+        // anchor: the tile's last in-range bar (xs index of bar t is t - t_lo)
+        const int ai = min((tile_begin + tid) * 32 + 31, T - 1);
+        const float an = xs[ai - t_lo];
+        anch[tid] = vta_isfinite(an) ? an : 0.0f;
+    }
+    __syncthreads();
+    // staging unit u: tile j = u / npair, pair i = u % npair (u < 16 * 143, so
+    // the float quotient of u + 0.5 is never within rounding of an integer)
+    const float inv_np = 1.0f / (float)npair;
+    const int units = nt * npair;
+    for (int u = tid; u < units; u += 256) {
+        const int j = (int)(((float)u + 0.5f) * inv_np);
+        const int i = u - j * npair;
+        const float anchor = anch[j];
+        const int xo = 32 * j + 2 * i;
+        const float x0 = xs[xo], x1 = xs[xo + 1], x2 = xs[xo + 2];
+        float v0 = x0 - anchor, v1 = x1 - anchor, v2 = x2 - anchor;
+        bool bad = false;
+        if (!vta_isfinite(x0)) { bad = true; v0 = 0.0f; }
+        if (!vta_isfinite(x1)) { bad = true; v1 = 0.0f; }
+        if (!vta_isfinite(x2)) { bad = true; v2 = 0.0f; }
+        if (bad) badf[j] = 1;
+        const uint16_t h0 = vta_bf16_rne(v0), h1 = vta_bf16_rne(v1), h2 = vta_bf16_rne(v2);
+        const uint16_t l0 = vta_bf16_rne(v0 - vta_bf16_to_f32(h0));
+        const uint16_t l1 = vta_bf16_rne(v1 - vta_bf16_to_f32(h1));
+        const uint16_t l2 = vta_bf16_rne(v2 - vta_bf16_to_f32(h2));
+        uint32_t *base = stage + j * 4 * stride + i;
+        base[0] = mf_pack(h0, h1);
+        base[stride] = mf_pack(h1, h2);
+        base[2 * stride] = mf_pack(l0, l1);
+        base[3 * stride] = mf_pack(l1, l2);
+    }
+    __syncthreads();
+    if (wave >= gd.y) return;
+    const int nb = gd.x + wave;
+    switch (a.nblocks[nb].x) {
+#define MFG_CASE(N)                                                                    \
+    case N:                                                                            \
+        alma_mfg_wave<N, R>(a, nb, tile_begin, nt, kg, stride, stage, anch, badf);     \
+        break;
+        MFG_CASE(1) MFG_CASE(2) MFG_CASE(3) MFG_CASE(4) MFG_CASE(5) MFG_CASE(6) MFG_CASE(7)
+        MFG_CASE(8) MFG_CASE(9) MFG_CASE(10) MFG_CASE(11) MFG_CASE(12) MFG_CASE(13)
+        MFG_CASE(14) MFG_CASE(15) MFG_CASE(16)
+#undef MFG_CASE
+        default: break;
+    }
+}
+
+extern "C" __global__ __launch_bounds__(256) void alma_batch_mfma_bf16x3_wg8(alma_mfg_args a) {
+    alma_mfg_entry<8>(a);
+}
+extern "C" __global__ __launch_bounds__(256) void alma_batch_mfma_bf16x3_wg16(alma_mfg_args a) {
+    alma_mfg_entry<16>(a);
+}
+
 // ---------------------------------------------------------------- direct fallbacks
 //
 // One output per lane, window read straight from global memory.  Used when a

@@ -306,6 +306,8 @@ vta_status vta_ctx_preload(vta_ctx *ctx) {
         {"alma", "alma_batch_mfma_bf16x3"},
         {"alma", "alma_batch_mfma_bf16x3_r8"},
         {"alma", "alma_batch_mfma_bf16x3_r32"},
+        {"alma", "alma_batch_mfma_bf16x3_wg8"},
+        {"alma", "alma_batch_mfma_bf16x3_wg16"},
         {"alma", "alma_batch_f32_direct"},
         {"alma", "alma_ms1p_f32_direct"},
         {"rec", "ema_batch_f32"},
