@@ -368,6 +368,41 @@ vta_status vta_medium_ad_many_series_one_param_device(vta_ctx *ctx, const float 
                                                       size_t rows, size_t period,
                                                       float *d_out_tm);
 
+/* ------------------------------------------------------- double-ALMA crossover */
+
+/* Backtest reduction of a double-ALMA crossover sweep (no reference
+ * counterpart: README.md:9 names the workload, 58,300 fast/slow pairs over
+ * 200,000 bars).  Pairs are fast x slow in that nesting order (pair
+ * i * n_slow + j); for each, pos_t = sign(ALMA_fast[t] - ALMA_slow[t]) with
+ * the batch ALMA weights (alma.rs:1426-1436), from the first bar where both
+ * are valid, reduced to
+ *   stats4[4p + 0] trades    (bars where the position changes),
+ *   stats4[4p + 1] n_long    (bars with pos = +1),
+ *   stats4[4p + 2] n_short   (bars with pos = -1),
+ *   stats4[4p + 3] checksum  = sum_t pos_t * (t + 1),
+ *   pnl[p]         = sum_{t <= len-2} pos_t * (price[t+1] - price[t]).
+ * Positions equal the f64 ALMA's signs exactly: the rows of the distinct
+ * parameter sets are computed on the device in f64 in the scalar kernel's
+ * operation order (alma.rs:1505-1527).  Prices must be finite after
+ * first_valid; periods <= 294.  stats4 and pnl NULL: *n_pairs only. */
+vta_status vta_double_alma_crossover(vta_ctx *ctx, const float *prices, size_t len,
+                                     const vta_alma_batch_range *fast,
+                                     const vta_alma_batch_range *slow, int64_t *stats4,
+                                     double *pnl, size_t cap, size_t *n_pairs);
+/* Prepared form: prices resident on the device, tables built once; launch is
+ * enqueue only (f64 rows, pair walk, segment reduction); results
+ * synchronises and copies. */
+typedef struct vta_double_alma_plan vta_double_alma_plan;
+vta_status vta_double_alma_plan_create(vta_ctx *ctx, const float *prices, size_t len,
+                                       const vta_alma_batch_range *fast,
+                                       const vta_alma_batch_range *slow,
+                                       vta_double_alma_plan **out);
+size_t vta_double_alma_plan_pairs(const vta_double_alma_plan *plan);
+vta_status vta_double_alma_plan_launch(vta_double_alma_plan *plan);
+vta_status vta_double_alma_plan_results(vta_double_alma_plan *plan, int64_t *stats4,
+                                        double *pnl, size_t cap);
+vta_status vta_double_alma_plan_destroy(vta_double_alma_plan *plan);
+
 #ifdef __cplusplus
 }
 #endif

@@ -76,6 +76,10 @@ def lib():
         L.or_many_series_3in.argtypes = [ctypes.c_int, dp, dp, dp, sz, sz, sz, dp,
                                          ctypes.POINTER(ctypes.c_int), ctypes.c_int]
         L.or_many_series_3in.restype = None
+        L.or_cross_pairs.argtypes = [dp, sz, sz, dp, ctypes.POINTER(ctypes.c_int),
+                                     ctypes.POINTER(ctypes.c_int), sz,
+                                     ctypes.POINTER(ctypes.c_longlong), dp, ctypes.c_int]
+        L.or_cross_pairs.restype = None
         _lib = L
     return _lib
 
@@ -231,3 +235,21 @@ def atr_batch(h, l, c, periods):
 
 def adx_batch(h, l, c, periods):
     return np.stack([adx(h, l, c, p) for p in periods])
+
+
+def cross_pairs(rows, x, pair_fast, pair_slow, nthreads=0):
+    """Double-ALMA crossover statistics of row pairs of f64 ALMA rows [nrows, len].
+    Returns (stats int64 [npairs, 4] = trades, n_long, n_short, checksum; pnl f64)."""
+    rows, rp = _d(rows)
+    nrows, n = rows.shape
+    x, xp = _d(x)
+    pf = np.ascontiguousarray(pair_fast, dtype=np.int32)
+    ps = np.ascontiguousarray(pair_slow, dtype=np.int32)
+    npairs = pf.size
+    stats = np.zeros((npairs, 4), dtype=np.int64)
+    pnl = np.zeros(npairs)
+    ip = ctypes.POINTER(ctypes.c_int)
+    lib().or_cross_pairs(rp, nrows, n, xp, pf.ctypes.data_as(ip), ps.ctypes.data_as(ip), npairs,
+                         stats.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), _d(pnl)[1],
+                         int(nthreads))
+    return stats, pnl

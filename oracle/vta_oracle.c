@@ -25,6 +25,9 @@
  *   MAD   src/indicators/medium_ad.rs:175-230, 293-402
  *   ADX   src/indicators/adx.rs:190-298, 299-422
  *   NaN prefixes: src/utilities/helpers.rs:102-164 (quiet NaN 0x7ff8...)
+ *   Double-ALMA crossover: not in the reference (README.md:9 names the
+ *         workload only); position = sign(ALMA_fast - ALMA_slow) of the f64
+ *         batch rows above, reduced to per-pair statistics (or_cross_pairs).
  *
  * Status codes: 0 ok, 1 empty input, 2 all NaN, 3 invalid period,
  * 4 not enough valid data, 5 invalid parameter (sigma/offset/percentage).
@@ -673,5 +676,50 @@ void or_many_series_3in(int kind, const double *h_tm, const double *l_tm, const 
         free(l);
         free(c);
         free(y);
+    }
+}
+
+/* ------------------------------------------------------------ double-ALMA crossover */
+
+/* Per pair (fast row f, slow row s of the f64 ALMA batch rows, [nrows][len]):
+ * from w = first bar where both rows are non-NaN, pos_t = sign(f_t - s_t)
+ * (+1, -1, or 0 when equal), and
+ *   stats[0] trades   = #{t > w : pos_t != pos_{t-1}}
+ *   stats[1] n_long   = #{t >= w : pos_t = +1}
+ *   stats[2] n_short  = #{t >= w : pos_t = -1}
+ *   stats[3] checksum = sum_{t >= w} pos_t * (t + 1)
+ *   pnl              = sum_{t = w}^{len-2} pos_t * (x_{t+1} - x_t)
+ * summed bar by bar in time order. */
+void or_cross_pairs(const double *rows, size_t nrows, size_t len, const double *x,
+                    const int *pair_fast, const int *pair_slow, size_t npairs,
+                    long long *stats, double *pnl, int nthreads) {
+    (void)nrows;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16) if (nthreads != 1)
+#endif
+    for (long p = 0; p < (long)npairs; ++p) {
+        const double *f = rows + (size_t)pair_fast[p] * len;
+        const double *s = rows + (size_t)pair_slow[p] * len;
+        size_t w = 0;
+        while (w < len && (isnan(f[w]) || isnan(s[w]))) ++w;
+        long long tr = 0, nl = 0, ns = 0, ck = 0;
+        double pn = 0.0;
+        int prev = 0;
+        for (size_t t = w; t < len; ++t) {
+            double d = f[t] - s[t];
+            int pos = d > 0.0 ? 1 : (d < 0.0 ? -1 : 0);
+            if (t > w && pos != prev) ++tr;
+            if (pos > 0) ++nl;
+            if (pos < 0) ++ns;
+            ck += (long long)pos * (long long)(t + 1);
+            if (t + 1 < len) pn += (double)pos * (x[t + 1] - x[t]);
+            prev = pos;
+        }
+        stats[p * 4 + 0] = tr;
+        stats[p * 4 + 1] = nl;
+        stats[p * 4 + 2] = ns;
+        stats[p * 4 + 3] = ck;
+        pnl[p] = pn;
     }
 }

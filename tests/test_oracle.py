@@ -302,3 +302,36 @@ def test_oracle_error_codes():
         O.alma(np.arange(50.0), 9, 1.5, 6.0)
     with pytest.raises(O.OracleError):
         O.pnr(np.arange(50.0), 5, 101.0)
+
+
+def test_cross_pairs_vectorised_and_antisymmetric():
+    """or_cross_pairs against an independent vectorised numpy formulation of the
+    same statistics, plus the identities fast<->slow swap (long<->short,
+    checksum and pnl negated), n_long + n_short <= bars, and a pair of identical
+    rows giving all zeros."""
+    from vectorta_amd import datagen
+    x = datagen.gbm_close(4_000, 0.015, seed=61).astype(np.float64)
+    combos = [(p, 0.85, 6.0) for p in (5, 9, 20, 33, 50)] + [(20, 0.5, 3.0)]
+    rows = O.alma_batch(x, combos)
+    n = len(combos)
+    pf = [i for i in range(n) for j in range(n)]
+    ps = [j for i in range(n) for j in range(n)]
+    st, pnl = O.cross_pairs(rows, x, pf, ps)
+    for k, (i, j) in enumerate(zip(pf, ps)):
+        d = rows[i] - rows[j]
+        ok = ~(np.isnan(rows[i]) | np.isnan(rows[j]))
+        w = int(np.argmax(ok))
+        pos = np.sign(d[w:]).astype(np.int64)
+        t = np.arange(w, x.size)
+        assert st[k, 0] == np.count_nonzero(np.diff(pos))
+        assert st[k, 1] == np.count_nonzero(pos > 0)
+        assert st[k, 2] == np.count_nonzero(pos < 0)
+        assert st[k, 3] == int(np.sum(pos * (t + 1)))
+        assert pnl[k] == pytest.approx(float(np.sum(pos[:-1] * np.diff(x[w:]))), rel=1e-12,
+                                       abs=1e-9)
+        assert st[k, 1] + st[k, 2] <= x.size - w
+        kk = j * n + i
+        assert st[kk, 1] == st[k, 2] and st[kk, 2] == st[k, 1] and st[kk, 0] == st[k, 0]
+        assert st[kk, 3] == -st[k, 3] and pnl[kk] == -pnl[k]
+        if i == j:
+            assert (st[k] == 0).all() and pnl[k] == 0.0

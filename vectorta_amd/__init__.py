@@ -7,6 +7,7 @@ it, mirroring the reference's ``src/cuda`` wrapper classes (``CudaAlma`` ...).
 """
 from ._lib import VtaError, lib
 from .alma import AlmaBatchRange, AlmaParams, AlmaSweepPlan, CudaAlma, HipAlma, expand_grid
+from .cross import DoubleAlmaCrossover, DoubleAlmaPlan
 from .device import DeviceArrayF32, DeviceBuffer, HipContext, cuda_available, cuda_device_count
 from .rec import (CudaAdx, CudaAtr, CudaEma, CudaRsi, ManySeriesOhlcPlan,
                   ema_rsi_atr_many_series_dev)
@@ -20,5 +21,5 @@ __all__ = [
     "cuda_device_count", "CudaEma", "CudaRsi", "CudaAtr", "CudaAdx", "ManySeriesOhlcPlan",
     "ema_rsi_atr_many_series_dev", "BollingerBandsBatchRange", "CudaBollingerBands",
     "CudaMediumAd", "CudaPercentileNearestRank", "PercentileNearestRankBatchRange",
-    "RollingManySeriesPlan",
+    "RollingManySeriesPlan", "DoubleAlmaCrossover", "DoubleAlmaPlan",
 ]

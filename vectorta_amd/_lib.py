@@ -163,6 +163,15 @@ SIGNATURES = {
     "vta_pnr_many_series_one_param_device": (_ST, [_P, _P, _P, _SZ, _SZ, _SZ, ctypes.c_double,
                                                    _P]),
     "vta_medium_ad_many_series_one_param_device": (_ST, [_P, _P, _P, _SZ, _SZ, _SZ, _P]),
+    "vta_double_alma_crossover": (_ST, [_P, _P, _SZ, ctypes.POINTER(AlmaBatchRangeC),
+                                        ctypes.POINTER(AlmaBatchRangeC), _P, _P, _SZ,
+                                        ctypes.POINTER(_SZ)]),
+    "vta_double_alma_plan_create": (_ST, [_P, _P, _SZ, ctypes.POINTER(AlmaBatchRangeC),
+                                          ctypes.POINTER(AlmaBatchRangeC), ctypes.POINTER(_P)]),
+    "vta_double_alma_plan_pairs": (_SZ, [_P]),
+    "vta_double_alma_plan_launch": (_ST, [_P]),
+    "vta_double_alma_plan_results": (_ST, [_P, _P, _P, _SZ]),
+    "vta_double_alma_plan_destroy": (_ST, [_P]),
     "vta_medium_ad_expand_grid": (_ST, [ctypes.POINTER(RangeUsize), _P, _SZ,
                                         ctypes.POINTER(_SZ)]),
     "vta_medium_ad_batch_dev": (_ST, [_P, _P, _SZ, ctypes.POINTER(RangeUsize),

@@ -206,7 +206,7 @@ struct alma_mfg_args_host {
     int n_groups;
     int n_tiles;
     int kmax;
-    int pad_;
+    int group_base;
 };
 static_assert(sizeof(alma_mfg_args_host) == 88, "alma_mfg_args layout");
 
@@ -221,6 +221,7 @@ struct vta_alma_sweep_plan {
     DevBuf frags, nblocks, groups, sorted_period, sorted_row, w_f32;
     int n_nblocks = 0;
     int n_groups = 0;
+    int n_groups_klo = 0;   // leading groups with K <= 128
     int kmax = 0;
 };
 
@@ -313,6 +314,9 @@ vta_status build_mfma_tables(vta_alma_sweep_plan *plan,
     }
     VTA_TRY(upload(ctx, plan->groups, grp.data(), grp.size() * sizeof(int32_t)));
     plan->n_groups = (int)(grp.size() / 4);
+    plan->n_groups_klo = 0;
+    while (plan->n_groups_klo < plan->n_groups && grp[plan->n_groups_klo * 4 + 2] <= 8)
+        ++plan->n_groups_klo;
     VTA_TRY(upload(ctx, plan->frags, frags.data(), frags.size() * sizeof(uint16_t)));
     VTA_TRY(upload(ctx, plan->nblocks, nbd.data(), nbd.size() * sizeof(int32_t)));
     VTA_TRY(upload(ctx, plan->sorted_period, sp.data(), sp.size() * sizeof(int32_t)));
@@ -377,10 +381,13 @@ vta_status plan_launch(vta_ctx *ctx, const vta_alma_sweep_plan *plan, const floa
             int v = atoi(e);
             if (v == 8 || v == 16) tiles = v;
         }
-        hipFunction_t f;
-        VTA_TRY(get_function(ctx, "alma",
-                             tiles == 8 ? "alma_batch_mfma_bf16x3_wg8" : "alma_batch_mfma_bf16x3_wg16",
-                             &f));
+        // ALMA_MF_TJ=2: two tiles per pass; ALMA_MF_KSPLIT=1: separate launches
+        // for K > 128 and K <= 128 (tuning knobs)
+        const char *tj_e = getenv("ALMA_MF_TJ");
+        const char *ks_e = getenv("ALMA_MF_KSPLIT");
+        const bool tj2 = tiles == 16 && tj_e && atoi(tj_e) == 2;
+        const bool ksplit = tiles == 16 && ks_e && atoi(ks_e) == 1 && plan->n_groups_klo > 0 &&
+                            plan->n_groups_klo < plan->n_groups;
         alma_mfg_args_host a{};
         a.prices = d_prices;
         a.frags = plan->frags.p;
@@ -392,7 +399,6 @@ vta_status plan_launch(vta_ctx *ctx, const vta_alma_sweep_plan *plan, const floa
         a.out = d_out;
         a.series_len = T;
         a.first_valid = (int)fv;
-        a.n_groups = plan->n_groups;
         a.n_tiles = (T + 31) / 32;
         a.kmax = plan->kmax;
         const int npair = plan->kmax / 2 + 15;
@@ -403,9 +409,31 @@ vta_status plan_launch(vta_ctx *ctx, const vta_alma_sweep_plan *plan, const floa
         if (smem > ctx->props.sharedMemPerBlock)
             return fail(VTA_ERR_LAUNCH_CONFIG_TOO_LARGE, "MFMA staging needs %zu bytes of LDS",
                         smem);
-        dim3 grid((a.n_tiles + tiles - 1) / tiles, plan->n_groups, 1);
+        const unsigned gx = (unsigned)((a.n_tiles + tiles - 1) / tiles);
         ctx->alma_selected = VTA_ALMA_BATCH_MFMA_BF16X3;
-        return launch(ctx, f, grid, dim3(256), (unsigned)smem, a);
+        struct Part { const char *name; int base, count; };
+        Part parts[2];
+        int np = 0;
+        if (ksplit) {
+            parts[np++] = {tj2 ? "alma_batch_mfma_bf16x3_wg16_t2" : "alma_batch_mfma_bf16x3_wg16",
+                           plan->n_groups_klo, plan->n_groups - plan->n_groups_klo};
+            parts[np++] = {tj2 ? "alma_batch_mfma_bf16x3_wg16_t2_klo" : "alma_batch_mfma_bf16x3_wg16_klo",
+                           0, plan->n_groups_klo};
+        } else {
+            parts[np++] = {tiles == 8 ? "alma_batch_mfma_bf16x3_wg8"
+                                      : (tj2 ? "alma_batch_mfma_bf16x3_wg16_t2"
+                                             : "alma_batch_mfma_bf16x3_wg16"),
+                           0, plan->n_groups};
+        }
+        for (int i = 0; i < np; ++i) {
+            hipFunction_t f;
+            VTA_TRY(get_function(ctx, "alma", parts[i].name, &f));
+            a.group_base = parts[i].base;
+            a.n_groups = parts[i].count;
+            VTA_TRY(launch(ctx, f, dim3(gx, (unsigned)parts[i].count, 1), dim3(256),
+                           (unsigned)smem, a));
+        }
+        return VTA_OK;
     }
     if (plan->kernel == VTA_ALMA_BATCH_MFMA_BF16X3) {
         // tiles per wave: 16 by default; ALMA_MF_TILES=8|32 overrides (tuning knob,
@@ -479,6 +507,15 @@ vta_status prepare_host(const float *prices, size_t len, const vta_alma_batch_ra
 }
 
 }  // namespace
+
+namespace vta {
+
+vta_status alma_expand_range(const vta_alma_batch_range *range,
+                             std::vector<vta_alma_params> &out) {
+    return expand(range, out);
+}
+
+}  // namespace vta
 
 extern "C" {
 

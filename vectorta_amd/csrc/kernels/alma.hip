@@ -515,10 +515,10 @@ struct alma_mfg_args {
     float *out;
     int series_len;
     int first_valid;
-    int n_groups;
+    int n_groups;                // groups of this launch: group_base .. group_base + n_groups
     int n_tiles;
     int kmax;
-    int pad_;
+    int group_base;
 };
 
 __device__ __forceinline__ int mfg_stride(int npair) { return ((npair + 15) >> 5) * 32 + 16; }
@@ -627,17 +627,131 @@ __device__ __forceinline__ void alma_mfg_wave(const alma_mfg_args &a, int nb, in
     }
 }
 
+// Two tiles per pass: the two accumulators are independent MFMA chains, so the
+// matrix core always has a second chain to issue while one waits, and the two
+// epilogues put eight stores in flight.  An odd last tile recomputes tile
+// nt-1 as its partner and skips the partner's stores.
+template <int KS, int R>
+__device__ __forceinline__ void alma_mfg_wave2(const alma_mfg_args &a, int nb, int tile_begin,
+                                               int nt, int kg, int stride,
+                                               const uint32_t *stage, const float *anch,
+                                               const int *badf) {
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 5;
+    const int m = lane & 31;
+    const int4 nbd = a.nblocks[nb];
+    const int frag_off = nbd.y;
+    const int combo_base = nbd.z;
+    constexpr int K = KS * 16;
+    uint4 bhi[KS], blo[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        bhi[s] = a.frags[frag_off + (2 * s + 0) * 64 + lane];
+        blo[s] = a.frags[frag_off + (2 * s + 1) * 64 + lane];
+    }
+    const int T = a.series_len;
+    const int my_period = a.sorted_period[combo_base + m];
+    const int my_row = a.sorted_row[combo_base + m];
+    const int my_warm = a.first_valid + my_period - 1;
+    int warm_max = my_warm;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) warm_max = max(warm_max, __shfl_xor(warm_max, o));
+    warm_max = __builtin_amdgcn_readfirstlane(warm_max);
+    const int r = m & 1;
+    const int d0 = (m >> 1) + 4 * g + ((kg - K) >> 1);
+    const int hi_off = (r ? stride : 0) + d0;
+    const int lo_off = 2 * stride + hi_off;
+    float *row = a.out + (size_t)(my_row < 0 ? 0 : my_row) * (size_t)T;
+    auto load_a = [&](int j, int s, uint4 &h, uint4 &l) {
+        const uint32_t *hb = stage + j * 4 * stride + hi_off + 8 * s;
+        const uint32_t *lb = stage + j * 4 * stride + lo_off + 8 * s;
+        h.x = hb[0]; h.y = hb[1]; h.z = hb[2]; h.w = hb[3];
+        l.x = lb[0]; l.y = lb[1]; l.z = lb[2]; l.w = lb[3];
+    };
+    auto epilogue = [&](const vta_f32x16 &acc, int j) {
+        const float anchor = anch[j];
+        const int t_tile = (tile_begin + j) * 32;
+        if (badf[j] == 0 && t_tile >= warm_max && t_tile + 32 <= T && (T & 3) == 0) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int tb = t_tile + 8 * q + 4 * g;
+                *reinterpret_cast<float4 *>(row + tb) =
+                    make_float4(acc[4 * q + 0] + anchor, acc[4 * q + 1] + anchor,
+                                acc[4 * q + 2] + anchor, acc[4 * q + 3] + anchor);
+            }
+            return;
+        }
+        const bool tile_bad = badf[j] != 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int tb = t_tile + 8 * q + 4 * g;
+            float v[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int t = tb + i;
+                float val = acc[4 * q + i] + anchor;
+                if (tile_bad && t >= my_warm && t < T) {
+                    const float *wrow = a.w_f32 + (size_t)(combo_base + m) * a.kmax;
+                    float dot = 0.0f;
+                    for (int k = 0; k < my_period; ++k)
+                        dot = fmaf(wrow[k], a.prices[t - my_period + 1 + k], dot);
+                    val = dot;
+                }
+                v[i] = (t >= my_warm) ? val : VTA_QNAN_F32;
+            }
+            if (((T & 3) == 0) && tb + 3 < T) {
+                *reinterpret_cast<float4 *>(row + tb) = make_float4(v[0], v[1], v[2], v[3]);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (tb + i < T) row[tb + i] = v[i];
+            }
+        }
+    };
+    uint4 ah0_n, al0_n, ah1_n, al1_n;
+    load_a(0, 0, ah0_n, al0_n);
+    load_a(min(1, nt - 1), 0, ah1_n, al1_n);
+    for (int j = 0; j < nt; j += 2) {
+        const int j1 = min(j + 1, nt - 1);
+        vta_f32x16 acc0 = {};
+        vta_f32x16 acc1 = {};
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const uint4 ah0 = ah0_n, al0 = al0_n, ah1 = ah1_n, al1 = al1_n;
+            if (s + 1 < KS) {
+                load_a(j, s + 1, ah0_n, al0_n);
+                load_a(j1, s + 1, ah1_n, al1_n);
+            }
+            const vta_bf16x8 bh = __builtin_bit_cast(vta_bf16x8, bhi[s]);
+            const vta_bf16x8 bl = __builtin_bit_cast(vta_bf16x8, blo[s]);
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(vta_bf16x8, ah0), bh, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(vta_bf16x8, ah1), bh, acc1, 0, 0, 0);
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(vta_bf16x8, ah0), bl, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(vta_bf16x8, ah1), bl, acc1, 0, 0, 0);
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(vta_bf16x8, al0), bh, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(vta_bf16x8, al1), bh, acc1, 0, 0, 0);
+        }
+        if (j + 2 < nt) {
+            load_a(j + 2, 0, ah0_n, al0_n);
+            load_a(min(j + 3, nt - 1), 0, ah1_n, al1_n);
+        }
+        if (my_row < 0) continue;
+        epilogue(acc0, j);
+        if (j1 != j) epilogue(acc1, j1);
+    }
+}
+
 // grid.x = tile runs of R tiles, grid.y = N-block groups; block = 256 (4 waves).
 // Dynamic LDS (dwords): XSN = 32R + kmax + 32 prices, R anchors, R flags, then
 // R * 4 * mfg_stride(kmax/2 + 15) staged dwords.
-template <int R>
+template <int R, int TJ, int KSLO, int KSHI>
 __device__ __forceinline__ void alma_mfg_entry(const alma_mfg_args &a) {
     extern __shared__ __attribute__((aligned(16))) uint32_t mfg_lds[];
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
     // groups are sorted by K ascending; dispatch the heaviest first so the
     // last workgroups to start are the short ones
-    const int4 gd = a.groups[a.n_groups - 1 - (int)blockIdx.y];
+    const int4 gd = a.groups[a.group_base + a.n_groups - 1 - (int)blockIdx.y];
     const int kg = gd.z * 16;
     const int npair = kg / 2 + 15;
     const int stride = mfg_stride(npair);
@@ -696,9 +810,13 @@ __device__ __forceinline__ void alma_mfg_entry(const alma_mfg_args &a) {
     if (wave >= gd.y) return;
     const int nb = gd.x + wave;
     switch (a.nblocks[nb].x) {
-#define MFG_CASE(N)                                                                    \
-    case N:                                                                            \
-        alma_mfg_wave<N, R>(a, nb, tile_begin, nt, kg, stride, stage, anch, badf);     \
+#define MFG_CASE(N)                                                                        \
+    case N:                                                                                \
+        if (N < KSLO || N > KSHI) break;                                                   \
+        if (TJ == 2)                                                                       \
+            alma_mfg_wave2<N, R>(a, nb, tile_begin, nt, kg, stride, stage, anch, badf);    \
+        else                                                                               \
+            alma_mfg_wave<N, R>(a, nb, tile_begin, nt, kg, stride, stage, anch, badf);     \
         break;
         MFG_CASE(1) MFG_CASE(2) MFG_CASE(3) MFG_CASE(4) MFG_CASE(5) MFG_CASE(6) MFG_CASE(7)
         MFG_CASE(8) MFG_CASE(9) MFG_CASE(10) MFG_CASE(11) MFG_CASE(12) MFG_CASE(13)
@@ -709,10 +827,23 @@ __device__ __forceinline__ void alma_mfg_entry(const alma_mfg_args &a) {
 }
 
 extern "C" __global__ __launch_bounds__(256) void alma_batch_mfma_bf16x3_wg8(alma_mfg_args a) {
-    alma_mfg_entry<8>(a);
+    alma_mfg_entry<8, 1, 1, 16>(a);
 }
 extern "C" __global__ __launch_bounds__(256) void alma_batch_mfma_bf16x3_wg16(alma_mfg_args a) {
-    alma_mfg_entry<16>(a);
+    alma_mfg_entry<16, 1, 1, 16>(a);
+}
+extern "C" __global__ __launch_bounds__(256) void alma_batch_mfma_bf16x3_wg16_t2(alma_mfg_args a) {
+    alma_mfg_entry<16, 2, 1, 16>(a);
+}
+// Short-K kernels: groups whose largest K is <= 128 can run in a separate
+// launch that is not held to the long-K register budget (the long-K launch
+// uses the full-range kernels above, since its first group may hold short
+// N-blocks too).
+extern "C" __global__ __launch_bounds__(256) void alma_batch_mfma_bf16x3_wg16_klo(alma_mfg_args a) {
+    alma_mfg_entry<16, 1, 1, 8>(a);
+}
+extern "C" __global__ __launch_bounds__(256) void alma_batch_mfma_bf16x3_wg16_t2_klo(alma_mfg_args a) {
+    alma_mfg_entry<16, 2, 1, 8>(a);
 }
 
 // ---------------------------------------------------------------- direct fallbacks

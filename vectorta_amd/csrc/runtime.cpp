@@ -308,6 +308,9 @@ vta_status vta_ctx_preload(vta_ctx *ctx) {
         {"alma", "alma_batch_mfma_bf16x3_r32"},
         {"alma", "alma_batch_mfma_bf16x3_wg8"},
         {"alma", "alma_batch_mfma_bf16x3_wg16"},
+        {"alma", "alma_batch_mfma_bf16x3_wg16_t2"},
+        {"alma", "alma_batch_mfma_bf16x3_wg16_klo"},
+        {"alma", "alma_batch_mfma_bf16x3_wg16_t2_klo"},
         {"alma", "alma_batch_f32_direct"},
         {"alma", "alma_ms1p_f32_direct"},
         {"rec", "ema_batch_f32"},
@@ -328,6 +331,9 @@ vta_status vta_ctx_preload(vta_ctx *ctx) {
         {"rolling", "pnr_batch_f32"},
         {"rolling", "pnr_ms1p_f32"},
         {"rolling", "mad_batch_f32"},
+        {"cross", "cross_rows_f64"},
+        {"cross", "cross_pairs_f64"},
+        {"cross", "cross_reduce"},
         {"rolling", "mad_ms1p_f32"},
     };
     for (auto &s : kSymbols) {

@@ -114,6 +114,10 @@ bool expand_f64(vta_range_f64 r, std::vector<double> &v);
 vta_status host_first_valids(const float *tm, size_t cols, size_t rows, size_t need,
                              std::vector<int32_t> &fvs);
 
+// ALMA grid expansion (alma.cpp), shared with the crossover module.
+vta_status alma_expand_range(const vta_alma_batch_range *range,
+                             std::vector<vta_alma_params> &out);
+
 // bf16 round-to-nearest-even of an f32 (bit pattern).
 uint16_t bf16_rne(float f);
 float bf16_to_f32(uint16_t h);

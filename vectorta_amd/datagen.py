@@ -88,3 +88,9 @@ def many_series_with_jumps(n_series: int = 1024, rows: int = 50_000, seed: int =
     sign = np.where(rng.random(size=(rows, n_series)) < 0.5, -1.0, 1.0)
     p = np.where(mask, p * (1.0 + sign * 5.0 * sd), p)
     return p.astype(np.float32)
+
+
+# Double-ALMA crossover sweep: 220 fast x 265 slow ALMA periods = 58,300 pairs
+# (offset 0.85, sigma 6 -- the ALMA defaults) over the 200,000-bar sweep series.
+DOUBLE_ALMA_FAST = dict(period=(5, 224, 1), offset=(0.85, 0.85, 0.0), sigma=(6.0, 6.0, 0.0))
+DOUBLE_ALMA_SLOW = dict(period=(30, 294, 1), offset=(0.85, 0.85, 0.0), sigma=(6.0, 6.0, 0.0))
