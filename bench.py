@@ -265,6 +265,60 @@ def bench_double_alma(steps, warmup, ev):
             "reference_published_ms": 158.71}
 
 
+def bench_serving(rates=(200, 1000), duration_s=5.0, n_slots=8, seed=6):
+    """Served mix: Poisson arrivals at each rate for duration_s seconds; each
+    request = indicator uniform over {alma, ema, rsi, bollinger_bands, adx},
+    series length log-uniform 1k..1M bars sliced from 8 resident-on-host OHLC
+    random walks, grid size log-uniform 1..2,000 combos, 30% naming an already
+    seen (cached) series.  The client releases each result as soon as it is
+    done.  Latency = completion - submission on the server's clock."""
+    import queue
+    import threading
+    pool = datagen.ohlc_pool(8, 1_000_000, seed=5)
+    out = {"workload": "served_mix_alma_ema_rsi_bb_adx", "n_slots": n_slots, "rates": []}
+    for rate in rates:
+        n = int(rate * duration_s)
+        specs = datagen.served_requests(n, seed=seed + int(rate))
+        rng = np.random.Generator(np.random.PCG64(seed + 7 * int(rate)))
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n))
+        srv = vta.IndicatorServer(0, n_slots=n_slots, hbm_budget=96 << 30)
+        q = queue.Queue()
+        done = []
+
+        def waiter():
+            while True:
+                rid = q.get()
+                if rid is None:
+                    return
+                r = srv.wait(rid)
+                done.append(r)
+                srv.release(rid)
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        for spec, at in zip(specs, arrivals):
+            dt = at - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            q.put(datagen.submit_spec(srv, spec, pool))
+        q.put(None)
+        th.join()
+        st = srv.stats()
+        srv.close()
+        ok = [r for r in done if r.ok]
+        lat = np.array([r.latency_s for r in ok]) * 1e3
+        span = max(r.t_done for r in ok) - min(r.t_submit for r in ok)
+        vals = sum(r.n_values for r in ok)
+        out["rates"].append({
+            "offered_req_per_s": rate, "requests": n, "failed": n - len(ok),
+            "achieved_req_per_s": len(ok) / span, "latency_ms_p50": float(np.percentile(lat, 50)),
+            "latency_ms_p99": float(np.percentile(lat, 99)), "outputs_per_s": vals / span,
+            "peak_in_flight": st["peak_in_flight"], "peak_hbm_in_use_GB": st["peak_in_use"] / 1e9})
+        print(f"[bench] serving {rate} req/s: {out['rates'][-1]}", file=sys.stderr, flush=True)
+    return out
+
+
 def bench_alma_host_boundary(reps=3):
     """The sweep through the host-buffer entry points, wall clock: alma_batch_dev
     (host prices in, device-resident output allocated by the call) and
@@ -328,7 +382,7 @@ def main():
                     help="skip the secondary workloads (many-series OHLC, rolling windows, "
                          "single ALMA)")
     ap.add_argument("--mode", default="sweep",
-                    choices=["sweep", "calib", "ohlc", "rolling", "cross"],
+                    choices=["sweep", "calib", "ohlc", "rolling", "cross", "serve"],
                     help="calib: 256 MiB device-to-device copies, the known-size reference "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
@@ -343,6 +397,9 @@ def main():
     if args.mode == "ohlc":
         r = bench_many_series_ohlc(args.steps, args.warmup, HipEvents())
         print(json.dumps(r))
+        return
+    if args.mode == "serve":
+        print(json.dumps(bench_serving()))
         return
     if args.mode == "cross":
         r = bench_double_alma(args.steps, args.warmup, HipEvents())
@@ -444,6 +501,7 @@ def main():
                          ("rolling_windows", lambda: bench_rolling(5, 1, ev)),
                          ("alma_host_boundary", lambda: bench_alma_host_boundary()),
                          ("double_alma_crossover", lambda: bench_double_alma(5, 1, ev)),
+                         ("serving", lambda: bench_serving()),
                          ("alma_single", lambda: bench_alma_single(20, ev))):
             extra[name] = fn()
             print(f"[bench] {name}: {extra[name]}", file=sys.stderr, flush=True)

@@ -44,7 +44,8 @@ typedef enum vta_status {
     VTA_ERR_DEVICE_MISMATCH = 7,         /* DeviceMismatch{buf, current} */
     VTA_ERR_NOT_IMPLEMENTED = 8,         /* NotImplemented */
     VTA_ERR_ARITHMETIC_OVERFLOW = 9,     /* ArithmeticOverflow{context} */
-    VTA_ERR_NO_DEVICE = 10               /* cuda_available() == false */
+    VTA_ERR_NO_DEVICE = 10,              /* cuda_available() == false */
+    VTA_ERR_TIMEOUT = 11                 /* vta_server_wait: not finished in time */
 } vta_status;
 
 const char *vta_last_error(void);
@@ -247,6 +248,13 @@ vta_status vta_adx_expand_grid(const vta_range_usize *range, size_t *periods, si
 vta_status vta_adx_batch_dev(vta_ctx *ctx, const float *high, const float *low,
                              const float *close, size_t len, const vta_range_usize *range,
                              vta_device_array_f32 *out);
+/* Enqueue-only device form (companion of vta_atr_batch_device; the reference
+ * has no ADX device form).  first_valid: the first bar where high, low and
+ * close are all non-NaN (adx.rs:190-198). */
+vta_status vta_adx_batch_device(vta_ctx *ctx, const float *d_high, const float *d_low,
+                                const float *d_close, const int32_t *d_periods,
+                                int32_t max_period, int32_t series_len, int32_t first_valid,
+                                int32_t n_combos, float *d_out);
 vta_status vta_adx_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *high_tm,
                                                         const float *low_tm,
                                                         const float *close_tm, size_t cols,
@@ -308,6 +316,16 @@ vta_status vta_bb_expand_grid(const vta_bb_batch_range *range, vta_bb_params *ou
 vta_status vta_bollinger_bands_batch_dev(vta_ctx *ctx, const float *prices, size_t len,
                                          const vta_bb_batch_range *range,
                                          vta_device_array_f32 *out3);
+/* Enqueue-only one-series batch on device-resident prices (no reference
+ * counterpart): per-combo d_periods / d_devup / d_devdn, outputs
+ * [combo][series_len] each; first_nonfinite = the first bar at or after
+ * first_valid that is NaN or +-Inf (series_len if none). */
+vta_status vta_bollinger_bands_batch_device(vta_ctx *ctx, const float *d_prices,
+                                            int32_t series_len, int32_t first_valid,
+                                            int32_t first_nonfinite, const int32_t *d_periods,
+                                            const double *d_devup, const double *d_devdn,
+                                            int32_t n_combos, int32_t max_period,
+                                            float *d_upper, float *d_middle, float *d_lower);
 /* bollinger_bands_wrapper.rs:605
  * (bollinger_bands_many_series_one_param_time_major_dev): out3 receives
  * {upper, middle, lower}, each time-major [rows][cols]. */
@@ -402,6 +420,63 @@ vta_status vta_double_alma_plan_launch(vta_double_alma_plan *plan);
 vta_status vta_double_alma_plan_results(vta_double_alma_plan *plan, int64_t *stats4,
                                         double *pnl, size_t cap);
 vta_status vta_double_alma_plan_destroy(vta_double_alma_plan *plan);
+
+/* ------------------------------------------------------------- request server */
+
+/* Continuous-batching request server (no reference counterpart).  n_slots
+ * worker threads, each with its own context and stream on device_id, take
+ * one-series batch requests from a FIFO queue; a request is admitted when a
+ * slot is free and hbm_budget bytes hold its outputs plus, on a series-cache
+ * miss, its series.  Results stay on the device (counted against the budget)
+ * until vta_server_release.  Series are cached under the client's key
+ * (series_key != 0 and cache = 1), evicted least-recently-used when admission
+ * needs room; a request always carries its host data, used on a miss.  Results
+ * are the library's batch kernels on the same inputs: identical whether a
+ * request runs alone or among others. */
+typedef enum vta_req_kind {
+    VTA_REQ_ALMA = 0,   /* alma: grid alma */
+    VTA_REQ_EMA = 1,    /* ema: grid period */
+    VTA_REQ_RSI = 2,    /* rsi: grid period */
+    VTA_REQ_BB = 3,     /* bollinger_bands: grid bb; three outputs */
+    VTA_REQ_ADX = 4     /* adx: grid period; needs high and low */
+} vta_req_kind;
+
+typedef struct vta_request {
+    int32_t kind;                /* vta_req_kind */
+    int32_t cache;               /* keep the series in the device cache */
+    uint64_t series_key;         /* 0: not cacheable */
+    const float *close;          /* host data, copied at submit */
+    const float *high;           /* NULL unless the indicator needs it */
+    const float *low;
+    size_t len;
+    vta_alma_batch_range alma;
+    vta_range_usize period;
+    vta_bb_batch_range bb;
+} vta_request;
+
+typedef struct vta_result {
+    int32_t status;              /* vta_status of the request */
+    int32_t n_outputs;           /* 1, or 3 for Bollinger Bands (upper, middle, lower) */
+    size_t n_combos;
+    vta_device_array_f32 out[3]; /* [combo][len], owned by the server until release */
+    double t_submit, t_start, t_done;   /* seconds on the server's clock */
+} vta_result;
+
+typedef struct vta_server_stats {
+    size_t budget, in_use, peak_in_use;
+    size_t queued, in_flight, peak_in_flight;
+    size_t cached_series;
+} vta_server_stats;
+
+typedef struct vta_server vta_server;
+vta_status vta_server_create(int device_id, int n_slots, size_t hbm_budget, vta_server **out);
+vta_status vta_server_destroy(vta_server *server);
+vta_status vta_server_submit(vta_server *server, const vta_request *req, uint64_t *req_id);
+/* timeout_ms < 0 waits for ever; VTA_ERR_TIMEOUT if not finished in time. */
+vta_status vta_server_wait(vta_server *server, uint64_t req_id, int timeout_ms, vta_result *res);
+vta_status vta_server_release(vta_server *server, uint64_t req_id);
+vta_status vta_server_get_stats(vta_server *server, vta_server_stats *out);
+double vta_server_now(const vta_server *server);
 
 #ifdef __cplusplus
 }

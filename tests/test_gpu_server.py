@@ -1,0 +1,130 @@
+"""Request server: results identical alone and in a concurrent mix, equal to the
+direct entry points, HBM-budget admission and the series cache.
+
+Bar: bit-identical outputs (the served requests run the same kernels on the
+same inputs; the parity of those kernels against the oracle is tested in the
+per-indicator test files)."""
+import numpy as np
+import pytest
+
+import vectorta_amd as vta
+from vectorta_amd import datagen
+from tests.conftest import require_gpu
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pool():
+    require_gpu()
+    return datagen.ohlc_pool(n_series=4, length=120_000, seed=81)
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+def _run_all(server, specs, pool, concurrent):
+    """Returns {index: [host outputs]} and the served results."""
+    outs, results = {}, {}
+    if concurrent:
+        ids = [datagen.submit_spec(server, s, pool) for s in specs]
+        for i, rid in enumerate(ids):
+            r = server.wait(rid)
+            assert r.ok, (i, specs[i], r.status)
+            outs[i] = [server.to_host(r, k) for k in range(r.n_outputs)]
+            results[i] = r
+            server.release(rid)
+    else:
+        for i, s in enumerate(specs):
+            rid = datagen.submit_spec(server, s, pool)
+            r = server.wait(rid)
+            assert r.ok, (i, s, r.status)
+            outs[i] = [server.to_host(r, k) for k in range(r.n_outputs)]
+            results[i] = r
+            server.release(rid)
+    return outs, results
+
+
+def test_mix_equals_alone(pool):
+    specs = datagen.served_requests(40, seed=82, pool_series=4, pool_len=120_000,
+                                    len_range=(2_000, 120_000), grid_range=(1, 60))
+    alone = vta.IndicatorServer(0, n_slots=1, hbm_budget=8 << 30)
+    a_out, _ = _run_all(alone, specs, pool, concurrent=False)
+    alone.close()
+    mix = vta.IndicatorServer(0, n_slots=4, hbm_budget=8 << 30)
+    m_out, m_res = _run_all(mix, specs, pool, concurrent=True)
+    st = mix.stats()
+    mix.close()
+    for i in range(len(specs)):
+        assert len(a_out[i]) == len(m_out[i])
+        for a, m in zip(a_out[i], m_out[i]):
+            assert np.array_equal(_bits(a), _bits(m)), (i, specs[i]["kind"])
+    assert st["peak_in_flight"] >= 2, st          # requests really overlapped
+    assert st["queued"] == 0 and st["in_flight"] == 0
+
+
+def test_served_equals_direct_entry_points(pool):
+    c, h, lo = pool
+    x, hh, ll = c[0, :50_000], h[0, :50_000], lo[0, :50_000]
+    srv = vta.IndicatorServer(0, n_slots=2, hbm_budget=4 << 30)
+    sweep = vta.AlmaBatchRange(period=(5, 60, 5), offset=(0.5, 0.85, 0.35), sigma=(6.0, 6.0, 0.0))
+    bb = vta.BollingerBandsBatchRange(period=(5, 40, 7), devup=(1.5, 2.5, 0.5),
+                                      devdn=(2.0, 2.0, 0.0))
+    cases = [
+        ("alma", dict(alma=sweep), lambda: [vta.CudaAlma(0).alma_batch_dev(x, sweep).to_host()]),
+        ("ema", dict(period=(3, 40, 3)),
+         lambda: [vta.CudaEma(0).ema_batch_dev(x, (3, 40, 3)).to_host()]),
+        ("rsi", dict(period=(2, 30, 4)),
+         lambda: [vta.CudaRsi(0).rsi_batch_dev(x, (2, 30, 4)).to_host()]),
+        ("bollinger_bands", dict(bb=bb),
+         lambda: [d.to_host() for d in vta.CudaBollingerBands(0).bollinger_bands_batch_dev(x, bb)]),
+        ("adx", dict(period=(5, 30, 5)),
+         lambda: [vta.CudaAdx(0).adx_batch_dev(hh, ll, x, (5, 30, 5)).to_host()]),
+    ]
+    for kind, kw, direct in cases:
+        if kind == "adx":
+            rid = srv.submit(kind, x, hh, ll, **kw)
+        else:
+            rid = srv.submit(kind, x, **kw)
+        r = srv.wait(rid)
+        assert r.ok, (kind, r.status)
+        got = [srv.to_host(r, k) for k in range(r.n_outputs)]
+        srv.release(rid)
+        ref = direct()
+        assert len(got) == len(ref)
+        for g, d in zip(got, ref):
+            assert np.array_equal(_bits(g), _bits(d)), kind
+    srv.close()
+
+
+def test_budget_admission_and_cache(pool):
+    c, _, _ = pool
+    x = c[1, :100_000]
+    # each request: 20 EMA rows x 100k bars = 8 MB of outputs; budget 40 MB
+    budget = 40 << 20
+    srv = vta.IndicatorServer(0, n_slots=4, hbm_budget=budget)
+    ids = [srv.submit("ema", x, key=7, cache=True, period=(2, 21, 1)) for _ in range(12)]
+    first = None
+    for rid in ids:
+        r = srv.wait(rid)
+        assert r.ok
+        g = srv.to_host(r)
+        if first is None:
+            first = g
+        assert np.array_equal(_bits(g), _bits(first))
+        srv.release(rid)
+    st = srv.stats()
+    assert st["peak_in_use"] <= budget
+    assert st["cached_series"] == 1
+    # a request that can never fit fails with OutOfMemory instead of blocking
+    rid = srv.submit("ema", x, period=(2, 111, 1))        # 44 MB of outputs > 40 MiB
+    r = srv.wait(rid, timeout_ms=60_000)
+    assert r.status == 2
+    srv.release(rid)
+    # an invalid request reports its error through the result
+    rid = srv.submit("rsi", x[:50], period=(60, 60, 0))
+    r = srv.wait(rid, timeout_ms=60_000)
+    assert r.status == 4
+    srv.release(rid)
+    srv.close()

@@ -11,6 +11,7 @@ from .cross import DoubleAlmaCrossover, DoubleAlmaPlan
 from .device import DeviceArrayF32, DeviceBuffer, HipContext, cuda_available, cuda_device_count
 from .rec import (CudaAdx, CudaAtr, CudaEma, CudaRsi, ManySeriesOhlcPlan,
                   ema_rsi_atr_many_series_dev)
+from .server import IndicatorServer, ServedResult
 from .rolling import (BollingerBandsBatchRange, CudaBollingerBands, CudaMediumAd,
                       CudaPercentileNearestRank, PercentileNearestRankBatchRange,
                       RollingManySeriesPlan)
@@ -21,5 +22,6 @@ __all__ = [
     "cuda_device_count", "CudaEma", "CudaRsi", "CudaAtr", "CudaAdx", "ManySeriesOhlcPlan",
     "ema_rsi_atr_many_series_dev", "BollingerBandsBatchRange", "CudaBollingerBands",
     "CudaMediumAd", "CudaPercentileNearestRank", "PercentileNearestRankBatchRange",
-    "RollingManySeriesPlan", "DoubleAlmaCrossover", "DoubleAlmaPlan",
+    "RollingManySeriesPlan", "DoubleAlmaCrossover", "DoubleAlmaPlan", "IndicatorServer",
+    "ServedResult",
 ]

@@ -15,7 +15,7 @@ VTA_OK = 0
 STATUS_NAMES = {
     0: "Ok", 1: "Hip", 2: "OutOfMemory", 3: "MissingKernelSymbol", 4: "InvalidInput",
     5: "InvalidPolicy", 6: "LaunchConfigTooLarge", 7: "DeviceMismatch", 8: "NotImplemented",
-    9: "ArithmeticOverflow", 10: "NoDevice",
+    9: "ArithmeticOverflow", 10: "NoDevice", 11: "Timeout",
 }
 
 
@@ -64,6 +64,25 @@ class PnrParamsC(ctypes.Structure):
 
 class PnrBatchRangeC(ctypes.Structure):
     _fields_ = [("length", RangeUsize), ("percentage", RangeF64)]
+
+
+class VtaRequestC(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("cache", ctypes.c_int32),
+                ("series_key", ctypes.c_uint64), ("close", ctypes.c_void_p),
+                ("high", ctypes.c_void_p), ("low", ctypes.c_void_p), ("len", ctypes.c_size_t),
+                ("alma", AlmaBatchRangeC), ("period", RangeUsize), ("bb", BbBatchRangeC)]
+
+
+class VtaResultC(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("n_outputs", ctypes.c_int32),
+                ("n_combos", ctypes.c_size_t), ("out", DeviceArrayF32C * 3),
+                ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double),
+                ("t_done", ctypes.c_double)]
+
+
+class VtaServerStatsC(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_size_t) for n in ("budget", "in_use", "peak_in_use", "queued",
+                                                "in_flight", "peak_in_flight", "cached_series")]
 
 
 # Every symbol include/vta.h declares, with (restype, argtypes).
@@ -172,6 +191,16 @@ SIGNATURES = {
     "vta_double_alma_plan_launch": (_ST, [_P]),
     "vta_double_alma_plan_results": (_ST, [_P, _P, _P, _SZ]),
     "vta_double_alma_plan_destroy": (_ST, [_P]),
+    "vta_adx_batch_device": (_ST, [_P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _P]),
+    "vta_bollinger_bands_batch_device": (_ST, [_P, _P, _I32, _I32, _I32, _P, _P, _P, _I32,
+                                               _I32, _P, _P, _P]),
+    "vta_server_create": (_ST, [ctypes.c_int, ctypes.c_int, _SZ, ctypes.POINTER(_P)]),
+    "vta_server_destroy": (_ST, [_P]),
+    "vta_server_submit": (_ST, [_P, ctypes.POINTER(VtaRequestC), ctypes.POINTER(ctypes.c_uint64)]),
+    "vta_server_wait": (_ST, [_P, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(VtaResultC)]),
+    "vta_server_release": (_ST, [_P, ctypes.c_uint64]),
+    "vta_server_get_stats": (_ST, [_P, ctypes.POINTER(VtaServerStatsC)]),
+    "vta_server_now": (ctypes.c_double, [_P]),
     "vta_medium_ad_expand_grid": (_ST, [ctypes.POINTER(RangeUsize), _P, _SZ,
                                         ctypes.POINTER(_SZ)]),
     "vta_medium_ad_batch_dev": (_ST, [_P, _P, _SZ, ctypes.POINTER(RangeUsize),

@@ -371,8 +371,10 @@ vta_status plan_create(vta_ctx *ctx, const std::vector<vta_alma_params> &combos,
 }
 
 vta_status plan_launch(vta_ctx *ctx, const vta_alma_sweep_plan *plan, const float *d_prices,
-                       size_t fv, float *d_out) {
-    const int T = (int)plan->series_len;
+                       size_t fv, float *d_out, size_t series_len = 0) {
+    // the tables do not depend on the series length; a caller may launch a
+    // plan over any length that covers its longest period
+    const int T = (int)(series_len ? series_len : plan->series_len);
     if (plan->kernel == VTA_ALMA_BATCH_MFMA_BF16X3 && mf_shared_staging()) {
         // shared-staging form: 4 N-blocks per workgroup, R tiles per run
         // (ALMA_MF_TILES=8|16, default 16)
@@ -514,6 +516,37 @@ vta_status alma_expand_range(const vta_alma_batch_range *range,
                              std::vector<vta_alma_params> &out) {
     return expand(range, out);
 }
+
+vta_status alma_grid_plan_create(vta_ctx *ctx, const vta_alma_batch_range *range,
+                                 vta_alma_sweep_plan **out, size_t *max_period,
+                                 size_t *n_combos) {
+    std::vector<vta_alma_params> combos;
+    VTA_TRY(expand(range, combos));
+    size_t mp = 0;
+    for (auto &c : combos) {
+        if (c.period == 0 || !(c.sigma > 0.0) || !(c.offset >= 0.0 && c.offset <= 1.0))
+            return fail(VTA_ERR_INVALID_INPUT, "invalid params: period=%zu, offset=%g, sigma=%g",
+                        c.period, c.offset, c.sigma);
+        mp = std::max(mp, c.period);
+    }
+    if (mp > (size_t)INT32_MAX) return fail(VTA_ERR_INVALID_INPUT, "period too large");
+    *max_period = mp;
+    *n_combos = combos.size();
+    vta_alma_batch_kernel k = resolve_kernel(ctx->alma_policy, combos.size(), mp);
+    return plan_create(ctx, combos, mp, mp, k, out);
+}
+
+vta_status alma_grid_plan_launch(vta_ctx *ctx, const vta_alma_sweep_plan *plan,
+                                 const float *d_prices, size_t series_len, size_t fv,
+                                 float *d_out) {
+    if (series_len == 0 || series_len > (size_t)INT32_MAX || fv >= series_len ||
+        series_len - fv < plan->max_period)
+        return fail(VTA_ERR_INVALID_INPUT, "not enough valid data (needed >= %zu)",
+                    plan->max_period);
+    return plan_launch(ctx, plan, d_prices, fv, d_out, series_len);
+}
+
+void alma_grid_plan_destroy(vta_alma_sweep_plan *plan) { delete plan; }
 
 }  // namespace vta
 

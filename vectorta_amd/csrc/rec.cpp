@@ -413,6 +413,13 @@ vta_status vta_adx_batch_dev(vta_ctx *ctx, const float *high, const float *low,
                              vta_device_array_f32 *out) {
     return batch_dev(ctx, IND_ADX, close, high, low, len, range, out);
 }
+vta_status vta_adx_batch_device(vta_ctx *ctx, const float *d_high, const float *d_low,
+                                const float *d_close, const int32_t *d_periods,
+                                int32_t max_period, int32_t series_len, int32_t first_valid,
+                                int32_t n_combos, float *d_out) {
+    return batch_device(ctx, IND_ADX, d_close, d_high, d_low, d_periods, max_period, series_len,
+                        first_valid, n_combos, d_out);
+}
 vta_status vta_adx_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *high_tm,
                                                         const float *low_tm,
                                                         const float *close_tm, size_t cols,

@@ -282,6 +282,42 @@ vta_status vta_bollinger_bands_batch_dev(vta_ctx *ctx, const float *prices, size
     return batch_dev(ctx, R_BB, prices, len, cs, 3, out3);
 }
 
+vta_status vta_bollinger_bands_batch_device(vta_ctx *ctx, const float *d_prices,
+                                            int32_t series_len, int32_t first_valid,
+                                            int32_t first_nonfinite, const int32_t *d_periods,
+                                            const double *d_devup, const double *d_devdn,
+                                            int32_t n_combos, int32_t max_period,
+                                            float *d_upper, float *d_middle, float *d_lower) {
+    clear_error();
+    if (!ctx || !d_prices || !d_periods || !d_devup || !d_devdn || !d_upper || !d_middle ||
+        !d_lower)
+        return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    if (series_len <= 0 || n_combos <= 0 || max_period <= 0 || first_valid < 0 ||
+        first_valid >= series_len || first_nonfinite < first_valid ||
+        first_nonfinite > series_len || series_len - first_valid < max_period)
+        return fail(VTA_ERR_INVALID_INPUT, "invalid sizes");
+    // per-call metadata {first_valid, first_nonfinite} in the context scratch,
+    // written by a fill kernel in stream order
+    int32_t *meta = static_cast<int32_t *>(ctx->scratch) + 128;
+    hipFunction_t fill;
+    VTA_TRY(get_function(ctx, "rec", "vta_fill_i32", &fill));
+    VTA_TRY(launch(ctx, fill, dim3(1), dim3(256), 0, meta, 1, (int)first_valid));
+    VTA_TRY(launch(ctx, fill, dim3(1), dim3(256), 0, meta + 1, 1, (int)first_nonfinite));
+    roll_args_host a{};
+    a.x = d_prices;
+    a.p_len = d_periods;
+    a.p_a = d_devup;
+    a.p_b = d_devdn;
+    a.fvs = meta;
+    a.poison = meta + 1;
+    a.out0 = d_upper;
+    a.out1 = d_middle;
+    a.out2 = d_lower;
+    a.T = series_len;
+    a.nlanes = n_combos;
+    return roll_launch(ctx, R_BB, true, a, (size_t)max_period, true);
+}
+
 vta_status vta_bollinger_bands_many_series_one_param_time_major_dev(
     vta_ctx *ctx, const float *data_tm, size_t cols, size_t rows, size_t period, double devup,
     double devdn, vta_device_array_f32 *out3) {

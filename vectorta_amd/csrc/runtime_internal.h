@@ -117,6 +117,16 @@ vta_status host_first_valids(const float *tm, size_t cols, size_t rows, size_t n
 // ALMA grid expansion (alma.cpp), shared with the crossover module.
 vta_status alma_expand_range(const vta_alma_batch_range *range,
                              std::vector<vta_alma_params> &out);
+// A sweep plan for a grid, independent of the series length (the server's
+// per-grid table cache): create once, launch over any long-enough series
+// (enqueue only), destroy without a context.
+vta_status alma_grid_plan_create(vta_ctx *ctx, const vta_alma_batch_range *range,
+                                 vta_alma_sweep_plan **out, size_t *max_period,
+                                 size_t *n_combos);
+vta_status alma_grid_plan_launch(vta_ctx *ctx, const vta_alma_sweep_plan *plan,
+                                 const float *d_prices, size_t series_len, size_t fv,
+                                 float *d_out);
+void alma_grid_plan_destroy(vta_alma_sweep_plan *plan);
 
 // bf16 round-to-nearest-even of an f32 (bit pattern).
 uint16_t bf16_rne(float f);

@@ -1,0 +1,617 @@
+// Request server: continuous batching of indicator requests on one device.
+//
+// No reference counterpart (the reference serves one call at a time through
+// its wrappers).  Requests (ALMA, EMA, RSI, Bollinger Bands, ADX one-series
+// batches) enter a FIFO queue.  Each of n_slots worker threads owns a vta_ctx
+// (its own stream), so admitted requests run concurrently on the device.  A
+// request is admitted when a slot is free and the HBM budget holds its
+// outputs plus, on a cache miss, its series upload; outputs stay reserved
+// until the client releases them.  Series live in a device cache keyed by the
+// client's 64-bit key; unreferenced entries are evicted least-recently-used
+// when admission needs room.  The kernels are the library's public entry
+// points, so a result is bit-identical whether the request runs alone or in
+// a mix.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <list>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "runtime_internal.h"
+
+using namespace vta;
+
+namespace {
+
+double now_s() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+// Host-side metadata of a series (computed at submit time).
+struct SeriesMeta {
+    size_t len = 0;
+    bool hlc = false;
+    int32_t fv_close = 0;   // first non-NaN close
+    int32_t z_close = 0;    // first non-finite close at or after fv_close (len if none)
+    int32_t fv_adx = 0;     // max of the three first non-NaN indices (adx.rs:190-198)
+};
+
+struct CacheEntry {
+    uint64_t key = 0;
+    SeriesMeta meta;
+    float *d_close = nullptr, *d_high = nullptr, *d_low = nullptr;
+    size_t bytes = 0;
+    int refs = 0;
+    bool ready = false;
+    bool cached = false;               // false: freed when its request completes
+    std::list<uint64_t>::iterator lru;
+};
+
+struct Req {
+    uint64_t id = 0;
+    vta_request r{};
+    std::vector<float> close, high, low;   // host copies
+    SeriesMeta meta;
+    size_t out_bytes = 0;                  // reserved for outputs
+    size_t series_bytes = 0;               // device bytes of the series
+    size_t series_reserved = 0;            // reserved at admission (misses only)
+    bool uploader = false;                 // this request uploads the series
+    std::shared_ptr<CacheEntry> entry;
+    vta_result res{};
+    bool done = false;
+};
+
+size_t first_nan_end(const float *x, size_t n) {
+    size_t i = 0;
+    while (i < n && std::isnan(x[i])) ++i;
+    return i;
+}
+
+}  // namespace
+
+struct vta_server {
+    int device = 0;
+    size_t budget = 0;
+    size_t in_use = 0;          // outputs of live results + cache + pending uploads
+    size_t peak_in_use = 0;
+    int in_flight = 0, peak_in_flight = 0;
+    bool stop = false;
+    uint64_t next_id = 1;
+    std::mutex mu;
+    std::condition_variable cv_admit;   // queue / budget / shutdown changes
+    std::condition_variable cv_done;    // request completions, entry readiness
+    std::deque<std::shared_ptr<Req>> queue;
+    std::unordered_map<uint64_t, std::shared_ptr<Req>> reqs;
+    std::unordered_map<uint64_t, std::shared_ptr<CacheEntry>> cache;
+    std::list<uint64_t> lru;            // front = least recently used
+    std::vector<vta_ctx *> ctxs;
+    std::vector<std::thread> workers;
+    hipStream_t free_stream = nullptr;  // releases and evictions
+    double t0 = 0.0;
+    // ALMA grid -> sweep tables (guarded by plan_mu; entries live until destroy)
+    std::mutex plan_mu;
+    std::unordered_map<std::string, vta_alma_sweep_plan *> alma_plans;
+    std::unordered_map<std::string, std::pair<size_t, size_t>> alma_plan_dims;
+};
+
+namespace {
+
+size_t grid_outputs(const vta_request &r, size_t *n_combos, int *n_out) {
+    *n_out = 1;
+    std::vector<size_t> ps;
+    switch (r.kind) {
+        case VTA_REQ_ALMA: {
+            size_t n = 0;
+            if (vta_alma_expand_grid(&r.alma, nullptr, 0, &n) != VTA_OK) return 0;
+            *n_combos = n;
+            return n;
+        }
+        case VTA_REQ_EMA:
+        case VTA_REQ_RSI:
+        case VTA_REQ_ADX: {
+            size_t n = 0;
+            vta_status st = r.kind == VTA_REQ_EMA   ? vta_ema_expand_grid(&r.period, nullptr, 0, &n)
+                            : r.kind == VTA_REQ_RSI ? vta_rsi_expand_grid(&r.period, nullptr, 0, &n)
+                                                    : vta_adx_expand_grid(&r.period, nullptr, 0, &n);
+            if (st != VTA_OK) return 0;
+            *n_combos = n;
+            return n;
+        }
+        case VTA_REQ_BB: {
+            size_t n = 0;
+            if (vta_bb_expand_grid(&r.bb, nullptr, 0, &n) != VTA_OK) return 0;
+            *n_combos = n;
+            *n_out = 3;
+            return 3 * n;
+        }
+        default:
+            return 0;
+    }
+}
+
+void evict_lru(vta_server *s, size_t need) {
+    auto it = s->lru.begin();
+    while (s->in_use + need > s->budget && it != s->lru.end()) {
+        auto e = s->cache.find(*it);
+        if (e == s->cache.end() || e->second->refs > 0 || !e->second->ready) {
+            ++it;
+            continue;
+        }
+        std::shared_ptr<CacheEntry> ent = e->second;
+        (void)hipFreeAsync(ent->d_close, s->free_stream);
+        if (ent->d_high) (void)hipFreeAsync(ent->d_high, s->free_stream);
+        if (ent->d_low) (void)hipFreeAsync(ent->d_low, s->free_stream);
+        s->in_use -= ent->bytes;
+        s->cache.erase(e);
+        it = s->lru.erase(it);
+    }
+}
+
+// Called with s->mu held.  Returns the admitted request or null.
+std::shared_ptr<Req> try_admit(vta_server *s) {
+    while (!s->queue.empty()) {
+        std::shared_ptr<Req> q = s->queue.front();
+        auto hit = q->r.series_key ? s->cache.find(q->r.series_key) : s->cache.end();
+        const bool is_hit = hit != s->cache.end();
+        const size_t series = is_hit ? 0 : q->series_bytes;
+        const size_t need = q->out_bytes + series;
+        if (need > s->budget) {
+            // can never fit: fail it now (nothing was reserved for it)
+            s->queue.pop_front();
+            q->out_bytes = 0;
+            q->res.status = VTA_ERR_OUT_OF_MEMORY;
+            q->res.t_start = q->res.t_done = now_s() - s->t0;
+            q->done = true;
+            s->cv_done.notify_all();
+            continue;
+        }
+        if (s->in_use + need > s->budget) evict_lru(s, need);
+        if (s->in_use + need > s->budget) return nullptr;
+        s->queue.pop_front();
+        s->in_use += need;
+        s->peak_in_use = std::max(s->peak_in_use, s->in_use);
+        q->series_reserved = series;
+        q->uploader = !is_hit;
+        if (is_hit) {
+            q->entry = hit->second;
+            q->entry->refs += 1;
+            s->lru.splice(s->lru.end(), s->lru, q->entry->lru);   // most recently used
+        } else {
+            auto e = std::make_shared<CacheEntry>();
+            e->key = q->r.series_key;
+            e->meta = q->meta;
+            e->bytes = q->series_bytes;
+            e->refs = 1;
+            e->cached = q->r.cache && q->r.series_key != 0;
+            if (e->cached) {
+                s->lru.push_back(e->key);
+                e->lru = std::prev(s->lru.end());
+                s->cache.emplace(e->key, e);
+            }
+            q->entry = e;
+        }
+        return q;
+    }
+    return nullptr;
+}
+
+vta_status upload_series(vta_ctx *ctx, Req &q) {
+    CacheEntry &e = *q.entry;
+    const size_t n = q.meta.len * sizeof(float);
+    VTA_HIP(hipMallocAsync(reinterpret_cast<void **>(&e.d_close), n, ctx->stream));
+    VTA_HIP(hipMemcpyAsync(e.d_close, q.close.data(), n, hipMemcpyHostToDevice, ctx->stream));
+    if (q.meta.hlc) {
+        VTA_HIP(hipMallocAsync(reinterpret_cast<void **>(&e.d_high), n, ctx->stream));
+        VTA_HIP(hipMallocAsync(reinterpret_cast<void **>(&e.d_low), n, ctx->stream));
+        VTA_HIP(hipMemcpyAsync(e.d_high, q.high.data(), n, hipMemcpyHostToDevice, ctx->stream));
+        VTA_HIP(hipMemcpyAsync(e.d_low, q.low.data(), n, hipMemcpyHostToDevice, ctx->stream));
+    }
+    VTA_HIP(hipStreamSynchronize(ctx->stream));
+    return VTA_OK;
+}
+
+// A small stream-ordered device table, freed in stream order after the
+// launches that read it.
+struct AsyncTable {
+    void *p = nullptr;
+    hipStream_t st = nullptr;
+    ~AsyncTable() {
+        if (p) (void)hipFreeAsync(p, st);
+    }
+    template <typename T>
+    vta_status put(vta_ctx *ctx, const std::vector<T> &v) {
+        st = ctx->stream;
+        VTA_HIP(hipMallocAsync(&p, v.size() * sizeof(T), st));
+        VTA_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+        return VTA_OK;
+    }
+    template <typename T>
+    T *as() const {
+        return static_cast<T *>(p);
+    }
+};
+
+vta_status alloc_out(vta_ctx *ctx, vta_device_array_f32 &o, size_t rows, size_t cols) {
+    void *p = nullptr;
+    VTA_HIP(hipMallocAsync(&p, rows * cols * sizeof(float), ctx->stream));
+    o.ptr = static_cast<float *>(p);
+    o.rows = rows;
+    o.cols = cols;
+    return VTA_OK;
+}
+
+vta_status alma_plan_for(vta_server *s, vta_ctx *ctx, const vta_alma_batch_range &r,
+                         vta_alma_sweep_plan **plan, size_t *max_period, size_t *n_combos) {
+    std::string key(reinterpret_cast<const char *>(&r), sizeof(r));
+    std::lock_guard<std::mutex> lk(s->plan_mu);
+    auto it = s->alma_plans.find(key);
+    if (it == s->alma_plans.end()) {
+        vta_alma_sweep_plan *p = nullptr;
+        size_t mp = 0, nc = 0;
+        VTA_TRY(alma_grid_plan_create(ctx, &r, &p, &mp, &nc));
+        it = s->alma_plans.emplace(key, p).first;
+        s->alma_plan_dims.emplace(key, std::make_pair(mp, nc));
+    }
+    *plan = it->second;
+    *max_period = s->alma_plan_dims[key].first;
+    *n_combos = s->alma_plan_dims[key].second;
+    return VTA_OK;
+}
+
+vta_status run_request(vta_server *s, vta_ctx *ctx, Req &q) {
+    CacheEntry &e = *q.entry;
+    const SeriesMeta &m = e.meta;
+    const vta_request &r = q.r;
+    vta_result &res = q.res;
+    const size_t T = m.len;
+    switch (r.kind) {
+        case VTA_REQ_ALMA: {
+            vta_alma_sweep_plan *plan = nullptr;
+            size_t mp = 0, nc = 0;
+            VTA_TRY(alma_plan_for(s, ctx, r.alma, &plan, &mp, &nc));
+            if ((size_t)m.fv_close >= T || T - (size_t)m.fv_close < mp)
+                return fail(VTA_ERR_INVALID_INPUT, "not enough valid data (needed >= %zu)", mp);
+            res.n_outputs = 1;
+            res.n_combos = nc;
+            VTA_TRY(alloc_out(ctx, res.out[0], nc, T));
+            VTA_TRY(alma_grid_plan_launch(ctx, plan, e.d_close, T, (size_t)m.fv_close,
+                                          res.out[0].ptr));
+            VTA_HIP(hipStreamSynchronize(ctx->stream));
+            return VTA_OK;
+        }
+        case VTA_REQ_EMA:
+        case VTA_REQ_RSI:
+        case VTA_REQ_ADX: {
+            size_t n = 0;
+            vta_status (*grid)(const vta_range_usize *, size_t *, size_t, size_t *) =
+                r.kind == VTA_REQ_EMA ? vta_ema_expand_grid
+                : r.kind == VTA_REQ_RSI ? vta_rsi_expand_grid
+                                        : vta_adx_expand_grid;
+            VTA_TRY(grid(&r.period, nullptr, 0, &n));
+            std::vector<size_t> ps(n);
+            VTA_TRY(grid(&r.period, ps.data(), n, &n));
+            if (n == 0 || T > (size_t)INT32_MAX || n > (size_t)INT32_MAX)
+                return fail(VTA_ERR_INVALID_INPUT, "invalid request sizes");
+            const size_t maxp = *std::max_element(ps.begin(), ps.end());
+            const int32_t fv = r.kind == VTA_REQ_ADX ? m.fv_adx : m.fv_close;
+            const size_t need = r.kind == VTA_REQ_EMA   ? maxp
+                                : r.kind == VTA_REQ_RSI ? maxp + 1
+                                                        : 2 * maxp;
+            if (maxp == 0 || (size_t)fv >= T || T - (size_t)fv < need)
+                return fail(VTA_ERR_INVALID_INPUT, "not enough valid data (needed >= %zu)", need);
+            if (r.kind == VTA_REQ_ADX && !m.hlc)
+                return fail(VTA_ERR_INVALID_INPUT, "ADX needs high, low and close");
+            std::vector<int32_t> per(ps.begin(), ps.end());
+            AsyncTable dp;
+            VTA_TRY(dp.put(ctx, per));
+            res.n_outputs = 1;
+            res.n_combos = n;
+            VTA_TRY(alloc_out(ctx, res.out[0], n, T));
+            if (r.kind == VTA_REQ_EMA)
+                VTA_TRY(vta_ema_batch_device(ctx, e.d_close, dp.as<int32_t>(), nullptr, (int32_t)maxp,
+                                             (int32_t)T, fv, (int32_t)n, res.out[0].ptr));
+            else if (r.kind == VTA_REQ_RSI)
+                VTA_TRY(vta_rsi_batch_device(ctx, e.d_close, dp.as<int32_t>(), (int32_t)maxp,
+                                             (int32_t)T, fv, (int32_t)n, res.out[0].ptr));
+            else
+                VTA_TRY(vta_adx_batch_device(ctx, e.d_high, e.d_low, e.d_close, dp.as<int32_t>(),
+                                             (int32_t)maxp, (int32_t)T, fv, (int32_t)n,
+                                             res.out[0].ptr));
+            VTA_HIP(hipStreamSynchronize(ctx->stream));
+            return VTA_OK;
+        }
+        case VTA_REQ_BB: {
+            size_t n = 0;
+            VTA_TRY(vta_bb_expand_grid(&r.bb, nullptr, 0, &n));
+            std::vector<vta_bb_params> cs(n);
+            VTA_TRY(vta_bb_expand_grid(&r.bb, cs.data(), n, &n));
+            if (n == 0 || T > (size_t)INT32_MAX || n > (size_t)INT32_MAX)
+                return fail(VTA_ERR_INVALID_INPUT, "invalid request sizes");
+            std::vector<int32_t> per(n);
+            std::vector<double> up(n), dn(n);
+            size_t maxp = 0;
+            for (size_t i = 0; i < n; ++i) {
+                if (cs[i].period == 0 || cs[i].period > T)
+                    return fail(VTA_ERR_INVALID_INPUT, "invalid period %zu", cs[i].period);
+                per[i] = (int32_t)cs[i].period;
+                up[i] = cs[i].devup;
+                dn[i] = cs[i].devdn;
+                maxp = std::max(maxp, cs[i].period);
+            }
+            if ((size_t)m.fv_close >= T || T - (size_t)m.fv_close < maxp)
+                return fail(VTA_ERR_INVALID_INPUT, "not enough valid data (needed >= %zu)", maxp);
+            AsyncTable dp, du, dd;
+            VTA_TRY(dp.put(ctx, per));
+            VTA_TRY(du.put(ctx, up));
+            VTA_TRY(dd.put(ctx, dn));
+            res.n_outputs = 3;
+            res.n_combos = n;
+            for (int i = 0; i < 3; ++i) VTA_TRY(alloc_out(ctx, res.out[i], n, T));
+            VTA_TRY(vta_bollinger_bands_batch_device(
+                ctx, e.d_close, (int32_t)T, m.fv_close, m.z_close, dp.as<int32_t>(),
+                du.as<double>(), dd.as<double>(), (int32_t)n, (int32_t)maxp, res.out[0].ptr,
+                res.out[1].ptr, res.out[2].ptr));
+            VTA_HIP(hipStreamSynchronize(ctx->stream));
+            return VTA_OK;
+        }
+        default:
+            return fail(VTA_ERR_INVALID_INPUT, "unknown request kind %d", r.kind);
+    }
+}
+
+void free_outputs(vta_result &res, hipStream_t st) {
+    for (int i = 0; i < 3; ++i)
+        if (res.out[i].ptr) {
+            (void)hipFreeAsync(res.out[i].ptr, st);
+            res.out[i].ptr = nullptr;
+        }
+}
+
+void worker(vta_server *s, int slot) {
+    vta_ctx *ctx = s->ctxs[slot];
+    (void)hipSetDevice(s->device);
+    for (;;) {
+        std::shared_ptr<Req> q;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            for (;;) {
+                if (s->stop) return;
+                q = try_admit(s);
+                if (q) break;
+                s->cv_admit.wait(lk);
+            }
+            s->in_flight += 1;
+            s->peak_in_flight = std::max(s->peak_in_flight, s->in_flight);
+            q->res.t_start = now_s() - s->t0;
+        }
+        CacheEntry &e = *q->entry;
+        vta_status st = VTA_OK;
+        if (q->uploader) {
+            st = upload_series(ctx, *q);
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (st != VTA_OK) {
+                if (e.d_close) (void)hipFreeAsync(e.d_close, ctx->stream);
+                if (e.d_high) (void)hipFreeAsync(e.d_high, ctx->stream);
+                if (e.d_low) (void)hipFreeAsync(e.d_low, ctx->stream);
+                e.d_close = e.d_high = e.d_low = nullptr;
+                if (e.cached) {
+                    // drop the broken entry; its bytes go back with this request
+                    s->cache.erase(e.key);
+                    s->lru.erase(e.lru);
+                    e.cached = false;
+                }
+            }
+            e.ready = true;
+            s->cv_done.notify_all();
+        } else {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_done.wait(lk, [&] { return e.ready; });
+        }
+        if (st == VTA_OK && e.d_close) {
+            clear_error();
+            st = run_request(s, ctx, *q);
+        } else if (st == VTA_OK) {
+            st = fail(VTA_ERR_HIP, "series upload failed");
+        }
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            q->res.status = st;
+            q->res.t_done = now_s() - s->t0;
+            if (st != VTA_OK) {
+                (void)hipStreamSynchronize(ctx->stream);
+                free_outputs(q->res, ctx->stream);
+                s->in_use -= q->out_bytes;
+                q->out_bytes = 0;
+            }
+            e.refs -= 1;
+            if (!e.cached) {
+                // a one-off series, or an entry dropped after a failed upload
+                if (e.refs == 0) {
+                    if (e.d_close) (void)hipFreeAsync(e.d_close, ctx->stream);
+                    if (e.d_high) (void)hipFreeAsync(e.d_high, ctx->stream);
+                    if (e.d_low) (void)hipFreeAsync(e.d_low, ctx->stream);
+                    e.d_close = e.d_high = e.d_low = nullptr;
+                }
+                s->in_use -= q->series_reserved;
+                q->series_reserved = 0;
+            }
+            s->in_flight -= 1;
+            q->done = true;
+            s->cv_done.notify_all();
+            s->cv_admit.notify_all();
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+vta_status vta_server_create(int device_id, int n_slots, size_t hbm_budget, vta_server **out) {
+    clear_error();
+    if (!out || n_slots <= 0 || n_slots > 64 || hbm_budget == 0)
+        return fail(VTA_ERR_INVALID_INPUT, "invalid server arguments");
+    auto *s = new vta_server();
+    s->device = device_id;
+    s->budget = hbm_budget;
+    s->t0 = now_s();
+    {
+        hipError_t e = hipSetDevice(device_id);
+        hipMemPool_t pool = nullptr;
+        if (e == hipSuccess) e = hipDeviceGetDefaultMemPool(&pool, device_id);
+        uint64_t keep = UINT64_MAX;   // freed blocks stay in the pool for reuse
+        if (e == hipSuccess) e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->free_stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete s;
+            return fail(VTA_ERR_HIP, "server set-up: %s", hipGetErrorString(e));
+        }
+    }
+    for (int i = 0; i < n_slots; ++i) {
+        vta_ctx *c = nullptr;
+        vta_status st = vta_ctx_create(device_id, &c);
+        if (st == VTA_OK) st = vta_ctx_preload(c);
+        if (st != VTA_OK) {
+            if (c) vta_ctx_destroy(c);
+            for (auto *x : s->ctxs) vta_ctx_destroy(x);
+            delete s;
+            return st;
+        }
+        s->ctxs.push_back(c);
+    }
+    for (int i = 0; i < n_slots; ++i) s->workers.emplace_back(worker, s, i);
+    *out = s;
+    return VTA_OK;
+}
+
+vta_status vta_server_destroy(vta_server *s) {
+    clear_error();
+    if (!s) return VTA_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        s->cv_admit.notify_all();
+        s->cv_done.notify_all();
+    }
+    for (auto &t : s->workers) t.join();
+    (void)hipSetDevice(s->device);
+    for (auto &kv : s->reqs)
+        if (kv.second->done) free_outputs(kv.second->res, s->free_stream);
+    for (auto &kv : s->cache) {
+        (void)hipFreeAsync(kv.second->d_close, s->free_stream);
+        if (kv.second->d_high) (void)hipFreeAsync(kv.second->d_high, s->free_stream);
+        if (kv.second->d_low) (void)hipFreeAsync(kv.second->d_low, s->free_stream);
+    }
+    (void)hipStreamSynchronize(s->free_stream);
+    (void)hipDeviceSynchronize();
+    for (auto &kv : s->alma_plans) alma_grid_plan_destroy(kv.second);
+    for (auto *c : s->ctxs) vta_ctx_destroy(c);
+    (void)hipStreamDestroy(s->free_stream);
+    delete s;
+    return VTA_OK;
+}
+
+vta_status vta_server_submit(vta_server *s, const vta_request *r, uint64_t *req_id) {
+    clear_error();
+    if (!s || !r || !req_id || !r->close || r->len == 0)
+        return fail(VTA_ERR_INVALID_INPUT, "invalid request");
+    if (r->kind == VTA_REQ_ADX && (!r->high || !r->low))
+        return fail(VTA_ERR_INVALID_INPUT, "ADX needs high, low and close");
+    auto q = std::make_shared<Req>();
+    q->r = *r;
+    const size_t n = r->len;
+    q->meta.len = n;
+    q->meta.hlc = r->high && r->low;
+    q->close.assign(r->close, r->close + n);
+    if (q->meta.hlc) {
+        q->high.assign(r->high, r->high + n);
+        q->low.assign(r->low, r->low + n);
+    }
+    q->r.close = q->r.high = q->r.low = nullptr;
+    const size_t fvc = first_nan_end(q->close.data(), n);
+    q->meta.fv_close = (int32_t)std::min(fvc, n);
+    size_t z = fvc;
+    while (z < n && std::isfinite(q->close[z])) ++z;
+    q->meta.z_close = (int32_t)z;
+    if (q->meta.hlc) {
+        size_t a = first_nan_end(q->high.data(), n), b = first_nan_end(q->low.data(), n);
+        q->meta.fv_adx = (int32_t)std::max({a, b, fvc});
+    }
+    size_t n_combos = 0;
+    int n_out = 1;
+    const size_t cells = grid_outputs(*r, &n_combos, &n_out);
+    if (cells == 0) return fail(VTA_ERR_INVALID_INPUT, "invalid parameter grid");
+    if (!mul_ok(cells, n, &q->out_bytes) || !mul_ok(q->out_bytes, sizeof(float), &q->out_bytes))
+        return fail(VTA_ERR_INVALID_INPUT, "output size overflow");
+    q->series_bytes = (q->meta.hlc ? 3 : 1) * n * sizeof(float);
+    std::lock_guard<std::mutex> lk(s->mu);
+    q->id = s->next_id++;
+    q->res.t_submit = now_s() - s->t0;
+    s->reqs.emplace(q->id, q);
+    s->queue.push_back(q);
+    *req_id = q->id;
+    s->cv_admit.notify_one();
+    return VTA_OK;
+}
+
+vta_status vta_server_wait(vta_server *s, uint64_t req_id, int timeout_ms, vta_result *res) {
+    clear_error();
+    if (!s || !res) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(req_id);
+    if (it == s->reqs.end()) return fail(VTA_ERR_INVALID_INPUT, "unknown request %llu",
+                                         (unsigned long long)req_id);
+    std::shared_ptr<Req> q = it->second;
+    auto ready = [&] { return q->done; };
+    if (timeout_ms < 0) {
+        s->cv_done.wait(lk, ready);
+    } else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) {
+        return fail(VTA_ERR_TIMEOUT, "request %llu not finished", (unsigned long long)req_id);
+    }
+    *res = q->res;
+    return VTA_OK;
+}
+
+vta_status vta_server_release(vta_server *s, uint64_t req_id) {
+    clear_error();
+    if (!s) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(req_id);
+    if (it == s->reqs.end()) return fail(VTA_ERR_INVALID_INPUT, "unknown request");
+    std::shared_ptr<Req> q = it->second;
+    if (!q->done) return fail(VTA_ERR_INVALID_INPUT, "request still running");
+    (void)hipSetDevice(s->device);
+    free_outputs(q->res, s->free_stream);
+    s->in_use -= q->out_bytes;
+    q->out_bytes = 0;
+    s->reqs.erase(it);
+    s->cv_admit.notify_all();
+    return VTA_OK;
+}
+
+vta_status vta_server_get_stats(vta_server *s, vta_server_stats *out) {
+    clear_error();
+    if (!s || !out) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    out->budget = s->budget;
+    out->in_use = s->in_use;
+    out->peak_in_use = s->peak_in_use;
+    out->queued = s->queue.size();
+    out->in_flight = (size_t)s->in_flight;
+    out->peak_in_flight = (size_t)s->peak_in_flight;
+    out->cached_series = s->cache.size();
+    return VTA_OK;
+}
+
+double vta_server_now(const vta_server *s) { return s ? now_s() - s->t0 : 0.0; }
+
+}  // extern "C"

@@ -94,3 +94,89 @@ def many_series_with_jumps(n_series: int = 1024, rows: int = 50_000, seed: int =
 # (offset 0.85, sigma 6 -- the ALMA defaults) over the 200,000-bar sweep series.
 DOUBLE_ALMA_FAST = dict(period=(5, 224, 1), offset=(0.85, 0.85, 0.0), sigma=(6.0, 6.0, 0.0))
 DOUBLE_ALMA_SLOW = dict(period=(30, 294, 1), offset=(0.85, 0.85, 0.0), sigma=(6.0, 6.0, 0.0))
+
+
+# ------------------------------------------------------------------ served mix
+
+SERVED_KINDS = ("alma", "ema", "rsi", "bollinger_bands", "adx")
+
+
+def ohlc_pool(n_series: int = 8, length: int = 1_000_000, seed: int = 5):
+    """Base OHLC random walks the served requests slice from: [n_series, length]
+    arrays (close, high, low) of f32."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    c = np.empty((n_series, length), dtype=np.float32)
+    h = np.empty_like(c)
+    lo = np.empty_like(c)
+    for s in range(n_series):
+        cs = gbm_close(length, 0.01, seed * 1000 + s)
+        _, hh, ll, cc = ohlc_from_close(cs, rng)
+        c[s], h[s], lo[s] = cc, hh, ll
+    return c, h, lo
+
+
+def _loguniform_int(rng, lo, hi):
+    return int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+
+
+def served_requests(n: int, seed: int = 6, pool_series: int = 8, pool_len: int = 1_000_000,
+                    len_range=(1_000, 1_000_000), grid_range=(1, 2_000), reuse: float = 0.3):
+    """Request specs of the served mix: indicator uniform over SERVED_KINDS,
+    series length and grid size log-uniform, a fraction `reuse` naming a
+    series already seen (same key, same data).  Each spec is a dict with
+    kind, series (pool index), start, len, key and the parameter grid."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    seen = {False: [], True: []}          # keyed series by "needs high/low"
+    specs = []
+    next_key = 1
+    for _ in range(n):
+        kind = SERVED_KINDS[rng.integers(len(SERVED_KINDS))]
+        hlc = kind == "adx"
+        if seen[hlc] and rng.random() < reuse:
+            base = seen[hlc][rng.integers(len(seen[hlc]))]
+            s, start, L, key = base["series"], base["start"], base["len"], base["key"]
+        else:
+            L = min(_loguniform_int(rng, *len_range), pool_len)
+            s = int(rng.integers(pool_series))
+            start = int(rng.integers(0, pool_len - L + 1))
+            key = next_key
+            next_key += 1
+            seen[hlc].append(dict(series=s, start=start, len=L, key=key))
+        g = _loguniform_int(rng, *grid_range)
+        spec = dict(kind=kind, series=s, start=start, len=L, key=key)
+        if kind == "alma":
+            q = max(1, min(250, (g + 9) // 10, L // 2 - 5))
+            spec["alma"] = dict(period=(5, 5 + q - 1, 1), offset=(0.05, 0.95, 0.1),
+                                sigma=(6.0, 6.0, 0.0)) if g > 10 else \
+                dict(period=(5, 5 + min(g, L // 2 - 5) - 1, 1), offset=(0.85, 0.85, 0.0),
+                     sigma=(6.0, 6.0, 0.0))
+        elif kind in ("ema", "rsi"):
+            q = max(1, min(g, L // 2 - 2))
+            spec["period"] = (2, 2 + q - 1, 1)
+        elif kind == "adx":
+            q = max(1, min(g, L // 4 - 2))
+            spec["period"] = (2, 2 + q - 1, 1)
+        else:
+            q = max(1, min((g + 4) // 5, L // 2 - 5))
+            spec["bb"] = dict(period=(5, 5 + q - 1, 1), devup=(1.0, 3.0, 0.5),
+                              devdn=(2.0, 2.0, 0.0))
+        specs.append(spec)
+    return specs
+
+
+def submit_spec(server, spec, pool):
+    """Submits one spec from served_requests to an IndicatorServer."""
+    from .alma import AlmaBatchRange
+    from .rolling import BollingerBandsBatchRange
+    c, h, lo = pool
+    s, a, L = spec["series"], spec["start"], spec["len"]
+    kw = dict(key=spec["key"], cache=True)
+    if "alma" in spec:
+        kw["alma"] = AlmaBatchRange(**spec["alma"])
+    if "period" in spec:
+        kw["period"] = spec["period"]
+    if "bb" in spec:
+        kw["bb"] = BollingerBandsBatchRange(**spec["bb"])
+    if spec["kind"] == "adx":
+        return server.submit("adx", c[s, a:a + L], h[s, a:a + L], lo[s, a:a + L], **kw)
+    return server.submit(spec["kind"], c[s, a:a + L], **kw)
