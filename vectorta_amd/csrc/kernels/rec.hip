@@ -84,6 +84,16 @@ struct EmaEng {
     __device__ double out_before_start(int t) const { return REC_NAN; }
 };
 
+// num / den for den > 0: an f32 reciprocal estimate refined by one f64 Newton
+// step (relative error ~1e-14 instead of the IEEE division sequence; the
+// outputs are f32).
+__device__ __forceinline__ double rec_ratio(double num, double den) {
+    if (!(den > 1e-30 && den < 1e30)) return num / den;   // outside the f32 estimate's range
+    double r = (double)__builtin_amdgcn_rcpf((float)den);
+    r = fma(fma(-den, r, 1.0), r, r);
+    return num * r;
+}
+
 struct RsiEng {
     static constexpr bool kSeedScan = true;
     double inv_p, beta, ag, al, px;
@@ -126,7 +136,7 @@ struct RsiEng {
         }
         if (poisoned) return REC_NAN;
         double den = ag + al;
-        return den == 0.0 ? 50.0 : 100.0 * ag / den;
+        return den == 0.0 ? 50.0 : rec_ratio(100.0 * ag, den);
     }
 };
 
@@ -216,10 +226,11 @@ struct AdxEng {
         atr = atr * omr + tr;
         pds = pds * omr + pdm;
         mds = mds * omr + mdm;
-        double pdi = 0.0, mdi = 0.0;
-        if (atr != 0.0) { pdi = (pds / atr) * 100.0; mdi = (mds / atr) * 100.0; }
-        double sdi = pdi + mdi;
-        double dx = sdi != 0.0 ? (fabs(pdi - mdi) / sdi) * 100.0 : 0.0;
+        // DX = 100 |DI+ - DI-| / (DI+ + DI-) with DI = 100 DM / ATR: the ATR
+        // cancels, leaving one ratio (zero when ATR or DM+ + DM- is zero, as
+        // adx.rs:370-390 gives)
+        const double sdm = pds + mds;
+        double dx = (atr != 0.0 && sdm != 0.0) ? rec_ratio(100.0 * fabs(pds - mds), sdm) : 0.0;
         if (dx_count < p) {
             dx_sum += dx;
             dx_count += 1;

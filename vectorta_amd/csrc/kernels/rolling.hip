@@ -17,7 +17,9 @@
 // PNR and MAD keep each lane's window sorted in LDS (column layout, stride 64
 // lanes, so every lane hits its own bank): one binary search for the outgoing
 // value, one for the incoming value, and a single shift of the span between
-// them.  NaN bars are kept as +Inf keys and counted, so every step is one
+// them.  These kernels are bound by LDS throughput: batching the shift or
+// replacing the binary search by block counts (fewer dependent reads, more
+// reads in total) measured slower.  NaN bars are kept as +Inf keys and counted, so every step is one
 // replace.  PNR reads its rank among the first (window - NaNs) keys; MAD takes
 // the median and selects from the two monotone deviation sequences either side
 // of it by a two-array order-statistic search.  All comparisons are on the
@@ -68,7 +70,41 @@ struct BbEng {
         o[1] = (float)m;
         o[2] = (float)(m - dn * sd);
     }
+    __device__ void step(double nw, double old) {
+        sum += nw - old;                                    // bollinger_bands.rs:640-641
+        sum_sq = (sum_sq - old * old) + nw * nw;
+    }
+    __device__ void emit(float *o) const {
+        const double m = sum * inv_n;
+        const double var = (sum_sq * inv_n) - (m * m);
+        out(m, var, o);
+    }
 };
+
+// One-series batch outputs are rows [combo][t] and a lane (combo) produces
+// one bar per step, so a wave stages RB_W bars of its 64 rows in LDS and then
+// stores row segments: each store instruction covers two rows x 32
+// consecutive bars instead of 64 scattered dwords.
+#define RB_W 32
+#define RB_LD 33
+
+__device__ __forceinline__ void rb_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void rb_flush(const float *ob, float *out, int row0, int nrows, int T,
+                                         int t0, int nb) {
+    const int lane = threadIdx.x & 63;
+    const int k = lane & 31;
+#pragma unroll 4
+    for (int i = 0; i < 32; ++i) {
+        const int r = 2 * i + (lane >> 5);
+        const int row = row0 + r;
+        if (row < nrows && k < nb) out[(size_t)row * T + t0 + k] = ob[r * RB_LD + k];
+    }
+}
 
 // Window sums at bar t from scratch: sum over [t-p+1, t] in order (the
 // oracle's first-window loop, bollinger_bands.rs:613-620).
@@ -106,16 +142,34 @@ __device__ __forceinline__ void bb_run(BbEng &e, int c0, int c1, int z, bool act
     if (cs < ce) {
         int t = cs < z ? cs : max(warm, z);
         bb_anchor(e, t, ld);
-        for (;;) {
+        auto emit = [&](int tt) {
             const double m = e.sum * e.inv_n;
             const double var = (e.sum_sq * e.inv_n) - (m * m);
             float o[3];
             e.out(m, var, o);
-            if (t >= cs) st(t, o);
-            if (++t >= ce) break;
-            const double nw = ld(t), old = ld(t - e.p);
-            e.sum += nw - old;                              // bollinger_bands.rs:640-641
-            e.sum_sq = (e.sum_sq - old * old) + nw * nw;
+            if (tt >= cs) st(tt, o);
+        };
+        emit(t);
+        // blocks of 8 bars: the 16 loads of a block are issued before its
+        // sequential updates (indices clamped into [t+1, ce-1])
+        while (t + 1 < ce) {
+            double nw[8], od[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int u = min(t + 1 + k, ce - 1);
+                nw[k] = ld(u);
+                od[k] = ld(u - e.p);
+            }
+            const int nk = min(8, ce - 1 - t);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                if (k < nk) {
+                    e.sum += nw[k] - od[k];                     // bollinger_bands.rs:640-641
+                    e.sum_sq = (e.sum_sq - od[k] * od[k]) + nw[k] * nw[k];
+                    emit(t + 1 + k);
+                }
+            }
+            t += nk;
         }
     }
     for (int u = (int)max((long)c0, min(zp, (long)c1)); u < c1; ++u) st(u, nan3);
@@ -144,9 +198,11 @@ extern "C" __global__ __launch_bounds__(64) void bb_ms1p_f32(roll_args a) {
 }
 
 // one series x many (period, devup, devdn): grid.x = combos / 64, grid.y = chunks.
-// Lane = combo; series reads are wave-uniform broadcasts from L1/L2.  Output
-// rows are [combo][t]; consecutive bars of one lane are consecutive addresses.
+// Lane = combo; all lanes step through the chunk's bars together (the same
+// output rule as bb_run: re-anchor at the first output bar of the chunk,
+// replaying at most p bars after the first non-finite bar z, NaN from z + p).
 extern "C" __global__ __launch_bounds__(64) void bb_batch_f32(roll_args a) {
+    __shared__ float ob[3][64 * RB_LD];
     const int lane = threadIdx.x & 63;
     const int c = blockIdx.x * 64 + lane;
     const bool active = c < a.nlanes;
@@ -158,13 +214,52 @@ extern "C" __global__ __launch_bounds__(64) void bb_batch_f32(roll_args a) {
     e.setup(a.p_len[cc], a.fvs[0], a.p_a[cc], a.p_b[cc]);
     const int z = a.poison ? a.poison[0] : T;
     auto ld = [&](int t) -> double { return (double)a.x[t]; };
-    auto st = [&](int t, const float *o) {
-        const size_t off = (size_t)c * T + t;
-        a.out0[off] = o[0];
-        a.out1[off] = o[1];
-        a.out2[off] = o[2];
-    };
-    bb_run(e, c0, c1, z, active, ld, st);
+    const int warm = e.fv + e.p - 1;
+    const long zp = (long)z + e.p;
+    const int cs = max(c0, warm);
+    const int ce = (int)min((long)c1, zp);
+    const float q = VTA_QNAN_F32;
+    bool have = false;
+    for (int t0 = c0; t0 < c1; t0 += RB_W) {
+        const int nb = min(RB_W, c1 - t0);
+        for (int k8 = 0; k8 < nb; k8 += 8) {
+            // the 16 loads of 8 bars first (indices clamped into the chunk)
+            double nw[8], od[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int u = min(t0 + k8 + j, c1 - 1);
+                nw[j] = ld(u);
+                od[j] = ld(max(u - e.p, 0));
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int k = k8 + j;
+                const int t = t0 + k;
+                float o[3] = {q, q, q};
+                if (k < nb && active && t >= cs && t < ce) {
+                    if (!have) {
+                        const int ta = cs < z ? cs : max(warm, z);
+                        bb_anchor(e, ta, ld);
+                        for (int u = ta + 1; u <= t; ++u) e.step(ld(u), ld(u - e.p));
+                        have = true;
+                    } else {
+                        e.step(nw[j], od[j]);
+                    }
+                    e.emit(o);
+                }
+                if (k < nb) {
+                    ob[0][lane * RB_LD + k] = o[0];
+                    ob[1][lane * RB_LD + k] = o[1];
+                    ob[2][lane * RB_LD + k] = o[2];
+                }
+            }
+        }
+        rb_sync();
+        rb_flush(ob[0], a.out0, blockIdx.x * 64, a.nlanes, T, t0, nb);
+        rb_flush(ob[1], a.out1, blockIdx.x * 64, a.nlanes, T, t0, nb);
+        rb_flush(ob[2], a.out2, blockIdx.x * 64, a.nlanes, T, t0, nb);
+        rb_sync();
+    }
 }
 
 // ---------------------------------------------------------------- sorted window
@@ -175,7 +270,9 @@ struct SortedWin {
     float *col;
     int n;
     __device__ float at(int i) const { return col[64 * i]; }
-    // number of elements < v (lower bound); `steps` binary-search rounds cover n
+    // number of elements < v (lower bound): count the full 8-element blocks
+    // whose last element is < v, then the elements < v inside the next block;
+    // every read of a level is independent of the others
     __device__ int lower(float v, int steps) const {
         int lo = 0, hi = n;
         for (int r = 0; r < steps; ++r) {
@@ -193,7 +290,9 @@ struct SortedWin {
         ++n;
     }
     // Replaces one element equal to `vo` (present) by `vi`: the elements
-    // strictly between the two positions move one slot towards vo's slot.
+    // strictly between the two positions move one slot towards vo's slot, in
+    // batches of 8 (eight reads, then eight writes; read indices are clamped
+    // to the lane's column, writes past the span are masked).
     __device__ void replace(float vo, float vi, int steps) {
         const int po = lower(vo, steps);
         const int pi = lower(vi, steps);
@@ -279,6 +378,8 @@ extern "C" __global__ __launch_bounds__(64) void pnr_ms1p_f32(roll_args a) {
             st);
 }
 
+// Lockstep one-series batch: lane = (length, percentage) combo; dynamic LDS =
+// 64 (wcap + 1) window slots followed by the 64 x RB_LD output staging.
 extern "C" __global__ __launch_bounds__(64) void pnr_batch_f32(roll_args a) {
     extern __shared__ float lds[];
     const int lane = threadIdx.x & 63;
@@ -289,9 +390,38 @@ extern "C" __global__ __launch_bounds__(64) void pnr_batch_f32(roll_args a) {
     const int c1 = min(c0 + a.chunk, T);
     const int cc = min(c, a.nlanes - 1);
     SortedWin w{lds + lane, 0};
+    float *ob = lds + 64 * (a.wcap + 1);
+    const int steps = bsearch_steps(a.wcap);
+    const int len = a.p_len[cc];
+    const double p_frac = a.p_a[cc] * 0.01;
+    const int cs = max(c0, a.fvs[0] + len - 1);      // first output bar of the chunk
     auto ld = [&](int t) -> double { return (double)a.x[t]; };
-    auto st = [&](int t, float v) { a.out0[(size_t)c * T + t] = v; };
-    pnr_run(w, a.p_len[cc], a.p_a[cc], a.fvs[0], c0, c1, active, bsearch_steps(a.wcap), ld, st);
+    int nan_cnt = 0;
+    bool have = false;
+    for (int t0 = c0; t0 < c1; t0 += RB_W) {
+        const int nb = min(RB_W, c1 - t0);
+        for (int k = 0; k < nb; ++k) {
+            const int t = t0 + k;
+            float o = VTA_QNAN_F32;
+            if (active && t >= cs) {
+                if (!have) {
+                    nan_cnt = fill_window(w, t, len, steps, ld);
+                    have = true;
+                } else {
+                    const float vo = a.x[t - len];
+                    const float vi = a.x[t];
+                    nan_cnt += (isnan(vi) ? 1 : 0) - (isnan(vo) ? 1 : 0);
+                    w.replace(rl_key(vo), rl_key(vi), steps);
+                }
+                const int wl = len - nan_cnt;
+                o = wl == 0 ? VTA_QNAN_F32 : w.at(pnr_rank(p_frac, wl));
+            }
+            ob[lane * RB_LD + k] = o;
+        }
+        rb_sync();
+        rb_flush(ob, a.out0, blockIdx.x * 64, a.nlanes, T, t0, nb);
+        rb_sync();
+    }
 }
 
 // ---------------------------------------------------------------- MAD
@@ -322,6 +452,21 @@ __device__ __forceinline__ double mad_kth(const SortedWin &w, double med, int sp
     return fmax(a, b);
 }
 
+// MAD of the current (NaN-free) sorted window of p values.
+__device__ __forceinline__ float mad_value(const SortedWin &w, int p, int steps) {
+    const int mid = p >> 1;
+    double med;
+    if (p & 1) med = (double)w.at(mid);
+    else med = 0.5 * ((double)w.at(mid - 1) + (double)w.at(mid));
+    // elements < med go left; ties with med (deviation 0) are on the right
+    int sp = w.lower((float)med, steps);
+    while (sp > 0 && (double)w.at(sp - 1) >= med) --sp;
+    while (sp < w.n && (double)w.at(sp) < med) ++sp;
+    double nxt;
+    const double kth = mad_kth(w, med, sp, (p & 1) ? mid : mid - 1, steps, &nxt);
+    return (float)((p & 1) ? kth : 0.5 * (kth + nxt));
+}
+
 template <class LD, class ST>
 __device__ __forceinline__ void mad_run(SortedWin &w, int p, int fv, int c0, int c1, bool active,
                                         int steps, LD ld, ST st) {
@@ -336,22 +481,8 @@ __device__ __forceinline__ void mad_run(SortedWin &w, int p, int fv, int c0, int
     }
     // any NaN in the window makes the output NaN (medium_ad.rs:343-348)
     int nan_cnt = fill_window(w, t, p, steps, ld);
-    const int mid = p >> 1;
     for (;;) {
-        float outv = VTA_QNAN_F32;
-        if (nan_cnt == 0) {
-            double med;
-            if (p & 1) med = (double)w.at(mid);
-            else med = 0.5 * ((double)w.at(mid - 1) + (double)w.at(mid));
-            // elements < med go left; ties with med (deviation 0) are on the right
-            int sp = w.lower((float)med, steps);
-            while (sp > 0 && (double)w.at(sp - 1) >= med) --sp;
-            while (sp < w.n && (double)w.at(sp) < med) ++sp;
-            double nxt;
-            const double kth = mad_kth(w, med, sp, (p & 1) ? mid : mid - 1, steps, &nxt);
-            outv = (float)((p & 1) ? kth : 0.5 * (kth + nxt));
-        }
-        st(t, outv);
+        st(t, nan_cnt == 0 ? mad_value(w, p, steps) : VTA_QNAN_F32);
         if (t + 1 >= c1) break;
         const float vo = (float)ld(t + 1 - p);
         const float vi = (float)ld(t + 1);
@@ -376,6 +507,7 @@ extern "C" __global__ __launch_bounds__(64) void mad_ms1p_f32(roll_args a) {
     mad_run(w, a.p1, active ? a.fvs[s] : a.T, c0, c1, active, bsearch_steps(a.wcap), ld, st);
 }
 
+// Lockstep one-series batch: lane = period; dynamic LDS as pnr_batch_f32.
 extern "C" __global__ __launch_bounds__(64) void mad_batch_f32(roll_args a) {
     extern __shared__ float lds[];
     const int lane = threadIdx.x & 63;
@@ -386,7 +518,38 @@ extern "C" __global__ __launch_bounds__(64) void mad_batch_f32(roll_args a) {
     const int c1 = min(c0 + a.chunk, T);
     const int cc = min(c, a.nlanes - 1);
     SortedWin w{lds + lane, 0};
+    float *ob = lds + 64 * (a.wcap + 1);
+    const int steps = bsearch_steps(a.wcap);
+    const int p = a.p_len[cc];
+    const int cs = max(c0, a.fvs[0] + p - 1);
     auto ld = [&](int t) -> double { return (double)a.x[t]; };
-    auto st = [&](int t, float v) { a.out0[(size_t)c * T + t] = v; };
-    mad_run(w, a.p_len[cc], a.fvs[0], c0, c1, active, bsearch_steps(a.wcap), ld, st);
+    int nan_cnt = 0;
+    bool have = false;
+    for (int t0 = c0; t0 < c1; t0 += RB_W) {
+        const int nb = min(RB_W, c1 - t0);
+        for (int k = 0; k < nb; ++k) {
+            const int t = t0 + k;
+            float o = VTA_QNAN_F32;
+            if (active && t >= cs) {
+                if (p == 1) {                                 // medium_ad.rs:326-333
+                    o = isnan(a.x[t]) ? VTA_QNAN_F32 : 0.0f;
+                } else {
+                    if (!have) {
+                        nan_cnt = fill_window(w, t, p, steps, ld);
+                        have = true;
+                    } else {
+                        const float vo = a.x[t - p];
+                        const float vi = a.x[t];
+                        nan_cnt += (isnan(vi) ? 1 : 0) - (isnan(vo) ? 1 : 0);
+                        w.replace(rl_key(vo), rl_key(vi), steps);
+                    }
+                    if (nan_cnt == 0) o = mad_value(w, p, steps);
+                }
+            }
+            ob[lane * RB_LD + k] = o;
+        }
+        rb_sync();
+        rb_flush(ob, a.out0, blockIdx.x * 64, a.nlanes, T, t0, nb);
+        rb_sync();
+    }
 }

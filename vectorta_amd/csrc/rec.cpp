@@ -68,6 +68,26 @@ int pick_chunk(Ind k, size_t max_p, size_t T, size_t lane_groups, bool clean) {
     return (int)std::min(c, T);
 }
 
+// One-series batches: a chunk of at least the burn-in (so the burn-in at most
+// doubles the work) and small enough for ~2048 waves.  With long periods the
+// burn-in is tens of thousands of bars (25-32 nats at p ~ 2000); the 4x-burn-in
+// floor of the many-series path would leave only a few waves per lane group,
+// while smaller chunks would multiply the work, which a device shared by
+// concurrent requests pays for in full.
+int pick_chunk_batch(Ind k, size_t max_p, size_t T, size_t lane_groups, bool clean) {
+    if (!clean) return (int)T;
+    if (const char *e = getenv("VTA_REC_CHUNK")) {
+        long v = atol(e);
+        if (v >= 64) return (int)std::min<size_t>((size_t)v, T);
+    }
+    const size_t kb = (size_t)burn_bars(k, max_p);
+    const size_t want_chunks = std::max<size_t>(1, 2048 / std::max<size_t>(lane_groups, 1));
+    const size_t by_par = (T + want_chunks - 1) / want_chunks;
+    size_t c = std::max<size_t>({kb, by_par, 256});
+    c = (c + 63) & ~(size_t)63;
+    return (int)std::min(c, T);
+}
+
 // Grid expansion for the one-parameter indicators.
 vta_status expand_periods(Ind k, const vta_range_usize *r, std::vector<size_t> &ps) {
     if (!r) return fail(VTA_ERR_INVALID_INPUT, "range is null");
@@ -127,7 +147,7 @@ vta_status batch_launch(vta_ctx *ctx, Ind k, const float *d_x, const float *d_h,
     a.T = (int)T;
     a.nlanes = (int)C;
     unsigned groups = (unsigned)((C + 63) / 64);
-    a.chunk = pick_chunk(k, max_p, T, groups, clean);
+    a.chunk = pick_chunk_batch(k, max_p, T, groups, clean);
     unsigned chunks = (unsigned)((T + a.chunk - 1) / a.chunk);
     return launch_rec(ctx, batch_kernel(k), a, groups, chunks);
 }

@@ -67,7 +67,10 @@ vta_status roll_launch(vta_ctx *ctx, RInd k, bool batch, roll_args_host &a, size
     a.wcap = (int)max_w;
     unsigned groups = (unsigned)((a.nlanes + 63) / 64);
     unsigned chunks = (unsigned)((a.T + a.chunk - 1) / a.chunk);
-    size_t smem = (k == R_BB) ? 0 : (size_t)64 * (max_w + 1) * sizeof(float);
+    // PNR / MAD: the lanes' sorted windows, plus (batch) the 64 x 33 output staging
+    size_t smem = (k == R_BB) ? 0
+                              : (size_t)64 * (max_w + 1) * sizeof(float) +
+                                    (batch ? (size_t)64 * 33 * sizeof(float) : 0);
     if (smem > ctx->props.sharedMemPerBlock)
         return fail(VTA_ERR_INVALID_INPUT, "window length %zu exceeds the LDS capacity", max_w);
     return launch(ctx, f, dim3(groups, chunks, 1), dim3(64), (unsigned)smem, a);
