@@ -265,7 +265,7 @@ def bench_double_alma(steps, warmup, ev):
             "reference_published_ms": 158.71}
 
 
-def bench_serving(rates=(200, 1000), duration_s=5.0, n_slots=8, seed=6):
+def bench_serving(rates=(200, 1000, 2000, 5000), duration_s=4.0, n_slots=8, seed=6):
     """Served mix: Poisson arrivals at each rate for duration_s seconds; each
     request = indicator uniform over {alma, ema, rsi, bollinger_bands, adx},
     series length log-uniform 1k..1M bars sliced from 8 resident-on-host OHLC
@@ -312,6 +312,7 @@ def bench_serving(rates=(200, 1000), duration_s=5.0, n_slots=8, seed=6):
         vals = sum(r.n_values for r in ok)
         out["rates"].append({
             "offered_req_per_s": rate, "requests": n, "failed": n - len(ok),
+            "latency_ms_p90": float(np.percentile(lat, 90)),
             "achieved_req_per_s": len(ok) / span, "latency_ms_p50": float(np.percentile(lat, 50)),
             "latency_ms_p99": float(np.percentile(lat, 99)), "outputs_per_s": vals / span,
             "peak_in_flight": st["peak_in_flight"], "peak_hbm_in_use_GB": st["peak_in_use"] / 1e9})

@@ -460,6 +460,7 @@ typedef struct vta_result {
     size_t n_combos;
     vta_device_array_f32 out[3]; /* [combo][len], owned by the server until release */
     double t_submit, t_start, t_done;   /* seconds on the server's clock */
+    double t_series, t_launch;          /* series resident; kernels enqueued */
 } vta_result;
 
 typedef struct vta_server_stats {

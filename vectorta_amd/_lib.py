@@ -77,7 +77,8 @@ class VtaResultC(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int32), ("n_outputs", ctypes.c_int32),
                 ("n_combos", ctypes.c_size_t), ("out", DeviceArrayF32C * 3),
                 ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double),
-                ("t_done", ctypes.c_double)]
+                ("t_done", ctypes.c_double), ("t_series", ctypes.c_double),
+                ("t_launch", ctypes.c_double)]
 
 
 class VtaServerStatsC(ctypes.Structure):

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <deque>
 #include <list>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -76,6 +77,79 @@ size_t first_nan_end(const float *x, size_t n) {
     return i;
 }
 
+// Best-fit allocator over one device allocation; coalesces on free.
+class Arena {
+  public:
+    vta_status init(size_t cap) {
+        cap = (cap + 255) & ~(size_t)255;
+        VTA_HIP(hipMalloc(reinterpret_cast<void **>(&base_), cap));
+        cap_ = cap;
+        free_.emplace(0, cap);
+        bylen_.emplace(cap, 0);
+        return VTA_OK;
+    }
+    ~Arena() {
+        if (base_) (void)hipFree(base_);
+    }
+    void *alloc(size_t n) {
+        n = std::max<size_t>(256, (n + 255) & ~(size_t)255);
+        std::lock_guard<std::mutex> lk(mu_);
+        auto it = bylen_.lower_bound(n);
+        if (it == bylen_.end()) return nullptr;
+        const size_t len = it->first, off = it->second;
+        bylen_.erase(it);
+        free_.erase(off);
+        if (len > n) {
+            free_.emplace(off + n, len - n);
+            bylen_.emplace(len - n, off + n);
+        }
+        used_.emplace(off, n);
+        return base_ + off;
+    }
+    void release(void *p) {
+        if (!p) return;
+        std::lock_guard<std::mutex> lk(mu_);
+        size_t off = static_cast<size_t>(static_cast<char *>(p) - base_);
+        auto u = used_.find(off);
+        if (u == used_.end()) return;
+        size_t n = u->second;
+        used_.erase(u);
+        auto next = free_.lower_bound(off);
+        if (next != free_.end() && next->first == off + n) {
+            erase_len(next->second, next->first);
+            n += next->second;
+            next = free_.erase(next);
+        }
+        if (next != free_.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {
+                erase_len(prev->second, prev->first);
+                off = prev->first;
+                n += prev->second;
+                free_.erase(prev);
+            }
+        }
+        free_.emplace(off, n);
+        bylen_.emplace(n, off);
+    }
+
+  private:
+    void erase_len(size_t len, size_t off) {
+        auto r = bylen_.equal_range(len);
+        for (auto it = r.first; it != r.second; ++it)
+            if (it->second == off) {
+                bylen_.erase(it);
+                return;
+            }
+    }
+    char *base_ = nullptr;
+    size_t cap_ = 0;
+    std::map<size_t, size_t> free_;          // offset -> length
+    std::multimap<size_t, size_t> bylen_;    // length -> offset
+    std::unordered_map<size_t, size_t> used_;
+    std::mutex mu_;
+};
+
 }  // namespace
 
 struct vta_server {
@@ -95,7 +169,10 @@ struct vta_server {
     std::list<uint64_t> lru;            // front = least recently used
     std::vector<vta_ctx *> ctxs;
     std::vector<std::thread> workers;
-    hipStream_t free_stream = nullptr;  // releases and evictions
+    Arena arena;                        // all served-path device memory
+    std::vector<void *> pending_free;   // collected under mu, returned after unlocking
+    std::vector<float *> pinned;        // per-slot pinned staging for series uploads
+    std::vector<size_t> pinned_cap;
     double t0 = 0.0;
     // ALMA grid -> sweep tables (guarded by plan_mu; entries live until destroy)
     std::mutex plan_mu;
@@ -147,9 +224,9 @@ void evict_lru(vta_server *s, size_t need) {
             continue;
         }
         std::shared_ptr<CacheEntry> ent = e->second;
-        (void)hipFreeAsync(ent->d_close, s->free_stream);
-        if (ent->d_high) (void)hipFreeAsync(ent->d_high, s->free_stream);
-        if (ent->d_low) (void)hipFreeAsync(ent->d_low, s->free_stream);
+        s->pending_free.push_back(ent->d_close);
+        if (ent->d_high) s->pending_free.push_back(ent->d_high);
+        if (ent->d_low) s->pending_free.push_back(ent->d_low);
         s->in_use -= ent->bytes;
         s->cache.erase(e);
         it = s->lru.erase(it);
@@ -204,34 +281,61 @@ std::shared_ptr<Req> try_admit(vta_server *s) {
     return nullptr;
 }
 
-vta_status upload_series(vta_ctx *ctx, Req &q) {
+// Copies the series through the slot's pinned staging buffer (grown on
+// demand), so the DMA engine reads page-locked memory directly.
+vta_status upload_series(vta_server *s, int slot, vta_ctx *ctx, Req &q) {
     CacheEntry &e = *q.entry;
     const size_t n = q.meta.len * sizeof(float);
-    VTA_HIP(hipMallocAsync(reinterpret_cast<void **>(&e.d_close), n, ctx->stream));
-    VTA_HIP(hipMemcpyAsync(e.d_close, q.close.data(), n, hipMemcpyHostToDevice, ctx->stream));
-    if (q.meta.hlc) {
-        VTA_HIP(hipMallocAsync(reinterpret_cast<void **>(&e.d_high), n, ctx->stream));
-        VTA_HIP(hipMallocAsync(reinterpret_cast<void **>(&e.d_low), n, ctx->stream));
-        VTA_HIP(hipMemcpyAsync(e.d_high, q.high.data(), n, hipMemcpyHostToDevice, ctx->stream));
-        VTA_HIP(hipMemcpyAsync(e.d_low, q.low.data(), n, hipMemcpyHostToDevice, ctx->stream));
+    const int arrays = q.meta.hlc ? 3 : 1;
+    if (s->pinned_cap[slot] < arrays * n) {
+        if (s->pinned[slot]) VTA_HIP(hipHostFree(s->pinned[slot]));
+        s->pinned[slot] = nullptr;
+        s->pinned_cap[slot] = 0;
+        void *p = nullptr;
+        VTA_HIP(hipHostMalloc(&p, arrays * n, hipHostMallocDefault));
+        s->pinned[slot] = static_cast<float *>(p);
+        s->pinned_cap[slot] = arrays * n;
     }
-    VTA_HIP(hipStreamSynchronize(ctx->stream));
+    float *stage = s->pinned[slot];
+    std::memcpy(stage, q.close.data(), n);
+    if (q.meta.hlc) {
+        std::memcpy(stage + q.meta.len, q.high.data(), n);
+        std::memcpy(stage + 2 * q.meta.len, q.low.data(), n);
+    }
+    e.d_close = static_cast<float *>(s->arena.alloc(n));
+    if (!e.d_close) return fail(VTA_ERR_OUT_OF_MEMORY, "device arena exhausted");
+    VTA_HIP(hipMemcpyAsync(e.d_close, stage, n, hipMemcpyHostToDevice, ctx->stream));
+    if (q.meta.hlc) {
+        e.d_high = static_cast<float *>(s->arena.alloc(n));
+        e.d_low = static_cast<float *>(s->arena.alloc(n));
+        if (!e.d_high || !e.d_low) {
+            (void)hipStreamSynchronize(ctx->stream);
+            return fail(VTA_ERR_OUT_OF_MEMORY, "device arena exhausted");
+        }
+        VTA_HIP(hipMemcpyAsync(e.d_high, stage + q.meta.len, n, hipMemcpyHostToDevice,
+                               ctx->stream));
+        VTA_HIP(hipMemcpyAsync(e.d_low, stage + 2 * q.meta.len, n, hipMemcpyHostToDevice,
+                               ctx->stream));
+    }
+    VTA_HIP(hipStreamSynchronize(ctx->stream));   // the staging buffer is reused next request
     return VTA_OK;
 }
 
-// A small stream-ordered device table, freed in stream order after the
-// launches that read it.
-struct AsyncTable {
+// A small parameter table in the arena; returned when it goes out of scope,
+// which run_request reaches only after its stream synchronised.
+struct ArenaTable {
+    Arena *ar = nullptr;
     void *p = nullptr;
-    hipStream_t st = nullptr;
-    ~AsyncTable() {
-        if (p) (void)hipFreeAsync(p, st);
+    ~ArenaTable() {
+        if (p) ar->release(p);
     }
     template <typename T>
-    vta_status put(vta_ctx *ctx, const std::vector<T> &v) {
-        st = ctx->stream;
-        VTA_HIP(hipMallocAsync(&p, v.size() * sizeof(T), st));
-        VTA_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    vta_status put(vta_server *s, vta_ctx *ctx, const std::vector<T> &v) {
+        ar = &s->arena;
+        p = ar->alloc(v.size() * sizeof(T));
+        if (!p) return fail(VTA_ERR_OUT_OF_MEMORY, "device arena exhausted");
+        VTA_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice,
+                               ctx->stream));
         return VTA_OK;
     }
     template <typename T>
@@ -240,9 +344,9 @@ struct AsyncTable {
     }
 };
 
-vta_status alloc_out(vta_ctx *ctx, vta_device_array_f32 &o, size_t rows, size_t cols) {
-    void *p = nullptr;
-    VTA_HIP(hipMallocAsync(&p, rows * cols * sizeof(float), ctx->stream));
+vta_status alloc_out(vta_server *s, vta_device_array_f32 &o, size_t rows, size_t cols) {
+    void *p = s->arena.alloc(rows * cols * sizeof(float));
+    if (!p) return fail(VTA_ERR_OUT_OF_MEMORY, "device arena exhausted");
     o.ptr = static_cast<float *>(p);
     o.rows = rows;
     o.cols = cols;
@@ -269,6 +373,7 @@ vta_status alma_plan_for(vta_server *s, vta_ctx *ctx, const vta_alma_batch_range
 
 vta_status run_request(vta_server *s, vta_ctx *ctx, Req &q) {
     CacheEntry &e = *q.entry;
+    const double t0 = s->t0;
     const SeriesMeta &m = e.meta;
     const vta_request &r = q.r;
     vta_result &res = q.res;
@@ -282,9 +387,10 @@ vta_status run_request(vta_server *s, vta_ctx *ctx, Req &q) {
                 return fail(VTA_ERR_INVALID_INPUT, "not enough valid data (needed >= %zu)", mp);
             res.n_outputs = 1;
             res.n_combos = nc;
-            VTA_TRY(alloc_out(ctx, res.out[0], nc, T));
+            VTA_TRY(alloc_out(s, res.out[0], nc, T));
             VTA_TRY(alma_grid_plan_launch(ctx, plan, e.d_close, T, (size_t)m.fv_close,
                                           res.out[0].ptr));
+            res.t_launch = now_s() - t0;
             VTA_HIP(hipStreamSynchronize(ctx->stream));
             return VTA_OK;
         }
@@ -311,11 +417,11 @@ vta_status run_request(vta_server *s, vta_ctx *ctx, Req &q) {
             if (r.kind == VTA_REQ_ADX && !m.hlc)
                 return fail(VTA_ERR_INVALID_INPUT, "ADX needs high, low and close");
             std::vector<int32_t> per(ps.begin(), ps.end());
-            AsyncTable dp;
-            VTA_TRY(dp.put(ctx, per));
+            ArenaTable dp;
+            VTA_TRY(dp.put(s, ctx, per));
             res.n_outputs = 1;
             res.n_combos = n;
-            VTA_TRY(alloc_out(ctx, res.out[0], n, T));
+            VTA_TRY(alloc_out(s, res.out[0], n, T));
             if (r.kind == VTA_REQ_EMA)
                 VTA_TRY(vta_ema_batch_device(ctx, e.d_close, dp.as<int32_t>(), nullptr, (int32_t)maxp,
                                              (int32_t)T, fv, (int32_t)n, res.out[0].ptr));
@@ -326,6 +432,7 @@ vta_status run_request(vta_server *s, vta_ctx *ctx, Req &q) {
                 VTA_TRY(vta_adx_batch_device(ctx, e.d_high, e.d_low, e.d_close, dp.as<int32_t>(),
                                              (int32_t)maxp, (int32_t)T, fv, (int32_t)n,
                                              res.out[0].ptr));
+            res.t_launch = now_s() - t0;
             VTA_HIP(hipStreamSynchronize(ctx->stream));
             return VTA_OK;
         }
@@ -349,17 +456,18 @@ vta_status run_request(vta_server *s, vta_ctx *ctx, Req &q) {
             }
             if ((size_t)m.fv_close >= T || T - (size_t)m.fv_close < maxp)
                 return fail(VTA_ERR_INVALID_INPUT, "not enough valid data (needed >= %zu)", maxp);
-            AsyncTable dp, du, dd;
-            VTA_TRY(dp.put(ctx, per));
-            VTA_TRY(du.put(ctx, up));
-            VTA_TRY(dd.put(ctx, dn));
+            ArenaTable dp, du, dd;
+            VTA_TRY(dp.put(s, ctx, per));
+            VTA_TRY(du.put(s, ctx, up));
+            VTA_TRY(dd.put(s, ctx, dn));
             res.n_outputs = 3;
             res.n_combos = n;
-            for (int i = 0; i < 3; ++i) VTA_TRY(alloc_out(ctx, res.out[i], n, T));
+            for (int i = 0; i < 3; ++i) VTA_TRY(alloc_out(s, res.out[i], n, T));
             VTA_TRY(vta_bollinger_bands_batch_device(
                 ctx, e.d_close, (int32_t)T, m.fv_close, m.z_close, dp.as<int32_t>(),
                 du.as<double>(), dd.as<double>(), (int32_t)n, (int32_t)maxp, res.out[0].ptr,
                 res.out[1].ptr, res.out[2].ptr));
+            res.t_launch = now_s() - t0;
             VTA_HIP(hipStreamSynchronize(ctx->stream));
             return VTA_OK;
         }
@@ -368,17 +476,30 @@ vta_status run_request(vta_server *s, vta_ctx *ctx, Req &q) {
     }
 }
 
-void free_outputs(vta_result &res, hipStream_t st) {
+// Moves the result's device pointers to `out` (freed later, outside mu).
+void take_outputs(vta_result &res, std::vector<void *> &out) {
     for (int i = 0; i < 3; ++i)
         if (res.out[i].ptr) {
-            (void)hipFreeAsync(res.out[i].ptr, st);
+            out.push_back(res.out[i].ptr);
             res.out[i].ptr = nullptr;
         }
+}
+
+// Called without mu held: stream-ordered frees of everything collected.
+void flush_frees(vta_server *s, std::vector<void *> &mine) {
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        mine.insert(mine.end(), s->pending_free.begin(), s->pending_free.end());
+        s->pending_free.clear();
+    }
+    for (void *p : mine) s->arena.release(p);
+    mine.clear();
 }
 
 void worker(vta_server *s, int slot) {
     vta_ctx *ctx = s->ctxs[slot];
     (void)hipSetDevice(s->device);
+    std::vector<void *> local_frees;
     for (;;) {
         std::shared_ptr<Req> q;
         {
@@ -393,15 +514,17 @@ void worker(vta_server *s, int slot) {
             s->peak_in_flight = std::max(s->peak_in_flight, s->in_flight);
             q->res.t_start = now_s() - s->t0;
         }
+        flush_frees(s, local_frees);   // evictions made room for this request
         CacheEntry &e = *q->entry;
         vta_status st = VTA_OK;
         if (q->uploader) {
-            st = upload_series(ctx, *q);
+            st = upload_series(s, slot, ctx, *q);
             std::lock_guard<std::mutex> lk(s->mu);
             if (st != VTA_OK) {
-                if (e.d_close) (void)hipFreeAsync(e.d_close, ctx->stream);
-                if (e.d_high) (void)hipFreeAsync(e.d_high, ctx->stream);
-                if (e.d_low) (void)hipFreeAsync(e.d_low, ctx->stream);
+                (void)hipStreamSynchronize(ctx->stream);
+                if (e.d_close) s->pending_free.push_back(e.d_close);
+                if (e.d_high) s->pending_free.push_back(e.d_high);
+                if (e.d_low) s->pending_free.push_back(e.d_low);
                 e.d_close = e.d_high = e.d_low = nullptr;
                 if (e.cached) {
                     // drop the broken entry; its bytes go back with this request
@@ -416,6 +539,7 @@ void worker(vta_server *s, int slot) {
             std::unique_lock<std::mutex> lk(s->mu);
             s->cv_done.wait(lk, [&] { return e.ready; });
         }
+        q->res.t_series = now_s() - s->t0;
         if (st == VTA_OK && e.d_close) {
             clear_error();
             st = run_request(s, ctx, *q);
@@ -428,7 +552,7 @@ void worker(vta_server *s, int slot) {
             q->res.t_done = now_s() - s->t0;
             if (st != VTA_OK) {
                 (void)hipStreamSynchronize(ctx->stream);
-                free_outputs(q->res, ctx->stream);
+                take_outputs(q->res, s->pending_free);
                 s->in_use -= q->out_bytes;
                 q->out_bytes = 0;
             }
@@ -436,9 +560,9 @@ void worker(vta_server *s, int slot) {
             if (!e.cached) {
                 // a one-off series, or an entry dropped after a failed upload
                 if (e.refs == 0) {
-                    if (e.d_close) (void)hipFreeAsync(e.d_close, ctx->stream);
-                    if (e.d_high) (void)hipFreeAsync(e.d_high, ctx->stream);
-                    if (e.d_low) (void)hipFreeAsync(e.d_low, ctx->stream);
+                    if (e.d_close) s->pending_free.push_back(e.d_close);
+                    if (e.d_high) s->pending_free.push_back(e.d_high);
+                    if (e.d_low) s->pending_free.push_back(e.d_low);
                     e.d_close = e.d_high = e.d_low = nullptr;
                 }
                 s->in_use -= q->series_reserved;
@@ -449,6 +573,7 @@ void worker(vta_server *s, int slot) {
             s->cv_done.notify_all();
             s->cv_admit.notify_all();
         }
+        flush_frees(s, local_frees);
     }
 }
 
@@ -465,15 +590,14 @@ vta_status vta_server_create(int device_id, int n_slots, size_t hbm_budget, vta_
     s->budget = hbm_budget;
     s->t0 = now_s();
     {
-        hipError_t e = hipSetDevice(device_id);
-        hipMemPool_t pool = nullptr;
-        if (e == hipSuccess) e = hipDeviceGetDefaultMemPool(&pool, device_id);
-        uint64_t keep = UINT64_MAX;   // freed blocks stay in the pool for reuse
-        if (e == hipSuccess) e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
-        if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->free_stream, hipStreamNonBlocking);
-        if (e != hipSuccess) {
+        // the budget plus slack for parameter tables and fragmentation
+        vta_status st = VTA_OK;
+        if (hipSetDevice(device_id) != hipSuccess)
+            st = fail(VTA_ERR_HIP, "hipSetDevice(%d) failed", device_id);
+        if (st == VTA_OK) st = s->arena.init(hbm_budget + hbm_budget / 16 + (64u << 20));
+        if (st != VTA_OK) {
             delete s;
-            return fail(VTA_ERR_HIP, "server set-up: %s", hipGetErrorString(e));
+            return st;
         }
     }
     for (int i = 0; i < n_slots; ++i) {
@@ -488,6 +612,8 @@ vta_status vta_server_create(int device_id, int n_slots, size_t hbm_budget, vta_
         }
         s->ctxs.push_back(c);
     }
+    s->pinned.assign(n_slots, nullptr);
+    s->pinned_cap.assign(n_slots, 0);
     for (int i = 0; i < n_slots; ++i) s->workers.emplace_back(worker, s, i);
     *out = s;
     return VTA_OK;
@@ -504,18 +630,20 @@ vta_status vta_server_destroy(vta_server *s) {
     }
     for (auto &t : s->workers) t.join();
     (void)hipSetDevice(s->device);
+    std::vector<void *> frees;
     for (auto &kv : s->reqs)
-        if (kv.second->done) free_outputs(kv.second->res, s->free_stream);
+        if (kv.second->done) take_outputs(kv.second->res, frees);
     for (auto &kv : s->cache) {
-        (void)hipFreeAsync(kv.second->d_close, s->free_stream);
-        if (kv.second->d_high) (void)hipFreeAsync(kv.second->d_high, s->free_stream);
-        if (kv.second->d_low) (void)hipFreeAsync(kv.second->d_low, s->free_stream);
+        frees.push_back(kv.second->d_close);
+        if (kv.second->d_high) frees.push_back(kv.second->d_high);
+        if (kv.second->d_low) frees.push_back(kv.second->d_low);
     }
-    (void)hipStreamSynchronize(s->free_stream);
+    flush_frees(s, frees);
     (void)hipDeviceSynchronize();
     for (auto &kv : s->alma_plans) alma_grid_plan_destroy(kv.second);
+    for (float *p : s->pinned)
+        if (p) (void)hipHostFree(p);
     for (auto *c : s->ctxs) vta_ctx_destroy(c);
-    (void)hipStreamDestroy(s->free_stream);
     delete s;
     return VTA_OK;
 }
@@ -584,17 +712,21 @@ vta_status vta_server_wait(vta_server *s, uint64_t req_id, int timeout_ms, vta_r
 vta_status vta_server_release(vta_server *s, uint64_t req_id) {
     clear_error();
     if (!s) return fail(VTA_ERR_INVALID_INPUT, "null argument");
-    std::lock_guard<std::mutex> lk(s->mu);
-    auto it = s->reqs.find(req_id);
-    if (it == s->reqs.end()) return fail(VTA_ERR_INVALID_INPUT, "unknown request");
-    std::shared_ptr<Req> q = it->second;
-    if (!q->done) return fail(VTA_ERR_INVALID_INPUT, "request still running");
+    std::vector<void *> frees;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(req_id);
+        if (it == s->reqs.end()) return fail(VTA_ERR_INVALID_INPUT, "unknown request");
+        std::shared_ptr<Req> q = it->second;
+        if (!q->done) return fail(VTA_ERR_INVALID_INPUT, "request still running");
+        take_outputs(q->res, frees);
+        s->in_use -= q->out_bytes;
+        q->out_bytes = 0;
+        s->reqs.erase(it);
+        s->cv_admit.notify_all();
+    }
     (void)hipSetDevice(s->device);
-    free_outputs(q->res, s->free_stream);
-    s->in_use -= q->out_bytes;
-    q->out_bytes = 0;
-    s->reqs.erase(it);
-    s->cv_admit.notify_all();
+    flush_frees(s, frees);
     return VTA_OK;
 }
 

@@ -28,6 +28,8 @@ class ServedResult:
     t_submit: float
     t_start: float
     t_done: float
+    t_series: float = 0.0
+    t_launch: float = 0.0
 
     @property
     def ok(self) -> bool:
@@ -82,7 +84,7 @@ class IndicatorServer:
         outs = [(res.out[i].ptr or 0, res.out[i].rows, res.out[i].cols)
                 for i in range(res.n_outputs if res.status == 0 else 0)]
         return ServedResult(req_id, res.status, res.n_outputs, res.n_combos, outs, res.t_submit,
-                            res.t_start, res.t_done)
+                            res.t_start, res.t_done, res.t_series, res.t_launch)
 
     def to_host(self, result: ServedResult, i: int = 0) -> np.ndarray:
         if not result.ok:
