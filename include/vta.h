@@ -261,10 +261,55 @@ vta_status vta_adx_many_series_one_param_time_major_dev(vta_ctx *ctx, const floa
                                                         size_t rows, size_t period,
                                                         vta_device_array_f32 *out);
 
+/* SMMA: smma_wrapper.rs:398 (smma_batch_dev), :377 (smma_batch_device; the
+ * per-combo warm indices are fv + p - 1 and derived here), :623
+ * (smma_multi_series_one_param_time_major_dev), :599
+ * (smma_multi_series_one_param_device).  Grid: smma.rs:430-471.  Seed = mean
+ * of the first p bars at fv + p - 1, then prev = (prev (p-1) + x) / p. */
+vta_status vta_smma_expand_grid(const vta_range_usize *range, size_t *periods, size_t cap,
+                                size_t *n_out);
+vta_status vta_smma_batch_dev(vta_ctx *ctx, const float *prices, size_t len,
+                              const vta_range_usize *range, vta_device_array_f32 *out);
+vta_status vta_smma_batch_device(vta_ctx *ctx, const float *d_prices, const int32_t *d_periods,
+                                 int32_t max_period, int32_t series_len, int32_t first_valid,
+                                 int32_t n_combos, float *d_out);
+vta_status vta_smma_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *data_tm,
+                                                         size_t cols, size_t rows, size_t period,
+                                                         vta_device_array_f32 *out);
+vta_status vta_smma_many_series_one_param_device(vta_ctx *ctx, const float *d_prices_tm,
+                                                 const int32_t *d_first_valids, size_t period,
+                                                 size_t num_series, size_t series_len,
+                                                 const int32_t *d_clean, float *d_out_tm);
+
+/* Wilders: wilders_wrapper.rs:134 (wilders_batch_dev; the first max_period
+ * bars after first_valid must be finite, :391-400), :252
+ * (wilders_batch_device; alphas and warm indices derived from the periods),
+ * :600 (wilders_many_series_one_param_time_major_dev; first_valid = first
+ * finite bar), :662 (wilders_many_series_one_param_device).  Grid:
+ * wilders_wrapper.rs:909-940. */
+vta_status vta_wilders_expand_grid(const vta_range_usize *range, size_t *periods, size_t cap,
+                                   size_t *n_out);
+vta_status vta_wilders_batch_dev(vta_ctx *ctx, const float *prices, size_t len,
+                                 const vta_range_usize *range, vta_device_array_f32 *out);
+vta_status vta_wilders_batch_device(vta_ctx *ctx, const float *d_prices,
+                                    const int32_t *d_periods, int32_t max_period,
+                                    int32_t series_len, int32_t first_valid, int32_t n_combos,
+                                    float *d_out);
+vta_status vta_wilders_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *data_tm,
+                                                            size_t cols, size_t rows,
+                                                            size_t period,
+                                                            vta_device_array_f32 *out);
+vta_status vta_wilders_many_series_one_param_device(vta_ctx *ctx, const float *d_prices_tm,
+                                                    const int32_t *d_first_valids,
+                                                    size_t period, size_t num_series,
+                                                    size_t series_len, const int32_t *d_clean,
+                                                    float *d_out_tm);
+
 /* Per-series metadata of device-resident time-major inputs, enqueue only:
  * first_valid (mode 0: x; 1: first bar with high, low, x all non-NaN; 2: ADX
- * triple rule; 3: x, with the mode-1 bar in d_first_valids_hlc) and the clean
- * flag.  The reference computes first_valids on the host
+ * triple rule; 3: x, with the mode-1 bar in d_first_valids_hlc; 4: x, with
+ * d_clean receiving the first non-finite bar at or after first_valid, T if
+ * none, for the finite-window averages) and the clean flag.  The reference computes first_valids on the host
  * (ema_wrapper.rs:637-700); this is its device-side counterpart. */
 vta_status vta_many_series_prepare(vta_ctx *ctx, const float *d_x_tm, const float *d_high_tm,
                                    const float *d_low_tm, size_t cols, size_t rows, int mode,
@@ -290,6 +335,64 @@ vta_status vta_ema_rsi_atr_many_series_dev(vta_ctx *ctx, const float *close_tm,
                                            size_t cols, size_t rows, size_t ema_period,
                                            size_t rsi_period, size_t atr_period,
                                            vta_device_array_f32 *out3);
+
+/* ------------------------------------------------------------------ finite-window averages */
+/* SMA (sma_wrapper.rs:548 sma_batch_dev, :830 sma_batch_dev_from_device, :750
+ * sma_multi_series_one_param_time_major_dev, :982 ..._dev_from_device), WMA
+ * (wma_wrapper.rs:713 wma_batch_dev, :747 wma_batch_device, :1014
+ * wma_multi_series_one_param_time_major_dev, :1049 ..._device) and SWMA
+ * (swma_wrapper.rs:678 swma_batch_dev, :663 swma_batch_device, :825
+ * swma_multi_series_one_param_time_major_dev, :787 swma_multi_series_one_param_device).
+ * Grids: sma.rs:705-755, wma_wrapper.rs:324-350, swma_wrapper.rs:275-301.
+ * Periods: SMA/SWMA >= 1, WMA >= 2.  _batch_device is enqueue only; the
+ * first non-finite bar is found on the device (the reference's warm-index and
+ * weight arguments are derived from the periods).  _many_series_..._device
+ * takes d_first_nonfinite per series (vta_many_series_prepare mode 4), or NULL
+ * when every bar from first_valid is finite. */
+vta_status vta_sma_expand_grid(const vta_range_usize *range, size_t *periods, size_t cap,
+                              size_t *n_out);
+vta_status vta_sma_batch_dev(vta_ctx *ctx, const float *prices, size_t len,
+                            const vta_range_usize *range, vta_device_array_f32 *out);
+vta_status vta_sma_batch_device(vta_ctx *ctx, const float *d_prices, const int32_t *d_periods,
+                               int32_t series_len, int32_t n_combos, int32_t first_valid,
+                               int32_t max_period, float *d_out);
+vta_status vta_sma_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *data_tm,
+                                                       size_t cols, size_t rows, size_t period,
+                                                       vta_device_array_f32 *out);
+vta_status vta_sma_many_series_one_param_device(vta_ctx *ctx, const float *d_prices_tm,
+                                               const int32_t *d_first_valids,
+                                               const int32_t *d_first_nonfinite, size_t cols,
+                                               size_t rows, size_t period, float *d_out_tm);
+
+vta_status vta_wma_expand_grid(const vta_range_usize *range, size_t *periods, size_t cap,
+                              size_t *n_out);
+vta_status vta_wma_batch_dev(vta_ctx *ctx, const float *prices, size_t len,
+                            const vta_range_usize *range, vta_device_array_f32 *out);
+vta_status vta_wma_batch_device(vta_ctx *ctx, const float *d_prices, const int32_t *d_periods,
+                               int32_t series_len, int32_t n_combos, int32_t first_valid,
+                               int32_t max_period, float *d_out);
+vta_status vta_wma_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *data_tm,
+                                                       size_t cols, size_t rows, size_t period,
+                                                       vta_device_array_f32 *out);
+vta_status vta_wma_many_series_one_param_device(vta_ctx *ctx, const float *d_prices_tm,
+                                               const int32_t *d_first_valids,
+                                               const int32_t *d_first_nonfinite, size_t cols,
+                                               size_t rows, size_t period, float *d_out_tm);
+
+vta_status vta_swma_expand_grid(const vta_range_usize *range, size_t *periods, size_t cap,
+                              size_t *n_out);
+vta_status vta_swma_batch_dev(vta_ctx *ctx, const float *prices, size_t len,
+                            const vta_range_usize *range, vta_device_array_f32 *out);
+vta_status vta_swma_batch_device(vta_ctx *ctx, const float *d_prices, const int32_t *d_periods,
+                               int32_t series_len, int32_t n_combos, int32_t first_valid,
+                               int32_t max_period, float *d_out);
+vta_status vta_swma_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *data_tm,
+                                                       size_t cols, size_t rows, size_t period,
+                                                       vta_device_array_f32 *out);
+vta_status vta_swma_many_series_one_param_device(vta_ctx *ctx, const float *d_prices_tm,
+                                               const int32_t *d_first_valids,
+                                               const int32_t *d_first_nonfinite, size_t cols,
+                                               size_t rows, size_t period, float *d_out_tm);
 
 /* ------------------------------------------------------------------ rolling windows */
 

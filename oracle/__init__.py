@@ -13,7 +13,8 @@ sees, widened), outputs are float64 arrays with the reference's NaN prefixes.
 Parity pins (see tests/test_oracle.py): the reference's own known-answer values
 on the Bitfinex 4h fixture (alma.rs:1835-1856, ema.rs:971-983, rsi.rs:1142-1156,
 atr.rs:1422-1436, adx.rs:2163-2175, bollinger_bands.rs:1705-1745,
-percentile_nearest_rank.rs:1395-1425, medium_ad.rs:1478-1497), the tiny PNR
+percentile_nearest_rank.rs:1395-1425, medium_ad.rs:1478-1497, sma.rs:1625-1641,
+wma.rs:855-883, swma.rs:1299-1318, smma.rs:915-931, wilders.rs:1226-1247), the tiny PNR
 vectors (tests/python/test_utils.py:165-179), the ALMA re-input values
 (test_utils.py:120-126), and closed-form / brute-force invariants.
 """
@@ -80,6 +81,12 @@ def lib():
                                      ctypes.POINTER(ctypes.c_int), sz,
                                      ctypes.POINTER(ctypes.c_longlong), dp, ctypes.c_int]
         L.or_cross_pairs.restype = None
+        for name in ("or_sma", "or_sma_batch_row", "or_wma", "or_wma_batch_row", "or_swma",
+                     "or_smma", "or_wilders"):
+            getattr(L, name).argtypes = [dp, sz, sz, dp]
+        L.or_ma_many.argtypes = [ctypes.c_int, dp, sz, sz, sz, dp, ctypes.POINTER(ctypes.c_int),
+                                 ctypes.c_int]
+        L.or_ma_many.restype = None
         _lib = L
     return _lib
 
@@ -253,3 +260,65 @@ def cross_pairs(rows, x, pair_fast, pair_slow, nthreads=0):
                          stats.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), _d(pnl)[1],
                          int(nthreads))
     return stats, pnl
+
+
+def _ma(name, x, period):
+    x, xp = _d(x)
+    out = np.full(x.size, np.nan)
+    _check(getattr(lib(), name)(xp, x.size, int(period), _d(out)[1]), name[3:])
+    return out
+
+
+def sma(x, period=9):
+    """sma.rs:302-330 (running sum)."""
+    return _ma("or_sma", x, period)
+
+
+def sma_batch_row(x, period=9):
+    """sma.rs:909-1021 (prefix-sum batch row)."""
+    return _ma("or_sma_batch_row", x, period)
+
+
+def wma(x, period=30):
+    """wma.rs:243-320 (running weighted sum)."""
+    return _ma("or_wma", x, period)
+
+
+def wma_batch_row(x, period=30):
+    """wma.rs:656-760 (prefix-sum batch row)."""
+    return _ma("or_wma_batch_row", x, period)
+
+
+def swma(x, period=5):
+    """swma.rs:406-476 (two running box sums; also the batch row)."""
+    return _ma("or_swma", x, period)
+
+
+def smma(x, period=7):
+    """smma.rs:243-271."""
+    return _ma("or_smma", x, period)
+
+
+def wilders(x, period=5):
+    """wilders.rs:195-320."""
+    return _ma("or_wilders", x, period)
+
+
+_MA_KIND = {"sma": 0, "wma": 1, "swma": 2, "smma": 3, "wilders": 4, "sma_batch_row": 5,
+            "wma_batch_row": 6}
+
+
+def ma_many(kind, tm, period, nthreads=0):
+    """One-period moving average of every column of a time-major [rows, cols] matrix."""
+    tm, tp = _d(tm)
+    rows, cols = tm.shape
+    out = np.full((rows, cols), np.nan)
+    st = (ctypes.c_int * cols)()
+    lib().or_ma_many(_MA_KIND[kind], tp, cols, rows, int(period), _d(out)[1], st, int(nthreads))
+    return out, np.frombuffer(st, dtype=np.int32).copy()
+
+
+def ma_batch(kind, x, periods):
+    """Rows of a one-series batch: [len(periods), len(x)]."""
+    f = globals()[kind]
+    return np.stack([f(x, p) for p in periods])

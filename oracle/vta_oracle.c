@@ -24,6 +24,15 @@
  *   PNR   src/indicators/percentile_nearest_rank.rs:148-266
  *   MAD   src/indicators/medium_ad.rs:175-230, 293-402
  *   ADX   src/indicators/adx.rs:190-298, 299-422
+ *   SMA   src/indicators/moving_averages/sma.rs:245-330 (scalar), 909-1021 (batch
+ *         prefix rows)
+ *   WMA   src/indicators/moving_averages/wma.rs:243-320 (scalar), 656-760 (batch
+ *         prefix rows)
+ *   SWMA  src/indicators/moving_averages/swma.rs:203-231, 406-476 (scalar; the
+ *         batch row swma_row_scalar :1155-1217 is the same algorithm)
+ *   SMMA  src/indicators/moving_averages/smma.rs:243-271, 369-400 (batch rows
+ *         call the scalar kernel, :775-862)
+ *   Wilders src/indicators/moving_averages/wilders.rs:195-231, 262-320
  *   NaN prefixes: src/utilities/helpers.rs:102-164 (quiet NaN 0x7ff8...)
  *   Double-ALMA crossover: not in the reference (README.md:9 names the
  *         workload only); position = sign(ALMA_fast - ALMA_slow) of the f64
@@ -721,5 +730,254 @@ void or_cross_pairs(const double *rows, size_t nrows, size_t len, const double *
         stats[p * 4 + 2] = ns;
         stats[p * 4 + 3] = ck;
         pnl[p] = pn;
+    }
+}
+
+/* ------------------------------------------------------------------ FIR moving averages */
+
+/* Validation shared by the one-period moving averages (sma.rs:245-280,
+ * swma.rs:203-231, smma.rs:369-400, wilders.rs:195-231): empty input, all
+ * NaN, period outside [min_p, len], fewer than period bars after first. */
+static int ma_prepare(const double *data, size_t len, size_t period, size_t min_p,
+                      size_t *first) {
+    if (len == 0) return OR_EMPTY;
+    *first = or_first_valid(data, len);
+    if (*first >= len) return OR_ALLNAN;
+    if (period < min_p || period > len) return OR_PERIOD;
+    if (len - *first < period) return OR_NOTENOUGH;
+    return OR_OK;
+}
+
+/* sma.rs:302-330 sma_scalar: running sum, sum += x[i] - x[i-p]. */
+int or_sma(const double *data, size_t len, size_t period, double *out) {
+    size_t first;
+    int st = ma_prepare(data, len, period, 1, &first);
+    if (st) return st;
+    fill_nan(out, first + period - 1);
+    if (period == 1) {
+        for (size_t i = first; i < len; ++i) out[i] = data[i];
+        return OR_OK;
+    }
+    double sum = 0.0;
+    for (size_t k = 0; k < period; ++k) sum += data[first + k];
+    double inv = 1.0 / (double)period;
+    out[first + period - 1] = sum * inv;
+    for (size_t i = first + period; i < len; ++i) {
+        sum += data[i] - data[i - period];
+        out[i] = sum * inv;
+    }
+    return OR_OK;
+}
+
+/* sma.rs:909-1021 sma_batch_inner_into, one row: inclusive prefix ps[i]
+ * (0 before first), out[i] = (ps[i] - ps[i-p]) * (1/p). */
+int or_sma_batch_row(const double *data, size_t len, size_t period, double *out) {
+    size_t first;
+    int st = ma_prepare(data, len, period, 1, &first);
+    if (st) return st;
+    double *ps = (double *)calloc(len, sizeof(double));
+    ps[first] = data[first];
+    for (size_t i = first + 1; i < len; ++i) ps[i] = ps[i - 1] + data[i];
+    size_t warm = first + period - 1;
+    fill_nan(out, warm);
+    double inv = 1.0 / (double)period;
+    double lo = warm >= period ? ps[warm - period] : 0.0;
+    out[warm] = (ps[warm] - lo) * inv;
+    for (size_t i = warm + 1; i < len; ++i) out[i] = (ps[i] - ps[i - period]) * inv;
+    free(ps);
+    return OR_OK;
+}
+
+/* wma.rs:243-320 wma_prepare + wma_scalar: weights 1..p (newest p), divided
+ * by p(p+1)/2; running weight_sum and sum. */
+int or_wma(const double *data, size_t len, size_t period, double *out) {
+    size_t first;
+    int st = ma_prepare(data, len, period, 2, &first);
+    if (st) return st;
+    fill_nan(out, first + period - 1);
+    size_t lookback = period - 1;
+    double pf = (double)period;
+    double weights = pf * (pf + 1.0) * 0.5;
+    const double *base = data + first;
+    double sum = 0.0, wsum = 0.0;
+    for (size_t k = 0; k < lookback; ++k) {
+        double v = base[k];
+        wsum += v * ((double)k + 1.0);
+        sum += v;
+    }
+    for (size_t i = first + lookback, j = 0; i < len; ++i, ++j) {
+        double v = data[i];
+        wsum += v * pf;
+        sum += v;
+        out[i] = wsum / weights;
+        wsum -= sum;
+        sum -= base[j];
+    }
+    return OR_OK;
+}
+
+/* wma.rs:656-760 wma_batch_inner_into, one row: prefixes pa (sum x) and pb
+ * (sum i*x), x = 0 before first; wsum = s_b - ((i+1-p) - 1) * s_a. */
+int or_wma_batch_row(const double *data, size_t len, size_t period, double *out) {
+    size_t first;
+    int st = ma_prepare(data, len, period, 2, &first);
+    if (st) return st;
+    double *pa = (double *)malloc(sizeof(double) * (len + 1));
+    double *pb = (double *)malloc(sizeof(double) * (len + 1));
+    pa[0] = 0.0;
+    pb[0] = 0.0;
+    for (size_t i = 0; i < len; ++i) {
+        double x = i < first ? 0.0 : data[i];
+        pa[i + 1] = pa[i] + x;
+        pb[i + 1] = pb[i] + (double)i * x;
+    }
+    size_t warm = first + period - 1;
+    fill_nan(out, warm);
+    double denom = (double)(period * (period + 1)) / 2.0;
+    double inv_div = 1.0 / denom;
+    for (size_t i = warm; i < len; ++i) {
+        double s_a = pa[i + 1] - pa[i + 1 - period];
+        double s_b = pb[i + 1] - pb[i + 1 - period];
+        double w = s_b - ((double)(i + 1 - period) - 1.0) * s_a;
+        out[i] = w * inv_div;
+    }
+    free(pa);
+    free(pb);
+    return OR_OK;
+}
+
+/* swma.rs:406-476 swma_scalar (batch row swma_row_scalar :1155-1217 is the
+ * same): the symmetric triangle as two running box sums of lengths a and b
+ * (odd p: a = b = (p+1)/2; even p: a = p/2, b = a+1), a ring of the last b
+ * first-stage sums, out = s2 / (a*b).  p = 1 copies, p = 2 averages pairs. */
+int or_swma(const double *data, size_t len, size_t period, double *out) {
+    size_t first;
+    int st = ma_prepare(data, len, period, 1, &first);
+    if (st) return st;
+    fill_nan(out, first + period - 1);
+    if (period == 1) {
+        for (size_t i = first; i < len; ++i) out[i] = data[i];
+        return OR_OK;
+    }
+    if (period == 2) {
+        for (size_t i = first + 1; i < len; ++i) out[i] = (data[i - 1] + data[i]) * 0.5;
+        return OR_OK;
+    }
+    size_t a, b;
+    if (period & 1) {
+        a = (period + 1) >> 1;
+        b = a;
+    } else {
+        a = period >> 1;
+        b = a + 1;
+    }
+    double inv_ab = 1.0 / ((double)a * (double)b);
+    size_t start_a = first + a - 1, start_ab = first + period - 1;
+    double *ring = (double *)calloc(b, sizeof(double));
+    size_t rb = 0;
+    double s1 = 0.0, s2 = 0.0;
+    for (size_t i = first; i < len; ++i) {
+        s1 += data[i];
+        if (i >= start_a) {
+            double old = ring[rb];
+            s2 = s2 + (s1 - old);
+            ring[rb] = s1;
+            if (++rb == b) rb = 0;
+            if (i >= start_ab) out[i] = s2 * inv_ab;
+            s1 -= data[i + 1 - a];
+        }
+    }
+    free(ring);
+    return OR_OK;
+}
+
+/* smma.rs:243-271 smma_scalar: seed = mean of the first p bars at first+p-1,
+ * then prev = (prev*(p-1) + x) / p.  p = 1 copies. */
+int or_smma(const double *data, size_t len, size_t period, double *out) {
+    size_t first;
+    int st = ma_prepare(data, len, period, 1, &first);
+    if (st) return st;
+    size_t end = first + period;
+    fill_nan(out, end - 1);
+    if (period == 1) {
+        for (size_t i = first; i < len; ++i) out[i] = data[i];
+        return OR_OK;
+    }
+    double sum = 0.0;
+    for (size_t i = first; i < end; ++i) sum += data[i];
+    double pf = (double)period, pm1 = pf - 1.0;
+    double prev = sum / pf;
+    out[end - 1] = prev;
+    for (size_t i = end; i < len; ++i) {
+        prev = (prev * pm1 + data[i]) / pf;
+        out[i] = prev;
+    }
+    return OR_OK;
+}
+
+/* wilders.rs:195-231 (the first p bars after first must be finite) and
+ * wilders_scalar :262-320: seed summed in groups of four, y = sum * (1/p),
+ * then y = fma(x - y, 1/p, y).  p = 1 copies after the seed bar. */
+int or_wilders(const double *data, size_t len, size_t period, double *out) {
+    size_t first;
+    int st = ma_prepare(data, len, period, 1, &first);
+    if (st) return st;
+    for (size_t i = 0; i < period; ++i)
+        if (!is_finite_bits(data[first + i])) return OR_NOTENOUGH;
+    size_t ws = first + period - 1;
+    fill_nan(out, ws);
+    const double *p = data + first;
+    double sum = 0.0;
+    size_t c4 = period / 4;
+    for (size_t g = 0; g < c4; ++g, p += 4) sum += p[0] + p[1] + p[2] + p[3];
+    switch (period - c4 * 4) {
+        case 3: sum += p[0] + p[1] + p[2]; break;
+        case 2: sum += p[0] + p[1]; break;
+        case 1: sum += p[0]; break;
+        default: break;
+    }
+    double inv_n = 1.0 / (double)period;
+    double y = sum * inv_n;
+    out[ws] = y;
+    if (period == 1) {
+        for (size_t i = ws + 1; i < len; ++i) out[i] = data[i];
+        return OR_OK;
+    }
+    for (size_t i = ws + 1; i < len; ++i) {
+        y = fma(data[i] - y, inv_n, y);
+        out[i] = y;
+    }
+    return OR_OK;
+}
+
+/* One-period moving average of every column of a time-major [rows][cols]
+ * matrix.  kind: 0 sma, 1 wma, 2 swma, 3 smma, 4 wilders (scalar kernels),
+ * 5 sma batch row, 6 wma batch row (prefix forms). */
+void or_ma_many(int kind, const double *tm, size_t cols, size_t rows, size_t period,
+                double *out_tm, int *status, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(static) if (nthreads != 1)
+#endif
+    for (long s = 0; s < (long)cols; ++s) {
+        double *x = (double *)malloc(sizeof(double) * rows);
+        double *y = (double *)malloc(sizeof(double) * rows);
+        for (size_t t = 0; t < rows; ++t) x[t] = tm[t * cols + s];
+        fill_nan(y, rows);
+        int st = OR_PARAM;
+        switch (kind) {
+            case 0: st = or_sma(x, rows, period, y); break;
+            case 1: st = or_wma(x, rows, period, y); break;
+            case 2: st = or_swma(x, rows, period, y); break;
+            case 3: st = or_smma(x, rows, period, y); break;
+            case 4: st = or_wilders(x, rows, period, y); break;
+            case 5: st = or_sma_batch_row(x, rows, period, y); break;
+            case 6: st = or_wma_batch_row(x, rows, period, y); break;
+            default: break;
+        }
+        status[s] = st;
+        for (size_t t = 0; t < rows; ++t) out_tm[t * cols + s] = y[t];
+        free(x);
+        free(y);
     }
 }

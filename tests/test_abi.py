@@ -120,3 +120,20 @@ def test_rolling_expand_grids_match_reference_semantics():
     # medium_ad_wrapper.rs:120-150
     assert vta.CudaMediumAd.expand_grid((64, 60, 2)) == [64, 62, 60]
     assert vta.CudaMediumAd.expand_grid((5, 9, 0)) == [5]
+
+
+def test_moving_average_grids_match_reference_semantics():
+    # expand_grid_sma (sma.rs:705-755), swma_wrapper.rs:275-301, smma.rs:430-471,
+    # wilders_wrapper.rs:909-940: ascending start..=end, or counting down to end
+    for op in (vta.CudaSma, vta.CudaSwma, vta.CudaSmma, vta.CudaWilders):
+        e = op.expand_grid
+        assert e((5, 20, 5)) == [5, 10, 15, 20]
+        assert e((5, 22, 5)) == [5, 10, 15, 20]
+        assert e((10, 5, 3)) == [10, 7]
+        assert e((10, 4, 3)) == [10, 7, 4]
+        assert e((9, 9, 3)) == [9] and e((9, 30, 0)) == [9]
+    # wma_wrapper.rs:324-350: the countdown appends end when it misses it
+    e = vta.CudaWma.expand_grid
+    assert e((10, 5, 3)) == [10, 7, 5]
+    assert e((10, 4, 3)) == [10, 7, 4]
+    assert e((4, 12, 4)) == [4, 8, 12]

@@ -1,0 +1,228 @@
+"""SMA / WMA / SWMA / SMMA / Wilders parity on the GPU (batch, enqueue-only
+batch, many-series, many-series enqueue-only).
+
+Bar: NaN warm-up prefixes and the non-finite rules bit-exact (NaN masks and
+infinities equal); values within 1e-5 relative of the f64 oracle on the same
+f32 inputs.  The reference's own GPU tests (tests/{sma,wma,swma,smma,wilders}
+_cuda.rs) are also replayed with their data, sweeps and tolerances (absolute,
+against the CPU on the unrounded f64 data).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle as O
+import vectorta_amd as vta
+from vectorta_amd import datagen
+from vectorta_amd._lib import check
+from tests.conftest import require_gpu
+from tests.helpers import max_rel_err, nan_mask_equal
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ("sma", "wma", "swma", "smma", "wilders")
+
+
+@pytest.fixture(scope="module")
+def ops():
+    require_gpu()
+    return {"sma": vta.CudaSma(0), "wma": vta.CudaWma(0), "swma": vta.CudaSwma(0),
+            "smma": vta.CudaSmma(0), "wilders": vta.CudaWilders(0)}
+
+
+def _check(g, ref, what, bound=1e-5):
+    nan_mask_equal(g, ref)
+    r = np.asarray(ref, dtype=np.float64)
+    inf = np.isinf(r)
+    assert (np.asarray(g, dtype=np.float64)[inf] == r[inf]).all(), f"{what}: infinities differ"
+    err = max_rel_err(g, ref)
+    print(f"{what}: max rel err {err:.3e}")
+    assert err <= bound, (what, err)
+
+
+def _batch_oracle(name, x, periods):
+    """The reference's batch rows: prefix-sum rows for SMA; the running-sum
+    kernels for the others (the WMA prefix row loses ~eps*len^2/p^2 on long
+    series, so long WMA series are checked against the running sums)."""
+    xd = x.astype(np.float64)
+    f = {"sma": O.sma_batch_row, "wma": O.wma, "swma": O.swma, "smma": O.smma,
+         "wilders": O.wilders}[name]
+    return np.stack([f(xd, p) for p in periods])
+
+
+def _batch(ops, name, x, period):
+    return getattr(ops[name], f"{name}_batch_dev")(x, period).to_host()
+
+
+@pytest.mark.parametrize("name,period", [("sma", (1, 64, 9)), ("wma", (2, 64, 9)),
+                                         ("swma", (1, 64, 9)), ("smma", (1, 64, 9)),
+                                         ("wilders", (1, 64, 9))])
+def test_batch_small(ops, name, period):
+    x = datagen.gbm_close(5_000, 0.01, seed=201)
+    x[:9] = np.nan
+    ps = ops[name].expand_grid(period)
+    _check(_batch(ops, name, x, period), _batch_oracle(name, x, ps), f"{name} batch")
+    if name == "wma":   # small series: the reference's batch prefix rows too
+        _check(_batch(ops, name, x, period),
+               np.stack([O.wma_batch_row(x.astype(np.float64), p) for p in ps]), "wma prefix")
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_batch_long_series_many_tiles(ops, name):
+    """300k bars, periods up to 900: many 1024-bar tiles, several combo groups,
+    the recurrences' chunked burn-in."""
+    x = datagen.gbm_close(300_000, 0.01, seed=202)
+    x[:3] = np.nan
+    period = (2, 900, 47)
+    ps = ops[name].expand_grid(period)
+    assert len(ps) > 16
+    _check(_batch(ops, name, x, period), _batch_oracle(name, x, ps), f"{name} long batch")
+
+
+@pytest.mark.parametrize("bad", [np.nan, np.inf, -np.inf])
+def test_batch_nonfinite_rules(ops, bad):
+    """A non-finite bar after first_valid: window sums then NaN (SMA), NaN
+    from the bar on (WMA batch), [z, z+a) running sums then NaN (SWMA), the
+    recurrences' sequential values; the p <= 2 SWMA forms stay window-only."""
+    x = datagen.gbm_close(6_000, 0.01, seed=203)
+    x[:5] = np.nan
+    x[3_000] = bad
+    xd = x.astype(np.float64)
+    for name, period in (("sma", (1, 40, 13)), ("swma", (1, 40, 13)), ("smma", (1, 40, 13)),
+                         ("wilders", (1, 40, 13))):
+        ps = ops[name].expand_grid(period)
+        _check(_batch(ops, name, x, period), _batch_oracle(name, x, ps), f"{name} {bad}")
+    ps = ops["wma"].expand_grid((2, 40, 13))
+    ref = np.stack([O.wma_batch_row(xd, p) for p in ps])
+    ref[:, 3_000:] = np.nan        # the batch prefix rows: NaN from the bar on
+    _check(_batch(ops, "wma", x, (2, 40, 13)), ref, f"wma {bad}")
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_many_series(ops, name):
+    rng = np.random.default_rng(204)
+    rows, cols = 6_000, 70
+    tm = (100 + np.cumsum(rng.normal(0, 1, (rows, cols)), axis=0)).astype(np.float32)
+    lead = rng.integers(0, 400, size=cols)
+    for s in range(cols):
+        tm[:lead[s], s] = np.nan
+    if name != "wilders":
+        tm[2_500, 5] = np.nan
+        tm[4_000, 6] = np.inf
+    for p in ((2, 9, 33) if name == "wma" else (1, 2, 9, 33)):
+        g = getattr(ops[name], {"sma": "sma_multi_series_one_param_time_major_dev",
+                                "wma": "wma_multi_series_one_param_time_major_dev",
+                                "swma": "swma_multi_series_one_param_time_major_dev",
+                                "smma": "smma_multi_series_one_param_time_major_dev",
+                                "wilders": "wilders_many_series_one_param_time_major_dev"}[name])(
+            tm, cols, rows, p).to_host()
+        ref, st = O.ma_many(name, tm.astype(np.float64), p)
+        assert (st == 0).all()
+        _check(g, ref, f"{name} many-series p={p}")
+
+
+def test_device_entry_points(ops):
+    """Enqueue-only forms on device-resident inputs equal the host-input forms."""
+    x = datagen.gbm_close(20_000, 0.01, seed=205)
+    x[:4] = np.nan
+    x[12_000] = np.nan
+    ctx = ops["sma"].ctx
+    for name, period in (("sma", (3, 50, 7)), ("wma", (2, 50, 7)), ("swma", (1, 50, 7))):
+        op = ops[name]
+        ps = np.asarray(op.expand_grid(period), dtype=np.int32)
+        host = getattr(op, f"{name}_batch_dev")(x, period).to_host()
+        d_x = op.ctx.to_device(x)
+        d_p = op.ctx.to_device(ps)
+        d_o = op.ctx.alloc(ps.size * x.size * 4)
+        getattr(op, f"{name}_batch_device")(d_x, d_p, x.size, ps.size, 4, int(ps.max()), d_o)
+        op.synchronize()
+        dev = d_o.copy_to_host(np.float32).reshape(ps.size, x.size)
+        assert np.array_equal(host.view(np.uint32), dev.view(np.uint32)), name
+    for name, period in (("smma", (1, 50, 7)), ("wilders", (1, 50, 7))):
+        op = ops[name]
+        y = x.copy()
+        y[12_000] = 1.0
+        ps = np.asarray(op.expand_grid(period), dtype=np.int32)
+        host = getattr(op, f"{name}_batch_dev")(y, period).to_host()
+        d_x = op.ctx.to_device(y)
+        d_p = op.ctx.to_device(ps)
+        d_o = op.ctx.alloc(ps.size * y.size * 4)
+        getattr(op, f"{name}_batch_device")(d_x, d_p, int(ps.max()), y.size, 4, ps.size, d_o)
+        op.synchronize()
+        dev = d_o.copy_to_host(np.float32).reshape(ps.size, y.size)
+        assert np.array_equal(host.view(np.uint32), dev.view(np.uint32)), name
+    # many-series device form with prepared metadata (first_valid, first non-finite)
+    rng = np.random.default_rng(206)
+    rows, cols = 3_000, 40
+    tm = (50 + np.cumsum(rng.normal(0, 1, (rows, cols)), axis=0)).astype(np.float32)
+    tm[:17, 3] = np.nan
+    tm[1_000, 4] = np.nan
+    d_tm = ctx.to_device(tm)
+    d_fv = ctx.alloc(cols * 4)
+    d_z = ctx.alloc(cols * 4)
+    check(vta.lib().vta_many_series_prepare(ctx.handle, ctypes.c_void_p(d_tm.ptr), None, None,
+                                            cols, rows, 4, ctypes.c_void_p(d_fv.ptr), None,
+                                            ctypes.c_void_p(d_z.ptr)))
+    ctx.synchronize()
+    fv = d_fv.copy_to_host(np.int32)
+    z = d_z.copy_to_host(np.int32)
+    assert fv[3] == 17 and fv[0] == 0 and z[4] == 1_000 and z[0] == rows
+    for name in ("sma", "wma", "swma"):
+        op = ops[name]
+        host = getattr(op, f"{name}_multi_series_one_param_time_major_dev")(
+            tm, cols, rows, 12).to_host()
+        d_o = ctx.alloc(rows * cols * 4)
+        meth = {"sma": "sma_multi_series_one_param_device",
+                "wma": "wma_multi_series_one_param_time_major_device",
+                "swma": "swma_multi_series_one_param_device"}[name]
+        getattr(op, meth)(d_tm, d_fv, cols, rows, 12, d_o, d_first_nonfinite=d_z)
+        op.synchronize()
+        dev = d_o.copy_to_host(np.float32).reshape(rows, cols)
+        assert np.array_equal(host.view(np.uint32), dev.view(np.uint32)), name
+
+
+def _ref_data(n, lead, fa, fb):
+    d = np.full(n, np.nan)
+    i = np.arange(lead, n, dtype=np.float64)
+    d[lead:] = np.sin(i * fa) + fb * i
+    return d
+
+
+@pytest.mark.parametrize("name,n,lead,fa,fb,period,tol", [
+    ("sma", 4096, 5, 0.001, 0.0002, (5, 45, 5), 1e-5),        # tests/sma_cuda.rs:30-80
+    ("wma", 4096, 4, 0.002, 0.0002, (4, 128, 2), 1e-5),       # tests/wma_cuda.rs:30-80
+    ("swma", 4096, 10, 0.0013, 0.0005, (5, 160, 5), 2e-4),    # tests/swma_cuda.rs:30-75
+    ("smma", 4096, 12, 0.0017, 0.0003, (5, 140, 5), 2.5e-4),  # tests/smma_cuda.rs:30-75
+    ("wilders", 4096, 4, 0.0015, 0.0002, (5, 48, 3), 1e-5),   # tests/wilders_cuda.rs:30-75
+])
+def test_reference_gpu_tests_replayed(ops, name, n, lead, fa, fb, period, tol):
+    data = _ref_data(n, lead, fa, fb)
+    g = _batch(ops, name, data.astype(np.float32), period).astype(np.float64)
+    ps = ops[name].expand_grid(period)
+    f = {"sma": O.sma_batch_row, "wma": O.wma_batch_row, "swma": O.swma, "smma": O.smma,
+         "wilders": O.wilders}[name]
+    cpu = np.stack([f(data, p) for p in ps])
+    nan_mask_equal(g, cpu)
+    m = np.isfinite(cpu)
+    assert np.max(np.abs(g[m] - cpu[m])) <= tol
+
+
+def test_errors(ops):
+    x = datagen.gbm_close(100, 0.01, seed=207)
+    with pytest.raises(vta.VtaError):
+        ops["wma"].wma_batch_dev(x, (1, 5, 1))            # WMA needs period > 1
+    with pytest.raises(vta.VtaError):
+        ops["sma"].sma_batch_dev(x, (0, 0, 0))
+    with pytest.raises(vta.VtaError):
+        ops["swma"].swma_batch_dev(x, (101, 101, 0))      # period > len
+    y = x.copy()
+    y[:60] = np.nan
+    with pytest.raises(vta.VtaError):
+        ops["smma"].smma_batch_dev(y, (41, 41, 0))        # not enough valid data
+    z = x.copy()
+    z[5] = np.inf
+    with pytest.raises(vta.VtaError):
+        ops["wilders"].wilders_batch_dev(z, (3, 8, 1))    # non-finite in the warm window
+    with pytest.raises(vta.VtaError):
+        ops["sma"].sma_batch_dev(np.full(10, np.nan, np.float32), (2, 2, 0))

@@ -335,3 +335,117 @@ def test_cross_pairs_vectorised_and_antisymmetric():
         assert st[kk, 3] == -st[k, 3] and pnl[kk] == -pnl[k]
         if i == j:
             assert (st[k] == 0).all() and pnl[k] == 0.0
+
+
+# ------------------------------------------------------------ SMA / WMA / SWMA / SMMA / Wilders
+
+def test_sma_known_answer(candles):
+    # sma.rs:1625-1641 (SMA(9) of close, tol 1e-1)
+    exp = [59180.8, 59175.0, 59129.4, 59085.4, 59133.7]
+    for f in (O.sma, O.sma_batch_row):
+        assert np.all(np.abs(last5(f(candles["close"], 9)) - exp) < 1e-1)
+
+
+def test_wma_known_answer(candles):
+    # wma.rs:855-883 (WMA(30) of close, tol 1e-6)
+    exp = [59638.52903225806, 59563.7376344086, 59489.4064516129, 59432.02580645162,
+           59350.58279569892]
+    for f in (O.wma, O.wma_batch_row):
+        assert np.all(np.abs(last5(f(candles["close"], 30)) - exp) < 1e-6)
+
+
+def test_swma_known_answer(candles):
+    # swma.rs:1299-1318 (SWMA(5) of close, tol 1e-8)
+    exp = [59288.22222222222, 59301.99999999999, 59247.33333333333, 59179.88888888889,
+           59080.99999999999]
+    assert np.all(np.abs(last5(O.swma(candles["close"], 5)) - exp) < 1e-8)
+
+
+def test_smma_known_answer(candles):
+    # smma.rs:915-931 (SMMA(7) of close, tol 1e-1)
+    exp = [59434.4, 59398.2, 59346.9, 59319.4, 59224.5]
+    assert np.all(np.abs(last5(O.smma(candles["close"], 7)) - exp) < 1e-1)
+
+
+def test_wilders_known_answer(candles):
+    # wilders.rs:1226-1247 (Wilders(5) of close, tol 1e-8)
+    exp = [59302.18156619092, 59277.94525295273, 59230.15620236219, 59215.12496188975,
+           59103.0999695118]
+    assert np.all(np.abs(last5(O.wilders(candles["close"], 5)) - exp) < 1e-8)
+
+
+def _windows(x, p):
+    return np.lib.stride_tricks.sliding_window_view(x, p)
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 8, 31])
+def test_fir_averages_brute_force(p):
+    """Windowed dot products with the textbook weights: box (SMA), linear
+    1..p (WMA), symmetric triangle 1..m..1 (SWMA)."""
+    rng = np.random.default_rng(90 + p)
+    x = 50 + np.cumsum(rng.normal(0, 1, 400))
+    x[:6] = np.nan
+    w = _windows(x[6:], p)
+    tri = np.minimum(np.arange(1, p + 1), np.arange(p, 0, -1)).astype(float)
+    forms = {"sma": (O.sma, np.ones(p)), "sma_b": (O.sma_batch_row, np.ones(p)),
+             "swma": (O.swma, tri)}
+    if p >= 2:
+        forms["wma"] = (O.wma, np.arange(1, p + 1, dtype=float))
+        forms["wma_b"] = (O.wma_batch_row, np.arange(1, p + 1, dtype=float))
+    for name, (f, wt) in forms.items():
+        out = f(x, p)
+        assert np.isnan(out[:6 + p - 1]).all(), name
+        ref = w @ (wt / wt.sum())
+        # the batch prefix forms difference global prefix sums (sum i*x grows as
+        # len^2), which costs a few more digits than the running sums
+        rtol = 1e-9 if name.endswith("_b") else 1e-12
+        assert np.allclose(out[6 + p - 1:], ref, rtol=rtol, atol=0), name
+
+
+def test_swma_triangle_is_two_boxes():
+    """Closed form: SWMA_p = SMA_b(SMA_a(x)) with a + b = p + 1."""
+    rng = np.random.default_rng(95)
+    x = 10 + rng.random(300)
+    for p in (3, 4, 9, 10):
+        a, b = ((p + 1) // 2, (p + 1) // 2) if p % 2 else (p // 2, p // 2 + 1)
+        ref = O.sma(O.sma(x, a), b)
+        assert np.allclose(O.swma(x, p)[p - 1:], ref[p - 1:], rtol=1e-12), p
+
+
+def test_smma_wilders_closed_form():
+    """SMMA and Wilders are both y_t = y_{t-1} + (x_t - y_{t-1}) / p seeded by the
+    mean of the first p bars: check both against the unrolled sum
+    y_t = b^k y_w + a sum_j b^(k-j) x_(w+j), computed with numpy powers."""
+    rng = np.random.default_rng(96)
+    x = 100 + np.cumsum(rng.normal(0, 1, 200))
+    x[:3] = np.nan
+    for p in (1, 2, 5, 14):
+        w = 3 + p - 1
+        a, b = 1.0 / p, 1.0 - 1.0 / p
+        seed = np.mean(x[3:3 + p])
+        k = np.arange(1, x.size - w)
+        ref = np.empty(x.size - w)
+        ref[0] = seed
+        for i, kk in enumerate(k, start=1):
+            j = np.arange(1, kk + 1)
+            ref[i] = b ** kk * seed + a * np.sum(b ** (kk - j) * x[w + j])
+        for f in (O.smma, O.wilders):
+            out = f(x, p)
+            assert np.isnan(out[:w]).all()
+            assert np.allclose(out[w:], ref, rtol=1e-11), (f.__name__, p)
+
+
+def test_fir_nonfinite_rules():
+    """A NaN after first_valid poisons every later output of the running-sum
+    and prefix forms, but only the window of the p <= 2 SWMA forms."""
+    x = 10 + np.arange(60, dtype=float)
+    x[30] = np.nan
+    for f, p in ((O.sma, 4), (O.sma_batch_row, 4), (O.wma, 4), (O.wma_batch_row, 4),
+                 (O.swma, 5), (O.smma, 4)):
+        out = f(x, p)
+        assert np.isfinite(out[p - 1:30]).all() and np.isnan(out[30:]).all(), f.__name__
+    assert np.isnan(O.swma(x, 2)[30:32]).all() and np.isfinite(O.swma(x, 2)[32:]).all()
+    assert np.isnan(O.swma(x, 1)[30]) and np.isfinite(O.swma(x, 1)[31:]).all()
+    # Wilders rejects a non-finite bar inside the warm window (wilders.rs:223-230)
+    with pytest.raises(O.OracleError):
+        O.wilders(x[25:], 8)

@@ -9,7 +9,8 @@ from ._lib import VtaError, lib
 from .alma import AlmaBatchRange, AlmaParams, AlmaSweepPlan, CudaAlma, HipAlma, expand_grid
 from .cross import DoubleAlmaCrossover, DoubleAlmaPlan
 from .device import DeviceArrayF32, DeviceBuffer, HipContext, cuda_available, cuda_device_count
-from .rec import (CudaAdx, CudaAtr, CudaEma, CudaRsi, ManySeriesOhlcPlan,
+from .fir import CudaSma, CudaSwma, CudaWma
+from .rec import (CudaAdx, CudaAtr, CudaEma, CudaRsi, CudaSmma, CudaWilders, ManySeriesOhlcPlan,
                   ema_rsi_atr_many_series_dev)
 from .server import IndicatorServer, ServedResult
 from .rolling import (BollingerBandsBatchRange, CudaBollingerBands, CudaMediumAd,
@@ -23,5 +24,5 @@ __all__ = [
     "ema_rsi_atr_many_series_dev", "BollingerBandsBatchRange", "CudaBollingerBands",
     "CudaMediumAd", "CudaPercentileNearestRank", "PercentileNearestRankBatchRange",
     "RollingManySeriesPlan", "DoubleAlmaCrossover", "DoubleAlmaPlan", "IndicatorServer",
-    "ServedResult",
+    "ServedResult", "CudaSma", "CudaWma", "CudaSwma", "CudaSmma", "CudaWilders",
 ]

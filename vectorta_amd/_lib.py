@@ -139,6 +139,42 @@ SIGNATURES = {
     "vta_ema_many_series_one_param_time_major_dev": (_ST, [_P, _P, _SZ, _SZ, _SZ,
                                                            ctypes.POINTER(DeviceArrayF32C)]),
     "vta_ema_many_series_one_param_device": (_ST, [_P, _P, _P, _SZ, _SZ, _SZ, _P, _P]),
+    "vta_smma_expand_grid": (_ST, [ctypes.POINTER(RangeUsize), _P, _SZ, ctypes.POINTER(_SZ)]),
+    "vta_smma_batch_dev": (_ST, [_P, _P, _SZ, ctypes.POINTER(RangeUsize),
+                                ctypes.POINTER(DeviceArrayF32C)]),
+    "vta_smma_batch_device": (_ST, [_P, _P, _P, _I32, _I32, _I32, _I32, _P]),
+    "vta_smma_many_series_one_param_time_major_dev": (_ST, [_P, _P, _SZ, _SZ, _SZ,
+                                                           ctypes.POINTER(DeviceArrayF32C)]),
+    "vta_smma_many_series_one_param_device": (_ST, [_P, _P, _P, _SZ, _SZ, _SZ, _P, _P]),
+    "vta_wilders_expand_grid": (_ST, [ctypes.POINTER(RangeUsize), _P, _SZ, ctypes.POINTER(_SZ)]),
+    "vta_wilders_batch_dev": (_ST, [_P, _P, _SZ, ctypes.POINTER(RangeUsize),
+                                ctypes.POINTER(DeviceArrayF32C)]),
+    "vta_wilders_batch_device": (_ST, [_P, _P, _P, _I32, _I32, _I32, _I32, _P]),
+    "vta_wilders_many_series_one_param_time_major_dev": (_ST, [_P, _P, _SZ, _SZ, _SZ,
+                                                           ctypes.POINTER(DeviceArrayF32C)]),
+    "vta_wilders_many_series_one_param_device": (_ST, [_P, _P, _P, _SZ, _SZ, _SZ, _P, _P]),
+    # finite-window averages
+    "vta_sma_expand_grid": (_ST, [ctypes.POINTER(RangeUsize), _P, _SZ, ctypes.POINTER(_SZ)]),
+    "vta_sma_batch_dev": (_ST, [_P, _P, _SZ, ctypes.POINTER(RangeUsize),
+                                ctypes.POINTER(DeviceArrayF32C)]),
+    "vta_sma_batch_device": (_ST, [_P, _P, _P, _I32, _I32, _I32, _I32, _P]),
+    "vta_sma_many_series_one_param_time_major_dev": (_ST, [_P, _P, _SZ, _SZ, _SZ,
+                                                           ctypes.POINTER(DeviceArrayF32C)]),
+    "vta_sma_many_series_one_param_device": (_ST, [_P, _P, _P, _P, _SZ, _SZ, _SZ, _P]),
+    "vta_wma_expand_grid": (_ST, [ctypes.POINTER(RangeUsize), _P, _SZ, ctypes.POINTER(_SZ)]),
+    "vta_wma_batch_dev": (_ST, [_P, _P, _SZ, ctypes.POINTER(RangeUsize),
+                                ctypes.POINTER(DeviceArrayF32C)]),
+    "vta_wma_batch_device": (_ST, [_P, _P, _P, _I32, _I32, _I32, _I32, _P]),
+    "vta_wma_many_series_one_param_time_major_dev": (_ST, [_P, _P, _SZ, _SZ, _SZ,
+                                                           ctypes.POINTER(DeviceArrayF32C)]),
+    "vta_wma_many_series_one_param_device": (_ST, [_P, _P, _P, _P, _SZ, _SZ, _SZ, _P]),
+    "vta_swma_expand_grid": (_ST, [ctypes.POINTER(RangeUsize), _P, _SZ, ctypes.POINTER(_SZ)]),
+    "vta_swma_batch_dev": (_ST, [_P, _P, _SZ, ctypes.POINTER(RangeUsize),
+                                ctypes.POINTER(DeviceArrayF32C)]),
+    "vta_swma_batch_device": (_ST, [_P, _P, _P, _I32, _I32, _I32, _I32, _P]),
+    "vta_swma_many_series_one_param_time_major_dev": (_ST, [_P, _P, _SZ, _SZ, _SZ,
+                                                           ctypes.POINTER(DeviceArrayF32C)]),
+    "vta_swma_many_series_one_param_device": (_ST, [_P, _P, _P, _P, _SZ, _SZ, _SZ, _P]),
     "vta_rsi_expand_grid": (_ST, [ctypes.POINTER(RangeUsize), _P, _SZ, ctypes.POINTER(_SZ)]),
     "vta_rsi_batch_dev": (_ST, [_P, _P, _SZ, ctypes.POINTER(RangeUsize),
                                 ctypes.POINTER(DeviceArrayF32C)]),

@@ -1,0 +1,281 @@
+// Host entry points for the finite-window moving averages: CudaSma
+// (src/cuda/moving_averages/sma_wrapper.rs), CudaWma (wma_wrapper.rs) and
+// CudaSwma (swma_wrapper.rs).  Kernels: kernels/fir.hip.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "runtime_internal.h"
+
+using namespace vta;
+
+namespace {
+
+// Host mirror of struct fir_args in kernels/fir.hip (same field order/types).
+struct fir_args_host {
+    const float *x;
+    const int32_t *periods;
+    const int32_t *meta;
+    const int32_t *fvs;
+    const int32_t *zs;
+    float *out;
+    int T;
+    int nlanes;
+    int kind;
+    int tile;
+    int span_cap;
+    int chunk;
+    int period;
+    int pad_;
+};
+static_assert(sizeof(fir_args_host) == 80, "fir_args layout");
+
+enum Fir { FIR_SMA = 0, FIR_WMA = 1, FIR_SWMA = 2 };
+constexpr unsigned kGroup = 16;  // FIR_GC in fir.hip
+
+// Grids: SMA sma.rs:705-755 (expand_grid_sma), WMA wma_wrapper.rs:324-350
+// (countdown plus end), SWMA swma_wrapper.rs:275-301.
+vta_status expand(Fir k, const vta_range_usize *r, std::vector<size_t> &ps) {
+    if (!r) return fail(VTA_ERR_INVALID_INPUT, "range is null");
+    if (!expand_usize(*r, k == FIR_WMA ? AXIS_DOWN_END : AXIS_DOWN, ps) || ps.empty())
+        return fail(VTA_ERR_INVALID_INPUT, "invalid range: start=%zu end=%zu step=%zu", r->start,
+                    r->end, r->step);
+    return VTA_OK;
+}
+
+// Period validation of the wrappers' prepare_batch_inputs: SMA and SWMA need
+// period >= 1, WMA period > 1 (wma_wrapper.rs:379-384); all need period <= len
+// and len - first_valid >= period.
+size_t min_period(Fir k) { return k == FIR_WMA ? 2 : 1; }
+
+vta_status check_period(Fir k, size_t p, size_t len, size_t fv) {
+    if (p < min_period(k))
+        return fail(VTA_ERR_INVALID_INPUT, "invalid period %zu (must be >= %zu)", p,
+                    min_period(k));
+    if (p > len) return fail(VTA_ERR_INVALID_INPUT, "period %zu exceeds data length %zu", p, len);
+    if (len - fv < p)
+        return fail(VTA_ERR_INVALID_INPUT, "not enough valid data: needed >= %zu, valid = %zu", p,
+                    len - fv);
+    return VTA_OK;
+}
+
+// Batch launch: tile of 1024 bars unless the prefix arrays of the tile plus
+// the longest period would leave fewer than two workgroups per CU (then 512,
+// 256); the LDS holds (tile + max_p + 2) doubles per prefix array.
+vta_status batch_launch(vta_ctx *ctx, Fir k, const float *d_x, const int32_t *d_periods,
+                        const int32_t *d_meta, size_t T, size_t C, size_t max_p, float *d_out) {
+    const size_t narr = k == FIR_SMA ? 1 : 2;
+    const size_t limit = ctx->props.sharedMemPerBlock;
+    size_t tile = 1024, smem = 0;
+    for (;; tile >>= 1) {
+        smem = narr * (tile + max_p + 3) * sizeof(double);
+        if (smem <= 64 * 1024 || tile == 256) break;
+    }
+    if (smem > limit)
+        return fail(VTA_ERR_INVALID_INPUT,
+                    "period %zu exceeds the LDS window of one workgroup (%zu bytes)", max_p,
+                    limit);
+    hipFunction_t f;
+    VTA_TRY(get_function(ctx, "fir", "fir_batch_f32", &f));
+    fir_args_host a{};
+    a.x = d_x;
+    a.periods = d_periods;
+    a.meta = d_meta;
+    a.out = d_out;
+    a.T = (int)T;
+    a.nlanes = (int)C;
+    a.kind = (int)k;
+    a.tile = (int)tile;
+    a.span_cap = (int)(tile + max_p + 1);
+    unsigned gx = (unsigned)((T + tile - 1) / tile);
+    unsigned gy = (unsigned)((C + kGroup - 1) / kGroup);
+    if (gy > (unsigned)ctx->props.maxGridSize[1])
+        return fail(VTA_ERR_LAUNCH_CONFIG_TOO_LARGE, "too many combos for one launch (%zu)", C);
+    return launch(ctx, f, dim3(gx, gy, 1), dim3(256), (unsigned)smem, a);
+}
+
+// Many-series chunk: at least 4x the replayed look-back (p bars), at least
+// 1024 bars, small enough for ~2048 waves.
+int pick_chunk(size_t p, size_t T, size_t groups) {
+    size_t want = std::max<size_t>(1, 2048 / std::max<size_t>(groups, 1));
+    size_t c = std::max<size_t>({4 * p, 1024, (T + want - 1) / want});
+    c = (c + 63) & ~(size_t)63;
+    return (int)std::min(c, T);
+}
+
+vta_status ms_launch(vta_ctx *ctx, Fir k, const float *d_x, const int32_t *d_fvs,
+                     const int32_t *d_zs, size_t cols, size_t rows, size_t period, float *d_out) {
+    hipFunction_t f;
+    VTA_TRY(get_function(ctx, "fir", "fir_ms1p_f32", &f));
+    fir_args_host a{};
+    a.x = d_x;
+    a.fvs = d_fvs;
+    a.zs = d_zs;
+    a.out = d_out;
+    a.T = (int)rows;
+    a.nlanes = (int)cols;
+    a.kind = (int)k;
+    a.period = (int)period;
+    unsigned groups = (unsigned)((cols + 63) / 64);
+    a.chunk = pick_chunk(period, rows, groups);
+    unsigned chunks = (unsigned)((rows + a.chunk - 1) / a.chunk);
+    return launch(ctx, f, dim3(groups, chunks, 1), dim3(64), 0, a);
+}
+
+int32_t first_nonfinite(const float *x, size_t n, size_t fv, size_t ld) {
+    for (size_t i = fv; i < n; ++i)
+        if (!std::isfinite(x[i * ld])) return (int32_t)i;
+    return (int32_t)n;
+}
+
+vta_status batch_dev(vta_ctx *ctx, Fir k, const float *x, size_t len,
+                     const vta_range_usize *range, vta_device_array_f32 *out) {
+    clear_error();
+    if (!ctx || !x || !out) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    if (len == 0) return fail(VTA_ERR_INVALID_INPUT, "empty data");
+    if (len > (size_t)INT32_MAX) return fail(VTA_ERR_INVALID_INPUT, "series too long");
+    size_t fv = first_valid_f32(x, len);
+    if (fv >= len) return fail(VTA_ERR_INVALID_INPUT, "all values are NaN");
+    std::vector<size_t> ps;
+    VTA_TRY(expand(k, range, ps));
+    for (size_t p : ps) VTA_TRY(check_period(k, p, len, fv));
+    const size_t C = ps.size();
+    const size_t max_p = *std::max_element(ps.begin(), ps.end());
+    size_t n_out;
+    if (!mul_ok(C, len, &n_out) || !mul_ok(n_out, sizeof(float), &n_out))
+        return fail(VTA_ERR_INVALID_INPUT, "output bytes overflow");
+    VTA_TRY(check_fits(n_out + len * sizeof(float)));
+    std::vector<int32_t> per(ps.begin(), ps.end());
+    int32_t meta[2] = {(int32_t)fv, first_nonfinite(x, len, fv, 1)};
+    DevBuf dx, dp, dm, dout;
+    VTA_TRY(upload(ctx, dx, x, len * sizeof(float)));
+    VTA_TRY(upload(ctx, dp, per.data(), C * sizeof(int32_t)));
+    VTA_TRY(upload(ctx, dm, meta, sizeof(meta)));
+    VTA_TRY(dout.alloc(n_out));
+    VTA_TRY(batch_launch(ctx, k, dx.as<float>(), dp.as<int32_t>(), dm.as<int32_t>(), len, C,
+                         max_p, dout.as<float>()));
+    VTA_HIP(hipStreamSynchronize(ctx->stream));
+    out->ptr = static_cast<float *>(dout.release());
+    out->rows = C;
+    out->cols = len;
+    return VTA_OK;
+}
+
+// Device-resident one-series batch, enqueue only: first_valid from the
+// caller, the first non-finite bar found on the device into the context
+// scratch.
+vta_status batch_device(vta_ctx *ctx, Fir k, const float *d_x, const int32_t *d_periods,
+                        int32_t series_len, int32_t n_combos, int32_t first_valid,
+                        int32_t max_period, float *d_out) {
+    clear_error();
+    if (!ctx || !d_x || !d_periods || !d_out) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    if (series_len <= 0 || n_combos <= 0 || first_valid < 0 || first_valid >= series_len ||
+        max_period < (int32_t)min_period(k) || max_period > series_len - first_valid)
+        return fail(VTA_ERR_INVALID_INPUT,
+                    "series_len, n_combos must be positive, first_valid < series_len and "
+                    "%zu <= max_period <= series_len - first_valid", min_period(k));
+    int32_t *meta = static_cast<int32_t *>(ctx->scratch) + 192;
+    hipFunction_t fill, fnf;
+    VTA_TRY(get_function(ctx, "rec", "vta_fill_i32", &fill));
+    VTA_TRY(get_function(ctx, "fir", "series_first_nonfinite", &fnf));
+    VTA_TRY(launch(ctx, fill, dim3(1), dim3(256), 0, meta, 1, (int)first_valid));
+    VTA_TRY(launch(ctx, fill, dim3(1), dim3(256), 0, meta + 1, 1, (int)series_len));
+    unsigned nb = (unsigned)((series_len - first_valid + 255) / 256);
+    VTA_TRY(launch(ctx, fnf, dim3(nb), dim3(256), 0, d_x, (int)series_len, (int)first_valid,
+                   meta + 1));
+    return batch_launch(ctx, k, d_x, d_periods, meta, (size_t)series_len, (size_t)n_combos,
+                        (size_t)max_period, d_out);
+}
+
+vta_status ms_dev(vta_ctx *ctx, Fir k, const float *x_tm, size_t cols, size_t rows,
+                  size_t period, vta_device_array_f32 *out) {
+    clear_error();
+    if (!ctx || !x_tm || !out) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    if (cols == 0 || rows == 0) return fail(VTA_ERR_INVALID_INPUT, "cols or rows is zero");
+    if (period < min_period(k) || period > rows)
+        return fail(VTA_ERR_INVALID_INPUT, "invalid period %zu", period);
+    if (rows > (size_t)INT32_MAX || cols > (size_t)INT32_MAX)
+        return fail(VTA_ERR_INVALID_INPUT, "dimensions exceed int32");
+    size_t elems;
+    if (!mul_ok(cols, rows, &elems)) return fail(VTA_ERR_INVALID_INPUT, "cols*rows overflow");
+    std::vector<int32_t> fvs, zs(cols);
+    VTA_TRY(host_first_valids(x_tm, cols, rows, period, fvs));
+    for (size_t s = 0; s < cols; ++s) zs[s] = first_nonfinite(x_tm + s, rows, fvs[s], cols);
+    VTA_TRY(check_fits(2 * elems * sizeof(float)));
+    DevBuf dx, dfv, dz, dout;
+    VTA_TRY(upload(ctx, dx, x_tm, elems * sizeof(float)));
+    VTA_TRY(upload(ctx, dfv, fvs.data(), cols * sizeof(int32_t)));
+    VTA_TRY(upload(ctx, dz, zs.data(), cols * sizeof(int32_t)));
+    VTA_TRY(dout.alloc(elems * sizeof(float)));
+    VTA_TRY(ms_launch(ctx, k, dx.as<float>(), dfv.as<int32_t>(), dz.as<int32_t>(), cols, rows,
+                      period, dout.as<float>()));
+    VTA_HIP(hipStreamSynchronize(ctx->stream));
+    out->ptr = static_cast<float *>(dout.release());
+    out->rows = rows;
+    out->cols = cols;
+    return VTA_OK;
+}
+
+vta_status ms_device(vta_ctx *ctx, Fir k, const float *d_x_tm, const int32_t *d_fvs,
+                     const int32_t *d_zs, size_t cols, size_t rows, size_t period,
+                     float *d_out_tm) {
+    clear_error();
+    if (!ctx || !d_x_tm || !d_fvs || !d_out_tm || cols == 0 || rows == 0 ||
+        period < min_period(k) || period > rows)
+        return fail(VTA_ERR_INVALID_INPUT, "invalid arguments");
+    if (rows > (size_t)INT32_MAX || cols > (size_t)INT32_MAX)
+        return fail(VTA_ERR_INVALID_INPUT, "dimensions exceed int32");
+    return ms_launch(ctx, k, d_x_tm, d_fvs, d_zs, cols, rows, period, d_out_tm);
+}
+
+vta_status expand_out(Fir k, const vta_range_usize *range, size_t *out, size_t cap,
+                      size_t *n_out) {
+    clear_error();
+    std::vector<size_t> ps;
+    VTA_TRY(expand(k, range, ps));
+    if (n_out) *n_out = ps.size();
+    if (out) memcpy(out, ps.data(), std::min(cap, ps.size()) * sizeof(size_t));
+    return VTA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+#define VTA_FIR_ENTRY_POINTS(NAME, KIND)                                                        \
+    vta_status vta_##NAME##_expand_grid(const vta_range_usize *range, size_t *periods,        \
+                                        size_t cap, size_t *n_out) {                          \
+        return expand_out(KIND, range, periods, cap, n_out);                                  \
+    }                                                                                         \
+    vta_status vta_##NAME##_batch_dev(vta_ctx *ctx, const float *prices, size_t len,          \
+                                      const vta_range_usize *range,                           \
+                                      vta_device_array_f32 *out) {                            \
+        return batch_dev(ctx, KIND, prices, len, range, out);                                 \
+    }                                                                                         \
+    vta_status vta_##NAME##_batch_device(vta_ctx *ctx, const float *d_prices,                 \
+                                         const int32_t *d_periods, int32_t series_len,        \
+                                         int32_t n_combos, int32_t first_valid,               \
+                                         int32_t max_period, float *d_out) {                  \
+        return batch_device(ctx, KIND, d_prices, d_periods, series_len, n_combos,             \
+                            first_valid, max_period, d_out);                                  \
+    }                                                                                         \
+    vta_status vta_##NAME##_many_series_one_param_time_major_dev(                             \
+        vta_ctx *ctx, const float *data_tm, size_t cols, size_t rows, size_t period,          \
+        vta_device_array_f32 *out) {                                                          \
+        return ms_dev(ctx, KIND, data_tm, cols, rows, period, out);                           \
+    }                                                                                         \
+    vta_status vta_##NAME##_many_series_one_param_device(                                     \
+        vta_ctx *ctx, const float *d_prices_tm, const int32_t *d_first_valids,                \
+        const int32_t *d_first_nonfinite, size_t cols, size_t rows, size_t period,            \
+        float *d_out_tm) {                                                                    \
+        return ms_device(ctx, KIND, d_prices_tm, d_first_valids, d_first_nonfinite, cols,     \
+                         rows, period, d_out_tm);                                             \
+    }
+
+VTA_FIR_ENTRY_POINTS(sma, FIR_SMA)
+VTA_FIR_ENTRY_POINTS(wma, FIR_WMA)
+VTA_FIR_ENTRY_POINTS(swma, FIR_SWMA)
+#undef VTA_FIR_ENTRY_POINTS
+
+}  // extern "C"

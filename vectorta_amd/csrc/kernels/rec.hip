@@ -8,6 +8,8 @@
 //   RSI  src/indicators/rsi.rs:327-414
 //   ATR  src/indicators/atr.rs:329-460
 //   ADX  src/indicators/adx.rs:299-422
+//   SMMA src/indicators/moving_averages/smma.rs:243-271
+//   Wilders src/indicators/moving_averages/wilders.rs:262-320
 // so a lane that starts exactly at first_valid reproduces the oracle's f64
 // values and only rounds them to f32 on output.
 //
@@ -242,6 +244,69 @@ struct AdxEng {
     }
 };
 
+// SMMA (smma.rs:243-271): the mean of the first p bars at fv+p-1, then
+// prev = (prev*(p-1) + x) / p (here times 1/p).  p = 1 copies the input.
+struct SmmaEng {
+    static constexpr bool kSeedScan = false;
+    double pm1, rp, prev, sum;
+    int p, fv;
+    __device__ void setup(int p_, int fv_) {
+        p = p_; fv = fv_;
+        pm1 = (double)p - 1.0;
+        rp = 1.0 / (double)p;
+    }
+    __device__ int kb() const { return burn_bars(1.0 - rp, 25.0); }
+    __device__ int exact_start() const { return fv; }
+    __device__ int burn_ok_from() const { return fv + p; }
+    __device__ double begin_exact(double x) {
+        sum = x;
+        if (p == 1) return x;
+        return REC_NAN;
+    }
+    __device__ void begin_burn(double x) { prev = x; }
+    __device__ double step(int t, double x) {
+        if (p == 1) return x;
+        const int ws = fv + p - 1;
+        if (t < ws) { sum += x; return REC_NAN; }
+        if (t == ws) { sum += x; prev = sum / (double)p; return prev; }
+        prev = (prev * pm1 + x) * rp;
+        return prev;
+    }
+    __device__ double out_before_start(int t) const { return REC_NAN; }
+};
+
+// Wilders (wilders.rs:262-320): seed summed in groups of four,
+// sum += ((x0 + x1) + x2) + x3, y = sum * (1/p) at fv+p-1, then
+// y = fma(x - y, 1/p, y).  p = 1 copies after the seed bar.
+struct WildersEng {
+    static constexpr bool kSeedScan = false;
+    double al, y, sum, g;
+    int p, fv;
+    __device__ void setup(int p_, int fv_) {
+        p = p_; fv = fv_;
+        al = 1.0 / (double)p;
+    }
+    __device__ int kb() const { return burn_bars(1.0 - al, 25.0); }
+    __device__ int exact_start() const { return fv; }
+    __device__ int burn_ok_from() const { return fv + p; }
+    __device__ double seed_bar(int k, double x) {       // k = t - fv < p
+        if ((k & 3) == 0) g = x; else g += x;
+        if ((k & 3) == 3 || k == p - 1) sum += g;
+        if (k == p - 1) { y = sum * al; return y; }
+        return REC_NAN;
+    }
+    __device__ double begin_exact(double x) { sum = 0.0; return seed_bar(0, x); }
+    __device__ void begin_burn(double x) { y = x; }
+    __device__ double step(int t, double x) {
+        const int k = t - fv;
+        if (k < p) return seed_bar(k, x);
+        if (p == 1) return x;
+        y = fma(x - y, al, y);
+        return y;
+    }
+    __device__ double out_before_start(int t) const { return REC_NAN; }
+};
+
 // ---------------------------------------------------------------- chunk planning
 
 // Bar at which a lane starts (exact or burn-in) for chunk [c0, c1).
@@ -349,6 +414,12 @@ extern "C" __global__ __launch_bounds__(64) void rsi_ms1p_f32(rec_args a) {
 }
 extern "C" __global__ __launch_bounds__(64) void atr_ms1p_f32(rec_args a) {
     drive_ms<AtrEng, 3>(a, a.periods[0], a.out0);
+}
+extern "C" __global__ __launch_bounds__(64) void smma_ms1p_f32(rec_args a) {
+    drive_ms<SmmaEng, 1>(a, a.periods[0], a.out0);
+}
+extern "C" __global__ __launch_bounds__(64) void wilders_ms1p_f32(rec_args a) {
+    drive_ms<WildersEng, 1>(a, a.periods[0], a.out0);
 }
 extern "C" __global__ __launch_bounds__(64) void adx_ms1p_f32(rec_args a) {
     drive_ms<AdxEng, 3>(a, a.periods[0], a.out0);
@@ -593,6 +664,12 @@ extern "C" __global__ __launch_bounds__(64) void rsi_batch_f32(rec_args a) {
 extern "C" __global__ __launch_bounds__(64) void atr_batch_f32(rec_args a) {
     drive_batch<AtrEng, 3>(a, a.out0);
 }
+extern "C" __global__ __launch_bounds__(64) void smma_batch_f32(rec_args a) {
+    drive_batch<SmmaEng, 1>(a, a.out0);
+}
+extern "C" __global__ __launch_bounds__(64) void wilders_batch_f32(rec_args a) {
+    drive_batch<WildersEng, 1>(a, a.out0);
+}
 extern "C" __global__ __launch_bounds__(64) void adx_batch_f32(rec_args a) {
     drive_batch<AdxEng, 3>(a, a.out0);
 }
@@ -607,13 +684,15 @@ extern "C" __global__ __launch_bounds__(64) void adx_batch_f32(rec_args a) {
 //   mode 2: max of the per-input first non-NaN bars (ADX, adx.rs:190-198)
 //   mode 3: fv = first non-NaN x, fv2 = mode-1 bar, clean over all inputs
 //           from fv on (fused EMA/RSI/ATR)
+//   mode 4: fv = first non-NaN x; clean[s] receives the first bar at or after
+//           fv whose x is not finite (T if none), for the windowed averages
 extern "C" __global__ __launch_bounds__(64) void ms_prepare_meta(
     const float *__restrict__ x, const float *__restrict__ h, const float *__restrict__ l,
     int T, int N, int mode, int32_t *__restrict__ fvs, int32_t *__restrict__ fvs2,
     int32_t *__restrict__ clean) {
     const int s = blockIdx.x * 64 + (threadIdx.x & 63);
     if (s >= N) return;
-    const bool hl = mode != 0;
+    const bool hl = mode == 1 || mode == 2 || mode == 3;
     int fx = T, fh = T, fl = T, f3 = T;
     for (int t = 0; t < T; ++t) {
         const size_t off = (size_t)t * N + s;
@@ -627,14 +706,19 @@ extern "C" __global__ __launch_bounds__(64) void ms_prepare_meta(
         if (fx < T && f3 < T && fh < T && fl < T) break;
     }
     int f = mode == 0 ? fx : mode == 1 ? f3 : mode == 2 ? max(fx, max(fh, fl)) : fx;
-    bool ok = true;
+    int zf = T;
     for (int t = f; t < T; ++t) {
         const size_t off = (size_t)t * N + s;
-        ok = ok && vta_isfinite(x[off]) && (!hl || (vta_isfinite(h[off]) && vta_isfinite(l[off])));
+        const bool ok = vta_isfinite(x[off]) &&
+                        (!hl || (vta_isfinite(h[off]) && vta_isfinite(l[off])));
+        if (!ok) {
+            zf = t;
+            break;
+        }
     }
     fvs[s] = f;
     if (fvs2) fvs2[s] = f3;
-    clean[s] = ok ? 1 : 0;
+    clean[s] = mode == 4 ? zf : (zf == T ? 1 : 0);
 }
 
 // Single series: clean flag = no non-finite input in [first_valid, T).

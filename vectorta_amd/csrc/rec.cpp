@@ -32,14 +32,19 @@ struct rec_args_host {
     int p1, p2, p3;
 };
 
-enum Ind { IND_EMA = 0, IND_RSI = 1, IND_ATR = 2, IND_ADX = 3 };
+enum Ind { IND_EMA = 0, IND_RSI = 1, IND_ATR = 2, IND_ADX = 3, IND_SMMA = 4, IND_WILDERS = 5 };
+
+// Indicators computed from (high, low, close).
+bool hlc(Ind k) { return k == IND_ATR || k == IND_ADX; }
 
 const char *batch_kernel(Ind k) {
-    static const char *n[] = {"ema_batch_f32", "rsi_batch_f32", "atr_batch_f32", "adx_batch_f32"};
+    static const char *n[] = {"ema_batch_f32",  "rsi_batch_f32",  "atr_batch_f32",
+                              "adx_batch_f32",  "smma_batch_f32", "wilders_batch_f32"};
     return n[k];
 }
 const char *ms_kernel(Ind k) {
-    static const char *n[] = {"ema_ms1p_f32", "rsi_ms1p_f32", "atr_ms1p_f32", "adx_ms1p_f32"};
+    static const char *n[] = {"ema_ms1p_f32",  "rsi_ms1p_f32",  "atr_ms1p_f32",
+                              "adx_ms1p_f32",  "smma_ms1p_f32", "wilders_ms1p_f32"};
     return n[k];
 }
 
@@ -91,7 +96,11 @@ int pick_chunk_batch(Ind k, size_t max_p, size_t T, size_t lane_groups, bool cle
 // Grid expansion for the one-parameter indicators.
 vta_status expand_periods(Ind k, const vta_range_usize *r, std::vector<size_t> &ps) {
     if (!r) return fail(VTA_ERR_INVALID_INPUT, "range is null");
-    AxisKind ak = (k == IND_ATR) ? AXIS_ATR : (k == IND_ADX) ? AXIS_DOWN : AXIS_SORTED;
+    // SMMA: smma.rs:430-471; Wilders: wilders_wrapper.rs:909-940 (both count
+    // down to end when start > end, like AXIS_DOWN)
+    AxisKind ak = (k == IND_ATR)                                        ? AXIS_ATR
+                  : (k == IND_ADX || k == IND_SMMA || k == IND_WILDERS) ? AXIS_DOWN
+                                                                        : AXIS_SORTED;
     if (!expand_usize(*r, ak, ps) || ps.empty())
         return fail(VTA_ERR_INVALID_INPUT, "invalid period range: start=%zu end=%zu step=%zu",
                     r->start, r->end, r->step);
@@ -106,7 +115,7 @@ vta_status expand_periods(Ind k, const vta_range_usize *r, std::vector<size_t> &
 size_t need_bars(Ind k, size_t p) { return k == IND_ADX ? p + 1 : p; }
 
 size_t first_valid_of(Ind k, const float *x, const float *h, const float *l, size_t n) {
-    if (k == IND_EMA || k == IND_RSI) return first_valid_f32(x, n);
+    if (!hlc(k)) return first_valid_f32(x, n);
     if (k == IND_ATR) {
         size_t i = 0;
         while (i < n && (std::isnan(h[i]) || std::isnan(l[i]) || std::isnan(x[i]))) ++i;
@@ -155,7 +164,7 @@ vta_status batch_launch(vta_ctx *ctx, Ind k, const float *d_x, const float *d_h,
 vta_status batch_dev(vta_ctx *ctx, Ind k, const float *x, const float *h, const float *l,
                      size_t len, const vta_range_usize *range, vta_device_array_f32 *out) {
     clear_error();
-    if (!ctx || !out || !x || (k >= IND_ATR && (!h || !l)))
+    if (!ctx || !out || !x || (hlc(k) && (!h || !l)))
         return fail(VTA_ERR_INVALID_INPUT, "null argument");
     if (len == 0) return fail(VTA_ERR_INVALID_INPUT, "empty data");
     std::vector<size_t> ps;
@@ -167,11 +176,18 @@ vta_status batch_dev(vta_ctx *ctx, Ind k, const float *x, const float *h, const 
         return fail(VTA_ERR_INVALID_INPUT, "not enough valid data (needed >= %zu, valid = %zu)",
                     need_bars(k, max_p), len - fv);
     if (len > (size_t)INT32_MAX) return fail(VTA_ERR_INVALID_INPUT, "series too long");
-    bool clean = host_clean(x, k >= IND_ATR ? h : nullptr, l, len, fv);
+    if (k == IND_WILDERS) {
+        // wilders_wrapper.rs:391-400: every bar of the longest warm window finite
+        for (size_t i = 0; i < max_p; ++i)
+            if (!std::isfinite(x[fv + i]))
+                return fail(VTA_ERR_INVALID_INPUT, "non-finite value in warm window at offset %zu",
+                            i);
+    }
+    bool clean = host_clean(x, hlc(k) ? h : nullptr, l, len, fv);
     size_t C = ps.size(), n_out;
     if (!mul_ok(C, len, &n_out) || !mul_ok(n_out, sizeof(float), &n_out))
         return fail(VTA_ERR_INVALID_INPUT, "output bytes overflow");
-    size_t nin = k >= IND_ATR ? 3 : 1;
+    size_t nin = hlc(k) ? 3 : 1;
     VTA_TRY(check_fits(n_out + nin * len * sizeof(float)));
     std::vector<int32_t> per(ps.begin(), ps.end());
     int32_t meta[2] = {(int32_t)fv, clean ? 1 : 0};
@@ -201,7 +217,7 @@ vta_status batch_device(vta_ctx *ctx, Ind k, const float *d_x, const float *d_h,
                         int32_t series_len, int32_t first_valid, int32_t n_combos,
                         float *d_out) {
     clear_error();
-    if (!ctx || !d_x || !d_periods || !d_out || (k >= IND_ATR && (!d_h || !d_l)))
+    if (!ctx || !d_x || !d_periods || !d_out || (hlc(k) && (!d_h || !d_l)))
         return fail(VTA_ERR_INVALID_INPUT, "null argument");
     if (series_len <= 0 || n_combos <= 0 || max_period <= 0 || first_valid < 0 ||
         first_valid >= series_len)
@@ -216,8 +232,8 @@ vta_status batch_device(vta_ctx *ctx, Ind k, const float *d_x, const float *d_h,
     VTA_TRY(launch(ctx, fill, dim3(1), dim3(256), 0, meta, 1, (int)first_valid));
     VTA_TRY(launch(ctx, fill, dim3(1), dim3(256), 0, meta + 2, 1, 0));
     unsigned nb = (unsigned)((series_len - first_valid + 255) / 256);
-    VTA_TRY(launch(ctx, flag, dim3(nb), dim3(256), 0, d_x, k >= IND_ATR ? d_h : nullptr,
-                   k >= IND_ATR ? d_l : nullptr, (int)series_len, (int)first_valid, meta + 2));
+    VTA_TRY(launch(ctx, flag, dim3(nb), dim3(256), 0, d_x, hlc(k) ? d_h : nullptr,
+                   hlc(k) ? d_l : nullptr, (int)series_len, (int)first_valid, meta + 2));
     VTA_TRY(launch(ctx, inv, dim3(1), dim3(64), 0, meta + 2, meta + 3));
     // Chunked with burn-in: a series found unclean still runs correctly, since
     // every lane of an unclean stream starts exactly at first_valid.
@@ -253,7 +269,7 @@ vta_status ms_launch(vta_ctx *ctx, Ind k, const float *d_x, const float *d_h, co
 vta_status ms_dev(vta_ctx *ctx, Ind k, const float *x_tm, const float *h_tm, const float *l_tm,
                   size_t cols, size_t rows, size_t period, vta_device_array_f32 *out) {
     clear_error();
-    if (!ctx || !out || !x_tm || (k >= IND_ATR && (!h_tm || !l_tm)))
+    if (!ctx || !out || !x_tm || (hlc(k) && (!h_tm || !l_tm)))
         return fail(VTA_ERR_INVALID_INPUT, "null argument");
     if (cols == 0 || rows == 0) return fail(VTA_ERR_INVALID_INPUT, "cols or rows is zero");
     if (period == 0) return fail(VTA_ERR_INVALID_INPUT, "period must be > 0");
@@ -266,28 +282,34 @@ vta_status ms_dev(vta_ctx *ctx, Ind k, const float *x_tm, const float *h_tm, con
     // each column on the host too, e.g. ema_wrapper.rs:637-700)
     std::vector<int32_t> fvs(cols), cl(cols);
     std::vector<float> col_x(rows), col_h, col_l;
-    if (k >= IND_ATR) {
+    if (hlc(k)) {
         col_h.resize(rows);
         col_l.resize(rows);
     }
     for (size_t s = 0; s < cols; ++s) {
         for (size_t t = 0; t < rows; ++t) col_x[t] = x_tm[t * cols + s];
-        if (k >= IND_ATR)
+        if (hlc(k))
             for (size_t t = 0; t < rows; ++t) {
                 col_h[t] = h_tm[t * cols + s];
                 col_l[t] = l_tm[t * cols + s];
             }
         size_t fv = first_valid_of(k, col_x.data(), col_h.data(), col_l.data(), rows);
+        if (k == IND_WILDERS) {
+            // the Wilders many-series wrapper starts at the first finite value
+            // (wilders_wrapper.rs:329-349)
+            fv = 0;
+            while (fv < rows && !std::isfinite(col_x[fv])) ++fv;
+        }
         if (fv >= rows) return fail(VTA_ERR_INVALID_INPUT, "series %zu all NaN", s);
         if (rows - fv < need_bars(k, period))
             return fail(VTA_ERR_INVALID_INPUT,
                         "series %zu not enough valid data (needed >= %zu, valid = %zu)", s,
                         need_bars(k, period), rows - fv);
         fvs[s] = (int32_t)fv;
-        cl[s] = host_clean(col_x.data(), k >= IND_ATR ? col_h.data() : nullptr, col_l.data(),
+        cl[s] = host_clean(col_x.data(), hlc(k) ? col_h.data() : nullptr, col_l.data(),
                            rows, fv) ? 1 : 0;
     }
-    size_t nin = k >= IND_ATR ? 3 : 1;
+    size_t nin = hlc(k) ? 3 : 1;
     VTA_TRY(check_fits((nin + 1) * elems * sizeof(float)));
     DevBuf dx, dh, dl, dfv, dcl, dout;
     VTA_TRY(upload(ctx, dx, x_tm, elems * sizeof(float)));
@@ -320,6 +342,43 @@ vta_status expand_out(Ind k, const vta_range_usize *range, size_t *out, size_t c
 }  // namespace
 
 extern "C" {
+
+// ------------------------------------------------------------------ SMMA, Wilders
+#define VTA_ONE_INPUT_REC(NAME, IND)                                                              \
+    vta_status vta_##NAME##_expand_grid(const vta_range_usize *range, size_t *periods,          \
+                                        size_t cap, size_t *n_out) {                            \
+        return expand_out(IND, range, periods, cap, n_out);                                     \
+    }                                                                                           \
+    vta_status vta_##NAME##_batch_dev(vta_ctx *ctx, const float *prices, size_t len,            \
+                                      const vta_range_usize *range, vta_device_array_f32 *out) { \
+        return batch_dev(ctx, IND, prices, nullptr, nullptr, len, range, out);                  \
+    }                                                                                           \
+    vta_status vta_##NAME##_batch_device(vta_ctx *ctx, const float *d_prices,                   \
+                                         const int32_t *d_periods, int32_t max_period,          \
+                                         int32_t series_len, int32_t first_valid,               \
+                                         int32_t n_combos, float *d_out) {                      \
+        return batch_device(ctx, IND, d_prices, nullptr, nullptr, d_periods, max_period,        \
+                            series_len, first_valid, n_combos, d_out);                          \
+    }                                                                                           \
+    vta_status vta_##NAME##_many_series_one_param_time_major_dev(                               \
+        vta_ctx *ctx, const float *data_tm, size_t cols, size_t rows, size_t period,            \
+        vta_device_array_f32 *out) {                                                            \
+        return ms_dev(ctx, IND, data_tm, nullptr, nullptr, cols, rows, period, out);            \
+    }                                                                                           \
+    vta_status vta_##NAME##_many_series_one_param_device(                                       \
+        vta_ctx *ctx, const float *d_prices_tm, const int32_t *d_first_valids, size_t period,   \
+        size_t num_series, size_t series_len, const int32_t *d_clean, float *d_out_tm) {        \
+        clear_error();                                                                          \
+        if (!ctx || !d_prices_tm || !d_first_valids || !d_out_tm || period == 0 ||              \
+            num_series == 0 || series_len == 0 || period > series_len)                          \
+            return fail(VTA_ERR_INVALID_INPUT, "invalid arguments");                            \
+        return ms_launch(ctx, IND, d_prices_tm, nullptr, nullptr, num_series, series_len,       \
+                         period, d_first_valids, d_clean, d_out_tm);                            \
+    }
+
+VTA_ONE_INPUT_REC(smma, IND_SMMA)
+VTA_ONE_INPUT_REC(wilders, IND_WILDERS)
+#undef VTA_ONE_INPUT_REC
 
 // ------------------------------------------------------------------ EMA
 vta_status vta_ema_expand_grid(const vta_range_usize *range, size_t *periods, size_t cap,
@@ -455,7 +514,7 @@ vta_status vta_many_series_prepare(vta_ctx *ctx, const float *d_x_tm, const floa
                                    int32_t *d_clean) {
     clear_error();
     if (!ctx || !d_x_tm || !d_first_valids || !d_clean || cols == 0 || rows == 0 || mode < 0 ||
-        mode > 3 || (mode != 0 && (!d_high_tm || !d_low_tm)))
+        mode > 4 || (mode != 0 && mode != 4 && (!d_high_tm || !d_low_tm)))
         return fail(VTA_ERR_INVALID_INPUT, "invalid arguments");
     hipFunction_t f;
     VTA_TRY(get_function(ctx, "rec", "ms_prepare_meta", &f));

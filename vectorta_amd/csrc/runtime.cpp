@@ -164,11 +164,12 @@ bool expand_usize(vta_range_usize r, AxisKind kind, std::vector<size_t> &v) {
         if (r.start > r.end && kind == AXIS_ATR) std::reverse(v.begin(), v.end());
         return !v.empty();
     }
-    // AXIS_DOWN with start > end
+    // AXIS_DOWN / AXIS_DOWN_END with start > end
     for (size_t x = r.start;; x -= r.step) {
         v.push_back(x);
         if (x < r.step || x - r.step < r.end) break;
     }
+    if (kind == AXIS_DOWN_END && v.back() != r.end) v.push_back(r.end);
     return !v.empty();
 }
 
@@ -321,6 +322,13 @@ vta_status vta_ctx_preload(vta_ctx *ctx) {
         {"rec", "rsi_ms1p_f32"},
         {"rec", "atr_ms1p_f32"},
         {"rec", "adx_ms1p_f32"},
+        {"rec", "smma_batch_f32"},
+        {"rec", "wilders_batch_f32"},
+        {"rec", "smma_ms1p_f32"},
+        {"rec", "wilders_ms1p_f32"},
+        {"fir", "fir_batch_f32"},
+        {"fir", "fir_ms1p_f32"},
+        {"fir", "series_first_nonfinite"},
         {"rec", "ms_ema_rsi_atr_f32"},
         {"rec", "ms_prepare_meta"},
         {"rec", "series_nonfinite_flag"},

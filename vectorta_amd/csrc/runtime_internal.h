@@ -103,7 +103,9 @@ size_t first_valid_f32(const float *x, size_t n);  // n if all NaN
 //   AXIS_DOWN    ascending if start < end, else start, start-step, ... >= end
 //                (alma_wrapper.rs:1551-1575, adx_wrapper.rs:158-178)
 //   AXIS_ATR     ascending if start < end, else (end..=start step) reversed (atr_wrapper.rs:286-297)
-enum AxisKind { AXIS_SORTED, AXIS_DOWN, AXIS_ATR };
+//   AXIS_DOWN_END like AXIS_DOWN, then end appended when the countdown misses it
+//                (wma_wrapper.rs:324-350)
+enum AxisKind { AXIS_SORTED, AXIS_DOWN, AXIS_ATR, AXIS_DOWN_END };
 bool expand_usize(vta_range_usize r, AxisKind kind, std::vector<size_t> &v);
 // f64 axis: values accumulate x += step while x <= end + 1e-12 (or descend
 // while x + 1e-12 >= end), alma_wrapper.rs:1576-1612.

@@ -1,0 +1,112 @@
+"""SMA / WMA / SWMA operator surfaces: mirrors of CudaSma
+(src/cuda/moving_averages/sma_wrapper.rs), CudaWma (wma_wrapper.rs) and CudaSwma
+(swma_wrapper.rs) over the finite-window kernels (kernels/fir.hip)."""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Tuple
+
+import numpy as np
+
+from ._lib import DeviceArrayF32C, check, lib
+from .device import DeviceArrayF32, DeviceBuffer, HipContext, f32_ptr
+from .rec import _expand, _rng
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _vp(buf):
+    return ctypes.c_void_p(buf.ptr) if buf is not None else None
+
+
+class _Fir:
+    _name = ""
+
+    def __init__(self, device_id: int = 0):
+        self.ctx = HipContext(device_id)
+
+    def synchronize(self) -> None:
+        self.ctx.synchronize()
+
+    @classmethod
+    def expand_grid(cls, period: Tuple[int, int, int]) -> List[int]:
+        return _expand(getattr(lib(), f"vta_{cls._name}_expand_grid"), period)
+
+    def _batch_dev(self, prices, period) -> DeviceArrayF32:
+        x = _f32(prices)
+        rc = _rng(period)
+        out = DeviceArrayF32C()
+        check(getattr(lib(), f"vta_{self._name}_batch_dev")(self.ctx.handle, f32_ptr(x), x.size,
+                                                            ctypes.byref(rc), ctypes.byref(out)))
+        return DeviceArrayF32(self.ctx, out)
+
+    def _batch_device(self, d_prices: DeviceBuffer, d_periods: DeviceBuffer, series_len: int,
+                      n_combos: int, first_valid: int, max_period: int,
+                      d_out: DeviceBuffer) -> None:
+        check(getattr(lib(), f"vta_{self._name}_batch_device")(
+            self.ctx.handle, _vp(d_prices), _vp(d_periods), int(series_len), int(n_combos),
+            int(first_valid), int(max_period), _vp(d_out)))
+
+    def _ms_dev(self, data_tm, cols, rows, period) -> DeviceArrayF32:
+        x = _f32(data_tm).reshape(-1)
+        out = DeviceArrayF32C()
+        fn = getattr(lib(), f"vta_{self._name}_many_series_one_param_time_major_dev")
+        check(fn(self.ctx.handle, f32_ptr(x), int(cols), int(rows), int(period),
+                 ctypes.byref(out)))
+        return DeviceArrayF32(self.ctx, out)
+
+    def _ms_device(self, d_prices_tm, d_first_valids, cols, rows, period, d_out_tm,
+                   d_first_nonfinite=None) -> None:
+        fn = getattr(lib(), f"vta_{self._name}_many_series_one_param_device")
+        check(fn(self.ctx.handle, _vp(d_prices_tm), _vp(d_first_valids), _vp(d_first_nonfinite),
+                 int(cols), int(rows), int(period), _vp(d_out_tm)))
+
+
+class CudaSma(_Fir):
+    _name = "sma"
+
+    def sma_batch_dev(self, prices_f32, period: Tuple[int, int, int]) -> DeviceArrayF32:
+        return self._batch_dev(prices_f32, period)
+
+    def sma_batch_device(self, *args) -> None:
+        self._batch_device(*args)
+
+    def sma_multi_series_one_param_time_major_dev(self, data_tm, cols, rows, period):
+        return self._ms_dev(data_tm, cols, rows, period)
+
+    def sma_multi_series_one_param_device(self, *args, **kw) -> None:
+        self._ms_device(*args, **kw)
+
+
+class CudaWma(_Fir):
+    _name = "wma"
+
+    def wma_batch_dev(self, prices_f32, period: Tuple[int, int, int]) -> DeviceArrayF32:
+        return self._batch_dev(prices_f32, period)
+
+    def wma_batch_device(self, *args) -> None:
+        self._batch_device(*args)
+
+    def wma_multi_series_one_param_time_major_dev(self, data_tm, cols, rows, period):
+        return self._ms_dev(data_tm, cols, rows, period)
+
+    def wma_multi_series_one_param_time_major_device(self, *args, **kw) -> None:
+        self._ms_device(*args, **kw)
+
+
+class CudaSwma(_Fir):
+    _name = "swma"
+
+    def swma_batch_dev(self, prices_f32, period: Tuple[int, int, int]) -> DeviceArrayF32:
+        return self._batch_dev(prices_f32, period)
+
+    def swma_batch_device(self, *args) -> None:
+        self._batch_device(*args)
+
+    def swma_multi_series_one_param_time_major_dev(self, data_tm, cols, rows, period):
+        return self._ms_dev(data_tm, cols, rows, period)
+
+    def swma_multi_series_one_param_device(self, *args, **kw) -> None:
+        self._ms_device(*args, **kw)

@@ -41,8 +41,9 @@ class _OneParam:
     def synchronize(self) -> None:
         self.ctx.synchronize()
 
-    def expand_grid(self, period: Tuple[int, int, int]) -> List[int]:
-        return _expand(getattr(lib(), f"vta_{self._name}_expand_grid"), period)
+    @classmethod
+    def expand_grid(cls, period: Tuple[int, int, int]) -> List[int]:
+        return _expand(getattr(lib(), f"vta_{cls._name}_expand_grid"), period)
 
     def _batch_dev(self, arrays, period) -> DeviceArrayF32:
         arrs = [_f32(a) for a in arrays]
@@ -119,6 +120,65 @@ class CudaAdx(_OneParam):
     def adx_many_series_one_param_time_major_dev(self, high_tm, low_tm, close_tm, cols, rows,
                                                  period):
         return self._ms_dev([high_tm, low_tm, close_tm], cols, rows, period)
+
+
+class _OneInputMa(_OneParam):
+    """SMMA / Wilders: one-series batch, enqueue-only batch and many-series forms."""
+
+    def _batch(self, prices_f32, period) -> DeviceArrayF32:
+        return self._batch_dev([prices_f32], period)
+
+    def _batch_device(self, d_prices: DeviceBuffer, d_periods: DeviceBuffer, max_period: int,
+                      series_len: int, first_valid: int, n_combos: int,
+                      d_out: DeviceBuffer) -> None:
+        fn = getattr(lib(), f"vta_{self._name}_batch_device")
+        check(fn(self.ctx.handle, ctypes.c_void_p(d_prices.ptr), ctypes.c_void_p(d_periods.ptr),
+                 int(max_period), int(series_len), int(first_valid), int(n_combos),
+                 ctypes.c_void_p(d_out.ptr)))
+
+    def _ms_device(self, d_prices_tm, d_first_valids, period, num_series, series_len, d_out_tm,
+                   d_clean=None) -> None:
+        fn = getattr(lib(), f"vta_{self._name}_many_series_one_param_device")
+        check(fn(self.ctx.handle, ctypes.c_void_p(d_prices_tm.ptr),
+                 ctypes.c_void_p(d_first_valids.ptr), int(period), int(num_series),
+                 int(series_len), ctypes.c_void_p(d_clean.ptr) if d_clean is not None else None,
+                 ctypes.c_void_p(d_out_tm.ptr)))
+
+
+class CudaSmma(_OneInputMa):
+    """src/cuda/moving_averages/smma_wrapper.rs (:398 batch_dev, :377 batch_device,
+    :623 many-series _dev, :599 many-series _device)."""
+    _name = "smma"
+
+    def smma_batch_dev(self, prices_f32, period: Tuple[int, int, int]) -> DeviceArrayF32:
+        return self._batch(prices_f32, period)
+
+    def smma_batch_device(self, *args) -> None:
+        self._batch_device(*args)
+
+    def smma_multi_series_one_param_time_major_dev(self, data_tm, cols, rows, period):
+        return self._ms_dev([data_tm], cols, rows, period)
+
+    def smma_multi_series_one_param_device(self, *args, **kw) -> None:
+        self._ms_device(*args, **kw)
+
+
+class CudaWilders(_OneInputMa):
+    """src/cuda/moving_averages/wilders_wrapper.rs (:134 batch_dev, :252 batch_device,
+    :600 many-series _dev, :662 many-series _device)."""
+    _name = "wilders"
+
+    def wilders_batch_dev(self, prices_f32, period: Tuple[int, int, int]) -> DeviceArrayF32:
+        return self._batch(prices_f32, period)
+
+    def wilders_batch_device(self, *args) -> None:
+        self._batch_device(*args)
+
+    def wilders_many_series_one_param_time_major_dev(self, data_tm, cols, rows, period):
+        return self._ms_dev([data_tm], cols, rows, period)
+
+    def wilders_many_series_one_param_device(self, *args, **kw) -> None:
+        self._ms_device(*args, **kw)
 
 
 def ema_rsi_atr_many_series_dev(ctx: HipContext, close_tm, high_tm, low_tm, cols: int,
