@@ -231,6 +231,99 @@ def bench_rolling(steps, warmup, ev):
     return res
 
 
+def _time_launches(fn, stream, steps, warmup, ev):
+    for _ in range(warmup):
+        fn()
+    ks = []
+    for _ in range(steps):
+        a, b = ev.create(), ev.create()
+        ev.record(a, stream)
+        fn()
+        ev.record(b, stream)
+        ks.append(ev.elapsed_ms(a, b))
+        ev.destroy(a)
+        ev.destroy(b)
+    return np.array(ks)
+
+
+def bench_moving_averages(steps, warmup, ev):
+    """SMA / WMA / SWMA / SMMA / Wilders.  One series x 1,250 periods (2..=1251)
+    over the 200,000-bar sweep series (the ALMA sweep's output size, 1.0 GB),
+    and time-major 4,096 x 50,000 many-series at one period; inputs resident,
+    one enqueue-only launch per step.  Algorithmic bytes = outputs written +
+    inputs read once."""
+    x = datagen.alma_sweep_series()
+    T = x.size
+    res = {"workload": "ma_batch_200k_x_1250_and_many_series_4096x50000"}
+    ops = {"sma": vta.CudaSma(0), "wma": vta.CudaWma(0), "swma": vta.CudaSwma(0),
+           "smma": vta.CudaSmma(0), "wilders": vta.CudaWilders(0)}
+    ctx = ops["sma"].ctx
+    periods = np.arange(2, 1252, dtype=np.int32)
+    C = periods.size
+    d_x = ctx.to_device(x)
+    d_p = ctx.to_device(periods)
+    d_o = ctx.alloc(C * T * 4)
+    kern = {"sma": "fir_batch_f32", "wma": "fir_batch_f32", "swma": "fir_batch_f32",
+            "smma": "smma_batch_f32", "wilders": "wilders_batch_f32"}
+    for name, op in ops.items():
+        if name in ("smma", "wilders"):
+            fn = (lambda op=op, name=name: getattr(op, f"{name}_batch_device")(
+                d_x, d_p, int(periods.max()), T, 0, C, d_o))
+        else:
+            fn = (lambda op=op, name=name: getattr(op, f"{name}_batch_device")(
+                d_x, d_p, T, C, 0, int(periods.max()), d_o))
+        # every op launches on the stream of its own context
+        ks = _time_launches(fn, op.ctx.stream, steps, warmup, ev)
+        op.synchronize()
+        t = float(ks.mean()) * 1e-3
+        alg = 4 * (C * T + T)
+        res[f"{name}_batch"] = {"kernel": kern[name], "combos": C, "series_len": T,
+                                "kernel_ms_mean": float(ks.mean()),
+                                "kernel_ms_p50": float(np.percentile(ks, 50)),
+                                "outputs_per_s": C * T / t, "algorithmic_bytes": alg,
+                                "achieved_GBs": alg / t / 1e9,
+                                "frac_of_hbm_peak": alg / t / 1e9 / HBM_PEAK_GBS}
+    d_o.free()
+    d_x.free()
+    d_p.free()
+    rows, cols = 50_000, 4096
+    rng = np.random.Generator(np.random.PCG64(9))
+    tm = (100.0 * np.exp(np.cumsum(rng.normal(0, 0.01, (rows, cols)), axis=0))).astype(np.float32)
+    d_tm = ctx.to_device(tm)
+    fvs = np.zeros(cols, dtype=np.int32)
+    d_fv = ctx.to_device(fvs)
+    d_clean = ctx.to_device(np.ones(cols, dtype=np.int32))   # every bar finite
+    d_o = ctx.alloc(rows * cols * 4)
+    del tm
+    ms_meth = {"sma": "sma_multi_series_one_param_device",
+               "wma": "wma_multi_series_one_param_time_major_device",
+               "swma": "swma_multi_series_one_param_device",
+               "smma": "smma_multi_series_one_param_device",
+               "wilders": "wilders_many_series_one_param_device"}
+    for name, p in (("sma", 14), ("wma", 30), ("swma", 14), ("smma", 14), ("wilders", 14)):
+        op = ops[name]
+        meth = getattr(op, ms_meth[name])
+        if name in ("smma", "wilders"):
+            fn = (lambda meth=meth, p=p: meth(d_tm, d_fv, p, cols, rows, d_o, d_clean=d_clean))
+        else:
+            fn = (lambda meth=meth, p=p: meth(d_tm, d_fv, cols, rows, p, d_o))
+        ks = _time_launches(fn, op.ctx.stream, steps, warmup, ev)
+        op.synchronize()
+        t = float(ks.mean()) * 1e-3
+        alg = 4 * 2 * rows * cols
+        res[f"{name}_many_series"] = {"period": p, "kernel_ms_mean": float(ks.mean()),
+                                      "kernel_ms_p50": float(np.percentile(ks, 50)),
+                                      "outputs_per_s": rows * cols / t,
+                                      "algorithmic_bytes": alg,
+                                      "achieved_GBs": alg / t / 1e9,
+                                      "frac_of_hbm_peak": alg / t / 1e9 / HBM_PEAK_GBS}
+    d_o.free()
+    d_tm.free()
+    d_fv.free()
+    d_clean.free()
+    return res
+
+
 def bench_double_alma(steps, warmup, ev):
     """Double-ALMA crossover backtest reduction: 220 fast x 265 slow ALMA periods
     (58,300 pairs) over the 200,000-bar sweep series, prices resident.  A step is
@@ -383,7 +476,7 @@ def main():
                     help="skip the secondary workloads (many-series OHLC, rolling windows, "
                          "single ALMA)")
     ap.add_argument("--mode", default="sweep",
-                    choices=["sweep", "calib", "ohlc", "rolling", "cross", "serve"],
+                    choices=["sweep", "calib", "ohlc", "rolling", "cross", "serve", "ma"],
                     help="calib: 256 MiB device-to-device copies, the known-size reference "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
@@ -404,6 +497,10 @@ def main():
         return
     if args.mode == "cross":
         r = bench_double_alma(args.steps, args.warmup, HipEvents())
+        print(json.dumps(r))
+        return
+    if args.mode == "ma":
+        r = bench_moving_averages(args.steps, args.warmup, HipEvents())
         print(json.dumps(r))
         return
     if args.mode == "rolling":
@@ -500,6 +597,7 @@ def main():
         extra = {}
         for name, fn in (("many_series_ohlc", lambda: bench_many_series_ohlc(10, 2, ev)),
                          ("rolling_windows", lambda: bench_rolling(5, 1, ev)),
+                         ("moving_averages", lambda: bench_moving_averages(5, 1, ev)),
                          ("alma_host_boundary", lambda: bench_alma_host_boundary()),
                          ("double_alma_crossover", lambda: bench_double_alma(5, 1, ev)),
                          ("serving", lambda: bench_serving()),

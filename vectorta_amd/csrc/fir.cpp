@@ -3,6 +3,7 @@
 // CudaSwma (swma_wrapper.rs).  Kernels: kernels/fir.hip.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -27,12 +28,31 @@ struct fir_args_host {
     int span_cap;
     int chunk;
     int period;
+    int group;
+    int vec4;
     int pad_;
 };
-static_assert(sizeof(fir_args_host) == 80, "fir_args layout");
+static_assert(sizeof(fir_args_host) == 88, "fir_args layout");
 
 enum Fir { FIR_SMA = 0, FIR_WMA = 1, FIR_SWMA = 2 };
-constexpr unsigned kGroup = 16;  // FIR_GC in fir.hip
+
+// Combos per workgroup and the store form of the batch kernel (VTA_FIR_GROUP,
+// VTA_FIR_VEC4 override the defaults).  Measured on 1,250 periods x 200k bars:
+// 64 combos per workgroup amortise the tile's prefix scan best (SMA 0.24 ms
+// vs 0.30 ms at 16), and one 4-byte store per lane (lanes on consecutive bars)
+// beats lane groups of four bars with 16-byte stores by 5-15%, whose strided
+// LDS reads conflict.
+unsigned fir_group() {
+    if (const char *e = getenv("VTA_FIR_GROUP")) {
+        long v = atol(e);
+        if (v >= 1 && v <= 256) return (unsigned)v;
+    }
+    return 64;
+}
+int fir_vec4() {
+    if (const char *e = getenv("VTA_FIR_VEC4")) return atoi(e) != 0;
+    return 0;
+}
 
 // Grids: SMA sma.rs:705-755 (expand_grid_sma), WMA wma_wrapper.rs:324-350
 // (countdown plus end), SWMA swma_wrapper.rs:275-301.
@@ -88,18 +108,22 @@ vta_status batch_launch(vta_ctx *ctx, Fir k, const float *d_x, const int32_t *d_
     a.kind = (int)k;
     a.tile = (int)tile;
     a.span_cap = (int)(tile + max_p + 1);
+    const unsigned group = fir_group();
+    a.group = (int)group;
+    a.vec4 = fir_vec4();
     unsigned gx = (unsigned)((T + tile - 1) / tile);
-    unsigned gy = (unsigned)((C + kGroup - 1) / kGroup);
+    unsigned gy = (unsigned)((C + group - 1) / group);
     if (gy > (unsigned)ctx->props.maxGridSize[1])
         return fail(VTA_ERR_LAUNCH_CONFIG_TOO_LARGE, "too many combos for one launch (%zu)", C);
     return launch(ctx, f, dim3(gx, gy, 1), dim3(256), (unsigned)smem, a);
 }
 
-// Many-series chunk: at least 4x the replayed look-back (p bars), at least
-// 1024 bars, small enough for ~2048 waves.
+// Many-series chunk: at least 4x the replayed look-back (p bars) and 256
+// bars, small enough for ~8192 waves (32 per CU: each wave's bars are a
+// dependent chain, so latency is hidden by many waves in flight).
 int pick_chunk(size_t p, size_t T, size_t groups) {
-    size_t want = std::max<size_t>(1, 2048 / std::max<size_t>(groups, 1));
-    size_t c = std::max<size_t>({4 * p, 1024, (T + want - 1) / want});
+    size_t want = std::max<size_t>(1, 8192 / std::max<size_t>(groups, 1));
+    size_t c = std::max<size_t>({4 * p, 256, (T + want - 1) / want});
     c = (c + 63) & ~(size_t)63;
     return (int)std::min(c, T);
 }

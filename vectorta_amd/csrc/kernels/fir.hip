@@ -1,7 +1,7 @@
 // Finite-window moving averages for gfx950: SMA, WMA, SWMA.
 //
 // One series x many periods (fir_batch_f32).  A workgroup owns a tile of
-// consecutive bars [t0, t0 + tile) and a group of FIR_GC combos.  It stages
+// consecutive bars [t0, t0 + tile) and a group of consecutive combos.  It stages
 // the tile plus the group's look-back once and turns it into two local f64
 // prefix sums in LDS (block scan: wave scan by shuffles, wave totals through
 // LDS), with the local index i of bar s0 + i:
@@ -48,10 +48,11 @@ struct fir_args {
     int span_cap;             // batch: LDS doubles per prefix array, minus one
     int chunk;                // many-series: bars per chunk
     int period;               // many-series
+    int group;                // batch: combos per workgroup
+    int vec4;                 // batch: 1 = 16-byte stores of 4 consecutive bars per lane
     int pad_;
 };
 
-#define FIR_GC 16
 #define FIR_NAN_F __builtin_bit_cast(float, 0x7fc00000u)
 
 __device__ __forceinline__ void swma_ab(int p, int *a, int *b) {
@@ -64,6 +65,36 @@ __device__ __forceinline__ void swma_ab(int p, int *a, int *b) {
     }
 }
 
+// Writes row[t] = f(t) for t in [t0, tend).  vec4 = 0: lane l writes bars
+// t0 + l + 256k (one 4-byte store per lane, consecutive across the wave).
+// vec4 = 1: lane groups of 4 consecutive bars aligned to 16 bytes of the row
+// (rows start at c*T floats), one float4 store per group inside the range,
+// single stores at the range's ragged ends.
+template <class F>
+__device__ __forceinline__ void fir_emit_row(float *row, size_t rb, int t0, int tend, int tid,
+                                             int vec4, F f) {
+    if (!vec4) {
+        for (int t = t0 + tid; t < tend; t += 256) row[t] = f(t);
+        return;
+    }
+    const int g0 = t0 - (int)((rb + (size_t)t0) & 3);
+    for (int g = g0 + 4 * tid; g < tend; g += 1024) {
+        float v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int t = g + k;
+            v[k] = (t >= t0 && t < tend) ? f(t) : 0.0f;
+        }
+        if (g >= t0 && g + 3 < tend) {
+            *reinterpret_cast<float4 *>(row + g) = make_float4(v[0], v[1], v[2], v[3]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (g + k >= t0 && g + k < tend) row[g + k] = v[k];
+        }
+    }
+}
+
 extern "C" __global__ __launch_bounds__(256) void fir_batch_f32(fir_args a) {
     extern __shared__ double lds[];
     __shared__ double wtot[2][4];
@@ -72,8 +103,8 @@ extern "C" __global__ __launch_bounds__(256) void fir_batch_f32(fir_args a) {
     const int w = tid >> 6;
     const int T = a.T;
     const int t0 = blockIdx.x * a.tile;
-    const int c0 = blockIdx.y * FIR_GC;
-    const int nc = min(FIR_GC, a.nlanes - c0);
+    const int c0 = blockIdx.y * a.group;
+    const int nc = min(a.group, a.nlanes - c0);
     const int fv = a.meta[0];
     const int z = a.meta[1];
     int pg = 1;
@@ -140,73 +171,58 @@ extern "C" __global__ __launch_bounds__(256) void fir_batch_f32(fir_args a) {
         __syncthreads();
     }
 
-    // ---- outputs, one coalesced row segment per combo
+    // SWMA reads only C[k] = k A[k] - B[k] (the prefix of the prefix) and A:
+    // fold C into the B array once per tile
+    if (a.kind == 2) {
+        for (int k = tid; k <= span; k += 256) B[k] = (double)k * A[k] - B[k];
+        __syncthreads();
+    }
+    const double *C = B;
+
+    // ---- outputs: each lane writes 4 consecutive bars of a row, one 16-byte
+    // store where the group lies inside the tile
     for (int j = 0; j < nc; ++j) {
         const int c = c0 + j;
         const int p = a.periods[c];
-        float *row = a.out + (size_t)c * T;
+        const size_t rb = (size_t)c * T;
+        float *row = a.out + rb;
+        const int warm = fv + p - 1;
         if (a.kind == 0) {
-            const int warm = fv + p - 1;
             const double inv = 1.0 / (double)p;
-            for (int t = t0 + tid; t < tend; t += 256) {
-                float v = FIR_NAN_F;
-                if (t >= warm && t - p < z) {
-                    const int Lt = t - s0 + 1;
-                    v = (float)((A[Lt] - A[Lt - p]) * inv);
-                }
-                row[t] = v;
-            }
+            fir_emit_row(row, rb, t0, tend, tid, a.vec4, [&](int t) -> float {
+                if (t < warm || t - p >= z) return FIR_NAN_F;
+                const int Lt = t - s0 + 1;
+                return (float)((A[Lt] - A[Lt - p]) * inv);
+            });
         } else if (a.kind == 1) {
-            const int warm = fv + p - 1;
             const double inv = 1.0 / ((double)p * ((double)p + 1.0) * 0.5);
-            for (int t = t0 + tid; t < tend; t += 256) {
-                float v = FIR_NAN_F;
-                if (t >= warm && t < z) {
-                    const int Lt = t - s0 + 1;
-                    const int Lq = Lt - p;
-                    const double sa = A[Lt] - A[Lq];
-                    const double sb = B[Lt] - B[Lq];
-                    v = (float)((sb - (double)(Lq - 1) * sa) * inv);
-                }
-                row[t] = v;
-            }
+            fir_emit_row(row, rb, t0, tend, tid, a.vec4, [&](int t) -> float {
+                if (t < warm || t >= z) return FIR_NAN_F;
+                const int Lt = t - s0 + 1;
+                const int Lq = Lt - p;
+                const double sa = A[Lt] - A[Lq];
+                const double sb = B[Lt] - B[Lq];
+                return (float)((sb - (double)(Lq - 1) * sa) * inv);
+            });
+        } else if (p <= 2) {
+            // window-only forms (swma.rs:429-445): copy, or pair average
+            fir_emit_row(row, rb, t0, tend, tid, a.vec4, [&](int t) -> float {
+                if (t < warm) return FIR_NAN_F;
+                if (p == 1) return a.x[t];
+                return (float)(((double)a.x[t - 1] + (double)a.x[t]) * 0.5);
+            });
         } else {
-            if (p <= 2) {
-                // window-only forms (swma.rs:429-445): copy, or pair average
-                const int warm = fv + p - 1;
-                for (int t = t0 + tid; t < tend; t += 256) {
-                    float v = FIR_NAN_F;
-                    if (t >= warm) {
-                        if (p == 1) v = a.x[t];
-                        else v = (float)(((double)a.x[t - 1] + (double)a.x[t]) * 0.5);
-                    }
-                    row[t] = v;
-                }
-                continue;
-            }
-            int sa_, sb_;
-            swma_ab(p, &sa_, &sb_);
-            const int wa = sa_, wb = sb_;
-            const int warm = fv + p - 1;
+            int wa, wb;
+            swma_ab(p, &wa, &wb);
             const double inv = 1.0 / ((double)wa * (double)wb);
-            for (int t = t0 + tid; t < tend; t += 256) {
-                float v = FIR_NAN_F;
-                if (t >= warm && t - wa < z) {
-                    const int Lt = t - s0 + 1;
-                    if (t >= z) {
-                        // running sums hold the sum of the non-finite bars in [z, t]
-                        v = (float)(A[Lt] * inv);
-                    } else {
-                        const double c_t = (double)Lt * A[Lt] - B[Lt];
-                        const int k1 = Lt - wb, k2 = Lt - wa, k3 = Lt - wa - wb;
-                        const double c_1 = (double)k1 * A[k1] - B[k1];
-                        const double c_2 = (double)k2 * A[k2] - B[k2];
-                        const double c_3 = (double)k3 * A[k3] - B[k3];
-                        v = (float)(((c_t - c_1) - (c_2 - c_3)) * inv);
-                    }
-                }
-                row[t] = v;
-            }
+            fir_emit_row(row, rb, t0, tend, tid, a.vec4, [&](int t) -> float {
+                if (t < warm || t - wa >= z) return FIR_NAN_F;
+                const int Lt = t - s0 + 1;
+                // inside [z, z+a) the running sums hold the sum of the
+                // non-finite bars in [z, t]
+                if (t >= z) return (float)(A[Lt] * inv);
+                return (float)(((C[Lt] - C[Lt - wb]) - (C[Lt - wa] - C[Lt - wa - wb])) * inv);
+            });
         }
     }
 }
@@ -214,7 +230,10 @@ extern "C" __global__ __launch_bounds__(256) void fir_batch_f32(fir_args a) {
 // ---------------------------------------------------------------- many series
 //
 // grid.x: groups of 64 series, grid.y: chunks of a.chunk bars.  Lane = series;
-// each load/store is one coalesced row segment of the [T][N] matrix.
+// each load/store is one coalesced row segment of the [T][N] matrix.  Bars go
+// in blocks of 8: the block's loads (the new bars and the lagged bars that
+// leave the running sums, indices clamped into the matrix) are issued before
+// its sequential updates.
 extern "C" __global__ __launch_bounds__(64) void fir_ms1p_f32(fir_args a) {
     const int lane = threadIdx.x & 63;
     const int s = blockIdx.x * 64 + lane;
@@ -236,53 +255,76 @@ extern "C" __global__ __launch_bounds__(64) void fir_ms1p_f32(fir_args a) {
     for (int o = 32; o > 0; o >>= 1) tb = min(tb, __shfl_xor(tb, o));
     tb = __builtin_amdgcn_readfirstlane(tb);
     tb = min(tb, c0);
+    auto ld = [&](int t) -> double { return (double)xc[(size_t)min(max(t, 0), T - 1) * N]; };
+    auto put = [&](int t, float v) {
+        if (active && t >= c0) oc[(size_t)t * N] = t >= warm ? v : FIR_NAN_F;
+    };
 
     if (a.kind == 0 || (a.kind == 2 && p <= 2)) {
         const bool copy = p == 1;
         const bool pair = a.kind == 2 && p == 2;
+        const int lag = pair ? 1 : p;
         const double inv = 1.0 / (double)p;
         double sum = 0.0;
-        for (int t = tb; t < c1; ++t) {
-            const size_t off = (size_t)t * N;
-            const double xv = (double)xc[off];
-            float v = FIR_NAN_F;
-            if (copy) {
-                v = (float)xv;
-            } else if (pair) {
-                if (t >= 1) v = (float)(((double)xc[off - N] + xv) * 0.5);
-            } else if (t >= r0) {
-                // sma.rs:318-328: the first p bars summed, then sum += x[t] - x[t-p]
-                if (t < r0 + p) sum += xv;
-                else sum += xv - (double)xc[off - (size_t)p * N];
-                v = (float)(sum * inv);
-                if (t - p >= z) v = FIR_NAN_F;
+        for (int t = tb; t < c1; t += 8) {
+            double xn[8], xo[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                xn[k] = ld(t + k);
+                xo[k] = ld(t + k - lag);
             }
-            if (active && t >= c0) oc[off] = t >= warm ? v : FIR_NAN_F;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int u = t + k;
+                if (u >= c1) break;
+                float v = FIR_NAN_F;
+                if (copy) {
+                    v = (float)xn[k];
+                } else if (pair) {
+                    if (u >= 1) v = (float)((xo[k] + xn[k]) * 0.5);
+                } else if (u >= r0) {
+                    // sma.rs:318-328: the first p bars summed, then sum += x[t] - x[t-p]
+                    if (u < r0 + p) sum += xn[k];
+                    else sum += xn[k] - xo[k];
+                    v = (float)(sum * inv);
+                    if (u - p >= z) v = FIR_NAN_F;
+                }
+                put(u, v);
+            }
         }
     } else if (a.kind == 1) {
         // wma.rs:290-320: weights 1..p, running weighted sum and plain sum
         const double pf = (double)p;
         const double inv = 1.0 / (pf * (pf + 1.0) * 0.5);
         double s1 = 0.0, s2 = 0.0;
-        for (int t = tb; t < c1; ++t) {
-            const size_t off = (size_t)t * N;
-            const double xv = (double)xc[off];
-            float v = FIR_NAN_F;
-            if (t >= r0) {
-                const int k = t - r0;
-                if (k < p - 1) {
-                    s2 += xv * ((double)k + 1.0);
-                    s1 += xv;
-                } else {
-                    s2 += xv * pf;
-                    s1 += xv;
-                    v = (float)(s2 * inv);
-                    s2 -= s1;
-                    s1 -= (double)xc[off - (size_t)(p - 1) * N];
-                }
-                if (t > z) v = FIR_NAN_F;
+        for (int t = tb; t < c1; t += 8) {
+            double xn[8], xo[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                xn[k] = ld(t + k);
+                xo[k] = ld(t + k - p + 1);
             }
-            if (active && t >= c0) oc[off] = t >= warm ? v : FIR_NAN_F;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int u = t + k;
+                if (u >= c1) break;
+                float v = FIR_NAN_F;
+                if (u >= r0) {
+                    const int q = u - r0;
+                    if (q < p - 1) {
+                        s2 += xn[k] * ((double)q + 1.0);
+                        s1 += xn[k];
+                    } else {
+                        s2 += xn[k] * pf;
+                        s1 += xn[k];
+                        v = (float)(s2 * inv);
+                        s2 -= s1;
+                        s1 -= xo[k];
+                    }
+                    if (u > z) v = FIR_NAN_F;
+                }
+                put(u, v);
+            }
         }
     } else {
         // swma.rs:406-476 with the ring of the last b box sums replaced by a
@@ -290,26 +332,38 @@ extern "C" __global__ __launch_bounds__(64) void fir_ms1p_f32(fir_args a) {
         int wa, wb;
         swma_ab(p, &wa, &wb);
         const double inv = 1.0 / ((double)wa * (double)wb);
+        const int sa0 = r0 + wa - 1;                  // first full box
         double s1 = 0.0, s1b = 0.0, s2 = 0.0;
-        for (int t = tb; t < c1; ++t) {
-            const size_t off = (size_t)t * N;
-            float v = FIR_NAN_F;
-            if (t >= r0) {
-                const int sa0 = r0 + wa - 1;         // first full box
-                s1 += (double)xc[off];
-                double old = 0.0;
-                const int tl = t - wb;                // bar whose box leaves the sum
-                if (tl >= r0) s1b += (double)xc[off - (size_t)wb * N];
-                if (tl >= sa0) old = s1b;
-                if (t >= sa0) {
-                    s2 = s2 + (s1 - old);
-                    v = (float)(s2 * inv);
-                    s1 -= (double)xc[off - (size_t)(wa - 1) * N];
-                }
-                if (tl >= sa0) s1b -= (double)xc[off - (size_t)(wb + wa - 1) * N];
-                if (t - wa >= z) v = FIR_NAN_F;
+        for (int t = tb; t < c1; t += 8) {
+            double x0[8], xa[8], xb[8], xab[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                x0[k] = ld(t + k);
+                xa[k] = ld(t + k - wa + 1);
+                xb[k] = ld(t + k - wb);
+                xab[k] = ld(t + k - wb - wa + 1);
             }
-            if (active && t >= c0) oc[off] = t >= warm ? v : FIR_NAN_F;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int u = t + k;
+                if (u >= c1) break;
+                float v = FIR_NAN_F;
+                if (u >= r0) {
+                    s1 += x0[k];
+                    double old = 0.0;
+                    const int tl = u - wb;            // bar whose box leaves the sum
+                    if (tl >= r0) s1b += xb[k];
+                    if (tl >= sa0) old = s1b;
+                    if (u >= sa0) {
+                        s2 = s2 + (s1 - old);
+                        v = (float)(s2 * inv);
+                        s1 -= xa[k];
+                    }
+                    if (tl >= sa0) s1b -= xab[k];
+                    if (u - wa >= z) v = FIR_NAN_F;
+                }
+                put(u, v);
+            }
         }
     }
 }

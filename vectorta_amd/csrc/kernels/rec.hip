@@ -24,6 +24,18 @@
 // every bar after first_valid is finite (non-finite bars stop EMA decay and
 // poison ATR forever), so each stream carries a "clean" flag; an unclean
 // stream always starts exactly at first_valid.
+//
+// One-series batches of the single-stage recurrences (EMA, RSI, ATR, SMMA,
+// Wilders) use an exact chunked scan instead when the series is clean: after
+// the warm-up every step is affine in the state, s_t = beta s_{t-1} + f(x_t),
+// so a chunk of L bars maps its start state s to beta^L s + B, where B is the
+// chunk run from a zero state.  Pass 1 (phase 1) computes B for every (chunk,
+// combo) -- chunks that contain the warm-up run exactly from first_valid
+// instead --, a short per-combo pass (rec_scan_combine) folds the chunks in
+// order into the exact state at every chunk end, and pass 3 (phase 3) runs
+// every chunk again from its exact start state and writes the outputs.  The
+// chunk length no longer has to cover a burn-in of ~25 / -ln(beta) bars, so
+// long periods keep thousands of waves in flight.
 #include "vta_common.h"
 
 struct rec_args {
@@ -37,10 +49,13 @@ struct rec_args {
     float *out0;
     float *out1;
     float *out2;
+    double *carry;           // batch scan: [chunk][combo][states] chunk-end states
     int T;                   // bars
     int nlanes;              // combos (batch) or series (many-series)
     int chunk;               // bars per chunk
     int p1, p2, p3;          // fused kernel periods (EMA, RSI, ATR)
+    int phase;               // batch: 0 burn-in/exact, 1 scan pass, 3 output pass of the scan
+    int pad_;
 };
 
 #define REC_NAN __builtin_bit_cast(double, 0x7ff8000000000000ull)
@@ -84,6 +99,10 @@ struct EmaEng {
         return prev;
     }
     __device__ double out_before_start(int t) const { return REC_NAN; }
+    static constexpr int kScan = 1;                  // states carried by the chunked scan
+    __device__ double decay() const { return beta; }
+    __device__ void begin_carry(double x, const double *st) { prev = st[0]; vc = p; }
+    __device__ void get_state(double *st) const { st[0] = prev; }
 };
 
 // num / den for den > 0: an f32 reciprocal estimate refined by one f64 Newton
@@ -140,6 +159,10 @@ struct RsiEng {
         double den = ag + al;
         return den == 0.0 ? 50.0 : rec_ratio(100.0 * ag, den);
     }
+    static constexpr int kScan = 2;
+    __device__ double decay() const { return beta; }
+    __device__ void begin_carry(double x, const double *st) { px = x; ag = st[0]; al = st[1]; }
+    __device__ void get_state(double *st) const { st[0] = ag; st[1] = al; }
 };
 
 __device__ __forceinline__ double true_range(double hi, double lo, double pc) {
@@ -177,6 +200,13 @@ struct AtrEng {
         rma = fma(-alpha, rma, rma) + alpha * tr;  // atr.rs:393
         return rma;
     }
+    static constexpr int kScan = 1;
+    __device__ double decay() const { return 1.0 - alpha; }
+    __device__ void begin_carry(double h, double l, double c, const double *st) {
+        rma = st[0];
+        pc = c;
+    }
+    __device__ void get_state(double *st) const { st[0] = rma; }
 };
 
 struct AdxEng {
@@ -242,6 +272,12 @@ struct AdxEng {
         adx = (adx * pm1 + dx) * rp;
         return adx;
     }
+    // two coupled stages (the DX input of the second depends on the first's
+    // state): burn-in chunking only
+    static constexpr int kScan = 0;
+    __device__ double decay() const { return omr; }
+    __device__ void begin_carry(double h, double l, double c, const double *st) {}
+    __device__ void get_state(double *st) const {}
 };
 
 // SMMA (smma.rs:243-271): the mean of the first p bars at fv+p-1, then
@@ -260,12 +296,13 @@ struct SmmaEng {
     __device__ int burn_ok_from() const { return fv + p; }
     __device__ double begin_exact(double x) {
         sum = x;
+        prev = x;
         if (p == 1) return x;
         return REC_NAN;
     }
     __device__ void begin_burn(double x) { prev = x; }
     __device__ double step(int t, double x) {
-        if (p == 1) return x;
+        if (p == 1) { prev = x; return x; }
         const int ws = fv + p - 1;
         if (t < ws) { sum += x; return REC_NAN; }
         if (t == ws) { sum += x; prev = sum / (double)p; return prev; }
@@ -273,6 +310,10 @@ struct SmmaEng {
         return prev;
     }
     __device__ double out_before_start(int t) const { return REC_NAN; }
+    static constexpr int kScan = 1;
+    __device__ double decay() const { return pm1 * rp; }
+    __device__ void begin_carry(double x, const double *st) { prev = st[0]; }
+    __device__ void get_state(double *st) const { st[0] = prev; }
 };
 
 // Wilders (wilders.rs:262-320): seed summed in groups of four,
@@ -300,11 +341,15 @@ struct WildersEng {
     __device__ double step(int t, double x) {
         const int k = t - fv;
         if (k < p) return seed_bar(k, x);
-        if (p == 1) return x;
+        if (p == 1) { y = x; return x; }
         y = fma(x - y, al, y);
         return y;
     }
     __device__ double out_before_start(int t) const { return REC_NAN; }
+    static constexpr int kScan = 1;
+    __device__ double decay() const { return 1.0 - al; }
+    __device__ void begin_carry(double x, const double *st) { y = st[0]; }
+    __device__ void get_state(double *st) const { st[0] = y; }
 };
 
 // ---------------------------------------------------------------- chunk planning
@@ -590,15 +635,24 @@ __device__ __forceinline__ void drive_batch(const rec_args &a, float *out) {
     const int c = blockIdx.x * 64 + lane;
     const bool active = c < a.nlanes;
     const int T = a.T;
-    const int c0 = blockIdx.y * a.chunk;
+    const int k = blockIdx.y;
+    const int c0 = k * a.chunk;
     const int c1 = min(c0 + a.chunk, T);
     const int fv = a.fvs[0];
     const bool clean = a.clean != nullptr && a.clean[0] != 0;
     const int p = active ? a.periods[c] : 1;
     E e;
     e.setup(p, fv);
-    bool burn = false;
-    int start = active ? lane_start(e, c0, clean, &burn) : T;
+    bool burn = false, carried = false;
+    int start;
+    if (a.phase != 0) {
+        // chunked scan: chunks past the warm-up start from a carried state at
+        // c0 - 1; an unclean series runs every chunk exactly from first_valid
+        carried = active && clean && c0 >= e.burn_ok_from();
+        start = active ? (carried ? c0 - 1 : e.exact_start()) : T;
+    } else {
+        start = active ? lane_start(e, c0, clean, &burn) : T;
+    }
     if constexpr (E::kSeedScan) {
         // RSI burn-in: the seed window's NaN rule still has to be seen
         if (burn) {
@@ -606,6 +660,13 @@ __device__ __forceinline__ void drive_batch(const rec_args &a, float *out) {
             for (int t = fv + 1; t <= min(fv + p, T - 1); ++t) e.seed_scan(t, (double)a.x[t]);
         }
     }
+    double st0[2] = {0.0, 0.0};
+    if (carried && a.phase == 3) {
+        const double *cs = a.carry + ((size_t)(k - 1) * a.nlanes + c) * (E::kScan > 0 ? E::kScan : 1);
+        st0[0] = cs[0];
+        if (E::kScan > 1) st0[1] = cs[1];
+    }
+    const bool emit = a.phase != 1;
     const int t_begin = wave_min(min(start, c0)) & ~63;
     for (int tb = t_begin; tb < c1; tb += 64) {
         {
@@ -626,9 +687,12 @@ __device__ __forceinline__ void drive_batch(const rec_args &a, float *out) {
                 const double x = xs[0][j];
                 if (t == start) {
                     if constexpr (NIN == 1) {
-                        if (burn) e.begin_burn(x); else y = e.begin_exact(x);
+                        if (carried) e.begin_carry(x, st0);
+                        else if (burn) e.begin_burn(x);
+                        else y = e.begin_exact(x);
                     } else {
-                        if (burn) e.begin_burn(xs[1][j], xs[2][j], x);
+                        if (carried) e.begin_carry(xs[1][j], xs[2][j], x, st0);
+                        else if (burn) e.begin_burn(xs[1][j], xs[2][j], x);
                         else y = e.begin_exact(xs[1][j], xs[2][j], x);
                     }
                 } else if (t > start) {
@@ -636,22 +700,59 @@ __device__ __forceinline__ void drive_batch(const rec_args &a, float *out) {
                     else y = e.step(t, xs[1][j], xs[2][j], x);
                 }
             }
-            ob[lane][j] = (float)y;
+            if (emit) ob[lane][j] = (float)y;
+        }
+        if (emit) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // write rows: lane = bar offset
+            if (tb + 64 > c0) {
+                const int t = tb + lane;
+                const bool ok = t >= c0 && t < c1;
+                const int nrow = min(64, a.nlanes - blockIdx.x * 64);
+                for (int r = 0; r < nrow; ++r)
+                    if (ok) out[(size_t)(blockIdx.x * 64 + r) * T + t] = ob[r][lane];
+            }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // write rows: lane = bar offset
-        if (tb + 64 > c0) {
-            const int t = tb + lane;
-            const bool ok = t >= c0 && t < c1;
-            const int nrow = min(64, a.nlanes - blockIdx.x * 64);
-            for (int r = 0; r < nrow; ++r)
-                if (ok) out[(size_t)(blockIdx.x * 64 + r) * T + t] = ob[r][lane];
+    }
+    if constexpr (E::kScan > 0) {
+        if (a.phase == 1 && active && k + 1 < (int)gridDim.y) {
+            double st[2];
+            e.get_state(st);
+            double *cs = a.carry + ((size_t)k * a.nlanes + c) * E::kScan;
+            cs[0] = st[0];
+            if (E::kScan > 1) cs[1] = st[1];
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+// Folds the chunk-end states of pass 1 in chunk order, in place: for a chunk
+// past the warm-up, s_end = decay^L s_prev + B (B = its zero-start run); a
+// chunk holding the warm-up ran exactly from first_valid, so s_end = B.
+// grid.x = groups of 64 combos; lane = combo.
+template <class E>
+__device__ __forceinline__ void scan_combine(const rec_args &a) {
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+    if (c >= a.nlanes || a.clean == nullptr || a.clean[0] == 0) return;
+    E e;
+    e.setup(a.periods[c], a.fvs[0]);
+    constexpr int S = E::kScan > 0 ? E::kScan : 1;
+    const double dl = pow(e.decay(), (double)a.chunk);
+    const int nch = (a.T + a.chunk - 1) / a.chunk;
+    double y0 = 0.0, y1 = 0.0;
+    for (int k = 0; k + 1 < nch; ++k) {
+        double *cs = a.carry + ((size_t)k * a.nlanes + c) * S;
+        const bool carried = k * a.chunk >= e.burn_ok_from();
+        y0 = carried ? fma(dl, y0, cs[0]) : cs[0];
+        cs[0] = y0;
+        if (S > 1) {
+            y1 = carried ? fma(dl, y1, cs[1]) : cs[1];
+            cs[1] = y1;
+        }
     }
 }
 
@@ -672,6 +773,21 @@ extern "C" __global__ __launch_bounds__(64) void wilders_batch_f32(rec_args a) {
 }
 extern "C" __global__ __launch_bounds__(64) void adx_batch_f32(rec_args a) {
     drive_batch<AdxEng, 3>(a, a.out0);
+}
+extern "C" __global__ __launch_bounds__(64) void ema_scan_combine(rec_args a) {
+    scan_combine<EmaEng>(a);
+}
+extern "C" __global__ __launch_bounds__(64) void rsi_scan_combine(rec_args a) {
+    scan_combine<RsiEng>(a);
+}
+extern "C" __global__ __launch_bounds__(64) void atr_scan_combine(rec_args a) {
+    scan_combine<AtrEng>(a);
+}
+extern "C" __global__ __launch_bounds__(64) void smma_scan_combine(rec_args a) {
+    scan_combine<SmmaEng>(a);
+}
+extern "C" __global__ __launch_bounds__(64) void wilders_scan_combine(rec_args a) {
+    scan_combine<WildersEng>(a);
 }
 
 // ---------------------------------------------------------------- preparation

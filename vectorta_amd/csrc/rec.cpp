@@ -26,11 +26,15 @@ struct rec_args_host {
     float *out0;
     float *out1;
     float *out2;
+    double *carry;
     int T;
     int nlanes;
     int chunk;
     int p1, p2, p3;
+    int phase;
+    int pad_;
 };
+static_assert(sizeof(rec_args_host) == 120, "rec_args layout");
 
 enum Ind { IND_EMA = 0, IND_RSI = 1, IND_ATR = 2, IND_ADX = 3, IND_SMMA = 4, IND_WILDERS = 5 };
 
@@ -140,6 +144,43 @@ vta_status launch_rec(vta_ctx *ctx, const char *kname, rec_args_host &a, unsigne
     return launch(ctx, f, dim3(groups, chunks, 1), dim3(block), 0, a);
 }
 
+// Chunked-scan launch for the single-stage recurrences: chunks sized for
+// ~4096 waves (at least 256 bars), as many as the context's carry buffer holds
+// for C combos; pass 1, the per-combo fold, pass 3.  Returns false (nothing
+// launched) when one chunk would cover the series.
+const char *combine_kernel(Ind k) {
+    static const char *n[] = {"ema_scan_combine", "rsi_scan_combine", "atr_scan_combine",
+                              nullptr, "smma_scan_combine", "wilders_scan_combine"};
+    return n[k];
+}
+
+vta_status scan_launch(vta_ctx *ctx, Ind k, rec_args_host a, size_t T, size_t C, bool *done) {
+    *done = false;
+    if (k == IND_ADX || getenv("VTA_REC_NO_SCAN")) return VTA_OK;
+    const size_t S = k == IND_RSI ? 2 : 1;
+    const size_t groups = (C + 63) / 64;
+    const size_t want = std::max<size_t>(1, 4096 / groups);
+    size_t chunk = std::max<size_t>(256, (T + want - 1) / want);
+    const size_t max_chunks = vta_ctx::kCarryBytes / (C * S * sizeof(double));
+    if (max_chunks < 2) return VTA_OK;
+    if ((T + chunk - 1) / chunk > max_chunks) chunk = (T + max_chunks - 1) / max_chunks;
+    chunk = (chunk + 63) & ~(size_t)63;
+    const size_t nch = (T + chunk - 1) / chunk;
+    if (nch < 2) return VTA_OK;
+    a.chunk = (int)chunk;
+    a.carry = static_cast<double *>(ctx->carry);
+    hipFunction_t fb, fc;
+    VTA_TRY(get_function(ctx, "rec", batch_kernel(k), &fb));
+    VTA_TRY(get_function(ctx, "rec", combine_kernel(k), &fc));
+    a.phase = 1;
+    VTA_TRY(launch(ctx, fb, dim3((unsigned)groups, (unsigned)nch, 1), dim3(64), 0, a));
+    VTA_TRY(launch(ctx, fc, dim3((unsigned)groups), dim3(64), 0, a));
+    a.phase = 3;
+    VTA_TRY(launch(ctx, fb, dim3((unsigned)groups, (unsigned)nch, 1), dim3(64), 0, a));
+    *done = true;
+    return VTA_OK;
+}
+
 // One series x many periods, device input and output, fv and clean known.
 vta_status batch_launch(vta_ctx *ctx, Ind k, const float *d_x, const float *d_h,
                         const float *d_l, const int32_t *d_periods, size_t max_p, size_t T,
@@ -155,6 +196,11 @@ vta_status batch_launch(vta_ctx *ctx, Ind k, const float *d_x, const float *d_h,
     a.out0 = d_out;
     a.T = (int)T;
     a.nlanes = (int)C;
+    if (clean) {
+        bool done;
+        VTA_TRY(scan_launch(ctx, k, a, T, C, &done));
+        if (done) return VTA_OK;
+    }
     unsigned groups = (unsigned)((C + 63) / 64);
     a.chunk = pick_chunk_batch(k, max_p, T, groups, clean);
     unsigned chunks = (unsigned)((T + a.chunk - 1) / a.chunk);

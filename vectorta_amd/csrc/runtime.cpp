@@ -272,6 +272,12 @@ vta_status vta_ctx_create(int device_id, vta_ctx **out) {
         delete ctx;
         return fail(VTA_ERR_OUT_OF_MEMORY, "hipMalloc of the context scratch failed");
     }
+    if (hipMalloc(&ctx->carry, vta_ctx::kCarryBytes) != hipSuccess) {
+        (void)hipFree(ctx->scratch);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return fail(VTA_ERR_OUT_OF_MEMORY, "hipMalloc of the context carry buffer failed");
+    }
     const char *kd = getenv("VTA_KERNEL_DIR");
     ctx->kernel_dir = (kd && *kd) ? std::string(kd) : library_dir();
     *out = ctx;
@@ -284,6 +290,7 @@ vta_status vta_ctx_destroy(vta_ctx *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto &kv : ctx->modules) (void)hipModuleUnload(kv.second);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->carry) (void)hipFree(ctx->carry);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return VTA_OK;
@@ -322,6 +329,11 @@ vta_status vta_ctx_preload(vta_ctx *ctx) {
         {"rec", "rsi_ms1p_f32"},
         {"rec", "atr_ms1p_f32"},
         {"rec", "adx_ms1p_f32"},
+        {"rec", "ema_scan_combine"},
+        {"rec", "rsi_scan_combine"},
+        {"rec", "atr_scan_combine"},
+        {"rec", "smma_scan_combine"},
+        {"rec", "wilders_scan_combine"},
         {"rec", "smma_batch_f32"},
         {"rec", "wilders_batch_f32"},
         {"rec", "smma_ms1p_f32"},

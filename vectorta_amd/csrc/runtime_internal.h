@@ -25,6 +25,10 @@ struct vta_ctx {
     // of the enqueue-only entry points; reused in stream order)
     void *scratch = nullptr;
     static constexpr size_t kScratchBytes = 64 * 1024;
+    // chunk-end states of the recurrences' chunked scan ([chunk][combo][state]
+    // f64), reused in stream order by every batch launch of the context
+    void *carry = nullptr;
+    static constexpr size_t kCarryBytes = 16u << 20;
 };
 
 namespace vta {
