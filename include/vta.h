@@ -82,6 +82,16 @@ vta_status vta_device_array_copy_to_host(vta_ctx *ctx, const vta_device_array_f3
                                          float *host, size_t host_len);
 
 /* Raw device memory for the enqueue-only *_device entry points. */
+/* DLPack export (export_f32_cuda_dlpack_2d, src/utilities/dlpack_cuda.rs:13-276):
+ * moves the buffer of *arr into a new DLManagedTensorVersioned (versioned != 0,
+ * DLPack 1.0) or DLManagedTensor (legacy) with device {kDLROCM = 10, the
+ * context's device}, float32, shape {rows, cols}, strides {cols, 1}, after the
+ * context's stream has drained.  *arr is left empty; the tensor's deleter frees
+ * the buffer on its device.  vta_dlpack_delete runs that deleter, for a tensor
+ * that no consumer took. */
+vta_status vta_device_array_to_dlpack(vta_ctx *ctx, vta_device_array_f32 *arr, int versioned,
+                                      void **out_managed);
+void vta_dlpack_delete(void *managed, int versioned);
 vta_status vta_device_alloc(vta_ctx *ctx, size_t bytes, void **out);
 vta_status vta_device_free(vta_ctx *ctx, void *ptr);
 vta_status vta_memcpy_htod(vta_ctx *ctx, void *dst, const void *src, size_t bytes);

@@ -137,3 +137,45 @@ def test_moving_average_grids_match_reference_semantics():
     assert e((10, 5, 3)) == [10, 7, 5]
     assert e((10, 4, 3)) == [10, 7, 4]
     assert e((4, 12, 4)) == [4, 8, 12]
+
+
+class _DLTensor(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("device_type", ctypes.c_int32),
+                ("device_id", ctypes.c_int32), ("ndim", ctypes.c_int32),
+                ("code", ctypes.c_uint8), ("bits", ctypes.c_uint8), ("lanes", ctypes.c_uint16),
+                ("shape", ctypes.POINTER(ctypes.c_int64)),
+                ("strides", ctypes.POINTER(ctypes.c_int64)), ("byte_offset", ctypes.c_uint64)]
+
+
+class _DLManaged(ctypes.Structure):
+    _fields_ = [("dl_tensor", _DLTensor), ("manager_ctx", ctypes.c_void_p),
+                ("deleter", ctypes.c_void_p)]
+
+
+class _DLManagedVersioned(ctypes.Structure):
+    _fields_ = [("major", ctypes.c_uint32), ("minor", ctypes.c_uint32),
+                ("manager_ctx", ctypes.c_void_p), ("deleter", ctypes.c_void_p),
+                ("flags", ctypes.c_uint64), ("dl_tensor", _DLTensor)]
+
+
+@pytest.mark.parametrize("versioned", [0, 1])
+def test_dlpack_export_layout_without_device(versioned):
+    """The DLPack structs follow the public ABI (kDLROCM = 10, float32, row-major
+    strides in elements) and ownership moves out of the array; an empty array
+    exports without a device (dlpack_cuda.rs:13-276 is the reference form)."""
+    arr = _lib.DeviceArrayF32C()
+    arr.ptr = None
+    arr.rows, arr.cols = 0, 7
+    mt = ctypes.c_void_p()
+    _lib.check(_lib.lib().vta_device_array_to_dlpack(None, ctypes.byref(arr), versioned,
+                                                     ctypes.byref(mt)))
+    m = ctypes.cast(mt, ctypes.POINTER(_DLManagedVersioned if versioned else _DLManaged))[0]
+    if versioned:
+        assert (m.major, m.minor, m.flags) == (1, 0, 0)
+    d = m.dl_tensor
+    assert (d.device_type, d.device_id, d.ndim) == (10, 0, 2)
+    assert (d.code, d.bits, d.lanes, d.byte_offset) == (2, 32, 1, 0)
+    assert (d.shape[0], d.shape[1], d.strides[0], d.strides[1]) == (0, 7, 7, 1)
+    assert not d.data and m.deleter
+    assert arr.rows == 0 and arr.cols == 0
+    _lib.lib().vta_dlpack_delete(mt, versioned)

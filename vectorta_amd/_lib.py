@@ -131,6 +131,9 @@ SIGNATURES = {
         _ST, [_P, _P, _SZ, _SZ, ctypes.POINTER(AlmaParamsC), ctypes.POINTER(DeviceArrayF32C)]),
     "vta_alma_many_series_one_param_time_major_device": (
         _ST, [_P, _P, _P, _SZ, _F32, _SZ, _SZ, _P, _P]),
+    "vta_device_array_to_dlpack": (_ST, [_P, ctypes.POINTER(DeviceArrayF32C), ctypes.c_int,
+                                         ctypes.POINTER(_P)]),
+    "vta_dlpack_delete": (None, [_P, ctypes.c_int]),
     # recurrences
     "vta_ema_expand_grid": (_ST, [ctypes.POINTER(RangeUsize), _P, _SZ, ctypes.POINTER(_SZ)]),
     "vta_ema_batch_dev": (_ST, [_P, _P, _SZ, ctypes.POINTER(RangeUsize),

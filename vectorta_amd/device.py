@@ -140,6 +140,62 @@ class DeviceArrayF32:
         except Exception:
             pass
 
+    # ---- device-handle wire formats (src/utilities/dlpack_cuda.rs:278-374)
+    @property
+    def __cuda_array_interface__(self) -> dict:
+        """Version-3 array interface, the form torch-rocm and CuPy consume."""
+        n = self.rows * self.cols
+        return {"shape": (self.rows, self.cols), "typestr": "<f4",
+                "strides": (self.cols * 4, 4), "data": (self.device_ptr() if n else 0, False),
+                "version": 3}
+
+    def __dlpack_device__(self):
+        return (KDL_ROCM, self.ctx.device_id)
+
+    def __dlpack__(self, stream=None, max_version=None, dl_device=None, copy=None):
+        """Exports the buffer as a DLPack capsule ("dltensor_versioned" when the
+        consumer accepts DLPack >= 1, else "dltensor").  Ownership moves into
+        the capsule, as in the reference (this array is left empty); the data
+        are complete before return, so any consumer stream may read them."""
+        if dl_device is not None and tuple(dl_device) != self.__dlpack_device__():
+            raise BufferError("__dlpack__: dl_device differs and copies are not implemented"
+                              if copy else "__dlpack__: dl_device mismatch")
+        versioned = max_version is not None and int(tuple(max_version)[0]) >= 1
+        mt = ctypes.c_void_p()
+        check(lib().vta_device_array_to_dlpack(self.ctx.handle, ctypes.byref(self._c),
+                                               1 if versioned else 0, ctypes.byref(mt)))
+        return _capsule(mt.value, versioned)
+
+
+KDL_ROCM = 10
+_CAP_NAMES = {False: b"dltensor", True: b"dltensor_versioned"}
+_PyCapsule_Destructor = ctypes.CFUNCTYPE(None, ctypes.c_void_p)
+
+
+def _capsule_dtor(cap):
+    """Frees the tensor of a capsule no consumer took (a consumer renames it
+    to used_dltensor[_versioned] and owns the tensor from then on)."""
+    api = ctypes.pythonapi
+    for versioned, name in _CAP_NAMES.items():
+        if api.PyCapsule_IsValid(ctypes.c_void_p(cap), name):
+            p = api.PyCapsule_GetPointer(ctypes.c_void_p(cap), name)
+            lib().vta_dlpack_delete(ctypes.c_void_p(p), 1 if versioned else 0)
+            return
+
+
+_CAP_DTOR = _PyCapsule_Destructor(_capsule_dtor)
+
+
+def _capsule(ptr: int, versioned: bool):
+    api = ctypes.pythonapi
+    api.PyCapsule_New.restype = ctypes.py_object
+    api.PyCapsule_New.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _PyCapsule_Destructor]
+    api.PyCapsule_IsValid.restype = ctypes.c_int
+    api.PyCapsule_IsValid.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+    api.PyCapsule_GetPointer.restype = ctypes.c_void_p
+    api.PyCapsule_GetPointer.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+    return api.PyCapsule_New(ctypes.c_void_p(ptr), _CAP_NAMES[versioned], _CAP_DTOR)
+
 
 def f32_ptr(a: np.ndarray) -> ctypes.c_void_p:
     return a.ctypes.data_as(ctypes.c_void_p)
