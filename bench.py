@@ -91,23 +91,14 @@ def algorithmic_work(combos, T):
 
 
 def time_form(alma, plan, d_x, d_out, steps, warmup, ev):
-    stream = alma.ctx.stream
-    for _ in range(warmup):
-        plan.launch(d_x.ptr, 0, d_out.ptr)
-    alma.synchronize()
-    starts = [ev.create() for _ in range(steps)]
-    ends = [ev.create() for _ in range(steps)]
-    t0 = time.perf_counter()
-    for i in range(steps):
-        ev.record(starts[i], stream)
-        plan.launch(d_x.ptr, 0, d_out.ptr)
-        ev.record(ends[i], stream)
-    alma.synchronize()
-    wall = time.perf_counter() - t0
-    ks = np.array([ev.elapsed_ms(a, b) for a, b in zip(starts, ends)])
-    for e in starts + ends:
-        ev.destroy(e)
-    return wall * 1e3 / steps, ks
+    """Returns (wall ms per step, mean launch ms, per-launch ms array).  The mean
+    comes from `steps` launches enqueued back to back between two HIP events on
+    the library's stream (the GPU never waits on the host inside the bracket);
+    the per-launch array, for p50/p99, from a second pass with an event pair
+    around every launch."""
+    mean, ks, wall = _time_launches(lambda: plan.launch(d_x.ptr, 0, d_out.ptr), alma.ctx.stream,
+                                    steps, warmup, ev, sync=alma.synchronize)
+    return wall, mean, ks
 
 
 def cpu_baseline(x, combos, threads):
@@ -168,23 +159,12 @@ def bench_many_series_ohlc(steps, warmup, ev):
     ctx = vta.HipContext(0)
     plan = vta.ManySeriesOhlcPlan(ctx, c, h, lo)
     del o, h, lo, c
-    for _ in range(warmup):
-        plan.launch(20, 14, 14)
-    ctx.synchronize()
-    ks = []
-    for _ in range(steps):
-        a, b = ev.create(), ev.create()
-        ev.record(a, ctx.stream)
-        plan.launch(20, 14, 14)
-        ev.record(b, ctx.stream)
-        ks.append(ev.elapsed_ms(a, b))
-        ev.destroy(a)
-        ev.destroy(b)
-    ks = np.array(ks)
+    mean, ks, _ = _time_launches(lambda: plan.launch(20, 14, 14), ctx.stream, steps, warmup, ev,
+                                 sync=ctx.synchronize)
     alg = 3 * 4 * int(lens.sum()) + 3 * 4 * rows * cols
-    t = float(ks.mean()) * 1e-3
+    t = mean * 1e-3
     return {"workload": "ema20_rsi14_atr14_many_series_4096x50000", "kernel": "ms_ema_rsi_atr_f32",
-            "kernel_ms_mean": float(ks.mean()), "kernel_ms_p50": float(np.percentile(ks, 50)),
+            "kernel_ms_mean": mean, "kernel_ms_p50": float(np.percentile(ks, 50)),
             "kernel_ms_p99": float(np.percentile(ks, 99)), "outputs": 3 * rows * cols,
             "outputs_per_s": 3 * rows * cols / t, "algorithmic_bytes": alg,
             "traffic": profiled_traffic("ema_rsi_atr_many_series_4096x50000_traffic.txt",
@@ -206,22 +186,10 @@ def bench_rolling(steps, warmup, ev):
     for name, fn, nout, kern in (("bb", lambda: plan.launch_bb(20, 2.0, 2.0), 3, "bb_ms1p_f32"),
                                  ("pnr", lambda: plan.launch_pnr(100, 70.0), 1, "pnr_ms1p_f32"),
                                  ("mad", lambda: plan.launch_mad(64), 1, "mad_ms1p_f32")):
-        for _ in range(warmup):
-            fn()
-        ctx.synchronize()
-        ks = []
-        for _ in range(steps):
-            a, b = ev.create(), ev.create()
-            ev.record(a, ctx.stream)
-            fn()
-            ev.record(b, ctx.stream)
-            ks.append(ev.elapsed_ms(a, b))
-            ev.destroy(a)
-            ev.destroy(b)
-        ks = np.array(ks)
-        t = float(ks.mean()) * 1e-3
+        mean, ks, _ = _time_launches(fn, ctx.stream, steps, warmup, ev, sync=ctx.synchronize)
+        t = mean * 1e-3
         alg = 4 * rows * cols * (1 + nout)
-        res[name] = {"kernel": kern, "kernel_ms_mean": float(ks.mean()),
+        res[name] = {"kernel": kern, "kernel_ms_mean": mean,
                      "kernel_ms_p50": float(np.percentile(ks, 50)),
                      "kernel_ms_p99": float(np.percentile(ks, 99)),
                      "series_bars_per_s": rows * cols / t, "algorithmic_bytes": alg,
@@ -231,9 +199,25 @@ def bench_rolling(steps, warmup, ev):
     return res
 
 
-def _time_launches(fn, stream, steps, warmup, ev):
+def _time_launches(fn, stream, steps, warmup, ev, sync=None):
+    """(mean launch ms, per-launch ms array, wall ms per step) of enqueue-only
+    launches `fn` on `stream`: the mean from `steps` launches back to back
+    between two HIP events, the per-launch figures from a second pass with an
+    event pair around each launch."""
     for _ in range(warmup):
         fn()
+    if sync:
+        sync()
+    a, b = ev.create(), ev.create()
+    t0 = time.perf_counter()
+    ev.record(a, stream)
+    for _ in range(steps):
+        fn()
+    ev.record(b, stream)
+    total = ev.elapsed_ms(a, b)
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    ev.destroy(a)
+    ev.destroy(b)
     ks = []
     for _ in range(steps):
         a, b = ev.create(), ev.create()
@@ -243,7 +227,7 @@ def _time_launches(fn, stream, steps, warmup, ev):
         ks.append(ev.elapsed_ms(a, b))
         ev.destroy(a)
         ev.destroy(b)
-    return np.array(ks)
+    return total / steps, np.array(ks), wall
 
 
 def bench_moving_averages(steps, warmup, ev):
@@ -273,16 +257,19 @@ def bench_moving_averages(steps, warmup, ev):
             fn = (lambda op=op, name=name: getattr(op, f"{name}_batch_device")(
                 d_x, d_p, T, C, 0, int(periods.max()), d_o))
         # every op launches on the stream of its own context
-        ks = _time_launches(fn, op.ctx.stream, steps, warmup, ev)
+        mean, ks, _ = _time_launches(fn, op.ctx.stream, steps, warmup, ev, sync=op.synchronize)
         op.synchronize()
-        t = float(ks.mean()) * 1e-3
+        t = mean * 1e-3
         alg = 4 * (C * T + T)
         res[f"{name}_batch"] = {"kernel": kern[name], "combos": C, "series_len": T,
-                                "kernel_ms_mean": float(ks.mean()),
+                                "kernel_ms_mean": mean,
                                 "kernel_ms_p50": float(np.percentile(ks, 50)),
                                 "outputs_per_s": C * T / t, "algorithmic_bytes": alg,
                                 "achieved_GBs": alg / t / 1e9,
                                 "frac_of_hbm_peak": alg / t / 1e9 / HBM_PEAK_GBS}
+        if name in ("sma", "wma", "swma"):
+            res[f"{name}_batch"]["traffic"] = profiled_traffic(
+                "moving_averages_traffic.txt", f"fir_batch_f32[{name}]")
     d_o.free()
     d_x.free()
     d_p.free()
@@ -307,11 +294,11 @@ def bench_moving_averages(steps, warmup, ev):
             fn = (lambda meth=meth, p=p: meth(d_tm, d_fv, p, cols, rows, d_o, d_clean=d_clean))
         else:
             fn = (lambda meth=meth, p=p: meth(d_tm, d_fv, cols, rows, p, d_o))
-        ks = _time_launches(fn, op.ctx.stream, steps, warmup, ev)
+        mean, ks, _ = _time_launches(fn, op.ctx.stream, steps, warmup, ev, sync=op.synchronize)
         op.synchronize()
-        t = float(ks.mean()) * 1e-3
+        t = mean * 1e-3
         alg = 4 * 2 * rows * cols
-        res[f"{name}_many_series"] = {"period": p, "kernel_ms_mean": float(ks.mean()),
+        res[f"{name}_many_series"] = {"period": p, "kernel_ms_mean": mean,
                                       "kernel_ms_p50": float(np.percentile(ks, 50)),
                                       "outputs_per_s": rows * cols / t,
                                       "algorithmic_bytes": alg,
@@ -335,23 +322,11 @@ def bench_double_alma(steps, warmup, ev):
     ctx = vta.HipContext(0)
     plan = vta.DoubleAlmaPlan(ctx, x, vta.AlmaBatchRange(**datagen.DOUBLE_ALMA_FAST),
                               vta.AlmaBatchRange(**datagen.DOUBLE_ALMA_SLOW))
-    for _ in range(warmup):
-        plan.launch()
-    ctx.synchronize()
-    ks = []
-    for _ in range(steps):
-        a, b = ev.create(), ev.create()
-        ev.record(a, ctx.stream)
-        plan.launch()
-        ev.record(b, ctx.stream)
-        ks.append(ev.elapsed_ms(a, b))
-        ev.destroy(a)
-        ev.destroy(b)
+    mean, ks, _ = _time_launches(plan.launch, ctx.stream, steps, warmup, ev, sync=ctx.synchronize)
     st, _ = plan.results()
     plan.close()
-    ks = np.array(ks)
-    t = float(np.median(ks)) * 1e-3
-    return {"workload": "double_alma_crossover_58300_pairs_x_200000",
+    t = mean * 1e-3
+    return {"workload": "double_alma_crossover_58300_pairs_x_200000", "ms_mean": mean,
             "ms_p50": float(np.median(ks)), "ms_p99": float(np.percentile(ks, 99)),
             "pair_bars_per_s": 58_300 * x.size / t,
             "mean_trades_per_pair": float(st[:, 0].mean()),
@@ -523,15 +498,15 @@ def main():
     outputs = {}
     for form in args.forms.split(","):
         plan = alma.sweep_plan(sweep, T, kernel=form)
-        ms_step, ks = time_form(alma, plan, d_x, d_out, args.steps, args.warmup, ev)
+        ms_step, k_mean, ks = time_form(alma, plan, d_x, d_out, args.steps, args.warmup, ev)
         forms[form] = {
             "ms_per_step": ms_step,
-            "kernel_ms_mean": float(ks.mean()),
+            "kernel_ms_mean": k_mean,
             "kernel_ms_p50": float(np.percentile(ks, 50)),
             "kernel_ms_p99": float(np.percentile(ks, 99)),
             "outputs_per_s": C * T / (ms_step * 1e-3),
-            "achieved_GBs": alg_bytes / (ks.mean() * 1e-3) / 1e9,
-            "achieved_TFLOPs": alg_flops / (ks.mean() * 1e-3) / 1e12,
+            "achieved_GBs": alg_bytes / (k_mean * 1e-3) / 1e9,
+            "achieved_TFLOPs": alg_flops / (k_mean * 1e-3) / 1e12,
         }
         if not args.no_cpu:
             outputs[form] = d_out.copy_to_host().reshape(C, T)
@@ -561,7 +536,7 @@ def main():
 
     head = "mfma" if ok("mfma") else "fp32"
     hf = forms[head]
-    out_kernel = "alma_batch_mfma_bf16x3" if head == "mfma" else "alma_batch_f32"
+    out_kernel = "alma_batch_mfma_bf16x3_wg16" if head == "mfma" else "alma_batch_f32"
     mf = forms.get("mfma")
     ff = forms.get("fp32")
     out = {
