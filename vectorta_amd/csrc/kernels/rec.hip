@@ -100,6 +100,11 @@ struct EmaEng {
     }
     __device__ double out_before_start(int t) const { return REC_NAN; }
     static constexpr int kScan = 1;                  // states carried by the chunked scan
+    // the step for t >= burn_ok_from() (no warm-up branches)
+    __device__ double steady(double x) {
+        if (fin(x)) prev = fma(beta, prev, alpha * x);
+        return prev;
+    }
     __device__ double decay() const { return beta; }
     __device__ void begin_carry(double x, const double *st) { prev = st[0]; vc = p; }
     __device__ void get_state(double *st) const { st[0] = prev; }
@@ -160,6 +165,17 @@ struct RsiEng {
         return den == 0.0 ? 50.0 : rec_ratio(100.0 * ag, den);
     }
     static constexpr int kScan = 2;
+    __device__ double steady(double x) {
+        double d = x - px;
+        px = x;
+        double g = d > 0.0 ? d : 0.0;
+        double lo = d < 0.0 ? -d : 0.0;
+        ag = fma(ag, beta, inv_p * g);
+        al = fma(al, beta, inv_p * lo);
+        if (poisoned) return REC_NAN;
+        double den = ag + al;
+        return den == 0.0 ? 50.0 : rec_ratio(100.0 * ag, den);
+    }
     __device__ double decay() const { return beta; }
     __device__ void begin_carry(double x, const double *st) { px = x; ag = st[0]; al = st[1]; }
     __device__ void get_state(double *st) const { st[0] = ag; st[1] = al; }
@@ -201,6 +217,12 @@ struct AtrEng {
         return rma;
     }
     static constexpr int kScan = 1;
+    __device__ double steady(double h, double l, double c) {
+        double tr = true_range(h, l, pc);
+        pc = c;
+        rma = fma(-alpha, rma, rma) + alpha * tr;
+        return rma;
+    }
     __device__ double decay() const { return 1.0 - alpha; }
     __device__ void begin_carry(double h, double l, double c, const double *st) {
         rma = st[0];
@@ -275,6 +297,24 @@ struct AdxEng {
     // two coupled stages (the DX input of the second depends on the first's
     // state): burn-in chunking only
     static constexpr int kScan = 0;
+    __device__ double steady(double ch, double cl, double cc) {
+        double hl = ch - cl;
+        double hpc = fabs(ch - pc);
+        double lpc = fabs(cl - pc);
+        double tr = fmax(fmax(hl, hpc), lpc);
+        double up = ch - ph;
+        double down = pl - cl;
+        ph = ch; pl = cl; pc = cc;
+        double pdm = (up > down && up > 0.0) ? up : 0.0;
+        double mdm = (down > up && down > 0.0) ? down : 0.0;
+        atr = atr * omr + tr;
+        pds = pds * omr + pdm;
+        mds = mds * omr + mdm;
+        const double sdm = pds + mds;
+        double dx = (atr != 0.0 && sdm != 0.0) ? rec_ratio(100.0 * fabs(pds - mds), sdm) : 0.0;
+        adx = (adx * pm1 + dx) * rp;
+        return adx;
+    }
     __device__ double decay() const { return omr; }
     __device__ void begin_carry(double h, double l, double c, const double *st) {}
     __device__ void get_state(double *st) const {}
@@ -311,6 +351,11 @@ struct SmmaEng {
     }
     __device__ double out_before_start(int t) const { return REC_NAN; }
     static constexpr int kScan = 1;
+    __device__ double steady(double x) {
+        if (p == 1) { prev = x; return x; }
+        prev = (prev * pm1 + x) * rp;
+        return prev;
+    }
     __device__ double decay() const { return pm1 * rp; }
     __device__ void begin_carry(double x, const double *st) { prev = st[0]; }
     __device__ void get_state(double *st) const { st[0] = prev; }
@@ -347,6 +392,11 @@ struct WildersEng {
     }
     __device__ double out_before_start(int t) const { return REC_NAN; }
     static constexpr int kScan = 1;
+    __device__ double steady(double x) {
+        if (p == 1) { y = x; return x; }
+        y = fma(x - y, al, y);
+        return y;
+    }
     __device__ double decay() const { return 1.0 - al; }
     __device__ void begin_carry(double x, const double *st) { y = st[0]; }
     __device__ void get_state(double *st) const { st[0] = y; }
@@ -629,7 +679,7 @@ extern "C" __global__ __launch_bounds__(64) void ms_ema_rsi_atr_f32(rec_args a) 
 // segment is written as one coalesced 256-byte store.
 template <class E, int NIN>
 __device__ __forceinline__ void drive_batch(const rec_args &a, float *out) {
-    __shared__ float xs[3][64];
+    __shared__ __align__(16) float xs[3][64];
     __shared__ float ob[64][65];
     const int lane = threadIdx.x & 63;
     const int c = blockIdx.x * 64 + lane;
@@ -680,6 +730,36 @@ __device__ __forceinline__ void drive_batch(const rec_args &a, float *out) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // lean path: a full block in which every lane is past its start and
+        // its warm-up -- only the engine's steady update, bars read four at a
+        // time
+        const bool lean = tb + 64 <= c1 && __all(!active || (start < tb && tb >= e.burn_ok_from()));
+        if (lean) {
+#pragma unroll 2
+            for (int j = 0; j < 64; j += 4) {
+                const float4 x4 = *reinterpret_cast<const float4 *>(&xs[0][j]);
+                double y0, y1, y2, y3;
+                if constexpr (NIN == 1) {
+                    y0 = e.steady(x4.x);
+                    y1 = e.steady(x4.y);
+                    y2 = e.steady(x4.z);
+                    y3 = e.steady(x4.w);
+                } else {
+                    const float4 h4 = *reinterpret_cast<const float4 *>(&xs[1][j]);
+                    const float4 l4 = *reinterpret_cast<const float4 *>(&xs[2][j]);
+                    y0 = e.steady(h4.x, l4.x, x4.x);
+                    y1 = e.steady(h4.y, l4.y, x4.y);
+                    y2 = e.steady(h4.z, l4.z, x4.z);
+                    y3 = e.steady(h4.w, l4.w, x4.w);
+                }
+                if (emit) {
+                    ob[lane][j] = (float)y0;
+                    ob[lane][j + 1] = (float)y1;
+                    ob[lane][j + 2] = (float)y2;
+                    ob[lane][j + 3] = (float)y3;
+                }
+            }
+        } else
         for (int j = 0; j < 64; ++j) {
             const int t = tb + j;
             double y = REC_NAN;
