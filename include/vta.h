@@ -240,6 +240,17 @@ vta_status vta_atr_batch_device(vta_ctx *ctx, const float *d_high, const float *
                                 const float *d_close, const int32_t *d_periods,
                                 int32_t max_period, int32_t series_len, int32_t first_valid,
                                 int32_t n_combos, float *d_out);
+/* tr_from_hlc_device (atr_wrapper.rs:566-624): enqueue only; tr[t] = 0 before
+ * first_valid, high - low at first_valid, max(h-l, |h-c[t-1]|, |l-c[t-1]|)
+ * after.  atr_batch_device_with_tr (:626-700): ATR of every period over that
+ * TR series (seed = mean of the first p TRs at first_valid + p - 1); the
+ * reference's alphas and warm indices are derived from the periods. */
+vta_status vta_tr_from_hlc_device(vta_ctx *ctx, const float *d_high, const float *d_low,
+                                  const float *d_close, int32_t series_len, int32_t first_valid,
+                                  float *d_tr_out);
+vta_status vta_atr_batch_device_with_tr(vta_ctx *ctx, const float *d_tr, const int32_t *d_periods,
+                                        int32_t max_period, int32_t series_len,
+                                        int32_t first_valid, int32_t n_combos, float *d_out);
 vta_status vta_atr_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *high_tm,
                                                         const float *low_tm,
                                                         const float *close_tm, size_t cols,

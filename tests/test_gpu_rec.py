@@ -208,3 +208,33 @@ def test_errors(ctxs):
         ctxs["rsi"].rsi_batch_dev(np.ones(10, dtype=np.float32), (20, 20, 0))
     with pytest.raises(vta.VtaError):
         ctxs["ema"].ema_batch_dev(np.ones(10, dtype=np.float32), (0, 0, 0))
+
+
+def test_tr_series_and_atr_batch_with_tr(ctxs):
+    """tr_from_hlc_device + atr_batch_device_with_tr (atr_wrapper.rs:566, 626):
+    the TR series is f32(exact TR) bit for bit; ATR over it matches the ATR
+    oracle of the same high/low/close within the ATR bound (the TR is rounded
+    to f32 once, ~6e-8 relative, before the averaging)."""
+    h, lo, c = _hlc(60_000, 41, lead=9)
+    op = ctxs["atr"]
+    fv = 9
+    n = c.size
+    d_h, d_l, d_c = (op.ctx.to_device(a) for a in (h, lo, c))
+    d_tr = op.ctx.alloc(n * 4)
+    op.tr_from_hlc_device(d_h, d_l, d_c, n, fv, d_tr)
+    op.synchronize()
+    tr = d_tr.copy_to_host(np.float32)
+    hd, ld, cd = (a.astype(np.float64) for a in (h, lo, c))
+    ref = np.zeros(n)
+    ref[fv] = hd[fv] - ld[fv]
+    pc = cd[fv:-1]
+    ref[fv + 1:] = np.maximum.reduce([hd[fv + 1:] - ld[fv + 1:], np.abs(hd[fv + 1:] - pc),
+                                      np.abs(ld[fv + 1:] - pc)])
+    assert np.array_equal(tr.view(np.uint32), ref.astype(np.float32).view(np.uint32))
+    ps = np.asarray(op.expand_grid((3, 200, 13)), dtype=np.int32)
+    d_p = op.ctx.to_device(ps)
+    d_o = op.ctx.alloc(ps.size * n * 4)
+    op.atr_batch_device_with_tr(d_tr, d_p, int(ps.max()), n, fv, ps.size, d_o)
+    op.synchronize()
+    g = d_o.copy_to_host(np.float32).reshape(ps.size, n)
+    _check(g, np.stack([O.atr(hd, ld, cd, int(p)) for p in ps]), "atr")

@@ -156,6 +156,8 @@ SIGNATURES = {
     "vta_wilders_many_series_one_param_time_major_dev": (_ST, [_P, _P, _SZ, _SZ, _SZ,
                                                            ctypes.POINTER(DeviceArrayF32C)]),
     "vta_wilders_many_series_one_param_device": (_ST, [_P, _P, _P, _SZ, _SZ, _SZ, _P, _P]),
+    "vta_tr_from_hlc_device": (_ST, [_P, _P, _P, _P, _I32, _I32, _P]),
+    "vta_atr_batch_device_with_tr": (_ST, [_P, _P, _P, _I32, _I32, _I32, _I32, _P]),
     # finite-window averages
     "vta_sma_expand_grid": (_ST, [ctypes.POINTER(RangeUsize), _P, _SZ, ctypes.POINTER(_SZ)]),
     "vta_sma_batch_dev": (_ST, [_P, _P, _SZ, ctypes.POINTER(RangeUsize),

@@ -231,6 +231,43 @@ struct AtrEng {
     __device__ void get_state(double *st) const { st[0] = rma; }
 };
 
+// ATR fed by a precomputed true-range series (atr_wrapper.rs:626-700,
+// atr_batch_device_with_tr): seed = mean of tr[fv .. fv+p) at fv+p-1, then
+// the Wilder update of atr.rs:393 on tr.
+struct AtrTrEng {
+    static constexpr bool kSeedScan = false;
+    double alpha, sum, rma;
+    int p, fv;
+    __device__ void setup(int p_, int fv_) {
+        p = p_; fv = fv_;
+        alpha = 1.0 / (double)p;
+    }
+    __device__ int kb() const { return burn_bars(1.0 - alpha, 25.0); }
+    __device__ int exact_start() const { return fv; }
+    __device__ int burn_ok_from() const { return fv + p; }
+    __device__ double begin_exact(double tr) {
+        sum = tr;
+        if (p == 1) { rma = sum; return rma; }
+        return REC_NAN;
+    }
+    __device__ void begin_burn(double tr) { rma = tr; }
+    __device__ double step(int t, double tr) {
+        if (t < fv + p - 1) { sum += tr; return REC_NAN; }
+        if (t == fv + p - 1) { sum += tr; rma = sum / (double)p; return rma; }
+        rma = fma(-alpha, rma, rma) + alpha * tr;
+        return rma;
+    }
+    __device__ double out_before_start(int t) const { return REC_NAN; }
+    static constexpr int kScan = 1;
+    __device__ double steady(double tr) {
+        rma = fma(-alpha, rma, rma) + alpha * tr;
+        return rma;
+    }
+    __device__ double decay() const { return 1.0 - alpha; }
+    __device__ void begin_carry(double tr, const double *st) { rma = st[0]; }
+    __device__ void get_state(double *st) const { st[0] = rma; }
+};
+
 struct AdxEng {
     static constexpr bool kSeedScan = false;
     double rp, omr, pm1, atr, pds, mds, dx_sum, adx, ph, pl, pc;
@@ -854,6 +891,27 @@ extern "C" __global__ __launch_bounds__(64) void wilders_batch_f32(rec_args a) {
 extern "C" __global__ __launch_bounds__(64) void adx_batch_f32(rec_args a) {
     drive_batch<AdxEng, 3>(a, a.out0);
 }
+extern "C" __global__ __launch_bounds__(64) void atr_tr_batch_f32(rec_args a) {
+    drive_batch<AtrTrEng, 1>(a, a.out0);
+}
+extern "C" __global__ __launch_bounds__(64) void atr_tr_scan_combine(rec_args a) {
+    scan_combine<AtrTrEng>(a);
+}
+
+// True range of one series (tr_from_hlc_f32 in the reference): 0 before
+// first_valid, high - low at first_valid, then max(h - l, |h - c[t-1]|,
+// |l - c[t-1]|), evaluated in f64 (exact for f32 inputs) and rounded once.
+extern "C" __global__ __launch_bounds__(256) void tr_from_hlc_f32(
+    const float *__restrict__ h, const float *__restrict__ l, const float *__restrict__ c, int T,
+    int first_valid, float *__restrict__ tr) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= T) return;
+    float v = 0.0f;
+    if (t == first_valid) v = (float)((double)h[t] - (double)l[t]);
+    else if (t > first_valid) v = (float)true_range(h[t], l[t], c[t - 1]);
+    tr[t] = v;
+}
+
 extern "C" __global__ __launch_bounds__(64) void ema_scan_combine(rec_args a) {
     scan_combine<EmaEng>(a);
 }

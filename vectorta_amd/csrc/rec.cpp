@@ -36,19 +36,29 @@ struct rec_args_host {
 };
 static_assert(sizeof(rec_args_host) == 120, "rec_args layout");
 
-enum Ind { IND_EMA = 0, IND_RSI = 1, IND_ATR = 2, IND_ADX = 3, IND_SMMA = 4, IND_WILDERS = 5 };
+enum Ind {
+    IND_EMA = 0,
+    IND_RSI = 1,
+    IND_ATR = 2,
+    IND_ADX = 3,
+    IND_SMMA = 4,
+    IND_WILDERS = 5,
+    IND_ATR_TR = 6  // ATR over a precomputed true-range series
+};
 
 // Indicators computed from (high, low, close).
 bool hlc(Ind k) { return k == IND_ATR || k == IND_ADX; }
 
 const char *batch_kernel(Ind k) {
-    static const char *n[] = {"ema_batch_f32",  "rsi_batch_f32",  "atr_batch_f32",
-                              "adx_batch_f32",  "smma_batch_f32", "wilders_batch_f32"};
+    static const char *n[] = {"ema_batch_f32",  "rsi_batch_f32",     "atr_batch_f32",
+                              "adx_batch_f32",  "smma_batch_f32",    "wilders_batch_f32",
+                              "atr_tr_batch_f32"};
     return n[k];
 }
 const char *ms_kernel(Ind k) {
     static const char *n[] = {"ema_ms1p_f32",  "rsi_ms1p_f32",  "atr_ms1p_f32",
-                              "adx_ms1p_f32",  "smma_ms1p_f32", "wilders_ms1p_f32"};
+                              "adx_ms1p_f32",  "smma_ms1p_f32", "wilders_ms1p_f32",
+                              nullptr};
     return n[k];
 }
 
@@ -149,8 +159,9 @@ vta_status launch_rec(vta_ctx *ctx, const char *kname, rec_args_host &a, unsigne
 // for C combos; pass 1, the per-combo fold, pass 3.  Returns false (nothing
 // launched) when one chunk would cover the series.
 const char *combine_kernel(Ind k) {
-    static const char *n[] = {"ema_scan_combine", "rsi_scan_combine", "atr_scan_combine",
-                              nullptr, "smma_scan_combine", "wilders_scan_combine"};
+    static const char *n[] = {"ema_scan_combine",  "rsi_scan_combine",     "atr_scan_combine",
+                              nullptr,             "smma_scan_combine",    "wilders_scan_combine",
+                              "atr_tr_scan_combine"};
     return n[k];
 }
 
@@ -388,6 +399,29 @@ vta_status expand_out(Ind k, const vta_range_usize *range, size_t *out, size_t c
 }  // namespace
 
 extern "C" {
+
+// ------------------------------------------------------------------ ATR from a TR series
+vta_status vta_tr_from_hlc_device(vta_ctx *ctx, const float *d_high, const float *d_low,
+                                  const float *d_close, int32_t series_len, int32_t first_valid,
+                                  float *d_tr_out) {
+    clear_error();
+    if (!ctx || !d_high || !d_low || !d_close || !d_tr_out)
+        return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    if (series_len <= 0) return fail(VTA_ERR_INVALID_INPUT, "empty input");
+    if (first_valid < 0 || first_valid >= series_len)
+        return fail(VTA_ERR_INVALID_INPUT, "first_valid out of range");
+    hipFunction_t f;
+    VTA_TRY(get_function(ctx, "rec", "tr_from_hlc_f32", &f));
+    return launch(ctx, f, dim3((unsigned)((series_len + 255) / 256)), dim3(256), 0, d_high,
+                  d_low, d_close, (int)series_len, (int)first_valid, d_tr_out);
+}
+
+vta_status vta_atr_batch_device_with_tr(vta_ctx *ctx, const float *d_tr, const int32_t *d_periods,
+                                        int32_t max_period, int32_t series_len,
+                                        int32_t first_valid, int32_t n_combos, float *d_out) {
+    return batch_device(ctx, IND_ATR_TR, d_tr, nullptr, nullptr, d_periods, max_period,
+                        series_len, first_valid, n_combos, d_out);
+}
 
 // ------------------------------------------------------------------ SMMA, Wilders
 #define VTA_ONE_INPUT_REC(NAME, IND)                                                              \

@@ -110,6 +110,24 @@ class CudaAtr(_OneParam):
                                                  period):
         return self._ms_dev([high_tm, low_tm, close_tm], cols, rows, period)
 
+    def tr_from_hlc_device(self, d_high: DeviceBuffer, d_low: DeviceBuffer,
+                           d_close: DeviceBuffer, series_len: int, first_valid: int,
+                           d_tr_out: DeviceBuffer) -> None:
+        """atr_wrapper.rs:566: enqueue-only true range of one device-resident series."""
+        check(lib().vta_tr_from_hlc_device(self.ctx.handle, ctypes.c_void_p(d_high.ptr),
+                                           ctypes.c_void_p(d_low.ptr), ctypes.c_void_p(d_close.ptr),
+                                           int(series_len), int(first_valid),
+                                           ctypes.c_void_p(d_tr_out.ptr)))
+
+    def atr_batch_device_with_tr(self, d_tr: DeviceBuffer, d_periods: DeviceBuffer,
+                                 max_period: int, series_len: int, first_valid: int,
+                                 n_combos: int, d_out: DeviceBuffer) -> None:
+        """atr_wrapper.rs:626: enqueue-only ATR of every period over a TR series."""
+        check(lib().vta_atr_batch_device_with_tr(self.ctx.handle, ctypes.c_void_p(d_tr.ptr),
+                                                 ctypes.c_void_p(d_periods.ptr), int(max_period),
+                                                 int(series_len), int(first_valid),
+                                                 int(n_combos), ctypes.c_void_p(d_out.ptr)))
+
 
 class CudaAdx(_OneParam):
     _name = "adx"
