@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 
 import vectorta_amd as vta  # noqa: E402
 from vectorta_amd import datagen  # noqa: E402
+from vectorta_amd._lib import check, lib  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
@@ -311,6 +312,54 @@ def bench_moving_averages(steps, warmup, ev):
     return res
 
 
+def bench_recurrence_batches(steps, warmup, ev):
+    """EMA / RSI / ATR / ADX one series x 1,250 periods (2..=1251) over 200,000
+    bars (1.0 GB of outputs per launch), inputs resident, enqueue-only
+    `_batch_device` launches (the chunked scan; ADX its two-stage form).
+    Algorithmic bytes = outputs written + input bars read once (1 or 3 series)."""
+    rng = np.random.Generator(np.random.PCG64(12))
+    c = datagen.alma_sweep_series()
+    _, h, lo, c = datagen.ohlc_from_close(c, rng)
+    T = c.size
+    periods = np.arange(2, 1252, dtype=np.int32)
+    C = periods.size
+    ops = {"ema": vta.CudaEma(0), "rsi": vta.CudaRsi(0), "atr": vta.CudaAtr(0),
+           "adx": vta.CudaAdx(0)}
+    ctx = ops["ema"].ctx
+    d_c, d_h, d_l = ctx.to_device(c), ctx.to_device(h), ctx.to_device(lo)
+    d_p = ctx.to_device(periods)
+    d_o = ctx.alloc(C * T * 4)
+    L = lib()
+    P = ctypes.c_void_p
+    fns = {
+        "ema": lambda: L.vta_ema_batch_device(ops["ema"].ctx.handle, P(d_c.ptr), P(d_p.ptr), None,
+                                              int(periods.max()), T, 0, C, P(d_o.ptr)),
+        "rsi": lambda: L.vta_rsi_batch_device(ops["rsi"].ctx.handle, P(d_c.ptr), P(d_p.ptr),
+                                              int(periods.max()), T, 0, C, P(d_o.ptr)),
+        "atr": lambda: L.vta_atr_batch_device(ops["atr"].ctx.handle, P(d_h.ptr), P(d_l.ptr),
+                                              P(d_c.ptr), P(d_p.ptr), int(periods.max()), T, 0,
+                                              C, P(d_o.ptr)),
+        "adx": lambda: L.vta_adx_batch_device(ops["adx"].ctx.handle, P(d_h.ptr), P(d_l.ptr),
+                                              P(d_c.ptr), P(d_p.ptr), int(periods.max()), T, 0,
+                                              C, P(d_o.ptr)),
+    }
+    res = {"workload": "recurrence_batches_200k_x_1250"}
+    for name, fn in fns.items():
+        op = ops[name]
+        mean, ks, _ = _time_launches(lambda fn=fn: check(fn()), op.ctx.stream, steps, warmup, ev,
+                                     sync=op.synchronize)
+        nin = 3 if name in ("atr", "adx") else 1
+        alg = 4 * (C * T + nin * T)
+        res[name] = {"combos": C, "series_len": T, "kernel_ms_mean": mean,
+                     "kernel_ms_p50": float(np.percentile(ks, 50)),
+                     "outputs_per_s": C * T / (mean * 1e-3), "algorithmic_bytes": alg,
+                     "achieved_GBs": alg / (mean * 1e-3) / 1e9,
+                     "frac_of_hbm_peak": alg / (mean * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    for b in (d_c, d_h, d_l, d_p, d_o):
+        b.free()
+    return res
+
+
 def bench_double_alma(steps, warmup, ev):
     """Double-ALMA crossover backtest reduction: 220 fast x 265 slow ALMA periods
     (58,300 pairs) over the 200,000-bar sweep series, prices resident.  A step is
@@ -451,7 +500,7 @@ def main():
                     help="skip the secondary workloads (many-series OHLC, rolling windows, "
                          "single ALMA)")
     ap.add_argument("--mode", default="sweep",
-                    choices=["sweep", "calib", "ohlc", "rolling", "cross", "serve", "ma"],
+                    choices=["sweep", "calib", "ohlc", "rolling", "cross", "serve", "ma", "rec"],
                     help="calib: 256 MiB device-to-device copies, the known-size reference "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
@@ -472,6 +521,10 @@ def main():
         return
     if args.mode == "cross":
         r = bench_double_alma(args.steps, args.warmup, HipEvents())
+        print(json.dumps(r))
+        return
+    if args.mode == "rec":
+        r = bench_recurrence_batches(args.steps, args.warmup, HipEvents())
         print(json.dumps(r))
         return
     if args.mode == "ma":
@@ -573,6 +626,7 @@ def main():
         for name, fn in (("many_series_ohlc", lambda: bench_many_series_ohlc(10, 2, ev)),
                          ("rolling_windows", lambda: bench_rolling(5, 1, ev)),
                          ("moving_averages", lambda: bench_moving_averages(5, 1, ev)),
+                         ("recurrence_batches", lambda: bench_recurrence_batches(5, 1, ev)),
                          ("alma_host_boundary", lambda: bench_alma_host_boundary()),
                          ("double_alma_crossover", lambda: bench_double_alma(5, 1, ev)),
                          ("serving", lambda: bench_serving()),

@@ -100,6 +100,17 @@ struct EmaEng {
     }
     __device__ double out_before_start(int t) const { return REC_NAN; }
     static constexpr int kScan = 1;                  // states carried by the chunked scan
+    static constexpr bool kTwoStage = false;
+    // steps t < seed_end() after the start are the warm-up accumulation only
+    __device__ int seed_end() const { return fv + p; }
+    __device__ double seed_step(int t, double x) {
+        if (fin(x)) {
+            vc += 1;
+            double v = (double)vc;
+            prev = ((v - 1.0) * prev + x) / v;
+        }
+        return prev;
+    }
     // the step for t >= burn_ok_from() (no warm-up branches)
     __device__ double steady(double x) {
         if (fin(x)) prev = fma(beta, prev, alpha * x);
@@ -165,6 +176,18 @@ struct RsiEng {
         return den == 0.0 ? 50.0 : rec_ratio(100.0 * ag, den);
     }
     static constexpr int kScan = 2;
+    static constexpr bool kTwoStage = false;
+    __device__ int seed_end() const { return fv + p; }
+    __device__ double seed_step(int t, double x) {
+        double d = x - px;
+        px = x;
+        if (!poisoned) {
+            if (!fin(d)) poisoned = true;
+            else if (d > 0.0) ag += d;
+            else if (d < 0.0) al -= d;
+        }
+        return REC_NAN;
+    }
     __device__ double steady(double x) {
         double d = x - px;
         px = x;
@@ -217,6 +240,13 @@ struct AtrEng {
         return rma;
     }
     static constexpr int kScan = 1;
+    static constexpr bool kTwoStage = false;
+    __device__ int seed_end() const { return fv + p - 1; }
+    __device__ double seed_step(int t, double h, double l, double c) {
+        sum += true_range(h, l, pc);
+        pc = c;
+        return REC_NAN;
+    }
     __device__ double steady(double h, double l, double c) {
         double tr = true_range(h, l, pc);
         pc = c;
@@ -259,6 +289,12 @@ struct AtrTrEng {
     }
     __device__ double out_before_start(int t) const { return REC_NAN; }
     static constexpr int kScan = 1;
+    static constexpr bool kTwoStage = false;
+    __device__ int seed_end() const { return fv + p - 1; }
+    __device__ double seed_step(int t, double tr) {
+        sum += tr;
+        return REC_NAN;
+    }
     __device__ double steady(double tr) {
         rma = fma(-alpha, rma, rma) + alpha * tr;
         return rma;
@@ -331,9 +367,33 @@ struct AdxEng {
         adx = (adx * pm1 + dx) * rp;
         return adx;
     }
-    // two coupled stages (the DX input of the second depends on the first's
-    // state): burn-in chunking only
-    static constexpr int kScan = 0;
+    // two stages: (atr, +DM, -DM) decay by omr; adx decays by pm1 * rp once
+    // the DX sequence is known, which needs the first stage's exact states
+    static constexpr int kScan = 4;
+    static constexpr bool kTwoStage = true;
+    __device__ int seed_end() const { return fv + p; }
+    __device__ double seed_step(int t, double ch, double cl, double cc) {
+        double hl = ch - cl;
+        double hpc = fabs(ch - pc);
+        double lpc = fabs(cl - pc);
+        double tr = fmax(fmax(hl, hpc), lpc);
+        double up = ch - ph;
+        double down = pl - cl;
+        ph = ch; pl = cl; pc = cc;
+        if (up > down && up > 0.0) pds += up;
+        if (down > up && down > 0.0) mds += down;
+        atr += tr;
+        return REC_NAN;
+    }
+    __device__ double decay2() const { return pm1 * rp; }
+    __device__ void begin_carry(double h, double l, double c, const double *st) {
+        atr = st[0]; pds = st[1]; mds = st[2]; adx = st[3];
+        dx_count = p;
+        ph = h; pl = l; pc = c;
+    }
+    __device__ void get_state(double *st) const {
+        st[0] = atr; st[1] = pds; st[2] = mds; st[3] = adx;
+    }
     __device__ double steady(double ch, double cl, double cc) {
         double hl = ch - cl;
         double hpc = fabs(ch - pc);
@@ -353,8 +413,6 @@ struct AdxEng {
         return adx;
     }
     __device__ double decay() const { return omr; }
-    __device__ void begin_carry(double h, double l, double c, const double *st) {}
-    __device__ void get_state(double *st) const {}
 };
 
 // SMMA (smma.rs:243-271): the mean of the first p bars at fv+p-1, then
@@ -388,6 +446,12 @@ struct SmmaEng {
     }
     __device__ double out_before_start(int t) const { return REC_NAN; }
     static constexpr int kScan = 1;
+    static constexpr bool kTwoStage = false;
+    __device__ int seed_end() const { return p == 1 ? fv : fv + p - 1; }
+    __device__ double seed_step(int t, double x) {
+        sum += x;
+        return REC_NAN;
+    }
     __device__ double steady(double x) {
         if (p == 1) { prev = x; return x; }
         prev = (prev * pm1 + x) * rp;
@@ -429,6 +493,9 @@ struct WildersEng {
     }
     __device__ double out_before_start(int t) const { return REC_NAN; }
     static constexpr int kScan = 1;
+    static constexpr bool kTwoStage = false;
+    __device__ int seed_end() const { return fv + p - 1; }
+    __device__ double seed_step(int t, double x) { return seed_bar(t - fv, x); }
     __device__ double steady(double x) {
         if (p == 1) { y = x; return x; }
         y = fma(x - y, al, y);
@@ -717,7 +784,10 @@ extern "C" __global__ __launch_bounds__(64) void ms_ema_rsi_atr_f32(rec_args a) 
 template <class E, int NIN>
 __device__ __forceinline__ void drive_batch(const rec_args &a, float *out) {
     __shared__ __align__(16) float xs[3][64];
-    __shared__ float ob[64][65];
+    // 64 x 65 output staging in dynamic LDS, launched only with the passes
+    // that write rows (a pass without stores keeps more waves per CU)
+    extern __shared__ float ob_dyn[];
+    float (*ob)[65] = reinterpret_cast<float (*)[65]>(ob_dyn);
     const int lane = threadIdx.x & 63;
     const int c = blockIdx.x * 64 + lane;
     const bool active = c < a.nlanes;
@@ -747,26 +817,49 @@ __device__ __forceinline__ void drive_batch(const rec_args &a, float *out) {
             for (int t = fv + 1; t <= min(fv + p, T - 1); ++t) e.seed_scan(t, (double)a.x[t]);
         }
     }
-    double st0[2] = {0.0, 0.0};
-    if (carried && a.phase == 3) {
-        const double *cs = a.carry + ((size_t)(k - 1) * a.nlanes + c) * (E::kScan > 0 ? E::kScan : 1);
-        st0[0] = cs[0];
-        if (E::kScan > 1) st0[1] = cs[1];
+    // carried start state: zeros (pass 1), the exact first-stage states with a
+    // zero second stage (pass 2 of a two-stage engine), or all exact (pass 3)
+    constexpr int S = E::kScan > 0 ? E::kScan : 1;
+    double st0[S];
+#pragma unroll
+    for (int i = 0; i < S; ++i) st0[i] = 0.0;
+    if (carried && a.phase >= 2) {
+        const double *cs = a.carry + ((size_t)(k - 1) * a.nlanes + c) * S;
+        const int n = (E::kTwoStage && a.phase == 2) ? S - 1 : S;
+#pragma unroll
+        for (int i = 0; i < S; ++i)
+            if (i < n) st0[i] = cs[i];
     }
-    const bool emit = a.phase != 1;
+    const bool emit = a.phase == 0 || a.phase == 3;
     const int t_begin = wave_min(min(start, c0)) & ~63;
+    // bars of the next block are loaded into registers while the current one
+    // is processed
+    float nx0, nx1 = 0.0f, nx2 = 0.0f;
+    {
+        const long t = min(t_begin + lane, T - 1);
+        nx0 = a.x[t];
+        if (NIN == 3) {
+            nx1 = a.h[t];
+            nx2 = a.l[t];
+        }
+    }
     for (int tb = t_begin; tb < c1; tb += 64) {
-        {
-            const long t = min(tb + lane, T - 1);
-            xs[0][lane] = a.x[t];
-            if (NIN == 3) {
-                xs[1][lane] = a.h[t];
-                xs[2][lane] = a.l[t];
-            }
+        xs[0][lane] = nx0;
+        if (NIN == 3) {
+            xs[1][lane] = nx1;
+            xs[2][lane] = nx2;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        {
+            const long t = min(tb + 64 + lane, T - 1);
+            nx0 = a.x[t];
+            if (NIN == 3) {
+                nx1 = a.h[t];
+                nx2 = a.l[t];
+            }
+        }
         // lean path: a full block in which every lane is past its start and
         // its warm-up -- only the engine's steady update, bars read four at a
         // time
@@ -788,6 +881,32 @@ __device__ __forceinline__ void drive_batch(const rec_args &a, float *out) {
                     y1 = e.steady(h4.y, l4.y, x4.y);
                     y2 = e.steady(h4.z, l4.z, x4.z);
                     y3 = e.steady(h4.w, l4.w, x4.w);
+                }
+                if (emit) {
+                    ob[lane][j] = (float)y0;
+                    ob[lane][j + 1] = (float)y1;
+                    ob[lane][j + 2] = (float)y2;
+                    ob[lane][j + 3] = (float)y3;
+                }
+            }
+        } else if (tb + 64 <= c1 && __all(!active || (start < tb && tb + 64 <= e.seed_end()))) {
+            // every lane still accumulating its warm-up seed for the whole block
+#pragma unroll 2
+            for (int j = 0; j < 64; j += 4) {
+                const float4 x4 = *reinterpret_cast<const float4 *>(&xs[0][j]);
+                double y0, y1, y2, y3;
+                if constexpr (NIN == 1) {
+                    y0 = e.seed_step(tb + j, x4.x);
+                    y1 = e.seed_step(tb + j + 1, x4.y);
+                    y2 = e.seed_step(tb + j + 2, x4.z);
+                    y3 = e.seed_step(tb + j + 3, x4.w);
+                } else {
+                    const float4 h4 = *reinterpret_cast<const float4 *>(&xs[1][j]);
+                    const float4 l4 = *reinterpret_cast<const float4 *>(&xs[2][j]);
+                    y0 = e.seed_step(tb + j, h4.x, l4.x, x4.x);
+                    y1 = e.seed_step(tb + j + 1, h4.y, l4.y, x4.y);
+                    y2 = e.seed_step(tb + j + 2, h4.z, l4.z, x4.z);
+                    y3 = e.seed_step(tb + j + 3, h4.w, l4.w, x4.w);
                 }
                 if (emit) {
                     ob[lane][j] = (float)y0;
@@ -837,12 +956,16 @@ __device__ __forceinline__ void drive_batch(const rec_args &a, float *out) {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     if constexpr (E::kScan > 0) {
-        if (a.phase == 1 && active && k + 1 < (int)gridDim.y) {
-            double st[2];
+        // chunk-end states: all of them after pass 1; after pass 2 (two-stage
+        // engines) only the second stage's, next to the folded first stage
+        if ((a.phase == 1 || a.phase == 2) && active && k + 1 < (int)gridDim.y) {
+            double st[S];
             e.get_state(st);
-            double *cs = a.carry + ((size_t)k * a.nlanes + c) * E::kScan;
-            cs[0] = st[0];
-            if (E::kScan > 1) cs[1] = st[1];
+            double *cs = a.carry + ((size_t)k * a.nlanes + c) * S;
+            const int lo = a.phase == 2 ? S - 1 : 0;
+#pragma unroll
+            for (int i = 0; i < S; ++i)
+                if (i >= lo) cs[i] = st[i];
         }
     }
 }
@@ -851,6 +974,8 @@ __device__ __forceinline__ void drive_batch(const rec_args &a, float *out) {
 // past the warm-up, s_end = decay^L s_prev + B (B = its zero-start run); a
 // chunk holding the warm-up ran exactly from first_valid, so s_end = B.
 // grid.x = groups of 64 combos; lane = combo.
+// Two-stage engines fold their first-stage states after pass 1 (a.phase = 1)
+// and their second-stage state after pass 2 (a.phase = 2).
 template <class E>
 __device__ __forceinline__ void scan_combine(const rec_args &a) {
     const int c = blockIdx.x * 64 + (threadIdx.x & 63);
@@ -858,17 +983,29 @@ __device__ __forceinline__ void scan_combine(const rec_args &a) {
     E e;
     e.setup(a.periods[c], a.fvs[0]);
     constexpr int S = E::kScan > 0 ? E::kScan : 1;
-    const double dl = pow(e.decay(), (double)a.chunk);
+    int lo = 0, hi = S;
+    double d = e.decay();
+    if constexpr (E::kTwoStage) {
+        if (a.phase == 2) {
+            lo = S - 1;
+            d = e.decay2();
+        } else {
+            hi = S - 1;
+        }
+    }
+    const double dl = pow(d, (double)a.chunk);
     const int nch = (a.T + a.chunk - 1) / a.chunk;
-    double y0 = 0.0, y1 = 0.0;
+    double y[S];
+#pragma unroll
+    for (int i = 0; i < S; ++i) y[i] = 0.0;
     for (int k = 0; k + 1 < nch; ++k) {
         double *cs = a.carry + ((size_t)k * a.nlanes + c) * S;
         const bool carried = k * a.chunk >= e.burn_ok_from();
-        y0 = carried ? fma(dl, y0, cs[0]) : cs[0];
-        cs[0] = y0;
-        if (S > 1) {
-            y1 = carried ? fma(dl, y1, cs[1]) : cs[1];
-            cs[1] = y1;
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+            if (i < lo || i >= hi) continue;
+            y[i] = carried ? fma(dl, y[i], cs[i]) : cs[i];
+            cs[i] = y[i];
         }
     }
 }
@@ -890,6 +1027,9 @@ extern "C" __global__ __launch_bounds__(64) void wilders_batch_f32(rec_args a) {
 }
 extern "C" __global__ __launch_bounds__(64) void adx_batch_f32(rec_args a) {
     drive_batch<AdxEng, 3>(a, a.out0);
+}
+extern "C" __global__ __launch_bounds__(64) void adx_scan_combine(rec_args a) {
+    scan_combine<AdxEng>(a);
 }
 extern "C" __global__ __launch_bounds__(64) void atr_tr_batch_f32(rec_args a) {
     drive_batch<AtrTrEng, 1>(a, a.out0);

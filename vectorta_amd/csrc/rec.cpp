@@ -147,31 +147,46 @@ bool host_clean(const float *x, const float *h, const float *l, size_t n, size_t
     return true;
 }
 
+// Output staging of the one-series batch kernels (64 x 65 floats of dynamic
+// LDS), needed by the passes that write rows (phase 0 and 3).
+constexpr unsigned kBatchStageBytes = 64 * 65 * sizeof(float);
+
 vta_status launch_rec(vta_ctx *ctx, const char *kname, rec_args_host &a, unsigned groups,
-                      unsigned chunks, unsigned block = 64) {
+                      unsigned chunks, unsigned block = 64, unsigned smem = 0) {
     hipFunction_t f;
     VTA_TRY(get_function(ctx, "rec", kname, &f));
-    return launch(ctx, f, dim3(groups, chunks, 1), dim3(block), 0, a);
+    return launch(ctx, f, dim3(groups, chunks, 1), dim3(block), smem, a);
 }
 
-// Chunked-scan launch for the single-stage recurrences: chunks sized for
-// ~4096 waves (at least 256 bars), as many as the context's carry buffer holds
-// for C combos; pass 1, the per-combo fold, pass 3.  Returns false (nothing
-// launched) when one chunk would cover the series.
+// Chunked-scan launch: chunks sized for ~2,048 waves, at least 256 bars and at
+// least the longest warm-up (a lane then runs it exactly over at most two
+// chunks), as many as the context's carry buffer holds for C combos; pass 1,
+// the per-combo fold (two rounds for ADX), pass 3.  Measured on 1,250 periods
+// x 200k bars (EMA): 1,024-bar chunks (~3,900 waves) 0.68 ms, 2,048 0.43 ms,
+// 4,096 0.59 ms.  Sets *done = false (nothing launched) when one chunk would
+// cover the series.
 const char *combine_kernel(Ind k) {
     static const char *n[] = {"ema_scan_combine",  "rsi_scan_combine",     "atr_scan_combine",
-                              nullptr,             "smma_scan_combine",    "wilders_scan_combine",
+                              "adx_scan_combine",  "smma_scan_combine",    "wilders_scan_combine",
                               "atr_tr_scan_combine"};
     return n[k];
 }
 
-vta_status scan_launch(vta_ctx *ctx, Ind k, rec_args_host a, size_t T, size_t C, bool *done) {
+vta_status scan_launch(vta_ctx *ctx, Ind k, rec_args_host a, size_t T, size_t C, size_t max_p,
+                       bool *done) {
     *done = false;
-    if (k == IND_ADX || getenv("VTA_REC_NO_SCAN")) return VTA_OK;
-    const size_t S = k == IND_RSI ? 2 : 1;
+    if (getenv("VTA_REC_NO_SCAN")) return VTA_OK;
+    // states per (chunk, combo): RSI 2; ADX 4 (three first-stage states and
+    // the ADX itself, folded in two rounds); the others 1
+    const size_t S = k == IND_RSI ? 2 : k == IND_ADX ? 4 : 1;
     const size_t groups = (C + 63) / 64;
-    const size_t want = std::max<size_t>(1, 4096 / groups);
-    size_t chunk = std::max<size_t>(256, (T + want - 1) / want);
+    const size_t want = std::max<size_t>(1, 2048 / groups);
+    const size_t warm = k == IND_ADX ? 2 * max_p : max_p;   // ADX warms up over 2p bars
+    size_t chunk = std::max<size_t>({256, (T + want - 1) / want, warm});
+    if (const char *e = getenv("VTA_SCAN_CHUNK")) {
+        long v = atol(e);
+        if (v >= 64) chunk = (size_t)v;
+    }
     const size_t max_chunks = vta_ctx::kCarryBytes / (C * S * sizeof(double));
     if (max_chunks < 2) return VTA_OK;
     if ((T + chunk - 1) / chunk > max_chunks) chunk = (T + max_chunks - 1) / max_chunks;
@@ -186,8 +201,15 @@ vta_status scan_launch(vta_ctx *ctx, Ind k, rec_args_host a, size_t T, size_t C,
     a.phase = 1;
     VTA_TRY(launch(ctx, fb, dim3((unsigned)groups, (unsigned)nch, 1), dim3(64), 0, a));
     VTA_TRY(launch(ctx, fc, dim3((unsigned)groups), dim3(64), 0, a));
+    if (k == IND_ADX) {
+        // second stage: DX from the exact first-stage states, ADX from zero
+        a.phase = 2;
+        VTA_TRY(launch(ctx, fb, dim3((unsigned)groups, (unsigned)nch, 1), dim3(64), 0, a));
+        VTA_TRY(launch(ctx, fc, dim3((unsigned)groups), dim3(64), 0, a));
+    }
     a.phase = 3;
-    VTA_TRY(launch(ctx, fb, dim3((unsigned)groups, (unsigned)nch, 1), dim3(64), 0, a));
+    VTA_TRY(launch(ctx, fb, dim3((unsigned)groups, (unsigned)nch, 1), dim3(64), kBatchStageBytes,
+                   a));
     *done = true;
     return VTA_OK;
 }
@@ -209,13 +231,13 @@ vta_status batch_launch(vta_ctx *ctx, Ind k, const float *d_x, const float *d_h,
     a.nlanes = (int)C;
     if (clean) {
         bool done;
-        VTA_TRY(scan_launch(ctx, k, a, T, C, &done));
+        VTA_TRY(scan_launch(ctx, k, a, T, C, max_p, &done));
         if (done) return VTA_OK;
     }
     unsigned groups = (unsigned)((C + 63) / 64);
     a.chunk = pick_chunk_batch(k, max_p, T, groups, clean);
     unsigned chunks = (unsigned)((T + a.chunk - 1) / a.chunk);
-    return launch_rec(ctx, batch_kernel(k), a, groups, chunks);
+    return launch_rec(ctx, batch_kernel(k), a, groups, chunks, 64, kBatchStageBytes);
 }
 
 vta_status batch_dev(vta_ctx *ctx, Ind k, const float *x, const float *h, const float *l,

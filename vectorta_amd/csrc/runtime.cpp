@@ -334,6 +334,7 @@ vta_status vta_ctx_preload(vta_ctx *ctx) {
         {"rec", "atr_scan_combine"},
         {"rec", "smma_scan_combine"},
         {"rec", "wilders_scan_combine"},
+        {"rec", "adx_scan_combine"},
         {"rec", "atr_tr_batch_f32"},
         {"rec", "atr_tr_scan_combine"},
         {"rec", "tr_from_hlc_f32"},
