@@ -547,25 +547,51 @@ __device__ __forceinline__ void drive_ms(const rec_args &a, int p, float *out) {
     int start = active ? lane_start(e, c0, clean, &burn) : a.T;
     if (start > a.T) start = a.T;
     const int t_begin = wave_min(min(start, c0));
-    for (int t = t_begin; t < c1; ++t) {
-        const size_t off = (size_t)t * N + s;
-        const long loff = (long)min(max(t, active ? min(start, c0) : a.T - 1), a.T - 1) * N +
-                          min(s, N - 1);
-        const double x = (double)a.x[loff];
-        double hh = 0.0, ll = 0.0;
-        if (NIN == 3) { hh = (double)a.h[loff]; ll = (double)a.l[loff]; }
-        double y = REC_NAN;
-        if (active) {
-            if (t == start) {
-                if constexpr (NIN == 1) {
-                    if (burn) e.begin_burn(x); else y = e.begin_exact(x);
-                } else {
-                    if (burn) e.begin_burn(hh, ll, x); else y = e.begin_exact(hh, ll, x);
-                }
-            } else if (t > start) {
-                if constexpr (NIN == 1) y = e.step(t, x); else y = e.step(t, hh, ll, x);
+    const int lo = active ? min(start, c0) : a.T - 1;
+    const int sc = min(s, N - 1);
+    // blocks of 8 bars: the block's loads (rows clamped into [lo, T-1]) are
+    // issued before its sequential updates; a block in which every lane is
+    // past its start and its warm-up takes the engine's steady update only
+    for (int tb = t_begin; tb < c1; tb += 8) {
+        double xv[8], hv[8], lv[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const long loff = (long)min(max(tb + k, lo), a.T - 1) * N + sc;
+            xv[k] = (double)a.x[loff];
+            if (NIN == 3) {
+                hv[k] = (double)a.h[loff];
+                lv[k] = (double)a.l[loff];
             }
-            if (t >= c0) out[off] = (float)y;
+        }
+        const bool lean = tb + 8 <= c1 && __all(!active || (start < tb && tb >= e.burn_ok_from()));
+        if (lean) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                double y;
+                if constexpr (NIN == 1) y = e.steady(xv[k]);
+                else y = e.steady(hv[k], lv[k], xv[k]);
+                if (active && tb + k >= c0) out[(size_t)(tb + k) * N + s] = (float)y;
+            }
+            continue;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int t = tb + k;
+            if (t >= c1) break;
+            double y = REC_NAN;
+            if (active) {
+                if (t == start) {
+                    if constexpr (NIN == 1) {
+                        if (burn) e.begin_burn(xv[k]); else y = e.begin_exact(xv[k]);
+                    } else {
+                        if (burn) e.begin_burn(hv[k], lv[k], xv[k]);
+                        else y = e.begin_exact(hv[k], lv[k], xv[k]);
+                    }
+                } else if (t > start) {
+                    if constexpr (NIN == 1) y = e.step(t, xv[k]); else y = e.step(t, hv[k], lv[k], xv[k]);
+                }
+                if (t >= c0) out[(size_t)t * N + s] = (float)y;
+            }
         }
     }
 }
