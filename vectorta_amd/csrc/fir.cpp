@@ -29,29 +29,24 @@ struct fir_args_host {
     int chunk;
     int period;
     int group;
-    int vec4;
+    int pad0_;
     int pad_;
 };
 static_assert(sizeof(fir_args_host) == 88, "fir_args layout");
 
 enum Fir { FIR_SMA = 0, FIR_WMA = 1, FIR_SWMA = 2 };
 
-// Combos per workgroup and the store form of the batch kernel (VTA_FIR_GROUP,
-// VTA_FIR_VEC4 override the defaults).  Measured on 1,250 periods x 200k bars:
-// 64 combos per workgroup amortise the tile's prefix scan best (SMA 0.24 ms
-// vs 0.30 ms at 16), and one 4-byte store per lane (lanes on consecutive bars)
-// beats lane groups of four bars with 16-byte stores by 5-15%, whose strided
-// LDS reads conflict.
+// Combos per workgroup of the batch kernel (VTA_FIR_GROUP overrides, 1..256).
+// Measured on 1,250 periods x 200k bars: 64 per workgroup amortise the tile's
+// prefix scan best (SMA 0.24 ms vs 0.30 ms at 16).  Lane groups of four bars
+// with 16-byte stores were 5-15% slower than one 4-byte store per lane (their
+// strided LDS reads conflict) and were dropped.
 unsigned fir_group() {
     if (const char *e = getenv("VTA_FIR_GROUP")) {
         long v = atol(e);
         if (v >= 1 && v <= 256) return (unsigned)v;
     }
     return 64;
-}
-int fir_vec4() {
-    if (const char *e = getenv("VTA_FIR_VEC4")) return atoi(e) != 0;
-    return 0;
 }
 
 // Grids: SMA sma.rs:705-755 (expand_grid_sma), WMA wma_wrapper.rs:324-350
@@ -110,7 +105,6 @@ vta_status batch_launch(vta_ctx *ctx, Fir k, const float *d_x, const int32_t *d_
     a.span_cap = (int)(tile + max_p + 1);
     const unsigned group = fir_group();
     a.group = (int)group;
-    a.vec4 = fir_vec4();
     unsigned gx = (unsigned)((T + tile - 1) / tile);
     unsigned gy = (unsigned)((C + group - 1) / group);
     if (gy > (unsigned)ctx->props.maxGridSize[1])

@@ -48,8 +48,8 @@ struct fir_args {
     int span_cap;             // batch: LDS doubles per prefix array, minus one
     int chunk;                // many-series: bars per chunk
     int period;               // many-series
-    int group;                // batch: combos per workgroup
-    int vec4;                 // batch: 1 = 16-byte stores of 4 consecutive bars per lane
+    int group;                // batch: combos per workgroup (<= FIR_GMAX)
+    int pad0_;
     int pad_;
 };
 
@@ -65,35 +65,8 @@ __device__ __forceinline__ void swma_ab(int p, int *a, int *b) {
     }
 }
 
-// Writes row[t] = f(t) for t in [t0, tend).  vec4 = 0: lane l writes bars
-// t0 + l + 256k (one 4-byte store per lane, consecutive across the wave).
-// vec4 = 1: lane groups of 4 consecutive bars aligned to 16 bytes of the row
-// (rows start at c*T floats), one float4 store per group inside the range,
-// single stores at the range's ragged ends.
-template <class F>
-__device__ __forceinline__ void fir_emit_row(float *row, size_t rb, int t0, int tend, int tid,
-                                             int vec4, F f) {
-    if (!vec4) {
-        for (int t = t0 + tid; t < tend; t += 256) row[t] = f(t);
-        return;
-    }
-    const int g0 = t0 - (int)((rb + (size_t)t0) & 3);
-    for (int g = g0 + 4 * tid; g < tend; g += 1024) {
-        float v[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int t = g + k;
-            v[k] = (t >= t0 && t < tend) ? f(t) : 0.0f;
-        }
-        if (g >= t0 && g + 3 < tend) {
-            *reinterpret_cast<float4 *>(row + g) = make_float4(v[0], v[1], v[2], v[3]);
-        } else {
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-                if (g + k >= t0 && g + k < tend) row[g + k] = v[k];
-        }
-    }
-}
+// Combos per workgroup at most (the host's group size).
+#define FIR_GMAX 256
 
 extern "C" __global__ __launch_bounds__(256) void fir_batch_f32(fir_args a) {
     extern __shared__ double lds[];
@@ -121,6 +94,20 @@ extern "C" __global__ __launch_bounds__(256) void fir_batch_f32(fir_args a) {
         for (int j = 0; j < nc; ++j)
             for (int t = t0 + tid; t < tend; t += 256) a.out[(size_t)(c0 + j) * T + t] = FIR_NAN_F;
         return;
+    }
+    // per-combo constants of the group (period, 1/normaliser, SWMA boxes)
+    __shared__ int g_p[FIR_GMAX], g_a[FIR_GMAX], g_b[FIR_GMAX];
+    __shared__ double g_inv[FIR_GMAX];
+    if (tid < nc) {
+        const int p = a.periods[c0 + tid];
+        g_p[tid] = p;
+        int wa = 1, wb = 1;
+        swma_ab(p, &wa, &wb);
+        g_a[tid] = wa;
+        g_b[tid] = wb;
+        g_inv[tid] = a.kind == 0 ? 1.0 / (double)p
+                     : a.kind == 1 ? 1.0 / ((double)p * ((double)p + 1.0) * 0.5)
+                                   : 1.0 / ((double)wa * (double)wb);
     }
     const int s0 = t0 - pg - 1;            // first staged bar (may be negative)
     const int span = tend - s0;            // staged bars [s0, tend), <= span_cap
@@ -179,50 +166,74 @@ extern "C" __global__ __launch_bounds__(256) void fir_batch_f32(fir_args a) {
     }
     const double *C = B;
 
-    // ---- outputs: each lane writes 4 consecutive bars of a row, one 16-byte
-    // store where the group lies inside the tile
+    // ---- outputs: lane l owns bars t0 + l + 256 m (m < 4); the prefix values
+    // at its bars are read once into registers and shared by every combo of
+    // the group, so a combo costs one LDS read (SMA), two (WMA) or three
+    // (SWMA) per output plus its constants once per four bars.  Each store
+    // covers 64 consecutive bars of one row.
+    int Lt[4];
+    bool in[4];
+    double At[4], Pt[4], Xt[4], Xp[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const int t = t0 + tid + 256 * m;
+        in[m] = t < tend;
+        Lt[m] = in[m] ? t - s0 + 1 : span;
+        At[m] = A[Lt[m]];
+        Pt[m] = a.kind == 0 ? 0.0 : B[Lt[m]];          // B (WMA) or C (SWMA)
+        Xt[m] = (a.kind == 2 && in[m]) ? (double)a.x[t] : 0.0;
+        Xp[m] = (a.kind == 2 && in[m] && t >= 1) ? (double)a.x[t - 1] : 0.0;
+    }
     for (int j = 0; j < nc; ++j) {
-        const int c = c0 + j;
-        const int p = a.periods[c];
-        const size_t rb = (size_t)c * T;
-        float *row = a.out + rb;
+        const int p = g_p[j];
+        const double inv = g_inv[j];
         const int warm = fv + p - 1;
+        float *row = a.out + (size_t)(c0 + j) * T;
         if (a.kind == 0) {
-            const double inv = 1.0 / (double)p;
-            fir_emit_row(row, rb, t0, tend, tid, a.vec4, [&](int t) -> float {
-                if (t < warm || t - p >= z) return FIR_NAN_F;
-                const int Lt = t - s0 + 1;
-                return (float)((A[Lt] - A[Lt - p]) * inv);
-            });
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int t = t0 + tid + 256 * m;
+                float v = FIR_NAN_F;
+                if (t >= warm && t - p < z) v = (float)((At[m] - A[Lt[m] - p]) * inv);
+                if (in[m]) row[t] = v;
+            }
         } else if (a.kind == 1) {
-            const double inv = 1.0 / ((double)p * ((double)p + 1.0) * 0.5);
-            fir_emit_row(row, rb, t0, tend, tid, a.vec4, [&](int t) -> float {
-                if (t < warm || t >= z) return FIR_NAN_F;
-                const int Lt = t - s0 + 1;
-                const int Lq = Lt - p;
-                const double sa = A[Lt] - A[Lq];
-                const double sb = B[Lt] - B[Lq];
-                return (float)((sb - (double)(Lq - 1) * sa) * inv);
-            });
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int t = t0 + tid + 256 * m;
+                float v = FIR_NAN_F;
+                if (t >= warm && t < z) {
+                    const int Lq = Lt[m] - p;
+                    const double sa = At[m] - A[Lq];
+                    const double sb = Pt[m] - B[Lq];
+                    v = (float)((sb - (double)(Lq - 1) * sa) * inv);
+                }
+                if (in[m]) row[t] = v;
+            }
         } else if (p <= 2) {
             // window-only forms (swma.rs:429-445): copy, or pair average
-            fir_emit_row(row, rb, t0, tend, tid, a.vec4, [&](int t) -> float {
-                if (t < warm) return FIR_NAN_F;
-                if (p == 1) return a.x[t];
-                return (float)(((double)a.x[t - 1] + (double)a.x[t]) * 0.5);
-            });
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int t = t0 + tid + 256 * m;
+                float v = FIR_NAN_F;
+                if (t >= warm) v = p == 1 ? (float)Xt[m] : (float)((Xp[m] + Xt[m]) * 0.5);
+                if (in[m]) row[t] = v;
+            }
         } else {
-            int wa, wb;
-            swma_ab(p, &wa, &wb);
-            const double inv = 1.0 / ((double)wa * (double)wb);
-            fir_emit_row(row, rb, t0, tend, tid, a.vec4, [&](int t) -> float {
-                if (t < warm || t - wa >= z) return FIR_NAN_F;
-                const int Lt = t - s0 + 1;
+            const int wa = g_a[j], wb = g_b[j];
+            const int ab = wa + wb;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int t = t0 + tid + 256 * m;
+                const int L = Lt[m];
+                const double c1 = C[L - wb], c2 = C[L - wa], c3 = C[L - ab];
+                float v = (float)(((Pt[m] - c1) - (c2 - c3)) * inv);
                 // inside [z, z+a) the running sums hold the sum of the
-                // non-finite bars in [z, t]
-                if (t >= z) return (float)(A[Lt] * inv);
-                return (float)(((C[Lt] - C[Lt - wb]) - (C[Lt - wa] - C[Lt - wa - wb])) * inv);
-            });
+                // non-finite bars in [z, t]; NaN from z+a on and in the warm-up
+                if (t >= z) v = t - wa < z ? (float)(At[m] * inv) : FIR_NAN_F;
+                if (t < warm) v = FIR_NAN_F;
+                if (in[m]) row[t] = v;
+            }
         }
     }
 }
