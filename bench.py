@@ -353,6 +353,9 @@ def bench_recurrence_batches(steps, warmup, ev):
         res[name] = {"combos": C, "series_len": T, "kernel_ms_mean": mean,
                      "kernel_ms_p50": float(np.percentile(ks, 50)),
                      "outputs_per_s": C * T / (mean * 1e-3), "algorithmic_bytes": alg,
+                     "traffic_output_pass": profiled_traffic(
+                         "recurrence_batches_200k_x1250_traffic.txt",
+                         f"{name}_batch_f32[pass3]"),
                      "achieved_GBs": alg / (mean * 1e-3) / 1e9,
                      "frac_of_hbm_peak": alg / (mean * 1e-3) / 1e9 / HBM_PEAK_GBS}
     for b in (d_c, d_h, d_l, d_p, d_o):
