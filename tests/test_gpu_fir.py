@@ -110,7 +110,8 @@ def test_many_series(ops, name):
     if name != "wilders":
         tm[2_500, 5] = np.nan
         tm[4_000, 6] = np.inf
-    for p in ((2, 9, 33) if name == "wma" else (1, 2, 9, 33)):
+    # SWMA: periods up to 80 take the LDS ring of recent bars, 120 the cache
+    for p in ((2, 9, 33, 120) if name == "wma" else (1, 2, 9, 33, 120)):
         g = getattr(ops[name], {"sma": "sma_multi_series_one_param_time_major_dev",
                                 "wma": "wma_multi_series_one_param_time_major_dev",
                                 "swma": "swma_multi_series_one_param_time_major_dev",

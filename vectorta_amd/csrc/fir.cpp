@@ -29,7 +29,7 @@ struct fir_args_host {
     int chunk;
     int period;
     int group;
-    int pad0_;
+    int ring;
     int pad_;
 };
 static_assert(sizeof(fir_args_host) == 88, "fir_args layout");
@@ -115,6 +115,20 @@ vta_status batch_launch(vta_ctx *ctx, Fir k, const float *d_x, const int32_t *d_
 // Many-series chunk: at least 4x the replayed look-back (p bars) and 256
 // bars, small enough for ~8192 waves (32 per CU: each wave's bars are a
 // dependent chain, so latency is hidden by many waves in flight).
+// Rows of the many-series LDS ring at most (24 KiB per wave); longer periods
+// read the lagged bars from global memory.  VTA_FIR_NO_RING=1 turns the ring
+// off.
+constexpr size_t kRingMaxRows = 96;
+constexpr size_t kSwmaMsBlock = 16;   // bars per block of fir_ms1p_swma_f32
+
+bool ring_disabled() {
+    static const bool off = [] {
+        const char *e = getenv("VTA_FIR_NO_RING");
+        return e && e[0] == '1';
+    }();
+    return off;
+}
+
 int pick_chunk(size_t p, size_t T, size_t groups) {
     size_t want = std::max<size_t>(1, 8192 / std::max<size_t>(groups, 1));
     size_t c = std::max<size_t>({4 * p, 256, (T + want - 1) / want});
@@ -125,7 +139,9 @@ int pick_chunk(size_t p, size_t T, size_t groups) {
 vta_status ms_launch(vta_ctx *ctx, Fir k, const float *d_x, const int32_t *d_fvs,
                      const int32_t *d_zs, size_t cols, size_t rows, size_t period, float *d_out) {
     hipFunction_t f;
-    VTA_TRY(get_function(ctx, "fir", "fir_ms1p_f32", &f));
+    static const char *const kNames[] = {"fir_ms1p_sma_f32", "fir_ms1p_wma_f32",
+                                         "fir_ms1p_swma_f32"};
+    VTA_TRY(get_function(ctx, "fir", kNames[k], &f));
     fir_args_host a{};
     a.x = d_x;
     a.fvs = d_fvs;
@@ -138,7 +154,16 @@ vta_status ms_launch(vta_ctx *ctx, Fir k, const float *d_x, const int32_t *d_fvs
     unsigned groups = (unsigned)((cols + 63) / 64);
     a.chunk = pick_chunk(period, rows, groups);
     unsigned chunks = (unsigned)((rows + a.chunk - 1) / a.chunk);
-    return launch(ctx, f, dim3(groups, chunks, 1), dim3(64), 0, a);
+    // SWMA's three lagged streams come from a per-wave LDS ring of the last
+    // bars (the longest lag is at most the period, plus the block being
+    // read).  SMA and WMA have one lagged stream each: re-reading it through
+    // the cache measured as fast as the ring at the occupancy the ring's LDS
+    // allows, so they keep the cache.
+    a.ring = (k == FIR_SWMA && period > 2 && period + kSwmaMsBlock <= kRingMaxRows)
+                 ? (int)(period + kSwmaMsBlock) : 0;
+    if (ring_disabled()) a.ring = 0;
+    const unsigned smem = (unsigned)a.ring * 64u * (unsigned)sizeof(float);
+    return launch(ctx, f, dim3(groups, chunks, 1), dim3(64), smem, a);
 }
 
 int32_t first_nonfinite(const float *x, size_t n, size_t fv, size_t ld) {

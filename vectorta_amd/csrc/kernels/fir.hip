@@ -18,11 +18,13 @@
 // whole series, so the differences lose ~eps * (tile + p)^k / p^k, far below
 // f32 rounding; the kernel is bound by the coalesced row stores.
 //
-// Many series x one period, time-major (fir_ms1p_f32): one lane per series,
+// Many series x one period, time-major (fir_ms1p_{sma,wma,swma}_f32): one lane per series,
 // one wave per (64 series, time chunk).  Each chunk replays the reference's
 // scalar running sums exactly from p bars before the chunk (the windows have
-// finite memory, so the state at the chunk start is the exact one), reading
-// the lagged bars again through the cache.
+// finite memory, so the state at the chunk start is the exact one).  The
+// lagged bars (at most p back) come either from a per-wave LDS ring of the
+// last ring rows (each lane reads only its own column, so no barrier) or,
+// with no ring, again from global memory through the cache.
 //
 // Non-finite input after first_valid (z = first such bar), as the reference's
 // CPU paths give it:
@@ -49,7 +51,7 @@ struct fir_args {
     int chunk;                // many-series: bars per chunk
     int period;               // many-series
     int group;                // batch: combos per workgroup (<= FIR_GMAX)
-    int pad0_;
+    int ring;                 // many-series: rows of the LDS ring (>= longest lag + 8), 0 = none
     int pad_;
 };
 
@@ -245,13 +247,17 @@ extern "C" __global__ __launch_bounds__(256) void fir_batch_f32(fir_args a) {
 // in blocks of 8: the block's loads (the new bars and the lagged bars that
 // leave the running sums, indices clamped into the matrix) are issued before
 // its sequential updates.
-extern "C" __global__ __launch_bounds__(64) void fir_ms1p_f32(fir_args a) {
+// FMS_B = bars per block (loads in flight per wave).
+template <int KIND, int FMS_B>
+__device__ __forceinline__ void fir_ms1p(const fir_args &a) {
+    extern __shared__ float ring[];       // [a.ring rows][64 lanes]
     const int lane = threadIdx.x & 63;
     const int s = blockIdx.x * 64 + lane;
     const bool active = s < a.nlanes;
     const int N = a.nlanes;
     const int T = a.T;
     const int p = a.period;
+    const int R = a.ring;
     const int sc = min(s, N - 1);
     const int c0 = blockIdx.y * a.chunk;
     const int c1 = min(c0 + a.chunk, T);
@@ -266,78 +272,116 @@ extern "C" __global__ __launch_bounds__(64) void fir_ms1p_f32(fir_args a) {
     for (int o = 32; o > 0; o >>= 1) tb = min(tb, __shfl_xor(tb, o));
     tb = __builtin_amdgcn_readfirstlane(tb);
     tb = min(tb, c0);
-    auto ld = [&](int t) -> double { return (double)xc[(size_t)min(max(t, 0), T - 1) * N]; };
+    auto ldf = [&](int t) -> float { return xc[(size_t)min(max(t, 0), T - 1) * N]; };
     auto put = [&](int t, float v) {
         if (active && t >= c0) oc[(size_t)t * N] = t >= warm ? v : FIR_NAN_F;
     };
 
-    if (a.kind == 0 || (a.kind == 2 && p <= 2)) {
+    // Blocks of FMS_B bars: stage() loads bars [t, t+FMS_B) into cur and the
+    // ring; lagged(o, lag) fills o[k] with bar t+k-lag.  Every lagged bar the
+    // sums use lies at or after r0 >= tb (the replay loaded it) and at most
+    // p <= R-FMS_B bars back, so the ring still holds it.
+    int slot = 0;                         // ring slot of bar t
+    int t = tb;
+    float cur[FMS_B];
+    auto stage = [&]() {
+#pragma unroll
+        for (int k = 0; k < FMS_B; ++k) cur[k] = ldf(t + k);
+        if (R) {
+#pragma unroll
+            for (int k = 0; k < FMS_B; ++k) {
+                int sl = slot + k;
+                sl -= sl >= R ? R : 0;
+                ring[sl * 64 + lane] = cur[k];
+            }
+        }
+    };
+    auto lagged = [&](float (&o)[FMS_B], int lag) {
+        if (R) {
+#pragma unroll
+            for (int k = 0; k < FMS_B; ++k) {
+                int sl = slot + k - lag;
+                sl += sl < 0 ? R : 0;
+                sl -= sl >= R ? R : 0;
+                o[k] = ring[sl * 64 + lane];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < FMS_B; ++k) o[k] = ldf(t + k - lag);
+        }
+    };
+    auto next = [&]() {
+        slot += FMS_B;
+        slot -= slot >= R ? R : 0;
+        t += FMS_B;
+    };
+
+    if (KIND == 0 || (KIND == 2 && p <= 2)) {
         const bool copy = p == 1;
-        const bool pair = a.kind == 2 && p == 2;
+        const bool pair = KIND == 2 && p == 2;
         const int lag = pair ? 1 : p;
         const double inv = 1.0 / (double)p;
         double sum = 0.0;
-        for (int t = tb; t < c1; t += 8) {
-            double xn[8], xo[8];
+        while (t < c1) {
+            stage();
+            float xo[FMS_B];
+            if (!copy) lagged(xo, lag);
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                xn[k] = ld(t + k);
-                xo[k] = ld(t + k - lag);
-            }
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
+            for (int k = 0; k < FMS_B; ++k) {
                 const int u = t + k;
                 if (u >= c1) break;
+                const double xn = (double)cur[k];
                 float v = FIR_NAN_F;
                 if (copy) {
-                    v = (float)xn[k];
+                    v = cur[k];
                 } else if (pair) {
-                    if (u >= 1) v = (float)((xo[k] + xn[k]) * 0.5);
+                    // stored only from warm = fv + 1 and c0 on, where u-1 >= r0
+                    if (u > r0) v = (float)(((double)xo[k] + xn) * 0.5);
                 } else if (u >= r0) {
                     // sma.rs:318-328: the first p bars summed, then sum += x[t] - x[t-p]
-                    if (u < r0 + p) sum += xn[k];
-                    else sum += xn[k] - xo[k];
+                    if (u < r0 + p) sum += xn;
+                    else sum += xn - (double)xo[k];
                     v = (float)(sum * inv);
                     if (u - p >= z) v = FIR_NAN_F;
                 }
                 put(u, v);
             }
+            next();
         }
-    } else if (a.kind == 1) {
+    } else if constexpr (KIND == 1) {
         // wma.rs:290-320: weights 1..p, running weighted sum and plain sum
         const double pf = (double)p;
         const double inv = 1.0 / (pf * (pf + 1.0) * 0.5);
         double s1 = 0.0, s2 = 0.0;
-        for (int t = tb; t < c1; t += 8) {
-            double xn[8], xo[8];
+        while (t < c1) {
+            stage();
+            float xo[FMS_B];
+            lagged(xo, p - 1);
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                xn[k] = ld(t + k);
-                xo[k] = ld(t + k - p + 1);
-            }
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
+            for (int k = 0; k < FMS_B; ++k) {
                 const int u = t + k;
                 if (u >= c1) break;
+                const double xn = (double)cur[k];
                 float v = FIR_NAN_F;
                 if (u >= r0) {
                     const int q = u - r0;
                     if (q < p - 1) {
-                        s2 += xn[k] * ((double)q + 1.0);
-                        s1 += xn[k];
+                        s2 += xn * ((double)q + 1.0);
+                        s1 += xn;
                     } else {
-                        s2 += xn[k] * pf;
-                        s1 += xn[k];
+                        s2 += xn * pf;
+                        s1 += xn;
                         v = (float)(s2 * inv);
                         s2 -= s1;
-                        s1 -= xo[k];
+                        s1 -= (double)xo[k];
                     }
                     if (u > z) v = FIR_NAN_F;
                 }
                 put(u, v);
             }
+            next();
         }
-    } else {
+    } else if constexpr (KIND == 2) {
         // swma.rs:406-476 with the ring of the last b box sums replaced by a
         // second running box sum lagged by b bars (same operations, same bits)
         int wa, wb;
@@ -345,39 +389,44 @@ extern "C" __global__ __launch_bounds__(64) void fir_ms1p_f32(fir_args a) {
         const double inv = 1.0 / ((double)wa * (double)wb);
         const int sa0 = r0 + wa - 1;                  // first full box
         double s1 = 0.0, s1b = 0.0, s2 = 0.0;
-        for (int t = tb; t < c1; t += 8) {
-            double x0[8], xa[8], xb[8], xab[8];
+        while (t < c1) {
+            stage();
+            float xa[FMS_B], xb[FMS_B], xab[FMS_B];
+            lagged(xa, wa - 1);
+            lagged(xb, wb);
+            lagged(xab, wb + wa - 1);
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                x0[k] = ld(t + k);
-                xa[k] = ld(t + k - wa + 1);
-                xb[k] = ld(t + k - wb);
-                xab[k] = ld(t + k - wb - wa + 1);
-            }
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
+            for (int k = 0; k < FMS_B; ++k) {
                 const int u = t + k;
                 if (u >= c1) break;
                 float v = FIR_NAN_F;
                 if (u >= r0) {
-                    s1 += x0[k];
+                    s1 += (double)cur[k];
                     double old = 0.0;
                     const int tl = u - wb;            // bar whose box leaves the sum
-                    if (tl >= r0) s1b += xb[k];
+                    if (tl >= r0) s1b += (double)xb[k];
                     if (tl >= sa0) old = s1b;
                     if (u >= sa0) {
                         s2 = s2 + (s1 - old);
                         v = (float)(s2 * inv);
-                        s1 -= xa[k];
+                        s1 -= (double)xa[k];
                     }
-                    if (tl >= sa0) s1b -= xab[k];
+                    if (tl >= sa0) s1b -= (double)xab[k];
                     if (u - wa >= z) v = FIR_NAN_F;
                 }
                 put(u, v);
             }
+            next();
         }
     }
 }
+
+// One kernel per kind, so each gets its own register allocation.
+// SMA / WMA read their one lagged stream through the cache, 8-bar blocks;
+// SWMA takes its three lagged streams from the ring, 16-bar blocks.
+extern "C" __global__ __launch_bounds__(64) void fir_ms1p_sma_f32(fir_args a) { fir_ms1p<0, 8>(a); }
+extern "C" __global__ __launch_bounds__(64) void fir_ms1p_wma_f32(fir_args a) { fir_ms1p<1, 8>(a); }
+extern "C" __global__ __launch_bounds__(64) void fir_ms1p_swma_f32(fir_args a) { fir_ms1p<2, 16>(a); }
 
 // Single series: the first bar at or after first_valid whose value is not
 // finite.  z[0] must hold T on entry (vta_fill_i32).

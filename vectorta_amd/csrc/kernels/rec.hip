@@ -101,6 +101,7 @@ struct EmaEng {
     __device__ double out_before_start(int t) const { return REC_NAN; }
     static constexpr int kScan = 1;                  // states carried by the chunked scan
     static constexpr bool kTwoStage = false;
+    static constexpr bool kAdvance = false;
     // steps t < seed_end() after the start are the warm-up accumulation only
     __device__ int seed_end() const { return fv + p; }
     __device__ double seed_step(int t, double x) {
@@ -177,6 +178,7 @@ struct RsiEng {
     }
     static constexpr int kScan = 2;
     static constexpr bool kTwoStage = false;
+    static constexpr bool kAdvance = true;
     __device__ int seed_end() const { return fv + p; }
     __device__ double seed_step(int t, double x) {
         double d = x - px;
@@ -187,6 +189,15 @@ struct RsiEng {
             else if (d < 0.0) al -= d;
         }
         return REC_NAN;
+    }
+    // pass 1 of the chunked scan: the state update without the output ratio
+    __device__ void advance(double x) {
+        double d = x - px;
+        px = x;
+        double g = d > 0.0 ? d : 0.0;
+        double lo = d < 0.0 ? -d : 0.0;
+        ag = fma(ag, beta, inv_p * g);
+        al = fma(al, beta, inv_p * lo);
     }
     __device__ double steady(double x) {
         double d = x - px;
@@ -241,6 +252,7 @@ struct AtrEng {
     }
     static constexpr int kScan = 1;
     static constexpr bool kTwoStage = false;
+    static constexpr bool kAdvance = false;
     __device__ int seed_end() const { return fv + p - 1; }
     __device__ double seed_step(int t, double h, double l, double c) {
         sum += true_range(h, l, pc);
@@ -290,6 +302,7 @@ struct AtrTrEng {
     __device__ double out_before_start(int t) const { return REC_NAN; }
     static constexpr int kScan = 1;
     static constexpr bool kTwoStage = false;
+    static constexpr bool kAdvance = false;
     __device__ int seed_end() const { return fv + p - 1; }
     __device__ double seed_step(int t, double tr) {
         sum += tr;
@@ -371,6 +384,7 @@ struct AdxEng {
     // the DX sequence is known, which needs the first stage's exact states
     static constexpr int kScan = 4;
     static constexpr bool kTwoStage = true;
+    static constexpr bool kAdvance = true;
     __device__ int seed_end() const { return fv + p; }
     __device__ double seed_step(int t, double ch, double cl, double cc) {
         double hl = ch - cl;
@@ -393,6 +407,22 @@ struct AdxEng {
     }
     __device__ void get_state(double *st) const {
         st[0] = atr; st[1] = pds; st[2] = mds; st[3] = adx;
+    }
+    // pass 1 of the chunked scan: the first stage only (the ADX stage is
+    // folded after pass 2)
+    __device__ void advance(double ch, double cl, double cc) {
+        double hl = ch - cl;
+        double hpc = fabs(ch - pc);
+        double lpc = fabs(cl - pc);
+        double tr = fmax(fmax(hl, hpc), lpc);
+        double up = ch - ph;
+        double down = pl - cl;
+        ph = ch; pl = cl; pc = cc;
+        double pdm = (up > down && up > 0.0) ? up : 0.0;
+        double mdm = (down > up && down > 0.0) ? down : 0.0;
+        atr = atr * omr + tr;
+        pds = pds * omr + pdm;
+        mds = mds * omr + mdm;
     }
     __device__ double steady(double ch, double cl, double cc) {
         double hl = ch - cl;
@@ -447,6 +477,7 @@ struct SmmaEng {
     __device__ double out_before_start(int t) const { return REC_NAN; }
     static constexpr int kScan = 1;
     static constexpr bool kTwoStage = false;
+    static constexpr bool kAdvance = false;
     __device__ int seed_end() const { return p == 1 ? fv : fv + p - 1; }
     __device__ double seed_step(int t, double x) {
         sum += x;
@@ -494,6 +525,7 @@ struct WildersEng {
     __device__ double out_before_start(int t) const { return REC_NAN; }
     static constexpr int kScan = 1;
     static constexpr bool kTwoStage = false;
+    static constexpr bool kAdvance = false;
     __device__ int seed_end() const { return fv + p - 1; }
     __device__ double seed_step(int t, double x) { return seed_bar(t - fv, x); }
     __device__ double steady(double x) {
@@ -890,7 +922,33 @@ __device__ __forceinline__ void drive_batch(const rec_args &a, float *out) {
         // its warm-up -- only the engine's steady update, bars read four at a
         // time
         const bool lean = tb + 64 <= c1 && __all(!active || (start < tb && tb >= e.burn_ok_from()));
-        if (lean) {
+        bool advanced = false;
+        if constexpr (E::kAdvance) {
+            if (lean && a.phase == 1) {
+                // pass 1: the state update only, no outputs
+#pragma unroll 2
+                for (int j = 0; j < 64; j += 4) {
+                    const float4 x4 = *reinterpret_cast<const float4 *>(&xs[0][j]);
+                    if constexpr (NIN == 1) {
+                        e.advance(x4.x);
+                        e.advance(x4.y);
+                        e.advance(x4.z);
+                        e.advance(x4.w);
+                    } else {
+                        const float4 h4 = *reinterpret_cast<const float4 *>(&xs[1][j]);
+                        const float4 l4 = *reinterpret_cast<const float4 *>(&xs[2][j]);
+                        e.advance(h4.x, l4.x, x4.x);
+                        e.advance(h4.y, l4.y, x4.y);
+                        e.advance(h4.z, l4.z, x4.z);
+                        e.advance(h4.w, l4.w, x4.w);
+                    }
+                }
+                advanced = true;
+            }
+        }
+        if (advanced) {
+            // block done: pass 1 stores nothing
+        } else if (lean) {
 #pragma unroll 2
             for (int j = 0; j < 64; j += 4) {
                 const float4 x4 = *reinterpret_cast<const float4 *>(&xs[0][j]);

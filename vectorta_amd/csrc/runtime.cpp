@@ -343,7 +343,9 @@ vta_status vta_ctx_preload(vta_ctx *ctx) {
         {"rec", "smma_ms1p_f32"},
         {"rec", "wilders_ms1p_f32"},
         {"fir", "fir_batch_f32"},
-        {"fir", "fir_ms1p_f32"},
+        {"fir", "fir_ms1p_sma_f32"},
+        {"fir", "fir_ms1p_wma_f32"},
+        {"fir", "fir_ms1p_swma_f32"},
         {"fir", "series_first_nonfinite"},
         {"rec", "ms_ema_rsi_atr_f32"},
         {"rec", "ms_prepare_meta"},
