@@ -60,6 +60,24 @@ def test_rsi_batch(ctxs, n, period):
     _check(g, O.rsi_batch(x.astype(np.float64), ps), "rsi")
 
 
+@pytest.mark.parametrize("bad", [None, np.nan])
+def test_adx_batch_close_periods(ctxs, bad):
+    """Waves of 64 consecutive periods (300..427): every lane of a wave is in
+    its DX seed (p < i < 2p-1) for whole 64-bar blocks, and past its
+    first-stage seed early in chunk 0, so the DX-seed and state-only blocks
+    run; with an interior NaN (poison) the exact path replays every chunk."""
+    n = 50_000 if bad is None else 12_000
+    h, lo, c = _hlc(n, 35, lead=5)
+    if bad is not None:
+        h[9_000] = bad
+    period = (300, 427, 1)
+    ps = ctxs["adx"].expand_grid(period)
+    assert len(ps) == 128
+    g = ctxs["adx"].adx_batch_dev(h, lo, c, period).to_host()
+    f = [a.astype(np.float64) for a in (h, lo, c)]
+    _check(g, O.adx_batch(*f, ps), "adx", bound=2e-5)
+
+
 @pytest.mark.parametrize("n,period", [(3_000, (1, 60, 11)), (300_000, (14, 250, 59))])
 def test_atr_batch(ctxs, n, period):
     h, lo, c = _hlc(n, 33, lead=3)

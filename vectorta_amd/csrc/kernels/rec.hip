@@ -190,6 +190,7 @@ struct RsiEng {
         }
         return REC_NAN;
     }
+    __device__ int advance_from() const { return burn_ok_from(); }
     // pass 1 of the chunked scan: the state update without the output ratio
     __device__ void advance(double x) {
         double d = x - px;
@@ -400,6 +401,29 @@ struct AdxEng {
         return REC_NAN;
     }
     __device__ double decay2() const { return pm1 * rp; }
+    // pass 1 needs the first stage only, which is a plain smoothing from
+    // i = p + 1 on
+    __device__ int advance_from() const { return fv + p + 1; }
+    // DX seed (adx.rs:369-409): bars fv+p < t < fv+2p-1 smooth the first
+    // stage and add DX to the seed sum; the ADX itself is not yet defined
+    __device__ int dx_seed_end() const { return fv + 2 * p - 1; }
+    __device__ void dx_seed_step(double ch, double cl, double cc) {
+        double hl = ch - cl;
+        double hpc = fabs(ch - pc);
+        double lpc = fabs(cl - pc);
+        double tr = fmax(fmax(hl, hpc), lpc);
+        double up = ch - ph;
+        double down = pl - cl;
+        ph = ch; pl = cl; pc = cc;
+        double pdm = (up > down && up > 0.0) ? up : 0.0;
+        double mdm = (down > up && down > 0.0) ? down : 0.0;
+        atr = atr * omr + tr;
+        pds = pds * omr + pdm;
+        mds = mds * omr + mdm;
+        const double sdm = pds + mds;
+        double dx = (atr != 0.0 && sdm != 0.0) ? rec_ratio(100.0 * fabs(pds - mds), sdm) : 0.0;
+        dx_sum += dx;
+    }
     __device__ void begin_carry(double h, double l, double c, const double *st) {
         atr = st[0]; pds = st[1]; mds = st[2]; adx = st[3];
         dx_count = p;
@@ -924,7 +948,8 @@ __device__ __forceinline__ void drive_batch(const rec_args &a, float *out) {
         const bool lean = tb + 64 <= c1 && __all(!active || (start < tb && tb >= e.burn_ok_from()));
         bool advanced = false;
         if constexpr (E::kAdvance) {
-            if (lean && a.phase == 1) {
+            if (a.phase == 1 && tb + 64 <= c1 &&
+                __all(!active || (start < tb && tb >= e.advance_from()))) {
                 // pass 1: the state update only, no outputs
 #pragma unroll 2
                 for (int j = 0; j < 64; j += 4) {
@@ -946,8 +971,33 @@ __device__ __forceinline__ void drive_batch(const rec_args &a, float *out) {
                 advanced = true;
             }
         }
+        if constexpr (E::kTwoStage) {
+            // ADX's DX seed: every lane between its first-stage seed and its
+            // first ADX for the whole block (outputs NaN)
+            if (!advanced && tb + 64 <= c1 &&
+                __all(!active || (start < tb && tb > e.fv + e.p && tb + 64 <= e.dx_seed_end()))) {
+#pragma unroll 2
+                for (int j = 0; j < 64; j += 4) {
+                    const float4 x4 = *reinterpret_cast<const float4 *>(&xs[0][j]);
+                    const float4 h4 = *reinterpret_cast<const float4 *>(&xs[1][j]);
+                    const float4 l4 = *reinterpret_cast<const float4 *>(&xs[2][j]);
+                    e.dx_seed_step(h4.x, l4.x, x4.x);
+                    e.dx_seed_step(h4.y, l4.y, x4.y);
+                    e.dx_seed_step(h4.z, l4.z, x4.z);
+                    e.dx_seed_step(h4.w, l4.w, x4.w);
+                    if (emit) {
+                        ob[lane][j] = (float)REC_NAN;
+                        ob[lane][j + 1] = (float)REC_NAN;
+                        ob[lane][j + 2] = (float)REC_NAN;
+                        ob[lane][j + 3] = (float)REC_NAN;
+                    }
+                }
+                e.dx_count += 64;
+                advanced = true;
+            }
+        }
         if (advanced) {
-            // block done: pass 1 stores nothing
+            // block done
         } else if (lean) {
 #pragma unroll 2
             for (int j = 0; j < 64; j += 4) {
