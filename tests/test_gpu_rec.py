@@ -78,6 +78,39 @@ def test_adx_batch_close_periods(ctxs, bad):
     _check(g, O.adx_batch(*f, ps), "adx", bound=2e-5)
 
 
+@pytest.mark.parametrize("name", ["ema", "rsi", "atr", "adx", "smma", "wilders"])
+def test_scan_short_chunks_long_lead(ctxs, monkeypatch, name):
+    """Chunked scan with 256-bar chunks over a series whose first 3,000 bars
+    are NaN: eleven whole chunks end before first_valid (their chunk-end
+    states are undefined and must not reach the fold), and the warm-ups of
+    periods up to 1,200 (ADX: 2,400 bars) span several chunks, so carried
+    chunks start inside the seed sums, at the step that takes their mean and
+    inside ADX's DX seed."""
+    monkeypatch.setenv("VTA_SCAN_CHUNK", "256")
+    h, lo, c = _hlc(40_000, 36, lead=3_000)
+    period = (1, 1_201, 100) if name in ("atr", "smma", "wilders") else (2, 1_202, 100)
+    if name in ("ema", "rsi", "atr", "adx"):
+        op = ctxs[name]
+        ps = op.expand_grid(period)
+        if name in ("atr", "adx"):
+            g = getattr(op, f"{name}_batch_dev")(h, lo, c, period).to_host()
+            ref = getattr(O, f"{name}_batch")(*[a.astype(np.float64) for a in (h, lo, c)], ps)
+        else:
+            g = getattr(op, f"{name}_batch_dev")(c, period).to_host()
+            ref = getattr(O, f"{name}_batch")(c.astype(np.float64), ps)
+        _check(g, ref, name, bound=2e-5 if name == "adx" else 1e-5)
+        return
+    op = vta.CudaSmma(0) if name == "smma" else vta.CudaWilders(0)
+    ps = op.expand_grid(period)
+    g = getattr(op, f"{name}_batch_dev")(c, period).to_host()
+    f = O.smma if name == "smma" else O.wilders
+    ref = np.stack([f(c.astype(np.float64), p) for p in ps])
+    nan_mask_equal(g, ref)
+    err = max_rel_err(g, ref, floor=0.0)
+    print(f"{name}: max rel err {err:.3e}")
+    assert err <= 1e-5, (name, err)
+
+
 @pytest.mark.parametrize("n,period", [(3_000, (1, 60, 11)), (300_000, (14, 250, 59))])
 def test_atr_batch(ctxs, n, period):
     h, lo, c = _hlc(n, 33, lead=3)

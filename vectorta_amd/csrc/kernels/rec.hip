@@ -25,17 +25,19 @@
 // poison ATR forever), so each stream carries a "clean" flag; an unclean
 // stream always starts exactly at first_valid.
 //
-// One-series batches of the single-stage recurrences (EMA, RSI, ATR, SMMA,
-// Wilders) use an exact chunked scan instead when the series is clean: after
-// the warm-up every step is affine in the state, s_t = beta s_{t-1} + f(x_t),
-// so a chunk of L bars maps its start state s to beta^L s + B, where B is the
-// chunk run from a zero state.  Pass 1 (phase 1) computes B for every (chunk,
-// combo) -- chunks that contain the warm-up run exactly from first_valid
-// instead --, a short per-combo pass (rec_scan_combine) folds the chunks in
-// order into the exact state at every chunk end, and pass 3 (phase 3) runs
-// every chunk again from its exact start state and writes the outputs.  The
-// chunk length no longer has to cover a burn-in of ~25 / -ln(beta) bars, so
-// long periods keep thousands of waves in flight.
+// One-series batches (EMA, RSI, ATR, ADX, SMMA, Wilders) use an exact chunked
+// scan instead when the series is clean: every step after first_valid, the
+// warm-up included, is affine in the state, s_t = a_t s_{t-1} + f_t (a seed
+// sum has a_t = 1, the step that takes its mean 1/p, the smoothing beta), so
+// a chunk maps its start state s to g s + B, where g is the product of its
+// a_t and B is the chunk run from a zero state.  Pass 1 (phase 1) computes B
+// for every (chunk, combo) -- the chunk holding first_valid runs exactly from
+// there --, a per-combo pass (*_scan_combine) folds the chunks in order into
+// the exact state at every chunk end, and pass 3 (phase 3) runs every chunk
+// again from its exact start state and writes the outputs.  No lane replays
+// bars outside its chunk, so the chunk length is chosen for waves in flight
+// alone (Wilders, whose seed is summed in groups of four, starts carried
+// chunks only past its warm-up).
 #include "vta_common.h"
 
 struct rec_args {
@@ -49,7 +51,7 @@ struct rec_args {
     float *out0;
     float *out1;
     float *out2;
-    double *carry;           // batch scan: [chunk][combo][states] chunk-end states
+    double *carry;           // batch scan: [combo][chunk][states] chunk-end states
     int T;                   // bars
     int nlanes;              // combos (batch) or series (many-series)
     int chunk;               // bars per chunk
@@ -69,7 +71,21 @@ __device__ __forceinline__ int burn_bars(double beta, double nats) {
     return max(1, (int)ceil(k));
 }
 
+// Bars t of [c0, c1) with t >= from.
+__device__ __forceinline__ int bars_from(int c0, int c1, int from) {
+    return max(0, c1 - max(c0, from));
+}
+
 // ---------------------------------------------------------------- engines
+//
+// Chunked-scan interface (one-series batches): gain(c0, c1) is the product of
+// the per-bar multipliers of the state over bars [c0, c1) (every step, warm-up
+// included, is affine in the state: s_t = a_t s_{t-1} + f_t), begin_carry(t0,
+// ...) restarts the engine after bar t0 from a carried state and
+// get_state(t1, ...) reads it after bar t1; the state is whichever quantity
+// the step carries at that bar (a seed sum during the warm-up, the average
+// after).  kWarmScan engines accept a carried start anywhere after
+// first_valid; the others only past the warm-up.
 
 struct EmaEng {
     static constexpr bool kSeedScan = false;
@@ -117,9 +133,20 @@ struct EmaEng {
         if (fin(x)) prev = fma(beta, prev, alpha * x);
         return prev;
     }
-    __device__ double decay() const { return beta; }
-    __device__ void begin_carry(double x, const double *st) { prev = st[0]; vc = p; }
-    __device__ void get_state(double *st) const { st[0] = prev; }
+    // running mean: (v-1)/v per bar with v = t - fv + 1 (the product
+    // telescopes), then beta
+    static constexpr bool kWarmScan = true;
+    __device__ double gain(int c0, int c1) const {
+        const int m = min(c1, fv + p);
+        const double g = c0 < m ? (double)(c0 - fv) / (double)(m - fv) : 1.0;
+        const int n = bars_from(c0, c1, fv + p);
+        return n > 0 ? g * pow(beta, (double)n) : g;
+    }
+    __device__ void begin_carry(int t0, double x, const double *st) {
+        prev = st[0];
+        vc = t0 - fv + 1;
+    }
+    __device__ void get_state(int t1, double *st) const { st[0] = prev; }
 };
 
 // num / den for den > 0: an f32 reciprocal estimate refined by one f64 Newton
@@ -211,9 +238,20 @@ struct RsiEng {
         double den = ag + al;
         return den == 0.0 ? 50.0 : rec_ratio(100.0 * ag, den);
     }
-    __device__ double decay() const { return beta; }
-    __device__ void begin_carry(double x, const double *st) { px = x; ag = st[0]; al = st[1]; }
-    __device__ void get_state(double *st) const { st[0] = ag; st[1] = al; }
+    // seed sums (multiplier 1), times 1/p at fv+p, then beta
+    static constexpr bool kWarmScan = true;
+    __device__ double gain(int c0, int c1) const {
+        const int ws = fv + p;
+        const double g = (c0 <= ws && ws < c1) ? inv_p : 1.0;
+        const int n = bars_from(c0, c1, ws + 1);
+        return n > 0 ? g * pow(beta, (double)n) : g;
+    }
+    __device__ void begin_carry(int t0, double x, const double *st) {
+        px = x;
+        ag = st[0];
+        al = st[1];
+    }
+    __device__ void get_state(int t1, double *st) const { st[0] = ag; st[1] = al; }
 };
 
 __device__ __forceinline__ double true_range(double hi, double lo, double pc) {
@@ -266,12 +304,20 @@ struct AtrEng {
         rma = fma(-alpha, rma, rma) + alpha * tr;
         return rma;
     }
-    __device__ double decay() const { return 1.0 - alpha; }
-    __device__ void begin_carry(double h, double l, double c, const double *st) {
-        rma = st[0];
+    // seed sum up to fv+p-1, where the mean is taken (1/p), then 1 - alpha
+    static constexpr bool kWarmScan = true;
+    __device__ double gain(int c0, int c1) const {
+        const int ws = fv + p - 1;
+        const double g = (c0 <= ws && ws < c1) ? 1.0 / (double)p : 1.0;
+        const int n = bars_from(c0, c1, ws + 1);
+        return n > 0 ? g * pow(1.0 - alpha, (double)n) : g;
+    }
+    __device__ void begin_carry(int t0, double h, double l, double c, const double *st) {
+        if (t0 < fv + p - 1) sum = st[0];
+        else rma = st[0];
         pc = c;
     }
-    __device__ void get_state(double *st) const { st[0] = rma; }
+    __device__ void get_state(int t1, double *st) const { st[0] = t1 < fv + p - 1 ? sum : rma; }
 };
 
 // ATR fed by a precomputed true-range series (atr_wrapper.rs:626-700,
@@ -313,9 +359,18 @@ struct AtrTrEng {
         rma = fma(-alpha, rma, rma) + alpha * tr;
         return rma;
     }
-    __device__ double decay() const { return 1.0 - alpha; }
-    __device__ void begin_carry(double tr, const double *st) { rma = st[0]; }
-    __device__ void get_state(double *st) const { st[0] = rma; }
+    static constexpr bool kWarmScan = true;
+    __device__ double gain(int c0, int c1) const {
+        const int ws = fv + p - 1;
+        const double g = (c0 <= ws && ws < c1) ? 1.0 / (double)p : 1.0;
+        const int n = bars_from(c0, c1, ws + 1);
+        return n > 0 ? g * pow(1.0 - alpha, (double)n) : g;
+    }
+    __device__ void begin_carry(int t0, double tr, const double *st) {
+        if (t0 < fv + p - 1) sum = st[0];
+        else rma = st[0];
+    }
+    __device__ void get_state(int t1, double *st) const { st[0] = t1 < fv + p - 1 ? sum : rma; }
 };
 
 struct AdxEng {
@@ -400,7 +455,6 @@ struct AdxEng {
         atr += tr;
         return REC_NAN;
     }
-    __device__ double decay2() const { return pm1 * rp; }
     // pass 1 needs the first stage only, which is a plain smoothing from
     // i = p + 1 on
     __device__ int advance_from() const { return fv + p + 1; }
@@ -424,13 +478,39 @@ struct AdxEng {
         double dx = (atr != 0.0 && sdm != 0.0) ? rec_ratio(100.0 * fabs(pds - mds), sdm) : 0.0;
         dx_sum += dx;
     }
-    __device__ void begin_carry(double h, double l, double c, const double *st) {
-        atr = st[0]; pds = st[1]; mds = st[2]; adx = st[3];
-        dx_count = p;
+    // first stage: seed sums (multiplier 1) up to i = p, then omr; second
+    // stage: restarted at i = p (first DX), DX summed (1) up to i = 2p-2,
+    // the mean at i = 2p-1 (rp), then pm1 * rp
+    static constexpr bool kWarmScan = true;
+    __device__ double gain(int c0, int c1) const {
+        const int n = bars_from(c0, c1, fv + p + 1);
+        return n > 0 ? pow(omr, (double)n) : 1.0;
+    }
+    __device__ double gain2(int c0, int c1) const {
+        if (c0 <= fv + p) return 0.0;
+        const int w = fv + 2 * p - 1;
+        const double g = (c0 <= w && w < c1) ? rp : 1.0;
+        const int n = bars_from(c0, c1, w + 1);
+        return n > 0 ? g * pow(pm1 * rp, (double)n) : g;
+    }
+    // second-stage state after bar t: undefined before i = p, the DX sum up
+    // to i = 2p-2, the ADX from i = 2p-1 on
+    __device__ void begin_carry(int t0, double h, double l, double c, const double *st) {
+        atr = st[0]; pds = st[1]; mds = st[2];
+        const int i0 = t0 - fv;
+        if (i0 < p) {
+            dx_sum = 0.0; adx = 0.0; dx_count = 0;
+        } else if (i0 < 2 * p - 1) {
+            dx_sum = st[3]; adx = 0.0; dx_count = i0 - p + 1;
+        } else {
+            dx_sum = 0.0; adx = st[3]; dx_count = p;
+        }
         ph = h; pl = l; pc = c;
     }
-    __device__ void get_state(double *st) const {
-        st[0] = atr; st[1] = pds; st[2] = mds; st[3] = adx;
+    __device__ void get_state(int t1, double *st) const {
+        st[0] = atr; st[1] = pds; st[2] = mds;
+        const int i1 = t1 - fv;
+        st[3] = i1 < p ? 0.0 : (i1 < 2 * p - 1 ? dx_sum : adx);
     }
     // pass 1 of the chunked scan: the first stage only (the ADX stage is
     // folded after pass 2)
@@ -466,7 +546,6 @@ struct AdxEng {
         adx = (adx * pm1 + dx) * rp;
         return adx;
     }
-    __device__ double decay() const { return omr; }
 };
 
 // SMMA (smma.rs:243-271): the mean of the first p bars at fv+p-1, then
@@ -512,9 +591,22 @@ struct SmmaEng {
         prev = (prev * pm1 + x) * rp;
         return prev;
     }
-    __device__ double decay() const { return pm1 * rp; }
-    __device__ void begin_carry(double x, const double *st) { prev = st[0]; }
-    __device__ void get_state(double *st) const { st[0] = prev; }
+    // seed sum up to fv+p-1 (mean: 1/p), then pm1 * rp; p = 1 copies (0)
+    static constexpr bool kWarmScan = true;
+    __device__ double gain(int c0, int c1) const {
+        if (p == 1) return 0.0;
+        const int ws = fv + p - 1;
+        const double g = (c0 <= ws && ws < c1) ? 1.0 / (double)p : 1.0;
+        const int n = bars_from(c0, c1, ws + 1);
+        return n > 0 ? g * pow(pm1 * rp, (double)n) : g;
+    }
+    __device__ void begin_carry(int t0, double x, const double *st) {
+        if (p > 1 && t0 < fv + p - 1) sum = st[0];
+        else prev = st[0];
+    }
+    __device__ void get_state(int t1, double *st) const {
+        st[0] = (p > 1 && t1 < fv + p - 1) ? sum : prev;
+    }
 };
 
 // Wilders (wilders.rs:262-320): seed summed in groups of four,
@@ -557,9 +649,12 @@ struct WildersEng {
         y = fma(x - y, al, y);
         return y;
     }
-    __device__ double decay() const { return 1.0 - al; }
-    __device__ void begin_carry(double x, const double *st) { y = st[0]; }
-    __device__ void get_state(double *st) const { st[0] = y; }
+    // the seed is summed in groups of four (a two-word state), so a carried
+    // start is taken only past the warm-up: fma(x - y, al, y), i.e. 1 - al
+    static constexpr bool kWarmScan = false;
+    __device__ double gain(int c0, int c1) const { return pow(1.0 - al, (double)(c1 - c0)); }
+    __device__ void begin_carry(int t0, double x, const double *st) { y = st[0]; }
+    __device__ void get_state(int t1, double *st) const { st[0] = y; }
 };
 
 // ---------------------------------------------------------------- chunk planning
@@ -885,9 +980,11 @@ __device__ __forceinline__ void drive_batch(const rec_args &a, float *out) {
     bool burn = false, carried = false;
     int start;
     if (a.phase != 0) {
-        // chunked scan: chunks past the warm-up start from a carried state at
-        // c0 - 1; an unclean series runs every chunk exactly from first_valid
-        carried = active && clean && c0 >= e.burn_ok_from();
+        // chunked scan: chunks after first_valid (past the warm-up for engines
+        // without kWarmScan) start from a carried state at c0 - 1; an unclean
+        // series runs every chunk exactly from first_valid
+        carried = active && clean &&
+                  (E::kWarmScan ? c0 > e.exact_start() : c0 >= e.burn_ok_from());
         start = active ? (carried ? c0 - 1 : e.exact_start()) : T;
     } else {
         start = active ? lane_start(e, c0, clean, &burn) : T;
@@ -906,7 +1003,7 @@ __device__ __forceinline__ void drive_batch(const rec_args &a, float *out) {
 #pragma unroll
     for (int i = 0; i < S; ++i) st0[i] = 0.0;
     if (carried && a.phase >= 2) {
-        const double *cs = a.carry + ((size_t)(k - 1) * a.nlanes + c) * S;
+        const double *cs = a.carry + ((size_t)c * gridDim.y + (k - 1)) * S;
         const int n = (E::kTwoStage && a.phase == 2) ? S - 1 : S;
 #pragma unroll
         for (int i = 0; i < S; ++i)
@@ -1057,11 +1154,11 @@ __device__ __forceinline__ void drive_batch(const rec_args &a, float *out) {
                 const double x = xs[0][j];
                 if (t == start) {
                     if constexpr (NIN == 1) {
-                        if (carried) e.begin_carry(x, st0);
+                        if (carried) e.begin_carry(t, x, st0);
                         else if (burn) e.begin_burn(x);
                         else y = e.begin_exact(x);
                     } else {
-                        if (carried) e.begin_carry(xs[1][j], xs[2][j], x, st0);
+                        if (carried) e.begin_carry(t, xs[1][j], xs[2][j], x, st0);
                         else if (burn) e.begin_burn(xs[1][j], xs[2][j], x);
                         else y = e.begin_exact(xs[1][j], xs[2][j], x);
                     }
@@ -1094,8 +1191,8 @@ __device__ __forceinline__ void drive_batch(const rec_args &a, float *out) {
         // engines) only the second stage's, next to the folded first stage
         if ((a.phase == 1 || a.phase == 2) && active && k + 1 < (int)gridDim.y) {
             double st[S];
-            e.get_state(st);
-            double *cs = a.carry + ((size_t)k * a.nlanes + c) * S;
+            e.get_state(c1 - 1, st);
+            double *cs = a.carry + ((size_t)c * gridDim.y + k) * S;
             const int lo = a.phase == 2 ? S - 1 : 0;
 #pragma unroll
             for (int i = 0; i < S; ++i)
@@ -1104,42 +1201,73 @@ __device__ __forceinline__ void drive_batch(const rec_args &a, float *out) {
     }
 }
 
-// Folds the chunk-end states of pass 1 in chunk order, in place: for a chunk
-// past the warm-up, s_end = decay^L s_prev + B (B = its zero-start run); a
-// chunk holding the warm-up ran exactly from first_valid, so s_end = B.
-// grid.x = groups of 64 combos; lane = combo.
-// Two-stage engines fold their first-stage states after pass 1 (a.phase = 1)
-// and their second-stage state after pass 2 (a.phase = 2).
+// Folds the chunk-end states of pass 1 in chunk order, in place: a carried
+// chunk k maps the state at its start to s_k = g_k s_{k-1} + b_k, with b_k its
+// zero-start run and g_k = gain(c0, c1); a chunk that ran exactly from
+// first_valid has g_k = 0.  One wave per combo; the carry holds [combo]
+// [chunk][state].  64 chunks at a time: an inclusive scan of the affine maps
+// across the lanes, (g, b) o (g', b') = (g g', g b' + b), then the previous
+// row's last state applied.  Two-stage engines fold their first-stage states
+// after pass 1 (a.phase = 1) and their second-stage state after pass 2.
+// A chunk that ends before first_valid stores an undefined state; the chunk
+// after it always has g = 0, so a g = 0 step selects b instead of forming
+// 0 * s (which is NaN for an undefined s).
 template <class E>
 __device__ __forceinline__ void scan_combine(const rec_args &a) {
-    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int c = blockIdx.x;
+    const int lane = threadIdx.x & 63;
     if (c >= a.nlanes || a.clean == nullptr || a.clean[0] == 0) return;
     E e;
     e.setup(a.periods[c], a.fvs[0]);
     constexpr int S = E::kScan > 0 ? E::kScan : 1;
     int lo = 0, hi = S;
-    double d = e.decay();
+    bool second = false;
     if constexpr (E::kTwoStage) {
         if (a.phase == 2) {
             lo = S - 1;
-            d = e.decay2();
+            second = true;
         } else {
             hi = S - 1;
         }
     }
-    const double dl = pow(d, (double)a.chunk);
     const int nch = (a.T + a.chunk - 1) / a.chunk;
-    double y[S];
+    const int nst = nch - 1;                 // chunk-end states read by later chunks
+    double *base = a.carry + (size_t)c * nch * S;
+    double cin[S];
 #pragma unroll
-    for (int i = 0; i < S; ++i) y[i] = 0.0;
-    for (int k = 0; k + 1 < nch; ++k) {
-        double *cs = a.carry + ((size_t)k * a.nlanes + c) * S;
-        const bool carried = k * a.chunk >= e.burn_ok_from();
+    for (int i = 0; i < S; ++i) cin[i] = 0.0;
+    for (int r0 = 0; r0 < nst; r0 += 64) {
+        const int k = r0 + lane;
+        const bool ok = k < nst;
+        const int c0 = k * a.chunk;
+        const int c1 = min(c0 + a.chunk, a.T);
+        const bool carried =
+            ok && (E::kWarmScan ? c0 > e.exact_start() : c0 >= e.burn_ok_from());
+        double g = 0.0;
+        if (carried) {
+            if constexpr (E::kTwoStage) g = second ? e.gain2(c0, c1) : e.gain(c0, c1);
+            else g = e.gain(c0, c1);
+        }
+        double b[S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) b[i] = (ok && i >= lo && i < hi) ? base[(size_t)k * S + i] : 0.0;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const double gu = __shfl_up(g, o);
+            double bu[S];
+#pragma unroll
+            for (int i = 0; i < S; ++i) bu[i] = __shfl_up(b[i], o);
+            if (lane >= o) {
+#pragma unroll
+                for (int i = 0; i < S; ++i) b[i] = g != 0.0 ? fma(g, bu[i], b[i]) : b[i];
+                g *= gu;
+            }
+        }
 #pragma unroll
         for (int i = 0; i < S; ++i) {
-            if (i < lo || i >= hi) continue;
-            y[i] = carried ? fma(dl, y[i], cs[i]) : cs[i];
-            cs[i] = y[i];
+            const double y = g != 0.0 ? fma(g, cin[i], b[i]) : b[i];
+            if (ok && i >= lo && i < hi) base[(size_t)k * S + i] = y;
+            cin[i] = __shfl(y, 63);
         }
     }
 }

@@ -158,13 +158,14 @@ vta_status launch_rec(vta_ctx *ctx, const char *kname, rec_args_host &a, unsigne
     return launch(ctx, f, dim3(groups, chunks, 1), dim3(block), smem, a);
 }
 
-// Chunked-scan launch: chunks sized for ~2,048 waves, at least 256 bars and at
-// least the longest warm-up (a lane then runs it exactly over at most two
-// chunks), as many as the context's carry buffer holds for C combos; pass 1,
-// the per-combo fold (two rounds for ADX), pass 3.  Measured on 1,250 periods
-// x 200k bars (EMA): 1,024-bar chunks (~3,900 waves) 0.68 ms, 2,048 0.43 ms,
-// 4,096 0.59 ms.  Sets *done = false (nothing launched) when one chunk would
-// cover the series.
+// Chunked-scan launch: chunks sized for ~kScanWaves waves and at least 256
+// bars (Wilders: also at least the longest warm-up, which a lane then runs
+// exactly over at most two chunks), as many as the context's carry buffer
+// holds for C combos; pass 1, the per-combo fold (one wave per combo; two
+// rounds for ADX), pass 3.  Sets *done = false (nothing launched) when one
+// chunk would cover the series.
+constexpr size_t kScanWaves = 2048;
+
 const char *combine_kernel(Ind k) {
     static const char *n[] = {"ema_scan_combine",  "rsi_scan_combine",     "atr_scan_combine",
                               "adx_scan_combine",  "smma_scan_combine",    "wilders_scan_combine",
@@ -180,8 +181,13 @@ vta_status scan_launch(vta_ctx *ctx, Ind k, rec_args_host a, size_t T, size_t C,
     // the ADX itself, folded in two rounds); the others 1
     const size_t S = k == IND_RSI ? 2 : k == IND_ADX ? 4 : 1;
     const size_t groups = (C + 63) / 64;
-    const size_t want = std::max<size_t>(1, 2048 / groups);
-    const size_t warm = k == IND_ADX ? 2 * max_p : max_p;   // ADX warms up over 2p bars
+    size_t waves = kScanWaves;
+    if (const char *e = getenv("VTA_SCAN_WAVES")) {
+        long v = atol(e);
+        if (v >= 64) waves = (size_t)v;
+    }
+    const size_t want = std::max<size_t>(1, waves / groups);
+    const size_t warm = k == IND_WILDERS ? max_p : 0;
     size_t chunk = std::max<size_t>({256, (T + want - 1) / want, warm});
     if (const char *e = getenv("VTA_SCAN_CHUNK")) {
         long v = atol(e);
@@ -200,12 +206,12 @@ vta_status scan_launch(vta_ctx *ctx, Ind k, rec_args_host a, size_t T, size_t C,
     VTA_TRY(get_function(ctx, "rec", combine_kernel(k), &fc));
     a.phase = 1;
     VTA_TRY(launch(ctx, fb, dim3((unsigned)groups, (unsigned)nch, 1), dim3(64), 0, a));
-    VTA_TRY(launch(ctx, fc, dim3((unsigned)groups), dim3(64), 0, a));
+    VTA_TRY(launch(ctx, fc, dim3((unsigned)C), dim3(64), 0, a));
     if (k == IND_ADX) {
         // second stage: DX from the exact first-stage states, ADX from zero
         a.phase = 2;
         VTA_TRY(launch(ctx, fb, dim3((unsigned)groups, (unsigned)nch, 1), dim3(64), 0, a));
-        VTA_TRY(launch(ctx, fc, dim3((unsigned)groups), dim3(64), 0, a));
+        VTA_TRY(launch(ctx, fc, dim3((unsigned)C), dim3(64), 0, a));
     }
     a.phase = 3;
     VTA_TRY(launch(ctx, fb, dim3((unsigned)groups, (unsigned)nch, 1), dim3(64), kBatchStageBytes,
