@@ -231,6 +231,78 @@ def _time_launches(fn, stream, steps, warmup, ev, sync=None):
     return total / steps, np.array(ks), wall
 
 
+def bench_captured_batch(steps, warmup, ev):
+    """A batch of 16 small one-series requests on resident inputs (EMA, RSI,
+    ADX, SMA over 32 periods and Bollinger Bands over 4, on four 10,000-bar
+    OHLC series): the enqueue-only calls launched eagerly vs the same calls
+    captured once into a hipGraph and replayed.  Reported per batch: host
+    wall time of enqueue + synchronise, and GPU time between two HIP events."""
+    L = lib()
+    P = ctypes.c_void_p
+    T, fv, ns = 10_000, 0, 4
+    ctx = vta.HipContext(0)
+    rng = np.random.Generator(np.random.PCG64(77))
+    per = np.arange(4, 132, 4, dtype=np.int32)                 # 32 periods
+    bper = np.array([10, 20, 30, 40], dtype=np.int32)
+    d_p, d_bp = ctx.to_device(per), ctx.to_device(bper)
+    d_bu, d_bd = ctx.to_device(np.full(4, 2.0)), ctx.to_device(np.full(4, 2.0))
+    series, outs = [], []
+    for s in range(ns):
+        c = datagen.gbm_close(T, 0.01, seed=100 + s)
+        _, h, lo, c = datagen.ohlc_from_close(c, rng)
+        series.append((ctx.to_device(c), ctx.to_device(h), ctx.to_device(lo)))
+        outs.append([ctx.alloc(per.size * T * 4) for _ in range(4)] +
+                    [ctx.alloc(bper.size * T * 4) for _ in range(3)])
+    C, BC, mp = int(per.size), int(bper.size), int(per.max())
+
+    def batch():
+        h = ctx.handle
+        for (dc, dh, dl), o in zip(series, outs):
+            check(L.vta_ema_batch_device(h, P(dc.ptr), P(d_p.ptr), None, mp, T, fv, C, P(o[0].ptr)))
+            check(L.vta_rsi_batch_device(h, P(dc.ptr), P(d_p.ptr), mp, T, fv, C, P(o[1].ptr)))
+            check(L.vta_adx_batch_device(h, P(dh.ptr), P(dl.ptr), P(dc.ptr), P(d_p.ptr), mp, T,
+                                         fv, C, P(o[2].ptr)))
+            check(L.vta_sma_batch_device(h, P(dc.ptr), P(d_p.ptr), T, C, fv, mp, P(o[3].ptr)))
+            check(L.vta_bollinger_bands_batch_device(
+                h, P(dc.ptr), T, fv, T, P(d_bp.ptr), P(d_bu.ptr), P(d_bd.ptr), BC,
+                int(bper.max()), P(o[4].ptr), P(o[5].ptr), P(o[6].ptr)))
+
+    ctx.preload()
+    ctx.begin_capture()
+    batch()
+    graph = ctx.end_capture()
+    res = {"workload": "captured_batch_16_requests_10k_bars", "requests": 4 * ns,
+           "graph_nodes": graph.num_nodes,
+           "outputs_per_batch": ns * T * (4 * C + 3 * BC)}
+    for name, fn in (("eager", batch), ("graph", graph.launch)):
+        for _ in range(warmup):
+            fn()
+        ctx.synchronize()
+        walls = []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            fn()
+            ctx.synchronize()
+            walls.append((time.perf_counter() - t0) * 1e3)
+        a, b = ev.create(), ev.create()
+        ev.record(a, ctx.stream)
+        for _ in range(steps):
+            fn()
+        ev.record(b, ctx.stream)
+        gpu_ms = ev.elapsed_ms(a, b) / steps
+        ev.destroy(a)
+        ev.destroy(b)
+        res[name] = {"wall_ms_p50": float(np.percentile(walls, 50)),
+                     "wall_ms_p99": float(np.percentile(walls, 99)),
+                     "gpu_ms_mean_back_to_back": gpu_ms}
+    res["wall_speedup"] = res["eager"]["wall_ms_p50"] / res["graph"]["wall_ms_p50"]
+    graph.destroy()
+    for bufs in [d_p, d_bp, d_bu, d_bd] + [b for s in series for b in s] + \
+            [b for o in outs for b in o]:
+        bufs.free()
+    return res
+
+
 def bench_moving_averages(steps, warmup, ev):
     """SMA / WMA / SWMA / SMMA / Wilders.  One series x 1,250 periods (2..=1251)
     over the 200,000-bar sweep series (the ALMA sweep's output size, 1.0 GB),
@@ -503,7 +575,8 @@ def main():
                     help="skip the secondary workloads (many-series OHLC, rolling windows, "
                          "single ALMA)")
     ap.add_argument("--mode", default="sweep",
-                    choices=["sweep", "calib", "ohlc", "rolling", "cross", "serve", "ma", "rec"],
+                    choices=["sweep", "calib", "ohlc", "rolling", "cross", "serve", "ma", "rec",
+                             "graph"],
                     help="calib: 256 MiB device-to-device copies, the known-size reference "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
@@ -528,6 +601,10 @@ def main():
         return
     if args.mode == "rec":
         r = bench_recurrence_batches(args.steps, args.warmup, HipEvents())
+        print(json.dumps(r))
+        return
+    if args.mode == "graph":
+        r = bench_captured_batch(args.steps, args.warmup, HipEvents())
         print(json.dumps(r))
         return
     if args.mode == "ma":
@@ -633,6 +710,7 @@ def main():
                          ("alma_host_boundary", lambda: bench_alma_host_boundary()),
                          ("double_alma_crossover", lambda: bench_double_alma(5, 1, ev)),
                          ("serving", lambda: bench_serving()),
+                         ("captured_batch", lambda: bench_captured_batch(20, 3, ev)),
                          ("alma_single", lambda: bench_alma_single(20, ev))):
             extra[name] = fn()
             print(f"[bench] {name}: {extra[name]}", file=sys.stderr, flush=True)

@@ -68,6 +68,32 @@ int vta_ctx_device_id(const vta_ctx *ctx);      /* CudaAlma::device_id */
  * symbol exists (the reference JITs its PTX in new()). */
 vta_status vta_ctx_preload(vta_ctx *ctx);
 
+/* ------------------------------------------------- captured batches */
+/* No counterpart in the reference (its wrappers launch eagerly, one launch
+ * per call).  A sequence of enqueue-only calls on one context -- the
+ * `_device` entry points, vta_alma_sweep_launch, vta_many_series_prepare's
+ * launches, the fused many-series plan -- recorded once with stream capture
+ * and replayed as one hipGraph launch on the context's stream.
+ *
+ * vta_ctx_begin_capture loads every code object first (no module load happens
+ * while capturing) and starts a relaxed-mode capture of the context's stream.
+ * Between begin and end only enqueue-only entry points may be called: an
+ * entry point that allocates, copies from the host or synchronises (the `_dev`
+ * forms, vta_ctx_synchronize) fails while capturing and invalidates the
+ * capture.  vta_ctx_end_capture always ends the capture; it returns the error
+ * of an invalidated capture instead of a graph.
+ *
+ * The graph holds the device pointers and sizes given at capture time, and
+ * the context's scratch and chunk-carry buffers: replays read and write those
+ * same buffers, in stream order on the context's stream.  Replaying a graph
+ * while another stream of the program uses the same output buffers races. */
+typedef struct vta_graph vta_graph;
+vta_status vta_ctx_begin_capture(vta_ctx *ctx);
+vta_status vta_ctx_end_capture(vta_ctx *ctx, vta_graph **out);
+vta_status vta_graph_launch(vta_ctx *ctx, const vta_graph *graph);   /* enqueue-only */
+size_t vta_graph_num_nodes(const vta_graph *graph);
+vta_status vta_graph_destroy(vta_graph *graph);
+
 /* --------------------------------------------------- device arrays */
 /* DeviceArrayF32 {buf, rows, cols} (alma_wrapper.rs:103-117): owned device
  * buffer, freed with vta_device_array_free (Rust frees on drop). */

@@ -8,7 +8,8 @@ it, mirroring the reference's ``src/cuda`` wrapper classes (``CudaAlma`` ...).
 from ._lib import VtaError, lib
 from .alma import AlmaBatchRange, AlmaParams, AlmaSweepPlan, CudaAlma, HipAlma, expand_grid
 from .cross import DoubleAlmaCrossover, DoubleAlmaPlan
-from .device import DeviceArrayF32, DeviceBuffer, HipContext, cuda_available, cuda_device_count
+from .device import (DeviceArrayF32, DeviceBuffer, HipContext, HipGraph, cuda_available,
+                     cuda_device_count)
 from .fir import CudaSma, CudaSwma, CudaWma
 from .rec import (CudaAdx, CudaAtr, CudaEma, CudaRsi, CudaSmma, CudaWilders, ManySeriesOhlcPlan,
                   ema_rsi_atr_many_series_dev)
@@ -19,7 +20,7 @@ from .rolling import (BollingerBandsBatchRange, CudaBollingerBands, CudaMediumAd
 
 __all__ = [
     "VtaError", "lib", "AlmaBatchRange", "AlmaParams", "AlmaSweepPlan", "CudaAlma", "HipAlma",
-    "expand_grid", "DeviceArrayF32", "DeviceBuffer", "HipContext", "cuda_available",
+    "expand_grid", "DeviceArrayF32", "DeviceBuffer", "HipContext", "HipGraph", "cuda_available",
     "cuda_device_count", "CudaEma", "CudaRsi", "CudaAtr", "CudaAdx", "ManySeriesOhlcPlan",
     "ema_rsi_atr_many_series_dev", "BollingerBandsBatchRange", "CudaBollingerBands",
     "CudaMediumAd", "CudaPercentileNearestRank", "PercentileNearestRankBatchRange",

@@ -103,6 +103,11 @@ SIGNATURES = {
     "vta_ctx_stream": (_P, [_P]),
     "vta_ctx_device_id": (ctypes.c_int, [_P]),
     "vta_ctx_preload": (_ST, [_P]),
+    "vta_ctx_begin_capture": (_ST, [_P]),
+    "vta_ctx_end_capture": (_ST, [_P, ctypes.POINTER(_P)]),
+    "vta_graph_launch": (_ST, [_P, _P]),
+    "vta_graph_num_nodes": (_SZ, [_P]),
+    "vta_graph_destroy": (_ST, [_P]),
     "vta_device_array_free": (_ST, [_P, ctypes.POINTER(DeviceArrayF32C)]),
     "vta_device_array_copy_to_host": (_ST, [_P, ctypes.POINTER(DeviceArrayF32C), _P, _SZ]),
     "vta_device_alloc": (_ST, [_P, _SZ, ctypes.POINTER(_P)]),

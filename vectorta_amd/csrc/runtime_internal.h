@@ -25,7 +25,7 @@ struct vta_ctx {
     // of the enqueue-only entry points; reused in stream order)
     void *scratch = nullptr;
     static constexpr size_t kScratchBytes = 64 * 1024;
-    // chunk-end states of the recurrences' chunked scan ([chunk][combo][state]
+    // chunk-end states of the recurrences' chunked scan ([combo][chunk][state]
     // f64), reused in stream order by every batch launch of the context
     void *carry = nullptr;
     static constexpr size_t kCarryBytes = 16u << 20;

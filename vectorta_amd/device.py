@@ -60,6 +60,18 @@ class HipContext:
         except Exception:
             pass
 
+    # captured batches
+    def begin_capture(self) -> None:
+        """Starts recording the context's enqueue-only calls (see HipGraph)."""
+        check(lib().vta_ctx_begin_capture(self.handle))
+
+    def end_capture(self) -> "HipGraph":
+        """Ends the recording; raises if a call made while recording could not
+        be captured (the capture is ended either way)."""
+        g = ctypes.c_void_p()
+        check(lib().vta_ctx_end_capture(self.handle, ctypes.byref(g)))
+        return HipGraph(self, g)
+
     # raw device memory for the enqueue-only entry points
     def alloc(self, nbytes: int) -> "DeviceBuffer":
         p = ctypes.c_void_p()
@@ -72,6 +84,37 @@ class HipContext:
         check(lib().vta_memcpy_htod(self.handle, ctypes.c_void_p(buf.ptr),
                                     host.ctypes.data_as(ctypes.c_void_p), host.nbytes))
         return buf
+
+
+class HipGraph:
+    """A replayable batch: the enqueue-only calls made between
+    ``ctx.begin_capture()`` and ``ctx.end_capture()``, launched again with one
+    ``launch()`` on the context's stream.  It keeps the device pointers, sizes
+    and parameters of the capture; keep those buffers alive while it lives."""
+
+    def __init__(self, ctx: HipContext, ptr: ctypes.c_void_p):
+        self.ctx = ctx
+        self._ptr = ptr
+
+    @property
+    def num_nodes(self) -> int:
+        return int(lib().vta_graph_num_nodes(self._ptr)) if self._ptr else 0
+
+    def launch(self) -> None:
+        if not self._ptr:
+            raise VtaError(4, "graph is destroyed")
+        check(lib().vta_graph_launch(self.ctx.handle, self._ptr))
+
+    def destroy(self) -> None:
+        if self._ptr:
+            lib().vta_graph_destroy(self._ptr)
+            self._ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 class DeviceBuffer:
