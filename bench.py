@@ -33,6 +33,13 @@ HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
 FP32_VALU_PEAK_TFLOPS = 157.3
 
+# split-BF16 sweep kernel launched for each value of the alma_mfma_variant knob
+# (variant 4 also launches alma_batch_mfma_bf16x3_wg16_klo for its K <= 128 groups)
+MFMA_VARIANT_KERNEL = {1: "alma_batch_mfma_bf16x3_wg16", 2: "alma_batch_mfma_bf16x3_wg8",
+                       3: "alma_batch_mfma_bf16x3_wg16_t2", 4: "alma_batch_mfma_bf16x3_wg16",
+                       5: "alma_batch_mfma_bf16x3_r8", 6: "alma_batch_mfma_bf16x3",
+                       7: "alma_batch_mfma_bf16x3_r32"}
+
 METRIC = "ALMA sweep outputs/s at <=1e-5 rel err vs FP64 (MFMA vs FP32); p50/p99 latency"
 
 
@@ -571,6 +578,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and error check")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
     ap.add_argument("--forms", default="mfma,fp32")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="launch the MFMA sweep with the built-in kernel variant instead of "
+                         "the autotuned one")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the secondary workloads (many-series OHLC, rolling windows, "
                          "single ALMA)")
@@ -631,6 +641,15 @@ def main():
     outputs = {}
     for form in args.forms.split(","):
         plan = alma.sweep_plan(sweep, T, kernel=form)
+        tuned = None
+        if form == "mfma" and not args.no_tune:
+            # kernel variant picked by the autotuner on this workload (untimed)
+            from vectorta_amd import autotune
+            tuner = autotune.Autotuner(reps=5)
+            best = tuner.tune_alma_sweep(plan, d_x.ptr, 0, d_out.ptr)
+            tuned = {"knob": "alma_mfma_variant", "value": best,
+                     "candidate_ms": next(iter(tuner.table.values()))["ms"]}
+            print(f"[bench] autotune: {tuned}", file=sys.stderr, flush=True)
         ms_step, k_mean, ks = time_form(alma, plan, d_x, d_out, args.steps, args.warmup, ev)
         forms[form] = {
             "ms_per_step": ms_step,
@@ -641,6 +660,8 @@ def main():
             "achieved_GBs": alg_bytes / (k_mean * 1e-3) / 1e9,
             "achieved_TFLOPs": alg_flops / (k_mean * 1e-3) / 1e12,
         }
+        if tuned:
+            forms[form]["autotune"] = tuned
         if not args.no_cpu:
             outputs[form] = d_out.copy_to_host().reshape(C, T)
         plan.close()
@@ -669,7 +690,8 @@ def main():
 
     head = "mfma" if ok("mfma") else "fp32"
     hf = forms[head]
-    out_kernel = "alma_batch_mfma_bf16x3_wg16" if head == "mfma" else "alma_batch_f32"
+    variant = (forms.get("mfma") or {}).get("autotune", {}).get("value", 1)
+    out_kernel = MFMA_VARIANT_KERNEL[variant] if head == "mfma" else "alma_batch_f32"
     mf = forms.get("mfma")
     ff = forms.get("fp32")
     out = {

@@ -68,6 +68,43 @@ int vta_ctx_device_id(const vta_ctx *ctx);      /* CudaAlma::device_id */
  * symbol exists (the reference JITs its PTX in new()). */
 vta_status vta_ctx_preload(vta_ctx *ctx);
 
+/* ------------------------------------------------------------ tuning */
+/* Launch-shape knobs held per context, the counterpart of the reference's
+ * ALMA_TILE environment knob (alma_wrapper.rs:314) and kernel policies
+ * (:18-70), extended to every kernel family with a shape choice.  0 selects
+ * the built-in heuristic (which an environment variable of the same family
+ * may still override); any other value must be in the knob's range. */
+typedef enum vta_tune_knob {
+    /* split-BF16 ALMA sweep kernel: 1 workgroup of 4 N-blocks x 16 tiles,
+     * 2 x 8 tiles, 3 x 16 tiles two at a time, 4 x 16 tiles with K > 128 and
+     * K <= 128 in separate launches, 5 one wave per N-block x 8 tiles,
+     * 6 x 16 tiles, 7 x 32 tiles */
+    VTA_TUNE_ALMA_MFMA_VARIANT = 0,
+    VTA_TUNE_SCAN_WAVES = 1,  /* recurrence batch chunked scan: waves per pass, 64..65536 */
+    VTA_TUNE_FIR_GROUP = 2,   /* SMA/WMA/SWMA batch: combos per workgroup, 1..256 */
+    VTA_TUNE_MS_CHUNK = 3,    /* many-series recurrences: bars per chunk, >= 64 */
+    VTA_TUNE_KNOB_COUNT = 4
+} vta_tune_knob;
+vta_status vta_ctx_set_tuning(vta_ctx *ctx, vta_tune_knob knob, int64_t value);
+vta_status vta_ctx_get_tuning(const vta_ctx *ctx, vta_tune_knob knob, int64_t *value);
+
+/* Autotuner (no reference counterpart).  For each candidate value of `knob`
+ * it sets the knob and runs enqueue(ctx, user) once untimed; then, in three
+ * interleaved rounds over the candidates (so clock drift during the search
+ * does not favour the first ones), `reps` times back to back between two HIP
+ * events on the context's stream.  The value with the least round mean wins
+ * and the knob is left at it.  enqueue
+ * must only enqueue work on ctx (an enqueue-only entry point, a sweep plan
+ * launch, a graph launch) and return its status.  A candidate whose enqueue
+ * fails with VTA_ERR_LAUNCH_CONFIG_TOO_LARGE (e.g. too much LDS for that
+ * shape) is skipped; any other failure ends the search with that status and
+ * the knob restored.  ms_per_candidate (may be NULL) receives each
+ * candidate's mean ms (a negative value for a skipped one). */
+typedef vta_status (*vta_enqueue_fn)(vta_ctx *ctx, void *user);
+vta_status vta_autotune(vta_ctx *ctx, vta_tune_knob knob, const int64_t *candidates,
+                        size_t n_candidates, vta_enqueue_fn enqueue, void *user, int reps,
+                        int64_t *best, float *ms_per_candidate);
+
 /* ------------------------------------------------- captured batches */
 /* No counterpart in the reference (its wrappers launch eagerly, one launch
  * per call).  A sequence of enqueue-only calls on one context -- the

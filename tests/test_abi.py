@@ -20,6 +20,8 @@ PKG = os.path.join(ROOT, "vectorta_amd")
 def declared_symbols():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    # function-pointer typedefs name a type, not an exported symbol
+    text = re.sub(r"typedef\s+\w+\s*\(\s*\*[^;]*;", "", text)
     return sorted(set(re.findall(r"\b(vta_[a-z0-9_]+)\s*\(", text)))
 
 

@@ -103,6 +103,11 @@ SIGNATURES = {
     "vta_ctx_stream": (_P, [_P]),
     "vta_ctx_device_id": (ctypes.c_int, [_P]),
     "vta_ctx_preload": (_ST, [_P]),
+    "vta_ctx_set_tuning": (_ST, [_P, ctypes.c_int, ctypes.c_int64]),
+    "vta_ctx_get_tuning": (_ST, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]),
+    "vta_autotune": (_ST, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int64), _SZ, _P, _P,
+                           ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
+                           ctypes.POINTER(ctypes.c_float)]),
     "vta_ctx_begin_capture": (_ST, [_P]),
     "vta_ctx_end_capture": (_ST, [_P, ctypes.POINTER(_P)]),
     "vta_graph_launch": (_ST, [_P, _P]),

@@ -375,21 +375,33 @@ vta_status plan_launch(vta_ctx *ctx, const vta_alma_sweep_plan *plan, const floa
     // the tables do not depend on the series length; a caller may launch a
     // plan over any length that covers its longest period
     const int T = (int)(series_len ? series_len : plan->series_len);
-    if (plan->kernel == VTA_ALMA_BATCH_MFMA_BF16X3 && mf_shared_staging()) {
+    // kernel variant: the context's VTA_TUNE_ALMA_MFMA_VARIANT knob (1-4 shared
+    // staging, 5-7 one wave per N-block), else the ALMA_MF_* environment
+    // knobs, else the shared-staging form with 16 tiles
+    const int variant = (int)ctx->tune[VTA_TUNE_ALMA_MFMA_VARIANT];
+    const bool shared = variant == 0 ? mf_shared_staging() : variant <= 4;
+    if (plan->kernel == VTA_ALMA_BATCH_MFMA_BF16X3 && shared) {
         // shared-staging form: 4 N-blocks per workgroup, R tiles per run
         // (ALMA_MF_TILES=8|16, default 16)
         int tiles = 16;
-        if (const char *e = getenv("ALMA_MF_TILES")) {
-            int v = atoi(e);
-            if (v == 8 || v == 16) tiles = v;
+        bool tj2 = false, ks_req = false;
+        if (variant == 0) {
+            if (const char *e = getenv("ALMA_MF_TILES")) {
+                int v = atoi(e);
+                if (v == 8 || v == 16) tiles = v;
+            }
+            // ALMA_MF_TJ=2: two tiles per pass; ALMA_MF_KSPLIT=1: separate
+            // launches for K > 128 and K <= 128
+            const char *tj_e = getenv("ALMA_MF_TJ");
+            const char *ks_e = getenv("ALMA_MF_KSPLIT");
+            tj2 = tiles == 16 && tj_e && atoi(tj_e) == 2;
+            ks_req = tiles == 16 && ks_e && atoi(ks_e) == 1;
+        } else {
+            tiles = variant == 2 ? 8 : 16;
+            tj2 = variant == 3;
+            ks_req = variant == 4;
         }
-        // ALMA_MF_TJ=2: two tiles per pass; ALMA_MF_KSPLIT=1: separate launches
-        // for K > 128 and K <= 128 (tuning knobs)
-        const char *tj_e = getenv("ALMA_MF_TJ");
-        const char *ks_e = getenv("ALMA_MF_KSPLIT");
-        const bool tj2 = tiles == 16 && tj_e && atoi(tj_e) == 2;
-        const bool ksplit = tiles == 16 && ks_e && atoi(ks_e) == 1 && plan->n_groups_klo > 0 &&
-                            plan->n_groups_klo < plan->n_groups;
+        const bool ksplit = ks_req && plan->n_groups_klo > 0 && plan->n_groups_klo < plan->n_groups;
         alma_mfg_args_host a{};
         a.prices = d_prices;
         a.frags = plan->frags.p;
@@ -441,7 +453,9 @@ vta_status plan_launch(vta_ctx *ctx, const vta_alma_sweep_plan *plan, const floa
         // tiles per wave: 16 by default; ALMA_MF_TILES=8|32 overrides (tuning knob,
         // like the reference's ALMA_TILE, alma_wrapper.rs:314)
         int tiles = kMfTilesPerWave;
-        if (const char *e = getenv("ALMA_MF_TILES")) {
+        if (variant >= 5) {
+            tiles = variant == 5 ? 8 : variant == 6 ? 16 : 32;
+        } else if (const char *e = getenv("ALMA_MF_TILES")) {
             int v = atoi(e);
             if (v == 8 || v == 16 || v == 32) tiles = v;
         }

@@ -36,12 +36,14 @@ static_assert(sizeof(fir_args_host) == 88, "fir_args layout");
 
 enum Fir { FIR_SMA = 0, FIR_WMA = 1, FIR_SWMA = 2 };
 
-// Combos per workgroup of the batch kernel (VTA_FIR_GROUP overrides, 1..256).
+// Combos per workgroup of the batch kernel (the context's VTA_TUNE_FIR_GROUP
+// knob, else VTA_FIR_GROUP, 1..256).
 // Measured on 1,250 periods x 200k bars: 64 per workgroup amortise the tile's
 // prefix scan best (SMA 0.24 ms vs 0.30 ms at 16).  Lane groups of four bars
 // with 16-byte stores were 5-15% slower than one 4-byte store per lane (their
 // strided LDS reads conflict) and were dropped.
-unsigned fir_group() {
+unsigned fir_group(const vta_ctx *ctx) {
+    if (ctx->tune[VTA_TUNE_FIR_GROUP] > 0) return (unsigned)ctx->tune[VTA_TUNE_FIR_GROUP];
     if (const char *e = getenv("VTA_FIR_GROUP")) {
         long v = atol(e);
         if (v >= 1 && v <= 256) return (unsigned)v;
@@ -103,7 +105,7 @@ vta_status batch_launch(vta_ctx *ctx, Fir k, const float *d_x, const int32_t *d_
     a.kind = (int)k;
     a.tile = (int)tile;
     a.span_cap = (int)(tile + max_p + 1);
-    const unsigned group = fir_group();
+    const unsigned group = fir_group(ctx);
     a.group = (int)group;
     unsigned gx = (unsigned)((T + tile - 1) / tile);
     unsigned gy = (unsigned)((C + group - 1) / group);

@@ -72,13 +72,22 @@ double burn_bars(Ind k, size_t p) {
 
 // Chunk length: about four times the longest burn-in (so burn-in costs at
 // most ~25% extra bars), at least 1024 bars, small enough that the launch
-// has roughly 2048+ waves; VTA_REC_CHUNK overrides.
-int pick_chunk(Ind k, size_t max_p, size_t T, size_t lane_groups, bool clean) {
-    if (!clean) return (int)T;  // exact from first_valid in one pass
+// has roughly 2048+ waves; the context's VTA_TUNE_MS_CHUNK knob, else
+// VTA_REC_CHUNK, overrides.
+int ms_chunk_override(const vta_ctx *ctx, size_t T) {
+    if (ctx->tune[VTA_TUNE_MS_CHUNK] > 0)
+        return (int)std::min<size_t>((size_t)ctx->tune[VTA_TUNE_MS_CHUNK], T);
     if (const char *e = getenv("VTA_REC_CHUNK")) {
         long v = atol(e);
         if (v >= 64) return (int)std::min<size_t>((size_t)v, T);
     }
+    return 0;
+}
+
+int pick_chunk(const vta_ctx *ctx, Ind k, size_t max_p, size_t T, size_t lane_groups,
+               bool clean) {
+    if (!clean) return (int)T;  // exact from first_valid in one pass
+    if (int c = ms_chunk_override(ctx, T)) return c;
     double kb = burn_bars(k, max_p);
     size_t want_chunks = std::max<size_t>(1, 2048 / std::max<size_t>(lane_groups, 1));
     size_t by_par = (T + want_chunks - 1) / want_chunks;
@@ -182,7 +191,9 @@ vta_status scan_launch(vta_ctx *ctx, Ind k, rec_args_host a, size_t T, size_t C,
     const size_t S = k == IND_RSI ? 2 : k == IND_ADX ? 4 : 1;
     const size_t groups = (C + 63) / 64;
     size_t waves = kScanWaves;
-    if (const char *e = getenv("VTA_SCAN_WAVES")) {
+    if (ctx->tune[VTA_TUNE_SCAN_WAVES] > 0) {
+        waves = (size_t)ctx->tune[VTA_TUNE_SCAN_WAVES];
+    } else if (const char *e = getenv("VTA_SCAN_WAVES")) {
         long v = atol(e);
         if (v >= 64) waves = (size_t)v;
     }
@@ -344,7 +355,7 @@ vta_status ms_launch(vta_ctx *ctx, Ind k, const float *d_x, const float *d_h, co
     a.T = (int)rows;
     a.nlanes = (int)cols;
     unsigned groups = (unsigned)((cols + 63) / 64);
-    a.chunk = pick_chunk(k, period, rows, groups, d_clean != nullptr);
+    a.chunk = pick_chunk(ctx, k, period, rows, groups, d_clean != nullptr);
     // without clean flags every series runs exactly from first_valid in one chunk
     if (!d_clean) a.chunk = (int)rows;
     unsigned chunks = (unsigned)((rows + a.chunk - 1) / a.chunk);
@@ -666,8 +677,8 @@ vta_status vta_ema_rsi_atr_many_series_device(vta_ctx *ctx, const float *d_close
     size_t kb = std::max({(size_t)burn_bars(IND_EMA, ema_period),
                           (size_t)burn_bars(IND_RSI, rsi_period),
                           (size_t)burn_bars(IND_ATR, atr_period)});
-    a.chunk = pick_chunk(IND_ATR, 1, rows, groups, true);
-    if (!getenv("VTA_REC_CHUNK")) {
+    a.chunk = pick_chunk(ctx, IND_ATR, 1, rows, groups, true);
+    if (!ms_chunk_override(ctx, rows)) {
         size_t c = std::max<size_t>({4 * kb, 1024, (size_t)a.chunk});
         a.chunk = (int)std::min(((c + 63) & ~(size_t)63), rows);
     }

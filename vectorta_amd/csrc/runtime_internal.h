@@ -29,6 +29,8 @@ struct vta_ctx {
     // f64), reused in stream order by every batch launch of the context
     void *carry = nullptr;
     static constexpr size_t kCarryBytes = 16u << 20;
+    // launch-shape knobs (vta_tune_knob); 0 = built-in heuristic
+    int64_t tune[VTA_TUNE_KNOB_COUNT] = {};
 };
 
 namespace vta {
