@@ -310,6 +310,70 @@ def bench_captured_batch(steps, warmup, ev):
     return res
 
 
+def bench_tune_shapes(reps):
+    """Autotuner sweep over served shapes: for series lengths 2^12..2^20 and
+    ALMA grids of 8..2,000 combos (periods spread over 10..250), every
+    split-BF16 kernel variant timed by vta_autotune; and the chunked-scan
+    wave target for EMA and ADX batches at two shapes.  Prints the best
+    variant per shape (the table the library's built-in choice follows)."""
+    from vectorta_amd import autotune as at
+    alma = vta.CudaAlma(0)
+    ctx = alma.ctx
+    grids = {8: ((10, 250, 240), (0.25, 0.85, 0.6), (3.0, 6.0, 3.0)),
+             64: ((10, 250, 80), (0.05, 0.95, 0.3), (2.0, 8.0, 2.0)),
+             512: ((10, 250, 34), (0.05, 0.95, 0.128), (2.0, 9.0, 1.0)),
+             2000: ((10, 250, 16), (0.05, 0.95, 0.1), (2.0, 10.0, 2.0))}
+    res = {"workload": "autotune_shape_sweep", "alma": []}
+    for T in (1 << 12, 1 << 14, 1 << 16, 1 << 18, 1 << 20):
+        x = datagen.gbm_close(T, 0.015, seed=5)
+        d_x = ctx.to_device(x)
+        for target, (p, o, s) in grids.items():
+            sweep = vta.AlmaBatchRange(period=p, offset=o, sigma=s)
+            plan = vta.AlmaSweepPlan(alma, sweep, T, kernel="mfma")
+            d_o = ctx.alloc(plan.n_combos * T * 4)
+            best, times = at.measure(ctx, at.ALMA_MFMA_VARIANT, at.CANDIDATES[at.ALMA_MFMA_VARIANT],
+                                     lambda: plan.launch(d_x.ptr, 0, d_o.ptr), reps)
+            row = {"series_len": T, "combos": plan.n_combos, "n_tiles": (T + 31) // 32,
+                   "best": best, "ms": times}
+            res["alma"].append(row)
+            print(f"[tune] alma T={T} C={plan.n_combos} best={best} "
+                  f"{ {k: round(v, 4) for k, v in times.items() if v} }", file=sys.stderr,
+                  flush=True)
+            plan.close()
+            d_o.free()
+        d_x.free()
+    at.set_tuning(ctx, at.ALMA_MFMA_VARIANT, 0)
+    res["scan"] = []
+    ema, adx = vta.CudaEma(0), vta.CudaAdx(0)
+    for T, C in ((16_384, 64), (200_000, 1_250)):
+        c = datagen.gbm_close(T, 0.01, seed=6)
+        rng = np.random.Generator(np.random.PCG64(6))
+        _, h, lo, c = datagen.ohlc_from_close(c, rng)
+        per = np.arange(2, 2 + C, dtype=np.int32)
+        for name, op in (("ema", ema), ("adx", adx)):
+            cx = op.ctx
+            d_c, d_h, d_l, d_p = (cx.to_device(a) for a in (c, h, lo, per))
+            d_o = cx.alloc(C * T * 4)
+            P = ctypes.c_void_p
+            if name == "ema":
+                fn = lambda: check(lib().vta_ema_batch_device(
+                    cx.handle, P(d_c.ptr), P(d_p.ptr), None, int(per.max()), T, 0, C, P(d_o.ptr)))
+            else:
+                fn = lambda: check(lib().vta_adx_batch_device(
+                    cx.handle, P(d_h.ptr), P(d_l.ptr), P(d_c.ptr), P(d_p.ptr), int(per.max()), T,
+                    0, C, P(d_o.ptr)))
+            best, times = at.measure(cx, at.SCAN_WAVES, at.CANDIDATES[at.SCAN_WAVES], fn, reps)
+            res["scan"].append({"indicator": name, "series_len": T, "combos": C, "best": best,
+                                "ms": times})
+            print(f"[tune] {name} T={T} C={C} best={best} "
+                  f"{ {k: round(v, 4) for k, v in times.items() if v} }", file=sys.stderr,
+                  flush=True)
+            at.set_tuning(cx, at.SCAN_WAVES, 0)
+            for b in (d_c, d_h, d_l, d_p, d_o):
+                b.free()
+    return res
+
+
 def bench_moving_averages(steps, warmup, ev):
     """SMA / WMA / SWMA / SMMA / Wilders.  One series x 1,250 periods (2..=1251)
     over the 200,000-bar sweep series (the ALMA sweep's output size, 1.0 GB),
@@ -586,7 +650,7 @@ def main():
                          "single ALMA)")
     ap.add_argument("--mode", default="sweep",
                     choices=["sweep", "calib", "ohlc", "rolling", "cross", "serve", "ma", "rec",
-                             "graph"],
+                             "graph", "tune"],
                     help="calib: 256 MiB device-to-device copies, the known-size reference "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
@@ -611,6 +675,10 @@ def main():
         return
     if args.mode == "rec":
         r = bench_recurrence_batches(args.steps, args.warmup, HipEvents())
+        print(json.dumps(r))
+        return
+    if args.mode == "tune":
+        r = bench_tune_shapes(5)
         print(json.dumps(r))
         return
     if args.mode == "graph":

@@ -237,6 +237,23 @@ bool mf_shared_staging() {
 }
 constexpr size_t kMfMaxPeriod = 256;
 
+bool mf_env_override() {
+    return getenv("ALMA_MF_FORM") || getenv("ALMA_MF_TILES") || getenv("ALMA_MF_TJ") ||
+           getenv("ALMA_MF_KSPLIT");
+}
+
+// Built-in split-BF16 kernel variant (vta_tune_knob numbering), from an
+// autotuner sweep over series lengths 2^12..2^20 and grids of 8..800 combos
+// (every variant timed in interleaved rounds): with at most 64 combos one wave
+// per N-block over 8 tiles (variant 5) is within 4% of the best everywhere;
+// up to 2^27 outputs the 8-tile workgroups (2) are 5-35% faster than the
+// 16-tile ones, whose staging pays off above that (1).
+int mf_default_variant(size_t T, size_t C) {
+    if (C <= 64) return 5;
+    if ((double)T * (double)C <= 134217728.0) return 2;
+    return 1;
+}
+
 // Builds the MFMA operand tables.  Combos are sorted by period (stable) and
 // cut into N-blocks of 32; block b contracts over K_b = round_up(max period of
 // the block, 16) taps with shorter periods zero padded at the front.  Each
@@ -378,7 +395,8 @@ vta_status plan_launch(vta_ctx *ctx, const vta_alma_sweep_plan *plan, const floa
     // kernel variant: the context's VTA_TUNE_ALMA_MFMA_VARIANT knob (1-4 shared
     // staging, 5-7 one wave per N-block), else the ALMA_MF_* environment
     // knobs, else the shared-staging form with 16 tiles
-    const int variant = (int)ctx->tune[VTA_TUNE_ALMA_MFMA_VARIANT];
+    int variant = (int)ctx->tune[VTA_TUNE_ALMA_MFMA_VARIANT];
+    if (variant == 0 && !mf_env_override()) variant = mf_default_variant(T, plan->n_combos);
     const bool shared = variant == 0 ? mf_shared_staging() : variant <= 4;
     if (plan->kernel == VTA_ALMA_BATCH_MFMA_BF16X3 && shared) {
         // shared-staging form: 4 N-blocks per workgroup, R tiles per run

@@ -1010,7 +1010,14 @@ __device__ __forceinline__ void drive_batch(const rec_args &a, float *out) {
             if (i < n) st0[i] = cs[i];
     }
     const bool emit = a.phase == 0 || a.phase == 3;
-    const int t_begin = wave_min(min(start, c0)) & ~63;
+    // a carried lane restarts after bar c0 - 1 here, from that one bar read
+    // directly, so the wave need not run the 64-bar block holding it
+    if (carried) {
+        const int tp = c0 - 1;
+        if constexpr (NIN == 1) e.begin_carry(tp, (double)a.x[tp], st0);
+        else e.begin_carry(tp, (double)a.h[tp], (double)a.l[tp], (double)a.x[tp], st0);
+    }
+    const int t_begin = wave_min(carried ? c0 : min(start, c0)) & ~63;
     // bars of the next block are loaded into registers while the current one
     // is processed
     float nx0, nx1 = 0.0f, nx2 = 0.0f;
@@ -1153,13 +1160,14 @@ __device__ __forceinline__ void drive_batch(const rec_args &a, float *out) {
             if (active && t < c1) {
                 const double x = xs[0][j];
                 if (t == start) {
+                    // (a carried lane began before the loop)
                     if constexpr (NIN == 1) {
-                        if (carried) e.begin_carry(t, x, st0);
-                        else if (burn) e.begin_burn(x);
+                        if (carried) {
+                        } else if (burn) e.begin_burn(x);
                         else y = e.begin_exact(x);
                     } else {
-                        if (carried) e.begin_carry(t, xs[1][j], xs[2][j], x, st0);
-                        else if (burn) e.begin_burn(xs[1][j], xs[2][j], x);
+                        if (carried) {
+                        } else if (burn) e.begin_burn(xs[1][j], xs[2][j], x);
                         else y = e.begin_exact(xs[1][j], xs[2][j], x);
                     }
                 } else if (t > start) {
