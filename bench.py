@@ -40,6 +40,14 @@ MFMA_VARIANT_KERNEL = {1: "alma_batch_mfma_bf16x3_wg16", 2: "alma_batch_mfma_bf1
                        5: "alma_batch_mfma_bf16x3_r8", 6: "alma_batch_mfma_bf16x3",
                        7: "alma_batch_mfma_bf16x3_r32"}
 
+def builtin_mfma_variant(T, C):
+    """The library's built-in variant for a T-bar x C-combo sweep (its rule
+    in alma.cpp, mf_default_variant)."""
+    if C <= 64:
+        return 5
+    return 2 if T * C <= 134217728 else 1
+
+
 METRIC = "ALMA sweep outputs/s at <=1e-5 rel err vs FP64 (MFMA vs FP32); p50/p99 latency"
 
 
@@ -642,9 +650,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and error check")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
     ap.add_argument("--forms", default="mfma,fp32")
-    ap.add_argument("--no-tune", action="store_true",
-                    help="launch the MFMA sweep with the built-in kernel variant instead of "
-                         "the autotuned one")
+    ap.add_argument("--tune", action="store_true",
+                    help="autotune the MFMA sweep kernel variant on this workload before "
+                         "timing (default: the built-in variant, which the autotuner's "
+                         "shape sweep chose)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the secondary workloads (many-series OHLC, rolling windows, "
                          "single ALMA)")
@@ -710,7 +719,7 @@ def main():
     for form in args.forms.split(","):
         plan = alma.sweep_plan(sweep, T, kernel=form)
         tuned = None
-        if form == "mfma" and not args.no_tune:
+        if form == "mfma" and args.tune:
             # kernel variant picked by the autotuner on this workload (untimed)
             from vectorta_amd import autotune
             tuner = autotune.Autotuner(reps=5)
@@ -758,7 +767,8 @@ def main():
 
     head = "mfma" if ok("mfma") else "fp32"
     hf = forms[head]
-    variant = (forms.get("mfma") or {}).get("autotune", {}).get("value", 1)
+    variant = (forms.get("mfma") or {}).get("autotune", {}).get(
+        "value", builtin_mfma_variant(T, C))
     out_kernel = MFMA_VARIANT_KERNEL[variant] if head == "mfma" else "alma_batch_f32"
     mf = forms.get("mfma")
     ff = forms.get("fp32")

@@ -270,9 +270,7 @@ struct SortedWin {
     float *col;
     int n;
     __device__ float at(int i) const { return col[64 * i]; }
-    // number of elements < v (lower bound): count the full 8-element blocks
-    // whose last element is < v, then the elements < v inside the next block;
-    // every read of a level is independent of the others
+    // number of elements < v (lower bound); `steps` binary-search rounds cover n
     __device__ int lower(float v, int steps) const {
         int lo = 0, hi = n;
         for (int r = 0; r < steps; ++r) {
@@ -290,9 +288,11 @@ struct SortedWin {
         ++n;
     }
     // Replaces one element equal to `vo` (present) by `vi`: the elements
-    // strictly between the two positions move one slot towards vo's slot, in
-    // batches of 8 (eight reads, then eight writes; read indices are clamped
-    // to the lane's column, writes past the span are masked).
+    // strictly between the two positions move one slot towards vo's slot, one
+    // at a time.  (Batches of eight reads then eight writes, with clamped
+    // reads and masked writes for the lanes' different spans, and the two
+    // searches interleaved, measured 2x slower: the wave is bound by LDS
+    // instruction issue, and the clamped reads add instructions.)
     __device__ void replace(float vo, float vi, int steps) {
         const int po = lower(vo, steps);
         const int pi = lower(vi, steps);

@@ -37,17 +37,31 @@ static_assert(sizeof(roll_args_host) == 112, "roll_args layout");
 
 enum RInd { R_BB = 0, R_PNR = 1, R_MAD = 2 };
 
-// Chunk length: enough chunks for ~2048 waves, at least 4x the window so the
-// per-chunk rebuild stays a small fraction, at most 4096 bars (BB re-anchors
-// its f64 running sums at every chunk start).  `chunked` false: one chunk.
-int roll_chunk(size_t T, size_t lanes, size_t max_w, bool chunked) {
+// Chunk length: enough chunks for ~2048 waves, or for the PNR / MAD kernels
+// (whose sorted windows take smem bytes of LDS per wave) as many waves as
+// the CUs hold at once, so the launch runs in one round (PNR, window 100,
+// 1,024 x 50,000: 25.9 KB per wave, 6 waves per CU; 400-bar chunks = 2,000
+// waves in 1.3 rounds took 4.27 ms, 521-bar chunks = 1,536 waves 2.81 ms;
+// these kernels are bound by the latency of each lane's LDS shift chain, so a
+// second round costs a whole chunk's time); at least 4x the
+// window so the per-chunk rebuild stays a small fraction, at most 4096 bars
+// (BB re-anchors its f64 running sums at every chunk start).  `chunked`
+// false: one chunk.
+int roll_chunk(const vta_ctx *ctx, size_t T, size_t lanes, size_t max_w, bool chunked,
+               size_t smem) {
     if (!chunked) return (int)T;
     if (const char *e = getenv("VTA_ROLL_CHUNK")) {
         long v = atol(e);
         if (v >= 64) return (int)std::min<size_t>((size_t)v, T);
     }
+    size_t waves = 2048;
+    if (smem > 0) {
+        const size_t per_cu = std::max<size_t>(
+            1, (size_t)ctx->props.maxSharedMemoryPerMultiProcessor / smem);
+        waves = std::max<size_t>(256, per_cu * (size_t)ctx->props.multiProcessorCount);
+    }
     size_t groups = (lanes + 63) / 64;
-    size_t want = std::max<size_t>(1, 2048 / std::max<size_t>(groups, 1));
+    size_t want = std::max<size_t>(1, waves / std::max<size_t>(groups, 1));
     size_t c = std::max<size_t>({(T + want - 1) / want, 4 * max_w, 256});
     c = std::min<size_t>(c, 4096);
     return (int)std::min(c, T);
@@ -63,14 +77,14 @@ vta_status roll_launch(vta_ctx *ctx, RInd k, bool batch, roll_args_host &a, size
                        bool chunked) {
     hipFunction_t f;
     VTA_TRY(get_function(ctx, "rolling", kname(k, batch), &f));
-    a.chunk = roll_chunk((size_t)a.T, (size_t)a.nlanes, max_w, chunked);
-    a.wcap = (int)max_w;
-    unsigned groups = (unsigned)((a.nlanes + 63) / 64);
-    unsigned chunks = (unsigned)((a.T + a.chunk - 1) / a.chunk);
     // PNR / MAD: the lanes' sorted windows, plus (batch) the 64 x 33 output staging
     size_t smem = (k == R_BB) ? 0
                               : (size_t)64 * (max_w + 1) * sizeof(float) +
                                     (batch ? (size_t)64 * 33 * sizeof(float) : 0);
+    a.chunk = roll_chunk(ctx, (size_t)a.T, (size_t)a.nlanes, max_w, chunked, smem);
+    a.wcap = (int)max_w;
+    unsigned groups = (unsigned)((a.nlanes + 63) / 64);
+    unsigned chunks = (unsigned)((a.T + a.chunk - 1) / a.chunk);
     if (smem > ctx->props.sharedMemPerBlock)
         return fail(VTA_ERR_INVALID_INPUT, "window length %zu exceeds the LDS capacity", max_w);
     return launch(ctx, f, dim3(groups, chunks, 1), dim3(64), (unsigned)smem, a);
