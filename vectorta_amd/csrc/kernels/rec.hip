@@ -17,10 +17,10 @@
 // (64 streams, chunk).  A wave whose chunk starts far enough after the
 // warm-up begins K bars early with an approximate state and discards those
 // outputs ("burn-in").  All four recurrences contract the state error by
-// beta = 1 - 1/p (EMA: 1 - 2/(p+1)) per bar, so after K >= 25 / -ln(beta)
+// beta = 1 - 1/p (EMA: 1 - 2/(p+1)) per bar, so after K >= 20 / -ln(beta)
 // bars (ADX, two coupled stages: 32 / -ln(beta)) the error is below
-// e^-25 ~ 1.4e-11 of the state: four orders of magnitude under f32 rounding
-// of the output and six under the 1e-5 parity bound.  This holds only when
+// e^-20 ~ 2.1e-9 of the state: ~30x under f32 rounding of the output and four
+// orders under the 1e-5 parity bound.  This holds only when
 // every bar after first_valid is finite (non-finite bars stop EMA decay and
 // poison ATR forever), so each stream carries a "clean" flag; an unclean
 // stream always starts exactly at first_valid.
@@ -64,6 +64,12 @@ struct rec_args {
 
 __device__ __forceinline__ bool fin(double v) { return vta_isfinite_d(v); }
 
+// Burn-in length in nats of state decay: the carried error of a burnt-in
+// start is below e^-20 ~ 2.1e-9 of the state, ~30x under the f32 rounding of
+// an output and four orders under the 1e-5 parity bound (ADX, whose two
+// coupled stages compound the error, uses 32).
+#define REC_NATS 20.0
+
 __device__ __forceinline__ int burn_bars(double beta, double nats) {
     if (!(beta > 0.0)) return 1;
     double k = nats / -log(beta);
@@ -96,7 +102,7 @@ struct EmaEng {
         alpha = 2.0 / ((double)p + 1.0);
         beta = 1.0 - alpha;
     }
-    __device__ int kb() const { return burn_bars(beta, 25.0); }
+    __device__ int kb() const { return burn_bars(beta, REC_NATS); }
     __device__ int exact_start() const { return fv; }
     __device__ int burn_ok_from() const { return fv + p; }   // b >= this may burn in
     // exact start at bar fv
@@ -170,7 +176,7 @@ struct RsiEng {
         beta = 1.0 - inv_p;
         poisoned = false;
     }
-    __device__ int kb() const { return burn_bars(beta, 25.0); }
+    __device__ int kb() const { return burn_bars(beta, REC_NATS); }
     __device__ int exact_start() const { return fv; }
     __device__ int burn_ok_from() const { return fv + p + 1; }
     __device__ double begin_exact(double x) { px = x; ag = 0.0; al = 0.0; return REC_NAN; }
@@ -271,7 +277,7 @@ struct AtrEng {
         p = p_; fv = fv_;
         alpha = 1.0 / (double)p;
     }
-    __device__ int kb() const { return burn_bars(1.0 - alpha, 25.0); }
+    __device__ int kb() const { return burn_bars(1.0 - alpha, REC_NATS); }
     __device__ int exact_start() const { return fv; }
     __device__ int burn_ok_from() const { return fv + p; }
     __device__ double begin_exact(double h, double l, double c) {
@@ -331,7 +337,7 @@ struct AtrTrEng {
         p = p_; fv = fv_;
         alpha = 1.0 / (double)p;
     }
-    __device__ int kb() const { return burn_bars(1.0 - alpha, 25.0); }
+    __device__ int kb() const { return burn_bars(1.0 - alpha, REC_NATS); }
     __device__ int exact_start() const { return fv; }
     __device__ int burn_ok_from() const { return fv + p; }
     __device__ double begin_exact(double tr) {
@@ -559,7 +565,7 @@ struct SmmaEng {
         pm1 = (double)p - 1.0;
         rp = 1.0 / (double)p;
     }
-    __device__ int kb() const { return burn_bars(1.0 - rp, 25.0); }
+    __device__ int kb() const { return burn_bars(1.0 - rp, REC_NATS); }
     __device__ int exact_start() const { return fv; }
     __device__ int burn_ok_from() const { return fv + p; }
     __device__ double begin_exact(double x) {
@@ -620,7 +626,7 @@ struct WildersEng {
         p = p_; fv = fv_;
         al = 1.0 / (double)p;
     }
-    __device__ int kb() const { return burn_bars(1.0 - al, 25.0); }
+    __device__ int kb() const { return burn_bars(1.0 - al, REC_NATS); }
     __device__ int exact_start() const { return fv; }
     __device__ int burn_ok_from() const { return fv + p; }
     __device__ double seed_bar(int k, double x) {       // k = t - fv < p
@@ -814,8 +820,10 @@ extern "C" __global__ __launch_bounds__(64) void adx_ms1p_f32(rec_args a) {
 // general path is skipped entirely, so a series start costs its own lane ~p
 // general bars.  Loads of the next U rows are issued before the current U
 // rows are consumed (register double buffering); addresses are clamped so
-// the loads are unconditional and stay in flight.
-#define FUSED_U 16
+// the loads are unconditional and stay in flight.  U = 8 keeps the kernel at
+// 167 VGPRs (3 waves per SIMD); U = 16 (185 VGPRs, 2 waves) was 1.5% slower
+// on 4,096 x 50,000, U = 32 (286 VGPRs) holds one wave.
+#define FUSED_U 8
 
 __device__ __forceinline__ float rsi_ratio_f32(double ag, double al) {
     const double den = ag + al;

@@ -65,7 +65,7 @@ const char *ms_kernel(Ind k) {
 // Bars of burn-in each engine needs (same formula as burn_bars in rec.hip).
 double burn_bars(Ind k, size_t p) {
     double beta = k == IND_EMA ? 1.0 - 2.0 / ((double)p + 1.0) : 1.0 - 1.0 / (double)p;
-    double nats = k == IND_ADX ? 32.0 : 25.0;
+    double nats = k == IND_ADX ? 32.0 : 20.0;
     if (!(beta > 0.0)) return 1.0;
     return std::ceil(nats / -std::log(beta));
 }
