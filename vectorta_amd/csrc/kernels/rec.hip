@@ -165,6 +165,20 @@ __device__ __forceinline__ double rec_ratio(double num, double den) {
     return num * r;
 }
 
+// RSI output 100 ag / (ag + al) (50 when the denominator is 0), divided in
+// f32: a reciprocal estimate refined by one Newton step, <= ~2 ulp of f32 (the
+// output is f32, the parity bound 1e-5); outside the estimate's range the
+// f64 quotient.  The ratio is an output only; the state stays f64.
+__device__ __forceinline__ float rsi_ratio_f32(double ag, double al) {
+    const double den = ag + al;
+    if (den == 0.0) return 50.0f;
+    if (!(den > 1e-30 && den < 1e30)) return (float)(100.0 * ag / den);
+    const float d = (float)den;
+    float r = __builtin_amdgcn_rcpf(d);
+    r = fmaf(fmaf(-d, r, 1.0f), r, r);
+    return (float)(100.0 * ag) * r;
+}
+
 struct RsiEng {
     static constexpr bool kSeedScan = true;
     double inv_p, beta, ag, al, px;
@@ -206,8 +220,7 @@ struct RsiEng {
             al = fma(al, beta, inv_p * lo);
         }
         if (poisoned) return REC_NAN;
-        double den = ag + al;
-        return den == 0.0 ? 50.0 : rec_ratio(100.0 * ag, den);
+        return (double)rsi_ratio_f32(ag, al);
     }
     static constexpr int kScan = 2;
     static constexpr bool kTwoStage = false;
@@ -241,8 +254,7 @@ struct RsiEng {
         ag = fma(ag, beta, inv_p * g);
         al = fma(al, beta, inv_p * lo);
         if (poisoned) return REC_NAN;
-        double den = ag + al;
-        return den == 0.0 ? 50.0 : rec_ratio(100.0 * ag, den);
+        return (double)rsi_ratio_f32(ag, al);
     }
     // seed sums (multiplier 1), times 1/p at fv+p, then beta
     static constexpr bool kWarmScan = true;
@@ -825,14 +837,6 @@ extern "C" __global__ __launch_bounds__(64) void adx_ms1p_f32(rec_args a) {
 // on 4,096 x 50,000, U = 32 (286 VGPRs) holds one wave.
 #define FUSED_U 8
 
-__device__ __forceinline__ float rsi_ratio_f32(double ag, double al) {
-    const double den = ag + al;
-    if (den == 0.0) return 50.0f;
-    const float d = (float)den;
-    float r = __builtin_amdgcn_rcpf(d);
-    r = fmaf(fmaf(-d, r, 1.0f), r, r);
-    return (float)(100.0 * ag) * r;
-}
 
 extern "C" __global__ __launch_bounds__(64) void ms_ema_rsi_atr_f32(rec_args a) {
     const int lane = threadIdx.x & 63;
