@@ -314,6 +314,45 @@ vta_status vta_tr_from_hlc_device(vta_ctx *ctx, const float *d_high, const float
 vta_status vta_atr_batch_device_with_tr(vta_ctx *ctx, const float *d_tr, const int32_t *d_periods,
                                         int32_t max_period, int32_t series_len,
                                         int32_t first_valid, int32_t n_combos, float *d_out);
+
+/* ------------------------------------------------- packed one-series batches */
+/* No counterpart in the reference, whose wrappers run one series per call.
+ * Many one-series batches of one recurrence indicator -- series of uneven
+ * lengths, each with its own period table -- in one launch per scan pass
+ * through a device job table (offsets of every job's blocks and combos).
+ * Each job's outputs are bit-identical to its own vta_<ind>_batch_device call
+ * on a context with the same tuning (same chunking, same kernels), except
+ * where that call would skip the scan for a grid of more than ~260k combos
+ * (the pack then runs the job exactly in one chunk: within the parity bound).
+ * vta_rec_pack_create validates the jobs like the _device forms, uploads the
+ * table and allocates the pack's own chunk-carry buffer and flags (so
+ * launches allocate nothing); vta_rec_pack_launch is enqueue-only (graph-
+ * capturable).  Job pointers are device pointers that must stay valid while
+ * the pack is used; a launch overwrites every job's output. */
+typedef enum vta_rec_kind {
+    VTA_REC_EMA = 0,
+    VTA_REC_RSI = 1,
+    VTA_REC_ATR = 2,    /* high, low, close */
+    VTA_REC_ADX = 3,    /* high, low, close */
+    VTA_REC_SMMA = 4,
+    VTA_REC_WILDERS = 5
+} vta_rec_kind;
+typedef struct vta_rec_job {
+    const float *d_close;     /* prices (EMA, RSI, SMMA, Wilders) or close (ATR, ADX) */
+    const float *d_high;      /* ATR, ADX; NULL otherwise */
+    const float *d_low;       /* ATR, ADX; NULL otherwise */
+    const int32_t *d_periods; /* n_combos periods */
+    float *d_out;             /* [n_combos][series_len] */
+    int32_t series_len;
+    int32_t first_valid;      /* of close (ATR, ADX: of high, low and close) */
+    int32_t n_combos;
+    int32_t max_period;
+} vta_rec_job;
+typedef struct vta_rec_pack vta_rec_pack;
+vta_status vta_rec_pack_create(vta_ctx *ctx, vta_rec_kind kind, const vta_rec_job *jobs,
+                               size_t n_jobs, vta_rec_pack **out);
+vta_status vta_rec_pack_launch(vta_ctx *ctx, const vta_rec_pack *pack);
+vta_status vta_rec_pack_destroy(vta_rec_pack *pack);
 vta_status vta_atr_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *high_tm,
                                                         const float *low_tm,
                                                         const float *close_tm, size_t cols,

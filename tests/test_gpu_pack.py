@@ -1,0 +1,149 @@
+"""Packed one-series batches (vta_rec_pack_*): many series of uneven lengths,
+each with its own period grid, in one launch per scan pass.  Every job's
+outputs must equal, bit for bit, its own vta_<ind>_batch_device call on the
+same context (same chunking, same kernels) -- the "same alone and in the mix"
+property -- for all six recurrence engines, with jobs that exercise one-chunk
+series (<= 64 bars), a late first_valid, a non-finite bar after first_valid
+(the exact path, run once in chunk 0 of the output pass) and grids from one
+period to more than one 64-combo group.  One job per indicator is also
+checked against the oracle, and a captured replay of a pack against its
+eager launch."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle as O
+import vectorta_amd as vta
+from vectorta_amd import datagen, lib
+from vectorta_amd._lib import check
+from tests.conftest import require_gpu
+from tests.helpers import max_rel_err, nan_mask_equal
+
+pytestmark = pytest.mark.gpu
+
+P = ctypes.c_void_p
+KINDS = ("ema", "rsi", "atr", "adx", "smma", "wilders")
+HLC = ("atr", "adx")
+
+# (series_len, leading NaN, period grid start/stop/step, interior non-finite bar)
+SHAPES = [(50, 0, (2, 10, 4), None),
+          (3_000, 0, (2, 60, 1), None),
+          (20_000, 1_500, (5, 400, 5), None),
+          (61_000, 3, (1, 1_300, 10), None),
+          (9_000, 2, (3, 40, 3), 4_000),
+          (700, 0, (14, 14, 0), None)]
+
+
+def _direct(kind, ctx, d, job):
+    L, h = lib(), ctx.handle
+    T, fv, C, mp = job["series_len"], job["first_valid"], job["n_combos"], job["max_period"]
+    out = P(d["direct"].ptr)
+    if kind == "ema":
+        check(L.vta_ema_batch_device(h, P(d["c"].ptr), P(d["p"].ptr), None, mp, T, fv, C, out))
+    elif kind in ("rsi", "smma", "wilders"):
+        fn = getattr(L, f"vta_{kind}_batch_device")
+        check(fn(h, P(d["c"].ptr), P(d["p"].ptr), mp, T, fv, C, out))
+    else:
+        fn = getattr(L, f"vta_{kind}_batch_device")
+        check(fn(h, P(d["h"].ptr), P(d["l"].ptr), P(d["c"].ptr), P(d["p"].ptr), mp, T, fv, C,
+                 out))
+
+
+def _oracle(kind, c, h, lo, periods):
+    f = [a.astype(np.float64) for a in (h, lo, c)]
+    if kind == "ema":
+        return O.ema_batch(f[2], periods)
+    if kind == "rsi":
+        return O.rsi_batch(f[2], periods)
+    if kind == "atr":
+        return O.atr_batch(*f, periods)
+    if kind == "adx":
+        return O.adx_batch(*f, periods)
+    fn = O.smma if kind == "smma" else O.wilders
+    return np.stack([fn(f[2], p) for p in periods])
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_pack_matches_direct_calls(kind):
+    require_gpu()
+    ctx = vta.HipContext(0)
+    rng = np.random.Generator(np.random.PCG64(90 + KINDS.index(kind)))
+    jobs, bufs, host = [], [], []
+    for i, (T, lead, (p0, p1, ps), bad) in enumerate(SHAPES):
+        c = datagen.gbm_close(T, 0.01, seed=500 + i)
+        _, hi, lo, c = datagen.ohlc_from_close(c, rng)
+        for a in (hi, lo, c):
+            a[:lead] = np.nan
+        if bad is not None:
+            c[bad] = np.nan
+        periods = np.arange(p0, p1 + 1, ps if ps else 1, dtype=np.int32)
+        if kind == "adx":
+            periods = np.maximum(periods, 2)
+        C = int(periods.size)
+        d = {"c": ctx.to_device(c), "h": ctx.to_device(hi), "l": ctx.to_device(lo),
+             "p": ctx.to_device(periods), "out": ctx.alloc(C * T * 4),
+             "direct": ctx.alloc(C * T * 4)}
+        job = {"close": d["c"], "periods": d["p"], "out": d["out"], "series_len": T,
+               "first_valid": lead, "n_combos": C, "max_period": int(periods.max())}
+        if kind in HLC:
+            job["high"], job["low"] = d["h"], d["l"]
+        jobs.append(job)
+        bufs.append(d)
+        host.append((c, hi, lo, [int(p) for p in periods]))
+    pack = vta.RecPack(ctx, kind, jobs)
+    pack.launch()
+    for job, d in zip(jobs, bufs):
+        _direct(kind, ctx, d, job)
+    ctx.synchronize()
+    got = []
+    for i, (job, d) in enumerate(zip(jobs, bufs)):
+        C, T = job["n_combos"], job["series_len"]
+        g = d["out"].copy_to_host(np.float32).reshape(C, T)
+        r = d["direct"].copy_to_host(np.float32).reshape(C, T)
+        assert np.array_equal(g.view(np.uint32), r.view(np.uint32)), (kind, i, SHAPES[i])
+        got.append(g)
+    # the oracle on the job with a late first_valid and on the unclean one
+    for i in (2, 4):
+        c, hi, lo, periods = host[i]
+        ref = _oracle(kind, c, hi, lo, periods)
+        nan_mask_equal(got[i], ref)
+        err = max_rel_err(got[i], ref, floor=1.0 if kind in ("rsi", "adx") else 0.0)
+        print(f"{kind} job {i}: max rel err {err:.3e}")
+        assert err <= (2e-5 if kind == "adx" else 1e-5), (kind, i, err)
+    # a captured replay of the pack equals its eager launch
+    for d in bufs:
+        check(lib().vta_memcpy_htod(ctx.handle, P(d["out"].ptr),
+                                    np.full(d["out"].nbytes // 4, -3.0, np.float32).ctypes.data_as(P),
+                                    d["out"].nbytes))
+    ctx.begin_capture()
+    pack.launch()
+    graph = ctx.end_capture()
+    graph.launch()
+    ctx.synchronize()
+    for i, (job, d) in enumerate(zip(jobs, bufs)):
+        g = d["out"].copy_to_host(np.float32).reshape(job["n_combos"], job["series_len"])
+        assert np.array_equal(g.view(np.uint32), got[i].view(np.uint32)), ("graph", kind, i)
+    graph.destroy()
+    pack.close()
+    for d in bufs:
+        for b in d.values():
+            b.free()
+
+
+def test_pack_rejects_bad_jobs():
+    require_gpu()
+    ctx = vta.HipContext(0)
+    x = ctx.to_device(np.ones(100, np.float32))
+    p = ctx.to_device(np.array([5], np.int32))
+    o = ctx.alloc(400)
+    good = {"close": x, "periods": p, "out": o, "series_len": 100, "first_valid": 0,
+            "n_combos": 1, "max_period": 5}
+    for bad in ({"first_valid": 100}, {"n_combos": 0}, {"series_len": 0}):
+        with pytest.raises(vta.VtaError):
+            vta.RecPack(ctx, "ema", [dict(good, **bad)])
+    with pytest.raises(vta.VtaError):
+        vta.RecPack(ctx, "atr", [good])          # ATR needs high and low
+    with pytest.raises(vta.VtaError):
+        vta.RecPack(ctx, "macd", [good])
+    vta.RecPack(ctx, "ema", [good]).close()

@@ -12,7 +12,7 @@ from .device import (DeviceArrayF32, DeviceBuffer, HipContext, HipGraph, cuda_av
                      cuda_device_count)
 from .fir import CudaSma, CudaSwma, CudaWma
 from .rec import (CudaAdx, CudaAtr, CudaEma, CudaRsi, CudaSmma, CudaWilders, ManySeriesOhlcPlan,
-                  ema_rsi_atr_many_series_dev)
+                  RecPack, ema_rsi_atr_many_series_dev)
 from .server import IndicatorServer, ServedResult
 from .rolling import (BollingerBandsBatchRange, CudaBollingerBands, CudaMediumAd,
                       CudaPercentileNearestRank, PercentileNearestRankBatchRange,
@@ -25,5 +25,5 @@ __all__ = [
     "ema_rsi_atr_many_series_dev", "BollingerBandsBatchRange", "CudaBollingerBands",
     "CudaMediumAd", "CudaPercentileNearestRank", "PercentileNearestRankBatchRange",
     "RollingManySeriesPlan", "DoubleAlmaCrossover", "DoubleAlmaPlan", "IndicatorServer",
-    "ServedResult", "CudaSma", "CudaWma", "CudaSwma", "CudaSmma", "CudaWilders",
+    "ServedResult", "CudaSma", "CudaWma", "CudaSwma", "CudaSmma", "CudaWilders", "RecPack",
 ]

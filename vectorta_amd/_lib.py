@@ -49,6 +49,15 @@ class AlmaBatchRangeC(ctypes.Structure):
     _fields_ = [("period", RangeUsize), ("offset", RangeF64), ("sigma", RangeF64)]
 
 
+class RecJobC(ctypes.Structure):
+    """vta_rec_job (include/vta.h)."""
+    _fields_ = [("d_close", ctypes.c_void_p), ("d_high", ctypes.c_void_p),
+                ("d_low", ctypes.c_void_p), ("d_periods", ctypes.c_void_p),
+                ("d_out", ctypes.c_void_p), ("series_len", ctypes.c_int32),
+                ("first_valid", ctypes.c_int32), ("n_combos", ctypes.c_int32),
+                ("max_period", ctypes.c_int32)]
+
+
 class BbParamsC(ctypes.Structure):
     _fields_ = [("period", ctypes.c_size_t), ("devup", ctypes.c_double),
                 ("devdn", ctypes.c_double)]
@@ -108,6 +117,10 @@ SIGNATURES = {
     "vta_autotune": (_ST, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int64), _SZ, _P, _P,
                            ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
                            ctypes.POINTER(ctypes.c_float)]),
+    "vta_rec_pack_create": (_ST, [_P, ctypes.c_int, ctypes.POINTER(RecJobC), _SZ,
+                                  ctypes.POINTER(_P)]),
+    "vta_rec_pack_launch": (_ST, [_P, _P]),
+    "vta_rec_pack_destroy": (_ST, [_P]),
     "vta_ctx_begin_capture": (_ST, [_P]),
     "vta_ctx_end_capture": (_ST, [_P, ctypes.POINTER(_P)]),
     "vta_graph_launch": (_ST, [_P, _P]),

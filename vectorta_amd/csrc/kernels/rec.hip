@@ -970,22 +970,56 @@ extern "C" __global__ __launch_bounds__(64) void ms_ema_rsi_atr_f32(rec_args a) 
 // stages 64 bars of the series in LDS (one coalesced load, then broadcast
 // reads), and buffers 64 bars x 64 combos of output in LDS so each output row
 // segment is written as one coalesced 256-byte store.
+// What one block of a one-series batch pass (or one wave of a fold) works
+// on: the series, the combos' periods and outputs, the chunking, and which
+// combo group and chunk this block is (for a fold: `group` = the combo).
+struct batch_view {
+    const float *x, *h, *l;
+    const int32_t *periods;
+    double *carry;           // [combo][chunk][states]
+    int T, nlanes, chunk, nch, phase, fv, group, k;
+    bool clean;
+};
+
+// The view of a block of a direct launch: grid.x = combo groups (a fold: the
+// combo), grid.y = chunks.
+__device__ __forceinline__ batch_view view_of(const rec_args &a) {
+    batch_view v;
+    v.x = a.x; v.h = a.h; v.l = a.l;
+    v.periods = a.periods;
+    v.carry = a.carry;
+    v.T = a.T;
+    v.nlanes = a.nlanes;
+    v.chunk = a.chunk;
+    v.nch = (a.T + a.chunk - 1) / a.chunk;
+    v.phase = a.phase;
+    v.fv = a.fvs[0];
+    v.group = blockIdx.x;
+    v.k = blockIdx.y;
+    v.clean = a.clean != nullptr && a.clean[0] != 0;
+    return v;
+}
+
 template <class E, int NIN>
-__device__ __forceinline__ void drive_batch(const rec_args &a, float *out) {
+__device__ __forceinline__ void drive_batch(const batch_view &a, float *out) {
     __shared__ __align__(16) float xs[3][64];
     // 64 x 65 output staging in dynamic LDS, launched only with the passes
     // that write rows (a pass without stores keeps more waves per CU)
     extern __shared__ float ob_dyn[];
     float (*ob)[65] = reinterpret_cast<float (*)[65]>(ob_dyn);
     const int lane = threadIdx.x & 63;
-    const int c = blockIdx.x * 64 + lane;
+    const int c = a.group * 64 + lane;
     const bool active = c < a.nlanes;
     const int T = a.T;
-    const int k = blockIdx.y;
-    const int c0 = k * a.chunk;
-    const int c1 = min(c0 + a.chunk, T);
-    const int fv = a.fvs[0];
-    const bool clean = a.clean != nullptr && a.clean[0] != 0;
+    const int k = a.k;
+    const bool clean = a.clean;
+    // a scan pass over an unclean series: its lanes run exactly from
+    // first_valid, so the output pass runs the whole series in chunk 0 and
+    // every other block of the scan has nothing to do
+    if (a.phase != 0 && !clean && (a.phase != 3 || k != 0)) return;
+    const int c0 = (a.phase != 0 && !clean) ? 0 : k * a.chunk;
+    const int c1 = (a.phase != 0 && !clean) ? T : min(c0 + a.chunk, T);
+    const int fv = a.fv;
     const int p = active ? a.periods[c] : 1;
     E e;
     e.setup(p, fv);
@@ -1015,7 +1049,7 @@ __device__ __forceinline__ void drive_batch(const rec_args &a, float *out) {
 #pragma unroll
     for (int i = 0; i < S; ++i) st0[i] = 0.0;
     if (carried && a.phase >= 2) {
-        const double *cs = a.carry + ((size_t)c * gridDim.y + (k - 1)) * S;
+        const double *cs = a.carry + ((size_t)c * a.nch + (k - 1)) * S;
         const int n = (E::kTwoStage && a.phase == 2) ? S - 1 : S;
 #pragma unroll
         for (int i = 0; i < S; ++i)
@@ -1197,9 +1231,9 @@ __device__ __forceinline__ void drive_batch(const rec_args &a, float *out) {
             if (tb + 64 > c0) {
                 const int t = tb + lane;
                 const bool ok = t >= c0 && t < c1;
-                const int nrow = min(64, a.nlanes - blockIdx.x * 64);
+                const int nrow = min(64, a.nlanes - a.group * 64);
                 for (int r = 0; r < nrow; ++r)
-                    if (ok) out[(size_t)(blockIdx.x * 64 + r) * T + t] = ob[r][lane];
+                    if (ok) out[(size_t)(a.group * 64 + r) * T + t] = ob[r][lane];
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1209,10 +1243,10 @@ __device__ __forceinline__ void drive_batch(const rec_args &a, float *out) {
     if constexpr (E::kScan > 0) {
         // chunk-end states: all of them after pass 1; after pass 2 (two-stage
         // engines) only the second stage's, next to the folded first stage
-        if ((a.phase == 1 || a.phase == 2) && active && k + 1 < (int)gridDim.y) {
+        if ((a.phase == 1 || a.phase == 2) && active && k + 1 < a.nch) {
             double st[S];
             e.get_state(c1 - 1, st);
-            double *cs = a.carry + ((size_t)c * gridDim.y + k) * S;
+            double *cs = a.carry + ((size_t)c * a.nch + k) * S;
             const int lo = a.phase == 2 ? S - 1 : 0;
 #pragma unroll
             for (int i = 0; i < S; ++i)
@@ -1233,12 +1267,12 @@ __device__ __forceinline__ void drive_batch(const rec_args &a, float *out) {
 // after it always has g = 0, so a g = 0 step selects b instead of forming
 // 0 * s (which is NaN for an undefined s).
 template <class E>
-__device__ __forceinline__ void scan_combine(const rec_args &a) {
-    const int c = blockIdx.x;
+__device__ __forceinline__ void scan_combine(const batch_view &a) {
+    const int c = a.group;                  // the combo
     const int lane = threadIdx.x & 63;
-    if (c >= a.nlanes || a.clean == nullptr || a.clean[0] == 0) return;
+    if (c >= a.nlanes || !a.clean) return;
     E e;
-    e.setup(a.periods[c], a.fvs[0]);
+    e.setup(a.periods[c], a.fv);
     constexpr int S = E::kScan > 0 ? E::kScan : 1;
     int lo = 0, hi = S;
     bool second = false;
@@ -1293,36 +1327,141 @@ __device__ __forceinline__ void scan_combine(const rec_args &a) {
 }
 
 extern "C" __global__ __launch_bounds__(64) void ema_batch_f32(rec_args a) {
-    drive_batch<EmaEng, 1>(a, a.out0);
+    drive_batch<EmaEng, 1>(view_of(a), a.out0);
 }
 extern "C" __global__ __launch_bounds__(64) void rsi_batch_f32(rec_args a) {
-    drive_batch<RsiEng, 1>(a, a.out0);
+    drive_batch<RsiEng, 1>(view_of(a), a.out0);
 }
 extern "C" __global__ __launch_bounds__(64) void atr_batch_f32(rec_args a) {
-    drive_batch<AtrEng, 3>(a, a.out0);
+    drive_batch<AtrEng, 3>(view_of(a), a.out0);
 }
 extern "C" __global__ __launch_bounds__(64) void smma_batch_f32(rec_args a) {
-    drive_batch<SmmaEng, 1>(a, a.out0);
+    drive_batch<SmmaEng, 1>(view_of(a), a.out0);
 }
 extern "C" __global__ __launch_bounds__(64) void wilders_batch_f32(rec_args a) {
-    drive_batch<WildersEng, 1>(a, a.out0);
+    drive_batch<WildersEng, 1>(view_of(a), a.out0);
 }
 extern "C" __global__ __launch_bounds__(64) void adx_batch_f32(rec_args a) {
-    drive_batch<AdxEng, 3>(a, a.out0);
+    drive_batch<AdxEng, 3>(view_of(a), a.out0);
 }
 extern "C" __global__ __launch_bounds__(64) void adx_scan_combine(rec_args a) {
-    scan_combine<AdxEng>(a);
+    scan_combine<AdxEng>(view_of(a));
 }
 extern "C" __global__ __launch_bounds__(64) void atr_tr_batch_f32(rec_args a) {
-    drive_batch<AtrTrEng, 1>(a, a.out0);
+    drive_batch<AtrTrEng, 1>(view_of(a), a.out0);
 }
 extern "C" __global__ __launch_bounds__(64) void atr_tr_scan_combine(rec_args a) {
-    scan_combine<AtrTrEng>(a);
+    scan_combine<AtrTrEng>(view_of(a));
 }
 
 // True range of one series (tr_from_hlc_f32 in the reference): 0 before
 // first_valid, high - low at first_valid, then max(h - l, |h - c[t-1]|,
 // |l - c[t-1]|), evaluated in f64 (exact for f32 inputs) and rounded once.
+// ---------------------------------------------------------------- packed batches
+//
+// Many one-series batches of one indicator in one launch per scan pass: job j
+// is a series (its own length, first_valid and output) with its own period
+// table.  A pass is a 1-D grid over every job's (combo group, chunk) blocks,
+// job j owning blocks [block0, block0 + groups * nch); a fold is a 1-D grid
+// over every job's combos, job j owning [combo0, combo0 + nlanes).  A block
+// finds its job by binary search over block0 (combo0), then runs exactly the
+// block of that job's direct launch, so a job's outputs are bit-identical
+// alone and packed.
+struct rec_job {
+    const float *x, *h, *l;
+    const int32_t *periods;
+    float *out;
+    double *carry;           // this job's [combo][chunk][states]
+    int T, nlanes, fv, chunk, nch, groups, block0, combo0;
+};
+
+struct rec_pack_args {
+    const rec_job *jobs;
+    const int32_t *flags;    // per job: 1 if a bar at or after first_valid is non-finite
+    int n_jobs;
+    int phase;
+};
+
+// The last job whose first block (combo) is <= flat; jobs are in order.
+__device__ __forceinline__ int pack_find(const rec_pack_args &p, int flat, bool combos) {
+    int lo = 0, hi = p.n_jobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const int first = combos ? p.jobs[mid].combo0 : p.jobs[mid].block0;
+        if (first <= flat) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ batch_view pack_view(const rec_pack_args &p, int j) {
+    const rec_job &jb = p.jobs[j];
+    batch_view v;
+    v.x = jb.x; v.h = jb.h; v.l = jb.l;
+    v.periods = jb.periods;
+    v.carry = jb.carry;
+    v.T = jb.T;
+    v.nlanes = jb.nlanes;
+    v.chunk = jb.chunk;
+    v.nch = jb.nch;
+    v.phase = p.phase;
+    v.fv = jb.fv;
+    v.clean = p.flags[j] == 0;
+    v.group = 0;
+    v.k = 0;
+    return v;
+}
+
+template <class E, int NIN>
+__device__ __forceinline__ void packed_pass(const rec_pack_args &p) {
+    const int flat = blockIdx.x;
+    const int j = pack_find(p, flat, false);
+    batch_view v = pack_view(p, j);
+    const int local = flat - p.jobs[j].block0;
+    const int groups = p.jobs[j].groups;
+    v.group = local % groups;
+    v.k = local / groups;
+    drive_batch<E, NIN>(v, p.jobs[j].out);
+}
+
+template <class E>
+__device__ __forceinline__ void packed_fold(const rec_pack_args &p) {
+    const int flat = blockIdx.x;
+    const int j = pack_find(p, flat, true);
+    batch_view v = pack_view(p, j);
+    v.group = flat - p.jobs[j].combo0;
+    scan_combine<E>(v);
+}
+
+#define REC_PACKED(NAME, ENG, NIN)                                                           \
+    extern "C" __global__ __launch_bounds__(64) void NAME##_packed_f32(rec_pack_args p) {    \
+        packed_pass<ENG, NIN>(p);                                                            \
+    }                                                                                        \
+    extern "C" __global__ __launch_bounds__(64) void NAME##_packed_fold(rec_pack_args p) {   \
+        packed_fold<ENG>(p);                                                                 \
+    }
+REC_PACKED(ema, EmaEng, 1)
+REC_PACKED(rsi, RsiEng, 1)
+REC_PACKED(atr, AtrEng, 3)
+REC_PACKED(adx, AdxEng, 3)
+REC_PACKED(smma, SmmaEng, 1)
+REC_PACKED(wilders, WildersEng, 1)
+
+// Per-job non-finite flags: grid (blocks, jobs); block (b, j) checks bars
+// first_valid + b*256 + tid, stepping by gridDim.x * 256.  flags[j] must be 0
+// on entry; every failing lane stores 1.
+extern "C" __global__ __launch_bounds__(256) void packed_nonfinite_flag(const rec_job *jobs,
+                                                                        int32_t *flags,
+                                                                        int hlc) {
+    const rec_job &jb = jobs[blockIdx.y];
+    bool bad = false;
+    for (long t = (long)jb.fv + (long)blockIdx.x * 256 + threadIdx.x; t < jb.T;
+         t += (long)gridDim.x * 256) {
+        bad = bad || !vta_isfinite(jb.x[t]);
+        if (hlc) bad = bad || !vta_isfinite(jb.h[t]) || !vta_isfinite(jb.l[t]);
+    }
+    if (bad) flags[blockIdx.y] = 1;
+}
+
 extern "C" __global__ __launch_bounds__(256) void tr_from_hlc_f32(
     const float *__restrict__ h, const float *__restrict__ l, const float *__restrict__ c, int T,
     int first_valid, float *__restrict__ tr) {
@@ -1335,19 +1474,19 @@ extern "C" __global__ __launch_bounds__(256) void tr_from_hlc_f32(
 }
 
 extern "C" __global__ __launch_bounds__(64) void ema_scan_combine(rec_args a) {
-    scan_combine<EmaEng>(a);
+    scan_combine<EmaEng>(view_of(a));
 }
 extern "C" __global__ __launch_bounds__(64) void rsi_scan_combine(rec_args a) {
-    scan_combine<RsiEng>(a);
+    scan_combine<RsiEng>(view_of(a));
 }
 extern "C" __global__ __launch_bounds__(64) void atr_scan_combine(rec_args a) {
-    scan_combine<AtrEng>(a);
+    scan_combine<AtrEng>(view_of(a));
 }
 extern "C" __global__ __launch_bounds__(64) void smma_scan_combine(rec_args a) {
-    scan_combine<SmmaEng>(a);
+    scan_combine<SmmaEng>(view_of(a));
 }
 extern "C" __global__ __launch_bounds__(64) void wilders_scan_combine(rec_args a) {
-    scan_combine<WildersEng>(a);
+    scan_combine<WildersEng>(view_of(a));
 }
 
 // ---------------------------------------------------------------- preparation

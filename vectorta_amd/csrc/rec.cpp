@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "runtime_internal.h"
@@ -180,6 +181,8 @@ vta_status launch_rec(vta_ctx *ctx, const char *kname, rec_args_host &a, unsigne
 // 1.14 / 1.24 / 1.09 ms at 2,048 / 4,096 / 8,192 waves; flat on small shapes.
 constexpr size_t kScanWaves = 8192;
 
+size_t scan_states(Ind k) { return k == IND_RSI ? 2 : k == IND_ADX ? 4 : 1; }
+
 const char *combine_kernel(Ind k) {
     static const char *n[] = {"ema_scan_combine",  "rsi_scan_combine",     "atr_scan_combine",
                               "adx_scan_combine",  "smma_scan_combine",    "wilders_scan_combine",
@@ -187,13 +190,14 @@ const char *combine_kernel(Ind k) {
     return n[k];
 }
 
-vta_status scan_launch(vta_ctx *ctx, Ind k, rec_args_host a, size_t T, size_t C, size_t max_p,
-                       bool *done) {
-    *done = false;
-    if (getenv("VTA_REC_NO_SCAN")) return VTA_OK;
+// The chunked scan's chunk length for T bars x C combos of indicator k (the
+// rule above), or 0 when the series is not scanned (one chunk would cover
+// it, or the carry buffer holds fewer than two chunk-end states per combo).
+size_t scan_chunk(const vta_ctx *ctx, Ind k, size_t T, size_t C, size_t max_p) {
+    if (getenv("VTA_REC_NO_SCAN")) return 0;
     // states per (chunk, combo): RSI 2; ADX 4 (three first-stage states and
     // the ADX itself, folded in two rounds); the others 1
-    const size_t S = k == IND_RSI ? 2 : k == IND_ADX ? 4 : 1;
+    const size_t S = scan_states(k);
     const size_t groups = (C + 63) / 64;
     size_t waves = kScanWaves;
     if (ctx->tune[VTA_TUNE_SCAN_WAVES] > 0) {
@@ -210,11 +214,19 @@ vta_status scan_launch(vta_ctx *ctx, Ind k, rec_args_host a, size_t T, size_t C,
         if (v >= 64) chunk = (size_t)v;
     }
     const size_t max_chunks = vta_ctx::kCarryBytes / (C * S * sizeof(double));
-    if (max_chunks < 2) return VTA_OK;
+    if (max_chunks < 2) return 0;
     if ((T + chunk - 1) / chunk > max_chunks) chunk = (T + max_chunks - 1) / max_chunks;
     chunk = (chunk + 63) & ~(size_t)63;
+    return (T + chunk - 1) / chunk < 2 ? 0 : chunk;
+}
+
+vta_status scan_launch(vta_ctx *ctx, Ind k, rec_args_host a, size_t T, size_t C, size_t max_p,
+                       bool *done) {
+    *done = false;
+    const size_t chunk = scan_chunk(ctx, k, T, C, max_p);
+    if (chunk == 0) return VTA_OK;
+    const size_t groups = (C + 63) / 64;
     const size_t nch = (T + chunk - 1) / chunk;
-    if (nch < 2) return VTA_OK;
     a.chunk = (int)chunk;
     a.carry = static_cast<double *>(ctx->carry);
     hipFunction_t fb, fc;
@@ -739,6 +751,157 @@ vta_status vta_ema_rsi_atr_many_series_dev(vta_ctx *ctx, const float *close_tm,
         out3[i].rows = rows;
         out3[i].cols = cols;
     }
+    return VTA_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- packed batches
+
+namespace {
+
+// Device job table entry (rec_job in rec.hip).
+struct rec_job_host {
+    const float *x, *h, *l;
+    const int32_t *periods;
+    float *out;
+    double *carry;
+    int T, nlanes, fv, chunk, nch, groups, block0, combo0;
+};
+static_assert(sizeof(rec_job_host) == 80, "rec_job layout");
+
+struct rec_pack_args_host {
+    const void *jobs;
+    const int32_t *flags;
+    int n_jobs;
+    int phase;
+};
+static_assert(sizeof(rec_pack_args_host) == 24, "rec_pack_args layout");
+
+const char *packed_kernel(Ind k, bool fold) {
+    static const char *p[] = {"ema_packed_f32", "rsi_packed_f32", "atr_packed_f32",
+                              "adx_packed_f32", "smma_packed_f32", "wilders_packed_f32"};
+    static const char *f[] = {"ema_packed_fold", "rsi_packed_fold", "atr_packed_fold",
+                              "adx_packed_fold", "smma_packed_fold", "wilders_packed_fold"};
+    return fold ? f[k] : p[k];
+}
+
+}  // namespace
+
+struct vta_rec_pack {
+    int device = 0;
+    Ind kind = IND_EMA;
+    int n_jobs = 0;
+    unsigned total_blocks = 0, total_combos = 0, flag_blocks = 1;
+    DevBuf jobs, flags, carry;
+};
+
+extern "C" {
+
+vta_status vta_rec_pack_create(vta_ctx *ctx, vta_rec_kind kind, const vta_rec_job *jobs,
+                               size_t n_jobs, vta_rec_pack **out) {
+    clear_error();
+    if (!ctx || !jobs || !out) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    *out = nullptr;
+    if ((int)kind < (int)VTA_REC_EMA || (int)kind > (int)VTA_REC_WILDERS)
+        return fail(VTA_ERR_INVALID_INPUT, "unknown indicator kind %d", (int)kind);
+    if (n_jobs == 0 || n_jobs > 65535)
+        return fail(VTA_ERR_INVALID_INPUT, "n_jobs must be in 1..65535 (got %zu)", n_jobs);
+    const Ind k = (Ind)kind;
+    const size_t S = scan_states(k);
+    std::vector<rec_job_host> tab(n_jobs);
+    std::vector<size_t> carry_off(n_jobs);
+    size_t blocks = 0, combos = 0, carry_elems = 0, span = 1;
+    for (size_t j = 0; j < n_jobs; ++j) {
+        const vta_rec_job &q = jobs[j];
+        if (!q.d_close || !q.d_periods || !q.d_out || (hlc(k) && (!q.d_high || !q.d_low)))
+            return fail(VTA_ERR_INVALID_INPUT, "job %zu: null pointer", j);
+        if (q.series_len <= 0 || q.n_combos <= 0 || q.max_period <= 0 || q.first_valid < 0 ||
+            q.first_valid >= q.series_len)
+            return fail(VTA_ERR_INVALID_INPUT,
+                        "job %zu: series_len, n_combos and max_period must be positive and "
+                        "first_valid < series_len", j);
+        const size_t T = (size_t)q.series_len, C = (size_t)q.n_combos;
+        // the chunking of the job's own _device call (one exact chunk where
+        // that call would not scan)
+        size_t chunk = scan_chunk(ctx, k, T, C, (size_t)q.max_period);
+        if (chunk == 0) chunk = T;
+        const size_t nch = (T + chunk - 1) / chunk;
+        const size_t groups = (C + 63) / 64;
+        rec_job_host &e = tab[j];
+        e.x = q.d_close;
+        e.h = hlc(k) ? q.d_high : nullptr;
+        e.l = hlc(k) ? q.d_low : nullptr;
+        e.periods = q.d_periods;
+        e.out = q.d_out;
+        e.carry = nullptr;                      // set once the buffer exists
+        e.T = (int)T;
+        e.nlanes = (int)C;
+        e.fv = q.first_valid;
+        e.chunk = (int)chunk;
+        e.nch = (int)nch;
+        e.groups = (int)groups;
+        e.block0 = (int)blocks;
+        e.combo0 = (int)combos;
+        carry_off[j] = carry_elems;
+        blocks += groups * nch;
+        combos += C;
+        carry_elems += C * nch * S;
+        span = std::max(span, T - (size_t)q.first_valid);
+        if (blocks > (size_t)INT32_MAX || combos > (size_t)INT32_MAX)
+            return fail(VTA_ERR_LAUNCH_CONFIG_TOO_LARGE, "packed grid exceeds 2^31 blocks");
+    }
+    auto pk = std::make_unique<vta_rec_pack>();
+    pk->device = ctx->device;
+    pk->kind = k;
+    pk->n_jobs = (int)n_jobs;
+    pk->total_blocks = (unsigned)blocks;
+    pk->total_combos = (unsigned)combos;
+    pk->flag_blocks = (unsigned)std::min<size_t>((span + 255) / 256, 1024);
+    VTA_HIP(hipSetDevice(ctx->device));
+    VTA_TRY(pk->carry.alloc(carry_elems * sizeof(double)));
+    VTA_TRY(pk->flags.alloc(n_jobs * sizeof(int32_t)));
+    for (size_t j = 0; j < n_jobs; ++j) tab[j].carry = pk->carry.as<double>() + carry_off[j];
+    VTA_TRY(upload(ctx, pk->jobs, tab.data(), n_jobs * sizeof(rec_job_host)));
+    *out = pk.release();
+    return VTA_OK;
+}
+
+vta_status vta_rec_pack_launch(vta_ctx *ctx, const vta_rec_pack *pk) {
+    clear_error();
+    if (!ctx || !pk) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    if (pk->device != ctx->device)
+        return fail(VTA_ERR_DEVICE_MISMATCH, "pack built on device %d, context on %d", pk->device,
+                    ctx->device);
+    const Ind k = pk->kind;
+    hipFunction_t fill, flag, fp, ff;
+    VTA_TRY(get_function(ctx, "rec", "vta_fill_i32", &fill));
+    VTA_TRY(get_function(ctx, "rec", "packed_nonfinite_flag", &flag));
+    VTA_TRY(get_function(ctx, "rec", packed_kernel(k, false), &fp));
+    VTA_TRY(get_function(ctx, "rec", packed_kernel(k, true), &ff));
+    int32_t *flags = pk->flags.as<int32_t>();
+    const void *jobs = pk->jobs.p;
+    VTA_TRY(launch(ctx, fill, dim3((unsigned)((pk->n_jobs + 255) / 256)), dim3(256), 0, flags,
+                   pk->n_jobs, 0));
+    VTA_TRY(launch(ctx, flag, dim3(pk->flag_blocks, (unsigned)pk->n_jobs), dim3(256), 0, jobs,
+                   flags, hlc(k) ? 1 : 0));
+    rec_pack_args_host a{jobs, flags, pk->n_jobs, 1};
+    VTA_TRY(launch(ctx, fp, dim3(pk->total_blocks), dim3(64), 0, a));
+    VTA_TRY(launch(ctx, ff, dim3(pk->total_combos), dim3(64), 0, a));
+    if (k == IND_ADX) {
+        a.phase = 2;
+        VTA_TRY(launch(ctx, fp, dim3(pk->total_blocks), dim3(64), 0, a));
+        VTA_TRY(launch(ctx, ff, dim3(pk->total_combos), dim3(64), 0, a));
+    }
+    a.phase = 3;
+    VTA_TRY(launch(ctx, fp, dim3(pk->total_blocks), dim3(64), kBatchStageBytes, a));
+    return VTA_OK;
+}
+
+vta_status vta_rec_pack_destroy(vta_rec_pack *pk) {
+    if (!pk) return VTA_OK;
+    (void)hipSetDevice(pk->device);
+    delete pk;
     return VTA_OK;
 }
 

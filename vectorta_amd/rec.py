@@ -9,7 +9,7 @@ from typing import List, Sequence, Tuple
 
 import numpy as np
 
-from ._lib import DeviceArrayF32C, RangeUsize, check, lib
+from ._lib import DeviceArrayF32C, RangeUsize, RecJobC, VtaError, check, lib
 from .device import DeviceArrayF32, DeviceBuffer, HipContext, f32_ptr
 
 
@@ -239,3 +239,48 @@ class ManySeriesOhlcPlan:
 
     def outputs(self):
         return [b.copy_to_host().reshape(self.rows, self.cols) for b in self.d_out]
+
+
+REC_KINDS = {"ema": 0, "rsi": 1, "atr": 2, "adx": 3, "smma": 4, "wilders": 5}
+
+
+class RecPack:
+    """Many one-series batches of one recurrence indicator run together: one
+    launch per scan pass over a device job table (vta_rec_pack_*).  No
+    reference counterpart (its wrappers run one series per call).
+
+    jobs: dicts with device buffers ``close`` (and ``high``, ``low`` for ATR /
+    ADX), ``periods`` (int32), ``out`` (n_combos x series_len f32) and the ints
+    ``series_len``, ``first_valid``, ``n_combos``, ``max_period``.  The
+    buffers must outlive the pack; ``launch()`` is enqueue-only."""
+
+    def __init__(self, ctx: HipContext, kind: str, jobs: Sequence[dict]):
+        if kind not in REC_KINDS:
+            raise VtaError(4, f"unknown indicator kind {kind!r}")
+        self.ctx = ctx
+        self.kind = kind
+        arr = (RecJobC * len(jobs))()
+        for i, j in enumerate(jobs):
+            arr[i] = RecJobC(j["close"].ptr, j["high"].ptr if j.get("high") else None,
+                             j["low"].ptr if j.get("low") else None, j["periods"].ptr,
+                             j["out"].ptr, int(j["series_len"]), int(j["first_valid"]),
+                             int(j["n_combos"]), int(j["max_period"]))
+        self._p = ctypes.c_void_p()
+        check(lib().vta_rec_pack_create(ctx.handle, REC_KINDS[kind], arr, len(jobs),
+                                        ctypes.byref(self._p)))
+
+    def launch(self) -> None:
+        if not self._p:
+            raise VtaError(4, "pack is destroyed")
+        check(lib().vta_rec_pack_launch(self.ctx.handle, self._p))
+
+    def close(self) -> None:
+        if self._p:
+            lib().vta_rec_pack_destroy(self._p)
+            self._p = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
