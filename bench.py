@@ -40,6 +40,7 @@ MFMA_VARIANT_KERNEL = {1: "alma_batch_mfma_bf16x3_wg16", 2: "alma_batch_mfma_bf1
                        5: "alma_batch_mfma_bf16x3_r8", 6: "alma_batch_mfma_bf16x3",
                        7: "alma_batch_mfma_bf16x3_r32"}
 
+
 def builtin_mfma_variant(T, C):
     """The library's built-in variant for a T-bar x C-combo sweep (its rule
     in alma.cpp, mf_default_variant)."""
@@ -315,6 +316,64 @@ def bench_captured_batch(steps, warmup, ev):
     for bufs in [d_p, d_bp, d_bu, d_bd] + [b for s in series for b in s] + \
             [b for o in outs for b in o]:
         bufs.free()
+    return res
+
+
+def bench_packed_batch(steps, warmup, ev):
+    """64 one-series requests of one indicator (lengths log-uniform 1k-50k
+    bars, grids log-uniform 1-128 periods, seeded), inputs resident: one
+    vta_<ind>_batch_device call per request (back to back on one stream) vs
+    one packed launch (vta_rec_pack_launch).  Outputs are bit-identical; both
+    are timed between two HIP events over `steps` repetitions."""
+    L = lib()
+    P = ctypes.c_void_p
+    rng = np.random.Generator(np.random.PCG64(31))
+    n = 64
+    lens = np.exp(rng.uniform(np.log(1_000), np.log(50_000), n)).astype(int)
+    grids = np.exp(rng.uniform(0, np.log(128), n)).astype(int)
+    res = {"workload": "packed_batch_64_requests_1k_50k_bars", "requests": n,
+           "outputs_per_batch": int((lens * grids).sum())}
+    for kind in ("ema", "adx"):
+        ctx = vta.HipContext(0)
+        jobs, bufs = [], []
+        for i in range(n):
+            T, C = int(lens[i]), int(grids[i])
+            c = datagen.gbm_close(T, 0.01, seed=600 + i)
+            _, h, lo, c = datagen.ohlc_from_close(c, rng)
+            per = np.arange(2, 2 + C, dtype=np.int32) * 2
+            d = {"c": ctx.to_device(c), "h": ctx.to_device(h), "l": ctx.to_device(lo),
+                 "p": ctx.to_device(per), "out": ctx.alloc(C * T * 4)}
+            job = {"close": d["c"], "periods": d["p"], "out": d["out"], "series_len": T,
+                   "first_valid": 0, "n_combos": C, "max_period": int(per.max())}
+            if kind == "adx":
+                job["high"], job["low"] = d["h"], d["l"]
+            jobs.append(job)
+            bufs.append(d)
+        pack = vta.RecPack(ctx, kind, jobs)
+
+        def direct():
+            for j, d in zip(jobs, bufs):
+                if kind == "ema":
+                    check(L.vta_ema_batch_device(ctx.handle, P(d["c"].ptr), P(d["p"].ptr), None,
+                                                 j["max_period"], j["series_len"], 0,
+                                                 j["n_combos"], P(d["out"].ptr)))
+                else:
+                    check(L.vta_adx_batch_device(ctx.handle, P(d["h"].ptr), P(d["l"].ptr),
+                                                 P(d["c"].ptr), P(d["p"].ptr), j["max_period"],
+                                                 j["series_len"], 0, j["n_combos"],
+                                                 P(d["out"].ptr)))
+        r = {}
+        for name, fn in (("direct_calls", direct), ("packed", pack.launch)):
+            mean, ks, wall = _time_launches(fn, ctx.stream, steps, warmup, ev,
+                                            sync=ctx.synchronize)
+            r[name] = {"gpu_ms_mean": mean, "gpu_ms_p50": float(np.percentile(ks, 50)),
+                       "wall_ms_per_batch": wall}
+        r["speedup"] = r["direct_calls"]["gpu_ms_mean"] / r["packed"]["gpu_ms_mean"]
+        res[kind] = r
+        pack.close()
+        for d in bufs:
+            for b in d.values():
+                b.free()
     return res
 
 
@@ -659,7 +718,7 @@ def main():
                          "single ALMA)")
     ap.add_argument("--mode", default="sweep",
                     choices=["sweep", "calib", "ohlc", "rolling", "cross", "serve", "ma", "rec",
-                             "graph", "tune"],
+                             "graph", "tune", "pack"],
                     help="calib: 256 MiB device-to-device copies, the known-size reference "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
@@ -684,6 +743,10 @@ def main():
         return
     if args.mode == "rec":
         r = bench_recurrence_batches(args.steps, args.warmup, HipEvents())
+        print(json.dumps(r))
+        return
+    if args.mode == "pack":
+        r = bench_packed_batch(args.steps, args.warmup, HipEvents())
         print(json.dumps(r))
         return
     if args.mode == "tune":
@@ -811,6 +874,7 @@ def main():
                          ("double_alma_crossover", lambda: bench_double_alma(5, 1, ev)),
                          ("serving", lambda: bench_serving()),
                          ("captured_batch", lambda: bench_captured_batch(20, 3, ev)),
+                         ("packed_batch", lambda: bench_packed_batch(10, 2, ev)),
                          ("alma_single", lambda: bench_alma_single(20, ev))):
             extra[name] = fn()
             print(f"[bench] {name}: {extra[name]}", file=sys.stderr, flush=True)
