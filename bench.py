@@ -645,7 +645,8 @@ def bench_serving(rates=(200, 1000, 2000, 5000), duration_s=4.0, n_slots=8, seed
             "latency_ms_p90": float(np.percentile(lat, 90)),
             "achieved_req_per_s": len(ok) / span, "latency_ms_p50": float(np.percentile(lat, 50)),
             "latency_ms_p99": float(np.percentile(lat, 99)), "outputs_per_s": vals / span,
-            "peak_in_flight": st["peak_in_flight"], "peak_hbm_in_use_GB": st["peak_in_use"] / 1e9})
+            "peak_in_flight": st["peak_in_flight"], "peak_hbm_in_use_GB": st["peak_in_use"] / 1e9,
+            "packs": st["packs"], "packed_requests": st["packed_requests"]})
         print(f"[bench] serving {rate} req/s: {out['rates'][-1]}", file=sys.stderr, flush=True)
     return out
 

@@ -128,3 +128,41 @@ def test_budget_admission_and_cache(pool):
     assert r.status == 4
     srv.release(rid)
     srv.close()
+
+
+def test_small_requests_are_packed(pool):
+    """Small EMA / RSI / ADX requests queued behind others are served together
+    in packed launches (one launch per scan pass for the group); every result
+    equals its direct entry point bit for bit, packed or not."""
+    c, h, lo = pool
+    rng = np.random.Generator(np.random.PCG64(83))
+    srv = vta.IndicatorServer(0, n_slots=1, hbm_budget=4 << 30)
+    specs, ids = [], []
+    for i in range(36):
+        kind = ("ema", "rsi", "adx")[i % 3]
+        s = int(rng.integers(4))
+        L = int(rng.integers(2_000, 20_000))
+        a = int(rng.integers(0, 120_000 - L))
+        p0 = int(rng.integers(2, 10))
+        period = (p0, p0 + int(rng.integers(0, 40)), 1)
+        x, hh, ll = c[s, a:a + L], h[s, a:a + L], lo[s, a:a + L]
+        if kind == "adx":
+            ids.append(srv.submit(kind, x, hh, ll, period=period))
+        else:
+            ids.append(srv.submit(kind, x, period=period))
+        specs.append((kind, x, hh, ll, period))
+    direct = {"ema": vta.CudaEma(0), "rsi": vta.CudaRsi(0), "adx": vta.CudaAdx(0)}
+    for rid, (kind, x, hh, ll, period) in zip(ids, specs):
+        r = srv.wait(rid)
+        assert r.ok, (kind, r.status)
+        got = srv.to_host(r)
+        srv.release(rid)
+        if kind == "adx":
+            ref = direct[kind].adx_batch_dev(hh, ll, x, period).to_host()
+        else:
+            ref = getattr(direct[kind], f"{kind}_batch_dev")(x, period).to_host()
+        assert np.array_equal(_bits(got), _bits(ref)), (kind, len(x), period)
+    st = srv.stats()
+    srv.close()
+    print(f"packs {st['packs']}, packed requests {st['packed_requests']}")
+    assert st["packs"] >= 1 and st["packed_requests"] >= 2 * st["packs"], st

@@ -92,7 +92,8 @@ class VtaResultC(ctypes.Structure):
 
 class VtaServerStatsC(ctypes.Structure):
     _fields_ = [(n, ctypes.c_size_t) for n in ("budget", "in_use", "peak_in_use", "queued",
-                                                "in_flight", "peak_in_flight", "cached_series")]
+                                                "in_flight", "peak_in_flight", "cached_series",
+                                                "packs", "packed_requests")]
 
 
 # Every symbol include/vta.h declares, with (restype, argtypes).

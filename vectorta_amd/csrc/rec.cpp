@@ -758,17 +758,9 @@ vta_status vta_ema_rsi_atr_many_series_dev(vta_ctx *ctx, const float *close_tm,
 
 // ---------------------------------------------------------------- packed batches
 
-namespace {
+static_assert(sizeof(PackJob) == 80, "rec_job layout");
 
-// Device job table entry (rec_job in rec.hip).
-struct rec_job_host {
-    const float *x, *h, *l;
-    const int32_t *periods;
-    float *out;
-    double *carry;
-    int T, nlanes, fv, chunk, nch, groups, block0, combo0;
-};
-static_assert(sizeof(rec_job_host) == 80, "rec_job layout");
+namespace {
 
 struct rec_pack_args_host {
     const void *jobs;
@@ -788,30 +780,22 @@ const char *packed_kernel(Ind k, bool fold) {
 
 }  // namespace
 
-struct vta_rec_pack {
-    int device = 0;
-    Ind kind = IND_EMA;
-    int n_jobs = 0;
-    unsigned total_blocks = 0, total_combos = 0, flag_blocks = 1;
-    DevBuf jobs, flags, carry;
-};
+namespace vta {
 
-extern "C" {
-
-vta_status vta_rec_pack_create(vta_ctx *ctx, vta_rec_kind kind, const vta_rec_job *jobs,
-                               size_t n_jobs, vta_rec_pack **out) {
-    clear_error();
-    if (!ctx || !jobs || !out) return fail(VTA_ERR_INVALID_INPUT, "null argument");
-    *out = nullptr;
-    if ((int)kind < (int)VTA_REC_EMA || (int)kind > (int)VTA_REC_WILDERS)
-        return fail(VTA_ERR_INVALID_INPUT, "unknown indicator kind %d", (int)kind);
+vta_status pack_layout(const vta_ctx *ctx, int kind, const vta_rec_job *jobs, size_t n_jobs,
+                       std::vector<PackJob> &tab, std::vector<size_t> &carry_off,
+                       size_t *carry_elems, PackDims *dims) {
+    if (!ctx || !jobs || !dims || !carry_elems)
+        return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    if (kind < (int)VTA_REC_EMA || kind > (int)VTA_REC_WILDERS)
+        return fail(VTA_ERR_INVALID_INPUT, "unknown indicator kind %d", kind);
     if (n_jobs == 0 || n_jobs > 65535)
         return fail(VTA_ERR_INVALID_INPUT, "n_jobs must be in 1..65535 (got %zu)", n_jobs);
     const Ind k = (Ind)kind;
     const size_t S = scan_states(k);
-    std::vector<rec_job_host> tab(n_jobs);
-    std::vector<size_t> carry_off(n_jobs);
-    size_t blocks = 0, combos = 0, carry_elems = 0, span = 1;
+    tab.assign(n_jobs, PackJob{});
+    carry_off.assign(n_jobs, 0);
+    size_t blocks = 0, combos = 0, elems = 0, span = 1;
     for (size_t j = 0; j < n_jobs; ++j) {
         const vta_rec_job &q = jobs[j];
         if (!q.d_close || !q.d_periods || !q.d_out || (hlc(k) && (!q.d_high || !q.d_low)))
@@ -828,7 +812,7 @@ vta_status vta_rec_pack_create(vta_ctx *ctx, vta_rec_kind kind, const vta_rec_jo
         if (chunk == 0) chunk = T;
         const size_t nch = (T + chunk - 1) / chunk;
         const size_t groups = (C + 63) / 64;
-        rec_job_host &e = tab[j];
+        PackJob &e = tab[j];
         e.x = q.d_close;
         e.h = hlc(k) ? q.d_high : nullptr;
         e.l = hlc(k) ? q.d_low : nullptr;
@@ -843,26 +827,73 @@ vta_status vta_rec_pack_create(vta_ctx *ctx, vta_rec_kind kind, const vta_rec_jo
         e.groups = (int)groups;
         e.block0 = (int)blocks;
         e.combo0 = (int)combos;
-        carry_off[j] = carry_elems;
+        carry_off[j] = elems;
         blocks += groups * nch;
         combos += C;
-        carry_elems += C * nch * S;
+        elems += C * nch * S;
         span = std::max(span, T - (size_t)q.first_valid);
         if (blocks > (size_t)INT32_MAX || combos > (size_t)INT32_MAX)
             return fail(VTA_ERR_LAUNCH_CONFIG_TOO_LARGE, "packed grid exceeds 2^31 blocks");
     }
+    dims->kind = kind;
+    dims->n_jobs = (int)n_jobs;
+    dims->total_blocks = (unsigned)blocks;
+    dims->total_combos = (unsigned)combos;
+    dims->flag_blocks = (unsigned)std::min<size_t>((span + 255) / 256, 1024);
+    *carry_elems = elems;
+    return VTA_OK;
+}
+
+vta_status pack_launch(vta_ctx *ctx, const PackDims &d, const void *d_jobs, int32_t *d_flags) {
+    const Ind k = (Ind)d.kind;
+    hipFunction_t fill, flag, fp, ff;
+    VTA_TRY(get_function(ctx, "rec", "vta_fill_i32", &fill));
+    VTA_TRY(get_function(ctx, "rec", "packed_nonfinite_flag", &flag));
+    VTA_TRY(get_function(ctx, "rec", packed_kernel(k, false), &fp));
+    VTA_TRY(get_function(ctx, "rec", packed_kernel(k, true), &ff));
+    VTA_TRY(launch(ctx, fill, dim3((unsigned)((d.n_jobs + 255) / 256)), dim3(256), 0, d_flags,
+                   d.n_jobs, 0));
+    VTA_TRY(launch(ctx, flag, dim3(d.flag_blocks, (unsigned)d.n_jobs), dim3(256), 0, d_jobs,
+                   d_flags, hlc(k) ? 1 : 0));
+    rec_pack_args_host a{d_jobs, d_flags, d.n_jobs, 1};
+    VTA_TRY(launch(ctx, fp, dim3(d.total_blocks), dim3(64), 0, a));
+    VTA_TRY(launch(ctx, ff, dim3(d.total_combos), dim3(64), 0, a));
+    if (k == IND_ADX) {
+        a.phase = 2;
+        VTA_TRY(launch(ctx, fp, dim3(d.total_blocks), dim3(64), 0, a));
+        VTA_TRY(launch(ctx, ff, dim3(d.total_combos), dim3(64), 0, a));
+    }
+    a.phase = 3;
+    VTA_TRY(launch(ctx, fp, dim3(d.total_blocks), dim3(64), kBatchStageBytes, a));
+    return VTA_OK;
+}
+
+}  // namespace vta
+
+struct vta_rec_pack {
+    int device = 0;
+    PackDims dims{};
+    DevBuf jobs, flags, carry;
+};
+
+extern "C" {
+
+vta_status vta_rec_pack_create(vta_ctx *ctx, vta_rec_kind kind, const vta_rec_job *jobs,
+                               size_t n_jobs, vta_rec_pack **out) {
+    clear_error();
+    if (!ctx || !jobs || !out) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    *out = nullptr;
+    std::vector<PackJob> tab;
+    std::vector<size_t> off;
+    size_t elems = 0;
     auto pk = std::make_unique<vta_rec_pack>();
+    VTA_TRY(pack_layout(ctx, (int)kind, jobs, n_jobs, tab, off, &elems, &pk->dims));
     pk->device = ctx->device;
-    pk->kind = k;
-    pk->n_jobs = (int)n_jobs;
-    pk->total_blocks = (unsigned)blocks;
-    pk->total_combos = (unsigned)combos;
-    pk->flag_blocks = (unsigned)std::min<size_t>((span + 255) / 256, 1024);
     VTA_HIP(hipSetDevice(ctx->device));
-    VTA_TRY(pk->carry.alloc(carry_elems * sizeof(double)));
+    VTA_TRY(pk->carry.alloc(std::max<size_t>(elems, 1) * sizeof(double)));
     VTA_TRY(pk->flags.alloc(n_jobs * sizeof(int32_t)));
-    for (size_t j = 0; j < n_jobs; ++j) tab[j].carry = pk->carry.as<double>() + carry_off[j];
-    VTA_TRY(upload(ctx, pk->jobs, tab.data(), n_jobs * sizeof(rec_job_host)));
+    for (size_t j = 0; j < n_jobs; ++j) tab[j].carry = pk->carry.as<double>() + off[j];
+    VTA_TRY(upload(ctx, pk->jobs, tab.data(), n_jobs * sizeof(PackJob)));
     *out = pk.release();
     return VTA_OK;
 }
@@ -873,29 +904,7 @@ vta_status vta_rec_pack_launch(vta_ctx *ctx, const vta_rec_pack *pk) {
     if (pk->device != ctx->device)
         return fail(VTA_ERR_DEVICE_MISMATCH, "pack built on device %d, context on %d", pk->device,
                     ctx->device);
-    const Ind k = pk->kind;
-    hipFunction_t fill, flag, fp, ff;
-    VTA_TRY(get_function(ctx, "rec", "vta_fill_i32", &fill));
-    VTA_TRY(get_function(ctx, "rec", "packed_nonfinite_flag", &flag));
-    VTA_TRY(get_function(ctx, "rec", packed_kernel(k, false), &fp));
-    VTA_TRY(get_function(ctx, "rec", packed_kernel(k, true), &ff));
-    int32_t *flags = pk->flags.as<int32_t>();
-    const void *jobs = pk->jobs.p;
-    VTA_TRY(launch(ctx, fill, dim3((unsigned)((pk->n_jobs + 255) / 256)), dim3(256), 0, flags,
-                   pk->n_jobs, 0));
-    VTA_TRY(launch(ctx, flag, dim3(pk->flag_blocks, (unsigned)pk->n_jobs), dim3(256), 0, jobs,
-                   flags, hlc(k) ? 1 : 0));
-    rec_pack_args_host a{jobs, flags, pk->n_jobs, 1};
-    VTA_TRY(launch(ctx, fp, dim3(pk->total_blocks), dim3(64), 0, a));
-    VTA_TRY(launch(ctx, ff, dim3(pk->total_combos), dim3(64), 0, a));
-    if (k == IND_ADX) {
-        a.phase = 2;
-        VTA_TRY(launch(ctx, fp, dim3(pk->total_blocks), dim3(64), 0, a));
-        VTA_TRY(launch(ctx, ff, dim3(pk->total_combos), dim3(64), 0, a));
-    }
-    a.phase = 3;
-    VTA_TRY(launch(ctx, fp, dim3(pk->total_blocks), dim3(64), kBatchStageBytes, a));
-    return VTA_OK;
+    return pack_launch(ctx, pk->dims, pk->jobs.p, pk->flags.as<int32_t>());
 }
 
 vta_status vta_rec_pack_destroy(vta_rec_pack *pk) {

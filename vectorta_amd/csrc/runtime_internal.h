@@ -136,6 +136,29 @@ vta_status alma_grid_plan_launch(vta_ctx *ctx, const vta_alma_sweep_plan *plan,
                                  float *d_out);
 void alma_grid_plan_destroy(vta_alma_sweep_plan *plan);
 
+// Packed one-series recurrence batches (rec.cpp), for callers that bring their
+// own device memory (the request server's arena): the job table (device
+// layout of rec_job in rec.hip) and launch dimensions of a pack, and the
+// enqueue-only launch over a table already on the device.
+struct PackJob {
+    const float *x, *h, *l;
+    const int32_t *periods;
+    float *out;
+    double *carry;
+    int T, nlanes, fv, chunk, nch, groups, block0, combo0;
+};
+struct PackDims {
+    int kind = 0, n_jobs = 0;
+    unsigned total_blocks = 0, total_combos = 0, flag_blocks = 1;
+};
+// Validates the jobs like the _device forms and lays out the table; each
+// job's carry pointer is left null, its offset (in doubles) in carry_off.
+vta_status pack_layout(const vta_ctx *ctx, int kind, const vta_rec_job *jobs, size_t n_jobs,
+                       std::vector<PackJob> &tab, std::vector<size_t> &carry_off,
+                       size_t *carry_elems, PackDims *dims);
+// d_flags: n_jobs int32 words (cleared by the launch itself).
+vta_status pack_launch(vta_ctx *ctx, const PackDims &d, const void *d_jobs, int32_t *d_flags);
+
 // bf16 round-to-nearest-even of an f32 (bit pattern).
 uint16_t bf16_rne(float f);
 float bf16_to_f32(uint16_t h);

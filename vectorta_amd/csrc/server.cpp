@@ -158,6 +158,7 @@ struct vta_server {
     size_t in_use = 0;          // outputs of live results + cache + pending uploads
     size_t peak_in_use = 0;
     int in_flight = 0, peak_in_flight = 0;
+    size_t packs = 0, packed_requests = 0;
     bool stop = false;
     uint64_t next_id = 1;
     std::mutex mu;
@@ -233,15 +234,49 @@ void evict_lru(vta_server *s, size_t need) {
     }
 }
 
+// Called with s->mu held: admits the queued request at `it` if the HBM budget
+// holds its outputs (and its series, on a cache miss), evicting cached series
+// if needed.  Returns it (removed from the queue) or null.
+std::shared_ptr<Req> admit_at(vta_server *s, std::deque<std::shared_ptr<Req>>::iterator it) {
+    std::shared_ptr<Req> q = *it;
+    auto hit = q->r.series_key ? s->cache.find(q->r.series_key) : s->cache.end();
+    const bool is_hit = hit != s->cache.end();
+    const size_t series = is_hit ? 0 : q->series_bytes;
+    const size_t need = q->out_bytes + series;
+    if (s->in_use + need > s->budget) evict_lru(s, need);
+    if (s->in_use + need > s->budget) return nullptr;
+    s->queue.erase(it);
+    s->in_use += need;
+    s->peak_in_use = std::max(s->peak_in_use, s->in_use);
+    q->series_reserved = series;
+    q->uploader = !is_hit;
+    if (is_hit) {
+        q->entry = hit->second;
+        q->entry->refs += 1;
+        s->lru.splice(s->lru.end(), s->lru, q->entry->lru);   // most recently used
+    } else {
+        auto e = std::make_shared<CacheEntry>();
+        e->key = q->r.series_key;
+        e->meta = q->meta;
+        e->bytes = q->series_bytes;
+        e->refs = 1;
+        e->cached = q->r.cache && q->r.series_key != 0;
+        if (e->cached) {
+            s->lru.push_back(e->key);
+            e->lru = std::prev(s->lru.end());
+            s->cache.emplace(e->key, e);
+        }
+        q->entry = e;
+    }
+    return q;
+}
+
 // Called with s->mu held.  Returns the admitted request or null.
 std::shared_ptr<Req> try_admit(vta_server *s) {
     while (!s->queue.empty()) {
         std::shared_ptr<Req> q = s->queue.front();
-        auto hit = q->r.series_key ? s->cache.find(q->r.series_key) : s->cache.end();
-        const bool is_hit = hit != s->cache.end();
-        const size_t series = is_hit ? 0 : q->series_bytes;
-        const size_t need = q->out_bytes + series;
-        if (need > s->budget) {
+        const bool is_hit = q->r.series_key && s->cache.count(q->r.series_key);
+        if (q->out_bytes + (is_hit ? 0 : q->series_bytes) > s->budget) {
             // can never fit: fail it now (nothing was reserved for it)
             s->queue.pop_front();
             q->out_bytes = 0;
@@ -251,32 +286,34 @@ std::shared_ptr<Req> try_admit(vta_server *s) {
             s->cv_done.notify_all();
             continue;
         }
-        if (s->in_use + need > s->budget) evict_lru(s, need);
-        if (s->in_use + need > s->budget) return nullptr;
-        s->queue.pop_front();
-        s->in_use += need;
-        s->peak_in_use = std::max(s->peak_in_use, s->in_use);
-        q->series_reserved = series;
-        q->uploader = !is_hit;
-        if (is_hit) {
-            q->entry = hit->second;
-            q->entry->refs += 1;
-            s->lru.splice(s->lru.end(), s->lru, q->entry->lru);   // most recently used
-        } else {
-            auto e = std::make_shared<CacheEntry>();
-            e->key = q->r.series_key;
-            e->meta = q->meta;
-            e->bytes = q->series_bytes;
-            e->refs = 1;
-            e->cached = q->r.cache && q->r.series_key != 0;
-            if (e->cached) {
-                s->lru.push_back(e->key);
-                e->lru = std::prev(s->lru.end());
-                s->cache.emplace(e->key, e);
-            }
-            q->entry = e;
-        }
-        return q;
+        return admit_at(s, s->queue.begin());
+    }
+    return nullptr;
+}
+
+// Small one-series recurrence requests (EMA, RSI, ADX) are served together:
+// a slot that admits one also admits queued requests of the same kind, from
+// the first kPackScan in the queue, up to kPackMax, and runs them as one
+// packed batch (one launch per scan pass for all of them).  Small = at most
+// kPackOutputs outputs, where launch overhead and per-request latency chains
+// dominate; outputs are bit-identical to the request served alone.
+constexpr size_t kPackOutputs = 4u << 20;
+constexpr size_t kPackMax = 32;
+constexpr size_t kPackScan = 64;
+
+bool packable(const Req &q) {
+    return (q.r.kind == VTA_REQ_EMA || q.r.kind == VTA_REQ_RSI || q.r.kind == VTA_REQ_ADX) &&
+           q.out_bytes <= kPackOutputs * sizeof(float);
+}
+
+// Called with s->mu held: the next packable request of `kind` among the
+// first kPackScan queued, admitted, or null.
+std::shared_ptr<Req> try_admit_kind(vta_server *s, int kind) {
+    size_t seen = 0;
+    for (auto it = s->queue.begin(); it != s->queue.end() && seen < kPackScan; ++it, ++seen) {
+        const Req &q = **it;
+        if (q.r.kind != kind || !packable(q)) continue;
+        return admit_at(s, it);
     }
     return nullptr;
 }
@@ -496,83 +533,229 @@ void flush_frees(vta_server *s, std::vector<void *> &mine) {
     mine.clear();
 }
 
+// Uploads the request's series (its uploader) or waits until another slot
+// has (a cache hit or a concurrent miss on the same key).
+vta_status prepare_series(vta_server *s, int slot, vta_ctx *ctx, Req &q) {
+    CacheEntry &e = *q.entry;
+    vta_status st = VTA_OK;
+    if (q.uploader) {
+        st = upload_series(s, slot, ctx, q);
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (st != VTA_OK) {
+            (void)hipStreamSynchronize(ctx->stream);
+            if (e.d_close) s->pending_free.push_back(e.d_close);
+            if (e.d_high) s->pending_free.push_back(e.d_high);
+            if (e.d_low) s->pending_free.push_back(e.d_low);
+            e.d_close = e.d_high = e.d_low = nullptr;
+            if (e.cached) {
+                // drop the broken entry; its bytes go back with this request
+                s->cache.erase(e.key);
+                s->lru.erase(e.lru);
+                e.cached = false;
+            }
+        }
+        e.ready = true;
+        s->cv_done.notify_all();
+    } else {
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv_done.wait(lk, [&] { return e.ready; });
+    }
+    q.res.t_series = now_s() - s->t0;
+    if (st == VTA_OK && !e.d_close) st = fail(VTA_ERR_HIP, "series upload failed");
+    return st;
+}
+
+// Records the request's outcome and returns what it holds.
+void finish_request(vta_server *s, vta_ctx *ctx, Req &q, vta_status st) {
+    CacheEntry &e = *q.entry;
+    std::lock_guard<std::mutex> lk(s->mu);
+    q.res.status = st;
+    q.res.t_done = now_s() - s->t0;
+    if (st != VTA_OK) {
+        (void)hipStreamSynchronize(ctx->stream);
+        take_outputs(q.res, s->pending_free);
+        s->in_use -= q.out_bytes;
+        q.out_bytes = 0;
+    }
+    e.refs -= 1;
+    if (!e.cached) {
+        // a one-off series, or an entry dropped after a failed upload
+        if (e.refs == 0) {
+            if (e.d_close) s->pending_free.push_back(e.d_close);
+            if (e.d_high) s->pending_free.push_back(e.d_high);
+            if (e.d_low) s->pending_free.push_back(e.d_low);
+            e.d_close = e.d_high = e.d_low = nullptr;
+        }
+        s->in_use -= q.series_reserved;
+        q.series_reserved = 0;
+    }
+    s->in_flight -= 1;
+    q.done = true;
+    s->cv_done.notify_all();
+    s->cv_admit.notify_all();
+}
+
+// The packed form of run_request for a group of EMA / RSI / ADX requests of
+// one kind whose series are on the device.  st[i] receives each member's
+// status; a member that fails validation is left out of the pack.  Returns
+// false (nothing launched, st untouched) when the arena cannot hold the
+// pack's table and carry, and the members are then run one by one.
+bool run_packed(vta_server *s, vta_ctx *ctx, std::vector<std::shared_ptr<Req>> &g,
+                std::vector<vta_status> &st) {
+    const int kind = g[0]->r.kind;
+    const int rk = kind == VTA_REQ_EMA ? VTA_REC_EMA : kind == VTA_REQ_RSI ? VTA_REC_RSI
+                                                                             : VTA_REC_ADX;
+    std::vector<vta_rec_job> jobs;
+    std::vector<size_t> member;                     // job -> index in g
+    std::vector<std::unique_ptr<ArenaTable>> tables;
+    std::vector<std::vector<int32_t>> pers(g.size());
+    for (size_t i = 0; i < g.size(); ++i) {
+        Req &q = *g[i];
+        const CacheEntry &e = *q.entry;
+        const SeriesMeta &m = e.meta;
+        const size_t T = m.len;
+        size_t n = 0;
+        vta_status (*grid)(const vta_range_usize *, size_t *, size_t, size_t *) =
+            kind == VTA_REQ_EMA ? vta_ema_expand_grid
+            : kind == VTA_REQ_RSI ? vta_rsi_expand_grid : vta_adx_expand_grid;
+        vta_status v = grid(&q.r.period, nullptr, 0, &n);
+        std::vector<size_t> ps(n);
+        if (v == VTA_OK) v = grid(&q.r.period, ps.data(), n, &n);
+        if (v == VTA_OK && (n == 0 || T > (size_t)INT32_MAX || n > (size_t)INT32_MAX))
+            v = fail(VTA_ERR_INVALID_INPUT, "invalid request sizes");
+        size_t maxp = 0;
+        for (size_t p : ps) maxp = std::max(maxp, p);
+        const int32_t fv = kind == VTA_REQ_ADX ? m.fv_adx : m.fv_close;
+        const size_t need = kind == VTA_REQ_EMA ? maxp : kind == VTA_REQ_RSI ? maxp + 1 : 2 * maxp;
+        if (v == VTA_OK && (maxp == 0 || (size_t)fv >= T || T - (size_t)fv < need))
+            v = fail(VTA_ERR_INVALID_INPUT, "not enough valid data (needed >= %zu)", need);
+        if (v == VTA_OK && kind == VTA_REQ_ADX && !m.hlc)
+            v = fail(VTA_ERR_INVALID_INPUT, "ADX needs high, low and close");
+        if (v == VTA_OK) {
+            pers[i].assign(ps.begin(), ps.end());
+            tables.push_back(std::make_unique<ArenaTable>());
+            v = tables.back()->put(s, ctx, pers[i]);
+        }
+        if (v == VTA_OK) v = alloc_out(s, q.res.out[0], n, T);
+        if (v != VTA_OK) {
+            st[i] = v;
+            continue;
+        }
+        q.res.n_outputs = 1;
+        q.res.n_combos = n;
+        vta_rec_job j{};
+        j.d_close = e.d_close;
+        j.d_high = e.d_high;
+        j.d_low = e.d_low;
+        j.d_periods = tables.back()->as<int32_t>();
+        j.d_out = q.res.out[0].ptr;
+        j.series_len = (int32_t)T;
+        j.first_valid = fv;
+        j.n_combos = (int32_t)n;
+        j.max_period = (int32_t)maxp;
+        jobs.push_back(j);
+        member.push_back(i);
+    }
+    if (jobs.empty()) return true;
+    std::vector<PackJob> tab;
+    std::vector<size_t> off;
+    size_t elems = 0;
+    PackDims dims;
+    vta_status v = pack_layout(ctx, rk, jobs.data(), jobs.size(), tab, off, &elems, &dims);
+    void *d_tab = nullptr, *d_flags = nullptr, *d_carry = nullptr;
+    if (v == VTA_OK) {
+        d_tab = s->arena.alloc(tab.size() * sizeof(PackJob));
+        d_flags = s->arena.alloc(tab.size() * sizeof(int32_t));
+        d_carry = s->arena.alloc(std::max<size_t>(elems, 1) * sizeof(double));
+        if (!d_tab || !d_flags || !d_carry) {
+            for (void *p : {d_tab, d_flags, d_carry})
+                if (p) s->arena.release(p);
+            (void)hipStreamSynchronize(ctx->stream);   // the period tables' copies
+            for (size_t i : member) {
+                Req &q = *g[i];
+                s->arena.release(q.res.out[0].ptr);
+                q.res.out[0].ptr = nullptr;
+            }
+            return false;
+        }
+        for (size_t j = 0; j < tab.size(); ++j)
+            tab[j].carry = static_cast<double *>(d_carry) + off[j];
+        hipError_t e = hipMemcpyAsync(d_tab, tab.data(), tab.size() * sizeof(PackJob),
+                                      hipMemcpyHostToDevice, ctx->stream);
+        if (e != hipSuccess) v = fail(VTA_ERR_HIP, "hipMemcpyAsync: %s", hipGetErrorString(e));
+    }
+    if (v == VTA_OK) v = pack_launch(ctx, dims, d_tab, static_cast<int32_t *>(d_flags));
+    const double t_launch = now_s() - s->t0;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (v == VTA_OK && e != hipSuccess)
+        v = fail(VTA_ERR_HIP, "hipStreamSynchronize: %s", hipGetErrorString(e));
+    for (void *p : {d_tab, d_flags, d_carry})
+        if (p) s->arena.release(p);
+    for (size_t i : member) {
+        st[i] = v;
+        g[i]->res.t_launch = t_launch;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->packs += 1;
+    s->packed_requests += member.size();
+    return true;
+}
+
 void worker(vta_server *s, int slot) {
     vta_ctx *ctx = s->ctxs[slot];
     (void)hipSetDevice(s->device);
     std::vector<void *> local_frees;
     for (;;) {
-        std::shared_ptr<Req> q;
+        std::vector<std::shared_ptr<Req>> g;
         {
             std::unique_lock<std::mutex> lk(s->mu);
             for (;;) {
                 if (s->stop) return;
-                q = try_admit(s);
-                if (q) break;
+                std::shared_ptr<Req> q = try_admit(s);
+                if (q) {
+                    g.push_back(q);
+                    break;
+                }
                 s->cv_admit.wait(lk);
             }
-            s->in_flight += 1;
+            if (packable(*g[0]))
+                while (g.size() < kPackMax) {
+                    std::shared_ptr<Req> q = try_admit_kind(s, g[0]->r.kind);
+                    if (!q) break;
+                    g.push_back(q);
+                }
+            const double t = now_s() - s->t0;
+            for (auto &q : g) q->res.t_start = t;
+            s->in_flight += (int)g.size();
             s->peak_in_flight = std::max(s->peak_in_flight, s->in_flight);
-            q->res.t_start = now_s() - s->t0;
         }
-        flush_frees(s, local_frees);   // evictions made room for this request
-        CacheEntry &e = *q->entry;
-        vta_status st = VTA_OK;
-        if (q->uploader) {
-            st = upload_series(s, slot, ctx, *q);
-            std::lock_guard<std::mutex> lk(s->mu);
-            if (st != VTA_OK) {
-                (void)hipStreamSynchronize(ctx->stream);
-                if (e.d_close) s->pending_free.push_back(e.d_close);
-                if (e.d_high) s->pending_free.push_back(e.d_high);
-                if (e.d_low) s->pending_free.push_back(e.d_low);
-                e.d_close = e.d_high = e.d_low = nullptr;
-                if (e.cached) {
-                    // drop the broken entry; its bytes go back with this request
-                    s->cache.erase(e.key);
-                    s->lru.erase(e.lru);
-                    e.cached = false;
+        flush_frees(s, local_frees);   // evictions made room for these requests
+        std::vector<vta_status> st(g.size(), VTA_OK);
+        for (size_t i = 0; i < g.size(); ++i) st[i] = prepare_series(s, slot, ctx, *g[i]);
+        bool done = false;
+        if (g.size() > 1) {
+            std::vector<std::shared_ptr<Req>> ready;
+            std::vector<size_t> idx;
+            for (size_t i = 0; i < g.size(); ++i)
+                if (st[i] == VTA_OK) {
+                    ready.push_back(g[i]);
+                    idx.push_back(i);
                 }
+            if (ready.size() > 1) {
+                clear_error();
+                std::vector<vta_status> rs(ready.size(), VTA_OK);
+                done = run_packed(s, ctx, ready, rs);
+                if (done)
+                    for (size_t k = 0; k < idx.size(); ++k) st[idx[k]] = rs[k];
             }
-            e.ready = true;
-            s->cv_done.notify_all();
-        } else {
-            std::unique_lock<std::mutex> lk(s->mu);
-            s->cv_done.wait(lk, [&] { return e.ready; });
         }
-        q->res.t_series = now_s() - s->t0;
-        if (st == VTA_OK && e.d_close) {
-            clear_error();
-            st = run_request(s, ctx, *q);
-        } else if (st == VTA_OK) {
-            st = fail(VTA_ERR_HIP, "series upload failed");
-        }
-        {
-            std::lock_guard<std::mutex> lk(s->mu);
-            q->res.status = st;
-            q->res.t_done = now_s() - s->t0;
-            if (st != VTA_OK) {
-                (void)hipStreamSynchronize(ctx->stream);
-                take_outputs(q->res, s->pending_free);
-                s->in_use -= q->out_bytes;
-                q->out_bytes = 0;
-            }
-            e.refs -= 1;
-            if (!e.cached) {
-                // a one-off series, or an entry dropped after a failed upload
-                if (e.refs == 0) {
-                    if (e.d_close) s->pending_free.push_back(e.d_close);
-                    if (e.d_high) s->pending_free.push_back(e.d_high);
-                    if (e.d_low) s->pending_free.push_back(e.d_low);
-                    e.d_close = e.d_high = e.d_low = nullptr;
+        if (!done)
+            for (size_t i = 0; i < g.size(); ++i)
+                if (st[i] == VTA_OK) {
+                    clear_error();
+                    st[i] = run_request(s, ctx, *g[i]);
                 }
-                s->in_use -= q->series_reserved;
-                q->series_reserved = 0;
-            }
-            s->in_flight -= 1;
-            q->done = true;
-            s->cv_done.notify_all();
-            s->cv_admit.notify_all();
-        }
+        for (size_t i = 0; i < g.size(); ++i) finish_request(s, ctx, *g[i], st[i]);
         flush_frees(s, local_frees);
     }
 }
@@ -741,6 +924,8 @@ vta_status vta_server_get_stats(vta_server *s, vta_server_stats *out) {
     out->in_flight = (size_t)s->in_flight;
     out->peak_in_flight = (size_t)s->peak_in_flight;
     out->cached_series = s->cache.size();
+    out->packs = s->packs;
+    out->packed_requests = s->packed_requests;
     return VTA_OK;
 }
 
