@@ -353,6 +353,25 @@ vta_status vta_rec_pack_create(vta_ctx *ctx, vta_rec_kind kind, const vta_rec_jo
                                size_t n_jobs, vta_rec_pack **out);
 vta_status vta_rec_pack_launch(vta_ctx *ctx, const vta_rec_pack *pack);
 vta_status vta_rec_pack_destroy(vta_rec_pack *pack);
+/* Packed one-series Bollinger Bands batches (no reference counterpart), the
+ * same idea for vta_bollinger_bands_batch_device: each job's three outputs
+ * are bit-identical to its own _device call; one launch for every job. */
+typedef struct vta_bb_job {
+    const float *d_prices;
+    const int32_t *d_periods;
+    const double *d_devup, *d_devdn;
+    float *d_upper, *d_middle, *d_lower;   /* [n_combos][series_len] each */
+    int32_t series_len;
+    int32_t first_valid;
+    int32_t first_nonfinite;               /* first NaN/Inf at or after first_valid; series_len if none */
+    int32_t n_combos;
+    int32_t max_period;
+} vta_bb_job;
+typedef struct vta_bb_pack vta_bb_pack;
+vta_status vta_bb_pack_create(vta_ctx *ctx, const vta_bb_job *jobs, size_t n_jobs,
+                              vta_bb_pack **out);
+vta_status vta_bb_pack_launch(vta_ctx *ctx, const vta_bb_pack *pack);
+vta_status vta_bb_pack_destroy(vta_bb_pack *pack);
 vta_status vta_atr_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *high_tm,
                                                         const float *low_tm,
                                                         const float *close_tm, size_t cols,
@@ -693,7 +712,7 @@ typedef struct vta_server_stats {
     size_t budget, in_use, peak_in_use;
     size_t queued, in_flight, peak_in_flight;
     size_t cached_series;
-    /* small EMA / RSI / ADX requests served together in packed launches:
+    /* small EMA / RSI / ADX / Bollinger Bands requests served together in packed launches:
      * launches (packs) and the requests they served */
     size_t packs, packed_requests;
 } vta_server_stats;

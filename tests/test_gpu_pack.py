@@ -147,3 +147,62 @@ def test_pack_rejects_bad_jobs():
     with pytest.raises(vta.VtaError):
         vta.RecPack(ctx, "macd", [good])
     vta.RecPack(ctx, "ema", [good]).close()
+
+
+def test_bb_pack_matches_direct_calls():
+    """Packed Bollinger Bands batches: every job's three outputs equal its own
+    vta_bollinger_bands_batch_device call bit for bit (uneven lengths, a late
+    first_valid, a non-finite bar and its poison rule, grids of 1 to 70
+    combos); one job also against the oracle."""
+    require_gpu()
+    ctx = vta.HipContext(0)
+    L = lib()
+    shapes = [(300, 0, (5, 20, 5), None), (5_000, 0, (2, 60, 1), None),
+              (40_000, 900, (10, 200, 10), None), (12_000, 3, (5, 50, 5), 6_000)]
+    jobs, bufs, host = [], [], []
+    for i, (T, lead, (p0, p1, ps), bad) in enumerate(shapes):
+        x = datagen.gbm_close(T, 0.01, seed=700 + i)
+        x[:lead] = np.nan
+        if bad is not None:
+            x[bad] = np.nan
+        per = np.arange(p0, p1 + 1, ps, dtype=np.int32)
+        C = int(per.size)
+        up = np.linspace(1.5, 2.5, C)
+        dn = np.full(C, 2.0)
+        fnf = int(np.argmax(~np.isfinite(x[lead:]))) + lead if not np.isfinite(x[lead:]).all() \
+            else T
+        d = {"x": ctx.to_device(x), "p": ctx.to_device(per), "u": ctx.to_device(up),
+             "d": ctx.to_device(dn)}
+        outs = [ctx.alloc(C * T * 4) for _ in range(6)]
+        jobs.append({"prices": d["x"], "periods": d["p"], "devup": d["u"], "devdn": d["d"],
+                     "upper": outs[0], "middle": outs[1], "lower": outs[2], "series_len": T,
+                     "first_valid": lead, "first_nonfinite": fnf, "n_combos": C,
+                     "max_period": int(per.max())})
+        bufs.append((d, outs))
+        host.append((x, per, up, dn))
+    pack = vta.BbPack(ctx, jobs)
+    pack.launch()
+    for j, (d, outs) in zip(jobs, bufs):
+        check(L.vta_bollinger_bands_batch_device(
+            ctx.handle, P(d["x"].ptr), j["series_len"], j["first_valid"], j["first_nonfinite"],
+            P(d["p"].ptr), P(d["u"].ptr), P(d["d"].ptr), j["n_combos"], j["max_period"],
+            P(outs[3].ptr), P(outs[4].ptr), P(outs[5].ptr)))
+    ctx.synchronize()
+    for i, (j, (d, outs)) in enumerate(zip(jobs, bufs)):
+        C, T = j["n_combos"], j["series_len"]
+        for o in range(3):
+            g = outs[o].copy_to_host(np.float32).reshape(C, T)
+            r = outs[3 + o].copy_to_host(np.float32).reshape(C, T)
+            assert np.array_equal(g.view(np.uint32), r.view(np.uint32)), (i, o, shapes[i])
+    x, per, up, dn = host[2]
+    xd = x.astype(np.float64)
+    for o, name in enumerate(("upper", "middle", "lower")):
+        ref = np.stack([O.bollinger_bands(xd, int(p), float(a), float(b))[o]
+                        for p, a, b in zip(per, up, dn)])
+        g = bufs[2][1][o].copy_to_host(np.float32).reshape(len(per), -1)
+        nan_mask_equal(g, ref)
+        assert max_rel_err(g, ref) <= 1e-5, name
+    pack.close()
+    for d, outs in bufs:
+        for b in list(d.values()) + outs:
+            b.free()

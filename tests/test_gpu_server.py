@@ -131,15 +131,16 @@ def test_budget_admission_and_cache(pool):
 
 
 def test_small_requests_are_packed(pool):
-    """Small EMA / RSI / ADX requests queued behind others are served together
-    in packed launches (one launch per scan pass for the group); every result
-    equals its direct entry point bit for bit, packed or not."""
+    """Small EMA / RSI / ADX / Bollinger Bands requests queued behind others
+    are served together in packed launches (one launch per scan pass for the
+    group); every result equals its direct entry point bit for bit, packed or
+    not."""
     c, h, lo = pool
     rng = np.random.Generator(np.random.PCG64(83))
     srv = vta.IndicatorServer(0, n_slots=1, hbm_budget=4 << 30)
     specs, ids = [], []
-    for i in range(36):
-        kind = ("ema", "rsi", "adx")[i % 3]
+    for i in range(48):
+        kind = ("ema", "rsi", "adx", "bollinger_bands")[i % 4]
         s = int(rng.integers(4))
         L = int(rng.integers(2_000, 20_000))
         a = int(rng.integers(0, 120_000 - L))
@@ -148,20 +149,29 @@ def test_small_requests_are_packed(pool):
         x, hh, ll = c[s, a:a + L], h[s, a:a + L], lo[s, a:a + L]
         if kind == "adx":
             ids.append(srv.submit(kind, x, hh, ll, period=period))
+        elif kind == "bollinger_bands":
+            period = vta.BollingerBandsBatchRange(period=period, devup=(1.5, 2.5, 0.5),
+                                                  devdn=(2.0, 2.0, 0.0))
+            ids.append(srv.submit(kind, x, bb=period))
         else:
             ids.append(srv.submit(kind, x, period=period))
         specs.append((kind, x, hh, ll, period))
-    direct = {"ema": vta.CudaEma(0), "rsi": vta.CudaRsi(0), "adx": vta.CudaAdx(0)}
+    direct = {"ema": vta.CudaEma(0), "rsi": vta.CudaRsi(0), "adx": vta.CudaAdx(0),
+              "bollinger_bands": vta.CudaBollingerBands(0)}
     for rid, (kind, x, hh, ll, period) in zip(ids, specs):
         r = srv.wait(rid)
         assert r.ok, (kind, r.status)
-        got = srv.to_host(r)
+        got = [srv.to_host(r, k) for k in range(r.n_outputs)]
         srv.release(rid)
         if kind == "adx":
-            ref = direct[kind].adx_batch_dev(hh, ll, x, period).to_host()
+            ref = [direct[kind].adx_batch_dev(hh, ll, x, period).to_host()]
+        elif kind == "bollinger_bands":
+            ref = [d.to_host() for d in direct[kind].bollinger_bands_batch_dev(x, period)]
         else:
-            ref = getattr(direct[kind], f"{kind}_batch_dev")(x, period).to_host()
-        assert np.array_equal(_bits(got), _bits(ref)), (kind, len(x), period)
+            ref = [getattr(direct[kind], f"{kind}_batch_dev")(x, period).to_host()]
+        assert len(got) == len(ref)
+        for g_, r_ in zip(got, ref):
+            assert np.array_equal(_bits(g_), _bits(r_)), (kind, len(x))
     st = srv.stats()
     srv.close()
     print(f"packs {st['packs']}, packed requests {st['packed_requests']}")

@@ -14,7 +14,7 @@ from .fir import CudaSma, CudaSwma, CudaWma
 from .rec import (CudaAdx, CudaAtr, CudaEma, CudaRsi, CudaSmma, CudaWilders, ManySeriesOhlcPlan,
                   RecPack, ema_rsi_atr_many_series_dev)
 from .server import IndicatorServer, ServedResult
-from .rolling import (BollingerBandsBatchRange, CudaBollingerBands, CudaMediumAd,
+from .rolling import (BbPack, BollingerBandsBatchRange, CudaBollingerBands, CudaMediumAd,
                       CudaPercentileNearestRank, PercentileNearestRankBatchRange,
                       RollingManySeriesPlan)
 
@@ -26,4 +26,5 @@ __all__ = [
     "CudaMediumAd", "CudaPercentileNearestRank", "PercentileNearestRankBatchRange",
     "RollingManySeriesPlan", "DoubleAlmaCrossover", "DoubleAlmaPlan", "IndicatorServer",
     "ServedResult", "CudaSma", "CudaWma", "CudaSwma", "CudaSmma", "CudaWilders", "RecPack",
+    "BbPack",
 ]

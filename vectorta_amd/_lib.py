@@ -58,6 +58,16 @@ class RecJobC(ctypes.Structure):
                 ("max_period", ctypes.c_int32)]
 
 
+class BbJobC(ctypes.Structure):
+    """vta_bb_job (include/vta.h)."""
+    _fields_ = [("d_prices", ctypes.c_void_p), ("d_periods", ctypes.c_void_p),
+                ("d_devup", ctypes.c_void_p), ("d_devdn", ctypes.c_void_p),
+                ("d_upper", ctypes.c_void_p), ("d_middle", ctypes.c_void_p),
+                ("d_lower", ctypes.c_void_p), ("series_len", ctypes.c_int32),
+                ("first_valid", ctypes.c_int32), ("first_nonfinite", ctypes.c_int32),
+                ("n_combos", ctypes.c_int32), ("max_period", ctypes.c_int32)]
+
+
 class BbParamsC(ctypes.Structure):
     _fields_ = [("period", ctypes.c_size_t), ("devup", ctypes.c_double),
                 ("devdn", ctypes.c_double)]
@@ -122,6 +132,9 @@ SIGNATURES = {
                                   ctypes.POINTER(_P)]),
     "vta_rec_pack_launch": (_ST, [_P, _P]),
     "vta_rec_pack_destroy": (_ST, [_P]),
+    "vta_bb_pack_create": (_ST, [_P, ctypes.POINTER(BbJobC), _SZ, ctypes.POINTER(_P)]),
+    "vta_bb_pack_launch": (_ST, [_P, _P]),
+    "vta_bb_pack_destroy": (_ST, [_P]),
     "vta_ctx_begin_capture": (_ST, [_P]),
     "vta_ctx_end_capture": (_ST, [_P, ctypes.POINTER(_P)]),
     "vta_graph_launch": (_ST, [_P, _P]),

@@ -201,18 +201,29 @@ extern "C" __global__ __launch_bounds__(64) void bb_ms1p_f32(roll_args a) {
 // Lane = combo; all lanes step through the chunk's bars together (the same
 // output rule as bb_run: re-anchor at the first output bar of the chunk,
 // replaying at most p bars after the first non-finite bar z, NaN from z + p).
-extern "C" __global__ __launch_bounds__(64) void bb_batch_f32(roll_args a) {
+// What one block of a one-series BB batch works on: the series, the combos'
+// periods / deviations and outputs, the chunking, first_valid, the first
+// non-finite bar z (T if none), and which combo group and chunk it is.
+struct bb_view {
+    const float *x;
+    const int32_t *p_len;
+    const double *p_a, *p_b;
+    float *out0, *out1, *out2;
+    int T, nlanes, chunk, fv, z, group, k;
+};
+
+__device__ __forceinline__ void bb_batch_body(const bb_view &a) {
     __shared__ float ob[3][64 * RB_LD];
     const int lane = threadIdx.x & 63;
-    const int c = blockIdx.x * 64 + lane;
+    const int c = a.group * 64 + lane;
     const bool active = c < a.nlanes;
     const int T = a.T;
-    const int c0 = blockIdx.y * a.chunk;
+    const int c0 = a.k * a.chunk;
     const int c1 = min(c0 + a.chunk, T);
     const int cc = min(c, a.nlanes - 1);
     BbEng e;
-    e.setup(a.p_len[cc], a.fvs[0], a.p_a[cc], a.p_b[cc]);
-    const int z = a.poison ? a.poison[0] : T;
+    e.setup(a.p_len[cc], a.fv, a.p_a[cc], a.p_b[cc]);
+    const int z = a.z;
     auto ld = [&](int t) -> double { return (double)a.x[t]; };
     const int warm = e.fv + e.p - 1;
     const long zp = (long)z + e.p;
@@ -255,11 +266,69 @@ extern "C" __global__ __launch_bounds__(64) void bb_batch_f32(roll_args a) {
             }
         }
         rb_sync();
-        rb_flush(ob[0], a.out0, blockIdx.x * 64, a.nlanes, T, t0, nb);
-        rb_flush(ob[1], a.out1, blockIdx.x * 64, a.nlanes, T, t0, nb);
-        rb_flush(ob[2], a.out2, blockIdx.x * 64, a.nlanes, T, t0, nb);
+        rb_flush(ob[0], a.out0, a.group * 64, a.nlanes, T, t0, nb);
+        rb_flush(ob[1], a.out1, a.group * 64, a.nlanes, T, t0, nb);
+        rb_flush(ob[2], a.out2, a.group * 64, a.nlanes, T, t0, nb);
         rb_sync();
     }
+}
+
+extern "C" __global__ __launch_bounds__(64) void bb_batch_f32(roll_args a) {
+    bb_view v;
+    v.x = a.x;
+    v.p_len = a.p_len;
+    v.p_a = a.p_a;
+    v.p_b = a.p_b;
+    v.out0 = a.out0;
+    v.out1 = a.out1;
+    v.out2 = a.out2;
+    v.T = a.T;
+    v.nlanes = a.nlanes;
+    v.chunk = a.chunk;
+    v.fv = a.fvs[0];
+    v.z = a.poison ? a.poison[0] : a.T;
+    v.group = blockIdx.x;
+    v.k = blockIdx.y;
+    bb_batch_body(v);
+}
+
+// Packed one-series BB batches: job j = one series with its own combos;
+// a 1-D grid over every job's (combo group, chunk) blocks, job j owning
+// [block0, block0 + groups * ceil(T / chunk)); each block finds its job by
+// binary search and runs exactly the block of the job's direct launch.
+struct bb_job {
+    const float *x;
+    const int32_t *p_len;
+    const double *p_a, *p_b;
+    float *out0, *out1, *out2;
+    int T, nlanes, fv, z, chunk, groups, block0, pad_;
+};
+
+extern "C" __global__ __launch_bounds__(64) void bb_packed_f32(const bb_job *jobs, int n_jobs) {
+    const int flat = blockIdx.x;
+    int lo = 0, hi = n_jobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].block0 <= flat) lo = mid; else hi = mid - 1;
+    }
+    const bb_job &j = jobs[lo];
+    const int local = flat - j.block0;
+    bb_view v;
+    v.x = j.x;
+    v.p_len = j.p_len;
+    v.p_a = j.p_a;
+    v.p_b = j.p_b;
+    v.out0 = j.out0;
+    v.out1 = j.out1;
+    v.out2 = j.out2;
+    v.T = j.T;
+    v.nlanes = j.nlanes;
+    v.chunk = j.chunk;
+    v.fv = j.fv;
+    v.z = j.z;
+    v.group = local % j.groups;
+    v.k = local / j.groups;
+    bb_batch_body(v);
 }
 
 // ---------------------------------------------------------------- sorted window

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "runtime_internal.h"
@@ -439,6 +440,107 @@ vta_status vta_medium_ad_many_series_one_param_device(vta_ctx *ctx, const float 
                                                       float *d_out_tm) {
     clear_error();
     return ms_device(ctx, R_MAD, d_data_tm, d_first_valids, cols, rows, period, 0.0, d_out_tm);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- packed BB batches
+
+static_assert(sizeof(BbPackJob) == 88, "bb_job layout");
+
+namespace vta {
+
+vta_status bb_pack_layout(const vta_ctx *ctx, const vta_bb_job *jobs, size_t n_jobs,
+                          std::vector<BbPackJob> &tab, unsigned *total_blocks) {
+    if (!ctx || !jobs || !total_blocks) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    if (n_jobs == 0 || n_jobs > 65535)
+        return fail(VTA_ERR_INVALID_INPUT, "n_jobs must be in 1..65535 (got %zu)", n_jobs);
+    tab.assign(n_jobs, BbPackJob{});
+    size_t blocks = 0;
+    for (size_t j = 0; j < n_jobs; ++j) {
+        const vta_bb_job &q = jobs[j];
+        // the checks of vta_bollinger_bands_batch_device
+        if (!q.d_prices || !q.d_periods || !q.d_devup || !q.d_devdn || !q.d_upper ||
+            !q.d_middle || !q.d_lower)
+            return fail(VTA_ERR_INVALID_INPUT, "job %zu: null pointer", j);
+        if (q.series_len <= 0 || q.n_combos <= 0 || q.max_period <= 0 || q.first_valid < 0 ||
+            q.first_valid >= q.series_len || q.first_nonfinite < q.first_valid ||
+            q.first_nonfinite > q.series_len || q.series_len - q.first_valid < q.max_period)
+            return fail(VTA_ERR_INVALID_INPUT, "job %zu: invalid sizes", j);
+        const size_t T = (size_t)q.series_len, C = (size_t)q.n_combos;
+        const size_t chunk = (size_t)roll_chunk(ctx, T, C, (size_t)q.max_period, true, 0);
+        const size_t nch = (T + chunk - 1) / chunk;
+        const size_t groups = (C + 63) / 64;
+        BbPackJob &e = tab[j];
+        e.x = q.d_prices;
+        e.p_len = q.d_periods;
+        e.p_a = q.d_devup;
+        e.p_b = q.d_devdn;
+        e.out0 = q.d_upper;
+        e.out1 = q.d_middle;
+        e.out2 = q.d_lower;
+        e.T = (int)T;
+        e.nlanes = (int)C;
+        e.fv = q.first_valid;
+        e.z = q.first_nonfinite;
+        e.chunk = (int)chunk;
+        e.groups = (int)groups;
+        e.block0 = (int)blocks;
+        blocks += groups * nch;
+        if (blocks > (size_t)INT32_MAX)
+            return fail(VTA_ERR_LAUNCH_CONFIG_TOO_LARGE, "packed grid exceeds 2^31 blocks");
+    }
+    *total_blocks = (unsigned)blocks;
+    return VTA_OK;
+}
+
+vta_status bb_pack_launch(vta_ctx *ctx, const void *d_jobs, int n_jobs, unsigned total_blocks) {
+    hipFunction_t f;
+    VTA_TRY(get_function(ctx, "rolling", "bb_packed_f32", &f));
+    return launch(ctx, f, dim3(total_blocks), dim3(64), 0, d_jobs, n_jobs);
+}
+
+}  // namespace vta
+
+struct vta_bb_pack {
+    int device = 0;
+    int n_jobs = 0;
+    unsigned total_blocks = 0;
+    DevBuf jobs;
+};
+
+extern "C" {
+
+vta_status vta_bb_pack_create(vta_ctx *ctx, const vta_bb_job *jobs, size_t n_jobs,
+                              vta_bb_pack **out) {
+    clear_error();
+    if (!ctx || !jobs || !out) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    *out = nullptr;
+    std::vector<BbPackJob> tab;
+    auto pk = std::make_unique<vta_bb_pack>();
+    VTA_TRY(bb_pack_layout(ctx, jobs, n_jobs, tab, &pk->total_blocks));
+    pk->device = ctx->device;
+    pk->n_jobs = (int)n_jobs;
+    VTA_HIP(hipSetDevice(ctx->device));
+    VTA_TRY(upload(ctx, pk->jobs, tab.data(), n_jobs * sizeof(BbPackJob)));
+    *out = pk.release();
+    return VTA_OK;
+}
+
+vta_status vta_bb_pack_launch(vta_ctx *ctx, const vta_bb_pack *pk) {
+    clear_error();
+    if (!ctx || !pk) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    if (pk->device != ctx->device)
+        return fail(VTA_ERR_DEVICE_MISMATCH, "pack built on device %d, context on %d", pk->device,
+                    ctx->device);
+    return bb_pack_launch(ctx, pk->jobs.p, pk->n_jobs, pk->total_blocks);
+}
+
+vta_status vta_bb_pack_destroy(vta_bb_pack *pk) {
+    if (!pk) return VTA_OK;
+    (void)hipSetDevice(pk->device);
+    delete pk;
+    return VTA_OK;
 }
 
 }  // extern "C"

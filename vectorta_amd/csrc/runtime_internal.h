@@ -159,6 +159,18 @@ vta_status pack_layout(const vta_ctx *ctx, int kind, const vta_rec_job *jobs, si
 // d_flags: n_jobs int32 words (cleared by the launch itself).
 vta_status pack_launch(vta_ctx *ctx, const PackDims &d, const void *d_jobs, int32_t *d_flags);
 
+// Packed one-series BB batches (rolling.cpp): bb_job of rolling.hip.
+struct BbPackJob {
+    const float *x;
+    const int32_t *p_len;
+    const double *p_a, *p_b;
+    float *out0, *out1, *out2;
+    int T, nlanes, fv, z, chunk, groups, block0, pad_;
+};
+vta_status bb_pack_layout(const vta_ctx *ctx, const vta_bb_job *jobs, size_t n_jobs,
+                          std::vector<BbPackJob> &tab, unsigned *total_blocks);
+vta_status bb_pack_launch(vta_ctx *ctx, const void *d_jobs, int n_jobs, unsigned total_blocks);
+
 // bf16 round-to-nearest-even of an f32 (bit pattern).
 uint16_t bf16_rne(float f);
 float bf16_to_f32(uint16_t h);

@@ -291,7 +291,7 @@ std::shared_ptr<Req> try_admit(vta_server *s) {
     return nullptr;
 }
 
-// Small one-series recurrence requests (EMA, RSI, ADX) are served together:
+// Small one-series requests (EMA, RSI, ADX, Bollinger Bands) are served together:
 // a slot that admits one also admits queued requests of the same kind, from
 // the first kPackScan in the queue, up to kPackMax, and runs them as one
 // packed batch (one launch per scan pass for all of them).  Small = at most
@@ -302,7 +302,8 @@ constexpr size_t kPackMax = 32;
 constexpr size_t kPackScan = 64;
 
 bool packable(const Req &q) {
-    return (q.r.kind == VTA_REQ_EMA || q.r.kind == VTA_REQ_RSI || q.r.kind == VTA_REQ_ADX) &&
+    return (q.r.kind == VTA_REQ_EMA || q.r.kind == VTA_REQ_RSI || q.r.kind == VTA_REQ_ADX ||
+            q.r.kind == VTA_REQ_BB) &&
            q.out_bytes <= kPackOutputs * sizeof(float);
 }
 
@@ -595,6 +596,107 @@ void finish_request(vta_server *s, vta_ctx *ctx, Req &q, vta_status st) {
     s->cv_admit.notify_all();
 }
 
+// The packed form of run_request for a group of Bollinger Bands requests
+// (see run_packed).
+bool run_packed_bb(vta_server *s, vta_ctx *ctx, std::vector<std::shared_ptr<Req>> &g,
+                   std::vector<vta_status> &st) {
+    std::vector<vta_bb_job> jobs;
+    std::vector<size_t> member;
+    std::vector<std::unique_ptr<ArenaTable>> tables;
+    std::vector<std::vector<int32_t>> pers(g.size());
+    std::vector<std::vector<double>> ups(g.size()), dns(g.size());
+    for (size_t i = 0; i < g.size(); ++i) {
+        Req &q = *g[i];
+        const CacheEntry &e = *q.entry;
+        const SeriesMeta &m = e.meta;
+        const size_t T = m.len;
+        size_t n = 0;
+        vta_status v = vta_bb_expand_grid(&q.r.bb, nullptr, 0, &n);
+        std::vector<vta_bb_params> cs(n);
+        if (v == VTA_OK) v = vta_bb_expand_grid(&q.r.bb, cs.data(), n, &n);
+        if (v == VTA_OK && (n == 0 || T > (size_t)INT32_MAX || n > (size_t)INT32_MAX))
+            v = fail(VTA_ERR_INVALID_INPUT, "invalid request sizes");
+        size_t maxp = 0;
+        for (size_t c = 0; v == VTA_OK && c < n; ++c) {
+            if (cs[c].period == 0 || cs[c].period > T)
+                v = fail(VTA_ERR_INVALID_INPUT, "invalid period %zu", cs[c].period);
+            maxp = std::max(maxp, cs[c].period);
+        }
+        if (v == VTA_OK && ((size_t)m.fv_close >= T || T - (size_t)m.fv_close < maxp))
+            v = fail(VTA_ERR_INVALID_INPUT, "not enough valid data (needed >= %zu)", maxp);
+        if (v == VTA_OK) {
+            for (size_t c = 0; c < n; ++c) {
+                pers[i].push_back((int32_t)cs[c].period);
+                ups[i].push_back(cs[c].devup);
+                dns[i].push_back(cs[c].devdn);
+            }
+            for (int t = 0; t < 3 && v == VTA_OK; ++t) {
+                tables.push_back(std::make_unique<ArenaTable>());
+                v = t == 0 ? tables.back()->put(s, ctx, pers[i])
+                           : tables.back()->put(s, ctx, t == 1 ? ups[i] : dns[i]);
+            }
+        }
+        for (int o = 0; o < 3 && v == VTA_OK; ++o) v = alloc_out(s, q.res.out[o], n, T);
+        if (v != VTA_OK) {
+            st[i] = v;
+            continue;
+        }
+        q.res.n_outputs = 3;
+        q.res.n_combos = n;
+        const size_t tb = tables.size();
+        vta_bb_job j{};
+        j.d_prices = e.d_close;
+        j.d_periods = tables[tb - 3]->as<int32_t>();
+        j.d_devup = tables[tb - 2]->as<double>();
+        j.d_devdn = tables[tb - 1]->as<double>();
+        j.d_upper = q.res.out[0].ptr;
+        j.d_middle = q.res.out[1].ptr;
+        j.d_lower = q.res.out[2].ptr;
+        j.series_len = (int32_t)T;
+        j.first_valid = m.fv_close;
+        j.first_nonfinite = m.z_close;
+        j.n_combos = (int32_t)n;
+        j.max_period = (int32_t)maxp;
+        jobs.push_back(j);
+        member.push_back(i);
+    }
+    if (jobs.empty()) return true;
+    std::vector<BbPackJob> tab;
+    unsigned blocks = 0;
+    vta_status v = bb_pack_layout(ctx, jobs.data(), jobs.size(), tab, &blocks);
+    void *d_tab = nullptr;
+    if (v == VTA_OK) {
+        d_tab = s->arena.alloc(tab.size() * sizeof(BbPackJob));
+        if (!d_tab) {
+            (void)hipStreamSynchronize(ctx->stream);   // the parameter tables' copies
+            for (auto &q : g)
+                for (int o = 0; o < 3; ++o)
+                    if (q->res.out[o].ptr) {
+                        s->arena.release(q->res.out[o].ptr);
+                        q->res.out[o].ptr = nullptr;
+                    }
+            return false;
+        }
+        hipError_t e = hipMemcpyAsync(d_tab, tab.data(), tab.size() * sizeof(BbPackJob),
+                                      hipMemcpyHostToDevice, ctx->stream);
+        if (e != hipSuccess) v = fail(VTA_ERR_HIP, "hipMemcpyAsync: %s", hipGetErrorString(e));
+    }
+    if (v == VTA_OK) v = bb_pack_launch(ctx, d_tab, (int)tab.size(), blocks);
+    const double t_launch = now_s() - s->t0;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (v == VTA_OK && e != hipSuccess)
+        v = fail(VTA_ERR_HIP, "hipStreamSynchronize: %s", hipGetErrorString(e));
+    if (d_tab) s->arena.release(d_tab);
+    for (size_t i : member) {
+        st[i] = v;
+        g[i]->res.t_launch = t_launch;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->packs += 1;
+    s->packed_requests += member.size();
+    return true;
+}
+
 // The packed form of run_request for a group of EMA / RSI / ADX requests of
 // one kind whose series are on the device.  st[i] receives each member's
 // status; a member that fails validation is left out of the pack.  Returns
@@ -603,6 +705,7 @@ void finish_request(vta_server *s, vta_ctx *ctx, Req &q, vta_status st) {
 bool run_packed(vta_server *s, vta_ctx *ctx, std::vector<std::shared_ptr<Req>> &g,
                 std::vector<vta_status> &st) {
     const int kind = g[0]->r.kind;
+    if (kind == VTA_REQ_BB) return run_packed_bb(s, ctx, g, st);
     const int rk = kind == VTA_REQ_EMA ? VTA_REC_EMA : kind == VTA_REQ_RSI ? VTA_REC_RSI
                                                                              : VTA_REC_ADX;
     std::vector<vta_rec_job> jobs;
@@ -671,11 +774,11 @@ bool run_packed(vta_server *s, vta_ctx *ctx, std::vector<std::shared_ptr<Req>> &
             for (void *p : {d_tab, d_flags, d_carry})
                 if (p) s->arena.release(p);
             (void)hipStreamSynchronize(ctx->stream);   // the period tables' copies
-            for (size_t i : member) {
-                Req &q = *g[i];
-                s->arena.release(q.res.out[0].ptr);
-                q.res.out[0].ptr = nullptr;
-            }
+            for (auto &q : g)
+                if (q->res.out[0].ptr) {
+                    s->arena.release(q->res.out[0].ptr);
+                    q->res.out[0].ptr = nullptr;
+                }
             return false;
         }
         for (size_t j = 0; j < tab.size(); ++j)

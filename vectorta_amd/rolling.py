@@ -10,7 +10,7 @@ from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
-from ._lib import (BbBatchRangeC, BbParamsC, DeviceArrayF32C, PnrBatchRangeC, PnrParamsC,
+from ._lib import (BbBatchRangeC, BbJobC, BbParamsC, DeviceArrayF32C, PnrBatchRangeC, PnrParamsC,
                    RangeF64, RangeUsize, check, lib)
 from .device import DeviceArrayF32, DeviceBuffer, HipContext, f32_ptr
 
@@ -203,3 +203,37 @@ class RollingManySeriesPlan:
 
     def outputs(self, n: int = 1):
         return [b.copy_to_host().reshape(self.rows, self.cols) for b in self.d_out[:n]]
+
+
+class BbPack:
+    """Many one-series Bollinger Bands batches in one launch
+    (vta_bb_pack_*; no reference counterpart).  jobs: dicts with device
+    buffers ``prices``, ``periods`` (int32), ``devup``, ``devdn`` (f64),
+    ``upper``, ``middle``, ``lower`` (n_combos x series_len f32 each) and the
+    ints ``series_len``, ``first_valid``, ``first_nonfinite``, ``n_combos``,
+    ``max_period``.  The buffers must outlive the pack."""
+
+    def __init__(self, ctx: HipContext, jobs: Sequence[dict]):
+        self.ctx = ctx
+        arr = (BbJobC * len(jobs))()
+        for i, j in enumerate(jobs):
+            arr[i] = BbJobC(j["prices"].ptr, j["periods"].ptr, j["devup"].ptr, j["devdn"].ptr,
+                            j["upper"].ptr, j["middle"].ptr, j["lower"].ptr,
+                            int(j["series_len"]), int(j["first_valid"]),
+                            int(j["first_nonfinite"]), int(j["n_combos"]), int(j["max_period"]))
+        self._p = ctypes.c_void_p()
+        check(lib().vta_bb_pack_create(ctx.handle, arr, len(jobs), ctypes.byref(self._p)))
+
+    def launch(self) -> None:
+        check(lib().vta_bb_pack_launch(self.ctx.handle, self._p))
+
+    def close(self) -> None:
+        if self._p:
+            lib().vta_bb_pack_destroy(self._p)
+            self._p = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
