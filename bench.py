@@ -118,15 +118,36 @@ def time_form(alma, plan, d_x, d_out, steps, warmup, ev):
     return wall, mean, ks
 
 
-def cpu_baseline(x, combos, threads):
-    """The f64 oracle (C restatement of alma_row_scalar) over the whole sweep."""
+def cpu_baseline(x, combos, threads, reps=5):
+    """The f64 oracle (C restatement of alma_row_scalar) over the whole sweep on
+    `threads` cores: the median of `reps` runs after a warm-up; returns the
+    last run's rows (the error reference) and the median seconds."""
     import oracle
     sel = [(c.period, c.offset, c.sigma) for c in combos]
-    oracle.alma_batch(x[:4096].astype(np.float64), sel[:2], nthreads=threads)  # warm-up
-    t0 = time.perf_counter()
-    ref = oracle.alma_batch(x.astype(np.float64), sel, nthreads=threads)
-    dt = time.perf_counter() - t0
-    return ref, dt
+    xd = x.astype(np.float64)
+    oracle.alma_batch(xd[:4096], sel[:2], nthreads=threads)  # warm-up
+    ts = []
+    ref = None
+    for _ in range(reps):
+        ref = None
+        t0 = time.perf_counter()
+        ref = oracle.alma_batch(xd, sel, nthreads=threads)
+        ts.append(time.perf_counter() - t0)
+    return ref, float(np.median(ts))
+
+
+def cpu_baseline_one_core(x, combos, every=10, reps=3):
+    """The same oracle on one core over every `every`-th combo of the sweep
+    (a bounded sample): median seconds of `reps` runs and the combos timed."""
+    import oracle
+    sel = [(c.period, c.offset, c.sigma) for c in combos][::every]
+    xd = x.astype(np.float64)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        oracle.alma_batch(xd, sel, nthreads=1)
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), len(sel)
 
 
 def max_rel_err(g, ref):
@@ -816,7 +837,11 @@ def main():
         ref, dt = cpu_baseline(x, combos, threads)
         cpu = {"value": C * T / dt, "unit": "outputs/s", "cores": threads, "kind": "port",
                "sample": f"full {C}-combo x {T}-bar sweep, f64 C restatement of "
-                         f"alma_row_scalar, OpenMP over rows, {dt:.2f} s"}
+                         f"alma_row_scalar, OpenMP over rows, median of 5 runs {dt:.2f} s"}
+        dt1, n1 = cpu_baseline_one_core(x, combos)
+        cpu["one_core"] = {"value": n1 * T / dt1, "unit": "outputs/s", "cores": 1,
+                           "sample": f"every 10th combo ({n1}) x {T} bars, median of 3 runs "
+                                     f"{dt1:.2f} s"}
         for form, g in outputs.items():
             err, nan_ok = max_rel_err(g, ref)
             forms[form]["max_rel_err_vs_f64"] = err
