@@ -136,6 +136,22 @@ def cpu_baseline(x, combos, threads, reps=5):
     return ref, float(np.median(ts))
 
 
+def cpu_baseline_avx2(x, combos, threads, reps=3):
+    """The AVX2+FMA row (alma_row_avx2, the reference's nightly-avx build) over
+    the whole sweep: median seconds of `reps` runs, None without AVX2."""
+    import oracle
+    sel = [(c.period, c.offset, c.sigma) for c in combos]
+    xd = x.astype(np.float64)
+    if oracle.alma_batch_avx2(xd[:4096], sel[:2], nthreads=threads) is None:
+        return None
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        oracle.alma_batch_avx2(xd, sel, nthreads=threads)
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
 def cpu_baseline_one_core(x, combos, every=10, reps=3):
     """The same oracle on one core over every `every`-th combo of the sweep
     (a bounded sample): median seconds of `reps` runs and the combos timed."""
@@ -838,6 +854,12 @@ def main():
         cpu = {"value": C * T / dt, "unit": "outputs/s", "cores": threads, "kind": "port",
                "sample": f"full {C}-combo x {T}-bar sweep, f64 C restatement of "
                          f"alma_row_scalar, OpenMP over rows, median of 5 runs {dt:.2f} s"}
+        dtv = cpu_baseline_avx2(x, combos, threads)
+        if dtv is not None:
+            cpu["avx2"] = {"value": C * T / dtv, "unit": "outputs/s", "cores": threads,
+                           "sample": f"full sweep, AVX2+FMA restatement of alma_row_avx2 "
+                                     f"(the reference's nightly-avx build), median of 3 runs "
+                                     f"{dtv:.2f} s"}
         dt1, n1 = cpu_baseline_one_core(x, combos)
         cpu["one_core"] = {"value": n1 * T / dt1, "unit": "outputs/s", "cores": 1,
                            "sample": f"every 10th combo ({n1}) x {T} bars, median of 3 runs "

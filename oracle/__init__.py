@@ -60,6 +60,8 @@ def lib():
         L.or_alma_weights_batch.restype = ctypes.c_double
         L.or_alma_single.argtypes = [dp, sz, sz, ctypes.c_double, ctypes.c_double, dp]
         L.or_alma_batch.argtypes = [dp, sz, ctypes.POINTER(sz), dp, dp, sz, dp, ctypes.c_int]
+        L.or_alma_batch_avx2.argtypes = [dp, sz, ctypes.POINTER(sz), dp, dp, sz, dp,
+                                         ctypes.c_int]
         L.or_ema.argtypes = [dp, sz, sz, dp]
         L.or_rsi.argtypes = [dp, sz, sz, dp]
         L.or_atr.argtypes = [dp, dp, dp, sz, sz, dp]
@@ -112,6 +114,23 @@ def alma_weights_batch(period, offset, sigma):
     w = np.zeros(period)
     inv = lib().or_alma_weights_batch(period, offset, sigma, _d(w)[1])
     return w, inv
+
+
+def alma_batch_avx2(x, combos, nthreads=1):
+    """alma_row_avx2 (alma.rs:1529-1573) over the same combos: a CPU timing
+    baseline (different summation order).  None when the CPU lacks AVX2/FMA."""
+    x, xp = _d(x)
+    n = len(combos)
+    per = (ctypes.c_size_t * n)(*[int(c[0]) for c in combos])
+    off = np.array([c[1] for c in combos], dtype=np.float64)
+    sig = np.array([c[2] for c in combos], dtype=np.float64)
+    out = np.empty((n, x.size))
+    st = lib().or_alma_batch_avx2(xp, x.size, per, _d(off)[1], _d(sig)[1], n, _d(out)[1],
+                                  int(nthreads))
+    if st == -1:
+        return None
+    _check(st, "alma_batch_avx2")
+    return out
 
 
 def alma_batch(x, combos, nthreads=1):

@@ -41,6 +41,7 @@
  * Status codes: 0 ok, 1 empty input, 2 all NaN, 3 invalid period,
  * 4 not enough valid data, 5 invalid parameter (sigma/offset/percentage).
  */
+#include <immintrin.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -179,6 +180,68 @@ int or_alma_batch(const double *data, size_t len, const size_t *periods, const d
         double *row = out + (size_t)c * len;
         fill_nan(row, first + p - 1);
         alma_dot_rows(data, len, first, p, w, inv_n, row);
+        free(w);
+    }
+    return OR_OK;
+}
+
+/* alma.rs:1529-1573 alma_row_avx2 (the reference's "nightly-avx" build; its
+ * default build runs alma_row_scalar, restated above): four f64 lanes of FMA
+ * over the window in blocks of four, a masked tail block, then
+ * ((l0 + l2) + (l1 + l3)) * inv_n.  A CPU timing baseline only: its summation
+ * order differs from alma_row_scalar, so it is checked against or_alma_batch
+ * to a relative 1e-12, not used as the parity reference. */
+__attribute__((target("avx2,fma"))) static void alma_dot_rows_avx2(
+    const double *data, size_t len, size_t first, size_t period, const double *w, double inv_n,
+    double *out) {
+    const size_t blocks = period / 4, tail = period % 4;
+    const __m256i mask = _mm256_setr_epi64x(tail > 0 ? -1 : 0, tail > 1 ? -1 : 0,
+                                            tail > 2 ? -1 : 0, 0);
+    for (size_t i = first + period - 1; i < len; ++i) {
+        const double *d = data + (i + 1 - period);
+        __m256d acc = _mm256_setzero_pd();
+        for (size_t b = 0; b < blocks; ++b)
+            acc = _mm256_fmadd_pd(_mm256_loadu_pd(d + 4 * b), _mm256_loadu_pd(w + 4 * b), acc);
+        if (tail)
+            acc = _mm256_fmadd_pd(_mm256_maskload_pd(d + 4 * blocks, mask),
+                                  _mm256_maskload_pd(w + 4 * blocks, mask), acc);
+        const __m128d s2 = _mm_add_pd(_mm256_castpd256_pd128(acc), _mm256_extractf128_pd(acc, 1));
+        const __m128d s1 = _mm_add_pd(s2, _mm_unpackhi_pd(s2, s2));
+        out[i] = _mm_cvtsd_f64(s1) * inv_n;
+    }
+}
+
+/* or_alma_batch with the AVX2 row; -1 when the CPU lacks AVX2 or FMA. */
+int or_alma_batch_avx2(const double *data, size_t len, const size_t *periods,
+                       const double *offsets, const double *sigmas, size_t ncombos, double *out,
+                       int nthreads) {
+    __builtin_cpu_init();
+    if (!__builtin_cpu_supports("avx2") || !__builtin_cpu_supports("fma")) return -1;
+    if (len == 0) return OR_ALLNAN;
+    size_t first = or_first_valid(data, len);
+    if (first >= len) return OR_ALLNAN;
+    size_t max_p8 = 0;
+    for (size_t c = 0; c < ncombos; ++c) {
+        size_t r8 = (periods[c] + 7) & ~(size_t)7;
+        if (r8 > max_p8) max_p8 = r8;
+    }
+    if (len - first < max_p8) return OR_NOTENOUGH;
+    for (size_t c = 0; c < ncombos; ++c) {
+        if (!(sigmas[c] > 0.0)) return OR_PARAM;
+        if (!(offsets[c] >= 0.0 && offsets[c] <= 1.0)) return OR_PARAM;
+        if (periods[c] == 0) return OR_PERIOD;
+    }
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(static) if (nthreads != 1)
+#endif
+    for (long c = 0; c < (long)ncombos; ++c) {
+        size_t p = periods[c];
+        double *w = (double *)malloc(sizeof(double) * p);
+        double inv_n = or_alma_weights_batch(p, offsets[c], sigmas[c], w);
+        double *row = out + (size_t)c * len;
+        fill_nan(row, first + p - 1);
+        alma_dot_rows_avx2(data, len, first, p, w, inv_n, row);
         free(w);
     }
     return OR_OK;

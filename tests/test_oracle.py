@@ -449,3 +449,20 @@ def test_fir_nonfinite_rules():
     # Wilders rejects a non-finite bar inside the warm window (wilders.rs:223-230)
     with pytest.raises(O.OracleError):
         O.wilders(x[25:], 8)
+
+
+def test_alma_avx2_baseline_matches_scalar_rows():
+    """The AVX2 restatement of alma_row_avx2 (a CPU timing baseline) sums in
+    another order than alma_row_scalar: same NaN prefixes, values within 1e-12
+    relative, for periods with every tail length (p mod 4 = 0..3)."""
+    rng = np.random.Generator(np.random.PCG64(44))
+    x = 100.0 * np.exp(np.cumsum(rng.normal(0.0, 0.015, 3_000)))
+    x[:5] = np.nan
+    combos = [(p, o, s) for p in (4, 5, 6, 7, 9, 64, 250) for o in (0.1, 0.85) for s in (2.0, 6.0)]
+    a = O.alma_batch_avx2(x, combos, nthreads=2)
+    if a is None:
+        pytest.skip("CPU without AVX2/FMA")
+    b = O.alma_batch(x, combos, nthreads=2)
+    assert np.array_equal(np.isnan(a), np.isnan(b))
+    m = np.isfinite(b)
+    assert np.max(np.abs(a[m] - b[m]) / np.abs(b[m])) < 1e-12
