@@ -15,7 +15,7 @@
  *
  * References (paths relative to the VectorTA checkout):
  *   ALMA  src/indicators/moving_averages/alma.rs:279-340, 426-463, 1198-1289,
- *         1368-1527
+ *         1368-1527; the AVX2 row :1529-1573 (a timing baseline only)
  *   EMA   src/indicators/moving_averages/ema.rs:287-324, 430-468, 858-893
  *   RSI   src/indicators/rsi.rs:189-227, 327-414 (batch row :903-990 identical)
  *   ATR   src/indicators/atr.rs:197-230, 329-460; batch-prefix :1101-1213,
