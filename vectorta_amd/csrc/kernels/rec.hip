@@ -179,6 +179,18 @@ __device__ __forceinline__ float rsi_ratio_f32(double ag, double al) {
     return (float)(100.0 * ag) * r;
 }
 
+// DX = 100 |+DM - -DM| / (+DM + -DM) for den > 0, divided in f32 (reciprocal
+// estimate and one Newton step, <= ~2 ulp of f32); outside the estimate's
+// range the f64 quotient.  Each DX enters the ADX as a weighted mean, so its
+// ~1e-7 relative error is not amplified.
+__device__ __forceinline__ double dx_ratio(double num, double den) {
+    if (!(den > 1e-30 && den < 1e30)) return num / den;
+    const float d = (float)den;
+    float r = __builtin_amdgcn_rcpf(d);
+    r = fmaf(fmaf(-d, r, 1.0f), r, r);
+    return (double)((float)num * r);
+}
+
 struct RsiEng {
     static constexpr bool kSeedScan = true;
     double inv_p, beta, ag, al, px;
@@ -444,7 +456,7 @@ struct AdxEng {
         // cancels, leaving one ratio (zero when ATR or DM+ + DM- is zero, as
         // adx.rs:370-390 gives)
         const double sdm = pds + mds;
-        double dx = (atr != 0.0 && sdm != 0.0) ? rec_ratio(100.0 * fabs(pds - mds), sdm) : 0.0;
+        double dx = (atr != 0.0 && sdm != 0.0) ? dx_ratio(100.0 * fabs(pds - mds), sdm) : 0.0;
         if (dx_count < p) {
             dx_sum += dx;
             dx_count += 1;
@@ -493,7 +505,7 @@ struct AdxEng {
         pds = pds * omr + pdm;
         mds = mds * omr + mdm;
         const double sdm = pds + mds;
-        double dx = (atr != 0.0 && sdm != 0.0) ? rec_ratio(100.0 * fabs(pds - mds), sdm) : 0.0;
+        double dx = (atr != 0.0 && sdm != 0.0) ? dx_ratio(100.0 * fabs(pds - mds), sdm) : 0.0;
         dx_sum += dx;
     }
     // first stage: seed sums (multiplier 1) up to i = p, then omr; second
@@ -560,7 +572,7 @@ struct AdxEng {
         pds = pds * omr + pdm;
         mds = mds * omr + mdm;
         const double sdm = pds + mds;
-        double dx = (atr != 0.0 && sdm != 0.0) ? rec_ratio(100.0 * fabs(pds - mds), sdm) : 0.0;
+        double dx = (atr != 0.0 && sdm != 0.0) ? dx_ratio(100.0 * fabs(pds - mds), sdm) : 0.0;
         adx = (adx * pm1 + dx) * rp;
         return adx;
     }
