@@ -201,6 +201,19 @@ def calibrate_copy(nbytes: int = 256 << 20, reps: int = 3) -> None:
             print(f"[calib] neg {label} {s_.numel() * 4} bytes: {ms:.4f} ms, "
                   f"{2 * s_.numel() * 4 / ms / 1e6:.1f} GB/s (read + write)", flush=True)
         assert float(d_[0]) == -1.0
+    # write-only streams: the ceiling for a kernel that only writes (the ALMA
+    # sweep writes 1.0 GB and reads 2 MB)
+    big = torch.empty(1 << 28, dtype=torch.float32, device="cuda")   # 1 GiB
+    for _ in range(reps + 2):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        big.fill_(1.0)
+        b.record()
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b)
+        print(f"[calib] fill {big.numel() * 4} bytes: {ms:.4f} ms, "
+              f"{big.numel() * 4 / ms / 1e6:.1f} GB/s (write only)", flush=True)
 
 
 def bench_many_series_ohlc(steps, warmup, ev):

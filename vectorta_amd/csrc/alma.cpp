@@ -435,9 +435,11 @@ vta_status plan_launch(vta_ctx *ctx, const vta_alma_sweep_plan *plan, const floa
         a.kmax = plan->kmax;
         const int npair = plan->kmax / 2 + 15;
         const int stride = ((npair + 15) >> 5) * 32 + 16;
+        // prices, anchors, flags, the staged dwords (rounded to 16 bytes) and
+        // four per-wave 32 x 36-float output transpose tiles
         const size_t smem =
             4 * ((size_t)(32 * tiles + plan->kmax + 32) + 2 * (size_t)tiles +
-                 (size_t)tiles * 4 * (size_t)stride);
+                 (((size_t)tiles * 4 * (size_t)stride + 3) & ~(size_t)3) + 4 * 32 * 36);
         if (smem > ctx->props.sharedMemPerBlock)
             return fail(VTA_ERR_LAUNCH_CONFIG_TOO_LARGE, "MFMA staging needs %zu bytes of LDS",
                         smem);

@@ -527,7 +527,7 @@ template <int KS, int R>
 __device__ __forceinline__ void alma_mfg_wave(const alma_mfg_args &a, int nb, int tile_begin,
                                               int nt, int kg, int stride,
                                               const uint32_t *stage, const float *anch,
-                                              const int *badf) {
+                                              const int *badf, float *tbuf) {
     const int lane = threadIdx.x & 63;
     const int g = lane >> 5;
     const int m = lane & 31;
@@ -554,6 +554,16 @@ __device__ __forceinline__ void alma_mfg_wave(const alma_mfg_args &a, int nb, in
     const int hi_off = (r ? stride : 0) + d0;
     const int lo_off = 2 * stride + hi_off;
     float *row = a.out + (size_t)(my_row < 0 ? 0 : my_row) * (size_t)T;
+    // Steady-state tiles leave through the wave's LDS tile `tbuf` ([32 combos]
+    // [32 bars], rows 36 floats apart): each lane holds 16 bars of one combo
+    // in four 16-byte pieces, and after the transpose each store instruction
+    // writes 8 combos x 128 contiguous bytes (whole lines) instead of 32
+    // combos x 32 bytes.  Lane L stores bars 4 (L & 7) .. +3 of combos
+    // (L >> 3) + 8 s, s = 0..3.
+    const int tcol = 4 * (lane & 7);
+    int trow[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) trow[s] = __shfl(my_row, (lane >> 3) + 8 * s);
 
     // A fragments of K-step s of tile j; the next step's pair is loaded while
     // the current step's three MFMAs run (and tile j+1's first step during
@@ -585,18 +595,31 @@ __device__ __forceinline__ void alma_mfg_wave(const alma_mfg_args &a, int nb, in
         if (j + 1 < nt) load_a(j + 1, 0, ah_n, al_n);
         const float anchor = anch[j];
         const int t_tile = (tile_begin + j) * 32;
-        if (my_row < 0) continue;
         if (badf[j] == 0 && t_tile >= warm_max && t_tile + 32 <= T && (T & 3) == 0) {
-            // steady state: every bar of the tile is a finite output
+            // steady state (wave-uniform): every bar of the tile is a finite
+            // output; transpose through LDS, then whole-line stores
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int tb = t_tile + 8 * q + 4 * g;
-                *reinterpret_cast<float4 *>(row + tb) =
+            for (int q = 0; q < 4; ++q)
+                *reinterpret_cast<float4 *>(tbuf + m * 36 + 8 * q + 4 * g) =
                     make_float4(acc[4 * q + 0] + anchor, acc[4 * q + 1] + anchor,
                                 acc[4 * q + 2] + anchor, acc[4 * q + 3] + anchor);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const float4 v =
+                    *reinterpret_cast<const float4 *>(tbuf + ((lane >> 3) + 8 * s) * 36 + tcol);
+                if (trow[s] >= 0)
+                    *reinterpret_cast<float4 *>(a.out + (size_t)trow[s] * (size_t)T + t_tile +
+                                                tcol) = v;
             }
             continue;
         }
+        if (my_row < 0) continue;
         const bool tile_bad = badf[j] != 0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -760,6 +783,9 @@ __device__ __forceinline__ void alma_mfg_entry(const alma_mfg_args &a) {
     float *anch = xs + xsn;
     int *badf = reinterpret_cast<int *>(anch + R);
     uint32_t *stage = reinterpret_cast<uint32_t *>(badf + R);
+    // per-wave output transpose tiles after the staged dwords (16-byte aligned)
+    float *tbuf = reinterpret_cast<float *>(stage) +
+                  ((R * 4 * mfg_stride(a.kmax / 2 + 15) + 3) & ~3) + wave * 32 * 36;
     const int T = a.series_len;
     const int fv = a.first_valid;
     const int tile_begin = blockIdx.x * R;
@@ -816,7 +842,7 @@ __device__ __forceinline__ void alma_mfg_entry(const alma_mfg_args &a) {
         if (TJ == 2)                                                                       \
             alma_mfg_wave2<N, R>(a, nb, tile_begin, nt, kg, stride, stage, anch, badf);    \
         else                                                                               \
-            alma_mfg_wave<N, R>(a, nb, tile_begin, nt, kg, stride, stage, anch, badf);     \
+            alma_mfg_wave<N, R>(a, nb, tile_begin, nt, kg, stride, stage, anch, badf, tbuf); \
         break;
         MFG_CASE(1) MFG_CASE(2) MFG_CASE(3) MFG_CASE(4) MFG_CASE(5) MFG_CASE(6) MFG_CASE(7)
         MFG_CASE(8) MFG_CASE(9) MFG_CASE(10) MFG_CASE(11) MFG_CASE(12) MFG_CASE(13)
