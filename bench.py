@@ -774,8 +774,8 @@ def main():
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU sharding is design-only", file=sys.stderr)
+        print(f"bench.py: vectorta_amd runs on one GPU only (asked for --gpus {args.gpus}); "
+              "sharding the sweep over several GPUs is not implemented", file=sys.stderr)
         sys.exit(2)
 
     if args.mode == "calib":
