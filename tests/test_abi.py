@@ -181,3 +181,60 @@ def test_dlpack_export_layout_without_device(versioned):
     assert not d.data and m.deleter
     assert arr.rows == 0 and arr.cols == 0
     _lib.lib().vta_dlpack_delete(mt, versioned)
+
+
+# Entry points whose null-handle result is a documented no-op or sentinel
+# rather than VTA_ERR_INVALID_INPUT: destructors accept NULL like free(), and
+# the getters return their "none" value.
+_NULL_SENTINELS = {
+    "vta_ctx_destroy": 0, "vta_device_free": 0, "vta_device_array_free": 0,
+    "vta_dlpack_delete": None, "vta_alma_sweep_plan_destroy": 0, "vta_rec_pack_destroy": 0,
+    "vta_bb_pack_destroy": 0, "vta_graph_destroy": 0, "vta_double_alma_plan_destroy": 0,
+    "vta_server_destroy": 0, "vta_ctx_device_id": -1, "vta_ctx_stream": None,
+    "vta_alma_selected_batch_kernel": 0, "vta_alma_sweep_plan_combos": 0,
+    "vta_graph_num_nodes": 0, "vta_double_alma_plan_pairs": 0, "vta_server_now": 0.0,
+}
+
+_NULL_PROBE = r"""
+import ctypes, json, sys
+from vectorta_amd import _lib
+L = _lib.lib()
+ints = (ctypes.c_int, ctypes.c_int32, ctypes.c_int64, ctypes.c_size_t, ctypes.c_uint32,
+        ctypes.c_uint64)
+for name, (res, args) in sorted(_lib.SIGNATURES.items()):
+    if not args:
+        continue
+    vals = [0.0 if a in (ctypes.c_float, ctypes.c_double) else 0 if a in ints else None
+            for a in args]
+    print("calling", name, flush=True)
+    r = getattr(L, name)(*vals)
+    e = L.vta_last_error()
+    print(json.dumps({"name": name, "ret": r, "err": e.decode() if e else ""}), flush=True)
+"""
+
+
+def test_null_arguments_rejected_without_device():
+    """Every entry point given null handles and zero sizes returns
+    VTA_ERR_INVALID_INPUT with a message before touching HIP (the reference
+    wrappers' InvalidInput, alma_wrapper.rs:72-101), or its documented no-op /
+    sentinel; none crashes.  Runs in a child process so a crash names the
+    entry point it happened in."""
+    import json
+    import subprocess
+    import sys
+    p = subprocess.run([sys.executable, "-c", _NULL_PROBE], cwd=ROOT, capture_output=True,
+                       text=True, timeout=120)
+    lines = p.stdout.splitlines()
+    assert p.returncode == 0, (p.returncode, lines[-1:] if lines else p.stderr[-500:])
+    seen = {}
+    for ln in lines:
+        if ln.startswith("{"):
+            d = json.loads(ln)
+            seen[d["name"]] = d
+    expect = {n for n, (_, a) in _lib.SIGNATURES.items() if a}
+    assert set(seen) == expect
+    for name, d in seen.items():
+        if name in _NULL_SENTINELS:
+            assert d["ret"] == _NULL_SENTINELS[name], d
+        else:
+            assert d["ret"] == 4 and d["err"], d

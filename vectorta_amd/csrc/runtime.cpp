@@ -362,6 +362,8 @@ vta_status vta_ctx_preload(vta_ctx *ctx) {
         {"cross", "cross_reduce"},
         {"rolling", "mad_ms1p_f32"},
     };
+    clear_error();
+    if (!ctx) return fail(VTA_ERR_INVALID_INPUT, "ctx is null");
     for (auto &s : kSymbols) {
         hipFunction_t f;
         VTA_TRY(get_function(ctx, s[0], s[1], &f));
