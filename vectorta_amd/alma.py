@@ -13,7 +13,7 @@ import numpy as np
 
 from ._lib import (AlmaBatchRangeC, AlmaParamsC, DeviceArrayF32C, RangeF64, RangeUsize,
                    VtaError, check, lib)
-from .device import DeviceArrayF32, DeviceBuffer, HipContext, f32_ptr
+from .device import DeviceArrayF32, DeviceBuffer, HipContext, f32_ptr, require_len
 
 
 @dataclass
@@ -116,6 +116,7 @@ class CudaAlma:
 
     def alma_batch_from_device_prices(self, d_prices: DeviceBuffer, series_len: int,
                                       first_valid: int, sweep: AlmaBatchRange) -> DeviceArrayF32:
+        require_len(d_prices, series_len, "prices")
         rc = sweep.to_c()
         out = DeviceArrayF32C()
         check(lib().vta_alma_batch_from_device_prices(
@@ -136,6 +137,12 @@ class CudaAlma:
     def alma_batch_device(self, d_prices, d_weights, d_periods, d_inv_norms, max_period: int,
                           series_len: int, n_combos: int, first_valid: int, d_out,
                           time_major: bool = False) -> None:
+        # alma_wrapper.rs:625-669 shapes: weights [n_combos][max_period]
+        require_len(d_prices, series_len, "prices")
+        require_len(d_weights, int(n_combos) * int(max_period), "weights")
+        require_len(d_periods, n_combos, "periods")
+        require_len(d_inv_norms, n_combos, "inv_norms")
+        require_len(d_out, int(n_combos) * int(series_len), "output")
         fn = lib().vta_alma_batch_device_tm if time_major else lib().vta_alma_batch_device
         check(fn(self.ctx.handle, ctypes.c_void_p(d_prices.ptr), ctypes.c_void_p(d_weights.ptr),
                  ctypes.c_void_p(d_periods.ptr), ctypes.c_void_p(d_inv_norms.ptr),

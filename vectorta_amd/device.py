@@ -13,6 +13,20 @@ import numpy as np
 from ._lib import DeviceArrayF32C, VtaError, check, lib
 
 
+def require_len(buf, count: int, what: str, itemsize: int = 4) -> None:
+    """Length check of a caller-owned device buffer, as the reference wrappers
+    make on their DeviceBuffer arguments (ema_wrapper.rs:282-296, 400-409): a
+    buffer shorter than what the launch reads or writes is InvalidInput, raised
+    before anything is enqueued.  ``None`` and objects that carry no byte size
+    (raw pointers) pass unchecked; the C ABI itself only sees pointers."""
+    if buf is None:
+        return
+    nbytes = getattr(buf, "nbytes", None)
+    if nbytes is not None and int(nbytes) < int(count) * itemsize:
+        raise VtaError(4, f"{what} buffer length mismatch: it holds {int(nbytes) // itemsize} "
+                          f"elements, the launch needs {int(count)}")
+
+
 def cuda_available() -> bool:
     """Device probe (src/cuda/mod.rs:412-582); VTA_FORCE_SKIP=1 forces False."""
     try:

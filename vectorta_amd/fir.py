@@ -9,7 +9,7 @@ from typing import List, Tuple
 import numpy as np
 
 from ._lib import DeviceArrayF32C, check, lib
-from .device import DeviceArrayF32, DeviceBuffer, HipContext, f32_ptr
+from .device import DeviceArrayF32, DeviceBuffer, HipContext, f32_ptr, require_len
 from .rec import _expand, _rng
 
 
@@ -45,6 +45,9 @@ class _Fir:
     def _batch_device(self, d_prices: DeviceBuffer, d_periods: DeviceBuffer, series_len: int,
                       n_combos: int, first_valid: int, max_period: int,
                       d_out: DeviceBuffer) -> None:
+        require_len(d_prices, series_len, "prices")
+        require_len(d_periods, n_combos, "periods")
+        require_len(d_out, int(n_combos) * int(series_len), "output")
         check(getattr(lib(), f"vta_{self._name}_batch_device")(
             self.ctx.handle, _vp(d_prices), _vp(d_periods), int(series_len), int(n_combos),
             int(first_valid), int(max_period), _vp(d_out)))
@@ -59,6 +62,10 @@ class _Fir:
 
     def _ms_device(self, d_prices_tm, d_first_valids, cols, rows, period, d_out_tm,
                    d_first_nonfinite=None) -> None:
+        require_len(d_prices_tm, int(cols) * int(rows), "time-major input")
+        require_len(d_out_tm, int(cols) * int(rows), "time-major output")
+        require_len(d_first_valids, cols, "first_valids")
+        require_len(d_first_nonfinite, cols, "first_nonfinite")
         fn = getattr(lib(), f"vta_{self._name}_many_series_one_param_device")
         check(fn(self.ctx.handle, _vp(d_prices_tm), _vp(d_first_valids), _vp(d_first_nonfinite),
                  int(cols), int(rows), int(period), _vp(d_out_tm)))

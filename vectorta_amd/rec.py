@@ -10,7 +10,7 @@ from typing import List, Sequence, Tuple
 import numpy as np
 
 from ._lib import DeviceArrayF32C, RangeUsize, RecJobC, VtaError, check, lib
-from .device import DeviceArrayF32, DeviceBuffer, HipContext, f32_ptr
+from .device import DeviceArrayF32, DeviceBuffer, HipContext, f32_ptr, require_len
 
 
 def _rng(r: Sequence[int]) -> RangeUsize:
@@ -64,6 +64,24 @@ class _OneParam:
         return DeviceArrayF32(self.ctx, out)
 
 
+def _need_batch(d_prices, d_periods, d_out, series_len, n_combos) -> None:
+    """One series x n_combos periods: series_len bars in, n_combos int32
+    periods, an n_combos x series_len output (ema_wrapper.rs:282-296)."""
+    require_len(d_prices, series_len, "prices")
+    require_len(d_periods, n_combos, "periods")
+    require_len(d_out, int(n_combos) * int(series_len), "output")
+
+
+def _need_ms(d_prices_tm, d_first_valids, d_out_tm, d_clean, num_series, series_len) -> None:
+    """Time-major many-series forms: series_len x num_series in and out, one
+    int32 first_valid (and clean flag) per series (ema_wrapper.rs:400-409)."""
+    total = int(num_series) * int(series_len)
+    require_len(d_prices_tm, total, "time-major input")
+    require_len(d_out_tm, total, "time-major output")
+    require_len(d_first_valids, num_series, "first_valids")
+    require_len(d_clean, num_series, "clean flags")
+
+
 class CudaEma(_OneParam):
     _name = "ema"
 
@@ -73,6 +91,7 @@ class CudaEma(_OneParam):
     def ema_batch_device(self, d_prices: DeviceBuffer, d_periods: DeviceBuffer, max_period: int,
                          series_len: int, first_valid: int, n_combos: int,
                          d_out: DeviceBuffer) -> None:
+        _need_batch(d_prices, d_periods, d_out, series_len, n_combos)
         check(lib().vta_ema_batch_device(self.ctx.handle, ctypes.c_void_p(d_prices.ptr),
                                          ctypes.c_void_p(d_periods.ptr), None, int(max_period),
                                          int(series_len), int(first_valid), int(n_combos),
@@ -83,6 +102,7 @@ class CudaEma(_OneParam):
 
     def ema_many_series_one_param_device(self, d_prices_tm, d_first_valids, period, num_series,
                                          series_len, d_out_tm, d_clean=None) -> None:
+        _need_ms(d_prices_tm, d_first_valids, d_out_tm, d_clean, num_series, series_len)
         check(lib().vta_ema_many_series_one_param_device(
             self.ctx.handle, ctypes.c_void_p(d_prices_tm.ptr), ctypes.c_void_p(d_first_valids.ptr),
             int(period), int(num_series), int(series_len),
@@ -114,6 +134,9 @@ class CudaAtr(_OneParam):
                            d_close: DeviceBuffer, series_len: int, first_valid: int,
                            d_tr_out: DeviceBuffer) -> None:
         """atr_wrapper.rs:566: enqueue-only true range of one device-resident series."""
+        for b, what in ((d_high, "high"), (d_low, "low"), (d_close, "close"),
+                        (d_tr_out, "tr output")):
+            require_len(b, series_len, what)
         check(lib().vta_tr_from_hlc_device(self.ctx.handle, ctypes.c_void_p(d_high.ptr),
                                            ctypes.c_void_p(d_low.ptr), ctypes.c_void_p(d_close.ptr),
                                            int(series_len), int(first_valid),
@@ -123,6 +146,7 @@ class CudaAtr(_OneParam):
                                  max_period: int, series_len: int, first_valid: int,
                                  n_combos: int, d_out: DeviceBuffer) -> None:
         """atr_wrapper.rs:626: enqueue-only ATR of every period over a TR series."""
+        _need_batch(d_tr, d_periods, d_out, series_len, n_combos)
         check(lib().vta_atr_batch_device_with_tr(self.ctx.handle, ctypes.c_void_p(d_tr.ptr),
                                                  ctypes.c_void_p(d_periods.ptr), int(max_period),
                                                  int(series_len), int(first_valid),
@@ -149,6 +173,7 @@ class _OneInputMa(_OneParam):
     def _batch_device(self, d_prices: DeviceBuffer, d_periods: DeviceBuffer, max_period: int,
                       series_len: int, first_valid: int, n_combos: int,
                       d_out: DeviceBuffer) -> None:
+        _need_batch(d_prices, d_periods, d_out, series_len, n_combos)
         fn = getattr(lib(), f"vta_{self._name}_batch_device")
         check(fn(self.ctx.handle, ctypes.c_void_p(d_prices.ptr), ctypes.c_void_p(d_periods.ptr),
                  int(max_period), int(series_len), int(first_valid), int(n_combos),
@@ -156,6 +181,7 @@ class _OneInputMa(_OneParam):
 
     def _ms_device(self, d_prices_tm, d_first_valids, period, num_series, series_len, d_out_tm,
                    d_clean=None) -> None:
+        _need_ms(d_prices_tm, d_first_valids, d_out_tm, d_clean, num_series, series_len)
         fn = getattr(lib(), f"vta_{self._name}_many_series_one_param_device")
         check(fn(self.ctx.handle, ctypes.c_void_p(d_prices_tm.ptr),
                  ctypes.c_void_p(d_first_valids.ptr), int(period), int(num_series),
@@ -261,6 +287,12 @@ class RecPack:
         self.kind = kind
         arr = (RecJobC * len(jobs))()
         for i, j in enumerate(jobs):
+            T, C = int(j["series_len"]), int(j["n_combos"])
+            require_len(j["close"], T, "close")
+            require_len(j["periods"], C, "periods")
+            require_len(j["out"], C * T, "output")
+            require_len(j.get("high"), T, "high")
+            require_len(j.get("low"), T, "low")
             arr[i] = RecJobC(j["close"].ptr, j["high"].ptr if j.get("high") else None,
                              j["low"].ptr if j.get("low") else None, j["periods"].ptr,
                              j["out"].ptr, int(j["series_len"]), int(j["first_valid"]),

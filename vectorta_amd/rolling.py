@@ -12,7 +12,7 @@ import numpy as np
 
 from ._lib import (BbBatchRangeC, BbJobC, BbParamsC, DeviceArrayF32C, PnrBatchRangeC, PnrParamsC,
                    RangeF64, RangeUsize, check, lib)
-from .device import DeviceArrayF32, DeviceBuffer, HipContext, f32_ptr
+from .device import DeviceArrayF32, DeviceBuffer, HipContext, f32_ptr, require_len
 
 
 def _f32(a) -> np.ndarray:
@@ -90,6 +90,12 @@ class CudaBollingerBands(_Rolling):
             d_first_nonfinite: Optional[DeviceBuffer], cols: int, rows: int, period: int,
             devup: float, devdn: float, d_upper: DeviceBuffer, d_middle: DeviceBuffer,
             d_lower: DeviceBuffer) -> None:
+        total = int(cols) * int(rows)
+        for b, what in ((d_data_tm, "time-major input"), (d_upper, "upper"),
+                        (d_middle, "middle"), (d_lower, "lower")):
+            require_len(b, total, what)
+        require_len(d_first_valids, cols, "first_valids")
+        require_len(d_first_nonfinite, cols, "first_nonfinite")
         check(lib().vta_bollinger_bands_many_series_one_param_device(
             self.ctx.handle, ctypes.c_void_p(d_data_tm.ptr), ctypes.c_void_p(d_first_valids.ptr),
             ctypes.c_void_p(d_first_nonfinite.ptr) if d_first_nonfinite is not None else None,
@@ -217,6 +223,13 @@ class BbPack:
         self.ctx = ctx
         arr = (BbJobC * len(jobs))()
         for i, j in enumerate(jobs):
+            T, C = int(j["series_len"]), int(j["n_combos"])
+            require_len(j["prices"], T, "prices")
+            require_len(j["periods"], C, "periods")
+            require_len(j["devup"], C, "devup", itemsize=8)
+            require_len(j["devdn"], C, "devdn", itemsize=8)
+            for band in ("upper", "middle", "lower"):
+                require_len(j[band], C * T, band)
             arr[i] = BbJobC(j["prices"].ptr, j["periods"].ptr, j["devup"].ptr, j["devdn"].ptr,
                             j["upper"].ptr, j["middle"].ptr, j["lower"].ptr,
                             int(j["series_len"]), int(j["first_valid"]),
