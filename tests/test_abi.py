@@ -238,3 +238,60 @@ def test_null_arguments_rejected_without_device():
             assert d["ret"] == _NULL_SENTINELS[name], d
         else:
             assert d["ret"] == 4 and d["err"], d
+
+
+_GRID_PROBE = r"""
+import ctypes, json
+from vectorta_amd import _lib
+L = _lib.lib()
+U, F = _lib.RangeUsize, _lib.RangeF64
+M, INF = 2**64 - 1, float("inf")
+n = ctypes.c_size_t()
+def run(tag, call):
+    print("calling", tag, flush=True)
+    st = call()
+    print(json.dumps({"tag": tag, "st": st, "n": n.value}), flush=True)
+alma = lambda p, o, s: L.vta_alma_expand_grid(
+    ctypes.byref(_lib.AlmaBatchRangeC(U(*p), F(*o), F(*s))), None, 0, ctypes.byref(n))
+run("alma_huge_period", lambda: alma((1, M, 1), (0.85, 0.85, 0), (6, 6, 0)))
+run("alma_inf_offset", lambda: alma((9, 9, 0), (0.0, INF, 0.1), (6, 6, 0)))
+run("alma_dense_sigma", lambda: alma((9, 9, 0), (0.85, 0.85, 0), (1.0, 100.0, 1e-9)))
+run("alma_grid_product", lambda: alma((1, 4096, 1), (0.0, 0.4095, 1e-4), (1.0, 2.0, 1.0)))
+run("ema_limit", lambda: L.vta_ema_expand_grid(ctypes.byref(U(1, 1 << 24, 1)), None, 0,
+                                              ctypes.byref(n)))
+run("ema_over_limit", lambda: L.vta_ema_expand_grid(ctypes.byref(U(1, (1 << 24) + 1, 1)), None,
+                                                   0, ctypes.byref(n)))
+run("mad_huge_down", lambda: L.vta_medium_ad_expand_grid(ctypes.byref(U(M, 1, 1)), None, 0,
+                                                        ctypes.byref(n)))
+run("bb_inf_devdn", lambda: L.vta_bb_expand_grid(
+    ctypes.byref(_lib.BbBatchRangeC(U(20, 20, 0), F(2, 2, 0), F(-INF, 2.0, 0.5))), None, 0,
+    ctypes.byref(n)))
+run("bb_grid_product", lambda: L.vta_bb_expand_grid(
+    ctypes.byref(_lib.BbBatchRangeC(U(1, 4096, 1), F(0, 4095, 1), F(1, 2, 1))), None, 0,
+    ctypes.byref(n)))
+run("pnr_dense_pct", lambda: L.vta_pnr_expand_grid(
+    ctypes.byref(_lib.PnrBatchRangeC(U(20, 20, 0), F(0.0, 100.0, 1e-6))), None, 0,
+    ctypes.byref(n)))
+"""
+
+
+def test_oversized_and_infinite_grids_refused():
+    """Parameter ranges whose expansion would exhaust host memory or never end
+    (a usize axis up to SIZE_MAX, an infinite f64 bound, a step of 1e-9 over a
+    span of 100, a grid product above 2^24) return InvalidInput instead of
+    aborting the process; an axis of exactly 2^24 values is still expanded.
+    Grids below the limit follow the reference's rules unchanged (the grid
+    tests above)."""
+    import json
+    import subprocess
+    import sys
+    p = subprocess.run([sys.executable, "-c", _GRID_PROBE], cwd=ROOT, capture_output=True,
+                       text=True, timeout=120)
+    lines = p.stdout.splitlines()
+    assert p.returncode == 0, (p.returncode, lines[-1:], p.stderr[-300:])
+    got = {d["tag"]: d for d in (json.loads(ln) for ln in lines if ln.startswith("{"))}
+    assert len(got) == 10
+    assert got["ema_limit"]["st"] == 0 and got["ema_limit"]["n"] == 1 << 24
+    for tag, d in got.items():
+        if tag != "ema_limit":
+            assert d["st"] == 4, d

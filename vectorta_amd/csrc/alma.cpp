@@ -21,6 +21,8 @@ bool axis_usize(vta_range_usize r, std::vector<size_t> &v) {
         v.push_back(r.start);
         return true;
     }
+    const size_t span = r.start < r.end ? r.end - r.start : r.start - r.end;
+    if (span / r.step >= kMaxGridValues) return false;   // > 2^24 values
     if (r.start < r.end) {
         for (size_t x = r.start; x <= r.end; x += r.step) {
             v.push_back(x);
@@ -43,6 +45,8 @@ bool axis_f64(vta_range_f64 r, std::vector<double> &v) {
         return true;
     }
     double st = std::fabs(r.step);
+    if (std::isinf(r.start) || std::isinf(r.end)) return false;   // would never end
+    if (std::fabs(r.end - r.start) / st >= (double)kMaxGridValues) return false;
     if (r.start < r.end) {
         for (double x = r.start; x <= r.end + 1e-12; x += st) v.push_back(x);
         return !v.empty();
@@ -57,11 +61,17 @@ vta_status expand(const vta_alma_batch_range *range, std::vector<vta_alma_params
     std::vector<double> os, ss;
     if (!axis_usize(range->period, ps) || !axis_f64(range->offset, os) ||
         !axis_f64(range->sigma, ss))
-        return fail(VTA_ERR_INVALID_INPUT, "Invalid range: start=%zu, end=%zu, step=%zu",
-                    range->period.start, range->period.end, range->period.step);
+        return fail(VTA_ERR_INVALID_INPUT,
+                    "Invalid range: period=(%zu, %zu, %zu) offset=(%g, %g, %g) "
+                    "sigma=(%g, %g, %g) (empty, infinite or above %zu values)",
+                    range->period.start, range->period.end, range->period.step,
+                    range->offset.start, range->offset.end, range->offset.step,
+                    range->sigma.start, range->sigma.end, range->sigma.step, kMaxGridValues);
     size_t cap;
     if (!mul_ok(ps.size(), os.size(), &cap) || !mul_ok(cap, ss.size(), &cap))
         return fail(VTA_ERR_INVALID_INPUT, "range size overflow");
+    if (!grid_fits(cap))
+        return fail(VTA_ERR_INVALID_INPUT, "parameter grid too large: %zu combinations", cap);
     out.clear();
     out.reserve(cap);
     for (size_t p : ps)

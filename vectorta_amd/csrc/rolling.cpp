@@ -117,6 +117,9 @@ vta_status expand_bb(const vta_bb_batch_range *r, std::vector<Combo> &out) {
     if (!expand_usize(r->period, AXIS_DOWN, ps) || !expand_f64(r->devup, us) ||
         !expand_f64(r->devdn, ds))
         return fail(VTA_ERR_INVALID_INPUT, "invalid range");
+    size_t n;
+    if (!mul_ok(ps.size(), us.size(), &n) || !mul_ok(n, ds.size(), &n) || !grid_fits(n))
+        return fail(VTA_ERR_INVALID_INPUT, "parameter grid too large");
     out.clear();
     for (size_t p : ps)
         for (double u : us)
@@ -130,6 +133,9 @@ vta_status expand_pnr(const vta_pnr_batch_range *r, std::vector<Combo> &out) {
     std::vector<double> ps;
     if (!expand_usize(r->length, AXIS_DOWN, ls) || !expand_f64(r->percentage, ps))
         return fail(VTA_ERR_INVALID_INPUT, "invalid range");
+    size_t n;
+    if (!mul_ok(ls.size(), ps.size(), &n) || !grid_fits(n))
+        return fail(VTA_ERR_INVALID_INPUT, "parameter grid too large");
     out.clear();
     for (size_t l : ls)
         for (double p : ps) out.push_back({l, p, 0.0});

@@ -156,6 +156,7 @@ bool expand_usize(vta_range_usize r, AxisKind kind, std::vector<size_t> &v) {
     }
     size_t lo = r.start < r.end ? r.start : r.end;
     size_t hi = r.start < r.end ? r.end : r.start;
+    if ((hi - lo) / r.step >= kMaxGridValues) return false;   // > 2^24 values
     if (r.start < r.end || kind == AXIS_SORTED || kind == AXIS_ATR) {
         for (size_t x = lo;; x += r.step) {
             v.push_back(x);
@@ -180,6 +181,10 @@ bool expand_f64(vta_range_f64 r, std::vector<double> &v) {
         return true;
     }
     double st = std::fabs(r.step);
+    // an infinite bound never ends the loop; a NaN one (or a NaN step) ends it
+    // at once, so only the span over a finite step is bounded here
+    if (std::isinf(r.start) || std::isinf(r.end)) return false;
+    if (std::fabs(r.end - r.start) / st >= (double)kMaxGridValues) return false;
     if (r.start < r.end) {
         for (double x = r.start; x <= r.end + 1e-12; x += st) v.push_back(x);
     } else {

@@ -112,6 +112,14 @@ size_t first_valid_f32(const float *x, size_t n);  // n if all NaN
 //   AXIS_DOWN_END like AXIS_DOWN, then end appended when the countdown misses it
 //                (wma_wrapper.rs:324-350)
 enum AxisKind { AXIS_SORTED, AXIS_DOWN, AXIS_ATR, AXIS_DOWN_END };
+// Largest axis, and largest grid (product of axes), an expansion builds:
+// 16,777,216 values.  Wider ranges, and f64 axes with an infinite bound that
+// survives the single-value rule, are refused (the expansion returns false,
+// the caller InvalidInput) instead of exhausting host memory or looping
+// forever, which is what the reference's collecting loops would do.
+constexpr size_t kMaxGridValues = size_t(1) << 24;
+// True when n values stay within kMaxGridValues.
+inline bool grid_fits(size_t n) { return n <= kMaxGridValues; }
 bool expand_usize(vta_range_usize r, AxisKind kind, std::vector<size_t> &v);
 // f64 axis: values accumulate x += step while x <= end + 1e-12 (or descend
 // while x + 1e-12 >= end), alma_wrapper.rs:1576-1612.
