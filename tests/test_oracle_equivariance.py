@@ -14,6 +14,8 @@ wrong sign, a shifted index or an unnormalised weight breaks them:
 - percentile nearest rank: selects an element, so it commutes exactly with
   any strictly increasing map (affine a > 0, exp);
 - rolling median absolute deviation: scales by |a|.
+Also: the time-major (one series per column) restatements that check the
+many-series kernels equal the single-series functions column by column.
 CPU only."""
 import numpy as np
 import pytest
@@ -104,3 +106,45 @@ def test_medium_ad_scales_by_abs_a():
     base = O.medium_ad(x, 15)
     for a, b in ((2.0, 9.0), (-3.0, 1.0)):
         _close(O.medium_ad(a * x + b, 15), abs(a) * base, abs(a) * 200.0 + abs(b))
+
+
+def _ragged_tm(seed, rows=400, cols=7):
+    rng = np.random.default_rng(seed)
+    tm = 50.0 * np.exp(np.cumsum(rng.normal(0.0, 0.01, (rows, cols)), axis=0))
+    for s, lead in enumerate((0, 3, 17, 50, 1, 120, 9)[:cols]):
+        tm[:lead, s] = np.nan
+    return tm
+
+
+def test_time_major_oracles_equal_their_columns():
+    """The time-major restatements (one series per column, ragged leading NaNs,
+    two threads) equal the single-series functions column by column, bit for
+    bit: a stride or column-offset mistake in the gathering fails here."""
+    tm = _ragged_tm(17)
+    single = {
+        "alma": (lambda x: O.alma(x, 12, 0.85, 6.0), (12, 0.85, 6.0)),
+        "ema": (lambda x: O.ema(x, 12), (12,)),
+        "rsi": (lambda x: O.rsi(x, 12), (12,)),
+        "pnr": (lambda x: O.pnr(x, 12, 70.0), (12, 70.0)),
+        "medium_ad": (lambda x: O.medium_ad(x, 12), (12,)),
+    }
+    for kind, (f, params) in single.items():
+        got, st = O.many_series(kind, tm, *params, nthreads=2)
+        assert not st.any(), (kind, st)
+        for s in range(tm.shape[1]):
+            np.testing.assert_array_equal(got[:, s], f(tm[:, s]), err_msg=f"{kind} col {s}")
+    for kind in ("sma", "wma", "swma", "smma", "wilders"):
+        got, st = O.ma_many(kind, tm, 9, nthreads=2)
+        assert not st.any(), (kind, st)
+        for s in range(tm.shape[1]):
+            np.testing.assert_array_equal(got[:, s], getattr(O, kind)(tm[:, s], 9),
+                                          err_msg=f"{kind} col {s}")
+    rng = np.random.default_rng(18)
+    spread = np.abs(rng.normal(0.0, 0.005, tm.shape))
+    h, lo = tm * (1 + spread), tm * (1 - spread)
+    for kind, f in (("atr", O.atr), ("adx", O.adx)):
+        got, st = O.many_series_hlc(kind, h, lo, tm, 10, nthreads=2)
+        assert not st.any(), (kind, st)
+        for s in range(tm.shape[1]):
+            np.testing.assert_array_equal(got[:, s], f(h[:, s], lo[:, s], tm[:, s], 10),
+                                          err_msg=f"{kind} col {s}")
