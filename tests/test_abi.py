@@ -272,13 +272,19 @@ run("bb_grid_product", lambda: L.vta_bb_expand_grid(
 run("pnr_dense_pct", lambda: L.vta_pnr_expand_grid(
     ctypes.byref(_lib.PnrBatchRangeC(U(20, 20, 0), F(0.0, 100.0, 1e-6))), None, 0,
     ctypes.byref(n)))
+run("alma_stuck_sigma", lambda: alma((9, 9, 0), (0.85, 0.85, 0), (1e17, 1e17 + 4096, 1.0)))
+run("alma_stuck_offset_down", lambda: alma((9, 9, 0), (1e17 + 4096, 1e17, 1.0), (6, 6, 0)))
+run("bb_stuck_devup", lambda: L.vta_bb_expand_grid(
+    ctypes.byref(_lib.BbBatchRangeC(U(20, 20, 0), F(1e17, 1e17 + 4096, 1.0), F(2, 2, 0))), None,
+    0, ctypes.byref(n)))
 """
 
 
 def test_oversized_and_infinite_grids_refused():
     """Parameter ranges whose expansion would exhaust host memory or never end
     (a usize axis up to SIZE_MAX, an infinite f64 bound, a step of 1e-9 over a
-    span of 100, a grid product above 2^24) return InvalidInput instead of
+    span of 100, a grid product above 2^24, a step below half the rounding
+    unit of the start, which never moves x += step) return InvalidInput instead of
     aborting the process; an axis of exactly 2^24 values is still expanded.
     Grids below the limit follow the reference's rules unchanged (the grid
     tests above)."""
@@ -290,7 +296,7 @@ def test_oversized_and_infinite_grids_refused():
     lines = p.stdout.splitlines()
     assert p.returncode == 0, (p.returncode, lines[-1:], p.stderr[-300:])
     got = {d["tag"]: d for d in (json.loads(ln) for ln in lines if ln.startswith("{"))}
-    assert len(got) == 10
+    assert len(got) == 13
     assert got["ema_limit"]["st"] == 0 and got["ema_limit"]["n"] == 1 << 24
     for tag, d in got.items():
         if tag != "ema_limit":

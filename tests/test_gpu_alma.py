@@ -184,3 +184,79 @@ def test_errors(alma):
     ref = O.alma_batch(x.astype(np.float64), _combos(sweep))
     nan_mask_equal(g, ref)
     assert max_rel_err(g, ref) <= 1e-5
+
+
+@pytest.mark.parametrize("policy", ["fp32", "auto"])
+def test_single_series_gbm_100k_vs_alma_scalar(alma, policy):
+    """Plain ALMA on one series: 100,000 closes from the seeded geometric walk
+    (log-return sd 0.01), period 9, offset 0.85, sigma 6, one combo through
+    the batch entry point, against the single-series scalar path
+    (alma_scalar via alma_with_kernel, alma.rs:342-350, 426-463, whose weight
+    rounding w = exp((-d*d)*inv_s2) differs from the batch table's).  The
+    8-element NaN prefix is exact and every value within 1e-5 relative."""
+    x = datagen.alma_single_series()
+    assert x.size == 100_000 and not np.isnan(x).any()
+    sweep = vta.AlmaBatchRange(period=(9, 9, 0), offset=(0.85, 0.85, 0.0), sigma=(6.0, 6.0, 0.0))
+    alma.set_policy(policy)
+    g = alma.alma_batch_dev(x, sweep).to_host()
+    assert g.shape == (1, 100_000)
+    assert alma.selected_batch_kernel() == "fp32"
+    ref = O.alma(x.astype(np.float64), 9, 0.85, 6.0)
+    assert np.isnan(ref[:8]).all() and not np.isnan(ref[8:]).any()
+    nan_mask_equal(g[0], ref)
+    err = max_rel_err(g[0], ref)
+    print(f"alma single 100k p=9 ({policy}) max rel err {err:.3e}")
+    assert err <= 1e-5
+    # the same series through the many-series form (one column)
+    m = alma.alma_multi_series_one_param_time_major_dev(x.reshape(-1, 1), 1, x.size,
+                                                        vta.AlmaParams(9, 0.85, 6.0)).to_host()
+    nan_mask_equal(m.ravel(), ref)
+    assert max_rel_err(m.ravel(), ref) <= 1e-5
+
+
+@pytest.mark.parametrize("policy", POLICIES)
+def test_batch_from_device_prices(alma, policy):
+    """alma_batch_from_device_prices (alma_wrapper.rs:671-681): prices already
+    on the device, caller-supplied first_valid; device-resident output."""
+    x = datagen.gbm_close(30_000, 0.015, seed=25)
+    x[:6] = np.nan
+    sweep = vta.AlmaBatchRange(period=(10, 250, 30), offset=(0.05, 0.95, 0.3),
+                               sigma=(2.0, 10.0, 4.0))
+    alma.set_policy(policy)
+    d_x = alma.ctx.to_device(x)
+    arr = alma.alma_batch_from_device_prices(d_x, x.size, 6, sweep)
+    combos = _combos(sweep)
+    assert (arr.rows, arr.cols) == (len(combos), x.size)
+    g = arr.to_host()
+    ref = O.alma_batch(x.astype(np.float64), combos)
+    nan_mask_equal(g, ref)
+    err = max_rel_err(g, ref)
+    print(f"alma from_device_prices {policy} max rel err {err:.3e}")
+    assert err <= 1e-5
+    with pytest.raises(vta.VtaError):
+        alma.alma_batch_from_device_prices(d_x, x.size, x.size - 5, sweep)
+
+
+@pytest.mark.parametrize("policy", POLICIES)
+def test_batch_into_host_f32(alma, policy):
+    """alma_batch_into_host_f32 (alma_wrapper.rs:783-817): the same grid copied
+    into a caller-owned host slice, row-major [combo][t]."""
+    x = datagen.gbm_close(25_000, 0.015, seed=26)
+    x[:2] = np.nan
+    sweep = vta.AlmaBatchRange(period=(9, 200, 17), offset=(0.15, 0.85, 0.35),
+                               sigma=(3.0, 9.0, 3.0))
+    combos = _combos(sweep)
+    alma.set_policy(policy)
+    out = np.full(len(combos) * x.size, -1.0, dtype=np.float32)
+    rows, cols, got = alma.alma_batch_into_host_f32(x, sweep, out)
+    assert (rows, cols) == (len(combos), x.size)
+    assert [(c.period, c.offset, c.sigma) for c in got] == combos
+    ref = O.alma_batch(x.astype(np.float64), combos)
+    g = out.reshape(rows, cols)
+    nan_mask_equal(g, ref)
+    assert max_rel_err(g, ref) <= 1e-5
+    # the same values as the device-resident form
+    d = alma.alma_batch_dev(x, sweep).to_host()
+    assert np.array_equal(d.view(np.uint32), g.view(np.uint32))
+    with pytest.raises(vta.VtaError):
+        alma.alma_batch_into_host_f32(x, sweep, np.empty(out.size - 1, dtype=np.float32))

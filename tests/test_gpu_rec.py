@@ -1,9 +1,12 @@
 """EMA / RSI / ATR / ADX parity on the GPU (batch, many-series, fused).
 
 Bar: NaN warm-up prefixes and poison rules bit-exact; values within 1e-5
-relative of the f64 oracle on the same f32 inputs (RSI and ADX live in
-[0, 100], so their bound is 1e-5 * max(|ref|, 1)).  Long series exercise the
-chunked burn-in path; series with interior NaN/Inf exercise the exact path.
+relative of the f64 oracle on the same f32 inputs, for every indicator.  RSI
+and ADX live in [0, 100] and can be exactly 0 (a run of losses, no
+directional movement), where a relative error is undefined: there the
+denominator is floored at 1e-3, so a value below 1e-3 must lie within 1e-8 of
+the oracle.  Long series exercise the chunked burn-in path; series with
+interior NaN/Inf exercise the exact path.
 """
 import numpy as np
 import pytest
@@ -16,7 +19,7 @@ from tests.helpers import max_rel_err, nan_mask_equal
 
 pytestmark = pytest.mark.gpu
 
-FLOOR = {"ema": 0.0, "rsi": 1.0, "atr": 0.0, "adx": 1.0}
+FLOOR = {"ema": 0.0, "rsi": 1e-3, "atr": 0.0, "adx": 1e-3}
 
 
 @pytest.fixture(scope="module")
@@ -75,7 +78,7 @@ def test_adx_batch_close_periods(ctxs, bad):
     assert len(ps) == 128
     g = ctxs["adx"].adx_batch_dev(h, lo, c, period).to_host()
     f = [a.astype(np.float64) for a in (h, lo, c)]
-    _check(g, O.adx_batch(*f, ps), "adx", bound=2e-5)
+    _check(g, O.adx_batch(*f, ps), "adx")
 
 
 @pytest.mark.parametrize("name", ["ema", "rsi", "atr", "adx", "smma", "wilders"])
@@ -98,7 +101,7 @@ def test_scan_short_chunks_long_lead(ctxs, monkeypatch, name):
         else:
             g = getattr(op, f"{name}_batch_dev")(c, period).to_host()
             ref = getattr(O, f"{name}_batch")(c.astype(np.float64), ps)
-        _check(g, ref, name, bound=2e-5 if name == "adx" else 1e-5)
+        _check(g, ref, name)
         return
     op = vta.CudaSmma(0) if name == "smma" else vta.CudaWilders(0)
     ps = op.expand_grid(period)
@@ -126,7 +129,7 @@ def test_adx_batch(ctxs, n, period):
     ps = ctxs["adx"].expand_grid(period)
     g = ctxs["adx"].adx_batch_dev(h, lo, c, period).to_host()
     f = [a.astype(np.float64) for a in (h, lo, c)]
-    _check(g, O.adx_batch(*f, ps), "adx", bound=2e-5)
+    _check(g, O.adx_batch(*f, ps), "adx")
 
 
 def test_batch_unclean_series_exact_path(ctxs):
@@ -184,7 +187,7 @@ def test_many_series_atr_adx(ctxs, rows):
         g = fn(h, lo, c, cols, rows, p).to_host()
         ref, st = O.many_series_hlc(name, *f, p)
         assert (st == 0).all()
-        _check(g, ref, name, bound=1e-5 if name == "atr" else 2e-5)
+        _check(g, ref, name)
 
 
 def test_fused_ema_rsi_atr_matches_separate_oracles(ctxs):

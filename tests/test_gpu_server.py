@@ -1,15 +1,18 @@
 """Request server: results identical alone and in a concurrent mix, equal to the
 direct entry points, HBM-budget admission and the series cache.
 
-Bar: bit-identical outputs (the served requests run the same kernels on the
-same inputs; the parity of those kernels against the oracle is tested in the
-per-indicator test files)."""
+Bars: a result is bit-identical alone, in the mix and through the direct
+entry point; served results are also checked against the f64 oracle (NaN
+prefixes exact, values within 1e-5 relative; RSI/ADX denominators floored at
+1e-3 on their 0-100 scale)."""
 import numpy as np
 import pytest
 
+import oracle as O
 import vectorta_amd as vta
 from vectorta_amd import datagen
 from tests.conftest import require_gpu
+from tests.helpers import max_rel_err, nan_mask_equal
 
 pytestmark = pytest.mark.gpu
 
@@ -176,3 +179,109 @@ def test_small_requests_are_packed(pool):
     srv.close()
     print(f"packs {st['packs']}, packed requests {st['packed_requests']}")
     assert st["packs"] >= 1 and st["packed_requests"] >= 2 * st["packs"], st
+
+
+def _oracle_rows(spec_kind, x, hh, ll, combos):
+    """Oracle outputs (list per output) of the given combos of one request."""
+    xd = x.astype(np.float64)
+    if spec_kind == "alma":
+        return [O.alma_batch(xd, combos, nthreads=0)]
+    if spec_kind == "ema":
+        return [O.ema_batch(xd, combos)]
+    if spec_kind == "rsi":
+        return [O.rsi_batch(xd, combos)]
+    if spec_kind == "adx":
+        return [O.adx_batch(hh.astype(np.float64), ll.astype(np.float64), xd, combos)]
+    bands = [O.bollinger_bands(xd, p, u, d) for p, u, d in combos]
+    return [np.stack([b[k] for b in bands]) for k in range(3)]
+
+
+def _grid(spec):
+    if "alma" in spec:
+        return [(c.period, c.offset, c.sigma)
+                for c in vta.expand_grid(vta.AlmaBatchRange(**spec["alma"]))]
+    if "bb" in spec:
+        return vta.CudaBollingerBands.expand_grid(vta.BollingerBandsBatchRange(**spec["bb"]))
+    return vta.CudaEma.expand_grid(spec["period"])
+
+
+def _check_vs_oracle(spec, outs, pool, rows_to_check):
+    c, h, lo = pool
+    s, a, L = spec["series"], spec["start"], spec["len"]
+    x, hh, ll = c[s, a:a + L], h[s, a:a + L], lo[s, a:a + L]
+    grid = _grid(spec)
+    sel = sorted(set(r for r in rows_to_check(len(grid))))
+    ref = _oracle_rows(spec["kind"], x, hh, ll, [grid[r] for r in sel])
+    floor = 1e-3 if spec["kind"] in ("rsi", "adx") else 0.0
+    assert len(ref) == len(outs)
+    for g, r in zip(outs, ref):
+        assert g.shape == (len(grid), L)
+        nan_mask_equal(g[sel], r)
+        err = max_rel_err(g[sel], r, floor=floor)
+        assert err <= 1e-5, (spec["kind"], L, len(grid), err)
+    return len(sel)
+
+
+def test_served_results_match_oracle(pool):
+    """One served request of each kind, every combo against the oracle."""
+    c, h, lo = pool
+    srv = vta.IndicatorServer(0, n_slots=2, hbm_budget=4 << 30)
+    specs = [
+        dict(kind="alma", series=0, start=1_000, len=60_000, key=1,
+             alma=dict(period=(5, 250, 15), offset=(0.05, 0.95, 0.3), sigma=(2.0, 10.0, 4.0))),
+        dict(kind="ema", series=1, start=7, len=90_000, key=2, period=(2, 400, 21)),
+        dict(kind="rsi", series=2, start=30_000, len=80_000, key=3, period=(2, 300, 17)),
+        dict(kind="bollinger_bands", series=3, start=5, len=70_000, key=4,
+             bb=dict(period=(5, 200, 13), devup=(1.0, 3.0, 0.5), devdn=(2.0, 2.0, 0.0))),
+        dict(kind="adx", series=0, start=2_000, len=100_000, key=5, period=(2, 200, 11)),
+    ]
+    ids = [datagen.submit_spec(srv, sp, pool) for sp in specs]
+    for sp, rid in zip(specs, ids):
+        r = srv.wait(rid)
+        assert r.ok, (sp["kind"], r.status)
+        outs = [srv.to_host(r, k) for k in range(r.n_outputs)]
+        srv.release(rid)
+        n = _check_vs_oracle(sp, outs, pool, lambda C: range(C))
+        print(f"served {sp['kind']}: {n} combos x {sp['len']} bars vs oracle ok")
+    srv.close()
+
+
+@pytest.fixture(scope="module")
+def pool_1m():
+    require_gpu()
+    return datagen.ohlc_pool(n_series=4, length=1_000_000, seed=5)
+
+
+def test_mix_at_served_workload_shape(pool_1m):
+    """The served workload's real shape: 24 requests drawn from the served-mix
+    generator (series 1k-1M bars, grids 1-2,000 combos, 30% cached-series
+    reuse); this draw holds a 506,132-bar ALMA request and five requests of
+    1,072-1,920 combos over all five kinds (2.3 GB of outputs).  Every result
+    is bit-identical alone (one slot, one at a time) and in the concurrent mix
+    (four slots), and three combos of every request (first, middle, last)
+    match the oracle."""
+    specs = datagen.served_requests(24, seed=102, pool_series=4, pool_len=1_000_000)
+    grids = [len(_grid(sp)) for sp in specs]
+    assert max(sp["len"] for sp in specs) >= 500_000
+    assert max(grids) >= 1_000
+    assert {sp["kind"] for sp in specs} == set(datagen.SERVED_KINDS)
+    mix = vta.IndicatorServer(0, n_slots=4, hbm_budget=16 << 30)
+    m_out, _ = _run_all(mix, specs, pool_1m, concurrent=True)
+    st = mix.stats()
+    mix.close()
+    assert st["peak_in_flight"] >= 2, st
+    alone = vta.IndicatorServer(0, n_slots=1, hbm_budget=16 << 30)
+    checked = 0
+    for i, sp in enumerate(specs):
+        rid = datagen.submit_spec(alone, sp, pool_1m)
+        r = alone.wait(rid)
+        assert r.ok, (i, sp["kind"], r.status)
+        outs = [alone.to_host(r, k) for k in range(r.n_outputs)]
+        alone.release(rid)
+        assert len(outs) == len(m_out[i])
+        for a, m in zip(outs, m_out[i]):
+            assert np.array_equal(_bits(a), _bits(m)), (i, sp["kind"], sp["len"], grids[i])
+        checked += _check_vs_oracle(sp, outs, pool_1m, lambda C: (0, C // 2, C - 1))
+        m_out[i] = None
+    alone.close()
+    print(f"served mix: {len(specs)} requests alone == mix, {checked} combos vs oracle")

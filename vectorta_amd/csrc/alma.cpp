@@ -38,29 +38,12 @@ bool axis_usize(vta_range_usize r, std::vector<size_t> &v) {
     return !v.empty();
 }
 
-bool axis_f64(vta_range_f64 r, std::vector<double> &v) {
-    v.clear();
-    if (std::fabs(r.step) < 1e-12 || std::fabs(r.start - r.end) < 1e-12) {
-        v.push_back(r.start);
-        return true;
-    }
-    double st = std::fabs(r.step);
-    if (std::isinf(r.start) || std::isinf(r.end)) return false;   // would never end
-    if (std::fabs(r.end - r.start) / st >= (double)kMaxGridValues) return false;
-    if (r.start < r.end) {
-        for (double x = r.start; x <= r.end + 1e-12; x += st) v.push_back(x);
-        return !v.empty();
-    }
-    for (double x = r.start; x + 1e-12 >= r.end; x -= st) v.push_back(x);
-    return !v.empty();
-}
-
 vta_status expand(const vta_alma_batch_range *range, std::vector<vta_alma_params> &out) {
     if (!range) return fail(VTA_ERR_INVALID_INPUT, "range is null");
     std::vector<size_t> ps;
     std::vector<double> os, ss;
-    if (!axis_usize(range->period, ps) || !axis_f64(range->offset, os) ||
-        !axis_f64(range->sigma, ss))
+    if (!axis_usize(range->period, ps) || !expand_f64(range->offset, os) ||
+        !expand_f64(range->sigma, ss))
         return fail(VTA_ERR_INVALID_INPUT,
                     "Invalid range: period=(%zu, %zu, %zu) offset=(%g, %g, %g) "
                     "sigma=(%g, %g, %g) (empty, infinite or above %zu values)",
@@ -760,8 +743,10 @@ vta_status vta_alma_many_series_one_param_time_major_device(
         return fail(VTA_ERR_INVALID_INPUT, "invalid dimensions");
     hipFunction_t f;
     // largest tile height (multiple of 16) whose [tt + period - 1][64] tile fits the LDS
-    size_t limit = ctx->props.sharedMemPerBlock / sizeof(float);
-    long tt = (long)((limit - period) / 64) - (long)(period - 1);
+    // (signed: a period above the LDS float count leaves no tile and takes the
+    // direct kernel below)
+    const long limit = (long)(ctx->props.sharedMemPerBlock / sizeof(float));
+    long tt = (limit - (long)period) / 64 - ((long)period - 1);
     tt = (tt / 16) * 16;
     if (tt >= 16 && (size_t)((cols + 63) / 64) <= (size_t)ctx->props.maxGridSize[1]) {
         tt = std::min(tt, 256L);

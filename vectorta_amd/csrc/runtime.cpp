@@ -185,10 +185,19 @@ bool expand_f64(vta_range_f64 r, std::vector<double> &v) {
     // at once, so only the span over a finite step is bounded here
     if (std::isinf(r.start) || std::isinf(r.end)) return false;
     if (std::fabs(r.end - r.start) / st >= (double)kMaxGridValues) return false;
+    // The loops accumulate x += step as the reference does; a step below half
+    // the rounding unit of x leaves x unchanged (1e17 + 1 == 1e17), so each
+    // iteration checks that x moved and that the count stays in bounds.
     if (r.start < r.end) {
-        for (double x = r.start; x <= r.end + 1e-12; x += st) v.push_back(x);
+        for (double x = r.start; x <= r.end + 1e-12; x += st) {
+            if (v.size() >= kMaxGridValues || x + st == x) return false;
+            v.push_back(x);
+        }
     } else {
-        for (double x = r.start; x + 1e-12 >= r.end; x -= st) v.push_back(x);
+        for (double x = r.start; x + 1e-12 >= r.end; x -= st) {
+            if (v.size() >= kMaxGridValues || x - st == x) return false;
+            v.push_back(x);
+        }
     }
     return !v.empty();
 }
