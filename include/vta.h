@@ -209,6 +209,20 @@ vta_status vta_alma_batch_from_device_prices(vta_ctx *ctx, const float *d_prices
 vta_status vta_alma_batch_into_host_f32(vta_ctx *ctx, const float *prices, size_t len,
                                         const vta_alma_batch_range *range, float *out,
                                         size_t out_len);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the sweep of vta_alma_batch_from_device_prices into a
+ * caller-owned d_out of out_len = combos*len floats, row-major [combo][t],
+ * with the same kernels and results.  The per-combo tables go into a static
+ * 16 MiB device array per GPU instead of a per-call allocation; grids whose
+ * tables exceed it (about 4,000 MFMA combos) return InvalidInput.  The first
+ * call on a context loads the ALMA code object; later calls allocate nothing.
+ * Calls that overlap on one GPU race on the array; a call returns after its
+ * sweep has completed. */
+vta_status vta_alma_batch_from_device_prices_noalloc(vta_ctx *ctx, const float *d_prices,
+                                                     size_t len, size_t first_valid,
+                                                     const vta_alma_batch_range *range,
+                                                     float *d_out, size_t out_len);
 /* alma_batch_device (:565-593): enqueue-only FP32 sweep over caller-owned
  * buffers.  d_weights is [n_combos][max_period] f32, multiplied by
  * d_inv_norms[c] (the reference passes pre-normalised weights and 1.0). */

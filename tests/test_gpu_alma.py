@@ -260,3 +260,18 @@ def test_batch_into_host_f32(alma, policy):
     assert np.array_equal(d.view(np.uint32), g.view(np.uint32))
     with pytest.raises(vta.VtaError):
         alma.alma_batch_into_host_f32(x, sweep, np.empty(out.size - 1, dtype=np.float32))
+
+
+def test_many_series_period_above_lds_floats(alma):
+    """A period longer than the LDS holds floats (17,000 > 65,536 B / 4) takes
+    the direct many-series kernel instead of sizing an LDS tile from a wrapped
+    unsigned difference."""
+    rows = 20_000
+    x = datagen.gbm_close(rows, 0.01, seed=27)
+    x[:5] = np.nan
+    p = vta.AlmaParams(17_000, 0.85, 6.0)
+    g = alma.alma_multi_series_one_param_time_major_dev(x.reshape(-1, 1), 1, rows, p).to_host()
+    ref, st = O.many_series("alma", x.reshape(-1, 1).astype(np.float64), 17_000, 0.85, 6.0)
+    assert (st == 0).all()
+    nan_mask_equal(g, ref)
+    assert max_rel_err(g, ref) <= 1e-5

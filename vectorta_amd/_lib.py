@@ -157,6 +157,9 @@ SIGNATURES = {
                                                 ctypes.POINTER(DeviceArrayF32C)]),
     "vta_alma_batch_into_host_f32": (_ST, [_P, _P, _SZ, ctypes.POINTER(AlmaBatchRangeC), _P,
                                            _SZ]),
+    "vta_alma_batch_from_device_prices_noalloc": (_ST, [_P, _P, _SZ, _SZ,
+                                                        ctypes.POINTER(AlmaBatchRangeC), _P,
+                                                        _SZ]),
     "vta_alma_batch_device": (_ST, [_P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _P]),
     "vta_alma_batch_device_tm": (_ST, [_P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _P]),
     "vta_alma_sweep_plan_create": (_ST, [_P, ctypes.POINTER(AlmaBatchRangeC), _SZ, ctypes.c_int,

@@ -204,18 +204,9 @@ struct alma_mfg_args_host {
 static_assert(sizeof(alma_mfg_args_host) == 88, "alma_mfg_args layout");
 
 struct vta_alma_sweep_plan {
-    vta_alma_batch_kernel kernel = VTA_ALMA_BATCH_FP32;
     size_t series_len = 0;
-    size_t n_combos = 0;
-    size_t max_period = 0;
-    // FP32 form
-    DevBuf w, periods, inv;
-    // MFMA form
-    DevBuf frags, nblocks, groups, sorted_period, sorted_row, w_f32;
-    int n_nblocks = 0;
-    int n_groups = 0;
-    int n_groups_klo = 0;   // leading groups with K <= 128
-    int kmax = 0;
+    AlmaTables t;   // host copy of the tables and their layout
+    DevBuf dev;     // the tables on the device (one buffer, t.blob's layout)
 };
 
 namespace {
@@ -247,14 +238,28 @@ int mf_default_variant(size_t T, size_t C) {
     return 1;
 }
 
+// Appends one table to the blob at the next 256-byte boundary.
+template <typename V>
+void put_section(AlmaTables &t, int sec, const V &v) {
+    const size_t off = (t.blob.size() + 255) & ~(size_t)255;
+    const size_t bytes = v.size() * sizeof(typename V::value_type);
+    t.blob.resize(off + bytes);
+    if (bytes) memcpy(t.blob.data() + off, v.data(), bytes);
+    t.off[sec] = off;
+}
+
+template <typename P>
+const P *section(const AlmaTables &t, const unsigned char *base, int sec) {
+    return reinterpret_cast<const P *>(base + t.off[sec]);
+}
+
 // Builds the MFMA operand tables.  Combos are sorted by period (stable) and
 // cut into N-blocks of 32; block b contracts over K_b = round_up(max period of
 // the block, 16) taps with shorter periods zero padded at the front.  Each
 // weight is split hi = bf16(w), lo = bf16(w - hi) and stored in
 // v_mfma_f32_32x32x16_bf16 B-operand order: for K-step s and half h, lane l
 // holds taps 16s + 8(l/32) + j (j < 8) of combo l%32.
-vta_status build_mfma_tables(vta_alma_sweep_plan *plan,
-                             const std::vector<vta_alma_params> &combos, vta_ctx *ctx) {
+void build_mfma_tables(AlmaTables &t, const std::vector<vta_alma_params> &combos) {
     const size_t C = combos.size();
     std::vector<size_t> order(C);
     std::iota(order.begin(), order.end(), 0);
@@ -322,19 +327,17 @@ vta_status build_mfma_tables(vta_alma_sweep_plan *plan,
         grp.push_back(nbd[(b + cnt - 1) * 4 + 0]);
         grp.push_back(0);
     }
-    VTA_TRY(upload(ctx, plan->groups, grp.data(), grp.size() * sizeof(int32_t)));
-    plan->n_groups = (int)(grp.size() / 4);
-    plan->n_groups_klo = 0;
-    while (plan->n_groups_klo < plan->n_groups && grp[plan->n_groups_klo * 4 + 2] <= 8)
-        ++plan->n_groups_klo;
-    VTA_TRY(upload(ctx, plan->frags, frags.data(), frags.size() * sizeof(uint16_t)));
-    VTA_TRY(upload(ctx, plan->nblocks, nbd.data(), nbd.size() * sizeof(int32_t)));
-    VTA_TRY(upload(ctx, plan->sorted_period, sp.data(), sp.size() * sizeof(int32_t)));
-    VTA_TRY(upload(ctx, plan->sorted_row, sr.data(), sr.size() * sizeof(int32_t)));
-    VTA_TRY(upload(ctx, plan->w_f32, wf.data(), wf.size() * sizeof(float)));
-    plan->n_nblocks = nb;
-    plan->kmax = kmax;
-    return VTA_OK;
+    t.n_groups = (int)(grp.size() / 4);
+    t.n_groups_klo = 0;
+    while (t.n_groups_klo < t.n_groups && grp[t.n_groups_klo * 4 + 2] <= 8) ++t.n_groups_klo;
+    put_section(t, AlmaTables::GROUPS, grp);
+    put_section(t, AlmaTables::FRAGS, frags);
+    put_section(t, AlmaTables::NBLOCKS, nbd);
+    put_section(t, AlmaTables::SORTED_PERIOD, sp);
+    put_section(t, AlmaTables::SORTED_ROW, sr);
+    put_section(t, AlmaTables::W_F32, wf);
+    t.n_nblocks = nb;
+    t.kmax = kmax;
 }
 
 vta_alma_batch_kernel resolve_kernel(vta_alma_batch_kernel policy, size_t n_combos,
@@ -345,33 +348,37 @@ vta_alma_batch_kernel resolve_kernel(vta_alma_batch_kernel policy, size_t n_comb
     return VTA_ALMA_BATCH_FP32;
 }
 
+vta_status tables_build(const std::vector<vta_alma_params> &combos, size_t max_period,
+                        vta_alma_batch_kernel kernel, AlmaTables &t) {
+    t = AlmaTables();
+    t.kernel = kernel;
+    t.n_combos = combos.size();
+    t.max_period = max_period;
+    if (kernel == VTA_ALMA_BATCH_MFMA_BF16X3) {
+        if (max_period > kMfMaxPeriod)
+            return fail(VTA_ERR_INVALID_POLICY,
+                        "invalid policy: MFMA sweep supports periods <= %zu (got %zu)",
+                        kMfMaxPeriod, max_period);
+        build_mfma_tables(t, combos);
+        return VTA_OK;
+    }
+    if (kernel != VTA_ALMA_BATCH_FP32) return fail(VTA_ERR_INVALID_POLICY, "invalid policy");
+    std::vector<float> w, inv;
+    std::vector<int32_t> per;
+    weights_table_f32(combos, max_period, w, per, inv);
+    put_section(t, AlmaTables::W, w);
+    put_section(t, AlmaTables::PERIODS, per);
+    put_section(t, AlmaTables::INV, inv);
+    return VTA_OK;
+}
+
 vta_status plan_create(vta_ctx *ctx, const std::vector<vta_alma_params> &combos, size_t T,
                        size_t max_period, vta_alma_batch_kernel kernel,
                        vta_alma_sweep_plan **out) {
     auto *plan = new vta_alma_sweep_plan();
     plan->series_len = T;
-    plan->n_combos = combos.size();
-    plan->max_period = max_period;
-    plan->kernel = kernel;
-    vta_status st = VTA_OK;
-    if (kernel == VTA_ALMA_BATCH_MFMA_BF16X3) {
-        if (max_period > kMfMaxPeriod) {
-            delete plan;
-            return fail(VTA_ERR_INVALID_POLICY,
-                        "invalid policy: MFMA sweep supports periods <= %zu (got %zu)",
-                        kMfMaxPeriod, max_period);
-        }
-        st = build_mfma_tables(plan, combos, ctx);
-    } else if (kernel == VTA_ALMA_BATCH_FP32) {
-        std::vector<float> w, inv;
-        std::vector<int32_t> per;
-        weights_table_f32(combos, max_period, w, per, inv);
-        st = upload(ctx, plan->w, w.data(), w.size() * sizeof(float));
-        if (st == VTA_OK) st = upload(ctx, plan->periods, per.data(), per.size() * 4);
-        if (st == VTA_OK) st = upload(ctx, plan->inv, inv.data(), inv.size() * 4);
-    } else {
-        st = fail(VTA_ERR_INVALID_POLICY, "invalid policy");
-    }
+    vta_status st = tables_build(combos, max_period, kernel, plan->t);
+    if (st == VTA_OK) st = upload(ctx, plan->dev, plan->t.blob.data(), plan->t.blob.size());
     if (st != VTA_OK) {
         delete plan;
         return st;
@@ -380,18 +387,18 @@ vta_status plan_create(vta_ctx *ctx, const std::vector<vta_alma_params> &combos,
     return VTA_OK;
 }
 
-vta_status plan_launch(vta_ctx *ctx, const vta_alma_sweep_plan *plan, const float *d_prices,
-                       size_t fv, float *d_out, size_t series_len = 0) {
-    // the tables do not depend on the series length; a caller may launch a
-    // plan over any length that covers its longest period
-    const int T = (int)(series_len ? series_len : plan->series_len);
+// Enqueues the sweep over T bars with the tables at device address `base`
+// (laid out as t.blob).
+vta_status tables_launch(vta_ctx *ctx, const AlmaTables &t, const unsigned char *base,
+                         const float *d_prices, size_t fv, float *d_out, size_t series_len) {
+    const int T = (int)series_len;
     // kernel variant: the context's VTA_TUNE_ALMA_MFMA_VARIANT knob (1-4 shared
     // staging, 5-7 one wave per N-block), else the ALMA_MF_* environment
     // knobs, else the shared-staging form with 16 tiles
     int variant = (int)ctx->tune[VTA_TUNE_ALMA_MFMA_VARIANT];
-    if (variant == 0 && !mf_env_override()) variant = mf_default_variant(T, plan->n_combos);
+    if (variant == 0 && !mf_env_override()) variant = mf_default_variant(T, t.n_combos);
     const bool shared = variant == 0 ? mf_shared_staging() : variant <= 4;
-    if (plan->kernel == VTA_ALMA_BATCH_MFMA_BF16X3 && shared) {
+    if (t.kernel == VTA_ALMA_BATCH_MFMA_BF16X3 && shared) {
         // shared-staging form: 4 N-blocks per workgroup, R tiles per run
         // (ALMA_MF_TILES=8|16, default 16)
         int tiles = 16;
@@ -412,26 +419,26 @@ vta_status plan_launch(vta_ctx *ctx, const vta_alma_sweep_plan *plan, const floa
             tj2 = variant == 3;
             ks_req = variant == 4;
         }
-        const bool ksplit = ks_req && plan->n_groups_klo > 0 && plan->n_groups_klo < plan->n_groups;
+        const bool ksplit = ks_req && t.n_groups_klo > 0 && t.n_groups_klo < t.n_groups;
         alma_mfg_args_host a{};
         a.prices = d_prices;
-        a.frags = plan->frags.p;
-        a.nblocks = plan->nblocks.p;
-        a.groups = plan->groups.p;
-        a.sorted_period = plan->sorted_period.as<int32_t>();
-        a.sorted_row = plan->sorted_row.as<int32_t>();
-        a.w_f32 = plan->w_f32.as<float>();
+        a.frags = section<void>(t, base, AlmaTables::FRAGS);
+        a.nblocks = section<void>(t, base, AlmaTables::NBLOCKS);
+        a.groups = section<void>(t, base, AlmaTables::GROUPS);
+        a.sorted_period = section<int32_t>(t, base, AlmaTables::SORTED_PERIOD);
+        a.sorted_row = section<int32_t>(t, base, AlmaTables::SORTED_ROW);
+        a.w_f32 = section<float>(t, base, AlmaTables::W_F32);
         a.out = d_out;
         a.series_len = T;
         a.first_valid = (int)fv;
         a.n_tiles = (T + 31) / 32;
-        a.kmax = plan->kmax;
-        const int npair = plan->kmax / 2 + 15;
+        a.kmax = t.kmax;
+        const int npair = t.kmax / 2 + 15;
         const int stride = ((npair + 15) >> 5) * 32 + 16;
         // prices, anchors, flags, the staged dwords (rounded to 16 bytes) and
         // four per-wave 32 x 36-float output transpose tiles
         const size_t smem =
-            4 * ((size_t)(32 * tiles + plan->kmax + 32) + 2 * (size_t)tiles +
+            4 * ((size_t)(32 * tiles + t.kmax + 32) + 2 * (size_t)tiles +
                  (((size_t)tiles * 4 * (size_t)stride + 3) & ~(size_t)3) + 4 * 32 * 36);
         if (smem > ctx->props.sharedMemPerBlock)
             return fail(VTA_ERR_LAUNCH_CONFIG_TOO_LARGE, "MFMA staging needs %zu bytes of LDS",
@@ -443,14 +450,14 @@ vta_status plan_launch(vta_ctx *ctx, const vta_alma_sweep_plan *plan, const floa
         int np = 0;
         if (ksplit) {
             parts[np++] = {tj2 ? "alma_batch_mfma_bf16x3_wg16_t2" : "alma_batch_mfma_bf16x3_wg16",
-                           plan->n_groups_klo, plan->n_groups - plan->n_groups_klo};
+                           t.n_groups_klo, t.n_groups - t.n_groups_klo};
             parts[np++] = {tj2 ? "alma_batch_mfma_bf16x3_wg16_t2_klo" : "alma_batch_mfma_bf16x3_wg16_klo",
-                           0, plan->n_groups_klo};
+                           0, t.n_groups_klo};
         } else {
             parts[np++] = {tiles == 8 ? "alma_batch_mfma_bf16x3_wg8"
                                       : (tj2 ? "alma_batch_mfma_bf16x3_wg16_t2"
                                              : "alma_batch_mfma_bf16x3_wg16"),
-                           0, plan->n_groups};
+                           0, t.n_groups};
         }
         for (int i = 0; i < np; ++i) {
             hipFunction_t f;
@@ -462,7 +469,7 @@ vta_status plan_launch(vta_ctx *ctx, const vta_alma_sweep_plan *plan, const floa
         }
         return VTA_OK;
     }
-    if (plan->kernel == VTA_ALMA_BATCH_MFMA_BF16X3) {
+    if (t.kernel == VTA_ALMA_BATCH_MFMA_BF16X3) {
         // tiles per wave: 16 by default; ALMA_MF_TILES=8|32 overrides (tuning knob,
         // like the reference's ALMA_TILE, alma_wrapper.rs:314)
         int tiles = kMfTilesPerWave;
@@ -479,28 +486,36 @@ vta_status plan_launch(vta_ctx *ctx, const vta_alma_sweep_plan *plan, const floa
         VTA_TRY(get_function(ctx, "alma", name, &f));
         alma_mf_args_host a;
         a.prices = d_prices;
-        a.frags = plan->frags.p;
-        a.nblocks = plan->nblocks.p;
-        a.sorted_period = plan->sorted_period.as<int32_t>();
-        a.sorted_row = plan->sorted_row.as<int32_t>();
-        a.w_f32 = plan->w_f32.as<float>();
+        a.frags = section<void>(t, base, AlmaTables::FRAGS);
+        a.nblocks = section<void>(t, base, AlmaTables::NBLOCKS);
+        a.sorted_period = section<int32_t>(t, base, AlmaTables::SORTED_PERIOD);
+        a.sorted_row = section<int32_t>(t, base, AlmaTables::SORTED_ROW);
+        a.w_f32 = section<float>(t, base, AlmaTables::W_F32);
         a.out = d_out;
         a.series_len = T;
         a.first_valid = (int)fv;
-        a.n_nblocks = plan->n_nblocks;
+        a.n_nblocks = t.n_nblocks;
         a.tiles_per_wave = tiles;
         a.n_tiles = (T + 31) / 32;
-        a.kmax = plan->kmax;
-        dim3 grid((a.n_tiles + tiles - 1) / tiles, plan->n_nblocks, 1);
+        a.kmax = t.kmax;
+        dim3 grid((a.n_tiles + tiles - 1) / tiles, t.n_nblocks, 1);
         ctx->alma_selected = VTA_ALMA_BATCH_MFMA_BF16X3;
         return launch(ctx, f, grid, dim3(64), 0, a);
     }
     ctx->alma_selected = VTA_ALMA_BATCH_FP32;
-    return launch_fp32(ctx, d_prices, plan->w.as<float>(), plan->periods.as<int32_t>(),
-                       plan->inv.as<float>(), (int)plan->max_period, T, (int)plan->n_combos,
-                       (int)fv, d_out, false);
+    return launch_fp32(ctx, d_prices, section<float>(t, base, AlmaTables::W),
+                       section<int32_t>(t, base, AlmaTables::PERIODS),
+                       section<float>(t, base, AlmaTables::INV), (int)t.max_period, T,
+                       (int)t.n_combos, (int)fv, d_out, false);
 }
 
+vta_status plan_launch(vta_ctx *ctx, const vta_alma_sweep_plan *plan, const float *d_prices,
+                       size_t fv, float *d_out, size_t series_len = 0) {
+    // the tables do not depend on the series length; a caller may launch a
+    // plan over any length that covers its longest period
+    return tables_launch(ctx, plan->t, plan->dev.as<unsigned char>(), d_prices, fv, d_out,
+                         series_len ? series_len : plan->series_len);
+}
 // run_batch_with_prices_device (alma_wrapper.rs:683-781).
 vta_status run_batch(vta_ctx *ctx, const float *d_prices, size_t T, size_t fv,
                      const std::vector<vta_alma_params> &combos, size_t max_period,
@@ -567,13 +582,28 @@ vta_status alma_grid_plan_launch(vta_ctx *ctx, const vta_alma_sweep_plan *plan,
                                  const float *d_prices, size_t series_len, size_t fv,
                                  float *d_out) {
     if (series_len == 0 || series_len > (size_t)INT32_MAX || fv >= series_len ||
-        series_len - fv < plan->max_period)
+        series_len - fv < plan->t.max_period)
         return fail(VTA_ERR_INVALID_INPUT, "not enough valid data (needed >= %zu)",
-                    plan->max_period);
+                    plan->t.max_period);
     return plan_launch(ctx, plan, d_prices, fv, d_out, series_len);
 }
 
 void alma_grid_plan_destroy(vta_alma_sweep_plan *plan) { delete plan; }
+
+vta_status alma_tables_for_range(vta_ctx *ctx, const vta_alma_batch_range *range, size_t len,
+                                 size_t fv, AlmaTables &t) {
+    std::vector<vta_alma_params> combos;
+    size_t mp;
+    VTA_TRY(expand(range, combos));
+    VTA_TRY(validate_combos(combos, len, fv, &mp));
+    return tables_build(combos, mp, resolve_kernel(ctx->alma_policy, combos.size(), mp), t);
+}
+
+vta_status alma_tables_launch(vta_ctx *ctx, const AlmaTables &t, const void *d_tables,
+                              const float *d_prices, size_t len, size_t fv, float *d_out) {
+    return tables_launch(ctx, t, static_cast<const unsigned char *>(d_tables), d_prices, fv,
+                         d_out, len);
+}
 
 }  // namespace vta
 
@@ -718,16 +748,16 @@ vta_status vta_alma_sweep_plan_destroy(vta_ctx *ctx, vta_alma_sweep_plan *plan) 
 }
 
 size_t vta_alma_sweep_plan_combos(const vta_alma_sweep_plan *plan) {
-    return plan ? plan->n_combos : 0;
+    return plan ? plan->t.n_combos : 0;
 }
 
 vta_status vta_alma_sweep_launch(vta_ctx *ctx, const vta_alma_sweep_plan *plan,
                                  const float *d_prices, size_t first_valid, float *d_out) {
     clear_error();
     if (!ctx || !plan || !d_prices || !d_out) return fail(VTA_ERR_INVALID_INPUT, "null argument");
-    if (first_valid >= plan->series_len || plan->series_len - first_valid < plan->max_period)
+    if (first_valid >= plan->series_len || plan->series_len - first_valid < plan->t.max_period)
         return fail(VTA_ERR_INVALID_INPUT, "not enough valid data (needed >= %zu, valid = %zu)",
-                    plan->max_period,
+                    plan->t.max_period,
                     first_valid < plan->series_len ? plan->series_len - first_valid : (size_t)0);
     return plan_launch(ctx, plan, d_prices, first_valid, d_out);
 }

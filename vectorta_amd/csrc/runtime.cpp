@@ -294,6 +294,7 @@ vta_status vta_ctx_create(int device_id, vta_ctx **out) {
     }
     const char *kd = getenv("VTA_KERNEL_DIR");
     ctx->kernel_dir = (kd && *kd) ? std::string(kd) : library_dir();
+    noalloc_setup(device_id);   // device address of the allocation-free scratch
     *out = ctx;
     return VTA_OK;
 }

@@ -130,6 +130,41 @@ bool expand_f64(vta_range_f64 r, std::vector<double> &v);
 vta_status host_first_valids(const float *tm, size_t cols, size_t rows, size_t need,
                              std::vector<int32_t> &fvs);
 
+// Device tables of one ALMA sweep (alma.cpp), built on the host and laid out
+// in one buffer: each table starts at a 256-byte boundary at off[section] of
+// blob.  A sweep plan uploads the blob to a buffer of its own; the
+// allocation-free entry point copies it into a static device array.
+struct AlmaTables {
+    enum Section {
+        W, PERIODS, INV,                                                  // FP32 form
+        FRAGS, NBLOCKS, GROUPS, SORTED_PERIOD, SORTED_ROW, W_F32,         // MFMA form
+        COUNT
+    };
+    vta_alma_batch_kernel kernel = VTA_ALMA_BATCH_FP32;
+    size_t n_combos = 0;
+    size_t max_period = 0;
+    int n_nblocks = 0;
+    int n_groups = 0;
+    int n_groups_klo = 0;   // leading groups with K <= 128
+    int kmax = 0;
+    size_t off[COUNT] = {};
+    std::vector<unsigned char> blob;
+};
+// Expands and validates `range` against a series of len bars whose first
+// valid bar is fv (as alma_batch_from_device_prices does) and builds its
+// tables for the context's batch policy.
+vta_status alma_tables_for_range(vta_ctx *ctx, const vta_alma_batch_range *range, size_t len,
+                                 size_t fv, AlmaTables &t);
+// Enqueues the sweep on the context's stream with the tables at device
+// address d_tables (t.blob's layout); d_out holds n_combos x len floats.
+vta_status alma_tables_launch(vta_ctx *ctx, const AlmaTables &t, const void *d_tables,
+                              const float *d_prices, size_t len, size_t fv, float *d_out);
+
+// Looks up, once per device, the device address of the static scratch array
+// of the allocation-free entry point (noalloc.hip); `device` must be current.
+// A failure is left for that entry point to report.
+void noalloc_setup(int device);
+
 // ALMA grid expansion (alma.cpp), shared with the crossover module.
 vta_status alma_expand_range(const vta_alma_batch_range *range,
                              std::vector<vta_alma_params> &out);
