@@ -324,7 +324,7 @@ def bench_captured_batch(steps, warmup, ev):
     def batch():
         h = ctx.handle
         for (dc, dh, dl), o in zip(series, outs):
-            check(L.vta_ema_batch_device(h, P(dc.ptr), P(d_p.ptr), None, mp, T, fv, C, P(o[0].ptr)))
+            check(L.vta_ema_batch_device(h, P(dc.ptr), P(d_p.ptr), None, T, fv, C, P(o[0].ptr)))
             check(L.vta_rsi_batch_device(h, P(dc.ptr), P(d_p.ptr), mp, T, fv, C, P(o[1].ptr)))
             check(L.vta_adx_batch_device(h, P(dh.ptr), P(dl.ptr), P(dc.ptr), P(d_p.ptr), mp, T,
                                          fv, C, P(o[2].ptr)))
@@ -405,8 +405,8 @@ def bench_packed_batch(steps, warmup, ev):
             for j, d in zip(jobs, bufs):
                 if kind == "ema":
                     check(L.vta_ema_batch_device(ctx.handle, P(d["c"].ptr), P(d["p"].ptr), None,
-                                                 j["max_period"], j["series_len"], 0,
-                                                 j["n_combos"], P(d["out"].ptr)))
+                                                 j["series_len"], 0, j["n_combos"],
+                                                 P(d["out"].ptr)))
                 else:
                     check(L.vta_adx_batch_device(ctx.handle, P(d["h"].ptr), P(d["l"].ptr),
                                                  P(d["c"].ptr), P(d["p"].ptr), j["max_period"],
@@ -474,7 +474,7 @@ def bench_tune_shapes(reps):
             P = ctypes.c_void_p
             if name == "ema":
                 fn = lambda: check(lib().vta_ema_batch_device(
-                    cx.handle, P(d_c.ptr), P(d_p.ptr), None, int(per.max()), T, 0, C, P(d_o.ptr)))
+                    cx.handle, P(d_c.ptr), P(d_p.ptr), None, T, 0, C, P(d_o.ptr)))
             else:
                 fn = lambda: check(lib().vta_adx_batch_device(
                     cx.handle, P(d_h.ptr), P(d_l.ptr), P(d_c.ptr), P(d_p.ptr), int(per.max()), T,
@@ -510,10 +510,15 @@ def bench_moving_averages(steps, warmup, ev):
     d_o = ctx.alloc(C * T * 4)
     kern = {"sma": "fir_batch_f32", "wma": "fir_batch_f32", "swma": "fir_batch_f32",
             "smma": "smma_batch_f32", "wilders": "wilders_batch_f32"}
+    d_w = ctx.to_device((periods - 1).astype(np.int32))     # warm = first_valid + p - 1
     for name, op in ops.items():
-        if name in ("smma", "wilders"):
-            fn = (lambda op=op, name=name: getattr(op, f"{name}_batch_device")(
-                d_x, d_p, int(periods.max()), T, 0, C, d_o))
+        if name == "smma":
+            fn = (lambda op=op: op.smma_batch_device(d_x, d_p, d_w, 0, T, C, d_o))
+        elif name == "wilders":
+            fn = (lambda op=op: op.wilders_batch_device(d_x, d_p, None, d_w, T, 0, C, d_o))
+        elif name == "swma":
+            fn = (lambda op=op: op.swma_batch_device(d_x, d_p, d_w, T, C, int(periods.max()),
+                                                     d_o))
         else:
             fn = (lambda op=op, name=name: getattr(op, f"{name}_batch_device")(
                 d_x, d_p, T, C, 0, int(periods.max()), d_o))
@@ -531,30 +536,29 @@ def bench_moving_averages(steps, warmup, ev):
         if name in ("sma", "wma", "swma"):
             res[f"{name}_batch"]["traffic"] = profiled_traffic(
                 "moving_averages_traffic.txt", f"fir_batch_f32[{name}]")
+    op.synchronize()       # reports a warm-index mismatch, if any
     d_o.free()
     d_x.free()
     d_p.free()
+    d_w.free()
     rows, cols = 50_000, 4096
     rng = np.random.Generator(np.random.PCG64(9))
     tm = (100.0 * np.exp(np.cumsum(rng.normal(0, 0.01, (rows, cols)), axis=0))).astype(np.float32)
     d_tm = ctx.to_device(tm)
     fvs = np.zeros(cols, dtype=np.int32)
     d_fv = ctx.to_device(fvs)
-    d_clean = ctx.to_device(np.ones(cols, dtype=np.int32))   # every bar finite
+    # per-series metadata prepared once (every bar finite): clean flags for
+    # SMMA / Wilders, first non-finite bar (= rows) for SMA / WMA / SWMA
+    d_clean = ctx.to_device(np.ones(cols, dtype=np.int32))
+    d_z = ctx.to_device(np.full(cols, rows, dtype=np.int32))
     d_o = ctx.alloc(rows * cols * 4)
     del tm
-    ms_meth = {"sma": "sma_multi_series_one_param_device",
-               "wma": "wma_multi_series_one_param_time_major_device",
-               "swma": "swma_multi_series_one_param_device",
-               "smma": "smma_multi_series_one_param_device",
-               "wilders": "wilders_many_series_one_param_device"}
+    from vectorta_amd.rec import ma_many_series_prepared_device
     for name, p in (("sma", 14), ("wma", 30), ("swma", 14), ("smma", 14), ("wilders", 14)):
         op = ops[name]
-        meth = getattr(op, ms_meth[name])
-        if name in ("smma", "wilders"):
-            fn = (lambda meth=meth, p=p: meth(d_tm, d_fv, p, cols, rows, d_o, d_clean=d_clean))
-        else:
-            fn = (lambda meth=meth, p=p: meth(d_tm, d_fv, cols, rows, p, d_o))
+        meta = d_clean if name in ("smma", "wilders") else d_z
+        fn = (lambda op=op, name=name, p=p, meta=meta: ma_many_series_prepared_device(
+            op.ctx, name, d_tm, d_fv, meta, p, cols, rows, d_o))
         mean, ks, _ = _time_launches(fn, op.ctx.stream, steps, warmup, ev, sync=op.synchronize)
         op.synchronize()
         t = mean * 1e-3
@@ -569,6 +573,7 @@ def bench_moving_averages(steps, warmup, ev):
     d_tm.free()
     d_fv.free()
     d_clean.free()
+    d_z.free()
     return res
 
 
@@ -593,7 +598,7 @@ def bench_recurrence_batches(steps, warmup, ev):
     P = ctypes.c_void_p
     fns = {
         "ema": lambda: L.vta_ema_batch_device(ops["ema"].ctx.handle, P(d_c.ptr), P(d_p.ptr), None,
-                                              int(periods.max()), T, 0, C, P(d_o.ptr)),
+                                              T, 0, C, P(d_o.ptr)),
         "rsi": lambda: L.vta_rsi_batch_device(ops["rsi"].ctx.handle, P(d_c.ptr), P(d_p.ptr),
                                               int(periods.max()), T, 0, C, P(d_o.ptr)),
         "atr": lambda: L.vta_atr_batch_device(ops["atr"].ctx.handle, P(d_h.ptr), P(d_l.ptr),

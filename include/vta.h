@@ -262,6 +262,55 @@ vta_status vta_alma_many_series_one_param_time_major_device(
     float inv_norm, size_t cols, size_t rows, const int32_t *d_first_valids, float *d_out_tm);
 
 
+/* ====================================================== kernel policies */
+/* CudaEmaPolicy (ema_wrapper.rs:49-90), CudaRsiPolicy (rsi_wrapper.rs:92),
+ * CudaAdxPolicy (adx_wrapper.rs:49-66, set_policy :116) and CudaPnrPolicy
+ * (percentile_nearest_rank_wrapper.rs:51-90, set_policy/policy/
+ * selected_*_kernel :143-152): per call family (one-series batch,
+ * many-series) Auto or a fixed form, BatchKernelPolicy::Plain { block_x } /
+ * ManySeriesKernelPolicy::OneD { block_x }.  Here the fixed form is the plain
+ * kernel: one lane per combo (or series) over the whole series from
+ * first_valid, no time chunking.  These kernels run 64-lane waves, so
+ * block_x must be 0 (default) or 64; other values are InvalidPolicy.  The
+ * selected kernel of the last call is PLAIN or CHUNKED (time chunks: the
+ * chunked scan or burn-in starts) with block_x 64, NONE before the first
+ * call. */
+typedef enum vta_kernel_policy_kind {
+    VTA_KERNEL_AUTO = 0,
+    VTA_KERNEL_PLAIN = 1
+} vta_kernel_policy_kind;
+typedef struct vta_kernel_policy {
+    int32_t batch;               /* vta_kernel_policy_kind */
+    uint32_t batch_block_x;
+    int32_t many_series;         /* vta_kernel_policy_kind */
+    uint32_t many_series_block_x;
+} vta_kernel_policy;
+typedef enum vta_kernel_selected_kind {
+    VTA_SELECTED_NONE = 0,
+    VTA_SELECTED_PLAIN = 1,
+    VTA_SELECTED_CHUNKED = 2
+} vta_kernel_selected_kind;
+typedef struct vta_kernel_selected {
+    int32_t kind;                /* vta_kernel_selected_kind */
+    uint32_t block_x;
+} vta_kernel_selected;
+vta_status vta_ema_set_policy(vta_ctx *ctx, const vta_kernel_policy *policy);
+vta_status vta_ema_policy(const vta_ctx *ctx, vta_kernel_policy *out);
+vta_status vta_ema_selected_batch_kernel(const vta_ctx *ctx, vta_kernel_selected *out);
+vta_status vta_ema_selected_many_series_kernel(const vta_ctx *ctx, vta_kernel_selected *out);
+vta_status vta_rsi_set_policy(vta_ctx *ctx, const vta_kernel_policy *policy);
+vta_status vta_rsi_policy(const vta_ctx *ctx, vta_kernel_policy *out);
+vta_status vta_rsi_selected_batch_kernel(const vta_ctx *ctx, vta_kernel_selected *out);
+vta_status vta_rsi_selected_many_series_kernel(const vta_ctx *ctx, vta_kernel_selected *out);
+vta_status vta_adx_set_policy(vta_ctx *ctx, const vta_kernel_policy *policy);
+vta_status vta_adx_policy(const vta_ctx *ctx, vta_kernel_policy *out);
+vta_status vta_adx_selected_batch_kernel(const vta_ctx *ctx, vta_kernel_selected *out);
+vta_status vta_adx_selected_many_series_kernel(const vta_ctx *ctx, vta_kernel_selected *out);
+vta_status vta_pnr_set_policy(vta_ctx *ctx, const vta_kernel_policy *policy);
+vta_status vta_pnr_policy(const vta_ctx *ctx, vta_kernel_policy *out);
+vta_status vta_pnr_selected_batch_kernel(const vta_ctx *ctx, vta_kernel_selected *out);
+vta_status vta_pnr_selected_many_series_kernel(const vta_ctx *ctx, vta_kernel_selected *out);
+
 /* ====================================================== recurrences */
 /* One-parameter batch ranges are vta_range_usize over the period, expanded
  * like the reference wrappers: EMA/RSI ascending (ema_wrapper.rs:862-873,
@@ -269,25 +318,35 @@ vta_status vta_alma_many_series_one_param_time_major_device(
  * (adx_wrapper.rs:158-178).  Engines run in f64 with the reference's
  * operation order and round to f32 on output. */
 
-/* EMA: ema_wrapper.rs:193 (batch_dev), :256 (batch_device), :325
- * (many_series_one_param_time_major_dev), :381 (many_series_one_param_device).
- * d_alphas is accepted for signature parity and ignored (alpha = 2/(p+1) in f64).
- * d_clean (per series, 1 = every bar from first_valid finite) comes from
- * vta_many_series_prepare; NULL runs every series exactly in one pass. */
+/* EMA: ema_wrapper.rs:193 (batch_dev), :256 (batch_device), :309
+ * (batch_into_host_f32), :325 (many_series_one_param_time_major_dev), :381
+ * (many_series_one_param_device), :422
+ * (many_series_one_param_time_major_into_host_f32).
+ * The device forms use the caller's smoothing factors (d_alphas per combo,
+ * alpha), as the reference kernels do (its host code passes 2/(p+1) in f32);
+ * d_alphas NULL derives 2/(p+1) in f64 like the _dev forms.  The many-series
+ * device form finds each series' non-finite bars from its first_valid on the
+ * device.  *_into_host_f32: out_len must be combos*len (rows*cols). */
 vta_status vta_ema_expand_grid(const vta_range_usize *range, size_t *periods, size_t cap,
                                size_t *n_out);
 vta_status vta_ema_batch_dev(vta_ctx *ctx, const float *prices, size_t len,
                              const vta_range_usize *range, vta_device_array_f32 *out);
 vta_status vta_ema_batch_device(vta_ctx *ctx, const float *d_prices, const int32_t *d_periods,
-                                const float *d_alphas, int32_t max_period, int32_t series_len,
-                                int32_t first_valid, int32_t n_combos, float *d_out);
+                                const float *d_alphas, size_t series_len, size_t first_valid,
+                                size_t n_combos, float *d_out);
+vta_status vta_ema_batch_into_host_f32(vta_ctx *ctx, const float *prices, size_t len,
+                                       const vta_range_usize *range, float *out,
+                                       size_t out_len);
 vta_status vta_ema_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *data_tm,
                                                         size_t cols, size_t rows, size_t period,
                                                         vta_device_array_f32 *out);
 vta_status vta_ema_many_series_one_param_device(vta_ctx *ctx, const float *d_prices_tm,
-                                                const int32_t *d_first_valids, size_t period,
-                                                size_t num_series, size_t series_len,
-                                                const int32_t *d_clean, float *d_out_tm);
+                                                const int32_t *d_first_valids, int32_t period,
+                                                float alpha, size_t num_series,
+                                                size_t series_len, float *d_out_tm);
+vta_status vta_ema_many_series_one_param_time_major_into_host_f32(
+    vta_ctx *ctx, const float *data_tm, size_t cols, size_t rows, size_t period, float *out_tm,
+    size_t out_len);
 
 /* RSI (Wilder): rsi_wrapper.rs:152 (rsi_batch_dev), :271
  * (rsi_many_series_one_param_time_major_dev). */
@@ -416,50 +475,93 @@ vta_status vta_adx_many_series_one_param_time_major_dev(vta_ctx *ctx, const floa
                                                         const float *close_tm, size_t cols,
                                                         size_t rows, size_t period,
                                                         vta_device_array_f32 *out);
+/* adx_batch_into_host_f32 (:253-280) and
+ * adx_many_series_one_param_time_major_into_host_f32 (:417-440): out_len must
+ * be combos*len (rows*cols). */
+vta_status vta_adx_batch_into_host_f32(vta_ctx *ctx, const float *high, const float *low,
+                                       const float *close, size_t len,
+                                       const vta_range_usize *range, float *out,
+                                       size_t out_len);
+vta_status vta_adx_many_series_one_param_time_major_into_host_f32(
+    vta_ctx *ctx, const float *high_tm, const float *low_tm, const float *close_tm, size_t cols,
+    size_t rows, size_t period, float *out_tm, size_t out_len);
 
-/* SMMA: smma_wrapper.rs:398 (smma_batch_dev), :377 (smma_batch_device; the
- * per-combo warm indices are fv + p - 1 and derived here), :623
+/* SMMA: smma_wrapper.rs:398 (smma_batch_dev), :377 (smma_batch_device), :623
  * (smma_multi_series_one_param_time_major_dev), :599
  * (smma_multi_series_one_param_device).  Grid: smma.rs:430-471.  Seed = mean
- * of the first p bars at fv + p - 1, then prev = (prev (p-1) + x) / p. */
+ * of the first p bars at fv + p - 1, then prev = (prev (p-1) + x) / p.
+ *
+ * Warm indices (SMMA, Wilders, SWMA device forms): the reference's host code
+ * fills d_warms[c] = first_valid + period[c] - 1, and the kernels here derive
+ * the warm-up from the periods, so the buffers are checked on the device: a
+ * mismatch makes the next vta_ctx_synchronize return InvalidInput (the
+ * enqueue-only call itself cannot wait for the check), and that call's
+ * outputs are undefined; a NULL warm buffer is not checked.  The many-series
+ * device forms find each series' non-finite bars from its first_valid on the
+ * device. */
 vta_status vta_smma_expand_grid(const vta_range_usize *range, size_t *periods, size_t cap,
                                 size_t *n_out);
 vta_status vta_smma_batch_dev(vta_ctx *ctx, const float *prices, size_t len,
                               const vta_range_usize *range, vta_device_array_f32 *out);
 vta_status vta_smma_batch_device(vta_ctx *ctx, const float *d_prices, const int32_t *d_periods,
-                                 int32_t max_period, int32_t series_len, int32_t first_valid,
-                                 int32_t n_combos, float *d_out);
+                                 const int32_t *d_warms, size_t first_valid, size_t series_len,
+                                 size_t n_combos, float *d_out);
 vta_status vta_smma_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *data_tm,
                                                          size_t cols, size_t rows, size_t period,
                                                          vta_device_array_f32 *out);
-vta_status vta_smma_many_series_one_param_device(vta_ctx *ctx, const float *d_prices_tm,
-                                                 const int32_t *d_first_valids, size_t period,
-                                                 size_t num_series, size_t series_len,
-                                                 const int32_t *d_clean, float *d_out_tm);
+vta_status vta_smma_multi_series_one_param_device(vta_ctx *ctx, const float *d_prices_tm,
+                                                  int32_t period, int32_t num_series,
+                                                  int32_t series_len,
+                                                  const int32_t *d_first_valids,
+                                                  float *d_out_tm);
 
 /* Wilders: wilders_wrapper.rs:134 (wilders_batch_dev; the first max_period
  * bars after first_valid must be finite, :391-400), :252
- * (wilders_batch_device; alphas and warm indices derived from the periods),
+ * (wilders_batch_device: d_alphas used, NULL derives 1/p in f64; d_warm
+ * checked as above),
  * :600 (wilders_many_series_one_param_time_major_dev; first_valid = first
- * finite bar), :662 (wilders_many_series_one_param_device).  Grid:
- * wilders_wrapper.rs:909-940. */
+ * finite bar), :662 (wilders_many_series_one_param_device: alpha used).
+ * Grid: wilders_wrapper.rs:909-940. */
 vta_status vta_wilders_expand_grid(const vta_range_usize *range, size_t *periods, size_t cap,
                                    size_t *n_out);
 vta_status vta_wilders_batch_dev(vta_ctx *ctx, const float *prices, size_t len,
                                  const vta_range_usize *range, vta_device_array_f32 *out);
 vta_status vta_wilders_batch_device(vta_ctx *ctx, const float *d_prices,
-                                    const int32_t *d_periods, int32_t max_period,
-                                    int32_t series_len, int32_t first_valid, int32_t n_combos,
-                                    float *d_out);
+                                    const int32_t *d_periods, const float *d_alphas,
+                                    const int32_t *d_warm, int32_t series_len,
+                                    int32_t first_valid, int32_t n_combos, float *d_out);
 vta_status vta_wilders_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *data_tm,
                                                             size_t cols, size_t rows,
                                                             size_t period,
                                                             vta_device_array_f32 *out);
 vta_status vta_wilders_many_series_one_param_device(vta_ctx *ctx, const float *d_prices_tm,
                                                     const int32_t *d_first_valids,
-                                                    size_t period, size_t num_series,
-                                                    size_t series_len, const int32_t *d_clean,
+                                                    int32_t period, float alpha,
+                                                    size_t num_series, size_t series_len,
                                                     float *d_out_tm);
+
+/* No reference counterpart.  The one-input many-series device forms (EMA,
+ * SMMA, Wilders, SMA, WMA, SWMA) over per-series metadata prepared once with
+ * vta_many_series_prepare on the same device data -- d_meta is the clean flag
+ * (mode 0) for EMA / SMMA / Wilders and the first non-finite bar (mode 4) for
+ * SMA / WMA / SWMA -- for callers that launch repeatedly over that data; the
+ * reference-signature device forms above find the metadata on every call
+ * (one extra read of the matrix).  alpha <= 0 derives the smoothing factor
+ * from the period (EMA 2/(p+1), Wilders 1/p, in f64); other kinds ignore it. */
+typedef enum vta_ma_kind {
+    VTA_MA_EMA = 0,
+    VTA_MA_SMMA = 1,
+    VTA_MA_WILDERS = 2,
+    VTA_MA_SMA = 3,
+    VTA_MA_WMA = 4,
+    VTA_MA_SWMA = 5
+} vta_ma_kind;
+vta_status vta_ma_many_series_prepared_device(vta_ctx *ctx, vta_ma_kind kind,
+                                              const float *d_prices_tm,
+                                              const int32_t *d_first_valids,
+                                              const int32_t *d_meta, size_t period, float alpha,
+                                              size_t num_series, size_t series_len,
+                                              float *d_out_tm);
 
 /* Per-series metadata of device-resident time-major inputs, enqueue only:
  * first_valid (mode 0: x; 1: first bar with high, low, x all non-NaN; 2: ADX
@@ -500,11 +602,16 @@ vta_status vta_ema_rsi_atr_many_series_dev(vta_ctx *ctx, const float *close_tm,
  * (swma_wrapper.rs:678 swma_batch_dev, :663 swma_batch_device, :825
  * swma_multi_series_one_param_time_major_dev, :787 swma_multi_series_one_param_device).
  * Grids: sma.rs:705-755, wma_wrapper.rs:324-350, swma_wrapper.rs:275-301.
- * Periods: SMA/SWMA >= 1, WMA >= 2.  _batch_device is enqueue only; the
- * first non-finite bar is found on the device (the reference's warm-index and
- * weight arguments are derived from the periods).  _many_series_..._device
- * takes d_first_nonfinite per series (vta_many_series_prepare mode 4), or NULL
- * when every bar from first_valid is finite. */
+ * Periods: SMA/SWMA >= 1, WMA >= 2.  The device forms find the first
+ * non-finite bar (of each series) on the device.  swma_batch_device derives
+ * first_valid from the warm indices on the device and checks them (see the
+ * warm-index note above); swma_multi_series_one_param_device derives the
+ * triangular weights from the period (d_weights is not read: the reference's
+ * host code uploads those same normalised weights).  vta_sma_batch_device and
+ * vta_sma_many_series_one_param_device are enqueue-only SMA forms with no
+ * reference counterpart (the reference's SMA device-input forms allocate
+ * their output: vta_sma_batch_dev_from_device, where periods[] is the host
+ * array of SmaParams periods, and ..._dev_from_device). */
 vta_status vta_sma_expand_grid(const vta_range_usize *range, size_t *periods, size_t cap,
                               size_t *n_out);
 vta_status vta_sma_batch_dev(vta_ctx *ctx, const float *prices, size_t len,
@@ -512,13 +619,18 @@ vta_status vta_sma_batch_dev(vta_ctx *ctx, const float *prices, size_t len,
 vta_status vta_sma_batch_device(vta_ctx *ctx, const float *d_prices, const int32_t *d_periods,
                                int32_t series_len, int32_t n_combos, int32_t first_valid,
                                int32_t max_period, float *d_out);
+vta_status vta_sma_batch_dev_from_device(vta_ctx *ctx, const float *d_prices, size_t len,
+                                         const size_t *periods, size_t n_combos,
+                                         size_t first_valid, vta_device_array_f32 *out);
 vta_status vta_sma_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *data_tm,
                                                        size_t cols, size_t rows, size_t period,
                                                        vta_device_array_f32 *out);
 vta_status vta_sma_many_series_one_param_device(vta_ctx *ctx, const float *d_prices_tm,
-                                               const int32_t *d_first_valids,
-                                               const int32_t *d_first_nonfinite, size_t cols,
+                                               const int32_t *d_first_valids, size_t cols,
                                                size_t rows, size_t period, float *d_out_tm);
+vta_status vta_sma_multi_series_one_param_time_major_dev_from_device(
+    vta_ctx *ctx, const float *d_prices_tm, const int32_t *d_first_valids, size_t cols,
+    size_t rows, size_t period, vta_device_array_f32 *out);
 
 vta_status vta_wma_expand_grid(const vta_range_usize *range, size_t *periods, size_t cap,
                               size_t *n_out);
@@ -530,25 +642,28 @@ vta_status vta_wma_batch_device(vta_ctx *ctx, const float *d_prices, const int32
 vta_status vta_wma_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *data_tm,
                                                        size_t cols, size_t rows, size_t period,
                                                        vta_device_array_f32 *out);
-vta_status vta_wma_many_series_one_param_device(vta_ctx *ctx, const float *d_prices_tm,
-                                               const int32_t *d_first_valids,
-                                               const int32_t *d_first_nonfinite, size_t cols,
-                                               size_t rows, size_t period, float *d_out_tm);
+vta_status vta_wma_multi_series_one_param_time_major_device(vta_ctx *ctx,
+                                                            const float *d_prices_tm,
+                                                            int32_t period, int32_t num_series,
+                                                            int32_t series_len,
+                                                            const int32_t *d_first_valids,
+                                                            float *d_out_tm);
 
 vta_status vta_swma_expand_grid(const vta_range_usize *range, size_t *periods, size_t cap,
                               size_t *n_out);
 vta_status vta_swma_batch_dev(vta_ctx *ctx, const float *prices, size_t len,
                             const vta_range_usize *range, vta_device_array_f32 *out);
 vta_status vta_swma_batch_device(vta_ctx *ctx, const float *d_prices, const int32_t *d_periods,
-                               int32_t series_len, int32_t n_combos, int32_t first_valid,
-                               int32_t max_period, float *d_out);
+                                 const int32_t *d_warms, size_t series_len, size_t n_combos,
+                                 size_t max_period, float *d_out);
 vta_status vta_swma_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *data_tm,
                                                        size_t cols, size_t rows, size_t period,
                                                        vta_device_array_f32 *out);
-vta_status vta_swma_many_series_one_param_device(vta_ctx *ctx, const float *d_prices_tm,
-                                               const int32_t *d_first_valids,
-                                               const int32_t *d_first_nonfinite, size_t cols,
-                                               size_t rows, size_t period, float *d_out_tm);
+vta_status vta_swma_multi_series_one_param_device(vta_ctx *ctx, const float *d_prices_tm,
+                                                  const float *d_weights, int32_t period,
+                                                  int32_t num_series, int32_t series_len,
+                                                  const int32_t *d_first_valids,
+                                                  float *d_out_tm);
 
 /* ------------------------------------------------------------------ rolling windows */
 

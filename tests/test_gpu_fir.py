@@ -124,7 +124,8 @@ def test_many_series(ops, name):
 
 
 def test_device_entry_points(ops):
-    """Enqueue-only forms on device-resident inputs equal the host-input forms."""
+    """Enqueue-only forms on device-resident inputs, with the reference
+    wrappers' argument lists, equal the host-input forms."""
     x = datagen.gbm_close(20_000, 0.01, seed=205)
     x[:4] = np.nan
     x[12_000] = np.nan
@@ -136,51 +137,127 @@ def test_device_entry_points(ops):
         d_x = op.ctx.to_device(x)
         d_p = op.ctx.to_device(ps)
         d_o = op.ctx.alloc(ps.size * x.size * 4)
-        getattr(op, f"{name}_batch_device")(d_x, d_p, x.size, ps.size, 4, int(ps.max()), d_o)
+        if name == "swma":
+            # swma_wrapper.rs:663: warm indices first_valid + p - 1, no first_valid
+            d_w = op.ctx.to_device((4 + ps - 1).astype(np.int32))
+            op.swma_batch_device(d_x, d_p, d_w, x.size, ps.size, int(ps.max()), d_o)
+        else:
+            getattr(op, f"{name}_batch_device")(d_x, d_p, x.size, ps.size, 4, int(ps.max()), d_o)
         op.synchronize()
         dev = d_o.copy_to_host(np.float32).reshape(ps.size, x.size)
         assert np.array_equal(host.view(np.uint32), dev.view(np.uint32)), name
+    y = x.copy()
+    y[12_000] = 1.0
     for name, period in (("smma", (1, 50, 7)), ("wilders", (1, 50, 7))):
         op = ops[name]
-        y = x.copy()
-        y[12_000] = 1.0
         ps = np.asarray(op.expand_grid(period), dtype=np.int32)
         host = getattr(op, f"{name}_batch_dev")(y, period).to_host()
         d_x = op.ctx.to_device(y)
         d_p = op.ctx.to_device(ps)
+        d_w = op.ctx.to_device((4 + ps - 1).astype(np.int32))
         d_o = op.ctx.alloc(ps.size * y.size * 4)
-        getattr(op, f"{name}_batch_device")(d_x, d_p, int(ps.max()), y.size, 4, ps.size, d_o)
+        if name == "smma":
+            op.smma_batch_device(d_x, d_p, d_w, 4, y.size, ps.size, d_o)
+        else:
+            op.wilders_batch_device(d_x, d_p, None, d_w, y.size, 4, ps.size, d_o)
         op.synchronize()
         dev = d_o.copy_to_host(np.float32).reshape(ps.size, y.size)
         assert np.array_equal(host.view(np.uint32), dev.view(np.uint32)), name
-    # many-series device form with prepared metadata (first_valid, first non-finite)
+        if name == "wilders":
+            # the reference host code's f32 alphas 1/p: within the bound of the oracle
+            d_a = op.ctx.to_device((1.0 / ps.astype(np.float32)).astype(np.float32))
+            op.wilders_batch_device(d_x, d_p, d_a, d_w, y.size, 4, ps.size, d_o)
+            op.synchronize()
+            dev = d_o.copy_to_host(np.float32).reshape(ps.size, y.size)
+            _check(dev, np.stack([O.wilders(y.astype(np.float64), int(p)) for p in ps]),
+                   "wilders f32 alphas")
+    # many-series device forms (first non-finite bar found on the device)
     rng = np.random.default_rng(206)
     rows, cols = 3_000, 40
     tm = (50 + np.cumsum(rng.normal(0, 1, (rows, cols)), axis=0)).astype(np.float32)
     tm[:17, 3] = np.nan
     tm[1_000, 4] = np.nan
     d_tm = ctx.to_device(tm)
-    d_fv = ctx.alloc(cols * 4)
-    d_z = ctx.alloc(cols * 4)
-    check(vta.lib().vta_many_series_prepare(ctx.handle, ctypes.c_void_p(d_tm.ptr), None, None,
-                                            cols, rows, 4, ctypes.c_void_p(d_fv.ptr), None,
-                                            ctypes.c_void_p(d_z.ptr)))
-    ctx.synchronize()
-    fv = d_fv.copy_to_host(np.int32)
-    z = d_z.copy_to_host(np.int32)
-    assert fv[3] == 17 and fv[0] == 0 and z[4] == 1_000 and z[0] == rows
+    fv = np.argmax(~np.isnan(tm), axis=0).astype(np.int32)
+    d_fv = ctx.to_device(fv)
     for name in ("sma", "wma", "swma"):
         op = ops[name]
         host = getattr(op, f"{name}_multi_series_one_param_time_major_dev")(
             tm, cols, rows, 12).to_host()
         d_o = ctx.alloc(rows * cols * 4)
-        meth = {"sma": "sma_multi_series_one_param_device",
-                "wma": "wma_multi_series_one_param_time_major_device",
-                "swma": "swma_multi_series_one_param_device"}[name]
-        getattr(op, meth)(d_tm, d_fv, cols, rows, 12, d_o, d_first_nonfinite=d_z)
+        if name == "sma":
+            op.sma_many_series_one_param_device(d_tm, d_fv, cols, rows, 12, d_o)
+        elif name == "wma":
+            op.wma_multi_series_one_param_time_major_device(d_tm, 12, cols, rows, d_fv, d_o)
+        else:
+            op.swma_multi_series_one_param_device(d_tm, None, 12, cols, rows, d_fv, d_o)
         op.synchronize()
         dev = d_o.copy_to_host(np.float32).reshape(rows, cols)
         assert np.array_equal(host.view(np.uint32), dev.view(np.uint32)), name
+    tmc = np.abs(tm) + 1.0                       # SMMA / Wilders: clean, finite series
+    tmc[:17, 3] = np.nan
+    d_tmc = ctx.to_device(tmc)
+    for name in ("smma", "wilders"):
+        op = ops[name]
+        d_o = ctx.alloc(rows * cols * 4)
+        if name == "smma":
+            op.smma_multi_series_one_param_device(d_tmc, 12, cols, rows, d_fv, d_o)
+        else:
+            op.wilders_many_series_one_param_device(d_tmc, d_fv, 12, np.float32(1.0 / 12.0),
+                                                    cols, rows, d_o)
+        op.synchronize()
+        dev = d_o.copy_to_host(np.float32).reshape(rows, cols)
+        ref = O.ma_many(name, tmc.astype(np.float64), 12)[0]
+        _check(dev, ref, f"{name} many-series device")
+    # SMA's device-input forms that allocate their output (sma_wrapper.rs:830, 982)
+    op = ops["sma"]
+    ps = [3, 10, 44]
+    got = op.sma_batch_dev_from_device(ctx.to_device(x), x.size, ps, 4).to_host()
+    host = op.sma_batch_dev(x, (3, 44, 41)).to_host()
+    assert np.array_equal(got[[0, 2]].view(np.uint32), host.view(np.uint32))
+    _check(got, np.stack([O.sma_batch_row(x.astype(np.float64), p) for p in ps]), "sma")
+    got = op.sma_multi_series_one_param_time_major_dev_from_device(d_tm, d_fv, cols, rows,
+                                                                   12).to_host()
+    host = op.sma_multi_series_one_param_time_major_dev(tm, cols, rows, 12).to_host()
+    assert np.array_equal(got.view(np.uint32), host.view(np.uint32))
+
+
+def test_warm_index_mismatch_reported_at_synchronize(ops):
+    """Warm indices other than first_valid + period - 1 (the reference host
+    code's values) are an InvalidInput that the next synchronize reports once;
+    consistent ones report nothing."""
+    x = datagen.gbm_close(5_000, 0.01, seed=208)
+    x[:4] = np.nan
+    for name in ("smma", "wilders", "swma"):
+        op = ops[name]
+        ctx = op.ctx
+        ctx.synchronize()
+        ps = np.array([5, 9, 14], dtype=np.int32)
+        d_x, d_p = ctx.to_device(x), ctx.to_device(ps)
+        d_o = ctx.alloc(ps.size * x.size * 4)
+        bad = (4 + ps - 1).astype(np.int32)
+        bad[1] += 1
+        d_bad = ctx.to_device(bad)
+        if name == "smma":
+            op.smma_batch_device(d_x, d_p, d_bad, 4, x.size, ps.size, d_o)
+        elif name == "wilders":
+            op.wilders_batch_device(d_x, d_p, None, d_bad, x.size, 4, ps.size, d_o)
+        else:
+            op.swma_batch_device(d_x, d_p, d_bad, x.size, ps.size, int(ps.max()), d_o)
+        with pytest.raises(vta.VtaError) as e:
+            op.synchronize()
+        assert e.value.kind == "InvalidInput" and "warm index" in str(e.value)
+        op.synchronize()                             # reported once, then clear
+    # SWMA: a warm index implying a first_valid past the longest period's room
+    op = ops["swma"]
+    ps = np.array([5, 9], dtype=np.int32)
+    far = (x.size - 3 + ps - 1).astype(np.int32)
+    d_o = op.ctx.alloc(ps.size * x.size * 4)
+    op.swma_batch_device(op.ctx.to_device(x), op.ctx.to_device(ps), op.ctx.to_device(far),
+                         x.size, ps.size, 9, d_o)
+    with pytest.raises(vta.VtaError):
+        op.synchronize()
+    assert np.isnan(d_o.copy_to_host(np.float32)).all()
 
 
 def _ref_data(n, lead, fa, fb):

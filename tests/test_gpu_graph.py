@@ -31,8 +31,8 @@ def _batch(ctx, alma_plan, d, outs, T, fv):
     """Enqueues the whole batch on ctx's stream (no sync, no allocation)."""
     L = lib()
     h = ctx.handle
-    check(L.vta_ema_batch_device(h, P(d["c"].ptr), P(d["p"].ptr), None, d["maxp"], T, fv,
-                                 d["C"], P(outs["ema"].ptr)))
+    check(L.vta_ema_batch_device(h, P(d["c"].ptr), P(d["p"].ptr), None, T, fv, d["C"],
+                                 P(outs["ema"].ptr)))
     check(L.vta_rsi_batch_device(h, P(d["c"].ptr), P(d["p"].ptr), d["maxp"], T, fv, d["C"],
                                  P(outs["rsi"].ptr)))
     check(L.vta_adx_batch_device(h, P(d["h"].ptr), P(d["l"].ptr), P(d["c"].ptr), P(d["p"].ptr),

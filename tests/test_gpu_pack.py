@@ -40,10 +40,17 @@ def _direct(kind, ctx, d, job):
     T, fv, C, mp = job["series_len"], job["first_valid"], job["n_combos"], job["max_period"]
     out = P(d["direct"].ptr)
     if kind == "ema":
-        check(L.vta_ema_batch_device(h, P(d["c"].ptr), P(d["p"].ptr), None, mp, T, fv, C, out))
-    elif kind in ("rsi", "smma", "wilders"):
-        fn = getattr(L, f"vta_{kind}_batch_device")
-        check(fn(h, P(d["c"].ptr), P(d["p"].ptr), mp, T, fv, C, out))
+        check(L.vta_ema_batch_device(h, P(d["c"].ptr), P(d["p"].ptr), None, T, fv, C, out))
+    elif kind == "rsi":
+        check(L.vta_rsi_batch_device(h, P(d["c"].ptr), P(d["p"].ptr), mp, T, fv, C, out))
+    elif kind == "smma":
+        check(L.vta_smma_batch_device(h, P(d["c"].ptr), P(d["p"].ptr), P(d["w"].ptr), fv, T, C,
+                                      out))
+    elif kind == "wilders":
+        # warm = fv + p - 1 as the reference's host code sets it; no alphas
+        # buffer, so the smoothing factor is the f64 1/p the pack uses
+        check(L.vta_wilders_batch_device(h, P(d["c"].ptr), P(d["p"].ptr), None, P(d["w"].ptr),
+                                         T, fv, C, out))
     else:
         fn = getattr(L, f"vta_{kind}_batch_device")
         check(fn(h, P(d["h"].ptr), P(d["l"].ptr), P(d["c"].ptr), P(d["p"].ptr), mp, T, fv, C,
@@ -83,7 +90,8 @@ def test_pack_matches_direct_calls(kind):
         C = int(periods.size)
         d = {"c": ctx.to_device(c), "h": ctx.to_device(hi), "l": ctx.to_device(lo),
              "p": ctx.to_device(periods), "out": ctx.alloc(C * T * 4),
-             "direct": ctx.alloc(C * T * 4)}
+             "direct": ctx.alloc(C * T * 4),
+             "w": ctx.to_device((lead + periods - 1).astype(np.int32))}
         job = {"close": d["c"], "periods": d["p"], "out": d["out"], "series_len": T,
                "first_valid": lead, "n_combos": C, "max_period": int(periods.max())}
         if kind in HLC:

@@ -234,25 +234,152 @@ def test_device_entry_points(ctxs):
     ps = np.arange(5, 200, 7, dtype=np.int32)
     d_x, d_p = ctx.to_device(x), ctx.to_device(ps)
     d_out = ctx.alloc(ps.size * x.size * 4)
-    ctxs["ema"].ema_batch_device(d_x, d_p, int(ps.max()), x.size, 10, ps.size, d_out)
+    ref = O.ema_batch(x.astype(np.float64), ps.tolist())
+    # ema_wrapper.rs:256 argument order; no alphas: 2/(p+1) in f64
+    ctxs["ema"].ema_batch_device(d_x, d_p, None, x.size, 10, ps.size, d_out)
     ctx.synchronize()
     g = d_out.copy_to_host().reshape(ps.size, x.size)
-    _check(g, O.ema_batch(x.astype(np.float64), ps.tolist()), "ema")
-    # many-series device form with and without clean flags
+    _check(g, ref, "ema")
+    assert np.array_equal(g.view(np.uint32),
+                          ctxs["ema"].ema_batch_dev(x, (5, 199, 7)).to_host().view(np.uint32))
+    # the reference host code's f32 alphas 2/(p+1): within the bound of the f64 ones
+    d_a = ctx.to_device((2.0 / (ps.astype(np.float32) + 1.0)).astype(np.float32))
+    ctxs["ema"].ema_batch_device(d_x, d_p, d_a, x.size, 10, ps.size, d_out)
+    ctx.synchronize()
+    _check(d_out.copy_to_host().reshape(ps.size, x.size), ref, "ema")
+    # many-series device form: the caller's alpha; clean flags found on the device
     h, lo, c = _ms_data(96, 10_000, 44)
+    c[7_000, 5] = np.nan                                   # one unclean series
     rows, cols = c.shape
     d_c = ctx.to_device(c)
     fv = np.argmax(~np.isnan(c), axis=0).astype(np.int32)
     d_fv = ctx.to_device(fv)
     d_o = ctx.alloc(rows * cols * 4)
     ref = O.many_series("ema", c.astype(np.float64), 20)[0]
-    ctxs["ema"].ema_many_series_one_param_device(d_c, d_fv, 20, cols, rows, d_o, None)
+    ctxs["ema"].ema_many_series_one_param_device(d_c, d_fv, 20, np.float32(2.0 / 21.0), cols,
+                                                 rows, d_o)
     ctx.synchronize()
     _check(d_o.copy_to_host().reshape(rows, cols), ref, "ema")
-    d_cl = ctx.to_device(np.ones(cols, dtype=np.int32))
-    ctxs["ema"].ema_many_series_one_param_device(d_c, d_fv, 20, cols, rows, d_o, d_cl)
+    assert ctxs["ema"].selected_many_series_kernel() == ("chunked", 64)
+
+
+def _ema_custom_alpha(x, p, alpha):
+    """EMA with the reference kernels' structure and a given smoothing factor:
+    running mean over the first p valid bars (outputs from first_valid), then
+    prev = (1 - alpha) prev + alpha x, skipping non-finite bars."""
+    out = np.full(x.size, np.nan)
+    fv = int(np.argmax(~np.isnan(x)))
+    prev, vc = 0.0, 0
+    for t in range(fv, x.size):
+        v = float(x[t])
+        if t < fv + p:
+            if np.isfinite(v):
+                vc += 1
+                prev = ((vc - 1) * prev + v) / vc
+        elif np.isfinite(v):
+            prev = (1.0 - alpha) * prev + alpha * v
+        out[t] = prev
+    return out
+
+
+def test_device_forms_use_caller_alphas(ctxs):
+    """ema_batch_device / ema_many_series_one_param_device / the Wilders device
+    forms run with the caller's smoothing factors, not 2/(p+1) or 1/p."""
+    x = datagen.gbm_close(3_000, 0.01, seed=45).astype(np.float64)
+    x[:6] = np.nan
+    x = x.astype(np.float32)
+    ctx = ctxs["ema"].ctx
+    ps = np.array([5, 9, 30], dtype=np.int32)
+    al = np.array([0.05, 0.5, 0.9], dtype=np.float32)
+    d_x, d_p, d_a = ctx.to_device(x), ctx.to_device(ps), ctx.to_device(al)
+    d_o = ctx.alloc(ps.size * x.size * 4)
+    ctxs["ema"].ema_batch_device(d_x, d_p, d_a, x.size, 6, ps.size, d_o)
     ctx.synchronize()
-    _check(d_o.copy_to_host().reshape(rows, cols), ref, "ema")
+    g = d_o.copy_to_host().reshape(ps.size, x.size)
+    ref = np.stack([_ema_custom_alpha(x.astype(np.float64), int(p), float(a))
+                    for p, a in zip(ps, al)])
+    nan_mask_equal(g, ref)
+    assert max_rel_err(g, ref) <= 1e-5
+    # many-series, one alpha for all series
+    tm = np.stack([x, np.roll(x, 0) * 1.1], axis=1).astype(np.float32)
+    d_tm = ctx.to_device(tm)
+    d_fv = ctx.to_device(np.array([6, 6], dtype=np.int32))
+    d_o2 = ctx.alloc(tm.size * 4)
+    ctxs["ema"].ema_many_series_one_param_device(d_tm, d_fv, 9, np.float32(0.25), 2, x.size,
+                                                 d_o2)
+    ctx.synchronize()
+    g2 = d_o2.copy_to_host().reshape(x.size, 2)
+    for j in range(2):
+        r = _ema_custom_alpha(tm[:, j].astype(np.float64), 9, 0.25)
+        nan_mask_equal(g2[:, j], r)
+        assert max_rel_err(g2[:, j], r) <= 1e-5
+
+
+def test_policies_and_selected_kernels(ctxs):
+    """CudaEmaPolicy / CudaRsiPolicy / CudaAdxPolicy: the plain form (one lane
+    per combo over the whole series) gives the same values within the bound,
+    and the selected kernel reports which form ran."""
+    x = datagen.gbm_close(40_000, 0.01, seed=46)
+    x[:3] = np.nan
+    h, lo, c = _hlc(40_000, 46, lead=3)
+    for name in ("ema", "rsi", "adx"):
+        op = ctxs[name]
+        assert op.selected_batch_kernel() is None or op.selected_batch_kernel()[1] == 64
+        args = (h, lo, c) if name == "adx" else (x,)
+        f = [a.astype(np.float64) for a in args]
+        ps = op.expand_grid((5, 60, 11))
+        ref = getattr(O, f"{name}_batch")(*f, ps)
+        op.set_policy(vta.KernelPolicy(batch="plain", batch_block_x=64))
+        assert op.policy() == vta.KernelPolicy("plain", 64, "auto", 0)
+        g = getattr(op, f"{name}_batch_dev")(*args, (5, 60, 11)).to_host()
+        assert op.selected_batch_kernel() == ("plain", 64)
+        _check(g, ref, name)
+        op.set_policy(vta.KernelPolicy())
+        g = getattr(op, f"{name}_batch_dev")(*args, (5, 60, 11)).to_host()
+        assert op.selected_batch_kernel() == ("chunked", 64)
+        _check(g, ref, name)
+        with pytest.raises(vta.VtaError) as e:
+            op.set_policy(vta.KernelPolicy(batch="plain", batch_block_x=256))
+        assert e.value.kind == "InvalidPolicy"
+    # many-series plain form
+    hm, lm, cm = _ms_data(70, 8_000, 47)
+    cols, rows = cm.shape[1], cm.shape[0]
+    op = vta.CudaRsi(0, policy=vta.KernelPolicy(many_series="plain"))
+    g = op.rsi_many_series_one_param_time_major_dev(cm, cols, rows, 14).to_host()
+    assert op.selected_many_series_kernel() == ("plain", 64)
+    _check(g, O.many_series("rsi", cm.astype(np.float64), 14)[0], "rsi")
+
+
+def test_host_copy_forms(ctxs):
+    """ema_batch_into_host_f32 / ema_many_series_..._into_host_f32 and the ADX
+    pair (ema_wrapper.rs:309, 422; adx_wrapper.rs:253, 417): the device
+    results copied into caller-owned host slices."""
+    x = datagen.gbm_close(20_000, 0.01, seed=48)
+    x[:2] = np.nan
+    h, lo, c = _hlc(20_000, 48, lead=2)
+    ps = ctxs["ema"].expand_grid((4, 90, 9))
+    out = np.full(len(ps) * x.size, -1.0, np.float32)
+    ctxs["ema"].ema_batch_into_host_f32(x, (4, 90, 9), out)
+    _check(out.reshape(len(ps), -1), O.ema_batch(x.astype(np.float64), ps), "ema")
+    with pytest.raises(vta.VtaError):
+        ctxs["ema"].ema_batch_into_host_f32(x, (4, 90, 9), out[:-1])
+    aps = ctxs["adx"].expand_grid((5, 40, 7))
+    out = np.empty(len(aps) * c.size, np.float32)
+    rows, cols, got = ctxs["adx"].adx_batch_into_host_f32(h, lo, c, (5, 40, 7), out)
+    assert (rows, cols, got) == (len(aps), c.size, aps)
+    f = [a.astype(np.float64) for a in (h, lo, c)]
+    _check(out.reshape(rows, cols), O.adx_batch(*f, aps), "adx")
+    hm, lm, cm = _ms_data(33, 5_000, 49)
+    R, N = cm.shape
+    o = np.empty(R * N, np.float32)
+    ctxs["ema"].ema_many_series_one_param_time_major_into_host_f32(cm, N, R, 20, o)
+    _check(o.reshape(R, N), O.many_series("ema", cm.astype(np.float64), 20)[0], "ema")
+    ctxs["adx"].adx_many_series_one_param_time_major_into_host_f32(hm, lm, cm, N, R, 14, o)
+    fm = [a.astype(np.float64) for a in (hm, lm, cm)]
+    _check(o.reshape(R, N), O.many_series_hlc("adx", *fm, 14)[0], "adx")
+    with pytest.raises(vta.VtaError):
+        ctxs["adx"].adx_many_series_one_param_time_major_into_host_f32(hm, lm, cm, N, R, 14,
+                                                                       o[:-2])
 
 
 def test_errors(ctxs):

@@ -80,7 +80,7 @@ def test_scan_waves_and_ms_chunk_autotune():
     periods = np.arange(2, 402, 4, dtype=np.int32)
     d_x, d_p = ctx.to_device(x), ctx.to_device(periods)
     d_o = ctx.alloc(periods.size * T * 4)
-    run = lambda: ema.ema_batch_device(d_x, d_p, int(periods.max()), T, 0, int(periods.size), d_o)
+    run = lambda: ema.ema_batch_device(d_x, d_p, None, T, 0, int(periods.size), d_o)
     best, times = at.measure(ctx, at.SCAN_WAVES, at.CANDIDATES[at.SCAN_WAVES], run, reps=3)
     print(f"scan waves {best}: {times}")
     assert best in at.CANDIDATES[at.SCAN_WAVES] and all(v is not None for v in times.values())

@@ -44,25 +44,48 @@ def _refused_by_length(fn, what):
 def _cases():
     """(name, call(sizes) -> None, exact sizes, {argument: what})."""
     ema, smma, atr = op(vta.CudaEma), op(vta.CudaSmma), op(vta.CudaAtr)
+    wil, swma, wma = op(vta.CudaWilders), op(vta.CudaSwma), op(vta.CudaWma)
     sma, bb, alma = op(vta.CudaSma), op(vta.CudaBollingerBands), op(vta.CudaAlma)
     batch = {"prices": T, "periods": C, "output": C * T}
-    ms = {"time-major input": CR, "first_valids": COLS, "time-major output": CR,
-          "clean flags": COLS}
+    ms = {"time-major input": CR, "first_valids": COLS, "time-major output": CR}
     return [
         ("ema_batch_device",
-         lambda s: ema.ema_batch_device(buf(s["prices"]), buf(s["periods"]), MAXP, T, 0, C,
-                                        buf(s["output"])), batch),
+         lambda s: ema.ema_batch_device(buf(s["prices"]), buf(s["periods"]), buf(s["alphas"]),
+                                        T, 0, C, buf(s["output"])),
+         dict(batch, alphas=C)),
         ("ema_many_series_one_param_device",
          lambda s: ema.ema_many_series_one_param_device(
-             buf(s["time-major input"]), buf(s["first_valids"]), 5, COLS, ROWS,
-             buf(s["time-major output"]), buf(s["clean flags"])), ms),
+             buf(s["time-major input"]), buf(s["first_valids"]), 5, 0.3, COLS, ROWS,
+             buf(s["time-major output"])), ms),
         ("smma_batch_device",
-         lambda s: smma.smma_batch_device(buf(s["prices"]), buf(s["periods"]), MAXP, T, 0, C,
-                                          buf(s["output"])), batch),
+         lambda s: smma.smma_batch_device(buf(s["prices"]), buf(s["periods"]), buf(s["warms"]),
+                                          0, T, C, buf(s["output"])),
+         dict(batch, warms=C)),
         ("smma_multi_series_one_param_device",
          lambda s: smma.smma_multi_series_one_param_device(
-             buf(s["time-major input"]), buf(s["first_valids"]), 5, COLS, ROWS,
-             buf(s["time-major output"]), buf(s["clean flags"])), ms),
+             buf(s["time-major input"]), 5, COLS, ROWS, buf(s["first_valids"]),
+             buf(s["time-major output"])), ms),
+        ("wilders_batch_device",
+         lambda s: wil.wilders_batch_device(buf(s["prices"]), buf(s["periods"]),
+                                            buf(s["alphas"]), buf(s["warms"]), T, 0, C,
+                                            buf(s["output"])),
+         dict(batch, alphas=C, warms=C)),
+        ("wilders_many_series_one_param_device",
+         lambda s: wil.wilders_many_series_one_param_device(
+             buf(s["time-major input"]), buf(s["first_valids"]), 5, 0.2, COLS, ROWS,
+             buf(s["time-major output"])), ms),
+        ("swma_batch_device",
+         lambda s: swma.swma_batch_device(buf(s["prices"]), buf(s["periods"]), buf(s["warms"]),
+                                          T, C, MAXP, buf(s["output"])),
+         dict(batch, warms=C)),
+        ("swma_multi_series_one_param_device",
+         lambda s: swma.swma_multi_series_one_param_device(
+             buf(s["time-major input"]), None, 5, COLS, ROWS, buf(s["first_valids"]),
+             buf(s["time-major output"])), ms),
+        ("wma_multi_series_one_param_time_major_device",
+         lambda s: wma.wma_multi_series_one_param_time_major_device(
+             buf(s["time-major input"]), 5, COLS, ROWS, buf(s["first_valids"]),
+             buf(s["time-major output"])), ms),
         ("tr_from_hlc_device",
          lambda s: atr.tr_from_hlc_device(buf(s["high"]), buf(s["low"]), buf(s["close"]), T, 0,
                                           buf(s["tr output"])),
@@ -73,12 +96,10 @@ def _cases():
         ("sma_batch_device",
          lambda s: sma.sma_batch_device(buf(s["prices"]), buf(s["periods"]), T, C, 0, MAXP,
                                         buf(s["output"])), batch),
-        ("sma_multi_series_one_param_device",
-         lambda s: sma.sma_multi_series_one_param_device(
+        ("sma_many_series_one_param_device",
+         lambda s: sma.sma_many_series_one_param_device(
              buf(s["time-major input"]), buf(s["first_valids"]), COLS, ROWS, 5,
-             buf(s["time-major output"]), d_first_nonfinite=buf(s["first_nonfinite"])),
-         {"time-major input": CR, "first_valids": COLS, "time-major output": CR,
-          "first_nonfinite": COLS}),
+             buf(s["time-major output"])), ms),
         ("bollinger_bands_many_series_one_param_device",
          lambda s: bb.bollinger_bands_many_series_one_param_device(
              buf(s["time-major input"]), buf(s["first_valids"]), buf(s["first_nonfinite"]),
@@ -123,7 +144,7 @@ def test_device_buffer_lengths_checked(case):
 def test_unsized_buffers_pass_through():
     """Raw pointers (no byte size) and absent optional buffers are not checked."""
     ema = op(vta.CudaEma)
-    _refused_by_library(lambda: ema.ema_batch_device(NS(ptr=0), NS(ptr=0), MAXP, T, 0, C,
+    _refused_by_library(lambda: ema.ema_batch_device(NS(ptr=0), NS(ptr=0), None, T, 0, C,
                                                      NS(ptr=0)))
     _refused_by_library(lambda: ema.ema_many_series_one_param_device(
-        buf(CR), buf(COLS), 5, COLS, ROWS, buf(CR), None))
+        NS(ptr=0), buf(COLS), 5, 0.3, COLS, ROWS, buf(CR)))

@@ -11,8 +11,8 @@ from .cross import DoubleAlmaCrossover, DoubleAlmaPlan
 from .device import (DeviceArrayF32, DeviceBuffer, HipContext, HipGraph, cuda_available,
                      cuda_device_count)
 from .fir import CudaSma, CudaSwma, CudaWma
-from .rec import (CudaAdx, CudaAtr, CudaEma, CudaRsi, CudaSmma, CudaWilders, ManySeriesOhlcPlan,
-                  RecPack, ema_rsi_atr_many_series_dev)
+from .rec import (CudaAdx, CudaAtr, CudaEma, CudaRsi, CudaSmma, CudaWilders, KernelPolicy,
+                  ManySeriesOhlcPlan, RecPack, ema_rsi_atr_many_series_dev)
 from .server import IndicatorServer, ServedResult
 from .rolling import (BbPack, BollingerBandsBatchRange, CudaBollingerBands, CudaMediumAd,
                       CudaPercentileNearestRank, PercentileNearestRankBatchRange,
@@ -26,5 +26,5 @@ __all__ = [
     "CudaMediumAd", "CudaPercentileNearestRank", "PercentileNearestRankBatchRange",
     "RollingManySeriesPlan", "DoubleAlmaCrossover", "DoubleAlmaPlan", "IndicatorServer",
     "ServedResult", "CudaSma", "CudaWma", "CudaSwma", "CudaSmma", "CudaWilders", "RecPack",
-    "BbPack",
+    "BbPack", "KernelPolicy",
 ]

@@ -100,6 +100,15 @@ class VtaResultC(ctypes.Structure):
                 ("t_launch", ctypes.c_double)]
 
 
+class KernelPolicyC(ctypes.Structure):
+    _fields_ = [("batch", ctypes.c_int32), ("batch_block_x", ctypes.c_uint32),
+                ("many_series", ctypes.c_int32), ("many_series_block_x", ctypes.c_uint32)]
+
+
+class KernelSelectedC(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("block_x", ctypes.c_uint32)]
+
+
 class VtaServerStatsC(ctypes.Structure):
     _fields_ = [(n, ctypes.c_size_t) for n in ("budget", "in_use", "peak_in_use", "queued",
                                                 "in_flight", "peak_in_flight", "cached_series",
@@ -178,24 +187,29 @@ SIGNATURES = {
     "vta_ema_expand_grid": (_ST, [ctypes.POINTER(RangeUsize), _P, _SZ, ctypes.POINTER(_SZ)]),
     "vta_ema_batch_dev": (_ST, [_P, _P, _SZ, ctypes.POINTER(RangeUsize),
                                 ctypes.POINTER(DeviceArrayF32C)]),
-    "vta_ema_batch_device": (_ST, [_P, _P, _P, _P, _I32, _I32, _I32, _I32, _P]),
+    "vta_ema_batch_device": (_ST, [_P, _P, _P, _P, _SZ, _SZ, _SZ, _P]),
+    "vta_ema_batch_into_host_f32": (_ST, [_P, _P, _SZ, ctypes.POINTER(RangeUsize), _P, _SZ]),
     "vta_ema_many_series_one_param_time_major_dev": (_ST, [_P, _P, _SZ, _SZ, _SZ,
                                                            ctypes.POINTER(DeviceArrayF32C)]),
-    "vta_ema_many_series_one_param_device": (_ST, [_P, _P, _P, _SZ, _SZ, _SZ, _P, _P]),
+    "vta_ema_many_series_one_param_device": (_ST, [_P, _P, _P, _I32, _F32, _SZ, _SZ, _P]),
+    "vta_ema_many_series_one_param_time_major_into_host_f32": (_ST, [_P, _P, _SZ, _SZ, _SZ, _P,
+                                                                     _SZ]),
     "vta_smma_expand_grid": (_ST, [ctypes.POINTER(RangeUsize), _P, _SZ, ctypes.POINTER(_SZ)]),
     "vta_smma_batch_dev": (_ST, [_P, _P, _SZ, ctypes.POINTER(RangeUsize),
                                 ctypes.POINTER(DeviceArrayF32C)]),
-    "vta_smma_batch_device": (_ST, [_P, _P, _P, _I32, _I32, _I32, _I32, _P]),
+    "vta_smma_batch_device": (_ST, [_P, _P, _P, _P, _SZ, _SZ, _SZ, _P]),
     "vta_smma_many_series_one_param_time_major_dev": (_ST, [_P, _P, _SZ, _SZ, _SZ,
                                                            ctypes.POINTER(DeviceArrayF32C)]),
-    "vta_smma_many_series_one_param_device": (_ST, [_P, _P, _P, _SZ, _SZ, _SZ, _P, _P]),
+    "vta_smma_multi_series_one_param_device": (_ST, [_P, _P, _I32, _I32, _I32, _P, _P]),
     "vta_wilders_expand_grid": (_ST, [ctypes.POINTER(RangeUsize), _P, _SZ, ctypes.POINTER(_SZ)]),
     "vta_wilders_batch_dev": (_ST, [_P, _P, _SZ, ctypes.POINTER(RangeUsize),
                                 ctypes.POINTER(DeviceArrayF32C)]),
-    "vta_wilders_batch_device": (_ST, [_P, _P, _P, _I32, _I32, _I32, _I32, _P]),
+    "vta_wilders_batch_device": (_ST, [_P, _P, _P, _P, _P, _I32, _I32, _I32, _P]),
     "vta_wilders_many_series_one_param_time_major_dev": (_ST, [_P, _P, _SZ, _SZ, _SZ,
                                                            ctypes.POINTER(DeviceArrayF32C)]),
-    "vta_wilders_many_series_one_param_device": (_ST, [_P, _P, _P, _SZ, _SZ, _SZ, _P, _P]),
+    "vta_wilders_many_series_one_param_device": (_ST, [_P, _P, _P, _I32, _F32, _SZ, _SZ, _P]),
+    "vta_ma_many_series_prepared_device": (_ST, [_P, ctypes.c_int, _P, _P, _P, _SZ, _F32, _SZ,
+                                                 _SZ, _P]),
     "vta_tr_from_hlc_device": (_ST, [_P, _P, _P, _P, _I32, _I32, _P]),
     "vta_atr_batch_device_with_tr": (_ST, [_P, _P, _P, _I32, _I32, _I32, _I32, _P]),
     # finite-window averages
@@ -205,21 +219,26 @@ SIGNATURES = {
     "vta_sma_batch_device": (_ST, [_P, _P, _P, _I32, _I32, _I32, _I32, _P]),
     "vta_sma_many_series_one_param_time_major_dev": (_ST, [_P, _P, _SZ, _SZ, _SZ,
                                                            ctypes.POINTER(DeviceArrayF32C)]),
-    "vta_sma_many_series_one_param_device": (_ST, [_P, _P, _P, _P, _SZ, _SZ, _SZ, _P]),
+    "vta_sma_many_series_one_param_device": (_ST, [_P, _P, _P, _SZ, _SZ, _SZ, _P]),
+    "vta_sma_batch_dev_from_device": (_ST, [_P, _P, _SZ, _P, _SZ, _SZ,
+                                            ctypes.POINTER(DeviceArrayF32C)]),
+    "vta_sma_multi_series_one_param_time_major_dev_from_device": (
+        _ST, [_P, _P, _P, _SZ, _SZ, _SZ, ctypes.POINTER(DeviceArrayF32C)]),
     "vta_wma_expand_grid": (_ST, [ctypes.POINTER(RangeUsize), _P, _SZ, ctypes.POINTER(_SZ)]),
     "vta_wma_batch_dev": (_ST, [_P, _P, _SZ, ctypes.POINTER(RangeUsize),
                                 ctypes.POINTER(DeviceArrayF32C)]),
     "vta_wma_batch_device": (_ST, [_P, _P, _P, _I32, _I32, _I32, _I32, _P]),
     "vta_wma_many_series_one_param_time_major_dev": (_ST, [_P, _P, _SZ, _SZ, _SZ,
                                                            ctypes.POINTER(DeviceArrayF32C)]),
-    "vta_wma_many_series_one_param_device": (_ST, [_P, _P, _P, _P, _SZ, _SZ, _SZ, _P]),
+    "vta_wma_multi_series_one_param_time_major_device": (_ST, [_P, _P, _I32, _I32, _I32, _P,
+                                                                _P]),
     "vta_swma_expand_grid": (_ST, [ctypes.POINTER(RangeUsize), _P, _SZ, ctypes.POINTER(_SZ)]),
     "vta_swma_batch_dev": (_ST, [_P, _P, _SZ, ctypes.POINTER(RangeUsize),
                                 ctypes.POINTER(DeviceArrayF32C)]),
-    "vta_swma_batch_device": (_ST, [_P, _P, _P, _I32, _I32, _I32, _I32, _P]),
+    "vta_swma_batch_device": (_ST, [_P, _P, _P, _P, _SZ, _SZ, _SZ, _P]),
     "vta_swma_many_series_one_param_time_major_dev": (_ST, [_P, _P, _SZ, _SZ, _SZ,
                                                            ctypes.POINTER(DeviceArrayF32C)]),
-    "vta_swma_many_series_one_param_device": (_ST, [_P, _P, _P, _P, _SZ, _SZ, _SZ, _P]),
+    "vta_swma_multi_series_one_param_device": (_ST, [_P, _P, _P, _I32, _I32, _I32, _P, _P]),
     "vta_rsi_expand_grid": (_ST, [ctypes.POINTER(RangeUsize), _P, _SZ, ctypes.POINTER(_SZ)]),
     "vta_rsi_batch_dev": (_ST, [_P, _P, _SZ, ctypes.POINTER(RangeUsize),
                                 ctypes.POINTER(DeviceArrayF32C)]),
@@ -239,6 +258,26 @@ SIGNATURES = {
                                 ctypes.POINTER(DeviceArrayF32C)]),
     "vta_adx_many_series_one_param_time_major_dev": (_ST, [_P, _P, _P, _P, _SZ, _SZ, _SZ,
                                                            ctypes.POINTER(DeviceArrayF32C)]),
+    "vta_adx_batch_into_host_f32": (_ST, [_P, _P, _P, _P, _SZ, ctypes.POINTER(RangeUsize), _P,
+                                          _SZ]),
+    "vta_adx_many_series_one_param_time_major_into_host_f32": (_ST, [_P, _P, _P, _P, _SZ, _SZ,
+                                                                     _SZ, _P, _SZ]),
+    "vta_ema_set_policy": (_ST, [_P, ctypes.POINTER(KernelPolicyC)]),
+    "vta_ema_policy": (_ST, [_P, ctypes.POINTER(KernelPolicyC)]),
+    "vta_ema_selected_batch_kernel": (_ST, [_P, ctypes.POINTER(KernelSelectedC)]),
+    "vta_ema_selected_many_series_kernel": (_ST, [_P, ctypes.POINTER(KernelSelectedC)]),
+    "vta_rsi_set_policy": (_ST, [_P, ctypes.POINTER(KernelPolicyC)]),
+    "vta_rsi_policy": (_ST, [_P, ctypes.POINTER(KernelPolicyC)]),
+    "vta_rsi_selected_batch_kernel": (_ST, [_P, ctypes.POINTER(KernelSelectedC)]),
+    "vta_rsi_selected_many_series_kernel": (_ST, [_P, ctypes.POINTER(KernelSelectedC)]),
+    "vta_adx_set_policy": (_ST, [_P, ctypes.POINTER(KernelPolicyC)]),
+    "vta_adx_policy": (_ST, [_P, ctypes.POINTER(KernelPolicyC)]),
+    "vta_adx_selected_batch_kernel": (_ST, [_P, ctypes.POINTER(KernelSelectedC)]),
+    "vta_adx_selected_many_series_kernel": (_ST, [_P, ctypes.POINTER(KernelSelectedC)]),
+    "vta_pnr_set_policy": (_ST, [_P, ctypes.POINTER(KernelPolicyC)]),
+    "vta_pnr_policy": (_ST, [_P, ctypes.POINTER(KernelPolicyC)]),
+    "vta_pnr_selected_batch_kernel": (_ST, [_P, ctypes.POINTER(KernelSelectedC)]),
+    "vta_pnr_selected_many_series_kernel": (_ST, [_P, ctypes.POINTER(KernelSelectedC)]),
     "vta_many_series_prepare": (_ST, [_P, _P, _P, _P, _SZ, _SZ, ctypes.c_int, _P, _P, _P]),
     "vta_ema_rsi_atr_many_series_device": (_ST, [_P, _P, _P, _P, _SZ, _SZ, _SZ, _SZ, _SZ, _P,
                                                  _P, _P, _P, _P, _P]),

@@ -209,28 +209,74 @@ vta_status batch_dev(vta_ctx *ctx, Fir k, const float *x, size_t len,
 
 // Device-resident one-series batch, enqueue only: first_valid from the
 // caller, the first non-finite bar found on the device into the context
-// scratch.
+// scratch.  With d_warms (SWMA's device form, which passes warm indices
+// first_valid + period - 1 and no first_valid) first_valid is derived from
+// combo 0 on the device and every warm index checked against it.
 vta_status batch_device(vta_ctx *ctx, Fir k, const float *d_x, const int32_t *d_periods,
-                        int32_t series_len, int32_t n_combos, int32_t first_valid,
-                        int32_t max_period, float *d_out) {
+                        int64_t series_len, int64_t n_combos, int64_t first_valid,
+                        int64_t max_period, float *d_out, const int32_t *d_warms = nullptr) {
     clear_error();
     if (!ctx || !d_x || !d_periods || !d_out) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    if (d_warms) first_valid = 0;   // derived on the device
     if (series_len <= 0 || n_combos <= 0 || first_valid < 0 || first_valid >= series_len ||
-        max_period < (int32_t)min_period(k) || max_period > series_len - first_valid)
+        max_period < (int64_t)min_period(k) || max_period > series_len - first_valid ||
+        series_len > INT32_MAX || n_combos > INT32_MAX)
         return fail(VTA_ERR_INVALID_INPUT,
                     "series_len, n_combos must be positive, first_valid < series_len and "
                     "%zu <= max_period <= series_len - first_valid", min_period(k));
     int32_t *meta = static_cast<int32_t *>(ctx->scratch) + 192;
-    hipFunction_t fill, fnf;
+    hipFunction_t fill;
     VTA_TRY(get_function(ctx, "rec", "vta_fill_i32", &fill));
-    VTA_TRY(get_function(ctx, "fir", "series_first_nonfinite", &fnf));
-    VTA_TRY(launch(ctx, fill, dim3(1), dim3(256), 0, meta, 1, (int)first_valid));
     VTA_TRY(launch(ctx, fill, dim3(1), dim3(256), 0, meta + 1, 1, (int)series_len));
-    unsigned nb = (unsigned)((series_len - first_valid + 255) / 256);
-    VTA_TRY(launch(ctx, fnf, dim3(nb), dim3(256), 0, d_x, (int)series_len, (int)first_valid,
-                   meta + 1));
+    if (d_warms) {
+        VTA_TRY(check_warms_device(ctx, d_periods, d_warms, (int)n_combos, (int)series_len, -1,
+                                   (int)max_period, meta));
+        hipFunction_t fnf;
+        VTA_TRY(get_function(ctx, "fir", "series_first_nonfinite_dfv", &fnf));
+        VTA_TRY(launch(ctx, fnf, dim3((unsigned)((series_len + 255) / 256)), dim3(256), 0, d_x,
+                       (int)series_len, (const int32_t *)meta, meta + 1));
+    } else {
+        hipFunction_t fnf;
+        VTA_TRY(get_function(ctx, "fir", "series_first_nonfinite", &fnf));
+        VTA_TRY(launch(ctx, fill, dim3(1), dim3(256), 0, meta, 1, (int)first_valid));
+        unsigned nb = (unsigned)((series_len - first_valid + 255) / 256);
+        VTA_TRY(launch(ctx, fnf, dim3(nb), dim3(256), 0, d_x, (int)series_len, (int)first_valid,
+                       meta + 1));
+    }
     return batch_launch(ctx, k, d_x, d_periods, meta, (size_t)series_len, (size_t)n_combos,
                         (size_t)max_period, d_out);
+}
+
+// sma_batch_dev_from_device (sma_wrapper.rs:830-860): device prices, host
+// combos, device-resident output.
+vta_status batch_dev_from_device(vta_ctx *ctx, Fir k, const float *d_x, size_t len,
+                                 const size_t *periods, size_t n_combos, size_t first_valid,
+                                 vta_device_array_f32 *out) {
+    clear_error();
+    if (!ctx || !d_x || !periods || !out) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    if (len == 0 || n_combos == 0 || first_valid >= len || len > (size_t)INT32_MAX ||
+        n_combos > (size_t)INT32_MAX)
+        return fail(VTA_ERR_INVALID_INPUT, "len and n_combos must be positive, first_valid < len");
+    size_t max_p = 0;
+    for (size_t i = 0; i < n_combos; ++i) {
+        VTA_TRY(check_period(k, periods[i], len, first_valid));
+        max_p = std::max(max_p, periods[i]);
+    }
+    size_t n_out;
+    if (!mul_ok(n_combos, len, &n_out) || !mul_ok(n_out, sizeof(float), &n_out))
+        return fail(VTA_ERR_INVALID_INPUT, "output bytes overflow");
+    VTA_TRY(check_fits(n_out));
+    std::vector<int32_t> per(periods, periods + n_combos);
+    DevBuf dp, dout;
+    VTA_TRY(upload(ctx, dp, per.data(), n_combos * sizeof(int32_t)));
+    VTA_TRY(dout.alloc(n_out));
+    VTA_TRY(batch_device(ctx, k, d_x, dp.as<int32_t>(), (int64_t)len, (int64_t)n_combos,
+                         (int64_t)first_valid, (int64_t)max_p, dout.as<float>()));
+    VTA_HIP(hipStreamSynchronize(ctx->stream));
+    out->ptr = static_cast<float *>(dout.release());
+    out->rows = n_combos;
+    out->cols = len;
+    return VTA_OK;
 }
 
 vta_status ms_dev(vta_ctx *ctx, Fir k, const float *x_tm, size_t cols, size_t rows,
@@ -262,16 +308,50 @@ vta_status ms_dev(vta_ctx *ctx, Fir k, const float *x_tm, size_t cols, size_t ro
     return VTA_OK;
 }
 
+// Many-series device form: the first non-finite bar of each series (from the
+// caller's first_valid) is found on the device, into the context's carry
+// buffer (stream-ordered with its other uses).
 vta_status ms_device(vta_ctx *ctx, Fir k, const float *d_x_tm, const int32_t *d_fvs,
-                     const int32_t *d_zs, size_t cols, size_t rows, size_t period,
-                     float *d_out_tm) {
+                     int64_t cols, int64_t rows, int64_t period, float *d_out_tm) {
     clear_error();
-    if (!ctx || !d_x_tm || !d_fvs || !d_out_tm || cols == 0 || rows == 0 ||
-        period < min_period(k) || period > rows)
+    if (!ctx || !d_x_tm || !d_fvs || !d_out_tm || cols <= 0 || rows <= 0 ||
+        period < (int64_t)min_period(k) || period > rows)
         return fail(VTA_ERR_INVALID_INPUT, "invalid arguments");
-    if (rows > (size_t)INT32_MAX || cols > (size_t)INT32_MAX)
+    if (rows > INT32_MAX || cols > INT32_MAX)
         return fail(VTA_ERR_INVALID_INPUT, "dimensions exceed int32");
-    return ms_launch(ctx, k, d_x_tm, d_fvs, d_zs, cols, rows, period, d_out_tm);
+    if ((size_t)cols * sizeof(int32_t) > vta_ctx::kCarryBytes)
+        return fail(VTA_ERR_INVALID_INPUT, "more than %zu series in one launch",
+                    vta_ctx::kCarryBytes / sizeof(int32_t));
+    int32_t *zs = static_cast<int32_t *>(ctx->carry);
+    hipFunction_t f;
+    VTA_TRY(get_function(ctx, "rec", "ms_first_nonfinite_from_first_valids", &f));
+    VTA_TRY(launch(ctx, f, dim3((unsigned)((cols + 63) / 64)), dim3(64), 0, d_x_tm, (int)rows,
+                   (int)cols, d_fvs, zs));
+    return ms_launch(ctx, k, d_x_tm, d_fvs, zs, (size_t)cols, (size_t)rows, (size_t)period,
+                     d_out_tm);
+}
+
+// sma_multi_series_one_param_time_major_dev_from_device (sma_wrapper.rs:982-1000).
+vta_status ms_dev_from_device(vta_ctx *ctx, Fir k, const float *d_x_tm, const int32_t *d_fvs,
+                              size_t cols, size_t rows, size_t period,
+                              vta_device_array_f32 *out) {
+    clear_error();
+    if (!out) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    size_t elems;
+    if (!mul_ok(cols, rows, &elems) || !mul_ok(elems, sizeof(float), &elems))
+        return fail(VTA_ERR_INVALID_INPUT, "cols*rows overflow");
+    if (cols > (size_t)INT32_MAX || rows > (size_t)INT32_MAX)
+        return fail(VTA_ERR_INVALID_INPUT, "dimensions exceed int32");
+    VTA_TRY(check_fits(elems));
+    DevBuf dout;
+    VTA_TRY(dout.alloc(std::max<size_t>(elems, 4)));
+    VTA_TRY(ms_device(ctx, k, d_x_tm, d_fvs, (int64_t)cols, (int64_t)rows, (int64_t)period,
+                      dout.as<float>()));
+    VTA_HIP(hipStreamSynchronize(ctx->stream));
+    out->ptr = static_cast<float *>(dout.release());
+    out->rows = rows;
+    out->cols = cols;
+    return VTA_OK;
 }
 
 vta_status expand_out(Fir k, const vta_range_usize *range, size_t *out, size_t cap,
@@ -286,6 +366,22 @@ vta_status expand_out(Fir k, const vta_range_usize *range, size_t *out, size_t c
 
 }  // namespace
 
+namespace vta {
+
+vta_status fir_ms_prepared(vta_ctx *ctx, int kind, const float *d_x_tm, const int32_t *d_fvs,
+                           const int32_t *d_zs, size_t cols, size_t rows, size_t period,
+                           float *d_out_tm) {
+    const Fir k = (Fir)kind;
+    if (!ctx || !d_x_tm || !d_fvs || !d_out_tm || cols == 0 || rows == 0 ||
+        period < min_period(k) || period > rows)
+        return fail(VTA_ERR_INVALID_INPUT, "invalid arguments");
+    if (rows > (size_t)INT32_MAX || cols > (size_t)INT32_MAX)
+        return fail(VTA_ERR_INVALID_INPUT, "dimensions exceed int32");
+    return ms_launch(ctx, k, d_x_tm, d_fvs, d_zs, cols, rows, period, d_out_tm);
+}
+
+}  // namespace vta
+
 extern "C" {
 
 #define VTA_FIR_ENTRY_POINTS(NAME, KIND)                                                        \
@@ -298,29 +394,87 @@ extern "C" {
                                       vta_device_array_f32 *out) {                            \
         return batch_dev(ctx, KIND, prices, len, range, out);                                 \
     }                                                                                         \
-    vta_status vta_##NAME##_batch_device(vta_ctx *ctx, const float *d_prices,                 \
-                                         const int32_t *d_periods, int32_t series_len,        \
-                                         int32_t n_combos, int32_t first_valid,               \
-                                         int32_t max_period, float *d_out) {                  \
-        return batch_device(ctx, KIND, d_prices, d_periods, series_len, n_combos,             \
-                            first_valid, max_period, d_out);                                  \
-    }                                                                                         \
     vta_status vta_##NAME##_many_series_one_param_time_major_dev(                             \
         vta_ctx *ctx, const float *data_tm, size_t cols, size_t rows, size_t period,          \
         vta_device_array_f32 *out) {                                                          \
         return ms_dev(ctx, KIND, data_tm, cols, rows, period, out);                           \
-    }                                                                                         \
-    vta_status vta_##NAME##_many_series_one_param_device(                                     \
-        vta_ctx *ctx, const float *d_prices_tm, const int32_t *d_first_valids,                \
-        const int32_t *d_first_nonfinite, size_t cols, size_t rows, size_t period,            \
-        float *d_out_tm) {                                                                    \
-        return ms_device(ctx, KIND, d_prices_tm, d_first_valids, d_first_nonfinite, cols,     \
-                         rows, period, d_out_tm);                                             \
     }
 
 VTA_FIR_ENTRY_POINTS(sma, FIR_SMA)
 VTA_FIR_ENTRY_POINTS(wma, FIR_WMA)
 VTA_FIR_ENTRY_POINTS(swma, FIR_SWMA)
 #undef VTA_FIR_ENTRY_POINTS
+
+// ---- SMA: enqueue-only forms (no reference counterpart) and the reference's
+// device-input forms
+vta_status vta_sma_batch_device(vta_ctx *ctx, const float *d_prices, const int32_t *d_periods,
+                                int32_t series_len, int32_t n_combos, int32_t first_valid,
+                                int32_t max_period, float *d_out) {
+    return batch_device(ctx, FIR_SMA, d_prices, d_periods, series_len, n_combos, first_valid,
+                        max_period, d_out);
+}
+vta_status vta_sma_many_series_one_param_device(vta_ctx *ctx, const float *d_prices_tm,
+                                                const int32_t *d_first_valids, size_t cols,
+                                                size_t rows, size_t period, float *d_out_tm) {
+    if (cols > (size_t)INT32_MAX || rows > (size_t)INT32_MAX || period > (size_t)INT32_MAX)
+        return fail(VTA_ERR_INVALID_INPUT, "dimensions exceed int32");
+    return ms_device(ctx, FIR_SMA, d_prices_tm, d_first_valids, (int64_t)cols, (int64_t)rows,
+                     (int64_t)period, d_out_tm);
+}
+vta_status vta_sma_batch_dev_from_device(vta_ctx *ctx, const float *d_prices, size_t len,
+                                         const size_t *periods, size_t n_combos,
+                                         size_t first_valid, vta_device_array_f32 *out) {
+    return batch_dev_from_device(ctx, FIR_SMA, d_prices, len, periods, n_combos, first_valid,
+                                 out);
+}
+vta_status vta_sma_multi_series_one_param_time_major_dev_from_device(
+    vta_ctx *ctx, const float *d_prices_tm, const int32_t *d_first_valids, size_t cols,
+    size_t rows, size_t period, vta_device_array_f32 *out) {
+    return ms_dev_from_device(ctx, FIR_SMA, d_prices_tm, d_first_valids, cols, rows, period,
+                              out);
+}
+
+// ---- WMA: wma_batch_device (wma_wrapper.rs:747-790),
+// wma_multi_series_one_param_time_major_device (:1049-1080)
+vta_status vta_wma_batch_device(vta_ctx *ctx, const float *d_prices, const int32_t *d_periods,
+                                int32_t series_len, int32_t n_combos, int32_t first_valid,
+                                int32_t max_period, float *d_out) {
+    return batch_device(ctx, FIR_WMA, d_prices, d_periods, series_len, n_combos, first_valid,
+                        max_period, d_out);
+}
+vta_status vta_wma_multi_series_one_param_time_major_device(vta_ctx *ctx,
+                                                            const float *d_prices_tm,
+                                                            int32_t period, int32_t num_series,
+                                                            int32_t series_len,
+                                                            const int32_t *d_first_valids,
+                                                            float *d_out_tm) {
+    return ms_device(ctx, FIR_WMA, d_prices_tm, d_first_valids, num_series, series_len, period,
+                     d_out_tm);
+}
+
+// ---- SWMA: swma_batch_device (swma_wrapper.rs:663-676): first_valid derived
+// from the warm indices on the device; swma_multi_series_one_param_device
+// (:787-830): the triangular weights are derived from the period (the
+// reference's host code uploads the same normalised weights), d_weights is
+// accepted for the binding and not read.
+vta_status vta_swma_batch_device(vta_ctx *ctx, const float *d_prices, const int32_t *d_periods,
+                                 const int32_t *d_warms, size_t series_len, size_t n_combos,
+                                 size_t max_period, float *d_out) {
+    if (!d_warms) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    if (series_len > (size_t)INT32_MAX || n_combos > (size_t)INT32_MAX ||
+        max_period > (size_t)INT32_MAX)
+        return fail(VTA_ERR_INVALID_INPUT, "series_len, n_combos or max_period out of range");
+    return batch_device(ctx, FIR_SWMA, d_prices, d_periods, (int64_t)series_len,
+                        (int64_t)n_combos, 0, (int64_t)max_period, d_out, d_warms);
+}
+vta_status vta_swma_multi_series_one_param_device(vta_ctx *ctx, const float *d_prices_tm,
+                                                  const float *d_weights, int32_t period,
+                                                  int32_t num_series, int32_t series_len,
+                                                  const int32_t *d_first_valids,
+                                                  float *d_out_tm) {
+    (void)d_weights;
+    return ms_device(ctx, FIR_SWMA, d_prices_tm, d_first_valids, num_series, series_len, period,
+                     d_out_tm);
+}
 
 }  // extern "C"

@@ -430,6 +430,16 @@ extern "C" __global__ __launch_bounds__(64) void fir_ms1p_swma_f32(fir_args a) {
 
 // Single series: the first bar at or after first_valid whose value is not
 // finite.  z[0] must hold T on entry (vta_fill_i32).
+// The same with first_valid read from the device (*fvp, in [0, T]): grid over
+// the whole series.
+extern "C" __global__ __launch_bounds__(256) void series_first_nonfinite_dfv(
+    const float *__restrict__ x, int T, const int32_t *__restrict__ fvp,
+    int32_t *__restrict__ z) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= T || t < fvp[0]) return;
+    if (!vta_isfinite(x[t])) atomicMin(z, t);
+}
+
 extern "C" __global__ __launch_bounds__(256) void series_first_nonfinite(
     const float *__restrict__ x, int T, int first_valid, int32_t *__restrict__ z) {
     const int t = blockIdx.x * 256 + threadIdx.x + first_valid;

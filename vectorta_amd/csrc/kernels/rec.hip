@@ -52,6 +52,8 @@ struct rec_args {
     float *out1;
     float *out2;
     double *carry;           // batch scan: [combo][chunk][states] chunk-end states
+    const float *alphas;     // EMA / Wilders: caller's smoothing factor (batch: per combo,
+                             // many-series: [0]); NULL derives it from the period
     int T;                   // bars
     int nlanes;              // combos (batch) or series (many-series)
     int chunk;               // bars per chunk
@@ -102,6 +104,7 @@ struct EmaEng {
         alpha = 2.0 / ((double)p + 1.0);
         beta = 1.0 - alpha;
     }
+    __device__ void set_alpha(double a_) { alpha = a_; beta = 1.0 - a_; }
     __device__ int kb() const { return burn_bars(beta, REC_NATS); }
     __device__ int exact_start() const { return fv; }
     __device__ int burn_ok_from() const { return fv + p; }   // b >= this may burn in
@@ -650,6 +653,7 @@ struct WildersEng {
         p = p_; fv = fv_;
         al = 1.0 / (double)p;
     }
+    __device__ void set_alpha(double a_) { al = a_; }
     __device__ int kb() const { return burn_bars(1.0 - al, REC_NATS); }
     __device__ int exact_start() const { return fv; }
     __device__ int burn_ok_from() const { return fv + p; }
@@ -689,6 +693,16 @@ struct WildersEng {
 
 // ---------------------------------------------------------------- chunk planning
 
+// A caller-supplied smoothing factor (the reference's d_alphas / alpha
+// arguments of the EMA and Wilders device forms) replaces the one the engine
+// derives from the period; the other engines have none.
+template <class E>
+__device__ __forceinline__ void apply_alpha(E &e, const float *alphas, int i) {
+    if constexpr (__is_same(E, EmaEng) || __is_same(E, WildersEng)) {
+        if (alphas) e.set_alpha((double)alphas[i]);
+    }
+}
+
 // Bar at which a lane starts (exact or burn-in) for chunk [c0, c1).
 template <class E>
 __device__ __forceinline__ int lane_start(const E &e, int c0, bool clean, bool *burn) {
@@ -724,6 +738,7 @@ __device__ __forceinline__ void drive_ms(const rec_args &a, int p, float *out) {
     const bool clean = active ? (a.clean != nullptr && a.clean[s] != 0) : true;
     E e;
     e.setup(p, fv);
+    apply_alpha(e, a.alphas, 0);
     bool burn = false;
     int start = active ? lane_start(e, c0, clean, &burn) : a.T;
     if (start > a.T) start = a.T;
@@ -989,6 +1004,7 @@ struct batch_view {
     const float *x, *h, *l;
     const int32_t *periods;
     double *carry;           // [combo][chunk][states]
+    const float *alphas;     // per combo, or NULL (see apply_alpha)
     int T, nlanes, chunk, nch, phase, fv, group, k;
     bool clean;
 };
@@ -1000,6 +1016,7 @@ __device__ __forceinline__ batch_view view_of(const rec_args &a) {
     v.x = a.x; v.h = a.h; v.l = a.l;
     v.periods = a.periods;
     v.carry = a.carry;
+    v.alphas = a.alphas;
     v.T = a.T;
     v.nlanes = a.nlanes;
     v.chunk = a.chunk;
@@ -1035,6 +1052,7 @@ __device__ __forceinline__ void drive_batch(const batch_view &a, float *out) {
     const int p = active ? a.periods[c] : 1;
     E e;
     e.setup(p, fv);
+    if (active) apply_alpha(e, a.alphas, c);
     bool burn = false, carried = false;
     int start;
     if (a.phase != 0) {
@@ -1285,6 +1303,7 @@ __device__ __forceinline__ void scan_combine(const batch_view &a) {
     if (c >= a.nlanes || !a.clean) return;
     E e;
     e.setup(a.periods[c], a.fv);
+    apply_alpha(e, a.alphas, c);
     constexpr int S = E::kScan > 0 ? E::kScan : 1;
     int lo = 0, hi = S;
     bool second = false;
@@ -1411,6 +1430,7 @@ __device__ __forceinline__ batch_view pack_view(const rec_pack_args &p, int j) {
     v.x = jb.x; v.h = jb.h; v.l = jb.l;
     v.periods = jb.periods;
     v.carry = jb.carry;
+    v.alphas = nullptr;
     v.T = jb.T;
     v.nlanes = jb.nlanes;
     v.chunk = jb.chunk;
@@ -1548,6 +1568,36 @@ extern "C" __global__ __launch_bounds__(64) void ms_prepare_meta(
     clean[s] = mode == 4 ? zf : (zf == T ? 1 : 0);
 }
 
+// Per series s of a time-major [T][N] matrix, from the caller's first_valid
+// (clamped into [0, T]): clean[s] = 1 when every bar from it on is finite
+// (the recurrences' device forms), or, with want_index, the first non-finite
+// bar (T if none; the finite-window averages' device forms).
+__device__ __forceinline__ void ms_scan_from_fv(const float *__restrict__ x, int T, int N,
+                                                const int32_t *__restrict__ fvs,
+                                                int32_t *__restrict__ out, bool want_index) {
+    const int s = blockIdx.x * 64 + (threadIdx.x & 63);
+    if (s >= N) return;
+    const int f = min(max((int)fvs[s], 0), T);
+    int zf = T;
+    for (int t = f; t < T; ++t) {
+        if (!vta_isfinite(x[(size_t)t * N + s])) {
+            zf = t;
+            break;
+        }
+    }
+    out[s] = want_index ? zf : (zf == T ? 1 : 0);
+}
+extern "C" __global__ __launch_bounds__(64) void ms_clean_from_first_valids(
+    const float *__restrict__ x, int T, int N, const int32_t *__restrict__ fvs,
+    int32_t *__restrict__ clean) {
+    ms_scan_from_fv(x, T, N, fvs, clean, false);
+}
+extern "C" __global__ __launch_bounds__(64) void ms_first_nonfinite_from_first_valids(
+    const float *__restrict__ x, int T, int N, const int32_t *__restrict__ fvs,
+    int32_t *__restrict__ z) {
+    ms_scan_from_fv(x, T, N, fvs, z, true);
+}
+
 // Single series: clean flag = no non-finite input in [first_valid, T).
 // flag[0] must be 0 on entry (see vta_fill_i32); every failing lane stores 1.
 extern "C" __global__ __launch_bounds__(256) void series_nonfinite_flag(
@@ -1565,6 +1615,29 @@ extern "C" __global__ __launch_bounds__(256) void series_nonfinite_flag(
 extern "C" __global__ __launch_bounds__(64) void flag_to_clean(const int32_t *__restrict__ flag,
                                                                int32_t *__restrict__ clean) {
     if (threadIdx.x == 0) clean[0] = flag[0] == 0 ? 1 : 0;
+}
+
+// The reference's device forms of SMMA, SWMA and Wilders take per-combo warm
+// indices that its host code sets to first_valid + period - 1; the kernels
+// here derive the warm-up from the period, so the indices are checked: a
+// mismatch stores 1 in *err (the context reports it at its next synchronize).
+// first_valid < 0 derives first_valid from combo 0 (SWMA passes none) and
+// stores it in *fv_out; a derived value outside [0, T - max_period] is an
+// error too, and T (no output bar) is stored instead.
+extern "C" __global__ __launch_bounds__(256) void vta_check_warms(
+    const int32_t *__restrict__ periods, const int32_t *__restrict__ warms, int n, int T,
+    int first_valid, int max_period, int32_t *__restrict__ fv_out, int32_t *__restrict__ err) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    long long fv = first_valid;
+    bool bad = false;
+    if (first_valid < 0) {
+        fv = (long long)warms[0] - (long long)periods[0] + 1;
+        const bool ok = fv >= 0 && fv <= (long long)T - (long long)max_period;
+        if (c == 0 && fv_out) *fv_out = ok ? (int32_t)fv : T;
+        bad = !ok;
+    }
+    if (c < n && (long long)warms[c] != fv + (long long)periods[c] - 1) bad = true;
+    if (bad) err[0] = 1;
 }
 
 extern "C" __global__ __launch_bounds__(256) void vta_fill_i32(int32_t *__restrict__ p, int n,

@@ -28,6 +28,7 @@ struct rec_args_host {
     float *out1;
     float *out2;
     double *carry;
+    const float *alphas;
     int T;
     int nlanes;
     int chunk;
@@ -35,7 +36,7 @@ struct rec_args_host {
     int phase;
     int pad_;
 };
-static_assert(sizeof(rec_args_host) == 120, "rec_args layout");
+static_assert(sizeof(rec_args_host) == 128, "rec_args layout");
 
 enum Ind {
     IND_EMA = 0,
@@ -169,9 +170,10 @@ vta_status launch_rec(vta_ctx *ctx, const char *kname, rec_args_host &a, unsigne
 }
 
 // Chunked-scan launch: chunks sized for ~kScanWaves waves, at least 64 bars
-// and at most 1,024 chunks (the fold walks them 64 at a time), for Wilders
-// also at least the longest warm-up (which a lane then runs exactly over at
-// most two chunks), and as many as the context's carry buffer holds for C
+// and at most 1,024 chunks (the fold walks them 64 at a time; a Wilders lane
+// whose warm-up spans several chunks replays it exactly from first_valid in
+// each, which only the first chunks of the series pay), and as many as the
+// context's carry buffer holds for C
 // combos; pass 1, the per-combo fold (one wave per combo; two rounds for
 // ADX), pass 3.  Small requests are latency-bound, so their chunk is short:
 // 16 one-series requests of 10,000 bars x 32 periods (EMA, RSI, ADX, SMA,
@@ -207,8 +209,7 @@ size_t scan_chunk(const vta_ctx *ctx, Ind k, size_t T, size_t C, size_t max_p) {
         if (v >= 64) waves = (size_t)v;
     }
     const size_t want = std::max<size_t>(1, waves / groups);
-    const size_t warm = k == IND_WILDERS ? max_p : 0;
-    size_t chunk = std::max<size_t>({64, (T + want - 1) / want, (T + 1023) / 1024, warm});
+    size_t chunk = std::max<size_t>({64, (T + want - 1) / want, (T + 1023) / 1024});
     if (const char *e = getenv("VTA_SCAN_CHUNK")) {
         long v = atol(e);
         if (v >= 64) chunk = (size_t)v;
@@ -248,11 +249,35 @@ vta_status scan_launch(vta_ctx *ctx, Ind k, rec_args_host a, size_t T, size_t C,
     return VTA_OK;
 }
 
+// Index of the indicators with a kernel policy (CudaEmaPolicy, CudaRsiPolicy,
+// CudaAdxPolicy), -1 for the others.
+int policy_slot(Ind k) {
+    return k == IND_EMA ? VTA_POLICY_SLOT_EMA
+           : k == IND_RSI ? VTA_POLICY_SLOT_RSI
+           : k == IND_ADX ? VTA_POLICY_SLOT_ADX
+                          : -1;
+}
+
+bool plain_batch(const vta_ctx *ctx, Ind k) {
+    const int s = policy_slot(k);
+    return s >= 0 && ctx->policy[s].batch == VTA_KERNEL_PLAIN;
+}
+
+void record_batch(vta_ctx *ctx, Ind k, bool chunked) {
+    const int s = policy_slot(k);
+    if (s >= 0) ctx->sel_batch[s] = {chunked ? VTA_SELECTED_CHUNKED : VTA_SELECTED_PLAIN, 64};
+}
+
 // One series x many periods, device input and output, fv and clean known.
+// max_p = 0: the longest period is not known on the host (the reference's
+// device forms pass none), so no burn-in is planned from it: the series runs
+// as a chunked scan (Wilders chunks then are not stretched to the longest
+// warm-up), or else exactly from first_valid in one chunk.  d_alphas
+// (per combo, may be NULL) overrides the EMA / Wilders smoothing factor.
 vta_status batch_launch(vta_ctx *ctx, Ind k, const float *d_x, const float *d_h,
                         const float *d_l, const int32_t *d_periods, size_t max_p, size_t T,
                         size_t C, const int32_t *d_fv1, const int32_t *d_clean1, bool clean,
-                        float *d_out) {
+                        float *d_out, const float *d_alphas = nullptr) {
     rec_args_host a{};
     a.x = d_x;
     a.h = d_h;
@@ -261,16 +286,27 @@ vta_status batch_launch(vta_ctx *ctx, Ind k, const float *d_x, const float *d_h,
     a.fvs = d_fv1;
     a.clean = d_clean1;
     a.out0 = d_out;
+    a.alphas = d_alphas;
     a.T = (int)T;
     a.nlanes = (int)C;
-    if (clean) {
+    const bool plain = plain_batch(ctx, k);
+    // (a Wilders warm-up longer than a chunk is still exact: its lanes replay it
+    // from first_valid, see test_scan_short_chunks_long_lead)
+    if (clean && !plain) {
         bool done;
         VTA_TRY(scan_launch(ctx, k, a, T, C, max_p, &done));
-        if (done) return VTA_OK;
+        if (done) {
+            record_batch(ctx, k, true);
+            return VTA_OK;
+        }
     }
     unsigned groups = (unsigned)((C + 63) / 64);
-    a.chunk = pick_chunk_batch(k, max_p, T, groups, clean);
+    // a caller-supplied smoothing factor is not the one the burn-in length
+    // would be planned from, so those batches also run in one chunk
+    a.chunk = (plain || max_p == 0 || d_alphas) ? (int)T
+                                                : pick_chunk_batch(k, max_p, T, groups, clean);
     unsigned chunks = (unsigned)((T + a.chunk - 1) / a.chunk);
+    record_batch(ctx, k, chunks > 1);
     return launch_rec(ctx, batch_kernel(k), a, groups, chunks, 64, kBatchStageBytes);
 }
 
@@ -324,18 +360,26 @@ vta_status batch_dev(vta_ctx *ctx, Ind k, const float *x, const float *h, const 
 }
 
 // Device-resident one-series batch: the clean flag is computed on the device
-// into the context scratch.
+// into the context scratch.  max_period = 0 when the caller's signature has
+// none (see batch_launch); d_alphas and d_warms (per combo, may be NULL) are
+// the reference's EMA / SMMA / Wilders device arguments: the alphas are used,
+// the warm indices checked on the device against first_valid + period - 1.
 vta_status batch_device(vta_ctx *ctx, Ind k, const float *d_x, const float *d_h,
-                        const float *d_l, const int32_t *d_periods, int32_t max_period,
-                        int32_t series_len, int32_t first_valid, int32_t n_combos,
-                        float *d_out) {
+                        const float *d_l, const int32_t *d_periods, int64_t max_period,
+                        int64_t series_len, int64_t first_valid, int64_t n_combos,
+                        float *d_out, const float *d_alphas = nullptr,
+                        const int32_t *d_warms = nullptr) {
     clear_error();
     if (!ctx || !d_x || !d_periods || !d_out || (hlc(k) && (!d_h || !d_l)))
         return fail(VTA_ERR_INVALID_INPUT, "null argument");
-    if (series_len <= 0 || n_combos <= 0 || max_period <= 0 || first_valid < 0 ||
+    if (series_len <= 0 || n_combos <= 0 || max_period < 0 || first_valid < 0 ||
         first_valid >= series_len)
-        return fail(VTA_ERR_INVALID_INPUT, "series_len, n_combos and max_period must be positive "
+        return fail(VTA_ERR_INVALID_INPUT, "series_len and n_combos must be positive "
                                            "and first_valid < series_len");
+    if (series_len > INT32_MAX || n_combos > INT32_MAX || max_period > series_len)
+        return fail(VTA_ERR_INVALID_INPUT, "series_len, n_combos or max_period out of range");
+    if (d_warms) VTA_TRY(check_warms_device(ctx, d_periods, d_warms, (int)n_combos,
+                                            (int)series_len, (int)first_valid, 0, nullptr));
     // context scratch: [0] first_valid, [2] non-finite flag, [3] clean word
     int32_t *meta = static_cast<int32_t *>(ctx->scratch);
     hipFunction_t fill, flag, inv;
@@ -351,17 +395,29 @@ vta_status batch_device(vta_ctx *ctx, Ind k, const float *d_x, const float *d_h,
     // Chunked with burn-in: a series found unclean still runs correctly, since
     // every lane of an unclean stream starts exactly at first_valid.
     return batch_launch(ctx, k, d_x, d_h, d_l, d_periods, (size_t)max_period,
-                        (size_t)series_len, (size_t)n_combos, meta, meta + 3, true, d_out);
+                        (size_t)series_len, (size_t)n_combos, meta, meta + 3, true, d_out,
+                        d_alphas);
 }
 
 vta_status ms_launch(vta_ctx *ctx, Ind k, const float *d_x, const float *d_h, const float *d_l,
                      size_t cols, size_t rows, size_t period, const int32_t *d_fvs,
-                     const int32_t *d_clean, float *d_out) {
+                     const int32_t *d_clean, float *d_out, const float *alpha = nullptr) {
     int32_t *pp = static_cast<int32_t *>(ctx->scratch) + 64;
     hipFunction_t fill;
     VTA_TRY(get_function(ctx, "rec", "vta_fill_i32", &fill));
     VTA_TRY(launch(ctx, fill, dim3(1), dim3(256), 0, pp, 1, (int)period));
+    const float *d_alpha = nullptr;
+    if (alpha) {
+        // the caller's smoothing factor, as one f32 word after the period
+        int32_t bits;
+        memcpy(&bits, alpha, sizeof(bits));
+        VTA_TRY(launch(ctx, fill, dim3(1), dim3(256), 0, pp + 1, 1, (int)bits));
+        d_alpha = reinterpret_cast<const float *>(pp + 1);
+    }
+    const int slot = policy_slot(k);
+    const bool plain = slot >= 0 && ctx->policy[slot].many_series == VTA_KERNEL_PLAIN;
     rec_args_host a{};
+    a.alphas = d_alpha;
     a.x = d_x;
     a.h = d_h;
     a.l = d_l;
@@ -373,9 +429,26 @@ vta_status ms_launch(vta_ctx *ctx, Ind k, const float *d_x, const float *d_h, co
     a.nlanes = (int)cols;
     unsigned groups = (unsigned)((cols + 63) / 64);
     a.chunk = pick_chunk(ctx, k, period, rows, groups, d_clean != nullptr);
-    // without clean flags every series runs exactly from first_valid in one chunk
-    if (!d_clean) a.chunk = (int)rows;
+    if (alpha && d_clean && !plain) {
+        // burn-in planned from the caller's smoothing factor (the engines'
+        // own burn-in follows the factor they run with)
+        const double beta = 1.0 - (double)*alpha;
+        if (!(beta > 0.0 && beta < 1.0)) {
+            a.chunk = (int)rows;
+        } else if (!ms_chunk_override(ctx, rows)) {
+            const double kb = std::ceil(20.0 / -std::log(beta));
+            const size_t want = std::max<size_t>(1, 2048 / std::max<size_t>(groups, 1));
+            size_t c = std::max<size_t>({(size_t)std::min(4.0 * kb, 1e9), 1024,
+                                         (rows + want - 1) / want});
+            a.chunk = (int)std::min(((c + 63) & ~(size_t)63), rows);
+        }
+    }
+    // without clean flags, or under the plain policy, every series runs
+    // exactly from first_valid in one chunk
+    if (!d_clean || plain) a.chunk = (int)rows;
     unsigned chunks = (unsigned)((rows + a.chunk - 1) / a.chunk);
+    if (slot >= 0)
+        ctx->sel_ms[slot] = {chunks > 1 ? VTA_SELECTED_CHUNKED : VTA_SELECTED_PLAIN, 64};
     return launch_rec(ctx, ms_kernel(k), a, groups, chunks);
 }
 
@@ -442,6 +515,45 @@ vta_status ms_dev(vta_ctx *ctx, Ind k, const float *x_tm, const float *h_tm, con
     return VTA_OK;
 }
 
+// *_batch_into_host_f32 (ema_wrapper.rs:309-323, adx_wrapper.rs:253-280): the
+// batch into a caller-owned host slice of combos x len floats, [combo][t].
+vta_status batch_into_host(vta_ctx *ctx, Ind k, const float *x, const float *h, const float *l,
+                           size_t len, const vta_range_usize *range, float *out,
+                           size_t out_len) {
+    clear_error();
+    if (!ctx || !out) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    std::vector<size_t> ps;
+    VTA_TRY(expand_periods(k, range, ps));
+    size_t expected;
+    if (!mul_ok(ps.size(), len, &expected))
+        return fail(VTA_ERR_INVALID_INPUT, "combos*series_len overflow");
+    if (out_len != expected)
+        return fail(VTA_ERR_INVALID_INPUT, "output slice wrong length: got %zu, need %zu",
+                    out_len, expected);
+    vta_device_array_f32 arr{};
+    VTA_TRY(batch_dev(ctx, k, x, h, l, len, range, &arr));
+    vta_status st = vta_device_array_copy_to_host(ctx, &arr, out, out_len);
+    vta_device_array_free(ctx, &arr);
+    return st;
+}
+
+// *_many_series_one_param_time_major_into_host_f32 (ema_wrapper.rs:422-441,
+// adx_wrapper.rs:417-440): [rows][cols] into a caller-owned host slice.
+vta_status ms_into_host(vta_ctx *ctx, Ind k, const float *x_tm, const float *h_tm,
+                        const float *l_tm, size_t cols, size_t rows, size_t period,
+                        float *out_tm, size_t out_len) {
+    clear_error();
+    if (!ctx || !out_tm) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    size_t expected;
+    if (!mul_ok(cols, rows, &expected)) return fail(VTA_ERR_INVALID_INPUT, "rows*cols overflow");
+    if (out_len != expected) return fail(VTA_ERR_INVALID_INPUT, "out slice wrong length");
+    vta_device_array_f32 arr{};
+    VTA_TRY(ms_dev(ctx, k, x_tm, h_tm, l_tm, cols, rows, period, &arr));
+    vta_status st = vta_device_array_copy_to_host(ctx, &arr, out_tm, out_len);
+    vta_device_array_free(ctx, &arr);
+    return st;
+}
+
 vta_status expand_out(Ind k, const vta_range_usize *range, size_t *out, size_t cap,
                       size_t *n_out) {
     clear_error();
@@ -450,6 +562,34 @@ vta_status expand_out(Ind k, const vta_range_usize *range, size_t *out, size_t c
     if (n_out) *n_out = ps.size();
     if (out) memcpy(out, ps.data(), std::min(cap, ps.size()) * sizeof(size_t));
     return VTA_OK;
+}
+
+// Many-series device forms of EMA, SMMA and Wilders: per-series clean flags
+// against the caller's first_valids computed on the device (into the
+// context's carry buffer) so clean series run in time chunks.
+vta_status ms_device_one_input(vta_ctx *ctx, Ind k, const float *d_prices_tm,
+                               const int32_t *d_first_valids, int64_t period,
+                               int64_t num_series, int64_t series_len, float *d_out_tm,
+                               const float *alpha) {
+    clear_error();
+    if (!ctx || !d_prices_tm || !d_first_valids || !d_out_tm || period <= 0 ||
+        num_series <= 0 || series_len <= 0 || period > series_len)
+        return fail(VTA_ERR_INVALID_INPUT,
+                    "period, num_series and series_len must be positive, period <= series_len");
+    if (num_series > INT32_MAX || series_len > INT32_MAX)
+        return fail(VTA_ERR_INVALID_INPUT, "dimensions exceed int32");
+    const int32_t *d_clean = nullptr;
+    if ((size_t)num_series * sizeof(int32_t) <= vta_ctx::kCarryBytes) {
+        int32_t *cl = static_cast<int32_t *>(ctx->carry);
+        hipFunction_t f;
+        VTA_TRY(get_function(ctx, "rec", "ms_clean_from_first_valids", &f));
+        VTA_TRY(launch(ctx, f, dim3((unsigned)((num_series + 63) / 64)), dim3(64), 0,
+                       d_prices_tm, (int)series_len, (int)num_series, d_first_valids, cl));
+        d_clean = cl;
+    }
+    return ms_launch(ctx, k, d_prices_tm, nullptr, nullptr, (size_t)num_series,
+                     (size_t)series_len, (size_t)period, d_first_valids, d_clean, d_out_tm,
+                     alpha);
 }
 
 }  // namespace
@@ -489,32 +629,60 @@ vta_status vta_atr_batch_device_with_tr(vta_ctx *ctx, const float *d_tr, const i
                                       const vta_range_usize *range, vta_device_array_f32 *out) { \
         return batch_dev(ctx, IND, prices, nullptr, nullptr, len, range, out);                  \
     }                                                                                           \
-    vta_status vta_##NAME##_batch_device(vta_ctx *ctx, const float *d_prices,                   \
-                                         const int32_t *d_periods, int32_t max_period,          \
-                                         int32_t series_len, int32_t first_valid,               \
-                                         int32_t n_combos, float *d_out) {                      \
-        return batch_device(ctx, IND, d_prices, nullptr, nullptr, d_periods, max_period,        \
-                            series_len, first_valid, n_combos, d_out);                          \
-    }                                                                                           \
     vta_status vta_##NAME##_many_series_one_param_time_major_dev(                               \
         vta_ctx *ctx, const float *data_tm, size_t cols, size_t rows, size_t period,            \
         vta_device_array_f32 *out) {                                                            \
         return ms_dev(ctx, IND, data_tm, nullptr, nullptr, cols, rows, period, out);            \
-    }                                                                                           \
-    vta_status vta_##NAME##_many_series_one_param_device(                                       \
-        vta_ctx *ctx, const float *d_prices_tm, const int32_t *d_first_valids, size_t period,   \
-        size_t num_series, size_t series_len, const int32_t *d_clean, float *d_out_tm) {        \
-        clear_error();                                                                          \
-        if (!ctx || !d_prices_tm || !d_first_valids || !d_out_tm || period == 0 ||              \
-            num_series == 0 || series_len == 0 || period > series_len)                          \
-            return fail(VTA_ERR_INVALID_INPUT, "invalid arguments");                            \
-        return ms_launch(ctx, IND, d_prices_tm, nullptr, nullptr, num_series, series_len,       \
-                         period, d_first_valids, d_clean, d_out_tm);                            \
     }
 
 VTA_ONE_INPUT_REC(smma, IND_SMMA)
 VTA_ONE_INPUT_REC(wilders, IND_WILDERS)
 #undef VTA_ONE_INPUT_REC
+
+// smma_batch_device (smma_wrapper.rs:377-396): the warm indices are checked
+// against first_valid + period - 1 on the device (NULL: no check).
+vta_status vta_smma_batch_device(vta_ctx *ctx, const float *d_prices, const int32_t *d_periods,
+                                 const int32_t *d_warms, size_t first_valid, size_t series_len,
+                                 size_t n_combos, float *d_out) {
+    if (series_len > (size_t)INT32_MAX || n_combos > (size_t)INT32_MAX)
+        return fail(VTA_ERR_INVALID_INPUT, "series_len or n_combos out of range");
+    return batch_device(ctx, IND_SMMA, d_prices, nullptr, nullptr, d_periods, 0,
+                        (int64_t)series_len, (int64_t)first_valid, (int64_t)n_combos, d_out,
+                        nullptr, d_warms);
+}
+
+// wilders_batch_device (wilders_wrapper.rs:252-330): the alphas are used (NULL:
+// 1/p in f64), the warm indices checked against first_valid + period - 1 on
+// the device (NULL: no check).
+vta_status vta_wilders_batch_device(vta_ctx *ctx, const float *d_prices,
+                                    const int32_t *d_periods, const float *d_alphas,
+                                    const int32_t *d_warm, int32_t series_len,
+                                    int32_t first_valid, int32_t n_combos, float *d_out) {
+    return batch_device(ctx, IND_WILDERS, d_prices, nullptr, nullptr, d_periods, 0, series_len,
+                        first_valid, n_combos, d_out, d_alphas, d_warm);
+}
+
+// smma_multi_series_one_param_device (smma_wrapper.rs:599-621).
+vta_status vta_smma_multi_series_one_param_device(vta_ctx *ctx, const float *d_prices_tm,
+                                                  int32_t period, int32_t num_series,
+                                                  int32_t series_len,
+                                                  const int32_t *d_first_valids,
+                                                  float *d_out_tm) {
+    return ms_device_one_input(ctx, IND_SMMA, d_prices_tm, d_first_valids, period, num_series,
+                               series_len, d_out_tm, nullptr);
+}
+
+// wilders_many_series_one_param_device (wilders_wrapper.rs:662-700): alpha is used.
+vta_status vta_wilders_many_series_one_param_device(vta_ctx *ctx, const float *d_prices_tm,
+                                                    const int32_t *d_first_valids,
+                                                    int32_t period, float alpha,
+                                                    size_t num_series, size_t series_len,
+                                                    float *d_out_tm) {
+    if (num_series > (size_t)INT32_MAX || series_len > (size_t)INT32_MAX)
+        return fail(VTA_ERR_INVALID_INPUT, "dimensions exceed int32");
+    return ms_device_one_input(ctx, IND_WILDERS, d_prices_tm, d_first_valids, period,
+                               (int64_t)num_series, (int64_t)series_len, d_out_tm, &alpha);
+}
 
 // ------------------------------------------------------------------ EMA
 vta_status vta_ema_expand_grid(const vta_range_usize *range, size_t *periods, size_t cap,
@@ -526,27 +694,38 @@ vta_status vta_ema_batch_dev(vta_ctx *ctx, const float *prices, size_t len,
     return batch_dev(ctx, IND_EMA, prices, nullptr, nullptr, len, range, out);
 }
 vta_status vta_ema_batch_device(vta_ctx *ctx, const float *d_prices, const int32_t *d_periods,
-                                const float *d_alphas, int32_t max_period, int32_t series_len,
-                                int32_t first_valid, int32_t n_combos, float *d_out) {
-    (void)d_alphas;  // alpha = 2/(p+1) is recomputed in f64 from d_periods
-    return batch_device(ctx, IND_EMA, d_prices, nullptr, nullptr, d_periods, max_period,
-                        series_len, first_valid, n_combos, d_out);
+                                const float *d_alphas, size_t series_len, size_t first_valid,
+                                size_t n_combos, float *d_out) {
+    if (series_len > (size_t)INT32_MAX || n_combos > (size_t)INT32_MAX)
+        return fail(VTA_ERR_INVALID_INPUT, "series_len or n_combos out of range");
+    return batch_device(ctx, IND_EMA, d_prices, nullptr, nullptr, d_periods, 0,
+                        (int64_t)series_len, (int64_t)first_valid, (int64_t)n_combos, d_out,
+                        d_alphas);
+}
+vta_status vta_ema_batch_into_host_f32(vta_ctx *ctx, const float *prices, size_t len,
+                                       const vta_range_usize *range, float *out,
+                                       size_t out_len) {
+    return batch_into_host(ctx, IND_EMA, prices, nullptr, nullptr, len, range, out, out_len);
 }
 vta_status vta_ema_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *data_tm,
                                                         size_t cols, size_t rows, size_t period,
                                                         vta_device_array_f32 *out) {
     return ms_dev(ctx, IND_EMA, data_tm, nullptr, nullptr, cols, rows, period, out);
 }
+vta_status vta_ema_many_series_one_param_time_major_into_host_f32(
+    vta_ctx *ctx, const float *data_tm, size_t cols, size_t rows, size_t period, float *out_tm,
+    size_t out_len) {
+    return ms_into_host(ctx, IND_EMA, data_tm, nullptr, nullptr, cols, rows, period, out_tm,
+                        out_len);
+}
 vta_status vta_ema_many_series_one_param_device(vta_ctx *ctx, const float *d_prices_tm,
-                                                const int32_t *d_first_valids, size_t period,
-                                                size_t num_series, size_t series_len,
-                                                const int32_t *d_clean, float *d_out_tm) {
-    clear_error();
-    if (!ctx || !d_prices_tm || !d_first_valids || !d_out_tm || period == 0 || num_series == 0 ||
-        series_len == 0)
-        return fail(VTA_ERR_INVALID_INPUT, "invalid arguments");
-    return ms_launch(ctx, IND_EMA, d_prices_tm, nullptr, nullptr, num_series, series_len, period,
-                     d_first_valids, d_clean, d_out_tm);
+                                                const int32_t *d_first_valids, int32_t period,
+                                                float alpha, size_t num_series,
+                                                size_t series_len, float *d_out_tm) {
+    if (num_series > (size_t)INT32_MAX || series_len > (size_t)INT32_MAX)
+        return fail(VTA_ERR_INVALID_INPUT, "dimensions exceed int32");
+    return ms_device_one_input(ctx, IND_EMA, d_prices_tm, d_first_valids, period,
+                               (int64_t)num_series, (int64_t)series_len, d_out_tm, &alpha);
 }
 
 // ------------------------------------------------------------------ RSI
@@ -641,6 +820,50 @@ vta_status vta_adx_many_series_one_param_time_major_dev(vta_ctx *ctx, const floa
                                                         size_t rows, size_t period,
                                                         vta_device_array_f32 *out) {
     return ms_dev(ctx, IND_ADX, close_tm, high_tm, low_tm, cols, rows, period, out);
+}
+vta_status vta_adx_batch_into_host_f32(vta_ctx *ctx, const float *high, const float *low,
+                                       const float *close, size_t len,
+                                       const vta_range_usize *range, float *out,
+                                       size_t out_len) {
+    return batch_into_host(ctx, IND_ADX, close, high, low, len, range, out, out_len);
+}
+vta_status vta_adx_many_series_one_param_time_major_into_host_f32(
+    vta_ctx *ctx, const float *high_tm, const float *low_tm, const float *close_tm, size_t cols,
+    size_t rows, size_t period, float *out_tm, size_t out_len) {
+    return ms_into_host(ctx, IND_ADX, close_tm, high_tm, low_tm, cols, rows, period, out_tm,
+                        out_len);
+}
+
+// ------------------------------------------------------------------ prepared many-series
+vta_status vta_ma_many_series_prepared_device(vta_ctx *ctx, vta_ma_kind kind,
+                                              const float *d_prices_tm,
+                                              const int32_t *d_first_valids,
+                                              const int32_t *d_meta, size_t period, float alpha,
+                                              size_t num_series, size_t series_len,
+                                              float *d_out_tm) {
+    clear_error();
+    if (!ctx || !d_prices_tm || !d_first_valids || !d_out_tm || period == 0 || num_series == 0 ||
+        series_len == 0 || period > series_len)
+        return fail(VTA_ERR_INVALID_INPUT, "invalid arguments");
+    if (num_series > (size_t)INT32_MAX || series_len > (size_t)INT32_MAX)
+        return fail(VTA_ERR_INVALID_INPUT, "dimensions exceed int32");
+    switch (kind) {
+        case VTA_MA_EMA:
+        case VTA_MA_SMMA:
+        case VTA_MA_WILDERS: {
+            const Ind k = kind == VTA_MA_EMA ? IND_EMA : kind == VTA_MA_SMMA ? IND_SMMA
+                                                                             : IND_WILDERS;
+            const bool own = alpha > 0.0f && k != IND_SMMA;
+            return ms_launch(ctx, k, d_prices_tm, nullptr, nullptr, num_series, series_len,
+                             period, d_first_valids, d_meta, d_out_tm, own ? &alpha : nullptr);
+        }
+        case VTA_MA_SMA:
+        case VTA_MA_WMA:
+        case VTA_MA_SWMA:
+            return fir_ms_prepared(ctx, (int)kind - (int)VTA_MA_SMA, d_prices_tm, d_first_valids,
+                                   d_meta, num_series, series_len, period, d_out_tm);
+    }
+    return fail(VTA_ERR_INVALID_INPUT, "unknown moving-average kind %d", (int)kind);
 }
 
 // ------------------------------------------------------------------ shared prep

@@ -86,7 +86,17 @@ vta_status roll_launch(vta_ctx *ctx, RInd k, bool batch, roll_args_host &a, size
     size_t smem = (k == R_BB) ? 0
                               : (size_t)64 * (max_w + 1) * sizeof(float) +
                                     (batch ? (size_t)64 * 33 * sizeof(float) : 0);
+    if (k == R_PNR) {
+        // CudaPnrPolicy: the plain form runs each lane over the whole series
+        const vta_kernel_policy &pol = ctx->policy[VTA_POLICY_SLOT_PNR];
+        if ((batch ? pol.batch : pol.many_series) == VTA_KERNEL_PLAIN) chunked = false;
+    }
     a.chunk = roll_chunk(ctx, (size_t)a.T, (size_t)a.nlanes, max_w, chunked, smem);
+    if (k == R_PNR) {
+        const vta_kernel_selected sel = {a.chunk < a.T ? VTA_SELECTED_CHUNKED : VTA_SELECTED_PLAIN,
+                                         64};
+        (batch ? ctx->sel_batch : ctx->sel_ms)[VTA_POLICY_SLOT_PNR] = sel;
+    }
     a.wcap = (int)max_w;
     unsigned groups = (unsigned)((a.nlanes + 63) / 64);
     unsigned chunks = (unsigned)((a.T + a.chunk - 1) / a.chunk);

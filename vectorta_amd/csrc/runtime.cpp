@@ -286,6 +286,14 @@ vta_status vta_ctx_create(int device_id, vta_ctx **out) {
         delete ctx;
         return fail(VTA_ERR_OUT_OF_MEMORY, "hipMalloc of the context scratch failed");
     }
+    // zeroed once: the argument-error word must start clear
+    if (hipMemset(ctx->scratch, 0, vta_ctx::kScratchBytes) != hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess) {
+        (void)hipFree(ctx->scratch);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return fail(VTA_ERR_HIP, "clearing the context scratch failed");
+    }
     if (hipMalloc(&ctx->carry, vta_ctx::kCarryBytes) != hipSuccess) {
         (void)hipFree(ctx->scratch);
         (void)hipStreamDestroy(ctx->stream);
@@ -314,6 +322,25 @@ vta_status vta_ctx_destroy(vta_ctx *ctx) {
 vta_status vta_ctx_synchronize(vta_ctx *ctx) {
     if (!ctx) return fail(VTA_ERR_INVALID_INPUT, "ctx is null");
     VTA_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->arg_check_pending) {
+        // device-side argument checks of enqueue-only calls since the last
+        // synchronize (the reference's warm-index buffers)
+        ctx->arg_check_pending = false;
+        int32_t *err = static_cast<int32_t *>(ctx->scratch) + kArgErrorWord;
+        int32_t h = 0;
+        VTA_HIP(hipMemcpyAsync(&h, err, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+        VTA_HIP(hipStreamSynchronize(ctx->stream));
+        if (h != 0) {
+            hipFunction_t fill;
+            VTA_TRY(get_function(ctx, "rec", "vta_fill_i32", &fill));
+            VTA_TRY(launch(ctx, fill, dim3(1), dim3(256), 0, err, 1, 0));
+            VTA_HIP(hipStreamSynchronize(ctx->stream));
+            return fail(VTA_ERR_INVALID_INPUT,
+                        "invalid input in an enqueue-only call since the last synchronize: "
+                        "a warm index differs from first_valid + period - 1 (or implies a "
+                        "first_valid outside the series); that call's outputs are undefined");
+        }
+    }
     return VTA_OK;
 }
 
@@ -362,11 +389,15 @@ vta_status vta_ctx_preload(vta_ctx *ctx) {
         {"fir", "fir_ms1p_wma_f32"},
         {"fir", "fir_ms1p_swma_f32"},
         {"fir", "series_first_nonfinite"},
+        {"fir", "series_first_nonfinite_dfv"},
         {"rec", "ms_ema_rsi_atr_f32"},
         {"rec", "ms_prepare_meta"},
         {"rec", "series_nonfinite_flag"},
         {"rec", "flag_to_clean"},
         {"rec", "vta_fill_i32"},
+        {"rec", "vta_check_warms"},
+        {"rec", "ms_clean_from_first_valids"},
+        {"rec", "ms_first_nonfinite_from_first_valids"},
         {"rolling", "bb_batch_f32"},
         {"rolling", "bb_ms1p_f32"},
         {"rolling", "pnr_batch_f32"},
@@ -438,5 +469,85 @@ vta_status vta_memcpy_dtoh(vta_ctx *ctx, void *dst, const void *src, size_t byte
     VTA_HIP(hipStreamSynchronize(ctx->stream));
     return VTA_OK;
 }
+
+}  // extern "C"
+
+namespace vta {
+
+vta_status check_warms_device(vta_ctx *ctx, const int32_t *d_periods, const int32_t *d_warms,
+                              int n, int T, int first_valid, int max_period,
+                              int32_t *d_fv_out) {
+    hipFunction_t f;
+    VTA_TRY(get_function(ctx, "rec", "vta_check_warms", &f));
+    int32_t *err = static_cast<int32_t *>(ctx->scratch) + kArgErrorWord;
+    ctx->arg_check_pending = true;
+    return launch(ctx, f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, d_periods, d_warms,
+                  n, T, first_valid, max_period, d_fv_out, err);
+}
+
+}  // namespace vta
+
+namespace {
+
+vta_status set_policy(vta_ctx *ctx, int slot, const vta_kernel_policy *p) {
+    clear_error();
+    if (!ctx || !p) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    const int32_t kinds[2] = {p->batch, p->many_series};
+    const uint32_t bx[2] = {p->batch_block_x, p->many_series_block_x};
+    for (int i = 0; i < 2; ++i) {
+        if (kinds[i] != VTA_KERNEL_AUTO && kinds[i] != VTA_KERNEL_PLAIN)
+            return fail(VTA_ERR_INVALID_POLICY, "invalid policy kind %d", (int)kinds[i]);
+        if (kinds[i] == VTA_KERNEL_PLAIN && bx[i] != 0 && bx[i] != 64)
+            return fail(VTA_ERR_INVALID_POLICY,
+                        "invalid policy: block_x %u (these kernels run 64-lane waves; use 0 or "
+                        "64)", bx[i]);
+    }
+    vta_kernel_policy q = *p;
+    if (q.batch == VTA_KERNEL_PLAIN) q.batch_block_x = 64;
+    if (q.many_series == VTA_KERNEL_PLAIN) q.many_series_block_x = 64;
+    if (q.batch == VTA_KERNEL_AUTO) q.batch_block_x = 0;
+    if (q.many_series == VTA_KERNEL_AUTO) q.many_series_block_x = 0;
+    ctx->policy[slot] = q;
+    return VTA_OK;
+}
+
+vta_status get_policy(const vta_ctx *ctx, int slot, vta_kernel_policy *out) {
+    if (!ctx || !out) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    *out = ctx->policy[slot];
+    return VTA_OK;
+}
+
+vta_status get_selected(const vta_ctx *ctx, const vta_kernel_selected *v,
+                        vta_kernel_selected *out) {
+    if (!ctx || !out) return fail(VTA_ERR_INVALID_INPUT, "null argument");
+    *out = *v;
+    return VTA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+#define VTA_POLICY_ENTRY_POINTS(NAME, SLOT)                                                 \
+    vta_status vta_##NAME##_set_policy(vta_ctx *ctx, const vta_kernel_policy *policy) {     \
+        return set_policy(ctx, SLOT, policy);                                               \
+    }                                                                                       \
+    vta_status vta_##NAME##_policy(const vta_ctx *ctx, vta_kernel_policy *out) {            \
+        return get_policy(ctx, SLOT, out);                                                  \
+    }                                                                                       \
+    vta_status vta_##NAME##_selected_batch_kernel(const vta_ctx *ctx,                       \
+                                                  vta_kernel_selected *out) {               \
+        return get_selected(ctx, ctx ? &ctx->sel_batch[SLOT] : nullptr, out);               \
+    }                                                                                       \
+    vta_status vta_##NAME##_selected_many_series_kernel(const vta_ctx *ctx,                 \
+                                                        vta_kernel_selected *out) {         \
+        return get_selected(ctx, ctx ? &ctx->sel_ms[SLOT] : nullptr, out);                  \
+    }
+
+VTA_POLICY_ENTRY_POINTS(ema, VTA_POLICY_SLOT_EMA)
+VTA_POLICY_ENTRY_POINTS(rsi, VTA_POLICY_SLOT_RSI)
+VTA_POLICY_ENTRY_POINTS(adx, VTA_POLICY_SLOT_ADX)
+VTA_POLICY_ENTRY_POINTS(pnr, VTA_POLICY_SLOT_PNR)
+#undef VTA_POLICY_ENTRY_POINTS
 
 }  // extern "C"

@@ -31,9 +31,28 @@ struct vta_ctx {
     static constexpr size_t kCarryBytes = 16u << 20;
     // launch-shape knobs (vta_tune_knob); 0 = built-in heuristic
     int64_t tune[VTA_TUNE_KNOB_COUNT] = {};
+    // EMA / RSI / ADX / PNR kernel policies and the forms their last batch and
+    // many-series calls ran (indexed by vta::PolicySlot)
+    vta_kernel_policy policy[4] = {};
+    vta_kernel_selected sel_batch[4] = {};
+    vta_kernel_selected sel_ms[4] = {};
+    // an enqueue-only call with device-side argument checks (warm indices) has
+    // run since the last vta_ctx_synchronize, which then reads the error word
+    bool arg_check_pending = false;
 };
 
 namespace vta {
+
+enum PolicySlot {
+    VTA_POLICY_SLOT_EMA = 0,
+    VTA_POLICY_SLOT_RSI = 1,
+    VTA_POLICY_SLOT_ADX = 2,
+    VTA_POLICY_SLOT_PNR = 3
+};
+
+// int32 word of the context scratch that device-side argument checks set to 1
+// on a mismatch; vta_ctx_synchronize reports and clears it.
+constexpr size_t kArgErrorWord = 1024;
 
 vta_status fail(vta_status st, const char *fmt, ...);
 void clear_error();
@@ -159,6 +178,20 @@ vta_status alma_tables_for_range(vta_ctx *ctx, const vta_alma_batch_range *range
 // address d_tables (t.blob's layout); d_out holds n_combos x len floats.
 vta_status alma_tables_launch(vta_ctx *ctx, const AlmaTables &t, const void *d_tables,
                               const float *d_prices, size_t len, size_t fv, float *d_out);
+
+// Enqueues the check of the reference's per-combo warm indices against
+// first_valid + period - 1 (kernel vta_check_warms); first_valid < 0 derives
+// first_valid from combo 0 into *d_fv_out (SWMA's device form has no
+// first_valid argument).  A mismatch is reported by the next synchronize.
+vta_status check_warms_device(vta_ctx *ctx, const int32_t *d_periods, const int32_t *d_warms,
+                              int n, int T, int first_valid, int max_period,
+                              int32_t *d_fv_out);
+
+// Many-series launch of SMA / WMA / SWMA (fir.cpp; kind 0 / 1 / 2) over
+// caller-prepared first non-finite bars (d_zs, NULL: every bar finite).
+vta_status fir_ms_prepared(vta_ctx *ctx, int kind, const float *d_x_tm, const int32_t *d_fvs,
+                           const int32_t *d_zs, size_t cols, size_t rows, size_t period,
+                           float *d_out_tm);
 
 // Looks up, once per device, the device address of the static scratch array
 // of the allocation-free entry point (noalloc.hip); `device` must be current.

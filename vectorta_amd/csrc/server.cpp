@@ -461,8 +461,8 @@ vta_status run_request(vta_server *s, vta_ctx *ctx, Req &q) {
             res.n_combos = n;
             VTA_TRY(alloc_out(s, res.out[0], n, T));
             if (r.kind == VTA_REQ_EMA)
-                VTA_TRY(vta_ema_batch_device(ctx, e.d_close, dp.as<int32_t>(), nullptr, (int32_t)maxp,
-                                             (int32_t)T, fv, (int32_t)n, res.out[0].ptr));
+                VTA_TRY(vta_ema_batch_device(ctx, e.d_close, dp.as<int32_t>(), nullptr, T,
+                                             (size_t)fv, n, res.out[0].ptr));
             else if (r.kind == VTA_REQ_RSI)
                 VTA_TRY(vta_rsi_batch_device(ctx, e.d_close, dp.as<int32_t>(), (int32_t)maxp,
                                              (int32_t)T, fv, (int32_t)n, res.out[0].ptr));

@@ -60,16 +60,6 @@ class _Fir:
                  ctypes.byref(out)))
         return DeviceArrayF32(self.ctx, out)
 
-    def _ms_device(self, d_prices_tm, d_first_valids, cols, rows, period, d_out_tm,
-                   d_first_nonfinite=None) -> None:
-        require_len(d_prices_tm, int(cols) * int(rows), "time-major input")
-        require_len(d_out_tm, int(cols) * int(rows), "time-major output")
-        require_len(d_first_valids, cols, "first_valids")
-        require_len(d_first_nonfinite, cols, "first_nonfinite")
-        fn = getattr(lib(), f"vta_{self._name}_many_series_one_param_device")
-        check(fn(self.ctx.handle, _vp(d_prices_tm), _vp(d_first_valids), _vp(d_first_nonfinite),
-                 int(cols), int(rows), int(period), _vp(d_out_tm)))
-
 
 class CudaSma(_Fir):
     _name = "sma"
@@ -78,13 +68,49 @@ class CudaSma(_Fir):
         return self._batch_dev(prices_f32, period)
 
     def sma_batch_device(self, *args) -> None:
+        """Enqueue-only SMA batch (no reference counterpart): (d_prices,
+        d_periods, series_len, n_combos, first_valid, max_period, d_out)."""
         self._batch_device(*args)
+
+    def sma_batch_dev_from_device(self, d_prices: DeviceBuffer, len_: int, periods,
+                                  first_valid: int) -> DeviceArrayF32:
+        """sma_wrapper.rs:830: device prices, host combos (periods)."""
+        require_len(d_prices, len_, "prices")
+        ps = (ctypes.c_size_t * max(len(periods), 1))(*[int(p) for p in periods])
+        out = DeviceArrayF32C()
+        check(lib().vta_sma_batch_dev_from_device(self.ctx.handle, _vp(d_prices), int(len_), ps,
+                                                  len(periods), int(first_valid),
+                                                  ctypes.byref(out)))
+        return DeviceArrayF32(self.ctx, out)
 
     def sma_multi_series_one_param_time_major_dev(self, data_tm, cols, rows, period):
         return self._ms_dev(data_tm, cols, rows, period)
 
-    def sma_multi_series_one_param_device(self, *args, **kw) -> None:
-        self._ms_device(*args, **kw)
+    def sma_many_series_one_param_device(self, d_prices_tm, d_first_valids, cols, rows, period,
+                                         d_out_tm) -> None:
+        """Enqueue-only SMA many-series form (no reference counterpart)."""
+        _need_ms(d_prices_tm, d_first_valids, d_out_tm, cols, rows)
+        check(lib().vta_sma_many_series_one_param_device(
+            self.ctx.handle, _vp(d_prices_tm), _vp(d_first_valids), int(cols), int(rows),
+            int(period), _vp(d_out_tm)))
+
+    def sma_multi_series_one_param_time_major_dev_from_device(self, d_prices_tm, d_first_valids,
+                                                              cols, rows,
+                                                              period) -> DeviceArrayF32:
+        """sma_wrapper.rs:982."""
+        require_len(d_prices_tm, int(cols) * int(rows), "time-major input")
+        require_len(d_first_valids, cols, "first_valids")
+        out = DeviceArrayF32C()
+        check(lib().vta_sma_multi_series_one_param_time_major_dev_from_device(
+            self.ctx.handle, _vp(d_prices_tm), _vp(d_first_valids), int(cols), int(rows),
+            int(period), ctypes.byref(out)))
+        return DeviceArrayF32(self.ctx, out)
+
+
+def _need_ms(d_prices_tm, d_first_valids, d_out_tm, cols, rows) -> None:
+    require_len(d_prices_tm, int(cols) * int(rows), "time-major input")
+    require_len(d_out_tm, int(cols) * int(rows), "time-major output")
+    require_len(d_first_valids, cols, "first_valids")
 
 
 class CudaWma(_Fir):
@@ -94,13 +120,21 @@ class CudaWma(_Fir):
         return self._batch_dev(prices_f32, period)
 
     def wma_batch_device(self, *args) -> None:
+        """wma_wrapper.rs:747: (d_prices, d_periods, series_len, n_combos,
+        first_valid, max_period, d_out)."""
         self._batch_device(*args)
 
     def wma_multi_series_one_param_time_major_dev(self, data_tm, cols, rows, period):
         return self._ms_dev(data_tm, cols, rows, period)
 
-    def wma_multi_series_one_param_time_major_device(self, *args, **kw) -> None:
-        self._ms_device(*args, **kw)
+    def wma_multi_series_one_param_time_major_device(self, d_prices_tm, period: int,
+                                                     num_series: int, series_len: int,
+                                                     d_first_valids, d_out_tm) -> None:
+        """wma_wrapper.rs:1049."""
+        _need_ms(d_prices_tm, d_first_valids, d_out_tm, num_series, series_len)
+        check(lib().vta_wma_multi_series_one_param_time_major_device(
+            self.ctx.handle, _vp(d_prices_tm), int(period), int(num_series), int(series_len),
+            _vp(d_first_valids), _vp(d_out_tm)))
 
 
 class CudaSwma(_Fir):
@@ -109,11 +143,27 @@ class CudaSwma(_Fir):
     def swma_batch_dev(self, prices_f32, period: Tuple[int, int, int]) -> DeviceArrayF32:
         return self._batch_dev(prices_f32, period)
 
-    def swma_batch_device(self, *args) -> None:
-        self._batch_device(*args)
+    def swma_batch_device(self, d_prices, d_periods, d_warms, series_len: int, n_combos: int,
+                          max_period: int, d_out) -> None:
+        """swma_wrapper.rs:663: first_valid = d_warms[0] - periods[0] + 1 on the
+        device; every warm index checked (reported by the next synchronize)."""
+        require_len(d_prices, series_len, "prices")
+        require_len(d_periods, n_combos, "periods")
+        require_len(d_warms, n_combos, "warms")
+        require_len(d_out, int(n_combos) * int(series_len), "output")
+        check(lib().vta_swma_batch_device(self.ctx.handle, _vp(d_prices), _vp(d_periods),
+                                          _vp(d_warms), int(series_len), int(n_combos),
+                                          int(max_period), _vp(d_out)))
 
     def swma_multi_series_one_param_time_major_dev(self, data_tm, cols, rows, period):
         return self._ms_dev(data_tm, cols, rows, period)
 
-    def swma_multi_series_one_param_device(self, *args, **kw) -> None:
-        self._ms_device(*args, **kw)
+    def swma_multi_series_one_param_device(self, d_prices_tm, d_weights, period: int,
+                                           num_series: int, series_len: int, d_first_valids,
+                                           d_out_tm) -> None:
+        """swma_wrapper.rs:787: the weights follow from the period (d_weights is
+        not read)."""
+        _need_ms(d_prices_tm, d_first_valids, d_out_tm, num_series, series_len)
+        check(lib().vta_swma_multi_series_one_param_device(
+            self.ctx.handle, _vp(d_prices_tm), _vp(d_weights), int(period), int(num_series),
+            int(series_len), _vp(d_first_valids), _vp(d_out_tm)))

@@ -9,8 +9,67 @@ from typing import List, Sequence, Tuple
 
 import numpy as np
 
-from ._lib import DeviceArrayF32C, RangeUsize, RecJobC, VtaError, check, lib
+from dataclasses import dataclass
+from typing import Optional
+
+from ._lib import (DeviceArrayF32C, KernelPolicyC, KernelSelectedC, RangeUsize, RecJobC,
+                   VtaError, check, lib)
 from .device import DeviceArrayF32, DeviceBuffer, HipContext, f32_ptr, require_len
+
+_KIND = {"auto": 0, "plain": 1, "oned": 1}
+_KIND_NAME = {0: "auto", 1: "plain"}
+_SELECTED = {1: "plain", 2: "chunked"}
+
+
+@dataclass
+class KernelPolicy:
+    """CudaEmaPolicy / CudaRsiPolicy / CudaAdxPolicy / CudaPnrPolicy
+    (ema_wrapper.rs:49-90): per call family "auto" or "plain" (the reference's
+    Plain { block_x } / OneD { block_x }: one lane per combo or series over
+    the whole series, no time chunking).  block_x is 0 or 64 (64-lane waves)."""
+    batch: str = "auto"
+    batch_block_x: int = 0
+    many_series: str = "auto"
+    many_series_block_x: int = 0
+
+    def to_c(self) -> KernelPolicyC:
+        for k in (self.batch, self.many_series):
+            if k not in _KIND:
+                raise VtaError(5, f"invalid policy: {k}")
+        return KernelPolicyC(_KIND[self.batch], int(self.batch_block_x),
+                             _KIND[self.many_series], int(self.many_series_block_x))
+
+
+class _PolicyMixin:
+    """set_policy / policy / selected_batch_kernel / selected_many_series_kernel
+    (percentile_nearest_rank_wrapper.rs:143-152); new_with_policy is the
+    constructor's policy argument (ema_wrapper.rs:172)."""
+    _policy_name = ""
+
+    def set_policy(self, policy: KernelPolicy) -> None:
+        pc = policy.to_c()
+        check(getattr(lib(), f"vta_{self._policy_name}_set_policy")(self.ctx.handle,
+                                                                    ctypes.byref(pc)))
+
+    def policy(self) -> KernelPolicy:
+        pc = KernelPolicyC()
+        check(getattr(lib(), f"vta_{self._policy_name}_policy")(self.ctx.handle,
+                                                                ctypes.byref(pc)))
+        return KernelPolicy(_KIND_NAME[pc.batch], pc.batch_block_x,
+                            _KIND_NAME[pc.many_series], pc.many_series_block_x)
+
+    def _selected(self, which: str):
+        sc = KernelSelectedC()
+        check(getattr(lib(), f"vta_{self._policy_name}_selected_{which}_kernel")(
+            self.ctx.handle, ctypes.byref(sc)))
+        return (_SELECTED[sc.kind], sc.block_x) if sc.kind in _SELECTED else None
+
+    def selected_batch_kernel(self):
+        """None before the first batch call, else ("plain" | "chunked", 64)."""
+        return self._selected("batch")
+
+    def selected_many_series_kernel(self):
+        return self._selected("many_series")
 
 
 def _rng(r: Sequence[int]) -> RangeUsize:
@@ -82,36 +141,73 @@ def _need_ms(d_prices_tm, d_first_valids, d_out_tm, d_clean, num_series, series_
     require_len(d_clean, num_series, "clean flags")
 
 
-class CudaEma(_OneParam):
+def _vp(buf):
+    return ctypes.c_void_p(buf.ptr) if buf is not None else None
+
+
+def _into_host(fn, out: np.ndarray, *args) -> None:
+    if out.dtype != np.float32 or not out.flags.c_contiguous:
+        raise VtaError(4, "out must be a contiguous float32 array")
+    check(fn(*args, f32_ptr(out), out.size))
+
+
+class CudaEma(_OneParam, _PolicyMixin):
     _name = "ema"
+    _policy_name = "ema"
+
+    def __init__(self, device_id: int = 0, policy: Optional[KernelPolicy] = None):
+        super().__init__(device_id)
+        if policy is not None:
+            self.set_policy(policy)
 
     def ema_batch_dev(self, prices_f32, period: Tuple[int, int, int]) -> DeviceArrayF32:
         return self._batch_dev([prices_f32], period)
 
-    def ema_batch_device(self, d_prices: DeviceBuffer, d_periods: DeviceBuffer, max_period: int,
-                         series_len: int, first_valid: int, n_combos: int,
-                         d_out: DeviceBuffer) -> None:
+    def ema_batch_device(self, d_prices: DeviceBuffer, d_periods: DeviceBuffer,
+                         d_alphas: Optional[DeviceBuffer], series_len: int, first_valid: int,
+                         n_combos: int, d_out: DeviceBuffer) -> None:
+        """ema_wrapper.rs:256: enqueue-only; d_alphas[c] is combo c's smoothing
+        factor (None derives 2/(p+1) in f64)."""
         _need_batch(d_prices, d_periods, d_out, series_len, n_combos)
-        check(lib().vta_ema_batch_device(self.ctx.handle, ctypes.c_void_p(d_prices.ptr),
-                                         ctypes.c_void_p(d_periods.ptr), None, int(max_period),
-                                         int(series_len), int(first_valid), int(n_combos),
-                                         ctypes.c_void_p(d_out.ptr)))
+        require_len(d_alphas, n_combos, "alphas")
+        check(lib().vta_ema_batch_device(self.ctx.handle, _vp(d_prices), _vp(d_periods),
+                                         _vp(d_alphas), int(series_len), int(first_valid),
+                                         int(n_combos), _vp(d_out)))
+
+    def ema_batch_into_host_f32(self, prices_f32, period, out: np.ndarray) -> None:
+        x = _f32(prices_f32)
+        rc = _rng(period)
+        _into_host(lib().vta_ema_batch_into_host_f32, out, self.ctx.handle, f32_ptr(x), x.size,
+                   ctypes.byref(rc))
 
     def ema_many_series_one_param_time_major_dev(self, data_tm, cols, rows, period):
         return self._ms_dev([data_tm], cols, rows, period)
 
-    def ema_many_series_one_param_device(self, d_prices_tm, d_first_valids, period, num_series,
-                                         series_len, d_out_tm, d_clean=None) -> None:
-        _need_ms(d_prices_tm, d_first_valids, d_out_tm, d_clean, num_series, series_len)
+    def ema_many_series_one_param_device(self, d_prices_tm, d_first_valids, period: int,
+                                         alpha: float, num_series: int, series_len: int,
+                                         d_out_tm) -> None:
+        """ema_wrapper.rs:381: enqueue-only, the caller's smoothing factor."""
+        _need_ms(d_prices_tm, d_first_valids, d_out_tm, None, num_series, series_len)
         check(lib().vta_ema_many_series_one_param_device(
-            self.ctx.handle, ctypes.c_void_p(d_prices_tm.ptr), ctypes.c_void_p(d_first_valids.ptr),
-            int(period), int(num_series), int(series_len),
-            ctypes.c_void_p(d_clean.ptr) if d_clean is not None else None,
-            ctypes.c_void_p(d_out_tm.ptr)))
+            self.ctx.handle, _vp(d_prices_tm), _vp(d_first_valids), int(period), float(alpha),
+            int(num_series), int(series_len), _vp(d_out_tm)))
+
+    def ema_many_series_one_param_time_major_into_host_f32(self, data_tm, num_series,
+                                                           series_len, period,
+                                                           out_tm: np.ndarray) -> None:
+        x = _f32(data_tm).reshape(-1)
+        _into_host(lib().vta_ema_many_series_one_param_time_major_into_host_f32, out_tm,
+                   self.ctx.handle, f32_ptr(x), int(num_series), int(series_len), int(period))
 
 
-class CudaRsi(_OneParam):
+class CudaRsi(_OneParam, _PolicyMixin):
     _name = "rsi"
+    _policy_name = "rsi"
+
+    def __init__(self, device_id: int = 0, policy: Optional[KernelPolicy] = None):
+        super().__init__(device_id)
+        if policy is not None:
+            self.set_policy(policy)
 
     def rsi_batch_dev(self, prices_f32, period: Tuple[int, int, int]) -> DeviceArrayF32:
         return self._batch_dev([prices_f32], period)
@@ -153,76 +249,119 @@ class CudaAtr(_OneParam):
                                                  int(n_combos), ctypes.c_void_p(d_out.ptr)))
 
 
-class CudaAdx(_OneParam):
+class CudaAdx(_OneParam, _PolicyMixin):
     _name = "adx"
+    _policy_name = "adx"
+
+    def __init__(self, device_id: int = 0, policy: Optional[KernelPolicy] = None):
+        super().__init__(device_id)
+        if policy is not None:
+            self.set_policy(policy)
 
     def adx_batch_dev(self, high, low, close, period: Tuple[int, int, int]) -> DeviceArrayF32:
         return self._batch_dev([high, low, close], period)
+
+    def adx_batch_into_host_f32(self, high, low, close, period, out: np.ndarray):
+        """adx_wrapper.rs:253: returns (rows, cols, periods) like the reference."""
+        h, lo, c = (_f32(a) for a in (high, low, close))
+        rc = _rng(period)
+        _into_host(lib().vta_adx_batch_into_host_f32, out, self.ctx.handle, f32_ptr(h),
+                   f32_ptr(lo), f32_ptr(c), c.size, ctypes.byref(rc))
+        ps = self.expand_grid(period)
+        return len(ps), c.size, ps
 
     def adx_many_series_one_param_time_major_dev(self, high_tm, low_tm, close_tm, cols, rows,
                                                  period):
         return self._ms_dev([high_tm, low_tm, close_tm], cols, rows, period)
 
-
-class _OneInputMa(_OneParam):
-    """SMMA / Wilders: one-series batch, enqueue-only batch and many-series forms."""
-
-    def _batch(self, prices_f32, period) -> DeviceArrayF32:
-        return self._batch_dev([prices_f32], period)
-
-    def _batch_device(self, d_prices: DeviceBuffer, d_periods: DeviceBuffer, max_period: int,
-                      series_len: int, first_valid: int, n_combos: int,
-                      d_out: DeviceBuffer) -> None:
-        _need_batch(d_prices, d_periods, d_out, series_len, n_combos)
-        fn = getattr(lib(), f"vta_{self._name}_batch_device")
-        check(fn(self.ctx.handle, ctypes.c_void_p(d_prices.ptr), ctypes.c_void_p(d_periods.ptr),
-                 int(max_period), int(series_len), int(first_valid), int(n_combos),
-                 ctypes.c_void_p(d_out.ptr)))
-
-    def _ms_device(self, d_prices_tm, d_first_valids, period, num_series, series_len, d_out_tm,
-                   d_clean=None) -> None:
-        _need_ms(d_prices_tm, d_first_valids, d_out_tm, d_clean, num_series, series_len)
-        fn = getattr(lib(), f"vta_{self._name}_many_series_one_param_device")
-        check(fn(self.ctx.handle, ctypes.c_void_p(d_prices_tm.ptr),
-                 ctypes.c_void_p(d_first_valids.ptr), int(period), int(num_series),
-                 int(series_len), ctypes.c_void_p(d_clean.ptr) if d_clean is not None else None,
-                 ctypes.c_void_p(d_out_tm.ptr)))
+    def adx_many_series_one_param_time_major_into_host_f32(self, high_tm, low_tm, close_tm,
+                                                           cols, rows, period,
+                                                           out_tm: np.ndarray) -> None:
+        h, lo, c = (_f32(a).reshape(-1) for a in (high_tm, low_tm, close_tm))
+        _into_host(lib().vta_adx_many_series_one_param_time_major_into_host_f32, out_tm,
+                   self.ctx.handle, f32_ptr(h), f32_ptr(lo), f32_ptr(c), int(cols), int(rows),
+                   int(period))
 
 
-class CudaSmma(_OneInputMa):
+class CudaSmma(_OneParam):
     """src/cuda/moving_averages/smma_wrapper.rs (:398 batch_dev, :377 batch_device,
     :623 many-series _dev, :599 many-series _device)."""
     _name = "smma"
 
     def smma_batch_dev(self, prices_f32, period: Tuple[int, int, int]) -> DeviceArrayF32:
-        return self._batch(prices_f32, period)
+        return self._batch_dev([prices_f32], period)
 
-    def smma_batch_device(self, *args) -> None:
-        self._batch_device(*args)
+    def smma_batch_device(self, d_prices, d_periods, d_warms, first_valid: int, series_len: int,
+                          n_combos: int, d_out) -> None:
+        """Enqueue-only; d_warms[c] must be first_valid + period - 1 (checked on
+        the device, reported by the next synchronize)."""
+        _need_batch(d_prices, d_periods, d_out, series_len, n_combos)
+        require_len(d_warms, n_combos, "warms")
+        check(lib().vta_smma_batch_device(self.ctx.handle, _vp(d_prices), _vp(d_periods),
+                                          _vp(d_warms), int(first_valid), int(series_len),
+                                          int(n_combos), _vp(d_out)))
 
     def smma_multi_series_one_param_time_major_dev(self, data_tm, cols, rows, period):
         return self._ms_dev([data_tm], cols, rows, period)
 
-    def smma_multi_series_one_param_device(self, *args, **kw) -> None:
-        self._ms_device(*args, **kw)
+    def smma_multi_series_one_param_device(self, d_prices_tm, period: int, num_series: int,
+                                           series_len: int, d_first_valids, d_out_tm) -> None:
+        _need_ms(d_prices_tm, d_first_valids, d_out_tm, None, num_series, series_len)
+        check(lib().vta_smma_multi_series_one_param_device(
+            self.ctx.handle, _vp(d_prices_tm), int(period), int(num_series), int(series_len),
+            _vp(d_first_valids), _vp(d_out_tm)))
 
 
-class CudaWilders(_OneInputMa):
+class CudaWilders(_OneParam):
     """src/cuda/moving_averages/wilders_wrapper.rs (:134 batch_dev, :252 batch_device,
     :600 many-series _dev, :662 many-series _device)."""
     _name = "wilders"
 
     def wilders_batch_dev(self, prices_f32, period: Tuple[int, int, int]) -> DeviceArrayF32:
-        return self._batch(prices_f32, period)
+        return self._batch_dev([prices_f32], period)
 
-    def wilders_batch_device(self, *args) -> None:
-        self._batch_device(*args)
+    def wilders_batch_device(self, d_prices, d_periods, d_alphas, d_warm, series_len: int,
+                             first_valid: int, n_combos: int, d_out) -> None:
+        """Enqueue-only; the alphas are used, d_warm[c] must be
+        first_valid + period - 1 (checked on the device)."""
+        _need_batch(d_prices, d_periods, d_out, series_len, n_combos)
+        require_len(d_alphas, n_combos, "alphas")
+        require_len(d_warm, n_combos, "warms")
+        check(lib().vta_wilders_batch_device(self.ctx.handle, _vp(d_prices), _vp(d_periods),
+                                             _vp(d_alphas), _vp(d_warm), int(series_len),
+                                             int(first_valid), int(n_combos), _vp(d_out)))
 
     def wilders_many_series_one_param_time_major_dev(self, data_tm, cols, rows, period):
         return self._ms_dev([data_tm], cols, rows, period)
 
-    def wilders_many_series_one_param_device(self, *args, **kw) -> None:
-        self._ms_device(*args, **kw)
+    def wilders_many_series_one_param_device(self, d_prices_tm, d_first_valids, period: int,
+                                             alpha: float, num_series: int, series_len: int,
+                                             d_out_tm) -> None:
+        _need_ms(d_prices_tm, d_first_valids, d_out_tm, None, num_series, series_len)
+        check(lib().vta_wilders_many_series_one_param_device(
+            self.ctx.handle, _vp(d_prices_tm), _vp(d_first_valids), int(period), float(alpha),
+            int(num_series), int(series_len), _vp(d_out_tm)))
+
+
+MA_KINDS = {"ema": 0, "smma": 1, "wilders": 2, "sma": 3, "wma": 4, "swma": 5}
+
+
+def ma_many_series_prepared_device(ctx: HipContext, kind: str, d_prices_tm, d_first_valids,
+                                   d_meta, period: int, num_series: int, series_len: int,
+                                   d_out_tm, alpha: float = 0.0) -> None:
+    """Enqueue-only many-series EMA / SMMA / Wilders / SMA / WMA / SWMA over
+    metadata prepared once with vta_many_series_prepare (clean flags, mode 0,
+    for EMA / SMMA / Wilders; first non-finite bars, mode 4, for the others)."""
+    if kind not in MA_KINDS:
+        raise VtaError(4, f"unknown moving-average kind {kind!r}")
+    total = int(num_series) * int(series_len)
+    require_len(d_prices_tm, total, "time-major input")
+    require_len(d_out_tm, total, "time-major output")
+    require_len(d_first_valids, num_series, "first_valids")
+    require_len(d_meta, num_series, "metadata")
+    check(lib().vta_ma_many_series_prepared_device(
+        ctx.handle, MA_KINDS[kind], _vp(d_prices_tm), _vp(d_first_valids), _vp(d_meta),
+        int(period), float(alpha), int(num_series), int(series_len), _vp(d_out_tm)))
 
 
 def ema_rsi_atr_many_series_dev(ctx: HipContext, close_tm, high_tm, low_tm, cols: int,

@@ -6,13 +6,14 @@ from __future__ import annotations
 
 import ctypes
 from dataclasses import dataclass
-from typing import List, Optional, Sequence, Tuple
+from typing import Optional, List, Optional, Sequence, Tuple
 
 import numpy as np
 
 from ._lib import (BbBatchRangeC, BbJobC, BbParamsC, DeviceArrayF32C, PnrBatchRangeC, PnrParamsC,
                    RangeF64, RangeUsize, check, lib)
 from .device import DeviceArrayF32, DeviceBuffer, HipContext, f32_ptr, require_len
+from .rec import KernelPolicy, _PolicyMixin
 
 
 def _f32(a) -> np.ndarray:
@@ -104,7 +105,16 @@ class CudaBollingerBands(_Rolling):
             ctypes.c_void_p(d_lower.ptr)))
 
 
-class CudaPercentileNearestRank(_Rolling):
+class CudaPercentileNearestRank(_Rolling, _PolicyMixin):
+    """CudaPnr (percentile_nearest_rank_wrapper.rs), with its policy surface
+    (set_policy / policy / selected_*_kernel, :143-152)."""
+    _policy_name = "pnr"
+
+    def __init__(self, device_id: int = 0, policy: Optional[KernelPolicy] = None):
+        super().__init__(device_id)
+        if policy is not None:
+            self.set_policy(policy)
+
     @staticmethod
     def expand_grid(sweep: PercentileNearestRankBatchRange) -> List[Tuple[int, float]]:
         rc = sweep.to_c()
