@@ -75,10 +75,14 @@ vta_status vta_ctx_preload(vta_ctx *ctx);
  * the built-in heuristic (which an environment variable of the same family
  * may still override); any other value must be in the knob's range. */
 typedef enum vta_tune_knob {
-    /* split-BF16 ALMA sweep kernel: 1 workgroup of 4 N-blocks x 16 tiles,
+    /* split MFMA ALMA sweep kernel: 1 workgroup of 4 N-blocks x 16 tiles,
      * 2 x 8 tiles, 3 x 16 tiles two at a time, 4 x 16 tiles with K > 128 and
      * K <= 128 in separate launches, 5 one wave per N-block x 8 tiles,
-     * 6 x 16 tiles, 7 x 32 tiles */
+     * 6 x 16 tiles, 7 x 32 tiles (1-7: BF16 only); 8 diagonal form, 2-tile
+     * sub-runs, LDS-transposed stores, 9 the same with direct stores, 10
+     * 4-tile sub-runs, direct stores, 11 2-tile sub-runs whose transposed
+     * stores are issued during the next sub-run's MFMAs (runs as 8 when the
+     * series length is not a multiple of 4) (8-11: BF16 and F16) */
     VTA_TUNE_ALMA_MFMA_VARIANT = 0,
     VTA_TUNE_SCAN_WAVES = 1,  /* recurrence batch chunked scan: waves per pass, 64..65536 */
     VTA_TUNE_FIR_GROUP = 2,   /* SMA/WMA/SWMA batch: combos per workgroup, 1..256 */
@@ -186,11 +190,19 @@ vta_status vta_alma_expand_grid(const vta_alma_batch_range *range, vta_alma_para
 /* Batch kernel policy (BatchKernelPolicy / BatchKernelSelected,
  * alma_wrapper.rs:18-70).  FP32 = LDS-staged sliding dot (the reference's
  * Plain/Tiled family); MFMA_BF16X3 = split-BF16 Hankel x weights contraction
- * on the matrix cores with FP32 accumulation. */
+ * on the matrix cores with FP32 accumulation (hi/lo, three products);
+ * MFMA_F16X3 = the same contraction in split half precision with per-window
+ * and per-combo power-of-two scaling, whose split error is bounded per tile
+ * and checked: a tile where the bound exceeds 2e-6 of an output is
+ * recomputed with the FP32 dot.  AUTO picks MFMA_F16X3 for sweeps of at
+ * least 16 combos with periods up to 256, and FP32 otherwise (also for host
+ * series that change sign, where the check would send most tiles back to
+ * FP32). */
 typedef enum vta_alma_batch_kernel {
     VTA_ALMA_BATCH_AUTO = 0,
     VTA_ALMA_BATCH_FP32 = 1,
-    VTA_ALMA_BATCH_MFMA_BF16X3 = 2
+    VTA_ALMA_BATCH_MFMA_BF16X3 = 2,
+    VTA_ALMA_BATCH_MFMA_F16X3 = 3
 } vta_alma_batch_kernel;
 vta_status vta_alma_set_batch_policy(vta_ctx *ctx, vta_alma_batch_kernel policy);
 /* selected_batch_kernel (:239-245): kernel used by the last batch call, 0 if none. */

@@ -29,7 +29,7 @@ def test_cache_round_trip(tmp_path):
 
 def test_candidates_are_in_range():
     # the ranges vta_ctx_set_tuning accepts (include/vta.h, vta_tune_knob)
-    lo_hi = {at.ALMA_MFMA_VARIANT: (1, 7), at.SCAN_WAVES: (64, 65536), at.FIR_GROUP: (1, 256),
+    lo_hi = {at.ALMA_MFMA_VARIANT: (1, 11), at.SCAN_WAVES: (64, 65536), at.FIR_GROUP: (1, 256),
              at.MS_CHUNK: (64, 1 << 30)}
     for knob, cands in at.CANDIDATES.items():
         lo, hi = lo_hi[knob]

@@ -16,7 +16,8 @@ from tests.helpers import assert_close, max_rel_err, nan_mask_equal
 
 pytestmark = pytest.mark.gpu
 
-POLICIES = ["fp32", "mfma"]
+POLICIES = ["fp32", "mfma", "mfma_f16x3"]
+SELECTED = {"fp32": "fp32", "mfma": "mfma_bf16x3", "mfma_f16x3": "mfma_f16x3"}
 
 
 @pytest.fixture(scope="module")
@@ -43,7 +44,7 @@ def test_batch_matches_oracle_reference_case(alma, policy):
     sweep = vta.AlmaBatchRange(period=(9, 32, 1), offset=(0.85, 0.85, 0.0), sigma=(6.0, 6.0, 0.0))
     alma.set_policy(policy)
     arr = alma.alma_batch_dev(x, sweep)
-    assert alma.selected_batch_kernel() == ("fp32" if policy == "fp32" else "mfma_bf16x3")
+    assert alma.selected_batch_kernel() == SELECTED[policy]
     g = arr.to_host()
     ref = O.alma_batch(x.astype(np.float64), _combos(sweep))
     assert g.shape == ref.shape == (24, 2048)
@@ -275,3 +276,41 @@ def test_many_series_period_above_lds_floats(alma):
     assert (st == 0).all()
     nan_mask_equal(g, ref)
     assert max_rel_err(g, ref) <= 1e-5
+
+
+def test_f16x3_zero_crossing_noise_guard(alma):
+    """Split-F16 sweep on white noise (zero crossings everywhere): a tile whose
+    split-error bound exceeds 2e-6 |y| for any output is recomputed with the
+    FP32 dot, so every output meets the atol + rtol 5e-6 form; AUTO on host
+    prices that change sign resolves to the FP32 kernel."""
+    rng = np.random.default_rng(5)
+    z = rng.normal(0.0, 1.0, 50_000).astype(np.float32)
+    sweep = vta.AlmaBatchRange(period=(10, 250, 40), offset=(0.05, 0.95, 0.3),
+                               sigma=(2.0, 10.0, 4.0))
+    ref = O.alma_batch(z.astype(np.float64), _combos(sweep), nthreads=0)
+    try:
+        for policy, expect in (("mfma_f16x3", "mfma_f16x3"), ("auto", "fp32")):
+            alma.set_policy(policy)
+            g = alma.alma_batch_dev(z, sweep).to_host()
+            assert alma.selected_batch_kernel() == expect
+            nan_mask_equal(g, ref)
+            assert_close(g, ref, rtol=5e-6, atol=5e-6, what=f"noise {policy}")
+    finally:
+        alma.set_policy("auto")
+
+
+def test_auto_selects_checked_split_on_price_series(alma):
+    """AUTO on a positive price series with 16+ combos and periods <= 256 runs
+    the split-F16 sweep, within 1e-5 of the oracle with exact NaN prefixes."""
+    x = datagen.gbm_close(30_000, 0.015, seed=71)
+    x[:6] = np.nan
+    sweep = vta.AlmaBatchRange(period=(10, 250, 20), offset=(0.05, 0.95, 0.45),
+                               sigma=(2.0, 10.0, 4.0))
+    alma.set_policy("auto")
+    g = alma.alma_batch_dev(x, sweep).to_host()
+    assert alma.selected_batch_kernel() == "mfma_f16x3"
+    ref = O.alma_batch(x.astype(np.float64), _combos(sweep), nthreads=0)
+    nan_mask_equal(g, ref)
+    err = max_rel_err(g, ref)
+    print(f"alma auto (split-F16) max rel err {err:.3e}")
+    assert err <= 1e-5

@@ -20,7 +20,7 @@ def test_knob_range_checks():
     require_gpu()
     ctx = vta.HipContext(0)
     assert at.get_tuning(ctx, at.SCAN_WAVES) == 0
-    for knob, bad in ((at.ALMA_MFMA_VARIANT, 8), (at.SCAN_WAVES, 32), (at.FIR_GROUP, 257),
+    for knob, bad in ((at.ALMA_MFMA_VARIANT, 12), (at.SCAN_WAVES, 32), (at.FIR_GROUP, 257),
                       (at.MS_CHUNK, 63), (9, 1)):
         with pytest.raises(vta.VtaError):
             at.set_tuning(ctx, knob, bad)
@@ -30,7 +30,7 @@ def test_knob_range_checks():
 
 
 def test_alma_mfma_variants_and_autotune():
-    """All seven split-BF16 kernel variants meet 1e-5 against the oracle with
+    """All eleven split-BF16 kernel variants meet 1e-5 against the oracle with
     exact NaN prefixes; the autotuner picks one of them."""
     require_gpu()
     T = 20_000
@@ -45,7 +45,7 @@ def test_alma_mfma_variants_and_autotune():
     plan = vta.AlmaSweepPlan(alma, sweep, T, kernel="mfma")
     d_x = ctx.to_device(x)
     d_o = ctx.alloc(plan.n_combos * T * 4)
-    for v in range(1, 8):
+    for v in range(1, 12):
         at.set_tuning(ctx, at.ALMA_MFMA_VARIANT, v)
         plan.launch(d_x.ptr, 3, d_o.ptr)
         ctx.synchronize()
@@ -66,6 +66,46 @@ def test_alma_mfma_variants_and_autotune():
     plan.close()
     d_x.free()
     d_o.free()
+
+
+@pytest.mark.parametrize("kernel", ["mfma", "mfma_f16x3"])
+def test_sweep_diagonal_variants_ragged(kernel):
+    """Diagonal-form variants 8-11 on ragged series (a partial last tile, T % 4
+    == 0 and != 0, 37 leading NaNs, one interior NaN): each within 1e-5 of the
+    oracle with exact NaN prefixes; 9 (direct stores) and 11 (deferred stores)
+    share 8's arithmetic and are bit-identical to it (11 runs as 8 when
+    T % 4 != 0)."""
+    require_gpu()
+    sweep = vta.AlmaBatchRange(period=(10, 250, 30), offset=(0.05, 0.95, 0.3),
+                               sigma=(2.0, 10.0, 4.0))
+    combos = [(c.period, c.offset, c.sigma) for c in vta.expand_grid(sweep)]
+    alma = vta.CudaAlma(0)
+    ctx = alma.ctx
+    for T in (10_004, 10_003):
+        x = datagen.gbm_close(T, 0.015, seed=62)
+        x[:37] = np.nan
+        x[5000] = np.nan
+        ref = O.alma_batch(x.astype(np.float64), combos)
+        plan = vta.AlmaSweepPlan(alma, sweep, T, kernel=kernel)
+        d_x = ctx.to_device(x)
+        d_o = ctx.alloc(plan.n_combos * T * 4)
+        outs = {}
+        for v in (8, 9, 10, 11):
+            at.set_tuning(ctx, at.ALMA_MFMA_VARIANT, v)
+            plan.launch(d_x.ptr, 37, d_o.ptr)
+            ctx.synchronize()
+            g = d_o.copy_to_host(np.float32).reshape(plan.n_combos, T)
+            nan_mask_equal(g, ref)
+            err = max_rel_err(g, ref)
+            print(f"{kernel} T={T} variant {v}: max rel err {err:.3e}")
+            assert err <= 1e-5, (kernel, T, v, err)
+            outs[v] = g
+        for v in (9, 11):
+            assert np.array_equal(outs[8].view(np.uint32), outs[v].view(np.uint32)), (T, v)
+        at.set_tuning(ctx, at.ALMA_MFMA_VARIANT, 0)
+        plan.close()
+        d_x.free()
+        d_o.free()
 
 
 def test_scan_waves_and_ms_chunk_autotune():

@@ -37,8 +37,9 @@ class AlmaBatchRange:
                                RangeF64(*[float(v) for v in self.sigma]))
 
 
-_POLICY = {"auto": 0, "fp32": 1, "plain": 1, "mfma": 2, "mfma_bf16x3": 2}
-_SELECTED = {0: None, 1: "fp32", 2: "mfma_bf16x3"}
+_POLICY = {"auto": 0, "fp32": 1, "plain": 1, "mfma": 2, "mfma_bf16x3": 2, "mfma_f16x3": 3,
+           "f16x3": 3}
+_SELECTED = {0: None, 1: "fp32", 2: "mfma_bf16x3", 3: "mfma_f16x3"}
 
 
 def expand_grid(sweep: AlmaBatchRange) -> List[AlmaParams]:

@@ -203,6 +203,26 @@ struct alma_mfg_args_host {
 };
 static_assert(sizeof(alma_mfg_args_host) == 88, "alma_mfg_args layout");
 
+// Host mirror of struct alma_sw_args (the diagonal-form sweep kernels).
+struct alma_sw_args_host {
+    const float *prices;
+    const void *frags;
+    const void *nblocks;
+    const void *groups;
+    const int32_t *sorted_period;
+    const int32_t *sorted_row;
+    const float *w_f32;
+    const void *cscale;
+    float *out;
+    int series_len;
+    int first_valid;
+    int n_groups;
+    int n_tiles;
+    int kmax;
+    int group_base;
+};
+static_assert(sizeof(alma_sw_args_host) == 96, "alma_sw_args layout");
+
 struct vta_alma_sweep_plan {
     size_t series_len = 0;
     AlmaTables t;   // host copy of the tables and their layout
@@ -221,6 +241,27 @@ bool mf_shared_staging() {
 }
 constexpr size_t kMfMaxPeriod = 256;
 
+// Diagonal-form sweep kernels (alma_sweep_*): tiles per workgroup run, and the
+// variant the F16 form runs by default (direct stores: 299 us against 311 us
+// for the transposed stores on the 200,000 x 1,250 sweep at sustained clocks).
+constexpr int kSweepRunTiles = 16;
+constexpr int kSweepDefaultVariant = 9;
+
+int mfg_stride_host(int npair) { return ((npair + 15) >> 5) * 32 + 16; }
+
+// Dynamic LDS of alma_sweep_* (the layout in alma_sw_entry): prices, sub-run
+// anchors and scales, tile flags, block statistics, the staged sub-runs and,
+// with the transposed stores, 32 x 36 tiles: one per wave, or rs per wave
+// when the stores are deferred.
+size_t sweep_lds_bytes(int kmax, int rs, bool xpose, bool defer) {
+    const int nsub = kSweepRunTiles / rs;
+    const int xsn = 32 * kSweepRunTiles + kmax + 33;
+    const int head = (xsn + 3 * nsub + kSweepRunTiles + 96 + 3) & ~3;
+    const int stage = nsub * 4 * mfg_stride_host(kmax / 2 + 16 * rs);
+    const int tiles = defer ? 4 * rs : xpose ? 4 : 0;
+    return 4 * ((size_t)head + (size_t)stage + (size_t)tiles * 32 * 36);
+}
+
 bool mf_env_override() {
     return getenv("ALMA_MF_FORM") || getenv("ALMA_MF_TILES") || getenv("ALMA_MF_TJ") ||
            getenv("ALMA_MF_KSPLIT");
@@ -231,11 +272,13 @@ bool mf_env_override() {
 // (every variant timed in interleaved rounds): with at most 64 combos one wave
 // per N-block over 8 tiles (variant 5) is within 4% of the best everywhere;
 // up to 2^27 outputs the 8-tile workgroups (2) are 5-35% faster than the
-// 16-tile ones, whose staging pays off above that (1).
+// 16-tile ones, whose staging pays off above that.  Above 2^27 outputs the
+// diagonal form with direct stores (9) is the fastest: 250 us against 253
+// (8), 257 (11) and 259 (1) on the 200,000 x 1,250 sweep at sustained clocks.
 int mf_default_variant(size_t T, size_t C) {
     if (C <= 64) return 5;
     if ((double)T * (double)C <= 134217728.0) return 2;
-    return 1;
+    return 9;
 }
 
 // Appends one table to the blob at the next 256-byte boundary.
@@ -253,13 +296,44 @@ const P *section(const AlmaTables &t, const unsigned char *base, int sec) {
     return reinterpret_cast<const P *>(base + t.off[sec]);
 }
 
+// IEEE binary16 round-to-nearest-even of a double (|v| < 65520), as bits,
+// rounded once from the double (no f32 step), and its value.
+uint16_t f16_rne(double v) {
+    const uint16_t sign = std::signbit(v) ? 0x8000u : 0u;
+    const double a = std::fabs(v);
+    if (a == 0.0) return sign;
+    int e;
+    (void)std::frexp(a, &e);                   // a in [2^(e-1), 2^e)
+    const int E = std::max(e - 1, -14);        // subnormals share the 2^-24 quantum
+    const double q = std::ldexp(1.0, E - 10);
+    const double r = std::nearbyint(a / q) * q;  // ties to even (default rounding mode)
+    if (r == 0.0) return sign;                    // below half the smallest subnormal
+    int er;
+    (void)std::frexp(r, &er);
+    const int Er = er - 1;
+    if (Er < -14) return (uint16_t)(sign | (uint16_t)std::lround(r / std::ldexp(1.0, -24)));
+    const int mant = (int)std::lround((r / std::ldexp(1.0, Er) - 1.0) * 1024.0);
+    return (uint16_t)(sign | (uint16_t)((Er + 15) << 10) | (uint16_t)mant);
+}
+
+double f16_value(uint16_t h) {
+    const int ex = (h >> 10) & 0x1f, mant = h & 0x3ff;
+    const double mag = ex == 0 ? std::ldexp((double)mant, -24)
+                               : std::ldexp(1.0 + mant / 1024.0, ex - 15);
+    return (h & 0x8000u) ? -mag : mag;
+}
+
 // Builds the MFMA operand tables.  Combos are sorted by period (stable) and
 // cut into N-blocks of 32; block b contracts over K_b = round_up(max period of
 // the block, 16) taps with shorter periods zero padded at the front.  Each
-// weight is split hi = bf16(w), lo = bf16(w - hi) and stored in
-// v_mfma_f32_32x32x16_bf16 B-operand order: for K-step s and half h, lane l
-// holds taps 16s + 8(l/32) + j (j < 8) of combo l%32.
-void build_mfma_tables(AlmaTables &t, const std::vector<vta_alma_params> &combos) {
+// weight is split hi / lo and stored in v_mfma_f32_32x32x16 B-operand order:
+// for K-step s and half h, lane l holds taps 16s + 8(l/32) + j (j < 8) of
+// combo l%32.  BF16: hi = bf16(w), lo = bf16(w - hi).  F16: the combo's
+// weights are first scaled by 2^sw (largest in [0.5, 1)), hi = f16(w_s), lo =
+// f16(w_s - hi); cscale holds {2^-sw, (2^-22 sum w_s + sum |w_s - hi - lo| +
+// 2^-11 sum |lo|) 2^-sw} per sorted combo, the factor of the kernel's
+// split-error bound (rounded up).
+void build_mfma_tables(AlmaTables &t, const std::vector<vta_alma_params> &combos, bool f16) {
     const size_t C = combos.size();
     std::vector<size_t> order(C);
     std::iota(order.begin(), order.end(), 0);
@@ -270,6 +344,8 @@ void build_mfma_tables(AlmaTables &t, const std::vector<vta_alma_params> &combos
     std::vector<int32_t> sp(cpad, 1), sr(cpad, -1);
     std::vector<int32_t> nbd((size_t)nb * 4, 0);
     std::vector<uint16_t> frags;
+    std::vector<float> csc(2 * cpad, 0.0f);
+    for (size_t i = 0; i < cpad; ++i) csc[2 * i] = 1.0f;
     int kmax = 0;
     for (int b = 0; b < nb; ++b) {
         size_t maxp = 0;
@@ -282,15 +358,37 @@ void build_mfma_tables(AlmaTables &t, const std::vector<vta_alma_params> &combos
         nbd[b * 4 + 0] = K / 16;
         nbd[b * 4 + 1] = (int32_t)(frags.size() / 8);  // offset in 16-byte chunks
         nbd[b * 4 + 2] = b * 32;
-        // W[k][n], zero padded at the front
+        // W[k][n], zero padded at the front (F16: scaled by 2^sw per combo)
         std::vector<double> W((size_t)K * 32, 0.0), wd;
         for (int n = 0; n < 32; ++n) {
             size_t i = (size_t)b * 32 + n;
             if (i >= C) continue;
             const auto &cp = combos[order[i]];
             weights_f64(cp.period, cp.offset, cp.sigma, wd);
+            double scale = 1.0;
+            if (f16) {
+                const double wmax = *std::max_element(wd.begin(), wd.end());
+                int e;
+                (void)std::frexp(wmax, &e);            // wmax = f 2^e, f in [0.5, 1)
+                scale = std::ldexp(1.0, -e);
+                double sum_ws = 0.0, eps = 0.0, lsum = 0.0;
+                for (double w : wd) {
+                    const double ws = w * scale;
+                    const uint16_t hi = f16_rne(ws);
+                    const double hv = f16_value(hi);
+                    const uint16_t lo = f16_rne(ws - hv);
+                    const double lv = f16_value(lo);
+                    sum_ws += ws;
+                    eps += std::fabs(ws - hv - lv);
+                    lsum += std::fabs(lv);
+                }
+                const double bound =
+                    (std::ldexp(sum_ws, -22) + eps + std::ldexp(lsum, -11)) / scale;
+                csc[2 * i + 0] = (float)(1.0 / scale);
+                csc[2 * i + 1] = std::nextafter((float)(bound * (1.0 + 1e-6)), INFINITY);
+            }
             size_t lead = (size_t)K - cp.period;
-            for (size_t k = 0; k < cp.period; ++k) W[(lead + k) * 32 + n] = wd[k];
+            for (size_t k = 0; k < cp.period; ++k) W[(lead + k) * 32 + n] = wd[k] * scale;
             sp[i] = (int32_t)cp.period;
             sr[i] = (int32_t)order[i];
         }
@@ -300,8 +398,14 @@ void build_mfma_tables(AlmaTables &t, const std::vector<vta_alma_params> &combos
                     for (int j = 0; j < 8; ++j) {
                         int k = 16 * s + 8 * (l / 32) + j;
                         double w = W[(size_t)k * 32 + (l % 32)];
-                        uint16_t hi = bf16_rne((float)w);
-                        uint16_t lo = bf16_rne((float)(w - (double)bf16_to_f32(hi)));
+                        uint16_t hi, lo;
+                        if (f16) {
+                            hi = f16_rne(w);
+                            lo = f16_rne(w - f16_value(hi));
+                        } else {
+                            hi = bf16_rne((float)w);
+                            lo = bf16_rne((float)(w - (double)bf16_to_f32(hi)));
+                        }
                         frags.push_back(h == 0 ? hi : lo);
                     }
                 }
@@ -336,6 +440,7 @@ void build_mfma_tables(AlmaTables &t, const std::vector<vta_alma_params> &combos
     put_section(t, AlmaTables::SORTED_PERIOD, sp);
     put_section(t, AlmaTables::SORTED_ROW, sr);
     put_section(t, AlmaTables::W_F32, wf);
+    if (f16) put_section(t, AlmaTables::CSCALE, csc);
     t.n_nblocks = nb;
     t.kmax = kmax;
 }
@@ -343,8 +448,9 @@ void build_mfma_tables(AlmaTables &t, const std::vector<vta_alma_params> &combos
 vta_alma_batch_kernel resolve_kernel(vta_alma_batch_kernel policy, size_t n_combos,
                                      size_t max_period) {
     if (policy != VTA_ALMA_BATCH_AUTO) return policy;
-    // The matrix-core form needs wide sweeps to fill 32-combo N-blocks.
-    if (n_combos >= 16 && max_period <= kMfMaxPeriod) return VTA_ALMA_BATCH_MFMA_BF16X3;
+    // The matrix-core form needs wide sweeps to fill 32-combo N-blocks; its
+    // split-F16 form carries a checked error bound.
+    if (n_combos >= 16 && max_period <= kMfMaxPeriod) return VTA_ALMA_BATCH_MFMA_F16X3;
     return VTA_ALMA_BATCH_FP32;
 }
 
@@ -354,12 +460,12 @@ vta_status tables_build(const std::vector<vta_alma_params> &combos, size_t max_p
     t.kernel = kernel;
     t.n_combos = combos.size();
     t.max_period = max_period;
-    if (kernel == VTA_ALMA_BATCH_MFMA_BF16X3) {
+    if (kernel == VTA_ALMA_BATCH_MFMA_BF16X3 || kernel == VTA_ALMA_BATCH_MFMA_F16X3) {
         if (max_period > kMfMaxPeriod)
             return fail(VTA_ERR_INVALID_POLICY,
                         "invalid policy: MFMA sweep supports periods <= %zu (got %zu)",
                         kMfMaxPeriod, max_period);
-        build_mfma_tables(t, combos);
+        build_mfma_tables(t, combos, kernel == VTA_ALMA_BATCH_MFMA_F16X3);
         return VTA_OK;
     }
     if (kernel != VTA_ALMA_BATCH_FP32) return fail(VTA_ERR_INVALID_POLICY, "invalid policy");
@@ -397,6 +503,45 @@ vta_status tables_launch(vta_ctx *ctx, const AlmaTables &t, const unsigned char 
     // knobs, else the shared-staging form with 16 tiles
     int variant = (int)ctx->tune[VTA_TUNE_ALMA_MFMA_VARIANT];
     if (variant == 0 && !mf_env_override()) variant = mf_default_variant(T, t.n_combos);
+    const bool f16 = t.kernel == VTA_ALMA_BATCH_MFMA_F16X3;
+    if (f16 && variant < 8) variant = kSweepDefaultVariant;   // F16 runs the diagonal form only
+    // the deferred-store kernels write whole float4s: rows must be 16-byte aligned
+    if (variant == 11 && (T & 3) != 0) variant = 8;
+    if ((t.kernel == VTA_ALMA_BATCH_MFMA_BF16X3 || f16) && variant >= 8) {
+        // diagonal form: 4 N-blocks x 16 tiles per workgroup, sub-runs of RS tiles
+        const int rs = variant == 10 ? 4 : 2;
+        const bool xpose = variant == 8 || variant == 11;
+        const bool defer = variant == 11;
+        static const char *const kNames[2][4] = {
+            {"alma_sweep_bf16x3", "alma_sweep_bf16x3_d", "alma_sweep_bf16x3_r4",
+             "alma_sweep_bf16x3_p"},
+            {"alma_sweep_f16x3", "alma_sweep_f16x3_d", "alma_sweep_f16x3_r4", "alma_sweep_f16x3_p"}};
+        alma_sw_args_host a{};
+        a.prices = d_prices;
+        a.frags = section<void>(t, base, AlmaTables::FRAGS);
+        a.nblocks = section<void>(t, base, AlmaTables::NBLOCKS);
+        a.groups = section<void>(t, base, AlmaTables::GROUPS);
+        a.sorted_period = section<int32_t>(t, base, AlmaTables::SORTED_PERIOD);
+        a.sorted_row = section<int32_t>(t, base, AlmaTables::SORTED_ROW);
+        a.w_f32 = section<float>(t, base, AlmaTables::W_F32);
+        a.cscale = f16 ? section<void>(t, base, AlmaTables::CSCALE) : nullptr;
+        a.out = d_out;
+        a.series_len = T;
+        a.first_valid = (int)fv;
+        a.n_tiles = (T + 31) / 32;
+        a.kmax = t.kmax;
+        a.group_base = 0;
+        a.n_groups = t.n_groups;
+        const size_t smem = sweep_lds_bytes(t.kmax, rs, xpose, defer);
+        if (smem > ctx->props.sharedMemPerBlock)
+            return fail(VTA_ERR_LAUNCH_CONFIG_TOO_LARGE, "MFMA staging needs %zu bytes of LDS",
+                        smem);
+        hipFunction_t fn;
+        VTA_TRY(get_function(ctx, "alma", kNames[f16 ? 1 : 0][variant - 8], &fn));
+        ctx->alma_selected = t.kernel;
+        const unsigned gx = (unsigned)((a.n_tiles + kSweepRunTiles - 1) / kSweepRunTiles);
+        return launch(ctx, fn, dim3(gx, (unsigned)t.n_groups, 1), dim3(256), (unsigned)smem, a);
+    }
     const bool shared = variant == 0 ? mf_shared_staging() : variant <= 4;
     if (t.kernel == VTA_ALMA_BATCH_MFMA_BF16X3 && shared) {
         // shared-staging form: 4 N-blocks per workgroup, R tiles per run
@@ -516,14 +661,32 @@ vta_status plan_launch(vta_ctx *ctx, const vta_alma_sweep_plan *plan, const floa
     return tables_launch(ctx, plan->t, plan->dev.as<unsigned char>(), d_prices, fv, d_out,
                          series_len ? series_len : plan->series_len);
 }
-// run_batch_with_prices_device (alma_wrapper.rs:683-781).
+// True when the finite bars of a host series take both signs: the outputs
+// then cross zero, where the F16 form's error check would send most tiles to
+// the FP32 dot, so AUTO takes the FP32 kernel directly.
+bool changes_sign(const float *x, size_t n, size_t fv) {
+    bool neg = false, pos = false;
+    for (size_t i = fv; i < n; ++i) {
+        if (!std::isfinite(x[i])) continue;
+        neg = neg || x[i] < 0.0f;
+        pos = pos || x[i] > 0.0f;
+        if (neg && pos) return true;
+    }
+    return false;
+}
+
+// run_batch_with_prices_device (alma_wrapper.rs:683-781).  host_prices (may
+// be NULL) lets AUTO see the series.
 vta_status run_batch(vta_ctx *ctx, const float *d_prices, size_t T, size_t fv,
                      const std::vector<vta_alma_params> &combos, size_t max_period,
-                     vta_device_array_f32 *out) {
+                     vta_device_array_f32 *out, const float *host_prices = nullptr) {
     size_t n_out;
     if (!mul_ok(combos.size(), T, &n_out) || !mul_ok(n_out, sizeof(float), &n_out))
         return fail(VTA_ERR_INVALID_INPUT, "n_combos*series_len overflow");
     vta_alma_batch_kernel k = resolve_kernel(ctx->alma_policy, combos.size(), max_period);
+    if (ctx->alma_policy == VTA_ALMA_BATCH_AUTO && k != VTA_ALMA_BATCH_FP32 && host_prices &&
+        changes_sign(host_prices, T, fv))
+        k = VTA_ALMA_BATCH_FP32;
     vta_alma_sweep_plan *plan = nullptr;
     VTA_TRY(plan_create(ctx, combos, T, max_period, k, &plan));
     DevBuf dout;
@@ -621,7 +784,7 @@ vta_status vta_alma_expand_grid(const vta_alma_batch_range *range, vta_alma_para
 
 vta_status vta_alma_set_batch_policy(vta_ctx *ctx, vta_alma_batch_kernel policy) {
     if (!ctx) return fail(VTA_ERR_INVALID_INPUT, "ctx is null");
-    if (policy < VTA_ALMA_BATCH_AUTO || policy > VTA_ALMA_BATCH_MFMA_BF16X3)
+    if (policy < VTA_ALMA_BATCH_AUTO || policy > VTA_ALMA_BATCH_MFMA_F16X3)
         return fail(VTA_ERR_INVALID_POLICY, "invalid policy: %d", (int)policy);
     ctx->alma_policy = policy;
     return VTA_OK;
@@ -644,7 +807,7 @@ vta_status vta_alma_batch_dev(vta_ctx *ctx, const float *prices, size_t len,
     VTA_TRY(check_fits(req));
     DevBuf dp;
     VTA_TRY(upload(ctx, dp, prices, len * sizeof(float)));
-    return run_batch(ctx, dp.as<float>(), len, fv, combos, mp, out);
+    return run_batch(ctx, dp.as<float>(), len, fv, combos, mp, out, prices);
 }
 
 vta_status vta_alma_batch_from_device_prices(vta_ctx *ctx, const float *d_prices, size_t len,
@@ -678,7 +841,7 @@ vta_status vta_alma_batch_into_host_f32(vta_ctx *ctx, const float *prices, size_
     DevBuf dp;
     VTA_TRY(upload(ctx, dp, prices, len * sizeof(float)));
     vta_device_array_f32 arr{};
-    VTA_TRY(run_batch(ctx, dp.as<float>(), len, fv, combos, mp, &arr));
+    VTA_TRY(run_batch(ctx, dp.as<float>(), len, fv, combos, mp, &arr, prices));
     vta_status st = vta_device_array_copy_to_host(ctx, &arr, out, out_len);
     vta_device_array_free(ctx, &arr);
     return st;

@@ -917,3 +917,594 @@ extern "C" __global__ __launch_bounds__(256) void alma_ms1p_f32_direct(
     }
     out_tm[i] = r;
 }
+
+// ---------------------------------------------------------------- MFMA sweep, diagonal form
+//
+// The same split contraction, organised so that every A fragment loaded from
+// LDS feeds several tiles.  A workgroup of four waves takes up to four
+// N-blocks of one group over a run of SW_R 32-bar tiles, cut into sub-runs of
+// SW_RS tiles.  Each sub-run is staged once, with one anchor for its whole
+// window (K_g + 32 SW_RS bars), instead of once per tile.  In the Hankel
+// tile, the A fragment of tile j at K-step s is G(2j + s), the 16-tap slice n
+// of the sub-run window (lane l: elements 16n + l%32 + 8(l/32) .. +7 from the
+// window start).  A wave walks n = 0 .. KS + 2 SW_RS - 3, loads G(n) once
+// (hi and lo) and applies it to every tile j with s = n - 2j in [0, KS),
+// accumulating tile j in its own accumulator: SW_RS accumulators, B fragments
+// (the N-block's weights) in registers for the whole run, and consecutive
+// MFMAs on different accumulators.
+//
+// Element formats (template F16):
+//   BF16 (false): x' = x - a, a = midpoint of the window's finite bars; hi =
+//     bf16(x'), lo = bf16(x' - hi); weights unscaled.  The north-star form;
+//     its split error is bounded by ~3 * 2^-16 * sum |w x'| relative terms.
+//   F16 (true): x' = (x - a) * 2^-e with |x - a| < 2^e over the window, so
+//     |x'| < 1, and each combo's weights scaled by 2^sw (largest in [0.5, 1));
+//     hi/lo in half precision (11-bit significands).  Per output the split
+//     error is at most (2^-22 sum w_s + eps_w + 2^-11 L_w) 2^-(sw + e) in
+//     output units (eps_w = sum |w_s - hi - lo|, L_w = sum |lo| of the scaled
+//     weights: the x residual |x' - hi - lo| <= 2^-22 for |x'| < 1, the weight
+//     residual, and the dropped lo*lo product); the host folds it into one
+//     factor per combo, cscale.y, so the bound is cscale.y * 2^e.  A tile whose
+//     bound exceeds SW_GUARD * |y| for any of its outputs is recomputed with the
+//     plain FP32 dot, so the split never adds more than SW_GUARD relative error.
+// Tiles with a non-finite input at or after first_valid in their window are
+// recomputed the same way.
+#define SW_R 16
+#define SW_GUARD 2.0e-6f
+
+typedef __attribute__((ext_vector_type(8))) _Float16 vta_f16x8;
+
+struct alma_sw_args {
+    const float *prices;
+    const uint4 *frags;          // [sum_b ks_b*2*64] 16-byte lane chunks (hi, lo per K-step)
+    const int4 *nblocks;         // {ks, frag_off, combo_base, 0}
+    const int4 *groups;          // {first N-block, count (<= 4), ks of the group, 0}
+    const int32_t *sorted_period;
+    const int32_t *sorted_row;
+    const float *w_f32;          // [cpad][kmax] f32 weights (fallback path)
+    const float2 *cscale;        // F16: per sorted combo {2^-sw, split-error factor}
+    float *out;
+    int series_len;
+    int first_valid;
+    int n_groups;
+    int n_tiles;
+    int kmax;
+    int group_base;
+};
+
+// dwords per staged array of a sub-run of RS tiles: kg/2 + 16 RS pairs, padded
+// to 16 mod 32 so the two copies a wave reads in one instruction fall on
+// disjoint banks
+__device__ __forceinline__ int sw_stride(int kg, int rs) { return mfg_stride(kg / 2 + 16 * rs); }
+
+template <bool F16>
+__device__ __forceinline__ uint32_t sw_pack2(float v0, float v1, uint32_t *lo_pair) {
+    if constexpr (F16) {
+        const _Float16 h0 = (_Float16)v0, h1 = (_Float16)v1;
+        const _Float16 l0 = (_Float16)(v0 - (float)h0), l1 = (_Float16)(v1 - (float)h1);
+        *lo_pair = (uint32_t)__builtin_bit_cast(uint16_t, l0) |
+                   ((uint32_t)__builtin_bit_cast(uint16_t, l1) << 16);
+        return (uint32_t)__builtin_bit_cast(uint16_t, h0) |
+               ((uint32_t)__builtin_bit_cast(uint16_t, h1) << 16);
+    } else {
+        const uint16_t h0 = vta_bf16_rne(v0), h1 = vta_bf16_rne(v1);
+        const uint16_t l0 = vta_bf16_rne(v0 - vta_bf16_to_f32(h0));
+        const uint16_t l1 = vta_bf16_rne(v1 - vta_bf16_to_f32(h1));
+        *lo_pair = mf_pack(l0, l1);
+        return mf_pack(h0, h1);
+    }
+}
+
+template <bool F16>
+__device__ __forceinline__ vta_f32x16 sw_mfma(const uint4 &a, const uint4 &b, const vta_f32x16 &c) {
+    if constexpr (F16)
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(vta_f16x8, a),
+                                                      __builtin_bit_cast(vta_f16x8, b), c, 0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(vta_bf16x8, a),
+                                                       __builtin_bit_cast(vta_bf16x8, b), c, 0, 0,
+                                                       0);
+}
+
+// Stores (or recomputes) one 32-bar tile of this wave's N-block from its
+// accumulator: lane (m, g) holds combo m, bars 8q + 4g + i.
+template <bool F16, bool XPOSE>
+__device__ __forceinline__ void sw_epilogue(const alma_sw_args &a, const vta_f32x16 &acc,
+                                            int t_tile, float anchor, float unscale,
+                                            float cmul, float cguard, bool tile_bad,
+                                            int combo, int my_period, int my_row, int my_warm,
+                                            int warm_max, float *tbuf, const int *trow) {
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 5;
+    const int m = lane & 31;
+    const int T = a.series_len;
+    const float f = F16 ? cmul * unscale : 1.0f;
+    float y[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) y[i] = fmaf(acc[i], f, anchor);
+    bool redo = tile_bad;
+    if (F16 && !redo) {
+        // split-error guard: bound <= SW_GUARD |y| for every stored output
+        const float bound = cguard * unscale;
+        bool viol = false;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int t = t_tile + 8 * q + 4 * g + i;
+                viol = viol || (my_row >= 0 && t >= my_warm && t < T &&
+                                !(bound <= SW_GUARD * fabsf(y[4 * q + i])));
+            }
+        redo = __any(viol);
+    }
+    const bool steady = !redo && t_tile >= warm_max && t_tile + 32 <= T && (T & 3) == 0;
+    if (steady && XPOSE) {
+        // transpose through the wave's LDS tile: each store instruction then
+        // writes 8 combos x 128 contiguous bytes
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            *reinterpret_cast<float4 *>(tbuf + m * 36 + 8 * q + 4 * g) =
+                make_float4(y[4 * q + 0], y[4 * q + 1], y[4 * q + 2], y[4 * q + 3]);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int tcol = 4 * (lane & 7);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const float4 v =
+                *reinterpret_cast<const float4 *>(tbuf + ((lane >> 3) + 8 * s) * 36 + tcol);
+            if (trow[s] >= 0)
+                *reinterpret_cast<float4 *>(a.out + (size_t)trow[s] * (size_t)T + t_tile + tcol) =
+                    v;
+        }
+        return;
+    }
+    if (my_row < 0) return;
+    float *row = a.out + (size_t)my_row * (size_t)T;
+    if (steady) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            *reinterpret_cast<float4 *>(row + t_tile + 8 * q + 4 * g) =
+                make_float4(y[4 * q + 0], y[4 * q + 1], y[4 * q + 2], y[4 * q + 3]);
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int tb = t_tile + 8 * q + 4 * g;
+        float v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int t = tb + i;
+            float val = y[4 * q + i];
+            if (redo && t >= my_warm && t < T) {
+                // plain FP32 dot, same taps as the FP32 kernel
+                const float *wrow = a.w_f32 + (size_t)combo * a.kmax;
+                float dot = 0.0f;
+                for (int k = 0; k < my_period; ++k)
+                    dot = fmaf(wrow[k], a.prices[t - my_period + 1 + k], dot);
+                val = dot;
+            }
+            v[i] = (t >= my_warm) ? val : VTA_QNAN_F32;
+        }
+        if (((T & 3) == 0) && tb + 3 < T) {
+            *reinterpret_cast<float4 *>(row + tb) = make_float4(v[0], v[1], v[2], v[3]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (tb + i < T) row[tb + i] = v[i];
+        }
+    }
+}
+
+// Deferred-store form of one wave's run (DEFER kernels): the outputs of
+// sub-run q are written, transposed, to the wave's RS LDS tiles at its end,
+// and their global stores (4 RS float4 per lane: tile j, rows 8 s + lane / 8,
+// bars 4 (lane % 8) .. + 3) are spread over the MFMA diagonals of sub-run
+// q + 1, so the stores drain while the matrix cores run; the last sub-run's
+// are issued after the loop.  Every tile goes through LDS: warm-up and
+// recomputed tiles get their values in the epilogue, and a store is issued
+// only for rows of real combos and bars below T (T % 4 == 0, so a float4 is
+// wholly in or out).
+template <bool F16, int RS, int KS>
+__device__ __forceinline__ void sw_wave_deferred(const alma_sw_args &a, int nb, int tile_begin,
+                                                 int nt, int kg, int stride,
+                                                 const uint32_t *stage, const float *sub,
+                                                 const int *badf, float *tbuf) {
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 5;
+    const int m = lane & 31;
+    const int T = a.series_len;
+    const int4 nbd = a.nblocks[nb];
+    const int frag_off = nbd.y;
+    const int combo_base = nbd.z;
+    constexpr int K = KS * 16;
+    constexpr int NG = KS + 2 * RS - 2;
+    constexpr int NOPS = 4 * RS;
+    constexpr int TSLOT = 32 * 36;
+    uint4 bh[KS], bl[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        bh[s] = a.frags[frag_off + (2 * s + 0) * 64 + lane];
+        bl[s] = a.frags[frag_off + (2 * s + 1) * 64 + lane];
+    }
+    const int combo = combo_base + m;
+    const int my_period = a.sorted_period[combo];
+    const int my_row = a.sorted_row[combo];
+    const int my_warm = a.first_valid + my_period - 1;
+    float cmul = 1.0f, cguard = 0.0f;
+    if constexpr (F16) {
+        const float2 cs = a.cscale[combo];
+        cmul = cs.x;
+        cguard = cs.y;
+    }
+    int warm_max = my_warm;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) warm_max = max(warm_max, __shfl_xor(warm_max, o));
+    warm_max = __builtin_amdgcn_readfirstlane(warm_max);
+    // row pointers of the transposed stores (null for padding combos)
+    const int tcol = 4 * (lane & 7);
+    float *rowp[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int r = __shfl(my_row, (lane >> 3) + 8 * s);
+        rowp[s] = r >= 0 ? a.out + (size_t)r * (size_t)T + tcol : nullptr;
+    }
+    const int r = m & 1;
+    const int d0 = (m >> 1) + 4 * g + ((kg - K) >> 1);
+    const int hi_off = (r ? stride : 0) + d0;
+    const int lo_off = 2 * stride + hi_off;
+    const int nsub = (nt + RS - 1) / RS;
+    int pend_t = 1 << 30;            // first bar of the pending sub-run (none yet)
+    auto store_op = [&](int o) {
+        const int j = o / 4, s = o % 4;
+        const float4 v =
+            *reinterpret_cast<const float4 *>(tbuf + j * TSLOT + ((lane >> 3) + 8 * s) * 36 + tcol);
+        const int t = pend_t + 32 * j;
+        if (rowp[s] != nullptr && t + tcol < T) *reinterpret_cast<float4 *>(rowp[s] + t) = v;
+    };
+    for (int q = 0; q < nsub; ++q) {
+        const uint32_t *sb = stage + q * 4 * stride;
+        auto load_g = [&](int n, uint4 &h, uint4 &l) {
+            const uint32_t *hb = sb + hi_off + 8 * n;
+            const uint32_t *lb = sb + lo_off + 8 * n;
+            h.x = hb[0]; h.y = hb[1]; h.z = hb[2]; h.w = hb[3];
+            l.x = lb[0]; l.y = lb[1]; l.z = lb[2]; l.w = lb[3];
+        };
+        vta_f32x16 acc[RS];
+#pragma unroll
+        for (int j = 0; j < RS; ++j) acc[j] = vta_f32x16{};
+        uint4 gh, gl, nh, nl;
+        load_g(0, gh, gl);
+#pragma unroll
+        for (int n = 0; n < NG; ++n) {
+            if (n + 1 < NG) load_g(n + 1, nh, nl);
+#pragma unroll
+            for (int o = NOPS * n / NG; o < NOPS * (n + 1) / NG; ++o) store_op(o);
+#pragma unroll
+            for (int j = 0; j < RS; ++j) {
+                const int s = n - 2 * j;
+                if (s >= 0 && s < KS) acc[j] = sw_mfma<F16>(gh, bh[s], acc[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < RS; ++j) {
+                const int s = n - 2 * j;
+                if (s >= 0 && s < KS) acc[j] = sw_mfma<F16>(gh, bl[s], acc[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < RS; ++j) {
+                const int s = n - 2 * j;
+                if (s >= 0 && s < KS) acc[j] = sw_mfma<F16>(gl, bh[s], acc[j]);
+            }
+            gh = nh;
+            gl = nl;
+        }
+        const float anchor = sub[3 * q + 0];
+        const float unscale = sub[3 * q + 2];
+        const float f = F16 ? cmul * unscale : 1.0f;
+        // the pending stores have read their LDS tiles: overwrite them
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int j = 0; j < RS; ++j) {
+            const int tile = q * RS + j;
+            if (tile >= nt) break;
+            const int t_tile = (tile_begin + tile) * 32;
+            float y[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) y[i] = fmaf(acc[j][i], f, anchor);
+            bool redo = badf[tile] != 0;
+            if (F16 && !redo) {
+                const float bound = cguard * unscale;
+                bool viol = false;
+#pragma unroll
+                for (int qq = 0; qq < 4; ++qq)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int t = t_tile + 8 * qq + 4 * g + i;
+                        viol = viol || (my_row >= 0 && t >= my_warm && t < T &&
+                                        !(bound <= SW_GUARD * fabsf(y[4 * qq + i])));
+                    }
+                redo = __any(viol);
+            }
+            if (redo || t_tile < warm_max) {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const int t = t_tile + 8 * (i >> 2) + 4 * g + (i & 3);
+                    if (redo && my_row >= 0 && t >= my_warm && t < T) {
+                        // plain FP32 dot, same taps as the FP32 kernel
+                        const float *wrow = a.w_f32 + (size_t)combo * a.kmax;
+                        float dot = 0.0f;
+                        for (int k = 0; k < my_period; ++k)
+                            dot = fmaf(wrow[k], a.prices[t - my_period + 1 + k], dot);
+                        y[i] = dot;
+                    }
+                    if (t < my_warm) y[i] = VTA_QNAN_F32;
+                }
+            }
+#pragma unroll
+            for (int qq = 0; qq < 4; ++qq)
+                *reinterpret_cast<float4 *>(tbuf + j * TSLOT + m * 36 + 8 * qq + 4 * g) =
+                    make_float4(y[4 * qq + 0], y[4 * qq + 1], y[4 * qq + 2], y[4 * qq + 3]);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        pend_t = (tile_begin + q * RS) * 32;
+    }
+#pragma unroll
+    for (int o = 0; o < NOPS; ++o) store_op(o);
+}
+
+template <bool F16, bool XPOSE, int RS, int KS>
+__device__ __forceinline__ void sw_wave(const alma_sw_args &a, int nb, int tile_begin, int nt,
+                                        int kg, int stride, const uint32_t *stage,
+                                        const float *sub, const int *badf, float *tbuf) {
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 5;
+    const int m = lane & 31;
+    const int4 nbd = a.nblocks[nb];
+    const int frag_off = nbd.y;
+    const int combo_base = nbd.z;
+    constexpr int K = KS * 16;
+    constexpr int NG = KS + 2 * RS - 2;     // G(n), n < NG, per sub-run
+    uint4 bh[KS], bl[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        bh[s] = a.frags[frag_off + (2 * s + 0) * 64 + lane];
+        bl[s] = a.frags[frag_off + (2 * s + 1) * 64 + lane];
+    }
+    const int combo = combo_base + m;
+    const int my_period = a.sorted_period[combo];
+    const int my_row = a.sorted_row[combo];
+    const int my_warm = a.first_valid + my_period - 1;
+    float cmul = 1.0f, cguard = 0.0f;
+    if constexpr (F16) {
+        const float2 cs = a.cscale[combo];
+        cmul = cs.x;
+        cguard = cs.y;
+    }
+    int warm_max = my_warm;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) warm_max = max(warm_max, __shfl_xor(warm_max, o));
+    warm_max = __builtin_amdgcn_readfirstlane(warm_max);
+    int trow[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) trow[s] = __shfl(my_row, (lane >> 3) + 8 * s);
+    // lane's dword offset inside a staged copy at n = 0
+    const int r = m & 1;
+    const int d0 = (m >> 1) + 4 * g + ((kg - K) >> 1);
+    const int hi_off = (r ? stride : 0) + d0;
+    const int lo_off = 2 * stride + hi_off;
+    const int nsub = (nt + RS - 1) / RS;
+    for (int q = 0; q < nsub; ++q) {
+        const uint32_t *sb = stage + q * 4 * stride;
+        auto load_g = [&](int n, uint4 &h, uint4 &l) {
+            const uint32_t *hb = sb + hi_off + 8 * n;
+            const uint32_t *lb = sb + lo_off + 8 * n;
+            h.x = hb[0]; h.y = hb[1]; h.z = hb[2]; h.w = hb[3];
+            l.x = lb[0]; l.y = lb[1]; l.z = lb[2]; l.w = lb[3];
+        };
+        vta_f32x16 acc[RS];
+#pragma unroll
+        for (int j = 0; j < RS; ++j) acc[j] = vta_f32x16{};
+        uint4 gh, gl, nh, nl;
+        load_g(0, gh, gl);
+#pragma unroll
+        for (int n = 0; n < NG; ++n) {
+            if (n + 1 < NG) load_g(n + 1, nh, nl);
+            // products ordered hi*hi, hi*lo, lo*hi across the tiles of this
+            // diagonal, so consecutive MFMAs write different accumulators
+#pragma unroll
+            for (int j = 0; j < RS; ++j) {
+                const int s = n - 2 * j;
+                if (s >= 0 && s < KS) acc[j] = sw_mfma<F16>(gh, bh[s], acc[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < RS; ++j) {
+                const int s = n - 2 * j;
+                if (s >= 0 && s < KS) acc[j] = sw_mfma<F16>(gh, bl[s], acc[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < RS; ++j) {
+                const int s = n - 2 * j;
+                if (s >= 0 && s < KS) acc[j] = sw_mfma<F16>(gl, bh[s], acc[j]);
+            }
+            gh = nh;
+            gl = nl;
+        }
+        const float anchor = sub[3 * q + 0];
+        const float unscale = sub[3 * q + 2];
+#pragma unroll
+        for (int j = 0; j < RS; ++j) {
+            const int tile = q * RS + j;
+            if (tile >= nt) break;
+            sw_epilogue<F16, XPOSE>(a, acc[j], (tile_begin + tile) * 32, anchor, unscale, cmul,
+                                    cguard, badf[tile] != 0, combo, my_period, my_row, my_warm,
+                                    warm_max, tbuf, trow);
+        }
+    }
+}
+
+// grid.x = runs of SW_R tiles, grid.y = N-block groups (heaviest first);
+// block = 256 (4 waves).  Dynamic LDS, in floats: xs (32 SW_R + kmax + 33
+// prices), 3 per sub-run (anchor, scale, unscale), SW_R tile flags, 3 x 32
+// block statistics, then SW_NSUB x 4 x sw_stride(kmax) staged dwords (16-byte
+// aligned) and, for XPOSE, four 32 x 36 transpose tiles.
+template <bool F16, bool XPOSE, int RS, bool DEFER>
+__device__ __forceinline__ void alma_sw_entry(const alma_sw_args &a) {
+    constexpr int NSUB = SW_R / RS;
+    extern __shared__ __attribute__((aligned(16))) uint32_t sw_lds[];
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
+    const int lane = tid & 63;
+    const int4 gd = a.groups[a.group_base + a.n_groups - 1 - (int)blockIdx.y];
+    const int kg = gd.z * 16;
+    const int stride = sw_stride(kg, RS);
+    const int xsn = 32 * SW_R + a.kmax + 33;
+    float *xs = reinterpret_cast<float *>(sw_lds);
+    float *sub = xs + xsn;                          // 3 per sub-run
+    int *badf = reinterpret_cast<int *>(sub + 3 * NSUB);
+    float *bmin = reinterpret_cast<float *>(badf + SW_R);
+    float *bmax = bmin + 32;
+    int *bnf = reinterpret_cast<int *>(bmax + 32);
+    const int head = xsn + 3 * NSUB + SW_R + 96;
+    uint32_t *stage = sw_lds + ((head + 3) & ~3);
+    float *tbuf = reinterpret_cast<float *>(stage) + NSUB * 4 * sw_stride(a.kmax, RS) +
+                  wave * (DEFER ? RS : 1) * 32 * 36;
+    const int T = a.series_len;
+    const int fv = a.first_valid;
+    const int tile_begin = blockIdx.x * SW_R;
+    const int nt = min(SW_R, a.n_tiles - tile_begin);
+    const int t_lo = tile_begin * 32 - kg + 1;      // xs index 0
+    const int nx = nt * 32 + kg + 1;
+    for (int i = tid; i < nx; i += 256) {
+        const int idx = t_lo + i;
+        const float x = vta_ld_clamped(a.prices, idx, fv, T);
+        xs[i] = (idx >= fv && idx < T) ? x : 0.0f;
+    }
+    __syncthreads();
+    // statistics of 32-bar blocks of xs: min / max of the finite in-range bars,
+    // any non-finite in-range bar (lanes 0-31 and 32-63 take one block each)
+    const int nblk = (nx + 31) >> 5;                // <= SW_R + kmax / 32 + 1 <= 25
+    for (int b = 2 * wave + (lane >> 5); b < nblk; b += 8) {
+        const int i = 32 * b + (lane & 31);
+        const int idx = t_lo + i;
+        const bool in = i < nx && idx >= fv && idx < T;
+        const float x = in ? xs[i] : 0.0f;
+        const bool fin = in && vta_isfinite(x);
+        float mn = fin ? x : __builtin_inff(), mx = fin ? x : -__builtin_inff();
+        int nf = (in && !fin) ? 1 : 0;
+#pragma unroll
+        for (int o = 16; o > 0; o >>= 1) {
+            mn = fminf(mn, __shfl_xor(mn, o));
+            mx = fmaxf(mx, __shfl_xor(mx, o));
+            nf |= __shfl_xor(nf, o);
+        }
+        if ((lane & 31) == 0) {
+            bmin[b] = mn;
+            bmax[b] = mx;
+            bnf[b] = nf;
+        }
+    }
+    __syncthreads();
+    if (tid < nt) {
+        // tile flag: a non-finite bar in the tile's window xs[32 j, 32 j + kg + 31)
+        int bad = 0;
+        for (int b = tid; b <= min(tid + (kg + 30) / 32, nblk - 1); ++b) bad |= bnf[b];
+        badf[tid] = bad;
+    } else if (tid >= 64 && tid < 64 + NSUB) {
+        // sub-run q: window xs[32 RS q, 32 RS q + kg + 32 RS + 1): anchor at the
+        // midpoint of its finite bars and (F16) the power of two bounding |x - a|
+        const int q = tid - 64;
+        float mn = __builtin_inff(), mx = -__builtin_inff();
+        for (int b = RS * q; b <= min(RS * q + (kg + 32 * RS) / 32, nblk - 1); ++b) {
+            mn = fminf(mn, bmin[b]);
+            mx = fmaxf(mx, bmax[b]);
+        }
+        float anchor = 0.0f, scale = 1.0f, unscale = 1.0f;
+        if (mn <= mx) {
+            anchor = mn + 0.5f * (mx - mn);
+            if (!vta_isfinite(anchor)) anchor = 0.0f;
+            if constexpr (F16) {
+                const float h = fmaxf(fabsf(mn - anchor), fabsf(mx - anchor));
+                if (h > 0.0f && vta_isfinite(h)) {
+                    int e;
+                    (void)frexpf(h, &e);            // h = f 2^e, f in [0.5, 1)
+                    scale = __builtin_ldexpf(1.0f, -e);
+                    unscale = __builtin_ldexpf(1.0f, e);
+                }
+            }
+        }
+        sub[3 * q + 0] = anchor;
+        sub[3 * q + 1] = scale;
+        sub[3 * q + 2] = unscale;
+    }
+    __syncthreads();
+    // staging: pair i of sub-run q, copy 0 = (x'[2i], x'[2i+1]), copy 1 =
+    // (x'[2i+1], x'[2i+2]), x'[e] from xs[32 RS q + e]
+    const int npair = kg / 2 + 16 * RS;
+    const int nsub = (nt + RS - 1) / RS;
+    const int units = nsub * npair;
+    for (int u = tid; u < units; u += 256) {
+        const int q = u / npair;
+        const int i = u - q * npair;
+        const float anchor = sub[3 * q + 0];
+        const float scale = sub[3 * q + 1];
+        const int x0i = 32 * RS * q + 2 * i;
+        float v[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int xi = x0i + k;
+            const int idx = t_lo + xi;
+            const float x = xi < nx ? xs[xi] : 0.0f;
+            const bool use = xi < nx && idx >= fv && idx < T && vta_isfinite(x);
+            v[k] = use ? (x - anchor) * scale : 0.0f;
+        }
+        uint32_t l01, l12;
+        const uint32_t h01 = sw_pack2<F16>(v[0], v[1], &l01);
+        const uint32_t h12 = sw_pack2<F16>(v[1], v[2], &l12);
+        uint32_t *base = stage + q * 4 * stride + i;
+        base[0] = h01;
+        base[stride] = h12;
+        base[2 * stride] = l01;
+        base[3 * stride] = l12;
+    }
+    __syncthreads();
+    if (wave >= gd.y) return;
+    const int nb = gd.x + wave;
+    switch (a.nblocks[nb].x) {
+#define SW_CASE(N)                                                                          \
+    case N:                                                                                 \
+        if constexpr (DEFER)                                                                \
+            sw_wave_deferred<F16, RS, N>(a, nb, tile_begin, nt, kg, stride, stage, sub, badf, \
+                                         tbuf);                                             \
+        else                                                                                \
+            sw_wave<F16, XPOSE, RS, N>(a, nb, tile_begin, nt, kg, stride, stage, sub, badf, \
+                                       tbuf);                                               \
+        break;
+        SW_CASE(1) SW_CASE(2) SW_CASE(3) SW_CASE(4) SW_CASE(5) SW_CASE(6) SW_CASE(7)
+        SW_CASE(8) SW_CASE(9) SW_CASE(10) SW_CASE(11) SW_CASE(12) SW_CASE(13) SW_CASE(14)
+        SW_CASE(15) SW_CASE(16)
+#undef SW_CASE
+        default: break;
+    }
+}
+
+// RS = 2: two accumulators, fits two waves per SIMD; RS = 4: four, one wave.
+// _p: deferred stores (sw_wave_deferred; T % 4 == 0 only).
+#define SW_KERNEL(NAME, F16, XPOSE, RS, WPE, DEFER)                                           \
+    extern "C" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) \
+    void NAME(alma_sw_args a) { alma_sw_entry<F16, XPOSE, RS, DEFER>(a); }
+SW_KERNEL(alma_sweep_bf16x3, false, true, 2, 2, false)
+SW_KERNEL(alma_sweep_bf16x3_d, false, false, 2, 2, false)
+SW_KERNEL(alma_sweep_bf16x3_r4, false, false, 4, 1, false)
+SW_KERNEL(alma_sweep_bf16x3_p, false, true, 2, 2, true)
+SW_KERNEL(alma_sweep_f16x3, true, true, 2, 2, false)
+SW_KERNEL(alma_sweep_f16x3_d, true, false, 2, 2, false)
+SW_KERNEL(alma_sweep_f16x3_r4, true, false, 4, 1, false)
+SW_KERNEL(alma_sweep_f16x3_p, true, true, 2, 2, true)
+#undef SW_KERNEL

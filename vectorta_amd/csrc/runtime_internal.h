@@ -156,7 +156,8 @@ vta_status host_first_valids(const float *tm, size_t cols, size_t rows, size_t n
 struct AlmaTables {
     enum Section {
         W, PERIODS, INV,                                                  // FP32 form
-        FRAGS, NBLOCKS, GROUPS, SORTED_PERIOD, SORTED_ROW, W_F32,         // MFMA form
+        FRAGS, NBLOCKS, GROUPS, SORTED_PERIOD, SORTED_ROW, W_F32,         // MFMA forms
+        CSCALE,                                                           // MFMA F16 form
         COUNT
     };
     vta_alma_batch_kernel kernel = VTA_ALMA_BATCH_FP32;

@@ -19,7 +19,7 @@ namespace {
 bool knob_value_ok(vta_tune_knob knob, int64_t v) {
     if (v == 0) return true;
     switch (knob) {
-        case VTA_TUNE_ALMA_MFMA_VARIANT: return v >= 1 && v <= 7;
+        case VTA_TUNE_ALMA_MFMA_VARIANT: return v >= 1 && v <= 11;
         case VTA_TUNE_SCAN_WAVES: return v >= 64 && v <= 65536;
         case VTA_TUNE_FIR_GROUP: return v >= 1 && v <= 256;
         case VTA_TUNE_MS_CHUNK: return v >= 64 && v <= (int64_t)1 << 30;
