@@ -767,6 +767,89 @@ def bench_alma_host_boundary(reps=3):
             "into_host_outputs_per_s": n / (np.median(host_ms) * 1e-3)}
 
 
+def _smi_sample():
+    """(socket power W, mean GFX clock MHz over the XCDs) from amd-smi, or None."""
+    import subprocess
+    try:
+        txt = subprocess.run(["amd-smi", "metric", "-g", "0", "--power", "--clock"],
+                             capture_output=True, text=True, timeout=20).stdout
+    except (OSError, subprocess.SubprocessError):
+        return None
+    power, clocks, in_gfx = None, [], False
+    for line in txt.splitlines():
+        t = line.strip()
+        if t.startswith("SOCKET_POWER:"):
+            try:
+                power = float(t.split()[1])
+            except (IndexError, ValueError):
+                pass
+        elif t.startswith("GFX_") and t.endswith(":"):
+            in_gfx = True
+        elif t.startswith("MEM_") and t.endswith(":"):
+            in_gfx = False
+        elif in_gfx and t.startswith("CLK:"):
+            try:
+                clocks.append(float(t.split()[1]))
+            except (IndexError, ValueError):
+                pass
+            in_gfx = False
+    if power is None or not clocks:
+        return None
+    return power, sum(clocks) / len(clocks)
+
+
+def bench_sustained(seconds=6.0, forms=("mfma", "mfma_f16x3", "fp32")):
+    """Each sweep form launched back to back for `seconds` (host syncs every 50
+    launches), with amd-smi read every half second from the second second on:
+    the sustained launch time and the socket power and engine clock it runs at."""
+    import threading
+    x = datagen.alma_sweep_series()
+    sweep = vta.AlmaBatchRange(**datagen.ALMA_SWEEP_RANGE)
+    T, C = x.size, len(vta.expand_grid(sweep))
+    alma = vta.CudaAlma(0)
+    d_x = alma.ctx.to_device(x)
+    d_out = alma.ctx.alloc(T * C * 4)
+    idle = _smi_sample()
+    res = {"workload": "alma_sweep_200k_x_1250", "seconds": seconds,
+           "idle": {"socket_W": idle[0], "gfx_MHz": idle[1]} if idle else None, "forms": {}}
+    for form in forms:
+        plan = alma.sweep_plan(sweep, T, kernel=form)
+        plan.launch(d_x.ptr, 0, d_out.ptr)
+        alma.synchronize()
+        samples, stop = [], threading.Event()
+
+        def sampler():
+            if stop.wait(1.0):
+                return
+            while not stop.is_set():
+                v = _smi_sample()
+                if v:
+                    samples.append(v)
+                stop.wait(0.5)
+
+        th = threading.Thread(target=sampler)
+        th.start()
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < seconds:
+            for _ in range(50):
+                plan.launch(d_x.ptr, 0, d_out.ptr)
+            alma.synchronize()
+            n += 50
+        dt = time.perf_counter() - t0
+        stop.set()
+        th.join()
+        plan.close()
+        r = {"launches": n, "us_per_launch": dt / n * 1e6, "outputs_per_s": n * C * T / dt,
+             "samples": len(samples)}
+        if samples:
+            r["socket_W_mean"] = float(np.mean([p for p, _ in samples]))
+            r["gfx_MHz_mean"] = float(np.mean([c for _, c in samples]))
+        res["forms"][form] = r
+        print(f"[bench] sustained {form}: {r}", file=sys.stderr, flush=True)
+    d_out.free()
+    return res
+
+
 def bench_alma_single(steps, ev):
     """Plain ALMA on one 100,000-bar series (period 9): launch-latency bound."""
     x = datagen.alma_single_series()
@@ -810,7 +893,7 @@ def main():
                          "single ALMA)")
     ap.add_argument("--mode", default="sweep",
                     choices=["sweep", "calib", "ohlc", "rolling", "cross", "serve", "ma", "rec",
-                             "graph", "tune", "pack"],
+                             "graph", "tune", "pack", "sustain"],
                     help="calib: 256 MiB device-to-device copies, the known-size reference "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
@@ -823,6 +906,9 @@ def main():
 
     if args.mode == "calib":
         calibrate_copy()
+        return
+    if args.mode == "sustain":
+        print(json.dumps(bench_sustained()))
         return
     if args.mode == "ohlc":
         r = bench_many_series_ohlc(args.steps, args.warmup, HipEvents())

@@ -1006,6 +1006,34 @@ __device__ __forceinline__ vta_f32x16 sw_mfma(const uint4 &a, const uint4 &b, co
                                                        0);
 }
 
+// Split-F16 guard for one tile (wave-uniform result): true if, for any stored
+// output of the tile, the split-error bound exceeds SW_GUARD |y|.  Lane (m, g)
+// holds combo m, bars t_tile + 8q + 4g + i in y[4q + i].  In a tile past every
+// combo's warm-up and inside the series all 16 values are stored (padding
+// combos have zero weights and a zero bound), so one comparison against the
+// smallest |y| decides; otherwise each output is checked with its mask.
+__device__ __forceinline__ bool sw_guard_violated(const float (&y)[16], float bound, int t_tile,
+                                                  int g, int my_row, int my_warm, int warm_max,
+                                                  int T) {
+    bool viol = false;
+    if (t_tile >= warm_max && t_tile + 32 <= T) {
+        float mn = fabsf(y[0]);
+#pragma unroll
+        for (int i = 1; i < 16; ++i) mn = fminf(mn, fabsf(y[i]));
+        viol = !(bound <= SW_GUARD * mn);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int t = t_tile + 8 * q + 4 * g + i;
+                viol = viol || (my_row >= 0 && t >= my_warm && t < T &&
+                                !(bound <= SW_GUARD * fabsf(y[4 * q + i])));
+            }
+    }
+    return __any(viol);
+}
+
 // Stores (or recomputes) one 32-bar tile of this wave's N-block from its
 // accumulator: lane (m, g) holds combo m, bars 8q + 4g + i.
 template <bool F16, bool XPOSE>
@@ -1023,20 +1051,8 @@ __device__ __forceinline__ void sw_epilogue(const alma_sw_args &a, const vta_f32
 #pragma unroll
     for (int i = 0; i < 16; ++i) y[i] = fmaf(acc[i], f, anchor);
     bool redo = tile_bad;
-    if (F16 && !redo) {
-        // split-error guard: bound <= SW_GUARD |y| for every stored output
-        const float bound = cguard * unscale;
-        bool viol = false;
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int t = t_tile + 8 * q + 4 * g + i;
-                viol = viol || (my_row >= 0 && t >= my_warm && t < T &&
-                                !(bound <= SW_GUARD * fabsf(y[4 * q + i])));
-            }
-        redo = __any(viol);
-    }
+    if (F16 && !redo) redo = sw_guard_violated(y, cguard * unscale, t_tile, g, my_row, my_warm,
+                                               warm_max, T);
     const bool steady = !redo && t_tile >= warm_max && t_tile + 32 <= T && (T & 3) == 0;
     if (steady && XPOSE) {
         // transpose through the wave's LDS tile: each store instruction then
@@ -1217,19 +1233,9 @@ __device__ __forceinline__ void sw_wave_deferred(const alma_sw_args &a, int nb, 
 #pragma unroll
             for (int i = 0; i < 16; ++i) y[i] = fmaf(acc[j][i], f, anchor);
             bool redo = badf[tile] != 0;
-            if (F16 && !redo) {
-                const float bound = cguard * unscale;
-                bool viol = false;
-#pragma unroll
-                for (int qq = 0; qq < 4; ++qq)
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const int t = t_tile + 8 * qq + 4 * g + i;
-                        viol = viol || (my_row >= 0 && t >= my_warm && t < T &&
-                                        !(bound <= SW_GUARD * fabsf(y[4 * qq + i])));
-                    }
-                redo = __any(viol);
-            }
+            if (F16 && !redo)
+                redo = sw_guard_violated(y, cguard * unscale, t_tile, g, my_row, my_warm, warm_max,
+                                         T);
             if (redo || t_tile < warm_max) {
 #pragma unroll
                 for (int i = 0; i < 16; ++i) {
