@@ -148,7 +148,7 @@ def time_form(alma, plan, d_x, d_out, steps, warmup, ev):
 SETTLE_S = 1.0   # set from --settle
 
 
-def cpu_baseline(x, combos, threads, reps=5):
+def cpu_baseline(x, combos, threads, reps=10):
     """The f64 oracle (C restatement of alma_row_scalar) over the whole sweep on
     `threads` cores: the median of `reps` runs after a warm-up; returns the
     last run's rows (the error reference) and the median seconds."""
@@ -166,14 +166,16 @@ def cpu_baseline(x, combos, threads, reps=5):
     return ref, float(np.median(ts))
 
 
-def cpu_baseline_avx2(x, combos, threads, reps=3):
+def cpu_baseline_avx2(x, combos, threads, reps=10):
     """The AVX2+FMA row (alma_row_avx2, the reference's nightly-avx build) over
-    the whole sweep: median seconds of `reps` runs, None without AVX2."""
+    the whole sweep: median seconds of `reps` runs after a warm-up, None
+    without AVX2."""
     import oracle
     sel = [(c.period, c.offset, c.sigma) for c in combos]
     xd = x.astype(np.float64)
     if oracle.alma_batch_avx2(xd[:4096], sel[:2], nthreads=threads) is None:
         return None
+    oracle.alma_batch_avx2(xd, sel[:threads], nthreads=threads)  # warm-up
     ts = []
     for _ in range(reps):
         t0 = time.perf_counter()
@@ -182,12 +184,14 @@ def cpu_baseline_avx2(x, combos, threads, reps=3):
     return float(np.median(ts))
 
 
-def cpu_baseline_one_core(x, combos, every=10, reps=3):
+def cpu_baseline_one_core(x, combos, every=10, reps=10):
     """The same oracle on one core over every `every`-th combo of the sweep
-    (a bounded sample): median seconds of `reps` runs and the combos timed."""
+    (a bounded sample): median seconds of `reps` runs after a warm-up and the
+    combos timed."""
     import oracle
     sel = [(c.period, c.offset, c.sigma) for c in combos][::every]
     xd = x.astype(np.float64)
+    oracle.alma_batch(xd, sel[:2], nthreads=1)  # warm-up
     ts = []
     for _ in range(reps):
         t0 = time.perf_counter()
@@ -246,6 +250,25 @@ def calibrate_copy(nbytes: int = 256 << 20, reps: int = 3) -> None:
               f"{big.numel() * 4 / ms / 1e6:.1f} GB/s (write only)", flush=True)
 
 
+def cpu_median(fn, reps=10):
+    """Median seconds of `reps` calls of fn() after one warm-up call."""
+    fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def _cpu_threads():
+    return CPU_THREADS or len(os.sched_getaffinity(0))
+
+
+CPU_THREADS = 0      # set from --cpu-threads; 0 = every host core this process may use
+CPU_EXTRA = True     # --no-cpu turns the extra workloads' baselines off too
+
+
 def bench_many_series_ohlc(steps, warmup, ev):
     """Fused EMA(20) + RSI(14) + ATR(14) over 4,096 uneven tickers right-aligned
     in a [50,000 x 4,096] time-major matrix (one enqueue-only launch per step).
@@ -255,18 +278,42 @@ def bench_many_series_ohlc(steps, warmup, ev):
     rows, cols = c.shape
     ctx = vta.HipContext(0)
     plan = vta.ManySeriesOhlcPlan(ctx, c, h, lo)
-    del o, h, lo, c
+    del o
     mean, ks, _ = _time_launches(lambda: plan.launch(20, 14, 14), ctx.stream, steps, warmup, ev,
                                  sync=ctx.synchronize)
     alg = 3 * 4 * int(lens.sum()) + 3 * 4 * rows * cols
     t = mean * 1e-3
-    return {"workload": "ema20_rsi14_atr14_many_series_4096x50000", "kernel": "ms_ema_rsi_atr_f32",
-            "kernel_ms_mean": mean, "kernel_ms_p50": float(np.percentile(ks, 50)),
-            "kernel_ms_p99": float(np.percentile(ks, 99)), "outputs": 3 * rows * cols,
-            "outputs_per_s": 3 * rows * cols / t, "algorithmic_bytes": alg,
-            "traffic": profiled_traffic("ema_rsi_atr_many_series_4096x50000_traffic.txt",
-                                        "ms_ema_rsi_atr_f32"),
-            "achieved_GBs": alg / t / 1e9, "frac_of_hbm_peak": alg / t / 1e9 / HBM_PEAK_GBS}
+    res = {"workload": "ema20_rsi14_atr14_many_series_4096x50000", "kernel": "ms_ema_rsi_atr_f32",
+           "kernel_ms_mean": mean, "kernel_ms_p50": float(np.percentile(ks, 50)),
+           "kernel_ms_p99": float(np.percentile(ks, 99)), "outputs": 3 * rows * cols,
+           "outputs_per_s": 3 * rows * cols / t, "algorithmic_bytes": alg,
+           "traffic": profiled_traffic("ema_rsi_atr_many_series_4096x50000_traffic.txt",
+                                       "ms_ema_rsi_atr_f32"),
+           "achieved_GBs": alg / t / 1e9, "frac_of_hbm_peak": alg / t / 1e9 / HBM_PEAK_GBS}
+    if CPU_EXTRA:
+        # the oracle's scalar EMA, Wilder RSI and ATR per ticker, each ticker's
+        # NaN-padded column as one contiguous f64 series (the reference's CPU
+        # path runs series by series), OpenMP over tickers
+        import oracle
+        C, H, L = (np.ascontiguousarray(a.T, dtype=np.float64) for a in (c, h, lo))
+        thr = _cpu_threads()
+
+        def run(n, threads):
+            oracle.series_major("ema", C[:n], 20, nthreads=threads)
+            oracle.series_major("rsi", C[:n], 14, nthreads=threads)
+            oracle.series_major("atr", H[:n], 14, low=L[:n], close=C[:n], nthreads=threads)
+
+        dt = cpu_median(lambda: run(cols, thr))
+        sample = 256
+        dt1 = cpu_median(lambda: run(sample, 1), reps=3)
+        res["cpu_baseline"] = {
+            "value": 3 * rows * cols / dt, "unit": "outputs/s", "cores": thr, "kind": "port",
+            "sample": f"all {cols} tickers, f64 C restatements of ema/rsi/atr scalar, one "
+                      f"contiguous series per task, median of 10 runs {dt:.2f} s",
+            "one_core": {"value": 3 * rows * sample / dt1, "unit": "outputs/s", "cores": 1,
+                         "sample": f"first {sample} tickers, median of 3 runs {dt1:.2f} s"}}
+        del C, H, L
+    return res
 
 
 def bench_rolling(steps, warmup, ev):
@@ -278,7 +325,7 @@ def bench_rolling(steps, warmup, ev):
     rows, cols = x.shape
     ctx = vta.HipContext(0)
     plan = vta.RollingManySeriesPlan(ctx, x)
-    del x
+    x_host = x
     res = {"workload": "bb20_pnr100p70_mad64_many_series_1024x50000"}
     for name, fn, nout, kern in (("bb", lambda: plan.launch_bb(20, 2.0, 2.0), 3, "bb_ms1p_f32"),
                                  ("pnr", lambda: plan.launch_pnr(100, 70.0), 1, "pnr_ms1p_f32"),
@@ -293,6 +340,28 @@ def bench_rolling(steps, warmup, ev):
                      "traffic": profiled_traffic("rolling_windows_1024x50000_traffic.txt", kern),
                      "achieved_GBs": alg / t / 1e9,
                      "frac_of_hbm_peak": alg / t / 1e9 / HBM_PEAK_GBS}
+    if CPU_EXTRA:
+        # the oracle's scalar bollinger_bands / pnr / medium_ad per series on a
+        # bounded sample of contiguous f64 series (medium_ad sorts every
+        # window, ~2 us per bar on one core), OpenMP over series
+        import oracle
+        X = np.ascontiguousarray(x_host.T, dtype=np.float64)
+        thr = _cpu_threads()
+        for name, kind, prm, n in (("bb", "bb", (20, 2.0, 2.0), 512),
+                                   ("pnr", "pnr", (100, 70.0), 256),
+                                   ("mad", "medium_ad", (64,), 64)):
+            n = min(n, cols)
+            n1 = max(1, n // 16)
+            dt = cpu_median(lambda: oracle.series_major(kind, X[:n], *prm, nthreads=thr))
+            dt1 = cpu_median(lambda: oracle.series_major(kind, X[:n1], *prm, nthreads=1),
+                             reps=3)
+            res[name]["cpu_baseline"] = {
+                "value": rows * n / dt, "unit": "series_bars/s", "cores": thr, "kind": "port",
+                "sample": f"first {n} of {cols} series x {rows} bars, median of 10 runs "
+                          f"{dt:.3f} s",
+                "one_core": {"value": rows * n1 / dt1, "unit": "series_bars/s", "cores": 1,
+                             "sample": f"first {n1} series, median of 3 runs {dt1:.3f} s"}}
+        del X
     return res
 
 
@@ -897,8 +966,10 @@ def main():
                     help="calib: 256 MiB device-to-device copies, the known-size reference "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
-    global SETTLE_S
+    global SETTLE_S, CPU_THREADS, CPU_EXTRA
     SETTLE_S = max(0.0, args.settle)
+    CPU_THREADS = max(0, args.cpu_threads)
+    CPU_EXTRA = not args.no_cpu
     if args.gpus != 1:
         print(f"bench.py: vectorta_amd runs on one GPU only (asked for --gpus {args.gpus}); "
               "sharding the sweep over several GPUs is not implemented", file=sys.stderr)
@@ -991,20 +1062,20 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or len(os.sched_getaffinity(0))
+        threads = _cpu_threads()
         ref, dt = cpu_baseline(x, combos, threads)
         cpu = {"value": C * T / dt, "unit": "outputs/s", "cores": threads, "kind": "port",
                "sample": f"full {C}-combo x {T}-bar sweep, f64 C restatement of "
-                         f"alma_row_scalar, OpenMP over rows, median of 5 runs {dt:.2f} s"}
+                         f"alma_row_scalar, OpenMP over rows, median of 10 runs {dt:.2f} s"}
         dtv = cpu_baseline_avx2(x, combos, threads)
         if dtv is not None:
             cpu["avx2"] = {"value": C * T / dtv, "unit": "outputs/s", "cores": threads,
                            "sample": f"full sweep, AVX2+FMA restatement of alma_row_avx2 "
-                                     f"(the reference's nightly-avx build), median of 3 runs "
+                                     f"(the reference's nightly-avx build), median of 10 runs "
                                      f"{dtv:.2f} s"}
         dt1, n1 = cpu_baseline_one_core(x, combos)
         cpu["one_core"] = {"value": n1 * T / dt1, "unit": "outputs/s", "cores": 1,
-                           "sample": f"every 10th combo ({n1}) x {T} bars, median of 3 runs "
+                           "sample": f"every 10th combo ({n1}) x {T} bars, median of 10 runs "
                                      f"{dt1:.2f} s"}
         for form, g in outputs.items():
             err, nan_ok = max_rel_err(g, ref)

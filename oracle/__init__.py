@@ -79,6 +79,10 @@ def lib():
         L.or_many_series_3in.argtypes = [ctypes.c_int, dp, dp, dp, sz, sz, sz, dp,
                                          ctypes.POINTER(ctypes.c_int), ctypes.c_int]
         L.or_many_series_3in.restype = None
+        L.or_series_major.argtypes = [ctypes.c_int, dp, dp, dp, sz, sz, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_double, dp, dp, dp,
+                                      ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+        L.or_series_major.restype = None
         L.or_cross_pairs.argtypes = [dp, sz, sz, dp, ctypes.POINTER(ctypes.c_int),
                                      ctypes.POINTER(ctypes.c_int), sz,
                                      ctypes.POINTER(ctypes.c_longlong), dp, ctypes.c_int]
@@ -244,6 +248,27 @@ def many_series_hlc(kind, h_tm, l_tm, c_tm, period, nthreads=0):
     lib().or_many_series_3in(_KIND3[kind], hp, lp, cp, cols, rows, int(period), _d(out)[1], st,
                              int(nthreads))
     return out, np.frombuffer(st, dtype=np.int32).copy()
+
+
+_SM_KIND = {"ema": 0, "rsi": 1, "pnr": 2, "medium_ad": 3, "bb": 4, "atr": 5}
+
+
+def series_major(kind, sm, p0, p1=0.0, p2=0.0, low=None, close=None, nthreads=0):
+    """A single-series oracle over each row of a series-major [n_series, len]
+    f64 matrix (OpenMP over rows), the layout of the CPU baselines.  For "atr"
+    sm is the high matrix and low/close the others; "bb" returns (upper,
+    middle, lower).  Returns (outputs, per-series status)."""
+    a, ap = _d(sm)
+    n, length = a.shape
+    b, bp = _d(low) if low is not None else (a, ap)
+    c, cp = _d(close) if close is not None else (a, ap)
+    outs = [np.empty((n, length)) for _ in range(3 if kind == "bb" else 1)]
+    ptrs = [_d(o)[1] for o in outs] + [None] * (3 - len(outs))
+    st = (ctypes.c_int * n)()
+    lib().or_series_major(_SM_KIND[kind], ap, bp, cp, n, length, float(p0), float(p1),
+                          float(p2), ptrs[0], ptrs[1], ptrs[2], st, int(nthreads))
+    status = np.frombuffer(st, dtype=np.int32).copy()
+    return (tuple(outs) if kind == "bb" else outs[0]), status
 
 
 def ema_batch(x, periods):

@@ -751,6 +751,40 @@ void or_many_series_3in(int kind, const double *h_tm, const double *l_tm, const 
     }
 }
 
+/* Series-major driver (the CPU baselines' layout): n_series contiguous series
+ * of len bars, series s at a + s * len (b, c: low and close for ATR), outputs
+ * at out + s * len (BB: upper, middle, lower in out, out2, out3), OpenMP over
+ * series.  kind: 0 ema(p0), 1 rsi(p0), 2 pnr(p0, percentage p1), 3 mad(p0),
+ * 4 bb(p0, devup p1, devdn p2), 5 atr(p0) over (high a, low b, close c). */
+void or_series_major(int kind, const double *a, const double *b, const double *c,
+                     size_t n_series, size_t len, double p0, double p1, double p2, double *out,
+                     double *out2, double *out3, int *status, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) if (nthreads != 1)
+#endif
+    for (long s = 0; s < (long)n_series; ++s) {
+        const size_t o = (size_t)s * len;
+        double *y = out + o;
+        fill_nan(y, len);
+        int st = OR_PARAM;
+        switch (kind) {
+            case 0: st = or_ema(a + o, len, (size_t)p0, y); break;
+            case 1: st = or_rsi(a + o, len, (size_t)p0, y); break;
+            case 2: st = or_pnr(a + o, len, (size_t)p0, p1, y); break;
+            case 3: st = or_mad(a + o, len, (size_t)p0, y); break;
+            case 4:
+                fill_nan(out2 + o, len);
+                fill_nan(out3 + o, len);
+                st = or_bb(a + o, len, (size_t)p0, p1, p2, y, out2 + o, out3 + o);
+                break;
+            case 5: st = or_atr(a + o, b + o, c + o, len, (size_t)p0, y); break;
+            default: break;
+        }
+        status[s] = st;
+    }
+}
+
 /* ------------------------------------------------------------ double-ALMA crossover */
 
 /* Per pair (fast row f, slow row s of the f64 ALMA batch rows, [nrows][len]):

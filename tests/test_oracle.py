@@ -466,3 +466,38 @@ def test_alma_avx2_baseline_matches_scalar_rows():
     assert np.array_equal(np.isnan(a), np.isnan(b))
     m = np.isfinite(b)
     assert np.max(np.abs(a[m] - b[m]) / np.abs(b[m])) < 1e-12
+
+
+def test_series_major_driver_matches_single_series_calls():
+    """The series-major OpenMP driver (the CPU baselines' layout) returns, row
+    for row, what the single-series oracle returns, with NaN prefixes and
+    per-series status, for every kind it runs."""
+    rng = np.random.default_rng(11)
+    n, length = 5, 700
+    c = 100.0 + np.cumsum(rng.normal(0.0, 0.5, (n, length)), axis=1)
+    h, lo = c + rng.uniform(0.0, 1.0, c.shape), c - rng.uniform(0.0, 1.0, c.shape)
+    for arr in (c, h, lo):
+        arr[1, :37] = np.nan
+    c[3, :] = np.nan
+    h[3, :] = np.nan
+    lo[3, :] = np.nan
+    cases = [("ema", (20,), lambda x: O.ema(x, 20)), ("rsi", (14,), lambda x: O.rsi(x, 14)),
+             ("pnr", (30, 70.0), lambda x: O.pnr(x, 30, 70.0)),
+             ("medium_ad", (16,), lambda x: O.medium_ad(x, 16))]
+    for kind, params, one in cases:
+        got, st = O.series_major(kind, c, *params, nthreads=3)
+        for j in range(n):
+            if j == 3:
+                assert st[j] != 0
+                continue
+            assert st[j] == 0
+            np.testing.assert_array_equal(got[j], one(c[j]))
+    (up, mid, dn), st = O.series_major("bb", c, 20, 2.0, 1.5, nthreads=2)
+    for j in (0, 1, 2, 4):
+        u, m, d = O.bollinger_bands(c[j], 20, 2.0, 1.5)
+        np.testing.assert_array_equal(up[j], u)
+        np.testing.assert_array_equal(mid[j], m)
+        np.testing.assert_array_equal(dn[j], d)
+    got, st = O.series_major("atr", h, 14, low=lo, close=c, nthreads=2)
+    for j in (0, 1, 2, 4):
+        np.testing.assert_array_equal(got[j], O.atr(h[j], lo[j], c[j], 14))
