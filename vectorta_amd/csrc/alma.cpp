@@ -297,23 +297,32 @@ const P *section(const AlmaTables &t, const unsigned char *base, int sec) {
 }
 
 // IEEE binary16 round-to-nearest-even of a double (|v| < 65520), as bits,
-// rounded once from the double (no f32 step), and its value.
+// rounded once from the double (no f32 step): the significand is cut to 11
+// bits for a normal result or to the 2^-24 quantum for a subnormal one, ties
+// to the even neighbour; a carry out of the significand bumps the exponent.
 uint16_t f16_rne(double v) {
-    const uint16_t sign = std::signbit(v) ? 0x8000u : 0u;
-    const double a = std::fabs(v);
-    if (a == 0.0) return sign;
-    int e;
-    (void)std::frexp(a, &e);                   // a in [2^(e-1), 2^e)
-    const int E = std::max(e - 1, -14);        // subnormals share the 2^-24 quantum
-    const double q = std::ldexp(1.0, E - 10);
-    const double r = std::nearbyint(a / q) * q;  // ties to even (default rounding mode)
-    if (r == 0.0) return sign;                    // below half the smallest subnormal
-    int er;
-    (void)std::frexp(r, &er);
-    const int Er = er - 1;
-    if (Er < -14) return (uint16_t)(sign | (uint16_t)std::lround(r / std::ldexp(1.0, -24)));
-    const int mant = (int)std::lround((r / std::ldexp(1.0, Er) - 1.0) * 1024.0);
-    return (uint16_t)(sign | (uint16_t)((Er + 15) << 10) | (uint16_t)mant);
+    uint64_t b;
+    std::memcpy(&b, &v, sizeof b);
+    const uint16_t sign = (uint16_t)((b >> 48) & 0x8000u);
+    const int ex = (int)((b >> 52) & 0x7ff);
+    if (ex == 0) return sign;                    // zero or a double subnormal
+    const int e = ex - 1023;                     // |v| = 1.m 2^e
+    if (e < -25) return sign;                    // below half the smallest subnormal
+    const uint64_t sig = (b & ((1ull << 52) - 1)) | (1ull << 52);
+    const int shift = e >= -14 ? 42 : 28 - e;    // dropped low bits
+    const uint64_t q = sig >> shift;
+    const uint64_t rem = sig & ((1ull << shift) - 1);
+    const uint64_t half = 1ull << (shift - 1);
+    uint64_t r = q + ((rem > half || (rem == half && (q & 1))) ? 1 : 0);
+    if (e >= -14) {
+        int E = e + 15;
+        if (r == 2048) {                         // rounded up to the next binade
+            r = 1024;
+            ++E;
+        }
+        return (uint16_t)(sign | (uint16_t)(E << 10) | (uint16_t)(r - 1024));
+    }
+    return (uint16_t)(sign | (uint16_t)r);       // subnormal (r == 1024: smallest normal)
 }
 
 double f16_value(uint16_t h) {
@@ -358,22 +367,24 @@ void build_mfma_tables(AlmaTables &t, const std::vector<vta_alma_params> &combos
         nbd[b * 4 + 0] = K / 16;
         nbd[b * 4 + 1] = (int32_t)(frags.size() / 8);  // offset in 16-byte chunks
         nbd[b * 4 + 2] = b * 32;
-        // W[k][n], zero padded at the front (F16: scaled by 2^sw per combo)
-        std::vector<double> W((size_t)K * 32, 0.0), wd;
+        // hi / lo of W[k][n], zero padded at the front (F16: scaled by 2^sw per
+        // combo), each weight split once
+        std::vector<uint16_t> Wh((size_t)K * 32, 0), Wl((size_t)K * 32, 0);
+        std::vector<double> wd;
         for (int n = 0; n < 32; ++n) {
             size_t i = (size_t)b * 32 + n;
             if (i >= C) continue;
             const auto &cp = combos[order[i]];
             weights_f64(cp.period, cp.offset, cp.sigma, wd);
-            double scale = 1.0;
+            const size_t lead = (size_t)K - cp.period;
             if (f16) {
                 const double wmax = *std::max_element(wd.begin(), wd.end());
                 int e;
                 (void)std::frexp(wmax, &e);            // wmax = f 2^e, f in [0.5, 1)
-                scale = std::ldexp(1.0, -e);
+                const double scale = std::ldexp(1.0, -e);
                 double sum_ws = 0.0, eps = 0.0, lsum = 0.0;
-                for (double w : wd) {
-                    const double ws = w * scale;
+                for (size_t k = 0; k < cp.period; ++k) {
+                    const double ws = wd[k] * scale;
                     const uint16_t hi = f16_rne(ws);
                     const double hv = f16_value(hi);
                     const uint16_t lo = f16_rne(ws - hv);
@@ -381,36 +392,31 @@ void build_mfma_tables(AlmaTables &t, const std::vector<vta_alma_params> &combos
                     sum_ws += ws;
                     eps += std::fabs(ws - hv - lv);
                     lsum += std::fabs(lv);
+                    Wh[(lead + k) * 32 + n] = hi;
+                    Wl[(lead + k) * 32 + n] = lo;
                 }
                 const double bound =
                     (std::ldexp(sum_ws, -22) + eps + std::ldexp(lsum, -11)) / scale;
                 csc[2 * i + 0] = (float)(1.0 / scale);
                 csc[2 * i + 1] = std::nextafter((float)(bound * (1.0 + 1e-6)), INFINITY);
+            } else {
+                for (size_t k = 0; k < cp.period; ++k) {
+                    const double w = wd[k];
+                    const uint16_t hi = bf16_rne((float)w);
+                    Wh[(lead + k) * 32 + n] = hi;
+                    Wl[(lead + k) * 32 + n] = bf16_rne((float)(w - (double)bf16_to_f32(hi)));
+                }
             }
-            size_t lead = (size_t)K - cp.period;
-            for (size_t k = 0; k < cp.period; ++k) W[(lead + k) * 32 + n] = wd[k] * scale;
             sp[i] = (int32_t)cp.period;
             sr[i] = (int32_t)order[i];
         }
-        for (int s = 0; s < K / 16; ++s) {
+        for (int s = 0; s < K / 16; ++s)
             for (int h = 0; h < 2; ++h) {
-                for (int l = 0; l < 64; ++l) {
-                    for (int j = 0; j < 8; ++j) {
-                        int k = 16 * s + 8 * (l / 32) + j;
-                        double w = W[(size_t)k * 32 + (l % 32)];
-                        uint16_t hi, lo;
-                        if (f16) {
-                            hi = f16_rne(w);
-                            lo = f16_rne(w - f16_value(hi));
-                        } else {
-                            hi = bf16_rne((float)w);
-                            lo = bf16_rne((float)(w - (double)bf16_to_f32(hi)));
-                        }
-                        frags.push_back(h == 0 ? hi : lo);
-                    }
-                }
+                const std::vector<uint16_t> &src = h == 0 ? Wh : Wl;
+                for (int l = 0; l < 64; ++l)
+                    for (int j = 0; j < 8; ++j)
+                        frags.push_back(src[(size_t)(16 * s + 8 * (l / 32) + j) * 32 + (l % 32)]);
             }
-        }
     }
     // f32 weight table for the non-finite fallback, [cpad][kmax], taps from
     // the window start.

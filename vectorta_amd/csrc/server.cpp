@@ -359,21 +359,28 @@ vta_status upload_series(vta_server *s, int slot, vta_ctx *ctx, Req &q) {
     return VTA_OK;
 }
 
-// A small parameter table in the arena; returned when it goes out of scope,
-// which run_request reaches only after its stream synchronised.
+// A small parameter table in the arena, returned when it goes out of scope.
+// An early error return can leave kernels that read it queued on the slot's
+// stream, so the stream is drained first (a no-op after run_request's own
+// synchronisation): another slot may be handed the memory at once.
 struct ArenaTable {
     Arena *ar = nullptr;
+    vta_ctx *ctx = nullptr;
     void *p = nullptr;
     ~ArenaTable() {
-        if (p) ar->release(p);
+        if (p) {
+            (void)hipStreamSynchronize(ctx->stream);
+            ar->release(p);
+        }
     }
     template <typename T>
-    vta_status put(vta_server *s, vta_ctx *ctx, const std::vector<T> &v) {
+    vta_status put(vta_server *s, vta_ctx *c, const std::vector<T> &v) {
         ar = &s->arena;
+        ctx = c;
         p = ar->alloc(v.size() * sizeof(T));
         if (!p) return fail(VTA_ERR_OUT_OF_MEMORY, "device arena exhausted");
         VTA_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice,
-                               ctx->stream));
+                               c->stream));
         return VTA_OK;
     }
     template <typename T>
@@ -391,19 +398,34 @@ vta_status alloc_out(vta_server *s, vta_device_array_f32 &o, size_t rows, size_t
     return VTA_OK;
 }
 
+// The grid's sweep plan, built on a miss outside plan_mu (the host table build
+// of a large grid takes milliseconds and must not hold up the other slots);
+// of two slots that race on one new grid, the first insert wins and the
+// other's plan is dropped.
 vta_status alma_plan_for(vta_server *s, vta_ctx *ctx, const vta_alma_batch_range &r,
                          vta_alma_sweep_plan **plan, size_t *max_period, size_t *n_combos) {
     std::string key(reinterpret_cast<const char *>(&r), sizeof(r));
-    std::lock_guard<std::mutex> lk(s->plan_mu);
-    auto it = s->alma_plans.find(key);
-    if (it == s->alma_plans.end()) {
-        vta_alma_sweep_plan *p = nullptr;
-        size_t mp = 0, nc = 0;
-        VTA_TRY(alma_grid_plan_create(ctx, &r, &p, &mp, &nc));
-        it = s->alma_plans.emplace(key, p).first;
-        s->alma_plan_dims.emplace(key, std::make_pair(mp, nc));
+    {
+        std::lock_guard<std::mutex> lk(s->plan_mu);
+        auto it = s->alma_plans.find(key);
+        if (it != s->alma_plans.end()) {
+            *plan = it->second;
+            *max_period = s->alma_plan_dims[key].first;
+            *n_combos = s->alma_plan_dims[key].second;
+            return VTA_OK;
+        }
     }
-    *plan = it->second;
+    vta_alma_sweep_plan *p = nullptr;
+    size_t mp = 0, nc = 0;
+    VTA_TRY(alma_grid_plan_create(ctx, &r, &p, &mp, &nc));
+    std::lock_guard<std::mutex> lk(s->plan_mu);
+    auto ins = s->alma_plans.emplace(key, p);
+    if (ins.second) {
+        s->alma_plan_dims.emplace(key, std::make_pair(mp, nc));
+    } else {
+        alma_grid_plan_destroy(p);   // another slot built the same grid first
+    }
+    *plan = ins.first->second;
     *max_period = s->alma_plan_dims[key].first;
     *n_combos = s->alma_plan_dims[key].second;
     return VTA_OK;
