@@ -262,6 +262,37 @@ size_t vta_alma_sweep_plan_combos(const vta_alma_sweep_plan *plan);
 vta_status vta_alma_sweep_launch(vta_ctx *ctx, const vta_alma_sweep_plan *plan,
                                  const float *d_prices, size_t first_valid, float *d_out);
 
+/* Packed ALMA sweeps (no reference counterpart, whose wrappers run one series
+ * per call): one-series batches of uneven lengths, each with its own
+ * parameter grid, in one launch through a device table of jobs and work
+ * items (a 1,024-bar x 8-combo tile of a job for the FP32 kernel, a 512-bar
+ * run of a 128-combo group for the matrix-core forms).  All jobs run one
+ * kernel: `kernel` (AUTO: the split-F16 form when every job has 16+ combos
+ * and periods <= 256, else FP32).  Each job's outputs are bit-identical to a
+ * vta_alma_sweep_launch of its own grid with that kernel (for the split-BF16
+ * form, with VTA_TUNE_ALMA_MFMA_VARIANT 8, 9 or 11, the diagonal form the
+ * split-F16 form always runs).  vta_alma_pack_create expands and validates
+ * every job like vta_alma_batch_from_device_prices and uploads all tables at
+ * once; vta_alma_pack_launch is enqueue-only (graph-capturable).  Job
+ * pointers are device pointers that must stay valid while the pack is used;
+ * a launch overwrites every job's output ([n_combos][series_len], combos in
+ * expand_grid order). */
+typedef struct vta_alma_job {
+    const float *d_prices;
+    float *d_out;
+    vta_alma_batch_range range;
+    size_t series_len;
+    size_t first_valid;
+} vta_alma_job;
+typedef struct vta_alma_pack vta_alma_pack;
+vta_status vta_alma_pack_create(vta_ctx *ctx, vta_alma_batch_kernel kernel,
+                                const vta_alma_job *jobs, size_t n_jobs, vta_alma_pack **out);
+vta_status vta_alma_pack_launch(vta_ctx *ctx, const vta_alma_pack *pack);
+/* the kernel the pack runs, and job `job`'s number of combos (0 if out of range) */
+vta_alma_batch_kernel vta_alma_pack_kernel(const vta_alma_pack *pack);
+size_t vta_alma_pack_job_combos(const vta_alma_pack *pack, size_t job);
+vta_status vta_alma_pack_destroy(vta_alma_pack *pack);
+
 /* alma_multi_series_one_param_time_major_dev (:819-899): data_tm is
  * [rows=T][cols=N]; output DeviceArrayF32{rows=T, cols=N}. */
 vta_status vta_alma_many_series_one_param_time_major_dev(vta_ctx *ctx, const float *data_tm,

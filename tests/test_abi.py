@@ -193,6 +193,7 @@ _NULL_SENTINELS = {
     "vta_server_destroy": 0, "vta_ctx_device_id": -1, "vta_ctx_stream": None,
     "vta_alma_selected_batch_kernel": 0, "vta_alma_sweep_plan_combos": 0,
     "vta_graph_num_nodes": 0, "vta_double_alma_plan_pairs": 0, "vta_server_now": 0.0,
+    "vta_alma_pack_destroy": 0, "vta_alma_pack_kernel": 0, "vta_alma_pack_job_combos": 0,
 }
 
 _NULL_PROBE = r"""

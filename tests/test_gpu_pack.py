@@ -214,3 +214,97 @@ def test_bb_pack_matches_direct_calls():
     for d, outs in bufs:
         for b in list(d.values()) + outs:
             b.free()
+
+
+# (series_len, leading NaN, ALMA grid, interior non-finite bar): uneven lengths
+# (partial last tiles, T % 4 != 0), one combo to several 128-combo groups
+ALMA_JOBS = [(1_003, 0, dict(period=(9, 9, 0), offset=(0.85, 0.85, 0.0), sigma=(6.0, 6.0, 0.0)),
+              None),
+             (20_000, 37, dict(period=(10, 250, 10), offset=(0.05, 0.95, 0.3),
+                               sigma=(2.0, 10.0, 4.0)), None),
+             (6_002, 3, dict(period=(5, 40, 5), offset=(0.85, 0.85, 0.0), sigma=(6.0, 6.0, 0.0)),
+              None),
+             (50_000, 0, dict(period=(20, 180, 20), offset=(0.1, 0.9, 0.2),
+                              sigma=(3.0, 9.0, 3.0)), 31_000),
+             (64, 0, dict(period=(10, 60, 25), offset=(0.85, 0.85, 0.0), sigma=(6.0, 6.0, 0.0)),
+              None)]
+
+
+@pytest.mark.parametrize("kernel", ["fp32", "mfma", "mfma_f16x3", "auto"])
+def test_alma_pack_matches_own_sweeps(kernel):
+    """vta_alma_pack_*: five one-series sweeps (uneven lengths, own grids, a
+    late first_valid, an interior NaN) in one launch.  Each job's outputs equal,
+    bit for bit, a sweep plan of its own grid on the same kernel (the split
+    forms at variant 9, the diagonal form the pack runs), and two jobs are
+    checked against the oracle; AUTO takes FP32 here (a job has fewer than 16
+    combos)."""
+    require_gpu()
+    from vectorta_amd import autotune as at
+    ctx = vta.HipContext(0)
+    alma = vta.CudaAlma(0)
+    bufs, jobs, ref_rows = [], [], []
+    for i, (T, lead, grid, bad) in enumerate(ALMA_JOBS):
+        x = datagen.gbm_close(T, 0.015, seed=80 + i)
+        x[:lead] = np.nan
+        if bad is not None:
+            x[bad] = np.nan
+        sw = vta.AlmaBatchRange(**grid)
+        C = len(vta.expand_grid(sw))
+        d = {"x": x, "sweep": sw, "C": C, "T": T, "fv": lead,
+             "prices": ctx.to_device(x), "out": ctx.alloc(C * T * 4),
+             "direct": alma.ctx.alloc(C * T * 4)}
+        d["prices_a"] = alma.ctx.to_device(x)
+        bufs.append(d)
+        jobs.append({"prices": d["prices"], "out": d["out"], "sweep": sw, "series_len": T,
+                     "first_valid": lead})
+    pack = vta.AlmaPack(ctx, jobs, kernel=kernel)
+    expect = "fp32" if kernel in ("fp32", "auto") else \
+        ("mfma_bf16x3" if kernel == "mfma" else "mfma_f16x3")
+    assert pack.kernel == expect
+    assert pack.combos == [d["C"] for d in bufs]
+    pack.launch()
+    ctx.synchronize()
+    at.set_tuning(alma.ctx, at.ALMA_MFMA_VARIANT, 9)
+    plan_kernel = {"fp32": "fp32", "mfma_bf16x3": "mfma", "mfma_f16x3": "mfma_f16x3"}[expect]
+    for i, d in enumerate(bufs):
+        plan = alma.sweep_plan(d["sweep"], d["T"], kernel=plan_kernel)
+        plan.launch(d["prices_a"].ptr, d["fv"], d["direct"].ptr)
+        alma.synchronize()
+        g = d["out"].copy_to_host().reshape(d["C"], d["T"])
+        r = d["direct"].copy_to_host().reshape(d["C"], d["T"])
+        assert np.array_equal(g.view(np.uint32), r.view(np.uint32)), (kernel, i)
+        if i in (1, 3):
+            combos = [(c.period, c.offset, c.sigma) for c in vta.expand_grid(d["sweep"])]
+            ref = O.alma_batch(d["x"].astype(np.float64), combos)
+            nan_mask_equal(g, ref)
+            err = max_rel_err(g, ref)
+            print(f"alma pack {kernel} job {i}: max rel err {err:.3e}")
+            assert err <= 1e-5
+        plan.close()
+    at.set_tuning(alma.ctx, at.ALMA_MFMA_VARIANT, 0)
+    pack.close()
+    for d in bufs:
+        for k in ("prices", "out", "direct", "prices_a"):
+            d[k].free()
+
+
+def test_alma_pack_rejects_bad_jobs():
+    """Pack creation validates every job like alma_batch_from_device_prices:
+    null buffers, first_valid past the series, a period longer than the valid
+    data; and the explicit MFMA forms refuse periods above 256."""
+    require_gpu()
+    ctx = vta.HipContext(0)
+    x = datagen.gbm_close(1_000, 0.015, seed=90)
+    dx, do = ctx.to_device(x), ctx.alloc(1_000 * 4 * 4)
+    ok = {"prices": dx, "out": do, "series_len": 1_000, "first_valid": 0,
+          "sweep": vta.AlmaBatchRange(period=(9, 12, 1))}
+    vta.AlmaPack(ctx, [ok]).close()
+    for bad in ({"first_valid": 1_000}, {"sweep": vta.AlmaBatchRange(period=(2_000, 2_000, 0))}):
+        with pytest.raises(vta.VtaError):
+            vta.AlmaPack(ctx, [dict(ok, **bad)])
+    long = dict(ok, sweep=vta.AlmaBatchRange(period=(300, 300, 0)))
+    vta.AlmaPack(ctx, [long], kernel="fp32").close()
+    with pytest.raises(vta.VtaError):
+        vta.AlmaPack(ctx, [long], kernel="mfma_f16x3")
+    dx.free()
+    do.free()

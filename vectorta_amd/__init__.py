@@ -6,7 +6,8 @@ code object per indicator family; this package is the Python host layer over
 it, mirroring the reference's ``src/cuda`` wrapper classes (``CudaAlma`` ...).
 """
 from ._lib import VtaError, lib
-from .alma import AlmaBatchRange, AlmaParams, AlmaSweepPlan, CudaAlma, HipAlma, expand_grid
+from .alma import (AlmaBatchRange, AlmaPack, AlmaParams, AlmaSweepPlan, CudaAlma, HipAlma,
+                   expand_grid)
 from .cross import DoubleAlmaCrossover, DoubleAlmaPlan
 from .device import (DeviceArrayF32, DeviceBuffer, HipContext, HipGraph, cuda_available,
                      cuda_device_count)
@@ -19,7 +20,7 @@ from .rolling import (BbPack, BollingerBandsBatchRange, CudaBollingerBands, Cuda
                       RollingManySeriesPlan)
 
 __all__ = [
-    "VtaError", "lib", "AlmaBatchRange", "AlmaParams", "AlmaSweepPlan", "CudaAlma", "HipAlma",
+    "VtaError", "lib", "AlmaBatchRange", "AlmaPack", "AlmaParams", "AlmaSweepPlan", "CudaAlma", "HipAlma",
     "expand_grid", "DeviceArrayF32", "DeviceBuffer", "HipContext", "HipGraph", "cuda_available",
     "cuda_device_count", "CudaEma", "CudaRsi", "CudaAtr", "CudaAdx", "ManySeriesOhlcPlan",
     "ema_rsi_atr_many_series_dev", "BollingerBandsBatchRange", "CudaBollingerBands",

@@ -49,6 +49,13 @@ class AlmaBatchRangeC(ctypes.Structure):
     _fields_ = [("period", RangeUsize), ("offset", RangeF64), ("sigma", RangeF64)]
 
 
+class AlmaJobC(ctypes.Structure):
+    """vta_alma_job (include/vta.h)."""
+    _fields_ = [("d_prices", ctypes.c_void_p), ("d_out", ctypes.c_void_p),
+                ("range", AlmaBatchRangeC), ("series_len", ctypes.c_size_t),
+                ("first_valid", ctypes.c_size_t)]
+
+
 class RecJobC(ctypes.Structure):
     """vta_rec_job (include/vta.h)."""
     _fields_ = [("d_close", ctypes.c_void_p), ("d_high", ctypes.c_void_p),
@@ -141,6 +148,12 @@ SIGNATURES = {
                                   ctypes.POINTER(_P)]),
     "vta_rec_pack_launch": (_ST, [_P, _P]),
     "vta_rec_pack_destroy": (_ST, [_P]),
+    "vta_alma_pack_create": (_ST, [_P, ctypes.c_int, ctypes.POINTER(AlmaJobC), _SZ,
+                                   ctypes.POINTER(_P)]),
+    "vta_alma_pack_launch": (_ST, [_P, _P]),
+    "vta_alma_pack_kernel": (ctypes.c_int, [_P]),
+    "vta_alma_pack_job_combos": (_SZ, [_P, _SZ]),
+    "vta_alma_pack_destroy": (_ST, [_P]),
     "vta_bb_pack_create": (_ST, [_P, ctypes.POINTER(BbJobC), _SZ, ctypes.POINTER(_P)]),
     "vta_bb_pack_launch": (_ST, [_P, _P]),
     "vta_bb_pack_destroy": (_ST, [_P]),

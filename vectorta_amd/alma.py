@@ -11,8 +11,8 @@ from typing import List, Tuple
 
 import numpy as np
 
-from ._lib import (AlmaBatchRangeC, AlmaParamsC, DeviceArrayF32C, RangeF64, RangeUsize,
-                   VtaError, check, lib)
+from ._lib import (AlmaBatchRangeC, AlmaJobC, AlmaParamsC, DeviceArrayF32C, RangeF64,
+                   RangeUsize, VtaError, check, lib)
 from .device import DeviceArrayF32, DeviceBuffer, HipContext, f32_ptr, require_len
 
 
@@ -84,6 +84,46 @@ class AlmaSweepPlan:
         try:
             if self.alma.ctx._ptr:
                 self.close()
+        except Exception:
+            pass
+
+
+class AlmaPack:
+    """Many one-series ALMA sweeps (uneven lengths, own grids) in one launch
+    (vta_alma_pack_*; no reference counterpart).  jobs: dicts with device
+    buffers ``prices`` (series_len f32) and ``out`` (n_combos x series_len
+    f32), an AlmaBatchRange ``sweep`` and the ints ``series_len`` and
+    ``first_valid``.  kernel: "auto", "fp32", "mfma" (split-BF16) or
+    "mfma_f16x3".  The buffers must outlive the pack."""
+
+    def __init__(self, ctx: HipContext, jobs, kernel: str = "auto"):
+        self.ctx = ctx
+        arr = (AlmaJobC * max(len(jobs), 1))()
+        self.combos = []
+        for i, j in enumerate(jobs):
+            T = int(j["series_len"])
+            C = len(expand_grid(j["sweep"]))
+            require_len(j["prices"], T, "prices")
+            require_len(j["out"], C * T, "output")
+            arr[i] = AlmaJobC(j["prices"].ptr, j["out"].ptr, j["sweep"].to_c(), T,
+                              int(j["first_valid"]))
+        self._p = ctypes.c_void_p()
+        check(lib().vta_alma_pack_create(ctx.handle, _POLICY[kernel], arr, len(jobs),
+                                         ctypes.byref(self._p)))
+        self.kernel = _SELECTED[int(lib().vta_alma_pack_kernel(self._p))]
+        self.combos = [int(lib().vta_alma_pack_job_combos(self._p, i)) for i in range(len(jobs))]
+
+    def launch(self) -> None:
+        check(lib().vta_alma_pack_launch(self.ctx.handle, self._p))
+
+    def close(self) -> None:
+        if self._p:
+            lib().vta_alma_pack_destroy(self._p)
+            self._p = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass
 

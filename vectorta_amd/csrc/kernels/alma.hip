@@ -25,18 +25,21 @@
 #define ALMA_TT 1024
 #define ALMA_CB 8
 
-extern "C" __global__ __launch_bounds__(256) void alma_batch_f32(
-    const float *__restrict__ prices, const float *__restrict__ weights,
-    const int32_t *__restrict__ periods, const float *__restrict__ inv_norms, int max_period,
-    int series_len, int n_combos, int first_valid, float *__restrict__ out) {
-    extern __shared__ float4 smem4[];
-    float *smem = reinterpret_cast<float *>(smem4);
+// One block's tile: bars [bx * ALMA_TT, +ALMA_TT) of combos [by * ALMA_CB, +ALMA_CB).
+__device__ __forceinline__ void alma_f32_tile(const float *__restrict__ prices,
+                                              const float *__restrict__ weights,
+                                              const int32_t *__restrict__ periods,
+                                              const float *__restrict__ inv_norms, int max_period,
+                                              int series_len, int n_combos, int first_valid,
+                                              float *__restrict__ out, int bx, int by) {
+    extern __shared__ float4 f32_tile_lds[];
+    float *smem = reinterpret_cast<float *>(f32_tile_lds);
     const int p4max = (max_period + 3) & ~3;
     const int xs_len = p4max + ALMA_TT + 8;
     float *xs = smem;                 // xs[e] = x[t0 - p4max + e]
     float *ws = smem + xs_len;        // [ALMA_CB][p4max]
-    const int t0 = blockIdx.x * ALMA_TT;
-    const int c0 = blockIdx.y * ALMA_CB;
+    const int t0 = bx * ALMA_TT;
+    const int c0 = by * ALMA_CB;
     const int tid = threadIdx.x;
 
     // Staged prices are sanitised (0 outside [first_valid, T) and for
@@ -122,6 +125,37 @@ extern "C" __global__ __launch_bounds__(256) void alma_batch_f32(
             if (t + 3 < series_len) row[t + 3] = r3;
         }
     }
+}
+
+extern "C" __global__ __launch_bounds__(256) void alma_batch_f32(
+    const float *__restrict__ prices, const float *__restrict__ weights,
+    const int32_t *__restrict__ periods, const float *__restrict__ inv_norms, int max_period,
+    int series_len, int n_combos, int first_valid, float *__restrict__ out) {
+    alma_f32_tile(prices, weights, periods, inv_norms, max_period, series_len, n_combos,
+                  first_valid, out, blockIdx.x, blockIdx.y);
+}
+
+// Packed FP32 sweeps: several one-series batches (uneven lengths and grids) in
+// one launch.  Block b runs tile (items[b].y, items[b].z) of job items[b].x, so
+// every job's outputs are those of its own alma_batch_f32 launch.
+struct alma_f32_job {
+    const float *prices;
+    const float *weights;        // [n_combos][max_period]
+    const int32_t *periods;
+    const float *inv_norms;
+    float *out;                  // [n_combos][series_len]
+    int max_period;
+    int series_len;
+    int n_combos;
+    int first_valid;
+};
+
+extern "C" __global__ __launch_bounds__(256) void alma_batch_f32_pack(
+    const alma_f32_job *__restrict__ jobs, const int4 *__restrict__ items) {
+    const int4 it = items[blockIdx.x];
+    const alma_f32_job j = jobs[it.x];
+    alma_f32_tile(j.prices, j.weights, j.periods, j.inv_norms, j.max_period, j.series_len,
+                  j.n_combos, j.first_valid, j.out, it.y, it.z);
 }
 
 // ---------------------------------------------------------------- FP32, time-major out
@@ -1361,13 +1395,13 @@ __device__ __forceinline__ void sw_wave(const alma_sw_args &a, int nb, int tile_
 // block statistics, then SW_NSUB x 4 x sw_stride(kmax) staged dwords (16-byte
 // aligned) and, for XPOSE, four 32 x 36 transpose tiles.
 template <bool F16, bool XPOSE, int RS, bool DEFER>
-__device__ __forceinline__ void alma_sw_entry(const alma_sw_args &a) {
+__device__ __forceinline__ void alma_sw_entry(const alma_sw_args &a, int run, int gidx) {
     constexpr int NSUB = SW_R / RS;
     extern __shared__ __attribute__((aligned(16))) uint32_t sw_lds[];
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
     const int lane = tid & 63;
-    const int4 gd = a.groups[a.group_base + a.n_groups - 1 - (int)blockIdx.y];
+    const int4 gd = a.groups[gidx];
     const int kg = gd.z * 16;
     const int stride = sw_stride(kg, RS);
     const int xsn = 32 * SW_R + a.kmax + 33;
@@ -1383,7 +1417,7 @@ __device__ __forceinline__ void alma_sw_entry(const alma_sw_args &a) {
                   wave * (DEFER ? RS : 1) * 32 * 36;
     const int T = a.series_len;
     const int fv = a.first_valid;
-    const int tile_begin = blockIdx.x * SW_R;
+    const int tile_begin = run * SW_R;
     const int nt = min(SW_R, a.n_tiles - tile_begin);
     const int t_lo = tile_begin * 32 - kg + 1;      // xs index 0
     const int nx = nt * 32 + kg + 1;
@@ -1504,7 +1538,10 @@ __device__ __forceinline__ void alma_sw_entry(const alma_sw_args &a) {
 // _p: deferred stores (sw_wave_deferred; T % 4 == 0 only).
 #define SW_KERNEL(NAME, F16, XPOSE, RS, WPE, DEFER)                                           \
     extern "C" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) \
-    void NAME(alma_sw_args a) { alma_sw_entry<F16, XPOSE, RS, DEFER>(a); }
+    void NAME(alma_sw_args a) {                                                               \
+        alma_sw_entry<F16, XPOSE, RS, DEFER>(a, blockIdx.x,                                   \
+                                             a.group_base + a.n_groups - 1 - (int)blockIdx.y); \
+    }
 SW_KERNEL(alma_sweep_bf16x3, false, true, 2, 2, false)
 SW_KERNEL(alma_sweep_bf16x3_d, false, false, 2, 2, false)
 SW_KERNEL(alma_sweep_bf16x3_r4, false, false, 4, 1, false)
@@ -1514,3 +1551,19 @@ SW_KERNEL(alma_sweep_f16x3_d, true, false, 2, 2, false)
 SW_KERNEL(alma_sweep_f16x3_r4, true, false, 4, 1, false)
 SW_KERNEL(alma_sweep_f16x3_p, true, true, 2, 2, true)
 #undef SW_KERNEL
+
+// Packed sweeps: several one-series batches (uneven lengths and grids) in one
+// launch of the direct-store diagonal form.  Block b runs 16-tile run
+// items[b].y of group items[b].z of job items[b].x (the host orders the items
+// heaviest group first), so every job's outputs are those of its own
+// alma_sweep_*_d launch.
+#define SW_PACK_KERNEL(NAME, F16)                                                             \
+    extern "C" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))   \
+    void NAME(const alma_sw_args *__restrict__ jobs, const int4 *__restrict__ items) {        \
+        const int4 it = items[blockIdx.x];                                                    \
+        const alma_sw_args a = jobs[it.x];                                                    \
+        alma_sw_entry<F16, false, 2, false>(a, it.y, it.z);                                   \
+    }
+SW_PACK_KERNEL(alma_sweep_pack_bf16x3, false)
+SW_PACK_KERNEL(alma_sweep_pack_f16x3, true)
+#undef SW_PACK_KERNEL

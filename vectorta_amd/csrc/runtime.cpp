@@ -363,6 +363,17 @@ vta_status vta_ctx_preload(vta_ctx *ctx) {
         {"alma", "alma_batch_mfma_bf16x3_wg16_t2_klo"},
         {"alma", "alma_batch_f32_direct"},
         {"alma", "alma_ms1p_f32_direct"},
+        {"alma", "alma_sweep_bf16x3"},
+        {"alma", "alma_sweep_bf16x3_d"},
+        {"alma", "alma_sweep_bf16x3_r4"},
+        {"alma", "alma_sweep_bf16x3_p"},
+        {"alma", "alma_sweep_f16x3"},
+        {"alma", "alma_sweep_f16x3_d"},
+        {"alma", "alma_sweep_f16x3_r4"},
+        {"alma", "alma_sweep_f16x3_p"},
+        {"alma", "alma_batch_f32_pack"},
+        {"alma", "alma_sweep_pack_bf16x3"},
+        {"alma", "alma_sweep_pack_f16x3"},
         {"rec", "ema_batch_f32"},
         {"rec", "rsi_batch_f32"},
         {"rec", "atr_batch_f32"},
