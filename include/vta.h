@@ -884,8 +884,8 @@ typedef struct vta_server_stats {
     size_t budget, in_use, peak_in_use;
     size_t queued, in_flight, peak_in_flight;
     size_t cached_series;
-    /* small EMA / RSI / ADX / Bollinger Bands requests served together in packed launches:
-     * launches (packs) and the requests they served */
+    /* small ALMA / EMA / RSI / ADX / Bollinger Bands requests served together in packed
+     * launches: launches (packs) and the requests they served */
     size_t packs, packed_requests;
 } vta_server_stats;
 

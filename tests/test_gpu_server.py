@@ -181,6 +181,42 @@ def test_small_requests_are_packed(pool):
     assert st["packs"] >= 1 and st["packed_requests"] >= 2 * st["packs"], st
 
 
+def test_small_alma_requests_are_packed(pool):
+    """Small ALMA requests queued on one slot are served in packed launches
+    (vta_alma_pack's kernels over the cached per-grid plans): grids of fewer
+    than 16 combos run the FP32 form, larger ones the split-F16 form (AUTO),
+    each group in one launch; every result equals alma_batch_dev bit for bit."""
+    c, _, _ = pool
+    rng = np.random.Generator(np.random.PCG64(84))
+    srv = vta.IndicatorServer(0, n_slots=1, hbm_budget=4 << 30)
+    specs, ids = [], []
+    for i in range(40):
+        s = int(rng.integers(4))
+        L = int(rng.integers(1_000, 30_000))
+        a = int(rng.integers(0, 120_000 - L))
+        p0 = int(rng.integers(5, 30))
+        if i % 2:
+            sweep = vta.AlmaBatchRange(period=(p0, p0 + 5 * int(rng.integers(0, 3)), 5))
+        else:
+            sweep = vta.AlmaBatchRange(period=(p0, p0 + 120, 20), offset=(0.05, 0.95, 0.3),
+                                       sigma=(3.0, 9.0, 6.0))
+        x = c[s, a:a + L]
+        ids.append(srv.submit("alma", x, alma=sweep))
+        specs.append((x, sweep))
+    direct = vta.CudaAlma(0)
+    for rid, (x, sweep) in zip(ids, specs):
+        r = srv.wait(rid)
+        assert r.ok, r.status
+        got = srv.to_host(r, 0)
+        srv.release(rid)
+        ref = direct.alma_batch_dev(x, sweep).to_host()
+        assert np.array_equal(_bits(got), _bits(ref)), (len(x), sweep)
+    st = srv.stats()
+    srv.close()
+    print(f"packs {st['packs']}, packed requests {st['packed_requests']}")
+    assert st["packs"] >= 1 and st["packed_requests"] >= 2 * st["packs"], st
+
+
 def _oracle_rows(spec_kind, x, hh, ll, combos):
     """Oracle outputs (list per output) of the given combos of one request."""
     xd = x.astype(np.float64)

@@ -5,6 +5,7 @@
 #include <cstdlib>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <numeric>
 #include <vector>
 
@@ -774,8 +775,6 @@ vta_status alma_tables_launch(vta_ctx *ctx, const AlmaTables &t, const void *d_t
                          d_out, len);
 }
 
-}  // namespace vta
-
 // Host mirror of struct alma_f32_job (the packed FP32 kernel's job record).
 struct alma_f32_job_host {
     const float *prices;
@@ -790,22 +789,120 @@ struct alma_f32_job_host {
 };
 static_assert(sizeof(alma_f32_job_host) == 56, "alma_f32_job layout");
 
-// One device buffer: every job's tables (t.blob layouts at 256-byte aligned
-// offsets), the kernel's job records, then the work items {job, x, y, 0}.
+vta_status alma_pack_build(vta_ctx *ctx, vta_alma_batch_kernel kernel,
+                           const std::vector<AlmaPackSrc> &jobs, AlmaPackImage &img) {
+    const bool fp32 = kernel == VTA_ALMA_BATCH_FP32;
+    const size_t rec = fp32 ? sizeof(alma_f32_job_host) : sizeof(alma_sw_args_host);
+    const size_t lds_limit = ctx->props.sharedMemPerBlock;
+    img = AlmaPackImage();
+    img.kernel = kernel;
+    img.records.assign(jobs.size() * rec, 0);
+    struct Item {
+        int32_t job, x, y, w;
+    };
+    std::vector<Item> items;
+    for (size_t j = 0; j < jobs.size(); ++j) {
+        const AlmaTables &t = *jobs[j].t;
+        const unsigned char *tb = jobs[j].d_tables;
+        const size_t T = jobs[j].len;
+        if (t.kernel != kernel)
+            return fail(VTA_ERR_INVALID_POLICY, "job %zu: tables of another kernel", j);
+        if (T == 0 || T > (size_t)INT32_MAX || jobs[j].fv >= T)
+            return fail(VTA_ERR_INVALID_INPUT, "job %zu: series_len or first_valid out of range",
+                        j);
+        size_t need;
+        if (fp32) {
+            const size_t p4 = (t.max_period + 3) & ~(size_t)3;
+            need = (p4 + kFp32TT + 8 + kFp32CB * p4) * sizeof(float);
+        } else {
+            need = sweep_lds_bytes(t.kmax, 2, false, false);
+        }
+        if (need > lds_limit)
+            return fail(VTA_ERR_LAUNCH_CONFIG_TOO_LARGE,
+                        "job %zu: its windows need %zu bytes of LDS (limit %zu)", j, need,
+                        lds_limit);
+        img.smem = std::max(img.smem, need);
+        if (fp32) {
+            alma_f32_job_host r{};
+            r.prices = jobs[j].d_prices;
+            r.weights = section<float>(t, tb, AlmaTables::W);
+            r.periods = section<int32_t>(t, tb, AlmaTables::PERIODS);
+            r.inv_norms = section<float>(t, tb, AlmaTables::INV);
+            r.out = jobs[j].d_out;
+            r.max_period = (int)t.max_period;
+            r.series_len = (int)T;
+            r.n_combos = (int)t.n_combos;
+            r.first_valid = (int)jobs[j].fv;
+            memcpy(img.records.data() + j * rec, &r, sizeof r);
+            const int gx = (int)((T + kFp32TT - 1) / kFp32TT);
+            const int gy = (int)((t.n_combos + kFp32CB - 1) / kFp32CB);
+            for (int y = 0; y < gy; ++y)
+                for (int x = 0; x < gx; ++x)
+                    items.push_back({(int32_t)j, x, y, (int)t.max_period});
+        } else {
+            alma_sw_args_host a{};
+            a.prices = jobs[j].d_prices;
+            a.frags = section<void>(t, tb, AlmaTables::FRAGS);
+            a.nblocks = section<void>(t, tb, AlmaTables::NBLOCKS);
+            a.groups = section<void>(t, tb, AlmaTables::GROUPS);
+            a.sorted_period = section<int32_t>(t, tb, AlmaTables::SORTED_PERIOD);
+            a.sorted_row = section<int32_t>(t, tb, AlmaTables::SORTED_ROW);
+            a.w_f32 = section<float>(t, tb, AlmaTables::W_F32);
+            a.cscale = kernel == VTA_ALMA_BATCH_MFMA_F16X3
+                           ? section<void>(t, tb, AlmaTables::CSCALE)
+                           : nullptr;
+            a.out = jobs[j].d_out;
+            a.series_len = (int)T;
+            a.first_valid = (int)jobs[j].fv;
+            a.n_groups = t.n_groups;
+            a.n_tiles = (int)((T + 31) / 32);
+            a.kmax = t.kmax;
+            a.group_base = 0;
+            memcpy(img.records.data() + j * rec, &a, sizeof a);
+            const int32_t *grp =
+                reinterpret_cast<const int32_t *>(t.blob.data() + t.off[AlmaTables::GROUPS]);
+            const int runs = (a.n_tiles + kSweepRunTiles - 1) / kSweepRunTiles;
+            for (int g = 0; g < t.n_groups; ++g)
+                for (int r = 0; r < runs; ++r) items.push_back({(int32_t)j, r, g, grp[4 * g + 2]});
+        }
+        if (items.size() > (size_t)ctx->props.maxGridSize[0])
+            return fail(VTA_ERR_LAUNCH_CONFIG_TOO_LARGE, "pack of more than %d work items",
+                        ctx->props.maxGridSize[0]);
+    }
+    // the heaviest work first (FP32: taps per output; MFMA: K steps of the group)
+    std::stable_sort(items.begin(), items.end(),
+                     [](const Item &a, const Item &b) { return a.w > b.w; });
+    img.items.resize(items.size() * 4);
+    memcpy(img.items.data(), items.data(), items.size() * sizeof(Item));
+    img.n_items = (unsigned)items.size();
+    return VTA_OK;
+}
+
+vta_status alma_pack_enqueue(vta_ctx *ctx, const AlmaPackImage &img, const void *d_records,
+                             const void *d_items) {
+    const char *name = img.kernel == VTA_ALMA_BATCH_FP32          ? "alma_batch_f32_pack"
+                       : img.kernel == VTA_ALMA_BATCH_MFMA_BF16X3 ? "alma_sweep_pack_bf16x3"
+                                                                   : "alma_sweep_pack_f16x3";
+    hipFunction_t f;
+    VTA_TRY(get_function(ctx, "alma", name, &f));
+    ctx->alma_selected = img.kernel;
+    return launch(ctx, f, dim3(img.n_items), dim3(256), (unsigned)img.smem, d_records, d_items);
+}
+
+const AlmaTables &alma_plan_tables(const vta_alma_sweep_plan *plan) { return plan->t; }
+const unsigned char *alma_plan_device(const vta_alma_sweep_plan *plan) {
+    return plan->dev.as<unsigned char>();
+}
+
+}  // namespace vta
+
+// A pack owns its jobs' tables (one buffer) and its records and items (another).
 struct vta_alma_pack {
-    vta_alma_batch_kernel kernel = VTA_ALMA_BATCH_FP32;
-    DevBuf dev;
-    size_t jobs_off = 0, items_off = 0;
-    unsigned n_items = 0;
-    size_t smem = 0;
+    DevBuf tables, work;
+    AlmaPackImage img;
+    size_t items_off = 0;
     std::vector<size_t> combos;
 };
-
-namespace {
-
-size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
-
-}  // namespace
 
 extern "C" {
 
@@ -1068,142 +1165,44 @@ vta_status vta_alma_pack_create(vta_ctx *ctx, vta_alma_batch_kernel kernel,
     const vta_alma_batch_kernel k =
         kernel != VTA_ALMA_BATCH_AUTO ? kernel
                                       : (all_mfma ? VTA_ALMA_BATCH_MFMA_F16X3 : VTA_ALMA_BATCH_FP32);
-    const bool fp32 = k == VTA_ALMA_BATCH_FP32;
     std::vector<AlmaTables> ts(n_jobs);
-    size_t smem = 0;
-    const size_t lds_limit = ctx->props.sharedMemPerBlock;
-    for (size_t j = 0; j < n_jobs; ++j) {
-        VTA_TRY(tables_build(cs[j], mps[j], k, ts[j]));
-        size_t need;
-        if (fp32) {
-            const size_t p4 = (mps[j] + 3) & ~(size_t)3;
-            need = (p4 + kFp32TT + 8 + kFp32CB * p4) * sizeof(float);
-        } else {
-            need = sweep_lds_bytes(ts[j].kmax, 2, false, false);
-        }
-        if (need > lds_limit)
-            return fail(VTA_ERR_LAUNCH_CONFIG_TOO_LARGE,
-                        "job %zu: its windows need %zu bytes of LDS (limit %zu)", j, need,
-                        lds_limit);
-        smem = std::max(smem, need);
-    }
-    // work items, the heaviest first (FP32: taps per output; MFMA: K of the group)
-    struct Item {
-        int32_t job, x, y, w;
-    };
-    std::vector<Item> items;
-    for (size_t j = 0; j < n_jobs; ++j) {
-        const size_t T = jobs[j].series_len;
-        if (fp32) {
-            const int gx = (int)((T + kFp32TT - 1) / kFp32TT);
-            const int gy = (int)((cs[j].size() + kFp32CB - 1) / kFp32CB);
-            for (int y = 0; y < gy; ++y)
-                for (int x = 0; x < gx; ++x) items.push_back({(int32_t)j, x, y, (int)mps[j]});
-        } else {
-            const std::vector<int32_t> grp(
-                reinterpret_cast<const int32_t *>(ts[j].blob.data() + ts[j].off[AlmaTables::GROUPS]),
-                reinterpret_cast<const int32_t *>(ts[j].blob.data() + ts[j].off[AlmaTables::GROUPS]) +
-                    4 * ts[j].n_groups);
-            const int runs = (int)(((T + 31) / 32 + kSweepRunTiles - 1) / kSweepRunTiles);
-            for (int g = 0; g < ts[j].n_groups; ++g)
-                for (int r = 0; r < runs; ++r) items.push_back({(int32_t)j, r, g, grp[4 * g + 2]});
-        }
-        if (items.size() > (size_t)ctx->props.maxGridSize[0])
-            return fail(VTA_ERR_LAUNCH_CONFIG_TOO_LARGE, "pack of more than %d work items",
-                        ctx->props.maxGridSize[0]);
-    }
-    std::stable_sort(items.begin(), items.end(),
-                     [](const Item &a, const Item &b) { return a.w > b.w; });
-    // device layout
     std::vector<size_t> boff(n_jobs);
     size_t total = 0;
     for (size_t j = 0; j < n_jobs; ++j) {
-        boff[j] = align_up(total, 256);
+        VTA_TRY(tables_build(cs[j], mps[j], k, ts[j]));
+        boff[j] = (total + 255) & ~(size_t)255;
         total = boff[j] + ts[j].blob.size();
     }
-    const size_t rec = fp32 ? sizeof(alma_f32_job_host) : sizeof(alma_sw_args_host);
-    const size_t jobs_off = align_up(total, 256);
-    const size_t items_off = align_up(jobs_off + n_jobs * rec, 256);
-    total = items_off + items.size() * sizeof(Item);
-    auto *pk = new vta_alma_pack();
-    vta_status st = pk->dev.alloc(total);
-    if (st != VTA_OK) {
-        delete pk;
-        return st;
-    }
-    const unsigned char *base = pk->dev.as<unsigned char>();
-    std::vector<unsigned char> img(total, 0);
-    for (size_t j = 0; j < n_jobs; ++j) {
-        if (!ts[j].blob.empty()) memcpy(img.data() + boff[j], ts[j].blob.data(), ts[j].blob.size());
-        const AlmaTables &t = ts[j];
-        const unsigned char *tb = base + boff[j];
-        if (fp32) {
-            alma_f32_job_host r{};
-            r.prices = jobs[j].d_prices;
-            r.weights = section<float>(t, tb, AlmaTables::W);
-            r.periods = section<int32_t>(t, tb, AlmaTables::PERIODS);
-            r.inv_norms = section<float>(t, tb, AlmaTables::INV);
-            r.out = jobs[j].d_out;
-            r.max_period = (int)t.max_period;
-            r.series_len = (int)jobs[j].series_len;
-            r.n_combos = (int)t.n_combos;
-            r.first_valid = (int)jobs[j].first_valid;
-            memcpy(img.data() + jobs_off + j * rec, &r, sizeof r);
-        } else {
-            alma_sw_args_host a{};
-            a.prices = jobs[j].d_prices;
-            a.frags = section<void>(t, tb, AlmaTables::FRAGS);
-            a.nblocks = section<void>(t, tb, AlmaTables::NBLOCKS);
-            a.groups = section<void>(t, tb, AlmaTables::GROUPS);
-            a.sorted_period = section<int32_t>(t, tb, AlmaTables::SORTED_PERIOD);
-            a.sorted_row = section<int32_t>(t, tb, AlmaTables::SORTED_ROW);
-            a.w_f32 = section<float>(t, tb, AlmaTables::W_F32);
-            a.cscale = k == VTA_ALMA_BATCH_MFMA_F16X3 ? section<void>(t, tb, AlmaTables::CSCALE)
-                                                       : nullptr;
-            a.out = jobs[j].d_out;
-            a.series_len = (int)jobs[j].series_len;
-            a.first_valid = (int)jobs[j].first_valid;
-            a.n_groups = t.n_groups;
-            a.n_tiles = (int)((jobs[j].series_len + 31) / 32);
-            a.kmax = t.kmax;
-            a.group_base = 0;
-            memcpy(img.data() + jobs_off + j * rec, &a, sizeof a);
-        }
-    }
-    if (!items.empty()) memcpy(img.data() + items_off, items.data(), items.size() * sizeof(Item));
-    hipError_t e = hipMemcpyAsync(pk->dev.p, img.data(), total, hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) {
-        delete pk;
-        return fail(VTA_ERR_HIP, "pack upload: %s", hipGetErrorString(e));
-    }
-    pk->kernel = k;
-    pk->jobs_off = jobs_off;
-    pk->items_off = items_off;
-    pk->n_items = (unsigned)items.size();
-    pk->smem = smem;
+    auto pk = std::make_unique<vta_alma_pack>();
+    std::vector<unsigned char> blob(total, 0);
+    for (size_t j = 0; j < n_jobs; ++j)
+        if (!ts[j].blob.empty()) memcpy(blob.data() + boff[j], ts[j].blob.data(), ts[j].blob.size());
+    VTA_TRY(upload(ctx, pk->tables, blob.data(), total));
+    std::vector<AlmaPackSrc> src(n_jobs);
+    for (size_t j = 0; j < n_jobs; ++j)
+        src[j] = {&ts[j], pk->tables.as<unsigned char>() + boff[j], jobs[j].d_prices,
+                  jobs[j].d_out, jobs[j].series_len, jobs[j].first_valid};
+    VTA_TRY(alma_pack_build(ctx, k, src, pk->img));
+    pk->items_off = (pk->img.records.size() + 255) & ~(size_t)255;
+    std::vector<unsigned char> work(pk->items_off + pk->img.items.size() * sizeof(int32_t), 0);
+    memcpy(work.data(), pk->img.records.data(), pk->img.records.size());
+    memcpy(work.data() + pk->items_off, pk->img.items.data(),
+           pk->img.items.size() * sizeof(int32_t));
+    VTA_TRY(upload(ctx, pk->work, work.data(), work.size()));
     for (auto &c : cs) pk->combos.push_back(c.size());
-    *out = pk;
+    *out = pk.release();
     return VTA_OK;
 }
 
 vta_status vta_alma_pack_launch(vta_ctx *ctx, const vta_alma_pack *pack) {
     clear_error();
     if (!ctx || !pack) return fail(VTA_ERR_INVALID_INPUT, "null argument");
-    const char *name = pack->kernel == VTA_ALMA_BATCH_FP32          ? "alma_batch_f32_pack"
-                       : pack->kernel == VTA_ALMA_BATCH_MFMA_BF16X3 ? "alma_sweep_pack_bf16x3"
-                                                                     : "alma_sweep_pack_f16x3";
-    hipFunction_t f;
-    VTA_TRY(get_function(ctx, "alma", name, &f));
-    const unsigned char *base = pack->dev.as<unsigned char>();
-    const void *d_jobs = base + pack->jobs_off;
-    const void *d_items = base + pack->items_off;
-    ctx->alma_selected = pack->kernel;
-    return launch(ctx, f, dim3(pack->n_items), dim3(256), (unsigned)pack->smem, d_jobs, d_items);
+    const unsigned char *w = pack->work.as<unsigned char>();
+    return alma_pack_enqueue(ctx, pack->img, w, w + pack->items_off);
 }
 
 vta_alma_batch_kernel vta_alma_pack_kernel(const vta_alma_pack *pack) {
-    return pack ? pack->kernel : VTA_ALMA_BATCH_AUTO;
+    return pack ? pack->img.kernel : VTA_ALMA_BATCH_AUTO;
 }
 
 size_t vta_alma_pack_job_combos(const vta_alma_pack *pack, size_t job) {

@@ -180,6 +180,34 @@ vta_status alma_tables_for_range(vta_ctx *ctx, const vta_alma_batch_range *range
 vta_status alma_tables_launch(vta_ctx *ctx, const AlmaTables &t, const void *d_tables,
                               const float *d_prices, size_t len, size_t fv, float *d_out);
 
+// Packed sweeps over existing tables: one job per one-series sweep, its tables
+// (t, at device address d_tables, all of one kernel), prices, output, length
+// and first_valid.  alma_pack_build fills the kernel's job records and work
+// items (4 int32 each, heaviest first) and the launch's dynamic LDS;
+// alma_pack_enqueue launches them from device copies of records and items.
+struct AlmaPackSrc {
+    const AlmaTables *t;
+    const unsigned char *d_tables;
+    const float *d_prices;
+    float *d_out;
+    size_t len;
+    size_t fv;
+};
+struct AlmaPackImage {
+    vta_alma_batch_kernel kernel = VTA_ALMA_BATCH_FP32;
+    std::vector<unsigned char> records;
+    std::vector<int32_t> items;
+    unsigned n_items = 0;
+    size_t smem = 0;
+};
+vta_status alma_pack_build(vta_ctx *ctx, vta_alma_batch_kernel kernel,
+                           const std::vector<AlmaPackSrc> &jobs, AlmaPackImage &img);
+vta_status alma_pack_enqueue(vta_ctx *ctx, const AlmaPackImage &img, const void *d_records,
+                             const void *d_items);
+// A sweep plan's tables and their device copy.
+const AlmaTables &alma_plan_tables(const vta_alma_sweep_plan *plan);
+const unsigned char *alma_plan_device(const vta_alma_sweep_plan *plan);
+
 // Enqueues the check of the reference's per-combo warm indices against
 // first_valid + period - 1 (kernel vta_check_warms); first_valid < 0 derives
 // first_valid from combo 0 into *d_fv_out (SWMA's device form has no

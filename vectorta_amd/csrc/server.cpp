@@ -291,7 +291,7 @@ std::shared_ptr<Req> try_admit(vta_server *s) {
     return nullptr;
 }
 
-// Small one-series requests (EMA, RSI, ADX, Bollinger Bands) are served together:
+// Small one-series requests (ALMA, EMA, RSI, ADX, Bollinger Bands) are served together:
 // a slot that admits one also admits queued requests of the same kind, from
 // the first kPackScan in the queue, up to kPackMax, and runs them as one
 // packed batch (one launch per scan pass for all of them).  Small = at most
@@ -303,7 +303,7 @@ constexpr size_t kPackScan = 64;
 
 bool packable(const Req &q) {
     return (q.r.kind == VTA_REQ_EMA || q.r.kind == VTA_REQ_RSI || q.r.kind == VTA_REQ_ADX ||
-            q.r.kind == VTA_REQ_BB) &&
+            q.r.kind == VTA_REQ_BB || q.r.kind == VTA_REQ_ALMA) &&
            q.out_bytes <= kPackOutputs * sizeof(float);
 }
 
@@ -724,10 +724,107 @@ bool run_packed_bb(vta_server *s, vta_ctx *ctx, std::vector<std::shared_ptr<Req>
 // status; a member that fails validation is left out of the pack.  Returns
 // false (nothing launched, st untouched) when the arena cannot hold the
 // pack's table and carry, and the members are then run one by one.
+// The packed form of run_request for a group of ALMA requests: the requests
+// whose grid's cached sweep plan runs the first one's kernel (the split-F16
+// form for 16+ combos under the servers' AUTO policy, else FP32) share one
+// launch over their plans' tables (vta_alma_pack's kernels, bit-identical to
+// each plan's own launch); the others are served alone.
+bool run_packed_alma(vta_server *s, vta_ctx *ctx, std::vector<std::shared_ptr<Req>> &g,
+                     std::vector<vta_status> &st) {
+    std::vector<AlmaPackSrc> src;
+    std::vector<size_t> member, alone;
+    int kernel = -1;
+    for (size_t i = 0; i < g.size(); ++i) {
+        Req &q = *g[i];
+        const CacheEntry &e = *q.entry;
+        const SeriesMeta &m = e.meta;
+        const size_t T = m.len;
+        vta_alma_sweep_plan *plan = nullptr;
+        size_t mp = 0, nc = 0;
+        vta_status v = alma_plan_for(s, ctx, q.r.alma, &plan, &mp, &nc);
+        if (v == VTA_OK && ((size_t)m.fv_close >= T || T - (size_t)m.fv_close < mp))
+            v = fail(VTA_ERR_INVALID_INPUT, "not enough valid data (needed >= %zu)", mp);
+        if (v != VTA_OK) {
+            st[i] = v;
+            continue;
+        }
+        const AlmaTables &t = alma_plan_tables(plan);
+        if (kernel < 0) kernel = (int)t.kernel;
+        if ((int)t.kernel != kernel) {
+            alone.push_back(i);
+            continue;
+        }
+        v = alloc_out(s, q.res.out[0], nc, T);
+        if (v != VTA_OK) {
+            st[i] = v;
+            continue;
+        }
+        q.res.n_outputs = 1;
+        q.res.n_combos = nc;
+        src.push_back({&t, alma_plan_device(plan), e.d_close, q.res.out[0].ptr, T,
+                       (size_t)m.fv_close});
+        member.push_back(i);
+    }
+    vta_status v = VTA_OK;
+    AlmaPackImage img;
+    void *d_rec = nullptr, *d_items = nullptr;
+    if (!src.empty()) {
+        v = alma_pack_build(ctx, (vta_alma_batch_kernel)kernel, src, img);
+        if (v == VTA_OK) {
+            d_rec = s->arena.alloc(img.records.size());
+            d_items = s->arena.alloc(img.items.size() * sizeof(int32_t));
+            if (!d_rec || !d_items) v = fail(VTA_ERR_OUT_OF_MEMORY, "device arena exhausted");
+        }
+        if (v != VTA_OK) {
+            // served one by one instead
+            for (void *p : {d_rec, d_items})
+                if (p) s->arena.release(p);
+            d_rec = d_items = nullptr;
+            for (size_t i : member) {
+                s->arena.release(g[i]->res.out[0].ptr);
+                g[i]->res.out[0].ptr = nullptr;
+                alone.push_back(i);
+            }
+            member.clear();
+            v = VTA_OK;
+        }
+    }
+    if (!member.empty()) {
+        hipError_t e = hipMemcpyAsync(d_rec, img.records.data(), img.records.size(),
+                                      hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_items, img.items.data(), img.items.size() * sizeof(int32_t),
+                               hipMemcpyHostToDevice, ctx->stream);
+        if (e != hipSuccess) v = fail(VTA_ERR_HIP, "hipMemcpyAsync: %s", hipGetErrorString(e));
+        if (v == VTA_OK) v = alma_pack_enqueue(ctx, img, d_rec, d_items);
+        const double t_launch = now_s() - s->t0;
+        e = hipStreamSynchronize(ctx->stream);
+        if (v == VTA_OK && e != hipSuccess)
+            v = fail(VTA_ERR_HIP, "hipStreamSynchronize: %s", hipGetErrorString(e));
+        for (size_t i : member) {
+            st[i] = v;
+            g[i]->res.t_launch = t_launch;
+        }
+    }
+    for (void *p : {d_rec, d_items})
+        if (p) s->arena.release(p);
+    for (size_t i : alone) {
+        clear_error();
+        st[i] = run_request(s, ctx, *g[i]);
+    }
+    if (!member.empty()) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->packs += 1;
+        s->packed_requests += member.size();
+    }
+    return true;
+}
+
 bool run_packed(vta_server *s, vta_ctx *ctx, std::vector<std::shared_ptr<Req>> &g,
                 std::vector<vta_status> &st) {
     const int kind = g[0]->r.kind;
     if (kind == VTA_REQ_BB) return run_packed_bb(s, ctx, g, st);
+    if (kind == VTA_REQ_ALMA) return run_packed_alma(s, ctx, g, st);
     const int rk = kind == VTA_REQ_EMA ? VTA_REC_EMA : kind == VTA_REQ_RSI ? VTA_REC_RSI
                                                                              : VTA_REC_ADX;
     std::vector<vta_rec_job> jobs;
