@@ -475,8 +475,10 @@ def bench_packed_batch(steps, warmup, ev):
     """64 one-series requests of one indicator (lengths log-uniform 1k-50k
     bars, grids log-uniform 1-128 periods, seeded), inputs resident: one
     vta_<ind>_batch_device call per request (back to back on one stream) vs
-    one packed launch (vta_rec_pack_launch).  Outputs are bit-identical; both
-    are timed between two HIP events over `steps` repetitions."""
+    one packed launch (vta_rec_pack_launch); and 64 ALMA sweeps of 16-256
+    combos, one sweep-plan launch each vs one vta_alma_pack launch.  Outputs
+    are bit-identical; both are timed between two HIP events over `steps`
+    repetitions."""
     L = lib()
     P = ctypes.c_void_p
     rng = np.random.Generator(np.random.PCG64(31))
@@ -526,6 +528,44 @@ def bench_packed_batch(steps, warmup, ev):
         for d in bufs:
             for b in d.values():
                 b.free()
+    # ALMA: 64 sweeps of 16..256 combos (periods 10..250, offsets 0.05..0.95)
+    # over the same lengths, split-F16 (AUTO): one sweep-plan launch per request
+    # vs one vta_alma_pack launch
+    alma = vta.CudaAlma(0)
+    jobs, bufs, plans, outs = [], [], [], 0
+    for i in range(n):
+        T = int(lens[i])
+        np_, no = int(rng.integers(4, 17)), int(rng.integers(4, 17))
+        sw = vta.AlmaBatchRange(period=(10, 10 + 16 * (np_ - 1), 16),
+                                offset=(0.05, 0.95, 0.9 / (no - 1)), sigma=(6.0, 6.0, 0.0))
+        C = len(vta.expand_grid(sw))
+        x = datagen.gbm_close(T, 0.01, seed=700 + i)
+        d = {"x": alma.ctx.to_device(x), "out": alma.ctx.alloc(C * T * 4)}
+        jobs.append({"prices": d["x"], "out": d["out"], "sweep": sw, "series_len": T,
+                     "first_valid": 0})
+        plans.append(alma.sweep_plan(sw, T, kernel="mfma_f16x3"))
+        bufs.append(d)
+        outs += C * T
+    pack = vta.AlmaPack(alma.ctx, jobs)
+
+    def direct_alma():
+        for pl, d in zip(plans, bufs):
+            pl.launch(d["x"].ptr, 0, d["out"].ptr)
+
+    r = {"kernel": pack.kernel, "outputs_per_batch": outs}
+    for name, fn in (("direct_calls", direct_alma), ("packed", pack.launch)):
+        mean, ks, wall = _time_launches(fn, alma.ctx.stream, steps, warmup, ev,
+                                        sync=alma.synchronize)
+        r[name] = {"gpu_ms_mean": mean, "gpu_ms_p50": float(np.percentile(ks, 50)),
+                   "wall_ms_per_batch": wall}
+    r["speedup"] = r["direct_calls"]["gpu_ms_mean"] / r["packed"]["gpu_ms_mean"]
+    res["alma"] = r
+    pack.close()
+    for pl in plans:
+        pl.close()
+    for d in bufs:
+        for b in d.values():
+            b.free()
     return res
 
 
