@@ -16,10 +16,13 @@
 //
 // PNR and MAD keep each lane's window sorted in LDS (column layout, stride 64
 // lanes, so every lane hits its own bank): one binary search for the outgoing
-// value, one for the incoming value, and a single shift of the span between
-// them.  These kernels are bound by LDS throughput: batching the shift or
-// replacing the binary search by block counts (fewer dependent reads, more
-// reads in total) measured slower.  NaN bars are kept as +Inf keys and counted, so every step is one
+// value, one for the incoming value (run interleaved), and a single shift of
+// the span between them (four moves per batch).  Each step is a chain of
+// dependent LDS round trips and ~1,000 instructions at 1.5 waves per SIMD
+// (the windows' LDS caps residency), so the kernels are latency-bound: LDS is
+// busy ~40% of the cycles and waves wait about half their time.  The
+// many-series forms run time wave-uniform with their bars loaded a block
+// ahead.  NaN bars are kept as +Inf keys and counted, so every step is one
 // replace.  PNR reads its rank among the first (window - NaNs) keys; MAD takes
 // the median and selects from the two monotone deviation sequences either side
 // of it by a two-array order-statistic search.  All comparisons are on the
@@ -352,24 +355,65 @@ struct SortedWin {
     }
     __device__ void insert(float v, int steps) {
         const int pos = lower(v, steps);
-        for (int i = n; i > pos; --i) col[64 * i] = col[64 * (i - 1)];
+        int i = n;
+        for (; i - 4 >= pos; i -= 4) {
+            const float a = col[64 * (i - 1)], b = col[64 * (i - 2)], c = col[64 * (i - 3)],
+                        d = col[64 * (i - 4)];
+            col[64 * i] = a;
+            col[64 * (i - 1)] = b;
+            col[64 * (i - 2)] = c;
+            col[64 * (i - 3)] = d;
+        }
+        for (; i > pos; --i) col[64 * i] = col[64 * (i - 1)];
         col[64 * pos] = v;
         ++n;
     }
+    __device__ void set(int i, float v) { col[64 * i] = v; }
     // Replaces one element equal to `vo` (present) by `vi`: the elements
-    // strictly between the two positions move one slot towards vo's slot, one
-    // at a time.  (Batches of eight reads then eight writes, with clamped
-    // reads and masked writes for the lanes' different spans, and the two
-    // searches interleaved, measured 2x slower: the wave is bound by LDS
-    // instruction issue, and the clamped reads add instructions.)
+    // strictly between the two positions move one slot towards vo's slot.
+    // The step is bound by dependent LDS round trips, not LDS bandwidth, so
+    // the two binary searches run interleaved (their reads in flight
+    // together) and the moves go in batches of four whose reads are all
+    // issued before their writes (a batch's reads lie past the previous
+    // batch's writes), then at most three single moves.
     __device__ void replace(float vo, float vi, int steps) {
-        const int po = lower(vo, steps);
-        const int pi = lower(vi, steps);
-        const int dir = pi > po ? 1 : -1;
-        const int span = pi > po ? pi - 1 - po : po - pi;
-        int i = po;
-        for (int s = 0; s < span; ++s, i += dir) col[64 * i] = col[64 * (i + dir)];
-        col[64 * i] = vi;
+        int lo_o = 0, hi_o = n, lo_i = 0, hi_i = n;
+        for (int r = 0; r < steps; ++r) {
+            const int mo = (lo_o + hi_o) >> 1, mi = (lo_i + hi_i) >> 1;
+            const float ao = at(min(mo, n - 1)), ai = at(min(mi, n - 1));
+            if (lo_o < hi_o) {
+                if (ao < vo) lo_o = mo + 1; else hi_o = mo;
+            }
+            if (lo_i < hi_i) {
+                if (ai < vi) lo_i = mi + 1; else hi_i = mi;
+            }
+        }
+        const int po = lo_o, pi = lo_i;
+        int j = po;
+        if (pi > po) {
+            // col[j] = col[j + 1] for j in [po, pi - 1)
+            int span = pi - 1 - po;
+            for (; span >= 4; span -= 4, j += 4) {
+                const float a = at(j + 1), b = at(j + 2), c = at(j + 3), d = at(j + 4);
+                set(j, a);
+                set(j + 1, b);
+                set(j + 2, c);
+                set(j + 3, d);
+            }
+            for (; span > 0; --span, ++j) set(j, at(j + 1));
+        } else {
+            // col[j] = col[j - 1] for j from po down to pi + 1
+            int span = po - pi;
+            for (; span >= 4; span -= 4, j -= 4) {
+                const float a = at(j - 1), b = at(j - 2), c = at(j - 3), d = at(j - 4);
+                set(j, a);
+                set(j - 1, b);
+                set(j - 2, c);
+                set(j - 3, d);
+            }
+            for (; span > 0; --span, --j) set(j, at(j - 1));
+        }
+        set(j, vi);
     }
 };
 
@@ -402,6 +446,54 @@ __device__ __forceinline__ int fill_window(SortedWin &w, int t, int len, int ste
         w.insert(rl_key(v), steps);
     }
     return nan_cnt;
+}
+
+// One lane's sorted window over the output bars [c0, c1) of a many-series
+// chunk: first output bar `first` (the window is filled there), window `len`.
+// Time runs wave-uniform, so every bar's loads and stores are one row of the
+// time-major matrix, and bars arrive in blocks of RW_B whose loads (incoming
+// x[t], outgoing x[t - len]) are issued one block ahead: each step's LDS
+// work then never waits on a global load.  out(w, nan_cnt) gives the output
+// of the current window; st(t, v) stores it (NaN before `first`).
+#define RW_B 8
+template <class LD, class ST, class OUT>
+__device__ __forceinline__ void window_run(SortedWin &w, int len, int first, int c0, int c1,
+                                           bool active, int steps, LD ld, ST st, OUT out) {
+    float ci[RW_B], co[RW_B], ni[RW_B], no[RW_B];
+    auto load = [&](int t0, float *bi, float *bo) {
+#pragma unroll
+        for (int k = 0; k < RW_B; ++k) {
+            const int t = min(t0 + k, c1 - 1);
+            bi[k] = (float)ld(t);
+            bo[k] = (float)ld(max(t - len, 0));
+        }
+    };
+    int nan_cnt = 0;
+    load(c0, ci, co);
+    for (int t0 = c0; t0 < c1; t0 += RW_B) {
+        if (t0 + RW_B < c1) load(t0 + RW_B, ni, no);
+#pragma unroll
+        for (int k = 0; k < RW_B; ++k) {
+            const int t = t0 + k;
+            if (t >= c1) break;
+            float o = VTA_QNAN_F32;
+            if (active && t >= first) {
+                if (t == first) {
+                    nan_cnt = fill_window(w, t, len, steps, ld);
+                } else {
+                    nan_cnt += (isnan(ci[k]) ? 1 : 0) - (isnan(co[k]) ? 1 : 0);
+                    w.replace(rl_key(co[k]), rl_key(ci[k]), steps);
+                }
+                o = out(w, nan_cnt);
+            }
+            if (active) st(t, o);
+        }
+#pragma unroll
+        for (int k = 0; k < RW_B; ++k) {
+            ci[k] = ni[k];
+            co[k] = no[k];
+        }
+    }
 }
 
 // ---------------------------------------------------------------- PNR
@@ -443,8 +535,15 @@ extern "C" __global__ __launch_bounds__(64) void pnr_ms1p_f32(roll_args a) {
     const int sc = min(s, N - 1);
     auto ld = [&](int t) -> double { return (double)a.x[(size_t)t * N + sc]; };
     auto st = [&](int t, float v) { a.out0[(size_t)t * N + s] = v; };
-    pnr_run(w, a.p1, a.a1, active ? a.fvs[s] : a.T, c0, c1, active, bsearch_steps(a.wcap), ld,
-            st);
+    const int len = a.p1;
+    const double p_frac = a.a1 * 0.01;
+    const int fv = active ? a.fvs[s] : a.T;
+    const int first = max(c0, fv + len - 1);
+    auto out = [&](const SortedWin &win, int nan_cnt) -> float {
+        const int wl = len - nan_cnt;
+        return wl == 0 ? VTA_QNAN_F32 : win.at(pnr_rank(p_frac, wl));
+    };
+    window_run(w, len, first, c0, c1, active, bsearch_steps(a.wcap), ld, st, out);
 }
 
 // Lockstep one-series batch: lane = (length, percentage) combo; dynamic LDS =
@@ -573,7 +672,18 @@ extern "C" __global__ __launch_bounds__(64) void mad_ms1p_f32(roll_args a) {
     const int sc = min(s, N - 1);
     auto ld = [&](int t) -> double { return (double)a.x[(size_t)t * N + sc]; };
     auto st = [&](int t, float v) { a.out0[(size_t)t * N + s] = v; };
-    mad_run(w, a.p1, active ? a.fvs[s] : a.T, c0, c1, active, bsearch_steps(a.wcap), ld, st);
+    const int p = a.p1;
+    const int steps = bsearch_steps(a.wcap);
+    const int fv = active ? a.fvs[s] : a.T;
+    if (p == 1) {
+        mad_run(w, p, fv, c0, c1, active, steps, ld, st);
+        return;
+    }
+    // any NaN in the window makes the output NaN (medium_ad.rs:343-348)
+    auto out = [&](const SortedWin &win, int nan_cnt) -> float {
+        return nan_cnt == 0 ? mad_value(win, p, steps) : VTA_QNAN_F32;
+    };
+    window_run(w, p, max(c0, fv + p - 1), c0, c1, active, steps, ld, st, out);
 }
 
 // Lockstep one-series batch: lane = period; dynamic LDS as pnr_batch_f32.
