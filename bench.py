@@ -327,9 +327,13 @@ def bench_rolling(steps, warmup, ev):
     plan = vta.RollingManySeriesPlan(ctx, x)
     x_host = x
     res = {"workload": "bb20_pnr100p70_mad64_many_series_1024x50000"}
-    for name, fn, nout, kern in (("bb", lambda: plan.launch_bb(20, 2.0, 2.0), 3, "bb_ms1p_f32"),
-                                 ("pnr", lambda: plan.launch_pnr(100, 70.0), 1, "pnr_ms1p_f32"),
-                                 ("mad", lambda: plan.launch_mad(64), 1, "mad_ms1p_f32")):
+    from vectorta_amd import autotune as at
+    # PNR window 100 and MAD window 64 run the register-window kernels (window
+    # rounded up to 16 slots); the LDS sorted-window form is timed beside them
+    for name, fn, nout, kern, lds_kern in (
+            ("bb", lambda: plan.launch_bb(20, 2.0, 2.0), 3, "bb_ms1p_f32", None),
+            ("pnr", lambda: plan.launch_pnr(100, 70.0), 1, "pnr_ms1p_reg112", "pnr_ms1p_f32"),
+            ("mad", lambda: plan.launch_mad(64), 1, "mad_ms1p_reg64", "mad_ms1p_f32")):
         mean, ks, _ = _time_launches(fn, ctx.stream, steps, warmup, ev, sync=ctx.synchronize)
         t = mean * 1e-3
         alg = 4 * rows * cols * (1 + nout)
@@ -340,6 +344,12 @@ def bench_rolling(steps, warmup, ev):
                      "traffic": profiled_traffic("rolling_windows_1024x50000_traffic.txt", kern),
                      "achieved_GBs": alg / t / 1e9,
                      "frac_of_hbm_peak": alg / t / 1e9 / HBM_PEAK_GBS}
+        if lds_kern:
+            at.set_tuning(ctx, at.ROLL_FORM, 1)
+            lds_mean, _, _ = _time_launches(fn, ctx.stream, steps, warmup, ev,
+                                            sync=ctx.synchronize)
+            at.set_tuning(ctx, at.ROLL_FORM, 0)
+            res[name]["lds_form"] = {"kernel": lds_kern, "kernel_ms_mean": lds_mean}
     if CPU_EXTRA:
         # the oracle's scalar bollinger_bands / pnr / medium_ad per series on a
         # bounded sample of contiguous f64 series (medium_ad sorts every

@@ -87,7 +87,12 @@ typedef enum vta_tune_knob {
     VTA_TUNE_SCAN_WAVES = 1,  /* recurrence batch chunked scan: waves per pass, 64..65536 */
     VTA_TUNE_FIR_GROUP = 2,   /* SMA/WMA/SWMA batch: combos per workgroup, 1..256 */
     VTA_TUNE_MS_CHUNK = 3,    /* many-series recurrences: bars per chunk, >= 64 */
-    VTA_TUNE_KNOB_COUNT = 4
+    /* many-series percentile_nearest_rank / medium_ad window form: 1 sorted
+     * windows in LDS (any length), 2 sorted windows in registers (PNR lengths
+     * up to 128, MAD periods up to 64; longer windows run in LDS); the
+     * built-in choice is 2 where it applies */
+    VTA_TUNE_ROLL_FORM = 4,
+    VTA_TUNE_KNOB_COUNT = 5
 } vta_tune_knob;
 vta_status vta_ctx_set_tuning(vta_ctx *ctx, vta_tune_knob knob, int64_t value);
 vta_status vta_ctx_get_tuning(const vta_ctx *ctx, vta_tune_knob knob, int64_t *value);

@@ -30,7 +30,7 @@ def test_cache_round_trip(tmp_path):
 def test_candidates_are_in_range():
     # the ranges vta_ctx_set_tuning accepts (include/vta.h, vta_tune_knob)
     lo_hi = {at.ALMA_MFMA_VARIANT: (1, 11), at.SCAN_WAVES: (64, 65536), at.FIR_GROUP: (1, 256),
-             at.MS_CHUNK: (64, 1 << 30)}
+             at.MS_CHUNK: (64, 1 << 30), at.ROLL_FORM: (1, 2)}
     for knob, cands in at.CANDIDATES.items():
         lo, hi = lo_hi[knob]
         assert all(lo <= c <= hi for c in cands), knob

@@ -292,3 +292,136 @@ def test_rolling_plan_enqueue_only(ops):
     plan.launch_mad(64)
     ctx.synchronize()
     _exact(plan.outputs()[0][:, sel], O.many_series("medium_ad", xd, 64)[0], "plan mad")
+
+
+# ---------------------------------------------------------------- window forms
+
+def _ragged_panel(rows, cols, seed, quant=None):
+    """Time-major panel: GBM columns with ragged NaN leads, interior NaN runs,
+    a few +-Inf bars and (quant) values rounded to a grid so windows hold ties."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    x = np.stack([datagen.gbm_close(rows, 0.01, seed=int(s)) for s in rng.integers(1 << 30, size=cols)],
+                 axis=1).astype(np.float32)
+    if quant:
+        x = (np.round(x / quant) * quant).astype(np.float32)
+    lead = rng.integers(0, rows // 4, size=cols)
+    lead[:3] = (0, 1, rows - 200)               # one column with few valid bars
+    for s in range(cols):
+        x[:lead[s], s] = np.nan
+    for s in rng.integers(0, cols, size=cols // 4):
+        t = int(rng.integers(rows // 4, rows - 10))
+        x[t:t + int(rng.integers(1, 6)), s] = np.nan
+    x[rows // 2, 5] = np.inf
+    x[rows // 3, 6] = -np.inf
+    x[rows // 2 + 7, 8] = 1e-40                 # denormal keys
+    x[rows // 2 + 9, 8] = 3e-41
+    return x
+
+
+def _same_bits(a, b, what):
+    """Bit-identical values with identical NaN positions (NaN payloads aside:
+    a NaN computed from infinities may carry another sign or payload than the
+    canonical quiet NaN)."""
+    nan_mask_equal(a, b)
+    m = ~np.isnan(a)
+    assert np.array_equal(a.view(np.uint32)[m], b.view(np.uint32)[m]), what
+
+
+def _with_form(ctx, form, fn):
+    from vectorta_amd import autotune as at
+    at.set_tuning(ctx, at.ROLL_FORM, form)
+    try:
+        return fn()
+    finally:
+        at.set_tuning(ctx, at.ROLL_FORM, 0)
+
+
+@pytest.mark.parametrize("chunk", [None, "160"])
+def test_pnr_window_forms_many_series(ops, monkeypatch, chunk):
+    """Many-series PNR in the register form (every capacity 16..128, lengths at
+    and either side of each 16-slot boundary, percentages 0..100) and in the
+    LDS form: ragged leads, NaN runs, +-Inf, denormals and tied values, every
+    column bit-exact against the oracle, and the two forms bit-identical.
+    Lengths above 128 run the LDS form under either setting."""
+    if chunk:
+        monkeypatch.setenv("VTA_ROLL_CHUNK", chunk)
+    rows, cols = 2_500, 70
+    x = _ragged_panel(rows, cols, seed=91, quant=0.05)
+    xd = x.astype(np.float64)
+    pnr = ops["pnr"]
+    cases = [(1, 50.0), (2, 0.0), (15, 70.0), (16, 100.0), (17, 33.3), (31, 50.0), (47, 99.9),
+             (64, 12.5), (65, 70.0), (100, 70.0), (112, 1.0), (127, 50.0), (128, 87.5),
+             (129, 70.0)]
+    for length, pct in cases:
+        ref, st = O.many_series("pnr", xd, length, pct)
+        outs = {}
+        for form in (1, 2):
+            g = _with_form(pnr.ctx, form, lambda: pnr.pnr_many_series_one_param_time_major_dev(
+                x, cols, rows, length, pct).to_host())
+            ok = st == 0
+            _exact(g[:, ok], ref[:, ok], f"pnr form {form} len {length} pct {pct}")
+            assert np.isnan(g[:, ~ok]).all()
+            outs[form] = g
+        _same_bits(outs[1], outs[2], f"pnr forms len {length}")
+
+
+@pytest.mark.parametrize("chunk", [None, "160"])
+def test_mad_window_forms_many_series(ops, monkeypatch, chunk):
+    """Many-series MAD in the register form (capacities 16..64, even and odd
+    periods either side of each boundary, period 1) and in the LDS form: the
+    same ragged panel, every column bit-exact against the oracle, the two
+    forms bit-identical; periods above 64 run the LDS form."""
+    if chunk:
+        monkeypatch.setenv("VTA_ROLL_CHUNK", chunk)
+    rows, cols = 2_000, 70
+    x = _ragged_panel(rows, cols, seed=92, quant=0.05)
+    xd = x.astype(np.float64)
+    mad = ops["mad"]
+    for p in (1, 2, 3, 15, 16, 17, 31, 32, 33, 48, 49, 63, 64, 65):
+        ref, st = O.many_series("medium_ad", xd, p)
+        outs = {}
+        for form in (1, 2):
+            g = _with_form(mad.ctx, form, lambda: mad.medium_ad_many_series_one_param_time_major_dev(
+                x, cols, rows, p).to_host())
+            ok = st == 0
+            _exact(g[:, ok], ref[:, ok], f"mad form {form} period {p}")
+            assert np.isnan(g[:, ~ok]).all()
+            outs[form] = g
+        _same_bits(outs[1], outs[2], f"mad forms period {p}")
+
+
+def test_window_forms_signed_zeros(ops, monkeypatch):
+    """Windows holding -0.0 and +0.0 together.  The oracle erases the first key
+    equal to the outgoing bar, so which zero sign its window keeps depends on
+    the whole history.  One chunk per series replays that history: PNR bit-exact
+    in both forms (the register form takes its compare path whenever a -0.0
+    key is present).  With chunks, every PNR value still equals the oracle's
+    numerically (only a zero's sign may differ).  MAD's value does not depend
+    on zero signs: bit-exact either way."""
+    rows, cols = 1_500, 64
+    rng = np.random.Generator(np.random.PCG64(93))
+    x = np.round(rng.normal(0.0, 1.0, size=(rows, cols))).astype(np.float32)
+    x[rng.random((rows, cols)) < 0.2] = -0.0
+    x[:3, :] = np.nan
+    xd = x.astype(np.float64)
+    pnr, mad = ops["pnr"], ops["mad"]
+    for chunk in (str(rows), "200"):
+        monkeypatch.setenv("VTA_ROLL_CHUNK", chunk)
+        for length, pct in ((9, 50.0), (40, 50.0), (100, 30.0)):
+            ref, _ = O.many_series("pnr", xd, length, pct)
+            for form in (1, 2):
+                g = _with_form(pnr.ctx, form, lambda: pnr.pnr_many_series_one_param_time_major_dev(
+                    x, cols, rows, length, pct).to_host())
+                r32 = ref.astype(np.float32)
+                if chunk == str(rows):
+                    _exact(g, ref, f"pnr zeros form {form} len {length}")
+                else:
+                    nan_mask_equal(g, r32)
+                    m = ~np.isnan(r32)
+                    assert np.array_equal(g[m], r32[m]), (form, length)
+        for p in (8, 33, 64):
+            ref, _ = O.many_series("medium_ad", xd, p)
+            for form in (1, 2):
+                g = _with_form(mad.ctx, form, lambda: mad.medium_ad_many_series_one_param_time_major_dev(
+                    x, cols, rows, p).to_host())
+                _exact(g, ref, f"mad zeros form {form} period {p} chunk {chunk}")

@@ -21,7 +21,7 @@ def test_knob_range_checks():
     ctx = vta.HipContext(0)
     assert at.get_tuning(ctx, at.SCAN_WAVES) == 0
     for knob, bad in ((at.ALMA_MFMA_VARIANT, 12), (at.SCAN_WAVES, 32), (at.FIR_GROUP, 257),
-                      (at.MS_CHUNK, 63), (9, 1)):
+                      (at.MS_CHUNK, 63), (at.ROLL_FORM, 3), (9, 1)):
         with pytest.raises(vta.VtaError):
             at.set_tuning(ctx, knob, bad)
     at.set_tuning(ctx, at.FIR_GROUP, 32)

@@ -24,8 +24,9 @@ ALMA_MFMA_VARIANT = 0
 SCAN_WAVES = 1
 FIR_GROUP = 2
 MS_CHUNK = 3
+ROLL_FORM = 4
 KNOB_NAMES = {ALMA_MFMA_VARIANT: "alma_mfma_variant", SCAN_WAVES: "scan_waves",
-              FIR_GROUP: "fir_group", MS_CHUNK: "ms_chunk"}
+              FIR_GROUP: "fir_group", MS_CHUNK: "ms_chunk", ROLL_FORM: "roll_form"}
 
 # Candidate values per knob (every value is valid for vta_ctx_set_tuning).
 CANDIDATES: Dict[int, Tuple[int, ...]] = {
@@ -33,6 +34,7 @@ CANDIDATES: Dict[int, Tuple[int, ...]] = {
     SCAN_WAVES: (512, 1024, 2048, 4096, 8192),
     FIR_GROUP: (16, 32, 64, 128),
     MS_CHUNK: (512, 1024, 2048, 4096),
+    ROLL_FORM: (1, 2),
 }
 
 ENQUEUE_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p)

@@ -732,3 +732,336 @@ extern "C" __global__ __launch_bounds__(64) void mad_batch_f32(roll_args a) {
         rb_sync();
     }
 }
+
+// ---------------------------------------------------------------- register windows
+//
+// Many-series PNR and MAD with each lane's sorted window held in VGPRs: CAP
+// slots, CAP a compile-time multiple of 16 chosen as the window length rounded
+// up (one kernel per CAP), slots past the window holding +Inf.  Time runs
+// wave-uniform and every slot takes the same branch-free update each bar, so
+// a step is a fixed run of VALU instructions: no LDS round trips, no per-lane
+// shift counts, and no LDS to cap residency.  With the window keyed as in the
+// LDS form (NaN bars as +Inf, counted):
+//   removal of vo (present):  a[p] = w[p] < vo ? w[p] : w[p + 1]
+//     drops the first slot not below vo, the oracle's lower_bound erase;
+//   insertion of vi:          w'[p] = a[p] < vi ? a[p] : (a[p - 1] < vi ? vi : a[p - 1])
+//     puts vi at its lower_bound slot and moves the slots above it up one.
+// Zeros enter the window as +0.0 keys, so keys that compare equal are bitwise
+// equal and the insertion is clamp(vi, a[p - 1], a[p]), one v_med3_f32.  The
+// zero signs live beside the window: the oracle inserts a zero before every
+// zero already in its window and erases the first one, so its zero block,
+// front to back, is a stack of signs (last pushed first), kept here as a bit
+// stack per lane.  The k-th slot is read by a select tree over the bits of k.
+
+constexpr int rw_pow2(int n) { return n <= 1 ? 1 : 2 * rw_pow2((n + 1) >> 1); }
+constexpr int rw_log2(int n) { return n <= 1 ? 0 : 1 + rw_log2(n >> 1); }
+
+template <int N> struct RwVecF { typedef float v __attribute__((ext_vector_type(N))); };
+template <int N> struct RwVecD { typedef double v __attribute__((ext_vector_type(N))); };
+
+// The window: even slots in one vector, odd slots in another.  Every select
+// between neighbouring slots (the removal, the tree's first level) then reads
+// two different vectors; a select between two elements of one vector would be
+// folded by the compiler into an indexed read, lowered as a compare-and-select
+// over every slot.
+template <int CAP>
+struct RegWin {
+    static_assert(CAP % 16 == 0, "register window capacity");
+    typename RwVecF<CAP / 2>::v e, o;
+    __device__ __forceinline__ float get(int p) const {
+        return p >= CAP ? INFINITY : (p & 1) ? o[p >> 1] : e[p >> 1];
+    }
+    __device__ __forceinline__ void set(int p, float v) {
+        if (p & 1) o[p >> 1] = v;
+        else e[p >> 1] = v;
+    }
+};
+
+// Insertion of vi (no removal): the fill of a chunk's first window.
+template <int CAP>
+__device__ __forceinline__ void rw_insert(RegWin<CAP> &w, float vi) {
+    float pa = -INFINITY;
+#pragma unroll
+    for (int p = 0; p < CAP; ++p) {
+        const float ap = w.get(p);
+        w.set(p, __builtin_amdgcn_fmed3f(pa, ap, vi));
+        pa = ap;
+    }
+}
+
+// Removal of vo followed by insertion of vi, in one pass over the slots (slot
+// p's removal reads the old slot p + 1, which the pass has not yet written).
+template <int CAP>
+__device__ __forceinline__ void rw_replace(RegWin<CAP> &w, float vo, float vi) {
+    float pa = -INFINITY;
+#pragma unroll
+    for (int p = 0; p < CAP; ++p) {
+        const float cur = w.get(p);
+        const float ap = cur < vo ? cur : w.get(p + 1);
+        w.set(p, __builtin_amdgcn_fmed3f(pa, ap, vi));
+        pa = ap;
+    }
+}
+
+// Signs of the window's zero keys, last inserted at bit 0 (NW 32-bit words,
+// enough for a window of CAP keys).
+template <int NW>
+struct ZeroSigns {
+    unsigned b[NW];
+    __device__ __forceinline__ void clear() {
+#pragma unroll
+        for (int j = 0; j < NW; ++j) b[j] = 0;
+    }
+    __device__ __forceinline__ void push(unsigned neg) {
+#pragma unroll
+        for (int j = NW - 1; j > 0; --j) b[j] = (b[j] << 1) | (b[j - 1] >> 31);
+        b[0] = (b[0] << 1) | neg;
+    }
+    __device__ __forceinline__ void pop() {
+#pragma unroll
+        for (int j = 0; j + 1 < NW; ++j) b[j] = (b[j] >> 1) | (b[j + 1] << 31);
+        b[NW - 1] >>= 1;
+    }
+    __device__ __forceinline__ bool neg_at(int d) const {   // d-th from the top
+        unsigned wd = b[0];
+#pragma unroll
+        for (int j = 1; j < NW; ++j) wd = (d >> 5) == j ? b[j] : wd;
+        return (wd >> (d & 31)) & 1u;
+    }
+};
+
+// Window key of a bar: NaN as +Inf, either zero as +0.0.
+__device__ __forceinline__ float rw_key(float v) {
+    return isnan(v) ? INFINITY : v == 0.0f ? 0.0f : v;
+}
+
+// Slot k (per lane, k < CAP) by a select tree over the bits of k, bit 0 at
+// the leaves (even/odd neighbours); slots from CAP on read as +Inf.
+template <int CAP, int LO, int N>
+struct RwTree {
+    static __device__ __forceinline__ float pick(const RegWin<CAP> &w, int k) {
+        if (LO >= CAP) return INFINITY;
+        const float a = RwTree<CAP, LO, N / 2>::pick(w, k);
+        const float b = RwTree<CAP, LO + N / 2, N / 2>::pick(w, k);
+        return (k & (N / 2)) ? b : a;
+    }
+};
+template <int CAP, int LO>
+struct RwTree<CAP, LO, 1> {
+    static __device__ __forceinline__ float pick(const RegWin<CAP> &w, int) { return w.get(LO); }
+};
+
+template <int CAP>
+__device__ __forceinline__ float rw_pick(const RegWin<CAP> &w, int k) {
+    return RwTree<CAP, 0, rw_pow2(CAP)>::pick(w, k);
+}
+
+// Bars [t0, t1) in blocks of RW_B whose loads (x[t] and, for lag > 0,
+// x[t - lag]) are issued one block ahead; bars before 0 read as NaN.
+// f(t, vi, vo) runs per bar, wave-uniform.  The per-bar loop stays rolled (a
+// step is hundreds of instructions): the block's values rotate down one
+// register per bar, so the step always reads slot 0.
+template <class LD, class F>
+__device__ __forceinline__ void rw_bars(int t0, int t1, int lag, LD ld, F f) {
+    float ci[RW_B], co[RW_B], ni[RW_B], no[RW_B];
+    auto load = [&](int b0, float *bi, float *bo) {
+#pragma unroll
+        for (int k = 0; k < RW_B; ++k) {
+            const int t = min(b0 + k, t1 - 1);
+            bi[k] = t >= 0 ? ld(t) : VTA_QNAN_F32;
+            bo[k] = (lag > 0 && t - lag >= 0) ? ld(t - lag) : VTA_QNAN_F32;
+        }
+    };
+    if (t0 >= t1) return;
+    load(t0, ci, co);
+    for (int b0 = t0; b0 < t1; b0 += RW_B) {
+        if (b0 + RW_B < t1) load(b0 + RW_B, ni, no);
+        const int nb = min(RW_B, t1 - b0);
+#pragma unroll 1
+        for (int k = 0; k < nb; ++k) {
+            f(b0 + k, ci[0], co[0]);
+#pragma unroll
+            for (int j = 0; j + 1 < RW_B; ++j) {
+                ci[j] = ci[j + 1];
+                co[j] = co[j + 1];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < RW_B; ++k) {
+            ci[k] = ni[k];
+            co[k] = no[k];
+        }
+    }
+}
+
+// One chunk [c0, c1) of a many-series PNR / MAD over register windows.  The
+// chunk's first window (bars c0 - len + 1 .. c0, bars before 0 as NaN) is
+// built by insertions, so every lane runs the same steps whatever its first
+// valid bar; outputs before a lane's first output bar are NaN.  out(w, st)
+// gives a window's output from the keys and the counts in st.  ZS: keep the
+// zero signs (PNR returns a key itself; MAD's value does not depend on a
+// zero's sign).
+struct RwCounts {
+    int nan_cnt;   // NaN bars in the window
+    int neg_cnt;   // keys below zero
+};
+
+template <int CAP, bool ZS, class OUT>
+__device__ __forceinline__ void rw_chunk(const roll_args &a, OUT out) {
+    const int lane = threadIdx.x & 63;
+    const int s = blockIdx.x * 64 + lane;
+    const int N = a.nlanes;
+    const bool active = s < N;
+    const int sc = min(s, N - 1);
+    const int c0 = blockIdx.y * a.chunk;
+    const int c1 = min(c0 + a.chunk, a.T);
+    const int len = a.p1;
+    const int first = (active ? a.fvs[s] : a.T) + len - 1;
+    const float *xs = a.x + sc;
+    float *os = a.out0 + s;
+    auto ld = [&](int t) -> float { return xs[(size_t)t * N]; };
+    auto st = [&](int t, float v) {
+        if (active) os[(size_t)t * N] = v;
+    };
+    if (rl_wave_min(first) >= c1) {                   // no output in this chunk
+        for (int t = c0; t < c1; ++t) st(t, VTA_QNAN_F32);
+        return;
+    }
+    RegWin<CAP> w;
+#pragma unroll
+    for (int p = 0; p < CAP; ++p) w.set(p, INFINITY);
+    ZeroSigns<(CAP + 31) / 32> zs;
+    zs.clear();
+    RwCounts c{0, 0};
+    auto emit = [&](int t) {
+        float o = t >= first ? out(w, c) : VTA_QNAN_F32;
+        if (ZS && __any(o == 0.0f)) {
+            // the zero at slot k is the (k - neg_cnt)-th of the zero block
+            if (o == 0.0f && zs.neg_at(out.slot(c) - c.neg_cnt)) o = -0.0f;
+        }
+        st(t, o);
+    };
+    rw_bars(c0 - len + 1, c0 + 1, 0, ld, [&](int, float vi, float) {
+        c.nan_cnt += isnan(vi) ? 1 : 0;
+        c.neg_cnt += vi < 0.0f ? 1 : 0;
+        if (ZS && __any(vi == 0.0f)) {
+            if (vi == 0.0f) zs.push(signbit(vi) ? 1u : 0u);
+        }
+        rw_insert<CAP>(w, rw_key(vi));
+    });
+    emit(c0);
+    rw_bars(c0 + 1, c1, len, ld, [&](int t, float vi, float vo) {
+        c.nan_cnt += (isnan(vi) ? 1 : 0) - (isnan(vo) ? 1 : 0);
+        c.neg_cnt += (vi < 0.0f ? 1 : 0) - (vo < 0.0f ? 1 : 0);
+        if (ZS && __any(vi == 0.0f || vo == 0.0f)) {   // erase, then insert
+            if (vo == 0.0f) zs.pop();
+            if (vi == 0.0f) zs.push(signbit(vi) ? 1u : 0u);
+        }
+        rw_replace<CAP>(w, rw_key(vo), rw_key(vi));
+        emit(t);
+    });
+}
+
+template <int CAP>
+struct PnrRegOut {
+    int len;
+    double p_frac;
+    __device__ __forceinline__ int slot(const RwCounts &c) const {
+        return pnr_rank(p_frac, len - c.nan_cnt);
+    }
+    __device__ __forceinline__ float operator()(const RegWin<CAP> &w, const RwCounts &c) const {
+        return len == c.nan_cnt ? VTA_QNAN_F32 : rw_pick<CAP>(w, slot(c));
+    }
+};
+
+template <int CAP>
+__device__ __forceinline__ void pnr_reg_body(const roll_args &a) {
+    rw_chunk<CAP, true>(a, PnrRegOut<CAP>{a.p1, a.a1 * 0.01});
+}
+
+// MAD of a NaN-free register window of len keys.  med as the oracle's median
+// (f64 from the middle key or keys); the deviations |w[i] - med| over the
+// sorted slots fall then rise (a bitonic sequence, +Inf slots last, padded
+// with +Inf to a power of two NP), so the k-th smallest is found by
+// half-cleaners: at half size h, min(d[i], d[i + h]) holds the h smallest and
+// max(...) the h largest, each again bitonic; the path keeps the half holding
+// rank k.  Ranks mid - 1 and mid (even len) share the path until their halves
+// part.  The first two levels run together, so each path holds NP / 4 values.
+// A non-finite median gives NaN, as the oracle's sort of NaN deviations does.
+template <int CAP>
+__device__ __forceinline__ float rw_mad(const RegWin<CAP> &w, int len) {
+    constexpr int NP = rw_pow2(CAP);
+    static_assert(NP >= 16, "register MAD window");
+    typedef typename RwVecD<NP / 4>::v dvec;
+    const int mid = len >> 1;
+    const bool odd = len & 1;
+    const float hi = rw_pick<CAP>(w, mid);
+    const double med = odd ? (double)hi : 0.5 * ((double)rw_pick<CAP>(w, mid - 1) + (double)hi);
+    if (!isfinite(med)) return VTA_QNAN_F32;
+    const int k1 = odd ? mid : mid - 1, k2 = mid;
+    constexpr int h1 = NP / 2, h2 = NP / 4;
+    auto dev = [&](int i) -> double { return i < CAP ? fabs((double)w.get(i) - med) : INFINITY; };
+    auto op = [](bool mx, double x, double y) { return mx ? fmax(x, y) : fmin(x, y); };
+    dvec A, B;
+    const bool a1 = k1 & h1, a2 = k1 & h2, b1 = k2 & h1, b2 = k2 & h2;
+    bool split = a1 != b1 || a2 != b2;
+#pragma unroll
+    for (int i = 0; i < h2; ++i) {
+        const double d0 = dev(i), d1 = dev(i + h2), d2 = dev(i + h1), d3 = dev(i + h1 + h2);
+        const double lo0 = fmin(d0, d2), hi0 = fmax(d0, d2), lo1 = fmin(d1, d3), hi1 = fmax(d1, d3);
+        A[i] = op(a2, a1 ? hi0 : lo0, a1 ? hi1 : lo1);
+        B[i] = op(b2, b1 ? hi0 : lo0, b1 ? hi1 : lo1);
+    }
+#pragma unroll
+    for (int lv = 0; lv < rw_log2(NP) - 2; ++lv) {
+        const int h = NP / 8 >> lv;
+        const bool ba = k1 & h, bb = k2 & h;
+        if (!split) {
+            if (ba == bb) {
+#pragma unroll
+                for (int i = 0; i < h; ++i) A[i] = op(ba, A[i], A[i + h]);
+            } else {
+#pragma unroll
+                for (int i = 0; i < h; ++i) {
+                    B[i] = fmax(A[i], A[i + h]);
+                    A[i] = fmin(A[i], A[i + h]);
+                }
+                split = true;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < h; ++i) {
+                A[i] = op(ba, A[i], A[i + h]);
+                B[i] = op(bb, B[i], B[i + h]);
+            }
+        }
+    }
+    const double d1 = A[0], d2 = split ? B[0] : A[0];
+    return (float)(odd ? d1 : 0.5 * (d1 + d2));
+}
+
+// period 1: 0 for every non-NaN bar, +-Inf included (medium_ad.rs:326-333)
+template <int CAP>
+struct MadRegOut {
+    int len;
+    __device__ __forceinline__ int slot(const RwCounts &) const { return 0; }
+    __device__ __forceinline__ float operator()(const RegWin<CAP> &w, const RwCounts &c) const {
+        return c.nan_cnt != 0 ? VTA_QNAN_F32 : len == 1 ? 0.0f : rw_mad<CAP>(w, len);
+    }
+};
+
+template <int CAP>
+__device__ __forceinline__ void mad_reg_body(const roll_args &a) {
+    rw_chunk<CAP, false>(a, MadRegOut<CAP>{a.p1});
+}
+
+#define RW_PNR(CAP)                                                                        \
+    extern "C" __global__ __launch_bounds__(64) void pnr_ms1p_reg##CAP(roll_args a) {      \
+        pnr_reg_body<CAP>(a);                                                              \
+    }
+#define RW_MAD(CAP)                                                                        \
+    extern "C" __global__ __launch_bounds__(64) void mad_ms1p_reg##CAP(roll_args a) {      \
+        mad_reg_body<CAP>(a);                                                              \
+    }
+RW_PNR(16) RW_PNR(32) RW_PNR(48) RW_PNR(64) RW_PNR(80) RW_PNR(96) RW_PNR(112) RW_PNR(128)
+RW_MAD(16) RW_MAD(32) RW_MAD(48) RW_MAD(64)

@@ -4,6 +4,7 @@
 // (src/cuda/medium_ad_wrapper.rs).  Kernels: kernels/rolling.hip.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -78,9 +79,75 @@ const char *kname(RInd k, bool batch) {
     return batch ? b[k] : m[k];
 }
 
+// Register-window capacity (the window length rounded up to 16 slots) of the
+// many-series PNR / MAD kernels, 0 when the LDS form runs: the knob's choice,
+// else the register form wherever a kernel exists (PNR up to 128 slots, MAD
+// up to 64: a longer MAD window's f64 deviation paths no longer fit the
+// VGPRs beside the window).
+int reg_window_cap(const vta_ctx *ctx, RInd k, size_t len) {
+    if (k == R_BB || ctx->tune[VTA_TUNE_ROLL_FORM] == 1) return 0;
+    const size_t cap = (len + 15) / 16 * 16;
+    if (cap == 0 || cap > (k == R_PNR ? 128u : 64u)) return 0;
+    return (int)cap;
+}
+
+// Chunk length of the register-window form.  Its steps are VALU runs, so
+// the launch time is (rounds of resident waves) x (chunk length): the chunk
+// count per series is a whole number of rounds of the CUs' wave slots (waves
+// per SIMD from the kernel's VGPR count), the round count chosen to minimise
+// rounds x (chunk + half a window for the chunk's fill by insertions)
+// (1,024 x 50,000, PNR window 100 at 2 waves per SIMD: one round, 391-bar
+// chunks 0.846 ms; 2.0 rounds of 200 bars 0.90; 1.5 rounds of 260 0.89; 1.2
+// rounds of 330 1.09 ms).
+int reg_window_chunk(const vta_ctx *ctx, hipFunction_t f, size_t T, size_t lanes, size_t len) {
+    if (const char *e = getenv("VTA_ROLL_CHUNK")) {
+        long v = atol(e);
+        if (v >= 64) return (int)std::min<size_t>((size_t)v, T);
+    }
+    int regs = 0;
+    if (hipFuncGetAttribute(&regs, HIP_FUNC_ATTRIBUTE_NUM_REGS, f) != hipSuccess || regs <= 0)
+        regs = 256;
+    const size_t per_simd = std::max<size_t>(1, std::min<size_t>(8, 512 / (size_t)((regs + 7) / 8 * 8)));
+    const size_t slots = per_simd * 4 * (size_t)std::max(1, ctx->props.multiProcessorCount);
+    const size_t groups = std::max<size_t>(1, (lanes + 63) / 64);
+    size_t best = T;
+    double best_cost = 0.0;
+    for (size_t r = 1; r <= 4; ++r) {
+        const size_t per_series = std::max<size_t>(1, r * slots / groups);
+        const size_t c = std::max<size_t>((T + per_series - 1) / per_series, 64);
+        const size_t chunks = (T + c - 1) / c;
+        const size_t rounds = (chunks * groups + slots - 1) / slots;
+        const double cost = (double)rounds * ((double)std::min(c, T) + 0.5 * (double)len);
+        if (r == 1 || cost < best_cost) {
+            best_cost = cost;
+            best = c;
+        }
+    }
+    return (int)std::min(best, T);
+}
+
 vta_status roll_launch(vta_ctx *ctx, RInd k, bool batch, roll_args_host &a, size_t max_w,
                        bool chunked) {
     hipFunction_t f;
+    if (!batch) {
+        if (const int cap = reg_window_cap(ctx, k, max_w)) {
+            char name[32];
+            snprintf(name, sizeof name, "%s_ms1p_reg%d", k == R_PNR ? "pnr" : "mad", cap);
+            VTA_TRY(get_function(ctx, "rolling", name, &f));
+            if (k == R_PNR) {
+                const vta_kernel_policy &pol = ctx->policy[VTA_POLICY_SLOT_PNR];
+                if (pol.many_series == VTA_KERNEL_PLAIN) chunked = false;
+            }
+            a.chunk = chunked ? reg_window_chunk(ctx, f, (size_t)a.T, (size_t)a.nlanes, max_w) : a.T;
+            if (k == R_PNR)
+                ctx->sel_ms[VTA_POLICY_SLOT_PNR] = {
+                    a.chunk < a.T ? VTA_SELECTED_CHUNKED : VTA_SELECTED_PLAIN, 64};
+            a.wcap = (int)max_w;
+            const unsigned groups = (unsigned)((a.nlanes + 63) / 64);
+            const unsigned chunks = (unsigned)((a.T + a.chunk - 1) / a.chunk);
+            return launch(ctx, f, dim3(groups, chunks, 1), dim3(64), 0, a);
+        }
+    }
     VTA_TRY(get_function(ctx, "rolling", kname(k, batch), &f));
     // PNR / MAD: the lanes' sorted windows, plus (batch) the 64 x 33 output staging
     size_t smem = (k == R_BB) ? 0

@@ -418,6 +418,18 @@ vta_status vta_ctx_preload(vta_ctx *ctx) {
         {"cross", "cross_pairs_f64"},
         {"cross", "cross_reduce"},
         {"rolling", "mad_ms1p_f32"},
+        {"rolling", "pnr_ms1p_reg16"},
+        {"rolling", "pnr_ms1p_reg32"},
+        {"rolling", "pnr_ms1p_reg48"},
+        {"rolling", "pnr_ms1p_reg64"},
+        {"rolling", "pnr_ms1p_reg80"},
+        {"rolling", "pnr_ms1p_reg96"},
+        {"rolling", "pnr_ms1p_reg112"},
+        {"rolling", "pnr_ms1p_reg128"},
+        {"rolling", "mad_ms1p_reg16"},
+        {"rolling", "mad_ms1p_reg32"},
+        {"rolling", "mad_ms1p_reg48"},
+        {"rolling", "mad_ms1p_reg64"},
     };
     clear_error();
     if (!ctx) return fail(VTA_ERR_INVALID_INPUT, "ctx is null");

@@ -23,6 +23,7 @@ bool knob_value_ok(vta_tune_knob knob, int64_t v) {
         case VTA_TUNE_SCAN_WAVES: return v >= 64 && v <= 65536;
         case VTA_TUNE_FIR_GROUP: return v >= 1 && v <= 256;
         case VTA_TUNE_MS_CHUNK: return v >= 64 && v <= (int64_t)1 << 30;
+        case VTA_TUNE_ROLL_FORM: return v == 1 || v == 2;
         default: return false;
     }
 }
