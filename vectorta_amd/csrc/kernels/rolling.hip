@@ -777,16 +777,14 @@ struct RegWin {
     }
 };
 
-// Insertion of vi (no removal): the fill of a chunk's first window.
+// Insertion of vi (no removal): the fill of a chunk's first window.  Slots go
+// from the top down, so each med3 reads the slot it overwrites and the one
+// below it, still old (bottom up, every old slot would need a copy).
 template <int CAP>
 __device__ __forceinline__ void rw_insert(RegWin<CAP> &w, float vi) {
-    float pa = -INFINITY;
 #pragma unroll
-    for (int p = 0; p < CAP; ++p) {
-        const float ap = w.get(p);
-        w.set(p, __builtin_amdgcn_fmed3f(pa, ap, vi));
-        pa = ap;
-    }
+    for (int p = CAP - 1; p >= 0; --p)
+        w.set(p, __builtin_amdgcn_fmed3f(p > 0 ? w.get(p - 1) : -INFINITY, w.get(p), vi));
 }
 
 // Removal of vo followed by insertion of vi, in one pass over the slots (slot
@@ -856,17 +854,47 @@ __device__ __forceinline__ float rw_pick(const RegWin<CAP> &w, int k) {
     return RwTree<CAP, 0, rw_pow2(CAP)>::pick(w, k);
 }
 
+// Slot k for a wave-uniform k: a binary search of scalar branches down to one
+// register copy.  The copy is an asm statement, so the compiler can neither
+// speculate the branches into selects nor fold them into an indexed read.
+template <int CAP, int LO, int N>
+struct RwSwitch {
+    static __device__ __forceinline__ float pick(const RegWin<CAP> &w, int k) {
+        if (LO >= CAP) return INFINITY;
+        if (k < LO + N / 2) return RwSwitch<CAP, LO, N / 2>::pick(w, k);
+        return RwSwitch<CAP, LO + N / 2, N / 2>::pick(w, k);
+    }
+};
+template <int CAP, int LO>
+struct RwSwitch<CAP, LO, 1> {
+    static __device__ __forceinline__ float pick(const RegWin<CAP> &w, int) {
+        float v = w.get(LO);
+        asm volatile("" : "+v"(v));
+        return v;
+    }
+};
+
+// Slot k: the scalar search when every lane of the wave wants the same slot
+// (every lane's window holds the same number of NaN bars), else the tree.
+template <int CAP>
+__device__ __forceinline__ float rw_pick_any(const RegWin<CAP> &w, int k) {
+    const int k0 = __builtin_amdgcn_readfirstlane(k);
+    if (__all(k == k0)) return RwSwitch<CAP, 0, rw_pow2(CAP)>::pick(w, k0);
+    return rw_pick<CAP>(w, k);
+}
+
 // Bars [t0, t1) in blocks of RW_B whose loads (x[t] and, for lag > 0,
 // x[t - lag]) are issued one block ahead; bars before 0 read as NaN.
 // f(t, vi, vo) runs per bar, wave-uniform.  The per-bar loop stays rolled (a
 // step is hundreds of instructions): the block's values rotate down one
 // register per bar, so the step always reads slot 0.
+#define RW_RB 4   // a step is ~2,000 cycles: loads four bars ahead are in time
 template <class LD, class F>
 __device__ __forceinline__ void rw_bars(int t0, int t1, int lag, LD ld, F f) {
-    float ci[RW_B], co[RW_B], ni[RW_B], no[RW_B];
+    float ci[RW_RB], co[RW_RB], ni[RW_RB], no[RW_RB];
     auto load = [&](int b0, float *bi, float *bo) {
 #pragma unroll
-        for (int k = 0; k < RW_B; ++k) {
+        for (int k = 0; k < RW_RB; ++k) {
             const int t = min(b0 + k, t1 - 1);
             bi[k] = t >= 0 ? ld(t) : VTA_QNAN_F32;
             bo[k] = (lag > 0 && t - lag >= 0) ? ld(t - lag) : VTA_QNAN_F32;
@@ -874,20 +902,20 @@ __device__ __forceinline__ void rw_bars(int t0, int t1, int lag, LD ld, F f) {
     };
     if (t0 >= t1) return;
     load(t0, ci, co);
-    for (int b0 = t0; b0 < t1; b0 += RW_B) {
-        if (b0 + RW_B < t1) load(b0 + RW_B, ni, no);
-        const int nb = min(RW_B, t1 - b0);
+    for (int b0 = t0; b0 < t1; b0 += RW_RB) {
+        if (b0 + RW_RB < t1) load(b0 + RW_RB, ni, no);
+        const int nb = min(RW_RB, t1 - b0);
 #pragma unroll 1
         for (int k = 0; k < nb; ++k) {
             f(b0 + k, ci[0], co[0]);
 #pragma unroll
-            for (int j = 0; j + 1 < RW_B; ++j) {
+            for (int j = 0; j + 1 < RW_RB; ++j) {
                 ci[j] = ci[j + 1];
                 co[j] = co[j + 1];
             }
         }
 #pragma unroll
-        for (int k = 0; k < RW_B; ++k) {
+        for (int k = 0; k < RW_RB; ++k) {
             ci[k] = ni[k];
             co[k] = no[k];
         }
@@ -970,7 +998,7 @@ struct PnrRegOut {
         return pnr_rank(p_frac, len - c.nan_cnt);
     }
     __device__ __forceinline__ float operator()(const RegWin<CAP> &w, const RwCounts &c) const {
-        return len == c.nan_cnt ? VTA_QNAN_F32 : rw_pick<CAP>(w, slot(c));
+        return len == c.nan_cnt ? VTA_QNAN_F32 : rw_pick_any<CAP>(w, slot(c));
     }
 };
 
@@ -979,61 +1007,99 @@ __device__ __forceinline__ void pnr_reg_body(const roll_args &a) {
     rw_chunk<CAP, true>(a, PnrRegOut<CAP>{a.p1, a.a1 * 0.01});
 }
 
-// MAD of a NaN-free register window of len keys.  med as the oracle's median
-// (f64 from the middle key or keys); the deviations |w[i] - med| over the
-// sorted slots fall then rise (a bitonic sequence, +Inf slots last, padded
-// with +Inf to a power of two NP), so the k-th smallest is found by
-// half-cleaners: at half size h, min(d[i], d[i + h]) holds the h smallest and
-// max(...) the h largest, each again bitonic; the path keeps the half holding
-// rank k.  Ranks mid - 1 and mid (even len) share the path until their halves
-// part.  The first two levels run together, so each path holds NP / 4 values.
-// A non-finite median gives NaN, as the oracle's sort of NaN deviations does.
+// MAD of a NaN-free register window of len keys (len wave-uniform).  med as
+// the oracle's median (f64 from the middle key or keys); the deviations
+// |w[i] - med| over the sorted slots fall then rise (a bitonic sequence, +Inf
+// slots last, padded with +Inf to a power of two NP), so the k-th smallest is
+// found by half-cleaners: at half size h, min(d[i], d[i + h]) holds the h
+// smallest and max(...) the h largest, each again bitonic; the path keeps the
+// half holding rank k.  Ranks mid - 1 and mid (even len) share the path until
+// their halves part.  The ranks follow from len alone, so every path choice
+// is a scalar branch; the first two levels run together (each path holds
+// NP / 4 values).  A non-finite median gives NaN, as the oracle's sort of NaN
+// deviations does.
+template <bool MX>
+__device__ __forceinline__ double rw_op(double x, double y) {
+    return MX ? fmax(x, y) : fmin(x, y);
+}
+
+template <int CAP, bool A1, bool A2, bool B1, bool B2>
+__device__ __forceinline__ void rw_mad_l12(const RegWin<CAP> &w, double med,
+                                           typename RwVecD<rw_pow2(CAP) / 4>::v &A,
+                                           typename RwVecD<rw_pow2(CAP) / 4>::v &B) {
+    constexpr int NP = rw_pow2(CAP), h1 = NP / 2, h2 = NP / 4;
+    constexpr bool SPLIT = A1 != B1 || A2 != B2;
+    // an opaque copy of med keeps the deviations inside this case of the
+    // caller's switch (hoisted above it, all NP of them would be live at once)
+    double m = med;
+    asm volatile("" : "+v"(m));
+    auto dev = [&](int i) -> double { return i < CAP ? fabs((double)w.get(i) - m) : INFINITY; };
+#pragma unroll
+    for (int i = 0; i < h2; ++i) {
+        const double d0 = dev(i), d1 = dev(i + h2), d2 = dev(i + h1), d3 = dev(i + h1 + h2);
+        A[i] = rw_op<A2>(rw_op<A1>(d0, d2), rw_op<A1>(d1, d3));
+        if (SPLIT) B[i] = rw_op<B2>(rw_op<B1>(d0, d2), rw_op<B1>(d1, d3));
+    }
+}
+
+// one half-cleaner level on the first 2h values of v, keeping the max or min half
+template <int H, int N>
+__device__ __forceinline__ void rw_mad_level(typename RwVecD<N>::v &v, bool mx) {
+    if (mx) {
+#pragma unroll
+        for (int i = 0; i < H; ++i) v[i] = fmax(v[i], v[i + H]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < H; ++i) v[i] = fmin(v[i], v[i + H]);
+    }
+}
+
 template <int CAP>
 __device__ __forceinline__ float rw_mad(const RegWin<CAP> &w, int len) {
     constexpr int NP = rw_pow2(CAP);
     static_assert(NP >= 16, "register MAD window");
-    typedef typename RwVecD<NP / 4>::v dvec;
+    constexpr int Q = NP / 4;
+    typedef typename RwVecD<Q>::v dvec;
+    len = __builtin_amdgcn_readfirstlane(len);
     const int mid = len >> 1;
     const bool odd = len & 1;
-    const float hi = rw_pick<CAP>(w, mid);
-    const double med = odd ? (double)hi : 0.5 * ((double)rw_pick<CAP>(w, mid - 1) + (double)hi);
+    const float hi = RwSwitch<CAP, 0, NP>::pick(w, mid);
+    const double med =
+        odd ? (double)hi : 0.5 * ((double)RwSwitch<CAP, 0, NP>::pick(w, mid - 1) + (double)hi);
     if (!isfinite(med)) return VTA_QNAN_F32;
     const int k1 = odd ? mid : mid - 1, k2 = mid;
-    constexpr int h1 = NP / 2, h2 = NP / 4;
-    auto dev = [&](int i) -> double { return i < CAP ? fabs((double)w.get(i) - med) : INFINITY; };
-    auto op = [](bool mx, double x, double y) { return mx ? fmax(x, y) : fmin(x, y); };
     dvec A, B;
-    const bool a1 = k1 & h1, a2 = k1 & h2, b1 = k2 & h1, b2 = k2 & h2;
-    bool split = a1 != b1 || a2 != b2;
-#pragma unroll
-    for (int i = 0; i < h2; ++i) {
-        const double d0 = dev(i), d1 = dev(i + h2), d2 = dev(i + h1), d3 = dev(i + h1 + h2);
-        const double lo0 = fmin(d0, d2), hi0 = fmax(d0, d2), lo1 = fmin(d1, d3), hi1 = fmax(d1, d3);
-        A[i] = op(a2, a1 ? hi0 : lo0, a1 ? hi1 : lo1);
-        B[i] = op(b2, b1 ? hi0 : lo0, b1 ? hi1 : lo1);
+    const int code = ((k1 & (NP / 2)) ? 8 : 0) | ((k1 & Q) ? 4 : 0) | ((k2 & (NP / 2)) ? 2 : 0) |
+                     ((k2 & Q) ? 1 : 0);
+    switch (code) {
+#define RW_L12(C)                                                                          \
+    case C:                                                                                \
+        rw_mad_l12<CAP, (C & 8) != 0, (C & 4) != 0, (C & 2) != 0, (C & 1) != 0>(w, med, A, B); \
+        break;
+        RW_L12(0) RW_L12(1) RW_L12(2) RW_L12(3) RW_L12(4) RW_L12(5) RW_L12(6) RW_L12(7)
+        RW_L12(8) RW_L12(9) RW_L12(10) RW_L12(11) RW_L12(12) RW_L12(13) RW_L12(14) RW_L12(15)
+#undef RW_L12
     }
+    bool split = (code >> 2) != (code & 3);
 #pragma unroll
     for (int lv = 0; lv < rw_log2(NP) - 2; ++lv) {
-        const int h = NP / 8 >> lv;
+        const int h = Q / 2 >> lv;
         const bool ba = k1 & h, bb = k2 & h;
-        if (!split) {
-            if (ba == bb) {
+        if (!split && ba != bb) {            // the paths part: B continues from A
 #pragma unroll
-                for (int i = 0; i < h; ++i) A[i] = op(ba, A[i], A[i + h]);
-            } else {
-#pragma unroll
-                for (int i = 0; i < h; ++i) {
-                    B[i] = fmax(A[i], A[i + h]);
-                    A[i] = fmin(A[i], A[i + h]);
-                }
-                split = true;
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < h; ++i) {
-                A[i] = op(ba, A[i], A[i + h]);
-                B[i] = op(bb, B[i], B[i + h]);
-            }
+            for (int i = 0; i < 2 * h; ++i) B[i] = A[i];
+            split = true;
+        }
+        switch (h) {
+#define RW_LV(HH)                                                                          \
+    case HH:                                                                               \
+        if (HH < Q) {                                                                      \
+            rw_mad_level<(HH < Q ? HH : 1), Q>(A, ba);                                     \
+            if (split) rw_mad_level<(HH < Q ? HH : 1), Q>(B, bb);                          \
+        }                                                                                  \
+        break;
+            RW_LV(1) RW_LV(2) RW_LV(4) RW_LV(8) RW_LV(16)
+#undef RW_LV
         }
     }
     const double d1 = A[0], d2 = split ? B[0] : A[0];
@@ -1055,12 +1121,15 @@ __device__ __forceinline__ void mad_reg_body(const roll_args &a) {
     rw_chunk<CAP, false>(a, MadRegOut<CAP>{a.p1});
 }
 
-#define RW_PNR(CAP)                                                                        \
-    extern "C" __global__ __launch_bounds__(64) void pnr_ms1p_reg##CAP(roll_args a) {      \
+// Waves per SIMD hinted per kernel (2 at least, 3 for PNR windows up to 112
+// slots), the most that fit the VGPRs without spilling.
+#define RW_PNR(CAP)                                                                    \
+    extern "C" __global__ __launch_bounds__(64, CAP <= 112 ? 3 : 2) void pnr_ms1p_reg##CAP( \
+        roll_args a) {                                                                     \
         pnr_reg_body<CAP>(a);                                                              \
     }
 #define RW_MAD(CAP)                                                                        \
-    extern "C" __global__ __launch_bounds__(64) void mad_ms1p_reg##CAP(roll_args a) {      \
+    extern "C" __global__ __launch_bounds__(64, 2) void mad_ms1p_reg##CAP(roll_args a) {   \
         mad_reg_body<CAP>(a);                                                              \
     }
 RW_PNR(16) RW_PNR(32) RW_PNR(48) RW_PNR(64) RW_PNR(80) RW_PNR(96) RW_PNR(112) RW_PNR(128)
