@@ -129,6 +129,28 @@ def test_bb_many_series_jumps_shape(ops):
                                         for i in range(3)], "many-series")
 
 
+@pytest.mark.parametrize("chunk", [None, "64"])
+def test_bb_many_series_periods_ring(ops, monkeypatch, chunk):
+    """Many-series BB over periods either side of the 8-bar load block (the
+    outgoing bars of a block then come from the block itself) and of the
+    256-bar LDS ring (longer windows read the outgoing bars from HBM), on a
+    ragged panel with NaN runs and +-Inf bars (poison rules), chunked and as
+    one chunk per series: every column against the oracle."""
+    if chunk:
+        monkeypatch.setenv("VTA_ROLL_CHUNK", chunk)
+    rows, cols = 3_000, 70
+    x = _ragged_panel(rows, cols, seed=94, short=400)
+    xd = x.astype(np.float64)
+    for p in (2, 3, 7, 8, 9, 20, 255, 256, 257):
+        refs = [O.bollinger_bands(xd[:, s], p, 2.0, 1.5) for s in range(cols)]
+        ok = np.array([not np.isnan(r[0]).all() for r in refs])
+        ref3 = [np.stack([r[i] for r in refs], axis=1) for i in range(3)]
+        out = [d.to_host() for d in
+               ops["bb"].bollinger_bands_many_series_one_param_time_major_dev(x, cols, rows, p,
+                                                                               2.0, 1.5)]
+        _bb_check([o[:, ok] for o in out], [r[:, ok] for r in ref3], f"ring period {p}")
+
+
 def test_bb_many_series_unclean_and_device_form(ops):
     rows, cols = 30_000, 130
     x = datagen.many_series_with_jumps(cols, rows, seed=53)
@@ -296,7 +318,7 @@ def test_rolling_plan_enqueue_only(ops):
 
 # ---------------------------------------------------------------- window forms
 
-def _ragged_panel(rows, cols, seed, quant=None):
+def _ragged_panel(rows, cols, seed, quant=None, short=200):
     """Time-major panel: GBM columns with ragged NaN leads, interior NaN runs,
     a few +-Inf bars and (quant) values rounded to a grid so windows hold ties."""
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -305,7 +327,7 @@ def _ragged_panel(rows, cols, seed, quant=None):
     if quant:
         x = (np.round(x / quant) * quant).astype(np.float32)
     lead = rng.integers(0, rows // 4, size=cols)
-    lead[:3] = (0, 1, rows - 200)               # one column with few valid bars
+    lead[:3] = (0, 1, rows - short)             # one column with few valid bars
     for s in range(cols):
         x[:lead[s], s] = np.nan
     for s in rng.integers(0, cols, size=cols // 4):

@@ -46,7 +46,8 @@ struct roll_args {
     int T;
     int nlanes;
     int chunk;
-    int wcap;                // LDS window capacity per lane (PNR, MAD)
+    int wcap;                // LDS window capacity per lane (PNR, MAD); many-series BB:
+                             // ring length (0: outgoing bars read from HBM)
     int p1;                  // many-series: period / length
     int pad_;
 };
@@ -131,9 +132,15 @@ __device__ __forceinline__ void bb_anchor(BbEng &e, int t, LD ld) {
 // and z + p - 1 the sums' class (+-Inf or NaN) depends only on the non-finite
 // bars in the window, which an anchor at max(warm, z) sees too: a chunk
 // starting after z replays at most p bars from there.
-template <class LD, class ST>
+//
+// RING (many-series, windows up to 256): the lane's last p bars also sit in an
+// LDS ring (slot u mod p of the lane's column `ring`, stride 64): bar u's
+// outgoing bar u - p is read from the slot bar u then overwrites.  Read from
+// HBM instead, the outgoing row was fetched again (the output stream had
+// pushed it out of L2 since the same wave read it p bars before).
+template <bool RING, class LD, class ST>
 __device__ __forceinline__ void bb_run(BbEng &e, int c0, int c1, int z, bool active, LD ld,
-                                       ST st) {
+                                       ST st, float *ring) {
     if (!active) return;
     const int warm = e.fv + e.p - 1;
     const long zp = (long)z + e.p;
@@ -144,7 +151,19 @@ __device__ __forceinline__ void bb_run(BbEng &e, int c0, int c1, int z, bool act
     const int ce = (int)min((long)c1, zp);
     if (cs < ce) {
         int t = cs < z ? cs : max(warm, z);
-        bb_anchor(e, t, ld);
+        int slot = 0;                                 // ring slot of bar t + 1
+        if (RING) {
+            // the anchor's bars t - p + 1 .. t into their slots
+            auto ldr = [&](int u) -> double {
+                const double v = ld(u);
+                ring[64 * (u % e.p)] = (float)v;
+                return v;
+            };
+            bb_anchor(e, t, ldr);
+            slot = (t + 1) % e.p;
+        } else {
+            bb_anchor(e, t, ld);
+        }
         auto emit = [&](int tt) {
             const double m = e.sum * e.inv_n;
             const double var = (e.sum_sq * e.inv_n) - (m * m);
@@ -157,13 +176,32 @@ __device__ __forceinline__ void bb_run(BbEng &e, int c0, int c1, int z, bool act
         // sequential updates (indices clamped into [t+1, ce-1])
         while (t + 1 < ce) {
             double nw[8], od[8];
+            const int nk = min(8, ce - 1 - t);
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 const int u = min(t + 1 + k, ce - 1);
                 nw[k] = ld(u);
-                od[k] = ld(u - e.p);
+                if (!RING) od[k] = ld(u - e.p);
             }
-            const int nk = min(8, ce - 1 - t);
+            if (RING) {
+                // outgoing bars: ring slots (or, p < 8, this block's own new
+                // bars), all read before the block's bars go in
+                int sl = slot;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    od[k] = k < e.p ? (double)ring[64 * sl] : nw[k - e.p < 0 ? 0 : k - e.p];
+                    sl = sl + 1 == e.p ? 0 : sl + 1;
+                }
+                sl = slot;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    if (k < nk) {
+                        ring[64 * sl] = (float)nw[k];
+                        sl = sl + 1 == e.p ? 0 : sl + 1;
+                    }
+                }
+                slot = sl;
+            }
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 if (k < nk) {
@@ -178,8 +216,10 @@ __device__ __forceinline__ void bb_run(BbEng &e, int c0, int c1, int z, bool act
     for (int u = (int)max((long)c0, min(zp, (long)c1)); u < c1; ++u) st(u, nan3);
 }
 
-// many series: grid.x = 64-series groups, grid.y = chunks
+// many series: grid.x = 64-series groups, grid.y = chunks; dynamic LDS
+// 64 * period floats when the period fits the ring (a.wcap > 0), else none
 extern "C" __global__ __launch_bounds__(64) void bb_ms1p_f32(roll_args a) {
+    extern __shared__ float bb_ring[];
     const int lane = threadIdx.x & 63;
     const int s = blockIdx.x * 64 + lane;
     const bool active = s < a.nlanes;
@@ -197,7 +237,8 @@ extern "C" __global__ __launch_bounds__(64) void bb_ms1p_f32(roll_args a) {
         a.out1[off] = o[1];
         a.out2[off] = o[2];
     };
-    bb_run(e, c0, c1, z, active, ld, st);
+    if (a.wcap > 0) bb_run<true>(e, c0, c1, z, active, ld, st, bb_ring + lane);
+    else bb_run<false>(e, c0, c1, z, active, ld, st, nullptr);
 }
 
 // one series x many (period, devup, devdn): grid.x = combos / 64, grid.y = chunks.

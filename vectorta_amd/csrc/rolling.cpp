@@ -153,6 +153,10 @@ vta_status roll_launch(vta_ctx *ctx, RInd k, bool batch, roll_args_host &a, size
     size_t smem = (k == R_BB) ? 0
                               : (size_t)64 * (max_w + 1) * sizeof(float) +
                                     (batch ? (size_t)64 * 33 * sizeof(float) : 0);
+    // many-series BB: the lanes' rings of their last `period` bars (windows up
+    // to 256, 64 KiB); wcap 0 reads the outgoing bars from HBM instead
+    size_t ring = 0;
+    if (k == R_BB && !batch && max_w <= 256) ring = (size_t)64 * max_w * sizeof(float);
     if (k == R_PNR) {
         // CudaPnrPolicy: the plain form runs each lane over the whole series
         const vta_kernel_policy &pol = ctx->policy[VTA_POLICY_SLOT_PNR];
@@ -164,12 +168,12 @@ vta_status roll_launch(vta_ctx *ctx, RInd k, bool batch, roll_args_host &a, size
                                          64};
         (batch ? ctx->sel_batch : ctx->sel_ms)[VTA_POLICY_SLOT_PNR] = sel;
     }
-    a.wcap = (int)max_w;
+    a.wcap = k == R_BB ? (ring ? (int)max_w : 0) : (int)max_w;
     unsigned groups = (unsigned)((a.nlanes + 63) / 64);
     unsigned chunks = (unsigned)((a.T + a.chunk - 1) / a.chunk);
     if (smem > ctx->props.sharedMemPerBlock)
         return fail(VTA_ERR_INVALID_INPUT, "window length %zu exceeds the LDS capacity", max_w);
-    return launch(ctx, f, dim3(groups, chunks, 1), dim3(64), (unsigned)smem, a);
+    return launch(ctx, f, dim3(groups, chunks, 1), dim3(64), (unsigned)(smem + ring), a);
 }
 
 // First non-finite element at or after fv (n if none), stride `ld` apart.
