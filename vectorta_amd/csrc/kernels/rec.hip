@@ -127,6 +127,7 @@ struct EmaEng {
     static constexpr int kScan = 1;                  // states carried by the chunked scan
     static constexpr bool kTwoStage = false;
     static constexpr bool kAdvance = false;
+    static constexpr int kPre = 0;
     // steps t < seed_end() after the start are the warm-up accumulation only
     __device__ int seed_end() const { return fv + p; }
     __device__ double seed_step(int t, double x) {
@@ -240,6 +241,9 @@ struct RsiEng {
     static constexpr int kScan = 2;
     static constexpr bool kTwoStage = false;
     static constexpr bool kAdvance = true;
+    // the gain and loss of a bar are two ops: reading them from LDS
+    // (see drive_batch) measured slower (0.55 vs 0.47 ms, 1,250 x 200k)
+    static constexpr int kPre = 0;
     __device__ int seed_end() const { return fv + p; }
     __device__ double seed_step(int t, double x) {
         double d = x - px;
@@ -325,6 +329,7 @@ struct AtrEng {
     static constexpr int kScan = 1;
     static constexpr bool kTwoStage = false;
     static constexpr bool kAdvance = false;
+    static constexpr int kPre = 1;
     __device__ int seed_end() const { return fv + p - 1; }
     __device__ double seed_step(int t, double h, double l, double c) {
         sum += true_range(h, l, pc);
@@ -335,6 +340,16 @@ struct AtrEng {
         double tr = true_range(h, l, pc);
         pc = c;
         rma = fma(-alpha, rma, rma) + alpha * tr;
+        return rma;
+    }
+    // per-bar term shared by every combo: the true range of bar t
+    static __device__ void pre(double h, double l, double, double, double, double pc, double *o) {
+        o[0] = true_range(h, l, pc);
+    }
+    __device__ void set_prev(double, double, double c) { pc = c; }
+    __device__ void advance_pre(const double *) {}
+    __device__ double steady_pre(const double *o) {
+        rma = fma(-alpha, rma, rma) + alpha * o[0];
         return rma;
     }
     // seed sum up to fv+p-1, where the mean is taken (1/p), then 1 - alpha
@@ -383,6 +398,7 @@ struct AtrTrEng {
     static constexpr int kScan = 1;
     static constexpr bool kTwoStage = false;
     static constexpr bool kAdvance = false;
+    static constexpr int kPre = 0;
     __device__ int seed_end() const { return fv + p - 1; }
     __device__ double seed_step(int t, double tr) {
         sum += tr;
@@ -474,6 +490,7 @@ struct AdxEng {
     static constexpr int kScan = 4;
     static constexpr bool kTwoStage = true;
     static constexpr bool kAdvance = true;
+    static constexpr int kPre = 3;
     __device__ int seed_end() const { return fv + p; }
     __device__ double seed_step(int t, double ch, double cl, double cc) {
         double hl = ch - cl;
@@ -579,6 +596,33 @@ struct AdxEng {
         adx = (adx * pm1 + dx) * rp;
         return adx;
     }
+    // per-bar terms shared by every combo: the true range, +DM and -DM of bar t
+    static __device__ void pre(double ch, double cl, double, double ph_, double pl_, double pc_,
+                               double *o) {
+        double hl = ch - cl;
+        double hpc = fabs(ch - pc_);
+        double lpc = fabs(cl - pc_);
+        o[0] = fmax(fmax(hl, hpc), lpc);
+        double up = ch - ph_;
+        double down = pl_ - cl;
+        o[1] = (up > down && up > 0.0) ? up : 0.0;
+        o[2] = (down > up && down > 0.0) ? down : 0.0;
+    }
+    __device__ void set_prev(double h, double l, double c) { ph = h; pl = l; pc = c; }
+    __device__ void advance_pre(const double *o) {
+        atr = atr * omr + o[0];
+        pds = pds * omr + o[1];
+        mds = mds * omr + o[2];
+    }
+    __device__ double steady_pre(const double *o) {
+        atr = atr * omr + o[0];
+        pds = pds * omr + o[1];
+        mds = mds * omr + o[2];
+        const double sdm = pds + mds;
+        double dx = (atr != 0.0 && sdm != 0.0) ? dx_ratio(100.0 * fabs(pds - mds), sdm) : 0.0;
+        adx = (adx * pm1 + dx) * rp;
+        return adx;
+    }
 };
 
 // SMMA (smma.rs:243-271): the mean of the first p bars at fv+p-1, then
@@ -614,6 +658,7 @@ struct SmmaEng {
     static constexpr int kScan = 1;
     static constexpr bool kTwoStage = false;
     static constexpr bool kAdvance = false;
+    static constexpr int kPre = 0;
     __device__ int seed_end() const { return p == 1 ? fv : fv + p - 1; }
     __device__ double seed_step(int t, double x) {
         sum += x;
@@ -676,6 +721,7 @@ struct WildersEng {
     static constexpr int kScan = 1;
     static constexpr bool kTwoStage = false;
     static constexpr bool kAdvance = false;
+    static constexpr int kPre = 0;
     __device__ int seed_end() const { return fv + p - 1; }
     __device__ double seed_step(int t, double x) { return seed_bar(t - fv, x); }
     __device__ double steady(double x) {
@@ -1032,6 +1078,11 @@ __device__ __forceinline__ batch_view view_of(const rec_args &a) {
 template <class E, int NIN>
 __device__ __forceinline__ void drive_batch(const batch_view &a, float *out) {
     __shared__ __align__(16) float xs[3][64];
+    // per-bar terms every combo shares (E::kPre of them, f64): lane j computes
+    // them for bar tb + j, so the steady and pass-1 loops read them instead of
+    // recomputing them in every lane
+    constexpr int NPRE = E::kPre;
+    __shared__ __align__(16) double pv[NPRE > 0 ? NPRE : 1][64];
     // 64 x 65 output staging in dynamic LDS, launched only with the passes
     // that write rows (a pass without stores keeps more waves per CU)
     extern __shared__ float ob_dyn[];
@@ -1095,33 +1146,41 @@ __device__ __forceinline__ void drive_batch(const batch_view &a, float *out) {
     }
     const int t_begin = wave_min(carried ? c0 : min(start, c0)) & ~63;
     // bars of the next block are loaded into registers while the current one
-    // is processed
-    float nx0, nx1 = 0.0f, nx2 = 0.0f;
-    {
-        const long t = min(t_begin + lane, T - 1);
+    // is processed (with the bar before each, for the per-bar terms)
+    float nx0, nx1 = 0.0f, nx2 = 0.0f, np0 = 0.0f, np1 = 0.0f, np2 = 0.0f;
+    auto load_next = [&](int tb0) {
+        const long t = min(tb0 + lane, T - 1);
         nx0 = a.x[t];
         if (NIN == 3) {
             nx1 = a.h[t];
             nx2 = a.l[t];
         }
-    }
+        if constexpr (NPRE > 0) {
+            const long tp = max(t - 1, 0L);
+            np0 = a.x[tp];
+            if (NIN == 3) {
+                np1 = a.h[tp];
+                np2 = a.l[tp];
+            }
+        }
+    };
+    load_next(t_begin);
     for (int tb = t_begin; tb < c1; tb += 64) {
         xs[0][lane] = nx0;
         if (NIN == 3) {
             xs[1][lane] = nx1;
             xs[2][lane] = nx2;
         }
+        if constexpr (NPRE > 0) {
+            double o[NPRE];
+            E::pre((double)nx1, (double)nx2, (double)nx0, (double)np1, (double)np2, (double)np0, o);
+#pragma unroll
+            for (int i = 0; i < NPRE; ++i) pv[i][lane] = o[i];
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        {
-            const long t = min(tb + 64 + lane, T - 1);
-            nx0 = a.x[t];
-            if (NIN == 3) {
-                nx1 = a.h[t];
-                nx2 = a.l[t];
-            }
-        }
+        load_next(tb + 64);
         // lean path: a full block in which every lane is past its start and
         // its warm-up -- only the engine's steady update, bars read four at a
         // time
@@ -1131,6 +1190,16 @@ __device__ __forceinline__ void drive_batch(const batch_view &a, float *out) {
             if (a.phase == 1 && tb + 64 <= c1 &&
                 __all(!active || (start < tb && tb >= e.advance_from()))) {
                 // pass 1: the state update only, no outputs
+                if constexpr (NPRE > 0) {
+#pragma unroll 4
+                    for (int j = 0; j < 64; ++j) {
+                        double o[NPRE];
+#pragma unroll
+                        for (int i = 0; i < NPRE; ++i) o[i] = pv[i][j];
+                        e.advance_pre(o);
+                    }
+                    e.set_prev(xs[1][63], xs[2][63], xs[0][63]);
+                } else
 #pragma unroll 2
                 for (int j = 0; j < 64; j += 4) {
                     const float4 x4 = *reinterpret_cast<const float4 *>(&xs[0][j]);
@@ -1179,6 +1248,17 @@ __device__ __forceinline__ void drive_batch(const batch_view &a, float *out) {
         if (advanced) {
             // block done
         } else if (lean) {
+          if constexpr (NPRE > 0) {
+#pragma unroll 4
+            for (int j = 0; j < 64; ++j) {
+                double o[NPRE];
+#pragma unroll
+                for (int i = 0; i < NPRE; ++i) o[i] = pv[i][j];
+                const double y = e.steady_pre(o);
+                if (emit) ob[lane][j] = (float)y;
+            }
+            e.set_prev(xs[1][63], xs[2][63], xs[0][63]);
+          } else {
 #pragma unroll 2
             for (int j = 0; j < 64; j += 4) {
                 const float4 x4 = *reinterpret_cast<const float4 *>(&xs[0][j]);
@@ -1203,6 +1283,7 @@ __device__ __forceinline__ void drive_batch(const batch_view &a, float *out) {
                     ob[lane][j + 3] = (float)y3;
                 }
             }
+          }
         } else if (tb + 64 <= c1 && __all(!active || (start < tb && tb + 64 <= e.seed_end()))) {
             // every lane still accumulating its warm-up seed for the whole block
 #pragma unroll 2
