@@ -128,6 +128,7 @@ struct EmaEng {
     static constexpr bool kTwoStage = false;
     static constexpr bool kAdvance = false;
     static constexpr int kPre = 0;
+    static constexpr int kStageW = 64;
     // steps t < seed_end() after the start are the warm-up accumulation only
     __device__ int seed_end() const { return fv + p; }
     __device__ double seed_step(int t, double x) {
@@ -244,6 +245,7 @@ struct RsiEng {
     // the gain and loss of a bar are two ops: reading them from LDS
     // (see drive_batch) measured slower (0.55 vs 0.47 ms, 1,250 x 200k)
     static constexpr int kPre = 0;
+    static constexpr int kStageW = 32;
     __device__ int seed_end() const { return fv + p; }
     __device__ double seed_step(int t, double x) {
         double d = x - px;
@@ -330,6 +332,7 @@ struct AtrEng {
     static constexpr bool kTwoStage = false;
     static constexpr bool kAdvance = false;
     static constexpr int kPre = 1;
+    static constexpr int kStageW = 64;
     __device__ int seed_end() const { return fv + p - 1; }
     __device__ double seed_step(int t, double h, double l, double c) {
         sum += true_range(h, l, pc);
@@ -399,6 +402,7 @@ struct AtrTrEng {
     static constexpr bool kTwoStage = false;
     static constexpr bool kAdvance = false;
     static constexpr int kPre = 0;
+    static constexpr int kStageW = 64;
     __device__ int seed_end() const { return fv + p - 1; }
     __device__ double seed_step(int t, double tr) {
         sum += tr;
@@ -491,6 +495,7 @@ struct AdxEng {
     static constexpr bool kTwoStage = true;
     static constexpr bool kAdvance = true;
     static constexpr int kPre = 3;
+    static constexpr int kStageW = 32;
     __device__ int seed_end() const { return fv + p; }
     __device__ double seed_step(int t, double ch, double cl, double cc) {
         double hl = ch - cl;
@@ -659,6 +664,7 @@ struct SmmaEng {
     static constexpr bool kTwoStage = false;
     static constexpr bool kAdvance = false;
     static constexpr int kPre = 0;
+    static constexpr int kStageW = 64;
     __device__ int seed_end() const { return p == 1 ? fv : fv + p - 1; }
     __device__ double seed_step(int t, double x) {
         sum += x;
@@ -722,6 +728,7 @@ struct WildersEng {
     static constexpr bool kTwoStage = false;
     static constexpr bool kAdvance = false;
     static constexpr int kPre = 0;
+    static constexpr int kStageW = 64;
     __device__ int seed_end() const { return fv + p - 1; }
     __device__ double seed_step(int t, double x) { return seed_bar(t - fv, x); }
     __device__ double steady(double x) {
@@ -1085,8 +1092,14 @@ __device__ __forceinline__ void drive_batch(const batch_view &a, float *out) {
     __shared__ __align__(16) double pv[NPRE > 0 ? NPRE : 1][64];
     // 64 x 65 output staging in dynamic LDS, launched only with the passes
     // that write rows (a pass without stores keeps more waves per CU)
+    // 64 rows x SW bars (+1 pad), flushed every SW bars: engines whose steps
+    // are long (RSI, ADX) flush every 32 bars, which leaves LDS for ~3 waves
+    // per SIMD where 64-bar staging allows 2; the others keep 64-bar row
+    // segments
+    constexpr int SW = E::kStageW;    // the host launches 64 x (SW + 1) floats
+                                      // of dynamic LDS (batch_stage_bytes, rec.cpp)
     extern __shared__ float ob_dyn[];
-    float (*ob)[65] = reinterpret_cast<float (*)[65]>(ob_dyn);
+    float (*ob)[SW + 1] = reinterpret_cast<float (*)[SW + 1]>(ob_dyn);
     const int lane = threadIdx.x & 63;
     const int c = a.group * 64 + lane;
     const bool active = c < a.nlanes;
@@ -1137,6 +1150,26 @@ __device__ __forceinline__ void drive_batch(const batch_view &a, float *out) {
             if (i < n) st0[i] = cs[i];
     }
     const bool emit = a.phase == 0 || a.phase == 3;
+    // 32-bar staging: rows of bars tb + 32 hh .. + 31, each store two rows x
+    // 32 consecutive bars (lane & 31 = bar, lane >> 5 = row parity)
+    const int nrow = min(64, a.nlanes - a.group * 64);
+    auto flush_half = [&](int tb, int hh) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int t = tb + 32 * hh + (lane & 31);
+        if (tb + 32 * hh + 32 > c0) {
+            const bool ok = t >= c0 && t < c1;
+#pragma unroll 4
+            for (int i = 0; i < 32; ++i) {
+                const int r = 2 * i + (lane >> 5);
+                if (ok && r < nrow) out[(size_t)(a.group * 64 + r) * T + t] = ob[r][lane & (SW - 1)];
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
     // a carried lane restarts after bar c0 - 1 here, from that one bar read
     // directly, so the wave need not run the 64-bar block holding it
     if (carried) {
@@ -1235,10 +1268,11 @@ __device__ __forceinline__ void drive_batch(const batch_view &a, float *out) {
                     e.dx_seed_step(h4.z, l4.z, x4.z);
                     e.dx_seed_step(h4.w, l4.w, x4.w);
                     if (emit) {
-                        ob[lane][j] = (float)REC_NAN;
-                        ob[lane][j + 1] = (float)REC_NAN;
-                        ob[lane][j + 2] = (float)REC_NAN;
-                        ob[lane][j + 3] = (float)REC_NAN;
+                        ob[lane][j & (SW - 1)] = (float)REC_NAN;
+                        ob[lane][(j + 1) & (SW - 1)] = (float)REC_NAN;
+                        ob[lane][(j + 2) & (SW - 1)] = (float)REC_NAN;
+                        ob[lane][(j + 3) & (SW - 1)] = (float)REC_NAN;
+                        if (SW == 32 && ((j + 4) & 31) == 0) flush_half(tb, j >> 5);
                     }
                 }
                 e.dx_count += 64;
@@ -1255,7 +1289,10 @@ __device__ __forceinline__ void drive_batch(const batch_view &a, float *out) {
 #pragma unroll
                 for (int i = 0; i < NPRE; ++i) o[i] = pv[i][j];
                 const double y = e.steady_pre(o);
-                if (emit) ob[lane][j] = (float)y;
+                if (emit) {
+                    ob[lane][j & (SW - 1)] = (float)y;
+                    if (SW == 32 && (j & 31) == 31) flush_half(tb, j >> 5);
+                }
             }
             e.set_prev(xs[1][63], xs[2][63], xs[0][63]);
           } else {
@@ -1277,10 +1314,11 @@ __device__ __forceinline__ void drive_batch(const batch_view &a, float *out) {
                     y3 = e.steady(h4.w, l4.w, x4.w);
                 }
                 if (emit) {
-                    ob[lane][j] = (float)y0;
-                    ob[lane][j + 1] = (float)y1;
-                    ob[lane][j + 2] = (float)y2;
-                    ob[lane][j + 3] = (float)y3;
+                    ob[lane][j & (SW - 1)] = (float)y0;
+                    ob[lane][(j + 1) & (SW - 1)] = (float)y1;
+                    ob[lane][(j + 2) & (SW - 1)] = (float)y2;
+                    ob[lane][(j + 3) & (SW - 1)] = (float)y3;
+                    if (SW == 32 && ((j + 4) & 31) == 0) flush_half(tb, j >> 5);
                 }
             }
           }
@@ -1304,10 +1342,11 @@ __device__ __forceinline__ void drive_batch(const batch_view &a, float *out) {
                     y3 = e.seed_step(tb + j + 3, h4.w, l4.w, x4.w);
                 }
                 if (emit) {
-                    ob[lane][j] = (float)y0;
-                    ob[lane][j + 1] = (float)y1;
-                    ob[lane][j + 2] = (float)y2;
-                    ob[lane][j + 3] = (float)y3;
+                    ob[lane][j & (SW - 1)] = (float)y0;
+                    ob[lane][(j + 1) & (SW - 1)] = (float)y1;
+                    ob[lane][(j + 2) & (SW - 1)] = (float)y2;
+                    ob[lane][(j + 3) & (SW - 1)] = (float)y3;
+                    if (SW == 32 && ((j + 4) & 31) == 0) flush_half(tb, j >> 5);
                 }
             }
         } else
@@ -1332,19 +1371,22 @@ __device__ __forceinline__ void drive_batch(const batch_view &a, float *out) {
                     else y = e.step(t, xs[1][j], xs[2][j], x);
                 }
             }
-            if (emit) ob[lane][j] = (float)y;
+            if (emit) {
+                ob[lane][j & (SW - 1)] = (float)y;
+                if (SW == 32 && (j & 31) == 31) flush_half(tb, j >> 5);
+            }
         }
-        if (emit) {
+        // 64-bar staging: one flush per block, after whichever path ran; each
+        // store writes one row x 64 bars (lane = bar)
+        if (SW == 64 && emit) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            // write rows: lane = bar offset
             if (tb + 64 > c0) {
                 const int t = tb + lane;
                 const bool ok = t >= c0 && t < c1;
-                const int nrow = min(64, a.nlanes - a.group * 64);
                 for (int r = 0; r < nrow; ++r)
-                    if (ok) out[(size_t)(a.group * 64 + r) * T + t] = ob[r][lane];
+                    if (ok) out[(size_t)(a.group * 64 + r) * T + t] = ob[r][lane & (SW - 1)];
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

@@ -158,9 +158,14 @@ bool host_clean(const float *x, const float *h, const float *l, size_t n, size_t
     return true;
 }
 
-// Output staging of the one-series batch kernels (64 x 65 floats of dynamic
-// LDS), needed by the passes that write rows (phase 0 and 3).
-constexpr unsigned kBatchStageBytes = 64 * 65 * sizeof(float);
+// Output staging of the one-series batch kernels, needed by the passes that
+// write rows (phase 0 and 3): 64 rows x (W + 1) floats of dynamic LDS, W the
+// bars staged between flushes (32 for RSI and ADX, whose longer steps gain
+// from the extra resident waves; 64 for the others; kStageW in kernels/rec.hip).
+unsigned batch_stage_bytes(Ind k) {
+    const unsigned w = (k == IND_RSI || k == IND_ADX) ? 32u : 64u;
+    return 64u * (w + 1u) * (unsigned)sizeof(float);
+}
 
 vta_status launch_rec(vta_ctx *ctx, const char *kname, rec_args_host &a, unsigned groups,
                       unsigned chunks, unsigned block = 64, unsigned smem = 0) {
@@ -243,8 +248,8 @@ vta_status scan_launch(vta_ctx *ctx, Ind k, rec_args_host a, size_t T, size_t C,
         VTA_TRY(launch(ctx, fc, dim3((unsigned)C), dim3(64), 0, a));
     }
     a.phase = 3;
-    VTA_TRY(launch(ctx, fb, dim3((unsigned)groups, (unsigned)nch, 1), dim3(64), kBatchStageBytes,
-                   a));
+    VTA_TRY(launch(ctx, fb, dim3((unsigned)groups, (unsigned)nch, 1), dim3(64),
+                   batch_stage_bytes(k), a));
     *done = true;
     return VTA_OK;
 }
@@ -307,7 +312,7 @@ vta_status batch_launch(vta_ctx *ctx, Ind k, const float *d_x, const float *d_h,
                                                 : pick_chunk_batch(k, max_p, T, groups, clean);
     unsigned chunks = (unsigned)((T + a.chunk - 1) / a.chunk);
     record_batch(ctx, k, chunks > 1);
-    return launch_rec(ctx, batch_kernel(k), a, groups, chunks, 64, kBatchStageBytes);
+    return launch_rec(ctx, batch_kernel(k), a, groups, chunks, 64, batch_stage_bytes(k));
 }
 
 vta_status batch_dev(vta_ctx *ctx, Ind k, const float *x, const float *h, const float *l,
@@ -1087,7 +1092,7 @@ vta_status pack_launch(vta_ctx *ctx, const PackDims &d, const void *d_jobs, int3
         VTA_TRY(launch(ctx, ff, dim3(d.total_combos), dim3(64), 0, a));
     }
     a.phase = 3;
-    VTA_TRY(launch(ctx, fp, dim3(d.total_blocks), dim3(64), kBatchStageBytes, a));
+    VTA_TRY(launch(ctx, fp, dim3(d.total_blocks), dim3(64), batch_stage_bytes(k), a));
     return VTA_OK;
 }
 
