@@ -333,7 +333,7 @@ def bench_rolling(steps, warmup, ev):
     for name, fn, nout, kern, lds_kern in (
             ("bb", lambda: plan.launch_bb(20, 2.0, 2.0), 3, "bb_ms1p_f32", None),
             ("pnr", lambda: plan.launch_pnr(100, 70.0), 1, "pnr_ms1p_reg112", "pnr_ms1p_f32"),
-            ("mad", lambda: plan.launch_mad(64), 1, "mad_ms1p_reg64", "mad_ms1p_f32")):
+            ("mad", lambda: plan.launch_mad(64), 1, "mad_ms1p_len64", "mad_ms1p_f32")):
         mean, ks, _ = _time_launches(fn, ctx.stream, steps, warmup, ev, sync=ctx.synchronize)
         t = mean * 1e-3
         alg = 4 * rows * cols * (1 + nout)

@@ -1147,6 +1147,74 @@ __device__ __forceinline__ float rw_mad(const RegWin<CAP> &w, int len) {
     return (float)(odd ? d1 : 0.5 * (d1 + d2));
 }
 
+// MAD for a window length fixed at compile time (LEN >= 4): the ranks, the
+// median slots and every half-cleaner path choice are constants, so the
+// network is straight-line code (no path switch) over NP = LEN rounded up to
+// a power of two; entries past LEN are +Inf.  Same operations and order of
+// selection as rw_mad.
+template <int LEN>
+struct MadLen {
+    static constexpr int CAP = (LEN + 15) / 16 * 16;
+    static constexpr int NP = rw_pow2(LEN);
+    static constexpr int MID = LEN >> 1;
+    static constexpr bool ODD = (LEN & 1) != 0;
+    static constexpr int K1 = ODD ? MID : MID - 1;
+    static constexpr int K2 = MID;
+};
+
+template <int LEN>
+__device__ __forceinline__ float rw_mad_len(const RegWin<MadLen<LEN>::CAP> &w) {
+    typedef MadLen<LEN> M;
+    static_assert(M::NP >= 4, "MAD window length");
+    constexpr int NP = M::NP, h1 = NP / 2, h2 = NP / 4;
+    const float hi = w.get(M::MID);
+    const double med = M::ODD ? (double)hi : 0.5 * ((double)w.get(M::MID - 1) + (double)hi);
+    if (!isfinite(med)) return VTA_QNAN_F32;
+    auto dev = [&](int i) -> double { return i < LEN ? fabs((double)w.get(i) - med) : INFINITY; };
+    constexpr bool a1 = (M::K1 & h1) != 0, a2 = (M::K1 & h2) != 0;
+    constexpr bool b1 = (M::K2 & h1) != 0, b2 = (M::K2 & h2) != 0;
+    constexpr bool split12 = a1 != b1 || a2 != b2;
+    double A[h2], B[h2];
+#pragma unroll
+    for (int i = 0; i < h2; ++i) {
+        const double d0 = dev(i), d1 = dev(i + h2), d2 = dev(i + h1), d3 = dev(i + h1 + h2);
+        const double lo0 = fmin(d0, d2), hi0 = fmax(d0, d2), lo1 = fmin(d1, d3), hi1 = fmax(d1, d3);
+        const double x0 = a1 ? hi0 : lo0, x1 = a1 ? hi1 : lo1;
+        A[i] = a2 ? fmax(x0, x1) : fmin(x0, x1);
+        if (split12) {
+            const double y0 = b1 ? hi0 : lo0, y1 = b1 ? hi1 : lo1;
+            B[i] = b2 ? fmax(y0, y1) : fmin(y0, y1);
+        }
+    }
+    bool split = split12;
+#pragma unroll
+    for (int lv = 0; lv < rw_log2(NP) - 2; ++lv) {
+        const int h = h2 / 2 >> lv;
+        const bool ba = (M::K1 & h) != 0, bb = (M::K2 & h) != 0;
+        if (!split && ba != bb) {
+#pragma unroll
+            for (int i = 0; i < h2; ++i) B[i] = A[i];
+            split = true;
+        }
+#pragma unroll
+        for (int i = 0; i < h; ++i) {
+            A[i] = ba ? fmax(A[i], A[i + h]) : fmin(A[i], A[i + h]);
+            if (split) B[i] = bb ? fmax(B[i], B[i + h]) : fmin(B[i], B[i + h]);
+        }
+    }
+    const double d1 = A[0], d2 = split ? B[0] : A[0];
+    return (float)(M::ODD ? d1 : 0.5 * (d1 + d2));
+}
+
+template <int LEN>
+struct MadLenOut {
+    __device__ __forceinline__ int slot(const RwCounts &) const { return 0; }
+    __device__ __forceinline__ float operator()(const RegWin<MadLen<LEN>::CAP> &w,
+                                                const RwCounts &c) const {
+        return c.nan_cnt != 0 ? VTA_QNAN_F32 : rw_mad_len<LEN>(w);
+    }
+};
+
 // period 1: 0 for every non-NaN bar, +-Inf included (medium_ad.rs:326-333)
 template <int CAP>
 struct MadRegOut {
@@ -1174,4 +1242,17 @@ __device__ __forceinline__ void mad_reg_body(const roll_args &a) {
         mad_reg_body<CAP>(a);                                                              \
     }
 RW_PNR(16) RW_PNR(32) RW_PNR(48) RW_PNR(64) RW_PNR(80) RW_PNR(96) RW_PNR(112) RW_PNR(128)
-RW_MAD(16) RW_MAD(32) RW_MAD(48) RW_MAD(64)
+// periods 1-3 (the length-templated kernels below start at 4)
+RW_MAD(16)
+#define RW_MAD_LEN(L)                                                                      \
+    extern "C" __global__ __launch_bounds__(64, 2) void mad_ms1p_len##L(roll_args a) {     \
+        rw_chunk<MadLen<L>::CAP, false>(a, MadLenOut<L>{});                                \
+    }
+RW_MAD_LEN(4) RW_MAD_LEN(5) RW_MAD_LEN(6) RW_MAD_LEN(7) RW_MAD_LEN(8) RW_MAD_LEN(9) RW_MAD_LEN(10) RW_MAD_LEN(11)
+RW_MAD_LEN(12) RW_MAD_LEN(13) RW_MAD_LEN(14) RW_MAD_LEN(15) RW_MAD_LEN(16) RW_MAD_LEN(17) RW_MAD_LEN(18) RW_MAD_LEN(19)
+RW_MAD_LEN(20) RW_MAD_LEN(21) RW_MAD_LEN(22) RW_MAD_LEN(23) RW_MAD_LEN(24) RW_MAD_LEN(25) RW_MAD_LEN(26) RW_MAD_LEN(27)
+RW_MAD_LEN(28) RW_MAD_LEN(29) RW_MAD_LEN(30) RW_MAD_LEN(31) RW_MAD_LEN(32) RW_MAD_LEN(33) RW_MAD_LEN(34) RW_MAD_LEN(35)
+RW_MAD_LEN(36) RW_MAD_LEN(37) RW_MAD_LEN(38) RW_MAD_LEN(39) RW_MAD_LEN(40) RW_MAD_LEN(41) RW_MAD_LEN(42) RW_MAD_LEN(43)
+RW_MAD_LEN(44) RW_MAD_LEN(45) RW_MAD_LEN(46) RW_MAD_LEN(47) RW_MAD_LEN(48) RW_MAD_LEN(49) RW_MAD_LEN(50) RW_MAD_LEN(51)
+RW_MAD_LEN(52) RW_MAD_LEN(53) RW_MAD_LEN(54) RW_MAD_LEN(55) RW_MAD_LEN(56) RW_MAD_LEN(57) RW_MAD_LEN(58) RW_MAD_LEN(59)
+RW_MAD_LEN(60) RW_MAD_LEN(61) RW_MAD_LEN(62) RW_MAD_LEN(63) RW_MAD_LEN(64)

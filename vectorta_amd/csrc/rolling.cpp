@@ -131,8 +131,13 @@ vta_status roll_launch(vta_ctx *ctx, RInd k, bool batch, roll_args_host &a, size
     hipFunction_t f;
     if (!batch) {
         if (const int cap = reg_window_cap(ctx, k, max_w)) {
+            // MAD: one kernel per period from 4 (its ranks and selection
+            // paths are compile-time); PNR and MAD periods 1-3: per capacity
             char name[32];
-            snprintf(name, sizeof name, "%s_ms1p_reg%d", k == R_PNR ? "pnr" : "mad", cap);
+            if (k == R_MAD && max_w >= 4)
+                snprintf(name, sizeof name, "mad_ms1p_len%d", (int)max_w);
+            else
+                snprintf(name, sizeof name, "%s_ms1p_reg%d", k == R_PNR ? "pnr" : "mad", cap);
             VTA_TRY(get_function(ctx, "rolling", name, &f));
             if (k == R_PNR) {
                 const vta_kernel_policy &pol = ctx->policy[VTA_POLICY_SLOT_PNR];

@@ -418,24 +418,26 @@ vta_status vta_ctx_preload(vta_ctx *ctx) {
         {"cross", "cross_pairs_f64"},
         {"cross", "cross_reduce"},
         {"rolling", "mad_ms1p_f32"},
-        {"rolling", "pnr_ms1p_reg16"},
-        {"rolling", "pnr_ms1p_reg32"},
-        {"rolling", "pnr_ms1p_reg48"},
-        {"rolling", "pnr_ms1p_reg64"},
-        {"rolling", "pnr_ms1p_reg80"},
-        {"rolling", "pnr_ms1p_reg96"},
-        {"rolling", "pnr_ms1p_reg112"},
-        {"rolling", "pnr_ms1p_reg128"},
         {"rolling", "mad_ms1p_reg16"},
-        {"rolling", "mad_ms1p_reg32"},
-        {"rolling", "mad_ms1p_reg48"},
-        {"rolling", "mad_ms1p_reg64"},
     };
     clear_error();
     if (!ctx) return fail(VTA_ERR_INVALID_INPUT, "ctx is null");
     for (auto &s : kSymbols) {
         hipFunction_t f;
         VTA_TRY(get_function(ctx, s[0], s[1], &f));
+    }
+    // register-window PNR kernels (16..128 slots) and MAD kernels per period
+    // (4..64)
+    char name[32];
+    for (int cap = 16; cap <= 128; cap += 16) {
+        hipFunction_t f;
+        snprintf(name, sizeof name, "pnr_ms1p_reg%d", cap);
+        VTA_TRY(get_function(ctx, "rolling", name, &f));
+    }
+    for (int len = 4; len <= 64; ++len) {
+        hipFunction_t f;
+        snprintf(name, sizeof name, "mad_ms1p_len%d", len);
+        VTA_TRY(get_function(ctx, "rolling", name, &f));
     }
     return VTA_OK;
 }
