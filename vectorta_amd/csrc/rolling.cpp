@@ -50,16 +50,20 @@ enum RInd { R_BB = 0, R_PNR = 1, R_MAD = 2 };
 // (BB re-anchors its f64 running sums at every chunk start).  `chunked`
 // false: one chunk.
 int roll_chunk(const vta_ctx *ctx, size_t T, size_t lanes, size_t max_w, bool chunked,
-               size_t smem) {
+               size_t smem, size_t bb_waves = 8192) {
     if (!chunked) return (int)T;
     if (const char *e = getenv("VTA_ROLL_CHUNK")) {
         long v = atol(e);
         if (v >= 64) return (int)std::min<size_t>((size_t)v, T);
     }
-    // Bollinger Bands (no LDS windows): ~8,192 waves, chunks down to 128 bars
-    // (1,024 x 50,000, period 20: 391-bar chunks = 2,048 waves 0.291 ms,
-    // 256 0.254, 192 0.222, 128 0.216 ms)
-    size_t waves = 8192, min_chunk = 128;
+    // Bollinger Bands: bb_waves waves, chunks down to 128 bars.  Read from
+    // HBM, the outgoing bars favoured short chunks (1,024 x 50,000, period
+    // 20: 391-bar chunks = 2,048 waves 0.291 ms, 256 0.254, 192 0.222, 128
+    // 0.216 ms at ~8,192 waves); the many-series kernel's LDS ring of the
+    // last p bars, filled once per chunk, favours long ones (128-bar chunks
+    // 0.194 ms, 196 0.191, 391 = 2,048 waves 0.182, 520 = 1,552 waves 0.215
+    // ms), so its launches ask for ~2,048 waves.
+    size_t waves = bb_waves, min_chunk = 128;
     if (smem > 0) {
         const size_t per_cu = std::max<size_t>(
             1, (size_t)ctx->props.maxSharedMemoryPerMultiProcessor / smem);
@@ -154,20 +158,21 @@ vta_status roll_launch(vta_ctx *ctx, RInd k, bool batch, roll_args_host &a, size
         }
     }
     VTA_TRY(get_function(ctx, "rolling", kname(k, batch), &f));
-    // PNR / MAD: the lanes' sorted windows, plus (batch) the 64 x 33 output staging
-    size_t smem = (k == R_BB) ? 0
-                              : (size_t)64 * (max_w + 1) * sizeof(float) +
-                                    (batch ? (size_t)64 * 33 * sizeof(float) : 0);
     // many-series BB: the lanes' rings of their last `period` bars (windows up
     // to 256, 64 KiB); wcap 0 reads the outgoing bars from HBM instead
     size_t ring = 0;
     if (k == R_BB && !batch && max_w <= 256) ring = (size_t)64 * max_w * sizeof(float);
+    // PNR / MAD: the lanes' sorted windows, plus (batch) the 64 x 33 output staging
+    size_t smem = (k == R_BB) ? 0
+                              : (size_t)64 * (max_w + 1) * sizeof(float) +
+                                    (batch ? (size_t)64 * 33 * sizeof(float) : 0);
     if (k == R_PNR) {
         // CudaPnrPolicy: the plain form runs each lane over the whole series
         const vta_kernel_policy &pol = ctx->policy[VTA_POLICY_SLOT_PNR];
         if ((batch ? pol.batch : pol.many_series) == VTA_KERNEL_PLAIN) chunked = false;
     }
-    a.chunk = roll_chunk(ctx, (size_t)a.T, (size_t)a.nlanes, max_w, chunked, smem);
+    a.chunk = roll_chunk(ctx, (size_t)a.T, (size_t)a.nlanes, max_w, chunked, smem,
+                         ring ? 2048 : 8192);
     if (k == R_PNR) {
         const vta_kernel_selected sel = {a.chunk < a.T ? VTA_SELECTED_CHUNKED : VTA_SELECTED_PLAIN,
                                          64};
