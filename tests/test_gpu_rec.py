@@ -114,6 +114,26 @@ def test_scan_short_chunks_long_lead(ctxs, monkeypatch, name):
     assert err <= 1e-5, (name, err)
 
 
+def test_wilders_seeded_matches_replay(monkeypatch):
+    """Wilders batches of up to 4,096 combos start each exact lane from a
+    per-combo seed at its last warm-up bar; larger ones replay the warm-up
+    from first_valid.  Both are the same arithmetic, so with one chunk length
+    the 4,096 rows of a seeded batch equal bit for bit those of a 4,097-combo
+    batch over the same periods, and a sample of rows matches the oracle."""
+    require_gpu()
+    monkeypatch.setenv("VTA_SCAN_CHUNK", "256")
+    _, _, c = _hlc(9_000, 37, lead=700)
+    op = vta.CudaWilders(0)
+    seeded = op.wilders_batch_dev(c, (1, 4_096, 1)).to_host()
+    replay = op.wilders_batch_dev(c, (1, 4_097, 1)).to_host()
+    assert seeded.shape == (4_096, 9_000) and replay.shape == (4_097, 9_000)
+    assert np.array_equal(seeded.view(np.uint32), replay[:4_096].view(np.uint32))
+    rows = [0, 1, 2, 3, 4, 13, 255, 256, 1_000, 4_095]
+    ref = np.stack([O.wilders(c.astype(np.float64), r + 1) for r in rows])
+    nan_mask_equal(seeded[rows], ref)
+    assert max_rel_err(seeded[rows], ref, floor=0.0) <= 1e-5
+
+
 @pytest.mark.parametrize("n,period", [(3_000, (1, 60, 11)), (300_000, (14, 250, 59))])
 def test_atr_batch(ctxs, n, period):
     h, lo, c = _hlc(n, 33, lead=3)

@@ -54,6 +54,8 @@ struct rec_args {
     double *carry;           // batch scan: [combo][chunk][states] chunk-end states
     const float *alphas;     // EMA / Wilders: caller's smoothing factor (batch: per combo,
                              // many-series: [0]); NULL derives it from the period
+    double *seeds;           // Wilders batch: per-combo value at bar first_valid + period - 1
+                             // (wilders_seed), or NULL (exact lanes replay the warm-up)
     int T;                   // bars
     int nlanes;              // combos (batch) or series (many-series)
     int chunk;               // bars per chunk
@@ -129,6 +131,7 @@ struct EmaEng {
     static constexpr bool kAdvance = false;
     static constexpr int kPre = 0;
     static constexpr int kStageW = 64;
+    static constexpr bool kSeeded = false;
     // steps t < seed_end() after the start are the warm-up accumulation only
     __device__ int seed_end() const { return fv + p; }
     __device__ double seed_step(int t, double x) {
@@ -246,6 +249,7 @@ struct RsiEng {
     // (see drive_batch) measured slower (0.55 vs 0.47 ms, 1,250 x 200k)
     static constexpr int kPre = 0;
     static constexpr int kStageW = 32;
+    static constexpr bool kSeeded = false;
     __device__ int seed_end() const { return fv + p; }
     __device__ double seed_step(int t, double x) {
         double d = x - px;
@@ -333,6 +337,7 @@ struct AtrEng {
     static constexpr bool kAdvance = false;
     static constexpr int kPre = 1;
     static constexpr int kStageW = 64;
+    static constexpr bool kSeeded = false;
     __device__ int seed_end() const { return fv + p - 1; }
     __device__ double seed_step(int t, double h, double l, double c) {
         sum += true_range(h, l, pc);
@@ -403,6 +408,7 @@ struct AtrTrEng {
     static constexpr bool kAdvance = false;
     static constexpr int kPre = 0;
     static constexpr int kStageW = 64;
+    static constexpr bool kSeeded = false;
     __device__ int seed_end() const { return fv + p - 1; }
     __device__ double seed_step(int t, double tr) {
         sum += tr;
@@ -496,6 +502,7 @@ struct AdxEng {
     static constexpr bool kAdvance = true;
     static constexpr int kPre = 3;
     static constexpr int kStageW = 32;
+    static constexpr bool kSeeded = false;
     __device__ int seed_end() const { return fv + p; }
     __device__ double seed_step(int t, double ch, double cl, double cc) {
         double hl = ch - cl;
@@ -665,6 +672,7 @@ struct SmmaEng {
     static constexpr bool kAdvance = false;
     static constexpr int kPre = 0;
     static constexpr int kStageW = 64;
+    static constexpr bool kSeeded = false;
     __device__ int seed_end() const { return p == 1 ? fv : fv + p - 1; }
     __device__ double seed_step(int t, double x) {
         sum += x;
@@ -729,6 +737,10 @@ struct WildersEng {
     static constexpr bool kAdvance = false;
     static constexpr int kPre = 0;
     static constexpr int kStageW = 64;
+    // a scan lane that would replay its warm-up from first_valid starts at
+    // its last warm-up bar from the precomputed seed instead (wilders_seed)
+    static constexpr bool kSeeded = true;
+    __device__ double begin_seeded(double seed) { y = seed; return y; }
     __device__ int seed_end() const { return fv + p - 1; }
     __device__ double seed_step(int t, double x) { return seed_bar(t - fv, x); }
     __device__ double steady(double x) {
@@ -1058,6 +1070,7 @@ struct batch_view {
     const int32_t *periods;
     double *carry;           // [combo][chunk][states]
     const float *alphas;     // per combo, or NULL (see apply_alpha)
+    const double *seeds;     // Wilders: per-combo seed, or NULL
     int T, nlanes, chunk, nch, phase, fv, group, k;
     bool clean;
 };
@@ -1070,6 +1083,7 @@ __device__ __forceinline__ batch_view view_of(const rec_args &a) {
     v.periods = a.periods;
     v.carry = a.carry;
     v.alphas = a.alphas;
+    v.seeds = a.seeds;
     v.T = a.T;
     v.nlanes = a.nlanes;
     v.chunk = a.chunk;
@@ -1117,7 +1131,7 @@ __device__ __forceinline__ void drive_batch(const batch_view &a, float *out) {
     E e;
     e.setup(p, fv);
     if (active) apply_alpha(e, a.alphas, c);
-    bool burn = false, carried = false;
+    bool burn = false, carried = false, seeded = false;
     int start;
     if (a.phase != 0) {
         // chunked scan: chunks after first_valid (past the warm-up for engines
@@ -1128,6 +1142,15 @@ __device__ __forceinline__ void drive_batch(const batch_view &a, float *out) {
         start = active ? (carried ? c0 - 1 : e.exact_start()) : T;
     } else {
         start = active ? lane_start(e, c0, clean, &burn) : T;
+    }
+    double seedv = 0.0;
+    if constexpr (E::kSeeded) {
+        // an exact start whose seed is known begins at the last warm-up bar
+        if (active && !carried && !burn && a.seeds) {
+            seeded = true;
+            start = e.fv + e.p - 1;
+            seedv = a.seeds[c];
+        }
     }
     if constexpr (E::kSeedScan) {
         // RSI burn-in: the seed window's NaN rule still has to be seen
@@ -1196,6 +1219,14 @@ __device__ __forceinline__ void drive_batch(const batch_view &a, float *out) {
                 np2 = a.l[tp];
             }
         }
+    };
+    // a whole block that seeded lanes can run without the per-bar start logic
+    auto seeded_block = [&](int tb) {
+        if constexpr (E::kSeeded && NIN == 1)
+            return tb + 64 <= c1 &&
+                   __all(!active || seeded || (start < tb && tb >= e.burn_ok_from()));
+        else
+            return false;
     };
     load_next(t_begin);
     for (int tb = t_begin; tb < c1; tb += 64) {
@@ -1322,6 +1353,32 @@ __device__ __forceinline__ void drive_batch(const batch_view &a, float *out) {
                 }
             }
           }
+        } else if (seeded_block(tb)) {
+            // lanes that start from a seed in this block or a later one run the
+            // steady update on every bar, take the seed at their start bar and
+            // write NaN before it; the others are past their warm-ups
+            if constexpr (E::kSeeded) {
+                auto sbar = [&](int t, float x) {
+                    double y = e.steady(x);
+                    if (t == start) y = e.begin_seeded(seedv);
+                    return t < start ? REC_NAN : y;
+                };
+#pragma unroll 2
+                for (int j = 0; j < 64; j += 4) {
+                    const float4 x4 = *reinterpret_cast<const float4 *>(&xs[0][j]);
+                    const double y0 = sbar(tb + j, x4.x);
+                    const double y1 = sbar(tb + j + 1, x4.y);
+                    const double y2 = sbar(tb + j + 2, x4.z);
+                    const double y3 = sbar(tb + j + 3, x4.w);
+                    if (emit) {
+                        ob[lane][j & (SW - 1)] = (float)y0;
+                        ob[lane][(j + 1) & (SW - 1)] = (float)y1;
+                        ob[lane][(j + 2) & (SW - 1)] = (float)y2;
+                        ob[lane][(j + 3) & (SW - 1)] = (float)y3;
+                        if (SW == 32 && ((j + 4) & 31) == 0) flush_half(tb, j >> 5);
+                    }
+                }
+            }
         } else if (tb + 64 <= c1 && __all(!active || (start < tb && tb + 64 <= e.seed_end()))) {
             // every lane still accumulating its warm-up seed for the whole block
 #pragma unroll 2
@@ -1359,6 +1416,8 @@ __device__ __forceinline__ void drive_batch(const batch_view &a, float *out) {
                     // (a carried lane began before the loop)
                     if constexpr (NIN == 1) {
                         if (carried) {
+                        } else if (seeded) {
+                            if constexpr (E::kSeeded) y = e.begin_seeded(seedv);
                         } else if (burn) e.begin_burn(x);
                         else y = e.begin_exact(x);
                     } else {
@@ -1554,6 +1613,7 @@ __device__ __forceinline__ batch_view pack_view(const rec_pack_args &p, int j) {
     v.periods = jb.periods;
     v.carry = jb.carry;
     v.alphas = nullptr;
+    v.seeds = nullptr;
     v.T = jb.T;
     v.nlanes = jb.nlanes;
     v.chunk = jb.chunk;
@@ -1642,6 +1702,55 @@ extern "C" __global__ __launch_bounds__(64) void smma_scan_combine(rec_args a) {
 }
 extern "C" __global__ __launch_bounds__(64) void wilders_scan_combine(rec_args a) {
     scan_combine<WildersEng>(view_of(a));
+}
+
+// Per-combo Wilders seed of a one-series batch (wave = combo): the value the
+// lane reaches at bar fv + p - 1 by replaying its warm-up (seed_bar): groups
+// of four bars g = ((x0 + x1) + x2) + x3, each group's own bars added in
+// order, summed from 0.0 one group after another, times the smoothing
+// factor.  Lanes form 64 groups (256 bars) at a time from one coalesced
+// read; the wave then adds them in group order (each read out of its lane),
+// so the sum is the same sequence of f64 operations as the replay.  Combos
+// whose warm-up passes the series end get no seed (their lanes have no
+// output).
+extern "C" __global__ __launch_bounds__(256) void wilders_seed(rec_args a) {
+    const int lane = threadIdx.x & 63;
+    const int c = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (c >= a.nlanes) return;
+    const int fv = a.fvs[0];
+    WildersEng e;
+    e.setup(__builtin_amdgcn_readfirstlane(a.periods[c]), fv);
+    apply_alpha(e, a.alphas, c);
+    const int p = e.p;
+    if (p < 1 || (long)fv + p - 1 >= a.T) return;
+    const float *x = a.x + fv;
+    auto group = [&](int b) {
+        const int k0 = b + 4 * lane;
+        double g = 0.0;
+        if (k0 < p) {
+            g = (double)x[k0];
+            if (k0 + 1 < p) g += (double)x[k0 + 1];
+            if (k0 + 2 < p) g += (double)x[k0 + 2];
+            if (k0 + 3 < p) g += (double)x[k0 + 3];
+        }
+        return g;
+    };
+    double sum = 0.0;
+    double g = group(0);
+    for (int b = 0; b < p; b += 256) {
+        const double gn = b + 256 < p ? group(b + 256) : 0.0;   // next 256 bars in flight
+        const int ng = min(64, (p - b + 3) >> 2);
+        const unsigned long long gb = __builtin_bit_cast(unsigned long long, g);
+        const int lo = (int)(unsigned)gb, hi = (int)(unsigned)(gb >> 32);
+        for (int j = 0; j < ng; ++j) {
+            const unsigned long long v =
+                ((unsigned long long)(unsigned)__builtin_amdgcn_readlane(hi, j) << 32) |
+                (unsigned)__builtin_amdgcn_readlane(lo, j);
+            sum += __builtin_bit_cast(double, v);
+        }
+        g = gn;
+    }
+    if (lane == 0) a.seeds[c] = sum * e.al;
 }
 
 // ---------------------------------------------------------------- preparation

@@ -29,6 +29,7 @@ struct rec_args_host {
     float *out2;
     double *carry;
     const float *alphas;
+    double *seeds;
     int T;
     int nlanes;
     int chunk;
@@ -36,7 +37,7 @@ struct rec_args_host {
     int phase;
     int pad_;
 };
-static_assert(sizeof(rec_args_host) == 128, "rec_args layout");
+static_assert(sizeof(rec_args_host) == 136, "rec_args layout");
 
 enum Ind {
     IND_EMA = 0,
@@ -167,6 +168,12 @@ unsigned batch_stage_bytes(Ind k) {
     return 64u * (w + 1u) * (unsigned)sizeof(float);
 }
 
+// Wilders seeds: doubles from byte 32,768 of the context scratch, one per
+// combo, for batches of up to 4,096 combos (larger ones replay the warm-ups).
+constexpr size_t kSeedOffset = 32 * 1024;
+constexpr size_t kSeedCombos = (vta_ctx::kScratchBytes - kSeedOffset) / sizeof(double);
+static_assert(kSeedOffset >= (kArgErrorWord + 1) * sizeof(int32_t), "seeds after the error word");
+
 vta_status launch_rec(vta_ctx *ctx, const char *kname, rec_args_host &a, unsigned groups,
                       unsigned chunks, unsigned block = 64, unsigned smem = 0) {
     hipFunction_t f;
@@ -176,8 +183,9 @@ vta_status launch_rec(vta_ctx *ctx, const char *kname, rec_args_host &a, unsigne
 
 // Chunked-scan launch: chunks sized for ~kScanWaves waves, at least 64 bars
 // and at most 1,024 chunks (the fold walks them 64 at a time; a Wilders lane
-// whose warm-up spans several chunks replays it exactly from first_valid in
-// each, which only the first chunks of the series pay), and as many as the
+// whose warm-up spans several chunks starts from its seed at the last
+// warm-up bar, or, past kSeedCombos, replays the warm-up exactly from
+// first_valid in each), and as many as the
 // context's carry buffer holds for C
 // combos; pass 1, the per-combo fold (one wave per combo; two rounds for
 // ADX), pass 3.  Small requests are latency-bound, so their chunk is short:
@@ -295,8 +303,15 @@ vta_status batch_launch(vta_ctx *ctx, Ind k, const float *d_x, const float *d_h,
     a.T = (int)T;
     a.nlanes = (int)C;
     const bool plain = plain_batch(ctx, k);
-    // (a Wilders warm-up longer than a chunk is still exact: its lanes replay it
-    // from first_valid, see test_scan_short_chunks_long_lead)
+    if (k == IND_WILDERS && C <= kSeedCombos) {
+        // per-combo seeds (the second half of the context scratch): an exact
+        // lane then starts at its last warm-up bar instead of replaying the
+        // warm-up from first_valid in every chunk the warm-up spans
+        a.seeds = reinterpret_cast<double *>(static_cast<char *>(ctx->scratch) + kSeedOffset);
+        VTA_TRY(launch_rec(ctx, "wilders_seed", a, (unsigned)((C + 3) / 4), 1, 256));
+    }
+    // (a Wilders warm-up longer than a chunk without seeds is still exact:
+    // its lanes replay it from first_valid, see test_scan_short_chunks_long_lead)
     if (clean && !plain) {
         bool done;
         VTA_TRY(scan_launch(ctx, k, a, T, C, max_p, &done));
