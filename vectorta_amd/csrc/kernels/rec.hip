@@ -61,7 +61,8 @@ struct rec_args {
     int chunk;               // bars per chunk
     int p1, p2, p3;          // fused kernel periods (EMA, RSI, ATR)
     int phase;               // batch: 0 burn-in/exact, 1 scan pass, 3 output pass of the scan
-    int pad_;
+    int clean_epoch;         // batch: 0, or clean = (clean[0] != clean_epoch), the word
+                             // batch_prepare stores the call's epoch in on a non-finite bar
 };
 
 #define REC_NAN __builtin_bit_cast(double, 0x7ff8000000000000ull)
@@ -1092,7 +1093,8 @@ __device__ __forceinline__ batch_view view_of(const rec_args &a) {
     v.fv = a.fvs[0];
     v.group = blockIdx.x;
     v.k = blockIdx.y;
-    v.clean = a.clean != nullptr && a.clean[0] != 0;
+    v.clean = a.clean != nullptr &&
+              (a.clean_epoch != 0 ? a.clean[0] != a.clean_epoch : a.clean[0] != 0);
     return v;
 }
 
@@ -1831,6 +1833,31 @@ extern "C" __global__ __launch_bounds__(64) void ms_first_nonfinite_from_first_v
 }
 
 // Single series: clean flag = no non-finite input in [first_valid, T).
+// The one launch before a device-resident one-series batch: meta[0] =
+// first_valid (the batch kernels' fvs[0]); meta[2] = epoch when a bar of x
+// (and h, l) at or after first_valid is not finite, so the batch reads its
+// clean flag as meta[2] != epoch and the word needs no clearing (each call
+// brings a new epoch); and, with warms, vta_check_warms' check of the
+// reference's warm indices against first_valid + period - 1 (a mismatch
+// stores 1 in *err).  Grid: enough threads for max(T - first_valid, n).
+extern "C" __global__ __launch_bounds__(256) void batch_prepare(
+    const float *__restrict__ x, const float *__restrict__ h, const float *__restrict__ l,
+    int T, int first_valid, int epoch, int32_t *__restrict__ meta,
+    const int32_t *__restrict__ periods, const int32_t *__restrict__ warms, int n,
+    int32_t *__restrict__ err) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i == 0) meta[0] = first_valid;
+    const int t = i + first_valid;
+    if (t < T) {
+        bool bad = !vta_isfinite(x[t]);
+        if (h) bad = bad || !vta_isfinite(h[t]);
+        if (l) bad = bad || !vta_isfinite(l[t]);
+        if (bad) meta[2] = epoch;
+    }
+    if (warms && i < n && (long long)warms[i] != (long long)first_valid + periods[i] - 1)
+        err[0] = 1;
+}
+
 // flag[0] must be 0 on entry (see vta_fill_i32); every failing lane stores 1.
 extern "C" __global__ __launch_bounds__(256) void series_nonfinite_flag(
     const float *__restrict__ x, const float *__restrict__ h, const float *__restrict__ l,

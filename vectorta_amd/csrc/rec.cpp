@@ -35,7 +35,7 @@ struct rec_args_host {
     int chunk;
     int p1, p2, p3;
     int phase;
-    int pad_;
+    int clean_epoch;
 };
 static_assert(sizeof(rec_args_host) == 136, "rec_args layout");
 
@@ -290,7 +290,7 @@ void record_batch(vta_ctx *ctx, Ind k, bool chunked) {
 vta_status batch_launch(vta_ctx *ctx, Ind k, const float *d_x, const float *d_h,
                         const float *d_l, const int32_t *d_periods, size_t max_p, size_t T,
                         size_t C, const int32_t *d_fv1, const int32_t *d_clean1, bool clean,
-                        float *d_out, const float *d_alphas = nullptr) {
+                        float *d_out, const float *d_alphas = nullptr, int clean_epoch = 0) {
     rec_args_host a{};
     a.x = d_x;
     a.h = d_h;
@@ -298,6 +298,7 @@ vta_status batch_launch(vta_ctx *ctx, Ind k, const float *d_x, const float *d_h,
     a.periods = d_periods;
     a.fvs = d_fv1;
     a.clean = d_clean1;
+    a.clean_epoch = clean_epoch;
     a.out0 = d_out;
     a.alphas = d_alphas;
     a.T = (int)T;
@@ -398,25 +399,27 @@ vta_status batch_device(vta_ctx *ctx, Ind k, const float *d_x, const float *d_h,
                                            "and first_valid < series_len");
     if (series_len > INT32_MAX || n_combos > INT32_MAX || max_period > series_len)
         return fail(VTA_ERR_INVALID_INPUT, "series_len, n_combos or max_period out of range");
-    if (d_warms) VTA_TRY(check_warms_device(ctx, d_periods, d_warms, (int)n_combos,
-                                            (int)series_len, (int)first_valid, 0, nullptr));
-    // context scratch: [0] first_valid, [2] non-finite flag, [3] clean word
+    // context scratch: [0] first_valid, [2] the non-finite word; one launch
+    // (batch_prepare) stores both and checks the warm indices.  The word
+    // holds this call's epoch if a bar is not finite, so it is never
+    // cleared; an epoch repeated after 2^31 calls can only read a clean
+    // series as unclean, which runs exactly (and correctly) from first_valid.
     int32_t *meta = static_cast<int32_t *>(ctx->scratch);
-    hipFunction_t fill, flag, inv;
-    VTA_TRY(get_function(ctx, "rec", "vta_fill_i32", &fill));
-    VTA_TRY(get_function(ctx, "rec", "series_nonfinite_flag", &flag));
-    VTA_TRY(get_function(ctx, "rec", "flag_to_clean", &inv));
-    VTA_TRY(launch(ctx, fill, dim3(1), dim3(256), 0, meta, 1, (int)first_valid));
-    VTA_TRY(launch(ctx, fill, dim3(1), dim3(256), 0, meta + 2, 1, 0));
-    unsigned nb = (unsigned)((series_len - first_valid + 255) / 256);
-    VTA_TRY(launch(ctx, flag, dim3(nb), dim3(256), 0, d_x, hlc(k) ? d_h : nullptr,
-                   hlc(k) ? d_l : nullptr, (int)series_len, (int)first_valid, meta + 2));
-    VTA_TRY(launch(ctx, inv, dim3(1), dim3(64), 0, meta + 2, meta + 3));
+    if (++ctx->prep_epoch > INT32_MAX || ctx->prep_epoch == 0) ctx->prep_epoch = 1;
+    const int epoch = (int)ctx->prep_epoch;
+    hipFunction_t prep;
+    VTA_TRY(get_function(ctx, "rec", "batch_prepare", &prep));
+    int32_t *err = static_cast<int32_t *>(ctx->scratch) + kArgErrorWord;
+    if (d_warms) ctx->arg_check_pending = true;
+    const int64_t span = std::max<int64_t>(series_len - first_valid, d_warms ? n_combos : 0);
+    VTA_TRY(launch(ctx, prep, dim3((unsigned)((span + 255) / 256)), dim3(256), 0, d_x,
+                   hlc(k) ? d_h : nullptr, hlc(k) ? d_l : nullptr, (int)series_len,
+                   (int)first_valid, epoch, meta, d_periods, d_warms, (int)n_combos, err));
     // Chunked with burn-in: a series found unclean still runs correctly, since
     // every lane of an unclean stream starts exactly at first_valid.
     return batch_launch(ctx, k, d_x, d_h, d_l, d_periods, (size_t)max_period,
-                        (size_t)series_len, (size_t)n_combos, meta, meta + 3, true, d_out,
-                        d_alphas);
+                        (size_t)series_len, (size_t)n_combos, meta, meta + 2, true, d_out,
+                        d_alphas, epoch);
 }
 
 vta_status ms_launch(vta_ctx *ctx, Ind k, const float *d_x, const float *d_h, const float *d_l,

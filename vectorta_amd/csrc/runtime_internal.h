@@ -39,6 +39,8 @@ struct vta_ctx {
     // an enqueue-only call with device-side argument checks (warm indices) has
     // run since the last vta_ctx_synchronize, which then reads the error word
     bool arg_check_pending = false;
+    // epoch of the last device-resident one-series batch (batch_prepare)
+    uint32_t prep_epoch = 0;
 };
 
 namespace vta {
