@@ -283,6 +283,29 @@ def test_device_entry_points(ctxs):
     assert ctxs["ema"].selected_many_series_kernel() == ("chunked", 64)
 
 
+def test_device_form_clean_word_per_call(ctxs):
+    """The device forms' clean flag is a scratch word that a call stamps with
+    its own epoch when it finds a non-finite bar, never cleared: a series with
+    an interior NaN (EMA skips it, which only the exact path does) between two
+    clean calls on one context must run exactly, and the clean calls after it
+    must not inherit its mark -- all three against the oracle."""
+    x = datagen.gbm_close(30_000, 0.01, seed=47)
+    x[:5] = np.nan
+    bad = x.copy()
+    bad[12_345] = np.nan
+    ctx = ctxs["ema"].ctx
+    ps = np.arange(2, 400, 9, dtype=np.int32)
+    d_p = ctx.to_device(ps)
+    d_o = ctx.alloc(ps.size * x.size * 4)
+    for series in (x, bad, x, bad):
+        d_x = ctx.to_device(series)
+        ctxs["ema"].ema_batch_device(d_x, d_p, None, series.size, 5, ps.size, d_o)
+        ctx.synchronize()
+        g = d_o.copy_to_host().reshape(ps.size, series.size)
+        _check(g, O.ema_batch(series.astype(np.float64), ps.tolist()), "ema")
+        d_x.free()
+
+
 def _ema_custom_alpha(x, p, alpha):
     """EMA with the reference kernels' structure and a given smoothing factor:
     running mean over the first p valid bars (outputs from first_valid), then
