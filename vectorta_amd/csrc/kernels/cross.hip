@@ -105,22 +105,26 @@ extern "C" __global__ __launch_bounds__(256) void cross_rows_f64(cross_args a) {
 struct CrossRun {
     long long tr = 0, nl = 0, ns = 0, ck = 0;
     double pnl = 0.0;
-    // run [r0, r1) at position p
-    __device__ void close(int p, int r0, int r1, const float *x, int T) {
+    // run [r0, r1) at position p; xa = x[r0], xe = x[min(r1, T - 1)],
+    // t0 = tri(r0), t1 = tri(r1)
+    __device__ void close(int p, int r0, int r1, long long t0, long long t1, float xa, float xe) {
         if (r1 <= r0 || p == 0) return;
         const long long n = (long long)(r1 - r0);
         if (p > 0) nl += n; else ns += n;
-        const long long s = ((long long)r1 * (r1 + 1) - (long long)r0 * (r0 + 1)) / 2;
+        const long long s = t1 - t0;             // sum of t + 1 over [r0, r1)
         ck += p > 0 ? s : -s;
-        const int e = min(r1, T - 1);
-        const double dx = (double)x[e] - (double)x[r0];
+        const double dx = (double)xe - (double)xa;
         pnl += p > 0 ? dx : -dx;
     }
 };
 
+// r (r + 1) / 2: the sum of t + 1 over [0, r)
+__device__ __forceinline__ long long tri(int r) { return ((long long)r * ((long long)r + 1)) >> 1; }
+
 extern "C" __global__ __launch_bounds__(256) void cross_pairs_f64(cross_args a) {
     __shared__ double lf[16 * CR_LD];
     __shared__ double ls[16 * CR_LD];
+    __shared__ float lx[CR_CH];                  // the chunk's prices
     __shared__ int rowid[32];
     const int tid = threadIdx.x;
     const int tf = blockIdx.x / a.ntiles_s;
@@ -142,14 +146,25 @@ extern "C" __global__ __launch_bounds__(256) void cross_pairs_f64(cross_args a) 
     const int tstart = b0 > w ? b0 - 1 : b0;     // one bar of look-back for the position
     const bool walk = active && rf != rs && b0 < seg1;   // identical rows: always 0
     CrossRun acc;
+    // the position at tstart, read from the rows directly, opens the first run
     int prev = 0, run_a = b0;
-    bool have = false;
+    double prevd = 0.0;                          // prev as a double
+    float xa = 0.0f;                             // price at run_a
+    long long ta = 0;                            // tri(run_a)
+    if (walk) {
+        const double d = a.rows[(size_t)rf * T + tstart] - a.rows[(size_t)rs * T + tstart];
+        prev = d > 0.0 ? 1 : (d < 0.0 ? -1 : 0);
+        prevd = (double)prev;
+        xa = a.prices[b0];
+        ta = tri(b0);
+    }
     __syncthreads();
     for (int c0 = max(seg0 - 1, 0); c0 < seg1; c0 += CR_CH) {
         // stage the tile's 32 rows over [c0, c0 + CR_CH), two rows per pass
         const int rh = tid >> 7;
         const int tc = tid & 127;
         const int t = min(c0 + tc, T - 1);
+        if (rh == 0) lx[tc] = a.prices[t];
 #pragma unroll 4
         for (int r2 = 0; r2 < 32; r2 += 2) {
             const int r = r2 + rh;
@@ -159,32 +174,44 @@ extern "C" __global__ __launch_bounds__(256) void cross_pairs_f64(cross_args a) 
         }
         __syncthreads();
         if (walk) {
-            const int lo = max(tstart - c0, 0);
+            const int lo = max(tstart + 1 - c0, 0);
             const int hi = min(seg1 - c0, CR_CH);
             const double *pf = lf + fi * CR_LD;
             const double *ps = ls + sj * CR_LD;
-            for (int tl = lo; tl < hi; ++tl) {
-                const double d = pf[tl] - ps[tl];        // the oracle's f_t - s_t
-                if (have && (double)prev * d > 0.0) continue;   // position unchanged
+            auto bar = [&](int tl, double d) {   // d = the oracle's f_t - s_t
+                if (prevd * d > 0.0) return;   // position unchanged
                 const int pos = d > 0.0 ? 1 : (d < 0.0 ? -1 : 0);
-                if (!have) {
-                    have = true;
-                    prev = pos;
-                    run_a = b0;
-                    continue;
-                }
                 if (pos != prev) {
+                    // tt < seg1 <= T: its price is this chunk's lx[tl]
                     const int tt = c0 + tl;
-                    acc.close(prev, run_a, tt, a.prices, T);
+                    const float xt = lx[tl];
+                    const long long tt1 = tri(tt);
+                    acc.close(prev, run_a, tt, ta, tt1, xa, xt);
                     acc.tr += 1;
                     prev = pos;
+                    prevd = (double)pos;
                     run_a = tt;
+                    xa = xt;
+                    ta = tt1;
                 }
+            };
+            // four bars' differences are read (one LDS wait) before their
+            // sequential position logic; reads past hi are clamped into the chunk
+            for (int tl0 = lo; tl0 < hi; tl0 += 4) {
+                double d[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int tl = min(tl0 + k, hi - 1);
+                    d[k] = pf[tl] - ps[tl];
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (tl0 + k < hi) bar(tl0 + k, d[k]);
             }
         }
         __syncthreads();
     }
-    if (walk && have) acc.close(prev, run_a, seg1, a.prices, T);
+    if (walk) acc.close(prev, run_a, seg1, ta, tri(seg1), xa, a.prices[min(seg1, T - 1)]);
     if (active) {
         const size_t pair = (size_t)f * a.nslow + s;
         const size_t npairs = (size_t)a.nfast * a.nslow;
