@@ -54,8 +54,9 @@ struct cross_args {
     int pad_;
 };
 
-#define CR_CH 128
+#define CR_CH 64
 #define CR_LD (CR_CH + 1)
+#define CR_RPP (256 / CR_CH)
 #define RW_T 1024             // bars per row workgroup (4 per lane)
 #define RW_PMAX 296
 
@@ -161,12 +162,12 @@ extern "C" __global__ __launch_bounds__(256) void cross_pairs_f64(cross_args a) 
     __syncthreads();
     for (int c0 = max(seg0 - 1, 0); c0 < seg1; c0 += CR_CH) {
         // stage the tile's 32 rows over [c0, c0 + CR_CH), two rows per pass
-        const int rh = tid >> 7;
-        const int tc = tid & 127;
+        const int rh = tid / CR_CH;              // CR_RPP rows per staging pass
+        const int tc = tid % CR_CH;
         const int t = min(c0 + tc, T - 1);
         if (rh == 0) lx[tc] = a.prices[t];
 #pragma unroll 4
-        for (int r2 = 0; r2 < 32; r2 += 2) {
+        for (int r2 = 0; r2 < 32; r2 += CR_RPP) {
             const int r = r2 + rh;
             const double v = a.rows[(size_t)rowid[r] * T + t];
             if (r < 16) lf[r * CR_LD + tc] = v;
