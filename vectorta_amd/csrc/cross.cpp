@@ -2,6 +2,7 @@
 // reference names the workload only, README.md:9).  Kernels: kernels/cross.hip.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <tuple>
@@ -40,6 +41,12 @@ static_assert(sizeof(cross_args_host) == 128, "cross_args layout");
 
 constexpr size_t kMaxPeriod = 294;   // LDS staging of the f64 row kernel
 constexpr int kRowTile = 1024;       // bars per cross_rows_f64 workgroup
+// Workgroups of the pair walk (16 x 16 pairs x one time segment each).
+// Tiles of short periods trade more often and walk longer, so small units
+// balance the device: 58,300 pairs x 200k bars took 6.73 / 5.76 / 5.28 /
+// 5.18 / 5.12 / 5.11 ms at 2,048 / 4,046 / 8,100 / 12,000 / 16,000 / 23,000
+// workgroups (at most one segment per 2,048 bars).
+constexpr size_t kCrossWorkgroups = 16384;
 
 // f64 batch weights in the oracle's order (alma.rs:1426-1436), not normalised.
 double weights_batch_raw(size_t period, double offset, double sigma, double *w) {
@@ -112,9 +119,14 @@ vta_status plan_build(vta_ctx *ctx, const float *prices, size_t len,
     if (!mul_ok(fast.size(), slow.size(), &npairs) || npairs > (size_t)INT32_MAX ||
         !mul_ok(nrows, len * sizeof(double), &rows_bytes))
         return fail(VTA_ERR_INVALID_INPUT, "grid too large");
-    // time segments: enough workgroups to fill the device several times
+    // time segments: about kCrossWorkgroups workgroups (VTA_CROSS_WGS overrides)
     const size_t ntiles = ((fast.size() + 15) / 16) * ((slow.size() + 15) / 16);
-    size_t nseg = std::max<size_t>(1, (2048 + ntiles - 1) / ntiles);
+    size_t wgs = kCrossWorkgroups;
+    if (const char *e = getenv("VTA_CROSS_WGS")) {
+        const long v = atol(e);
+        if (v >= 1) wgs = (size_t)v;
+    }
+    size_t nseg = std::max<size_t>(1, (wgs + ntiles - 1) / ntiles);
     nseg = std::min(nseg, std::max<size_t>(1, len / 2048));
     const size_t seg_len = (len + nseg - 1) / nseg;
     nseg = (len + seg_len - 1) / seg_len;
