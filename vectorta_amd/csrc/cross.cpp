@@ -40,7 +40,7 @@ struct cross_args_host {
 static_assert(sizeof(cross_args_host) == 128, "cross_args layout");
 
 constexpr size_t kMaxPeriod = 294;   // LDS staging of the f64 row kernel
-constexpr int kRowTile = 1024;       // bars per cross_rows_f64 workgroup
+constexpr int kRowTile = 2048;       // bars per cross_rows_f64 workgroup (RW_T)
 // Workgroups of the pair walk (16 x 16 pairs x one time segment each).
 // Tiles of short periods trade more often and walk longer, so small units
 // balance the device: 58,300 pairs x 200k bars took 6.73 / 5.76 / 5.28 /

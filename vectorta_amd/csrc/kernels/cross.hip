@@ -57,14 +57,15 @@ struct cross_args {
 #define CR_CH 64
 #define CR_LD (CR_CH + 1)
 #define CR_RPP (256 / CR_CH)
-#define RW_T 1024             // bars per row workgroup (4 per lane)
+#define RW_NB 8               // consecutive bars per lane of the row kernel
+#define RW_T (256 * RW_NB)    // bars per row workgroup
 #define RW_PMAX 296
 
 // f64 ALMA rows in the oracle's order (alma.rs:1505-1527): for output t,
 // sum over 4-tap groups g = (x0 w0 + x1 w1) + x2 w2 + x3 w3, sum += g, then
 // the tail taps one at a time, out = sum * inv_norm.  NaN before fv + p - 1.
 // grid.x = bar tiles of RW_T, grid.y = rows; lane l computes bars
-// t0 + 4 l .. t0 + 4 l + 3.
+// t0 + RW_NB l .. t0 + RW_NB (l + 1) - 1.
 extern "C" __global__ __launch_bounds__(256) void cross_rows_f64(cross_args a) {
 #pragma clang fp contract(off)
     __shared__ double xs[RW_T + RW_PMAX];
@@ -85,43 +86,36 @@ extern "C" __global__ __launch_bounds__(256) void cross_rows_f64(cross_args a) {
     const int p4 = p & ~3;
     const int warm = a.fv + p - 1;
     double *row = a.rows + (size_t)r * T;
-    // lane = four consecutive bars tb .. tb + 3: a 4-tap group of all four
-    // outputs reads the 7 prices x[tb + k .. tb + k + 6] (the last three are
-    // kept for the next group) and 4 weights, each output keeping its own
-    // operation order
-    const int tb = 4 * tid;
+    // lane = RW_NB consecutive bars tb .. tb + RW_NB - 1: a 4-tap group of
+    // all of them reads the RW_NB + 3 prices x[tb + k ..] once and 4 weights,
+    // each output keeping its own operation order
+    const int tb = RW_NB * tid;
     const double *x = xs + tb;              // x[i + k] = price[t0 + tb + i - p + 1 + k]
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    double v4 = 0.0, v5 = 0.0, v6 = 0.0;
-    if (p4 > 0) {
-        v4 = x[0];
-        v5 = x[1];
-        v6 = x[2];
-    }
+    double acc[RW_NB];
+#pragma unroll
+    for (int i = 0; i < RW_NB; ++i) acc[i] = 0.0;
     for (int k = 0; k < p4; k += 4) {
-        const double v0 = v4, v1 = v5, v2 = v6, v3 = x[k + 3];
-        v4 = x[k + 4];
-        v5 = x[k + 5];
-        v6 = x[k + 6];
+        double v[RW_NB + 3];
+#pragma unroll
+        for (int j = 0; j < RW_NB + 3; ++j) v[j] = x[k + j];
         const double w0 = ws[k], w1 = ws[k + 1], w2 = ws[k + 2], w3 = ws[k + 3];
-        double g;
-        g = v0 * w0 + v1 * w1; g = g + v2 * w2; g = g + v3 * w3; s0 += g;
-        g = v1 * w0 + v2 * w1; g = g + v3 * w2; g = g + v4 * w3; s1 += g;
-        g = v2 * w0 + v3 * w1; g = g + v4 * w2; g = g + v5 * w3; s2 += g;
-        g = v3 * w0 + v4 * w1; g = g + v5 * w2; g = g + v6 * w3; s3 += g;
+#pragma unroll
+        for (int i = 0; i < RW_NB; ++i) {
+            double g = v[i] * w0 + v[i + 1] * w1;
+            g = g + v[i + 2] * w2;
+            g = g + v[i + 3] * w3;
+            acc[i] += g;
+        }
     }
     for (int k = p4; k < p; ++k) {
         const double w = ws[k];
-        s0 += x[k] * w;
-        s1 += x[k + 1] * w;
-        s2 += x[k + 2] * w;
-        s3 += x[k + 3] * w;
-    }
-    const double sv[4] = {s0, s1, s2, s3};
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < RW_NB; ++i) acc[i] += x[k + i] * w;
+    }
+#pragma unroll
+    for (int i = 0; i < RW_NB; ++i) {
         const int t = t0 + tb + i;
-        if (t < T) row[t] = t >= warm ? sv[i] * inv : __builtin_bit_cast(double, 0x7ff8000000000000ull);
+        if (t < T) row[t] = t >= warm ? acc[i] * inv : __builtin_bit_cast(double, 0x7ff8000000000000ull);
     }
 }
 
