@@ -120,18 +120,19 @@ extern "C" __global__ __launch_bounds__(256) void cross_rows_f64(cross_args a) {
 }
 
 struct CrossRun {
-    long long tr = 0, nl = 0, ns = 0, ck = 0;
+    int tr = 0, nl = 0, ns = 0;                  // per segment: < 2^31
+    long long ck = 0;
     double pnl = 0.0;
-    // run [r0, r1) at position p; xa = x[r0], xe = x[min(r1, T - 1)],
-    // t0 = tri(r0), t1 = tri(r1)
+    // run [r0, r1) at position p (r1 >= r0); xa = x[r0], xe = x[min(r1, T - 1)],
+    // t0 = tri(r0), t1 = tri(r1).  Branch-free: p = 0 or an empty run adds
+    // zeros (an empty run has xe = xa).
     __device__ void close(int p, int r0, int r1, long long t0, long long t1, float xa, float xe) {
-        if (r1 <= r0 || p == 0) return;
-        const long long n = (long long)(r1 - r0);
-        if (p > 0) nl += n; else ns += n;
+        const int n = r1 - r0;
+        nl += p > 0 ? n : 0;
+        ns += p < 0 ? n : 0;
         const long long s = t1 - t0;             // sum of t + 1 over [r0, r1)
-        ck += p > 0 ? s : -s;
-        const double dx = (double)xe - (double)xa;
-        pnl += p > 0 ? dx : -dx;
+        ck += p > 0 ? s : (p < 0 ? -s : 0LL);
+        pnl += (double)p * ((double)xe - (double)xa);
     }
 };
 
