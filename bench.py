@@ -777,6 +777,27 @@ def bench_recurrence_batches(steps, warmup, ev):
                      "frac_of_hbm_peak": alg / (mean * 1e-3) / 1e9 / HBM_PEAK_GBS}
     for b in (d_c, d_h, d_l, d_p, d_o):
         b.free()
+    if CPU_EXTRA:
+        # the oracle's scalar ema / rsi / atr (the batch path's row) / adx,
+        # one row per period, on a bounded sample of the periods (every 5th),
+        # OpenMP over rows, plus a one-core sample (every 125th)
+        import oracle
+        f = [a.astype(np.float64) for a in (h, lo, c)]
+        thr = _cpu_threads()
+        sample, one = periods[::5], periods[::125]
+        for name in fns:
+            dt = cpu_median(lambda: oracle.batch_periods(name, f[2], sample, h=f[0], l=f[1],
+                                                         nthreads=thr), reps=3)
+            dt1 = cpu_median(lambda: oracle.batch_periods(name, f[2], one, h=f[0], l=f[1],
+                                                          nthreads=1), reps=3)
+            res[name]["cpu_baseline"] = {
+                "value": sample.size * T / dt, "unit": "outputs/s", "cores": thr,
+                "kind": "port",
+                "sample": f"every 5th period ({sample.size} of {C}) x {T} bars, median of 3 "
+                          f"runs {dt:.3f} s",
+                "one_core": {"value": one.size * T / dt1, "unit": "outputs/s", "cores": 1,
+                             "sample": f"every 125th period ({one.size}), median of 3 runs "
+                                       f"{dt1:.3f} s"}}
     return res
 
 

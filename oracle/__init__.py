@@ -83,6 +83,10 @@ def lib():
                                       ctypes.c_double, ctypes.c_double, dp, dp, dp,
                                       ctypes.POINTER(ctypes.c_int), ctypes.c_int]
         L.or_series_major.restype = None
+        L.or_batch_periods.argtypes = [ctypes.c_int, dp, dp, dp, sz,
+                                       ctypes.POINTER(ctypes.c_int), sz, dp,
+                                       ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+        L.or_batch_periods.restype = None
         L.or_cross_pairs.argtypes = [dp, sz, sz, dp, ctypes.POINTER(ctypes.c_int),
                                      ctypes.POINTER(ctypes.c_int), sz,
                                      ctypes.POINTER(ctypes.c_longlong), dp, ctypes.c_int]
@@ -286,6 +290,25 @@ def atr_batch(h, l, c, periods):
 
 def adx_batch(h, l, c, periods):
     return np.stack([adx(h, l, c, p) for p in periods])
+
+
+_BP_KIND = {"ema": 0, "rsi": 1, "atr": 2, "adx": 3}
+
+
+def batch_periods(kind, c, periods, h=None, l=None, nthreads=0):
+    """One series x many periods with the single-series oracle per row
+    (OpenMP over rows; "atr" is the batch path's row).  Returns
+    (outputs [n, len], per-row status)."""
+    cc, cp = _d(c)
+    hh, hp = _d(h) if h is not None else (cc, cp)
+    ll, lp = _d(l) if l is not None else (cc, cp)
+    ps = np.ascontiguousarray(periods, dtype=np.int32)
+    out = np.empty((ps.size, cc.size))
+    st = (ctypes.c_int * ps.size)()
+    lib().or_batch_periods(_BP_KIND[kind], hp, lp, cp, cc.size,
+                           ps.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ps.size, _d(out)[1],
+                           st, int(nthreads))
+    return out, np.frombuffer(st, dtype=np.int32).copy()
 
 
 def cross_pairs(rows, x, pair_fast, pair_slow, nthreads=0):

@@ -785,6 +785,32 @@ void or_series_major(int kind, const double *a, const double *b, const double *c
     }
 }
 
+/* One series x many periods (the recurrence batches' CPU baseline): row r
+ * of out is the single-series oracle at periods[r], OpenMP over rows.
+ * kind: 0 ema, 1 rsi (series c), 2 atr (the batch path's row, atr_batch_row),
+ * 3 adx (high h, low l, close c). */
+void or_batch_periods(int kind, const double *h, const double *l, const double *c, size_t len,
+                      const int *periods, size_t n, double *out, int *status, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) if (nthreads != 1)
+#endif
+    for (long r = 0; r < (long)n; ++r) {
+        double *y = out + (size_t)r * len;
+        fill_nan(y, len);
+        const size_t p = (size_t)periods[r];
+        int st = OR_PARAM;
+        switch (kind) {
+            case 0: st = or_ema(c, len, p, y); break;
+            case 1: st = or_rsi(c, len, p, y); break;
+            case 2: st = or_atr_batch_row(h, l, c, len, p, y); break;
+            case 3: st = or_adx(h, l, c, len, p, y); break;
+            default: break;
+        }
+        status[r] = st;
+    }
+}
+
 /* ------------------------------------------------------------ double-ALMA crossover */
 
 /* Per pair (fast row f, slow row s of the f64 ALMA batch rows, [nrows][len]):

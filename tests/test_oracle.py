@@ -501,3 +501,20 @@ def test_series_major_driver_matches_single_series_calls():
     got, st = O.series_major("atr", h, 14, low=lo, close=c, nthreads=2)
     for j in (0, 1, 2, 4):
         np.testing.assert_array_equal(got[j], O.atr(h[j], lo[j], c[j], 14))
+
+
+def test_batch_periods_driver_equals_single_series_rows():
+    """The recurrence batches' CPU-baseline driver (OpenMP over periods) gives
+    row for row what the single-series oracle gives."""
+    from vectorta_amd import datagen
+    rng = np.random.Generator(np.random.PCG64(12))
+    c = datagen.gbm_close(3_000, 0.01, 3)
+    _, h, lo, c = datagen.ohlc_from_close(c, rng)
+    f = [a.astype(np.float64) for a in (h, lo, c)]
+    ps = [2, 5, 14, 50, 300]
+    refs = {"ema": O.ema_batch(f[2], ps), "rsi": O.rsi_batch(f[2], ps),
+            "atr": O.atr_batch(*f, ps), "adx": O.adx_batch(*f, ps)}
+    for kind, ref in refs.items():
+        out, st = O.batch_periods(kind, f[2], ps, h=f[0], l=f[1], nthreads=2)
+        assert (st == 0).all(), kind
+        assert np.array_equal(out.view(np.uint64), ref.view(np.uint64)), kind
