@@ -114,9 +114,11 @@ vta_status batch_launch(vta_ctx *ctx, Fir k, const float *d_x, const int32_t *d_
     return launch(ctx, f, dim3(gx, gy, 1), dim3(256), (unsigned)smem, a);
 }
 
-// Many-series chunk: at least 4x the replayed look-back (p bars) and 256
+// Many-series chunk: at least 12x the replayed look-back (p bars) and 256
 // bars, small enough for ~8192 waves (32 per CU: each wave's bars are a
-// dependent chain, so latency is hidden by many waves in flight).
+// dependent chain, so latency is hidden by many waves in flight).  4,096 x
+// 50,000 at period 64: 0.52 ms with 448-bar chunks (8,192 waves), 0.44 ms
+// with ~830 (4,096); periods 14 / 30 keep their 448-bar chunks.
 // Rows of the many-series LDS ring at most (24 KiB per wave); longer periods
 // read the lagged bars from global memory.  VTA_FIR_NO_RING=1 turns the ring
 // off.
@@ -133,7 +135,7 @@ bool ring_disabled() {
 
 int pick_chunk(size_t p, size_t T, size_t groups) {
     size_t want = std::max<size_t>(1, 8192 / std::max<size_t>(groups, 1));
-    size_t c = std::max<size_t>({4 * p, 256, (T + want - 1) / want});
+    size_t c = std::max<size_t>({12 * p, 256, (T + want - 1) / want});
     c = (c + 63) & ~(size_t)63;
     return (int)std::min(c, T);
 }
