@@ -816,11 +816,36 @@ def bench_double_alma(steps, warmup, ev):
     st, _ = plan.results()
     plan.close()
     t = mean * 1e-3
-    return {"workload": "double_alma_crossover_58300_pairs_x_200000", "ms_mean": mean,
-            "ms_p50": float(np.median(ks)), "ms_p99": float(np.percentile(ks, 99)),
-            "pair_bars_per_s": 58_300 * x.size / t,
-            "mean_trades_per_pair": float(st[:, 0].mean()),
-            "reference_published_ms": 158.71}
+    res = {"workload": "double_alma_crossover_58300_pairs_x_200000", "ms_mean": mean,
+           "ms_p50": float(np.median(ks)), "ms_p99": float(np.percentile(ks, 99)),
+           "pair_bars_per_s": 58_300 * x.size / t,
+           "mean_trades_per_pair": float(st[:, 0].mean()),
+           "reference_published_ms": 158.71}
+    if CPU_EXTRA:
+        # the oracle on a bounded sample: every 4th fast x every 4th slow
+        # period, their f64 rows (alma_row_scalar order) and the pair walk,
+        # OpenMP over rows and over pairs
+        import oracle
+        xd = x.astype(np.float64)
+        fp = list(range(5, 225, 4))
+        sp = list(range(30, 295, 4))
+        keys = sorted(set(fp) | set(sp))
+        ix = {k: i for i, k in enumerate(keys)}
+        pf = [ix[a] for a in fp for b in sp]
+        ps = [ix[b] for a in fp for b in sp]
+        thr = _cpu_threads()
+
+        def run():
+            rows = oracle.alma_batch(xd, [(k, 0.85, 6.0) for k in keys], nthreads=thr)
+            oracle.cross_pairs(rows, xd, pf, ps, nthreads=thr)
+
+        dt = cpu_median(run, reps=3)
+        res["cpu_baseline"] = {
+            "value": len(pf) * x.size / dt, "unit": "pair_bars/s", "cores": thr, "kind": "port",
+            "sample": f"every 4th fast x every 4th slow period ({len(pf)} of 58,300 pairs, "
+                      f"{len(keys)} rows) x {x.size} bars, rows and pair walk, median of 3 "
+                      f"runs {dt:.3f} s"}
+    return res
 
 
 def bench_serving(rates=(200, 1000, 2000, 5000), duration_s=4.0, n_slots=8, seed=6):
