@@ -193,8 +193,11 @@ vta_status launch_rec(vta_ctx *ctx, const char *kname, rec_args_host &a, unsigne
 // BB) took 2.69 / 1.96 / 1.34 ms with 256- / 128- / 64-bar chunks.  Sets
 // *done = false (nothing launched) when one chunk would cover the series.
 // Measured on 1,250 periods x 200k bars: EMA 0.395 / 0.365 / 0.336 ms and ADX
-// 1.14 / 1.24 / 1.09 ms at 2,048 / 4,096 / 8,192 waves; flat on small shapes.
-constexpr size_t kScanWaves = 8192;
+// 1.14 / 1.24 / 1.09 ms at 2,048 / 4,096 / 8,192 waves (an earlier build);
+// later EMA 0.298 / 0.285 and RSI 0.442 / 0.398 ms at 8,192 / 16,384 waves
+// (256-bar chunks, the 1,024-chunk cap); flat on small shapes, whose chunks
+// sit at the 64-bar floor either way.
+constexpr size_t kScanWaves = 16384;
 
 size_t scan_states(Ind k) { return k == IND_RSI ? 2 : k == IND_ADX ? 4 : 1; }
 

@@ -26,9 +26,10 @@ struct vta_ctx {
     void *scratch = nullptr;
     static constexpr size_t kScratchBytes = 64 * 1024;
     // chunk-end states of the recurrences' chunked scan ([combo][chunk][state]
-    // f64), reused in stream order by every batch launch of the context
+    // f64), reused in stream order by every batch launch of the context;
+    // 64 MiB holds 1,250 ADX combos (four states) at 1,024 chunks
     void *carry = nullptr;
-    static constexpr size_t kCarryBytes = 16u << 20;
+    static constexpr size_t kCarryBytes = 64u << 20;
     // launch-shape knobs (vta_tune_knob); 0 = built-in heuristic
     int64_t tune[VTA_TUNE_KNOB_COUNT] = {};
     // EMA / RSI / ADX / PNR kernel policies and the forms their last batch and
