@@ -31,7 +31,7 @@ KNOB_NAMES = {ALMA_MFMA_VARIANT: "alma_mfma_variant", SCAN_WAVES: "scan_waves",
 # Candidate values per knob (every value is valid for vta_ctx_set_tuning).
 CANDIDATES: Dict[int, Tuple[int, ...]] = {
     ALMA_MFMA_VARIANT: (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11),
-    SCAN_WAVES: (512, 1024, 2048, 4096, 8192),
+    SCAN_WAVES: (1024, 2048, 4096, 8192, 16384),
     FIR_GROUP: (16, 32, 64, 128),
     MS_CHUNK: (512, 1024, 2048, 4096),
     ROLL_FORM: (1, 2),
