@@ -174,18 +174,22 @@ __device__ __forceinline__ double rec_ratio(double num, double den) {
     return num * r;
 }
 
-// RSI output 100 ag / (ag + al) (50 when the denominator is 0), divided in
-// f32: a reciprocal estimate refined by one Newton step, <= ~2 ulp of f32 (the
-// output is f32, the parity bound 1e-5); outside the estimate's range the
-// f64 quotient.  The ratio is an output only; the state stays f64.
+// RSI output 100 ag / (ag + al) (50 when the denominator is 0), formed in
+// f32 from the f64 averages: both rounded to f32, their f32 sum, a
+// reciprocal estimate refined by one Newton step and two f32 products --
+// within ~6 ulp of f32 (~4e-7 relative; the output is f32, the parity bound
+// 1e-5).  A sum outside [1e-30, 1e30] takes the f64 quotient (and 50 for a
+// zero denominator).  The ratio is an output only; the state stays f64.
 __device__ __forceinline__ float rsi_ratio_f32(double ag, double al) {
-    const double den = ag + al;
-    if (den == 0.0) return 50.0f;
-    if (!(den > 1e-30 && den < 1e30)) return (float)(100.0 * ag / den);
-    const float d = (float)den;
-    float r = __builtin_amdgcn_rcpf(d);
-    r = fmaf(fmaf(-d, r, 1.0f), r, r);
-    return (float)(100.0 * ag) * r;
+    const float a = (float)ag, b = (float)al;
+    const float den = a + b;
+    if (!(den > 1e-30f && den < 1e30f)) {
+        const double dd = ag + al;
+        return dd == 0.0 ? 50.0f : (float)(100.0 * ag / dd);
+    }
+    float r = __builtin_amdgcn_rcpf(den);
+    r = fmaf(fmaf(-den, r, 1.0f), r, r);
+    return (100.0f * a) * r;
 }
 
 // DX = 100 |+DM - -DM| / (+DM + -DM) for den > 0, divided in f32 (reciprocal
